@@ -1,0 +1,369 @@
+"""ctypes wrapper of the CPU restatement in oracle/liboracle.so.
+
+TEST INFRASTRUCTURE ONLY: tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg use this module as the checker.  The product package
+(powergrid_amd) never imports it.
+
+Arrays are numpy; complex data is complex64 (float variant) or complex128
+(double variant).  The classes mirror the reference operators
+(Gnufft.cpp, TimeSegmentation.cpp, SENSE.cpp, Gdft.cpp) with `*` = forward
+and `/` = adjoint, like the reference's operator* / operator/.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", _HERE])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        _lib = C.CDLL(_LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+_vp = C.c_void_p
+
+
+def _declare(L):
+    for suf in ("f", "d"):
+        R = C.c_float if suf == "f" else C.c_double
+        def f(name, res, *args):
+            fn = getattr(L, name + "_" + suf)
+            fn.restype = res
+            fn.argtypes = list(args)
+        f("orc_lut_size", C.c_size_t, R)
+        f("orc_lut", None, R, R, _vp)
+        f("orc_kb_beta", R, R, R)
+        f("orc_fft", None, _vp, C.c_int, C.c_int, C.c_int, C.c_int)
+        f("orc_fft1d", None, _vp, C.c_int, C.c_int)
+        f("orc_deapod", None, _vp, _vp, C.c_int, C.c_int, C.c_int, R, R, R)
+        f("orc_zero_pad", None, _vp, _vp, C.c_int, C.c_int, C.c_int, R)
+        f("orc_crop", None, _vp, _vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int)
+        f("orc_fftshift", None, _vp, _vp, C.c_int, C.c_int, C.c_int)
+        f("orc_ifftshift", None, _vp, _vp, C.c_int, C.c_int, C.c_int)
+        f("orc_grid_forward", None, C.c_int, C.c_int, C.c_int, C.c_int, R, R,
+          _vp, _vp, _vp, _vp, C.c_size_t, _vp, _vp)
+        f("orc_grid_adjoint", None, C.c_int, C.c_int, C.c_int, C.c_int, R, R,
+          _vp, _vp, _vp, _vp, C.c_size_t, _vp, _vp)
+        f("orc_ft", None, *([_vp] * 12), C.c_int, C.c_int)
+        f("orc_ift", None, *([_vp] * 12), C.c_int, C.c_int)
+        f("orc_gnufft_create", _vp, C.c_int, R, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp, R)
+        f("orc_gdft_create", _vp, C.c_int, C.c_int, *([_vp] * 8))
+        f("orc_ts_create", _vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int)
+        f("orc_sense_create", _vp, _vp, _vp, C.c_int, C.c_int, C.c_int)
+        f("orc_dense_create", _vp, _vp, C.c_int, C.c_int)
+        f("orc_op_forward", None, _vp, _vp, _vp)
+        f("orc_op_adjoint", None, _vp, _vp, _vp)
+        f("orc_op_n_in", C.c_int, _vp)
+        f("orc_op_n_out", C.c_int, _vp)
+        f("orc_op_destroy", None, _vp)
+        f("orc_ts_tables", C.c_int, _vp, _vp, _vp, _vp)
+        f("orc_penalty_create", _vp, C.c_int, C.c_int, C.c_int, C.c_int, R, R, C.c_int)
+        f("orc_penalty_destroy", None, _vp)
+        f("orc_penalty_cd", None, _vp, _vp, C.c_int, _vp)
+        f("orc_penalty_ctd", None, _vp, _vp, C.c_int, _vp)
+        f("orc_penalty_gradient", None, _vp, _vp, _vp)
+        f("orc_penalty_denom", None, _vp, _vp, _vp, _vp)
+        f("orc_penalty_value", R, _vp, _vp)
+        f("orc_pcg", C.c_int, _vp, _vp, _vp, _vp, _vp, C.c_int, _vp, _vp)
+    for name in ("orc_nudft_exact_fwd", "orc_nudft_exact_adj"):
+        fn = getattr(L, name)
+        fn.restype = None
+        fn.argtypes = [_vp] * 10 + [C.c_int, C.c_int]
+    L.orc_set_threads.argtypes = [C.c_int]
+    L.orc_set_threads.restype = None
+    L.orc_get_threads.restype = C.c_int
+
+
+def set_threads(n: int):
+    lib().orc_set_threads(int(n))
+
+
+def _suf(dtype):
+    return "f" if np.dtype(dtype) in (np.float32, np.complex64) else "d"
+
+
+def _rt(suf):
+    return np.float32 if suf == "f" else np.float64
+
+
+def _ct(suf):
+    return np.complex64 if suf == "f" else np.complex128
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+def _fn(name, suf):
+    return getattr(lib(), name + "_" + suf)
+
+
+def _r(a, suf):
+    return np.ascontiguousarray(a, dtype=_rt(suf))
+
+
+def _c(a, suf):
+    return np.ascontiguousarray(a, dtype=_ct(suf))
+
+
+# ----------------------------------------------------------- primitives --
+
+def kb_beta(gridos=2.0, width=4.0, prec="f"):
+    return float(_fn("orc_kb_beta", prec)(gridos, width))
+
+
+def lut(beta, width, prec="f"):
+    n = int(_fn("orc_lut_size", prec)(width))
+    out = np.zeros(n, dtype=_rt(prec))
+    _fn("orc_lut", prec)(beta, width, _p(out))
+    return out
+
+
+def fft(data, nx, ny=1, nz=1, sign=-1):
+    """In-place-style unnormalised C2C over a (nx fastest, ny, nz) buffer."""
+    suf = _suf(data.dtype)
+    a = _c(data, suf).copy()
+    _fn("orc_fft", suf)(_p(a), nx, ny, nz, sign)
+    return a
+
+
+def deapod(img, Nx, Ny, Nz, W, beta, os_=2.0):
+    suf = _suf(img.dtype)
+    src = _c(img, suf)
+    dst = np.zeros_like(src)
+    _fn("orc_deapod", suf)(_p(dst), _p(src), Nx, Ny, Nz, W, beta, os_)
+    return dst
+
+
+def zero_pad(img, Nx, Ny, Nz, os_=2.0):
+    suf = _suf(img.dtype)
+    src = _c(img, suf)
+    G = int(round(Nx * os_ * Ny * os_ * (Nz * os_ if Nz > 1 else 1)))
+    dst = np.zeros(G, dtype=_ct(suf))
+    _fn("orc_zero_pad", suf)(_p(dst), _p(src), Nx, Ny, Nz, os_)
+    return dst
+
+
+def crop(grid, Nx, Ny, Nz, Gx, Gy, Gz):
+    suf = _suf(grid.dtype)
+    src = _c(grid, suf)
+    dst = np.zeros(Nx * Ny * Nz, dtype=_ct(suf))
+    _fn("orc_crop", suf)(_p(dst), _p(src), Nx, Ny, Nz, Gx, Gy, Gz)
+    return dst
+
+
+def fftshift(grid, Gx, Gy, Gz=1, inverse=False):
+    suf = _suf(grid.dtype)
+    src = _c(grid, suf)
+    dst = np.zeros_like(src)
+    _fn("orc_ifftshift" if inverse else "orc_fftshift", suf)(_p(dst), _p(src), Gx, Gy, Gz)
+    return dst
+
+
+def grid_forward(grid, kx, ky, kz, Nx, Ny, Nz, os_=2.0, W=4.0, prec=None):
+    suf = prec or _suf(grid.dtype)
+    beta = kb_beta(os_, W, suf)
+    L = lut(beta, W, suf)
+    M = len(kx)
+    out = np.zeros(M, dtype=_ct(suf))
+    g = _c(grid, suf)
+    _fn("orc_grid_forward", suf)(M, Nx, Ny, Nz, os_, W, _p(_r(kx, suf)), _p(_r(ky, suf)),
+                                 _p(_r(kz, suf)), _p(L), len(L), _p(g), _p(out))
+    return out
+
+
+def grid_adjoint(samples, kx, ky, kz, Nx, Ny, Nz, Gsize, os_=2.0, W=4.0, prec=None):
+    suf = prec or _suf(samples.dtype)
+    beta = kb_beta(os_, W, suf)
+    L = lut(beta, W, suf)
+    M = len(kx)
+    out = np.zeros(Gsize, dtype=_ct(suf))
+    s = _c(samples, suf)
+    _fn("orc_grid_adjoint", suf)(M, Nx, Ny, Nz, os_, W, _p(_r(kx, suf)), _p(_r(ky, suf)),
+                                 _p(_r(kz, suf)), _p(L), len(L), _p(s), _p(out))
+    return out
+
+
+def nudft_exact(x, kx, ky, kz, ix, iy, iz, fm, t, adjoint=False):
+    """FP64 exact field-corrected NUDFT (double sin/cos)."""
+    d = np.float64
+    args = [np.ascontiguousarray(a, dtype=d) for a in (kx, ky, kz, ix, iy, iz, fm, t)]
+    xin = np.ascontiguousarray(x, dtype=np.complex128)
+    M, N = len(kx), len(ix)
+    out = np.zeros(N if adjoint else M, dtype=np.complex128)
+    fn = lib().orc_nudft_exact_adj if adjoint else lib().orc_nudft_exact_fwd
+    fn(_p(out), _p(xin), *[_p(a) for a in args], M, N)
+    return out
+
+
+# ------------------------------------------------------------ operators --
+
+class _Op:
+    def __init__(self, handle, suf, keep=()):
+        self.h = handle
+        self.suf = suf
+        self._keep = keep
+        self.n_in = _fn("orc_op_n_in", suf)(handle)
+        self.n_out = _fn("orc_op_n_out", suf)(handle)
+
+    def __mul__(self, x):
+        x = _c(x, self.suf)
+        assert x.size == self.n_in
+        out = np.zeros(self.n_out, dtype=_ct(self.suf))
+        _fn("orc_op_forward", self.suf)(self.h, _p(x), _p(out))
+        return out
+
+    def __truediv__(self, y):
+        y = _c(y, self.suf)
+        assert y.size == self.n_out
+        out = np.zeros(self.n_in, dtype=_ct(self.suf))
+        _fn("orc_op_adjoint", self.suf)(self.h, _p(y), _p(out))
+        return out
+
+    def __del__(self):
+        try:
+            if self.h and not self._keep:
+                _fn("orc_op_destroy", self.suf)(self.h)
+        except Exception:
+            pass
+
+
+class Gnufft(_Op):
+    """Gnufft.cpp:34-122 (kernelWidth defaults to the reference's 4.0)."""
+
+    def __init__(self, M, gridos, Nx, Ny, Nz, kx, ky, kz, kernel_width=4.0, prec="f"):
+        h = _fn("orc_gnufft_create", prec)(M, gridos, Nx, Ny, Nz, _p(_r(kx, prec)),
+                                           _p(_r(ky, prec)), _p(_r(kz, prec)), kernel_width)
+        super().__init__(h, prec)
+        self.Nx, self.Ny, self.Nz = Nx, Ny, Nz
+
+
+class Gdft(_Op):
+    """Gdft.cpp:32-49."""
+
+    def __init__(self, M, N, kx, ky, kz, ix, iy, iz, fm, t, prec="f"):
+        arrs = [_r(a, prec) for a in (kx, ky, kz, ix, iy, iz, fm, t)]
+        h = _fn("orc_gdft_create", prec)(M, N, *[_p(a) for a in arrs])
+        super().__init__(h, prec)
+
+
+class TimeSegmentation(_Op):
+    """TimeSegmentation.cpp:36-221 (Hanning interpolator)."""
+
+    def __init__(self, G, fm, t, M, N, L, interptype=1, shots=1):
+        prec = G.suf
+        h = _fn("orc_ts_create", prec)(G.h, _p(_r(fm, prec)), _p(_r(t, prec)), M, N, L,
+                                       interptype, shots)
+        super().__init__(h, prec)
+        self.G = G
+        # the composite does not own the inner op (TimeSegmentation.h:55);
+        # the C side frees only its own context.
+
+    def tables(self):
+        L = _fn("orc_ts_tables", self.suf)(self.h, None, None, None)
+        AA = np.zeros((L, self.n_out), dtype=_rt(self.suf))
+        Wo = np.zeros((L, self.n_in), dtype=_ct(self.suf))
+        tt = np.zeros(2, dtype=_rt(self.suf))
+        _fn("orc_ts_tables", self.suf)(self.h, _p(AA), _p(Wo), _p(tt))
+        return AA, Wo, float(tt[0]), float(tt[1])
+
+
+class SENSE(_Op):
+    """SENSE.cpp:32-44."""
+
+    def __init__(self, G, sens, M, N, nc):
+        prec = G.suf
+        h = _fn("orc_sense_create", prec)(G.h, _p(_c(sens, prec)), M, N, nc)
+        super().__init__(h, prec)
+        self.G = G
+
+
+class Dense(_Op):
+    """A dense complex matrix as an operator (test problems only)."""
+
+    def __init__(self, A, prec="d"):
+        A = np.asfortranarray(A, dtype=_ct(prec))
+        h = _fn("orc_dense_create", prec)(_p(A.ravel(order="F")), A.shape[0], A.shape[1])
+        super().__init__(h, prec)
+
+
+class Penalty:
+    """Robject.cpp / QuadPenalty.cpp (kind 0) / TVPenalty.cpp (kind 1)."""
+
+    def __init__(self, kind, Nx, Ny, Nz, beta, delta=0.0, dims2penalize=3, prec="f"):
+        self.suf = prec
+        self.N = Nx * Ny * Nz
+        self.h = _fn("orc_penalty_create", prec)(kind, Nx, Ny, Nz, beta, delta, dims2penalize)
+
+    def Cd(self, d, dim):
+        out = np.zeros(self.N, dtype=_ct(self.suf))
+        _fn("orc_penalty_cd", self.suf)(self.h, _p(_c(d, self.suf)), dim, _p(out))
+        return out
+
+    def Ctd(self, d, dim):
+        out = np.zeros(self.N, dtype=_ct(self.suf))
+        _fn("orc_penalty_ctd", self.suf)(self.h, _p(_c(d, self.suf)), dim, _p(out))
+        return out
+
+    def Gradient(self, x):
+        out = np.zeros(self.N, dtype=_ct(self.suf))
+        _fn("orc_penalty_gradient", self.suf)(self.h, _p(_c(x, self.suf)), _p(out))
+        return out
+
+    def Denom(self, ddir, x):
+        out = np.zeros(2, dtype=_rt(self.suf))
+        _fn("orc_penalty_denom", self.suf)(self.h, _p(_c(ddir, self.suf)),
+                                           _p(_c(x, self.suf)), _p(out))
+        return complex(out[0], out[1])
+
+    def Penalty(self, x):
+        return float(_fn("orc_penalty_value", self.suf)(self.h, _p(_c(x, self.suf))))
+
+    def __del__(self):
+        try:
+            _fn("orc_penalty_destroy", self.suf)(self.h)
+        except Exception:
+            pass
+
+
+def QuadPenalty(Nx, Ny, Nz, beta, dims2penalize=3, prec="f"):
+    return Penalty(0, Nx, Ny, Nz, beta, 0.0, dims2penalize, prec)
+
+
+def TVPenalty(Nx, Ny, Nz, beta, delta, dims2penalize=3, prec="f"):
+    return Penalty(1, Nx, Ny, Nz, beta, delta, dims2penalize, prec)
+
+
+def solve_pwls_pcg(x0, A, W, yi, R, niter):
+    """solve_pwls_pcg.hpp:48-167.  Returns (x, completed_iterations, resid)."""
+    suf = A.suf
+    x0 = _c(x0, suf)
+    y = _c(yi, suf)
+    Wr = None if W is None else _r(W, suf)
+    out = np.zeros(A.n_in, dtype=_ct(suf))
+    res = np.zeros(max(niter, 1), dtype=_rt(suf))
+    n = _fn("orc_pcg", suf)(A.h, R.h if R is not None else None, _p(x0), _p(Wr), _p(y),
+                            niter, _p(out), _p(res))
+    return out, int(n), res[:n]
+
+
+def reconSolve(data, Sg, R, Nx, Ny, Nz, niter):
+    """reconSolve.h:63-82: W = ones, x0 = 0."""
+    x0 = np.zeros(Nx * Ny * Nz, dtype=_ct(Sg.suf))
+    return solve_pwls_pcg(x0, Sg, None, data, R, niter)
