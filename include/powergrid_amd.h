@@ -1,0 +1,174 @@
+/*
+ * powergrid_amd.h -- C ABI of the MI355X (gfx950) implementation of
+ * PowerGrid's field-corrected non-Cartesian SENSE path.
+ *
+ * The reference exposes this path as C++ operator objects with
+ * `Col<complex<T>> operator*(const Col<complex<T>>&) const` (forward) and
+ * `operator/` (adjoint), consumed by solve_pwls_pcg / reconSolve
+ * (SENSE.cpp:51-97, TimeSegmentation.cpp:252-324, Gnufft.cpp:153-211,
+ * Gdft.cpp:52-136, solve_pwls_pcg.hpp:48-167, reconSolve.h:63-82).  Below the
+ * operators the reference crosses a raw-pointer seam, computeFd_CPU_Grid /
+ * computeFH_CPU_Grid (gridding.h:84-103).  This header replaces both: the
+ * operator objects become opaque handles with the reference's constructor
+ * arguments, and operator* / operator/ become pg_op_forward / pg_op_adjoint
+ * on device buffers (or pg_op_*_host on host buffers).  The C++ classes in
+ * include/powergrid/ rebuild the reference's class API on top of it.
+ *
+ * Conventions: complex data is interleaved float (re, im); the image vector
+ * is y-fastest (idx = y + x*Ny + z*Nx*Ny, reconSolve.cpp:30-54); multi-coil
+ * k-space is coil-major (sample i of coil c at c*M + i, SENSE.cpp:70); the
+ * field map is in rad/s and times in s (TimeSegmentation.h:56).  All calls
+ * return a pg_status; pg_last_error() describes the most recent failure.
+ * Nothing here exits the process (the reference exit(1)s on malloc failure,
+ * gridding.cpp:454-457).  Launches are asynchronous on the given stream
+ * (NULL = the default stream); a handle must not be used concurrently from
+ * two streams.
+ */
+#ifndef POWERGRID_AMD_H
+#define POWERGRID_AMD_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    PG_OK = 0,
+    PG_ERR_ARG = 1,          /* invalid argument */
+    PG_ERR_UNSUPPORTED = 2,  /* valid for the reference, not supported here */
+    PG_ERR_HIP = 3,          /* HIP runtime error */
+    PG_ERR_OOM = 4,          /* device allocation failed */
+    PG_ERR_STATE = 5,        /* call not valid for this handle */
+    PG_ERR_NUMERIC = 6       /* NaN/Inf detected where the reference warns */
+} pg_status;
+
+typedef struct pg_op_s* pg_op;           /* Gnufft, Gdft, TimeSegmentation or SENSE */
+typedef struct pg_penalty_s* pg_penalty; /* QuadPenalty or TVPenalty */
+
+const char* pg_status_string(int status);
+const char* pg_last_error(void);
+/* library version, e.g. "0.1.0" */
+const char* pg_version(void);
+
+/* ---- device plumbing (so a C or C++ caller needs no other runtime) ---- */
+int pg_set_device(int device);
+int pg_malloc(void** ptr, size_t bytes);
+int pg_free(void* ptr);
+int pg_memcpy_htod(void* dst_dev, const void* src_host, size_t bytes, void* stream);
+int pg_memcpy_dtoh(void* dst_host, const void* src_dev, size_t bytes, void* stream);
+int pg_memcpy_dtod(void* dst_dev, const void* src_dev, size_t bytes, void* stream);
+int pg_stream_create(void** stream);
+int pg_stream_destroy(void* stream);
+int pg_stream_synchronize(void* stream);
+
+/* ---- operators ---------------------------------------------------------
+ * pg_gnufft_create replaces Gnufft<float>::Gnufft (Gnufft.cpp:34-122,
+ * Gnufft.h:51-53): same arguments plus kernelWidth (original-grid units; the
+ * reference hard-codes 4.0 at Gnufft.cpp:84; W*gridos must be 6 or 8).
+ * Requirements: gridos*Nx integer and even, Nx == Ny (the reference's
+ * zero_pad2d/circshift2 are x-fastest and only consistent for square
+ * images, griddingSupport.cpp:488,565).  ix/iy/iz are accepted for
+ * signature parity and may be NULL (Gnufft does not use them). */
+int pg_gnufft_create(pg_op* out, size_t M, float gridos, size_t Nx, size_t Ny, size_t Nz,
+                     const float* kx, const float* ky, const float* kz, const float* ix,
+                     const float* iy, const float* iz, float kernelWidth);
+
+/* replaces Gdft<float>::Gdft (Gdft.cpp:32-49, Gdft.h:47-49).  precision 0:
+ * the reference's arithmetic class (FP32 phase, FP32 sin/cos,
+ * ftCpu.cpp:120-130); precision 1: FP64 phase and sin/cos with FP64
+ * accumulation (the exact-NUDFT oracle form). */
+int pg_gdft_create(pg_op* out, size_t M, size_t N, const float* kx, const float* ky,
+                   const float* kz, const float* ix, const float* iy, const float* iz,
+                   const float* fm, const float* t, int precision);
+
+/* replaces TimeSegmentation<float,G>::TimeSegmentation
+ * (TimeSegmentation.cpp:36-221, TimeSegmentation.h:70-71).  Only the
+ * Hanning interpolator (interptype 1) is implemented; G is not owned and
+ * must outlive the new handle (TimeSegmentation.h:55). */
+int pg_ts_create(pg_op* out, pg_op G, const float* fieldmap, const float* tvec, size_t M,
+                 size_t N, size_t L, size_t interptype, size_t shots);
+
+/* replaces SENSE<float,G>::SENSE (SENSE.cpp:32-44, SENSE.h:64); sens is N*nc
+ * interleaved complex, coil-major (SENSE.cpp:40).  G is not owned. */
+int pg_sense_create(pg_op* out, pg_op G, const float* sens, size_t M, size_t N, size_t nc);
+
+int pg_op_destroy(pg_op op);
+/* n_in = image length (N), n_out = data length (M, or M*nc for SENSE) */
+int pg_op_shape(pg_op op, size_t* n_in, size_t* n_out);
+
+/* PG_MODE_BATCHED (default): one launch sequence over all coils x segments;
+ * PG_MODE_PLAIN: one NUFFT at a time in the reference's loop order
+ * (SENSE.cpp:65-67, TimeSegmentation.cpp:272-288). */
+#define PG_MODE_BATCHED 0
+#define PG_MODE_PLAIN 1
+int pg_op_set_mode(pg_op op, int mode);
+
+/* operator* (forward) and operator/ (adjoint) on device buffers:
+ * replaces Gnufft::operator* / operator/ (Gnufft.cpp:153-211, i.e.
+ * computeFd_CPU_Grid / computeFH_CPU_Grid, gridding.h:84-103),
+ * TimeSegmentation::operator* / operator/ (TimeSegmentation.cpp:252-324),
+ * SENSE::operator* / operator/ (SENSE.cpp:51-97) and Gdft::operator* /
+ * operator/ (Gdft.cpp:52-136).  in and out must not alias. */
+int pg_op_forward(pg_op op, const void* in_dev, void* out_dev, void* stream);
+int pg_op_adjoint(pg_op op, const void* in_dev, void* out_dev, void* stream);
+/* the same on host buffers (copies in, runs, copies out, synchronizes) */
+int pg_op_forward_host(pg_op op, const void* in_host, void* out_host);
+int pg_op_adjoint_host(pg_op op, const void* in_host, void* out_host);
+
+/* ---- penalties ----------------------------------------------------------
+ * kind 0: QuadPenalty (QuadPenalty.cpp:32-64), kind 1: TVPenalty
+ * (TVPenalty.cpp:32-68, delta used); the Robject methods Cd / Ctd /
+ * Gradient / Denom / Penalty follow Robject.cpp:49-193. */
+#define PG_PENALTY_QUAD 0
+#define PG_PENALTY_TV 1
+int pg_penalty_create(pg_penalty* out, int kind, size_t Nx, size_t Ny, size_t Nz, float beta,
+                      float delta, size_t dims2penalize);
+int pg_penalty_destroy(pg_penalty R);
+int pg_penalty_gradient(pg_penalty R, const void* x_dev, void* g_dev, void* stream);
+/* Denom(ddir, x) (Robject.cpp:167-193), returned as (re, im) in out2 (host) */
+int pg_penalty_denom(pg_penalty R, const void* ddir_dev, const void* x_dev, double* out2,
+                     void* stream);
+int pg_penalty_value(pg_penalty R, const void* x_dev, double* out, void* stream);
+
+/* ---- solver --------------------------------------------------------------
+ * replaces solve_pwls_pcg<float,Tobj,Robj> (solve_pwls_pcg.hpp:48-167).
+ * The whole loop runs on the device: vectors FP32, inner products, step
+ * sizes and norms FP64.  W_dev may be NULL (= ones, as reconSolve.h:74-75).
+ * mode PG_CG_GRAPH: one iteration captured as a HIP graph and replayed,
+ * all branches decided on the device; PG_CG_HOST: host-driven loop that
+ * reads the scalars back every iteration (the reference's control flow).
+ * resid (host, niter entries, may be NULL) receives ||y - A x|| after each
+ * completed iteration (solve_pwls_pcg.hpp:162). */
+#define PG_CG_GRAPH 0
+#define PG_CG_HOST 1
+#define PG_EXIT_NITER 0       /* ran all iterations */
+#define PG_EXIT_ZERO_GRAD 1   /* solve_pwls_pcg.hpp:88-91 */
+#define PG_EXIT_NOT_DESCENT 2 /* :113-116 */
+#define PG_EXIT_BAD_DENOM 3   /* :137-145 */
+typedef struct {
+    int iterations;   /* completed updates */
+    int exit_reason;  /* PG_EXIT_* */
+    double final_resid;
+} pg_cg_info;
+int pg_solve_pwls_pcg(pg_op A, pg_penalty R, const void* x0_dev, const float* W_dev,
+                      const void* y_dev, size_t niter, int mode, void* x_out_dev, double* resid,
+                      pg_cg_info* info, void* stream);
+/* replaces reconSolve (reconSolve.h:63-82): W = ones, x0 = 0 */
+int pg_recon_solve(pg_op A, pg_penalty R, const void* y_dev, size_t niter, int mode,
+                   void* x_out_dev, double* resid, pg_cg_info* info, void* stream);
+
+/* ---- measurement --------------------------------------------------------
+ * When enabled, every kernel launch of the library is bracketed by HIP
+ * events on its stream and its duration accumulated per kernel name.  Not
+ * for use under graph capture. */
+int pg_profile_enable(int on);
+int pg_profile_reset(void);
+/* name: kernel family, e.g. "interp", "spread", "fwd_prep_fft", "adj_fft_post",
+ * "gdft_fwd"; returns the launch count and total milliseconds */
+int pg_profile_query(const char* name, int* count, double* total_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* POWERGRID_AMD_H */

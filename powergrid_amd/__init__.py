@@ -1,0 +1,333 @@
+"""powergrid_amd -- MI355X (gfx950) implementation of PowerGrid's
+field-corrected non-Cartesian SENSE path, behind the reference's operator API.
+
+The classes mirror the reference (PowerGrid/Gnufft.h:51-53, Gdft.h:47-49,
+TimeSegmentation.h:70-71, SENSE.h:64, QuadPenalty.h:43, TVPenalty.h:44,
+solve_pwls_pcg.hpp:48-51, reconSolve.h:63-66): `A * x` is the forward
+operator (operator*) and `A / y` the adjoint (operator/).  Every call goes
+through the C ABI of libpowergrid_amd.so (include/powergrid_amd.h); numpy
+arrays take the host-buffer entry points, DeviceArrays the
+device-buffer ones.  Device memory and streams come from the library
+itself (DeviceArray, Stream): no other GPU framework is loaded.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as _L
+from ._lib import CgInfo, PgError, check, lib
+
+MODE_BATCHED = 0
+MODE_PLAIN = 1
+CG_GRAPH = 0
+CG_HOST = 1
+EXIT_REASONS = {0: "niter", 1: "zero_gradient", 2: "not_descent", 3: "bad_denominator"}
+
+
+def _f32(a, n=None):
+    if a is None:
+        return None
+    a = np.ascontiguousarray(np.asarray(a, dtype=np.float32).ravel())
+    if n is not None and a.size != n:
+        raise ValueError(f"expected {n} values, got {a.size}")
+    return a
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+class Stream:
+    """A HIP stream owned by the library (pg_stream_create)."""
+
+    def __init__(self):
+        h = C.c_void_p()
+        check(lib().pg_stream_create(C.byref(h)))
+        self.handle = h
+
+    def synchronize(self):
+        check(lib().pg_stream_synchronize(self.handle))
+
+    def __del__(self):
+        try:
+            if self.handle:
+                lib().pg_stream_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+
+class DeviceArray:
+    """A 1-D device buffer allocated through the C ABI (pg_malloc).
+
+    Device memory is owned by the library's own HIP runtime; no other GPU
+    framework is involved."""
+
+    def __init__(self, n, dtype=np.complex64):
+        self.size = int(n)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = self.size * self.dtype.itemsize
+        h = C.c_void_p()
+        check(lib().pg_malloc(C.byref(h), max(self.nbytes, 1)))
+        self.ptr = h
+
+    @classmethod
+    def from_host(cls, a, dtype=None, stream=None):
+        a = np.ascontiguousarray(np.asarray(a, dtype=dtype).ravel())
+        d = cls(a.size, a.dtype)
+        if a.nbytes:
+            check(lib().pg_memcpy_htod(d.ptr, _p(a), a.nbytes, _sh(stream)))
+        return d
+
+    def to_host(self, stream=None):
+        out = np.empty(self.size, self.dtype)
+        if self.nbytes:
+            check(lib().pg_memcpy_dtoh(_p(out), self.ptr, self.nbytes, _sh(stream)))
+        return out
+
+    def copy_from(self, other, stream=None):
+        assert other.nbytes == self.nbytes
+        check(lib().pg_memcpy_dtod(self.ptr, other.ptr, self.nbytes, _sh(stream)))
+        return self
+
+    def __len__(self):
+        return self.size
+
+    def __del__(self):
+        try:
+            if self.ptr:
+                lib().pg_free(self.ptr)
+                self.ptr = None
+        except Exception:
+            pass
+
+
+def _sh(stream):
+    return None if stream is None else stream.handle
+
+
+def _as_device(a, dtype):
+    if isinstance(a, DeviceArray):
+        if a.dtype != np.dtype(dtype):
+            raise ValueError(f"expected a {np.dtype(dtype)} device array")
+        return a
+    return DeviceArray.from_host(a, dtype)
+
+
+class _Op:
+    """Common operator behaviour (operator* / operator/)."""
+
+    def __init__(self, handle, deps=()):
+        self._h = handle
+        self._deps = deps  # inner operators must outlive composites
+        n_in, n_out = C.c_size_t(), C.c_size_t()
+        check(lib().pg_op_shape(handle, C.byref(n_in), C.byref(n_out)))
+        self.n_in, self.n_out = n_in.value, n_out.value
+
+    @property
+    def handle(self):
+        return self._h
+
+    def set_mode(self, mode):
+        """MODE_BATCHED (default) or MODE_PLAIN (one NUFFT at a time)."""
+        check(lib().pg_op_set_mode(self._h, int(mode)))
+        return self
+
+    def forward(self, x, out=None, stream=None):
+        return self._apply(x, out, True, stream)
+
+    def adjoint(self, y, out=None, stream=None):
+        return self._apply(y, out, False, stream)
+
+    def _apply(self, x, out, fwd, stream):
+        n_in, n_out = (self.n_in, self.n_out) if fwd else (self.n_out, self.n_in)
+        if isinstance(x, DeviceArray):
+            if x.size != n_in or x.dtype != np.complex64:
+                raise ValueError(f"expected a complex64 device array of {n_in} elements")
+            if out is None:
+                out = DeviceArray(n_out, np.complex64)
+            fn = lib().pg_op_forward if fwd else lib().pg_op_adjoint
+            check(fn(self._h, x.ptr, out.ptr, _sh(stream)))
+            return out
+        xin = np.ascontiguousarray(np.asarray(x, dtype=np.complex64).ravel())
+        if xin.size != n_in:
+            raise ValueError(f"expected {n_in} values, got {xin.size}")
+        res = np.empty(n_out, dtype=np.complex64)
+        fn = lib().pg_op_forward_host if fwd else lib().pg_op_adjoint_host
+        check(fn(self._h, _p(xin), _p(res)))
+        return res
+
+    def __mul__(self, x):
+        return self.forward(x)
+
+    def __truediv__(self, y):
+        return self.adjoint(y)
+
+    def __del__(self):
+        try:
+            if self._h:
+                lib().pg_op_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+
+def _create(fn, *args):
+    h = C.c_void_p()
+    check(fn(C.byref(h), *args))
+    return h
+
+
+class Gnufft(_Op):
+    """Gnufft<float>(dataLength, gridos, nx, ny, nz, kx, ky, kz, ix, iy, iz)
+    (Gnufft.cpp:34-122); kernel_width defaults to the reference's 4.0."""
+
+    def __init__(self, M, gridos, Nx, Ny, Nz, kx, ky, kz, ix=None, iy=None, iz=None,
+                 kernel_width=4.0):
+        kx, ky = _f32(kx, M), _f32(ky, M)
+        kz = _f32(kz, M) if kz is not None else np.zeros(M, np.float32)
+        h = _create(lib().pg_gnufft_create, M, float(gridos), Nx, Ny, Nz, _p(kx), _p(ky), _p(kz),
+                    None, None, None, float(kernel_width))
+        super().__init__(h)
+        self.Nx, self.Ny, self.Nz = Nx, Ny, Nz
+        self.kernel_width = kernel_width
+
+
+class Gdft(_Op):
+    """Gdft<float>(a, b, kx, ky, kz, ix, iy, iz, fm, t) (Gdft.cpp:32-49).
+    precision 0: FP32 phase and trig (the reference's class); 1: FP64."""
+
+    def __init__(self, M, N, kx, ky, kz, ix, iy, iz, fm, t, precision=0):
+        arrs = [_f32(kx, M), _f32(ky, M), _f32(kz, M), _f32(ix, N), _f32(iy, N), _f32(iz, N),
+                _f32(fm, N), _f32(t, M)]
+        h = _create(lib().pg_gdft_create, M, N, *[_p(a) for a in arrs], int(precision))
+        super().__init__(h)
+
+
+class TimeSegmentation(_Op):
+    """TimeSegmentation<float, G>(G, fieldmap, tvec, M, N, L, interptype=1,
+    shots=1) (TimeSegmentation.cpp:36-221); Hanning interpolator."""
+
+    def __init__(self, G, fieldmap, tvec, M, N, L, interptype=1, shots=1):
+        fm, t = _f32(fieldmap, N), _f32(tvec, M)
+        h = _create(lib().pg_ts_create, G.handle, _p(fm), _p(t), M, N, L, interptype, shots)
+        super().__init__(h, (G,))
+        self.L = max(1, int(L))
+
+
+class SENSE(_Op):
+    """SENSE<float, G>(G, sens, M, N, nc) (SENSE.cpp:32-44); sens holds nc
+    coil maps of N values, coil-major."""
+
+    def __init__(self, G, sens, M, N, nc):
+        s = np.ascontiguousarray(np.asarray(sens, dtype=np.complex64).ravel())
+        if s.size != N * nc:
+            raise ValueError("sens must hold N*nc values")
+        h = _create(lib().pg_sense_create, G.handle, _p(s), M, N, nc)
+        super().__init__(h, (G,))
+        self.nc = nc
+
+
+class _Penalty:
+    def __init__(self, kind, Nx, Ny, Nz, beta, delta, dims2penalize):
+        self._h = _create(lib().pg_penalty_create, kind, Nx, Ny, Nz, float(beta), float(delta),
+                          dims2penalize)
+        self.N = Nx * Ny * Nz
+
+    @property
+    def handle(self):
+        return self._h
+
+    def Gradient(self, x, stream=None):
+        xd = _as_device(x, np.complex64)
+        g = DeviceArray(self.N, np.complex64)
+        check(lib().pg_penalty_gradient(self._h, xd.ptr, g.ptr, _sh(stream)))
+        return g if isinstance(x, DeviceArray) else g.to_host()
+
+    def Denom(self, ddir, x, stream=None):
+        out = (C.c_double * 2)()
+        # keep the device copies referenced until the call has returned
+        dd, xd = _as_device(ddir, np.complex64), _as_device(x, np.complex64)
+        check(lib().pg_penalty_denom(self._h, dd.ptr, xd.ptr, out, _sh(stream)))
+        return complex(out[0], out[1])
+
+    def Penalty(self, x, stream=None):
+        out = C.c_double()
+        xd = _as_device(x, np.complex64)
+        check(lib().pg_penalty_value(self._h, xd.ptr, C.byref(out), _sh(stream)))
+        return out.value
+
+    def __del__(self):
+        try:
+            if self._h:
+                lib().pg_penalty_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+
+class QuadPenalty(_Penalty):
+    """QuadPenalty<float>(nx, ny, nz, beta, dims2penalize=3) (QuadPenalty.cpp:32-42)."""
+
+    def __init__(self, Nx, Ny, Nz, beta, dims2penalize=3):
+        super().__init__(0, Nx, Ny, Nz, beta, 0.0, dims2penalize)
+
+
+class TVPenalty(_Penalty):
+    """TVPenalty<float>(nx, ny, nz, beta, delta, dims2penalize=3) (TVPenalty.cpp:32-41)."""
+
+    def __init__(self, Nx, Ny, Nz, beta, delta, dims2penalize=3):
+        super().__init__(1, Nx, Ny, Nz, beta, delta, dims2penalize)
+
+
+def solve_pwls_pcg(x0, A, W, yi, R, niter, mode=CG_GRAPH, stream=None):
+    """solve_pwls_pcg(xInitial, A, W, yi, R, niter) (solve_pwls_pcg.hpp:48-167),
+    run on the device.  Arguments are numpy arrays or DeviceArrays; the
+    result has the type of yi.  Returns (x, info dict)."""
+    y = _as_device(yi, np.complex64)
+    x0d = None if x0 is None else _as_device(x0, np.complex64)
+    Wd = None if W is None else _as_device(W, np.float32)
+    out = DeviceArray(A.n_in, np.complex64)
+    resid = (C.c_double * max(int(niter), 1))()
+    info = CgInfo()
+    check(lib().pg_solve_pwls_pcg(A.handle, R.handle if R is not None else None,
+                                  None if x0d is None else x0d.ptr,
+                                  None if Wd is None else Wd.ptr, y.ptr, int(niter), int(mode),
+                                  out.ptr, resid, C.byref(info), _sh(stream)))
+    res = dict(iterations=info.iterations, exit_reason=EXIT_REASONS.get(info.exit_reason, "?"),
+               resid=np.array(resid[:info.iterations]))
+    if not isinstance(yi, DeviceArray):
+        out = out.to_host()
+    return out, res
+
+
+def reconSolve(data, Sg, R, kx=None, ky=None, kz=None, Nx=None, Ny=None, Nz=None, tvec=None,
+               niter=10, mode=CG_GRAPH, stream=None):
+    """reconSolve(data, Sg, R, kx, ky, kz, Nx, Ny, Nz, tvec, niter)
+    (reconSolve.h:63-82): W = ones, x0 = 0."""
+    return solve_pwls_pcg(None, Sg, None, data, R, niter, mode, stream)
+
+
+class profile:
+    """Per-kernel timing from HIP events recorded by the library."""
+
+    @staticmethod
+    def enable(on=True):
+        check(lib().pg_profile_enable(1 if on else 0))
+
+    @staticmethod
+    def reset():
+        check(lib().pg_profile_reset())
+
+    @staticmethod
+    def query(name):
+        n, ms = C.c_int(), C.c_double()
+        check(lib().pg_profile_query(name.encode(), C.byref(n), C.byref(ms)))
+        return n.value, ms.value
+
+
+__all__ = ["DeviceArray", "Stream", "Gnufft", "Gdft", "TimeSegmentation", "SENSE", "QuadPenalty", "TVPenalty",
+           "solve_pwls_pcg", "reconSolve", "profile", "PgError", "MODE_BATCHED", "MODE_PLAIN",
+           "CG_GRAPH", "CG_HOST"]

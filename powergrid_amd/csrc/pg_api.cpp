@@ -1,0 +1,1236 @@
+// pg_api.cpp -- C ABI (include/powergrid_amd.h): operator handles with the
+// reference's constructor arguments, forward/adjoint dispatch (batched or
+// one NUFFT at a time), penalties and the device-resident PWLS-PCG.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "pg_dft.h"
+#include "pg_gnufft.h"
+#include "pg_internal.h"
+#include "pg_vec.h"
+
+namespace pg {
+
+static thread_local std::string g_last_error;
+void set_last_error(const std::string& s) { g_last_error = s; }
+
+// ----------------------------------------------------------- profiling --
+struct ProfRec {
+    std::string name;
+    hipEvent_t a, b;
+};
+static bool g_prof = false;
+static std::vector<ProfRec> g_prof_recs;
+static std::mutex g_prof_mu;
+
+struct Prof {
+    bool on = false;
+    hipStream_t s;
+    ProfRec r;
+    Prof(const char* name, hipStream_t st) : s(st)
+    {
+        if (!g_prof) return;
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        hipStreamIsCapturing(st, &cs);
+        if (cs != hipStreamCaptureStatusNone) return;
+        on = true;
+        r.name = name;
+        hipEventCreate(&r.a);
+        hipEventCreate(&r.b);
+        hipEventRecord(r.a, s);
+    }
+    ~Prof()
+    {
+        if (!on) return;
+        hipEventRecord(r.b, s);
+        std::lock_guard<std::mutex> lk(g_prof_mu);
+        g_prof_recs.push_back(r);
+    }
+};
+
+// ---------------------------------------------------------------- FFT --
+bool fft_length_supported(int n)
+{
+    if (n < 1) return false;
+    while (n % 2 == 0) n /= 2;
+    while (n % 3 == 0) n /= 3;
+    return n == 1;
+}
+
+FftPlan1D make_fft_plan(int n)
+{
+    PG_REQUIRE(fft_length_supported(n), PG_ERR_UNSUPPORTED,
+               "FFT length " + std::to_string(n) + " is not of the form 2^a 3^b");
+    FftPlan1D p;
+    p.n = n;
+    int rem = n;
+    while (rem > 1) {
+        PG_REQUIRE(p.nst < kMaxStages, PG_ERR_UNSUPPORTED, "FFT length too large");
+        int r = (rem % 4 == 0) ? 4 : (rem % 2 == 0) ? 2 : 3;
+        p.radix[p.nst++] = r;
+        rem /= r;
+    }
+    std::vector<float2> tw(n);
+    for (int k = 0; k < n; ++k) {
+        const double a = -2.0 * M_PI * (double)k / (double)n;
+        tw[k] = make_float2((float)std::cos(a), (float)std::sin(a));
+    }
+    PG_HIP(hipMalloc(&p.tw, sizeof(float2) * n));
+    PG_HIP(hipMemcpy(p.tw, tw.data(), sizeof(float2) * n, hipMemcpyHostToDevice));
+    return p;
+}
+
+void free_fft_plan(FftPlan1D& p)
+{
+    if (p.tw) hipFree(p.tw);
+    p.tw = nullptr;
+}
+
+template <class T>
+static T* dev_alloc(size_t n)
+{
+    if (n == 0) n = 1;
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, sizeof(T) * n);
+    if (e != hipSuccess)
+        throw Error{PG_ERR_OOM, "hipMalloc of " + std::to_string(sizeof(T) * n) + " bytes failed"};
+    return (T*)p;
+}
+
+template <class T>
+static T* dev_copy(const T* h, size_t n)
+{
+    T* d = dev_alloc<T>(n);
+    if (h && n) PG_HIP(hipMemcpy(d, h, sizeof(T) * n, hipMemcpyHostToDevice));
+    return d;
+}
+
+template <class T>
+static void dev_free(T*& p)
+{
+    if (p) hipFree((void*)p);
+    p = nullptr;
+}
+
+// ------------------------------------------------------ Gnufft setup ----
+// griddingSupport.cpp:33-48 (Numerical Recipes I0; evaluated in double, as
+// the reference's float LUT build promotes to double)
+static double bessi0(double x)
+{
+    double ax, ans, y;
+    if ((ax = std::fabs(x)) < 3.75) {
+        y = x / 3.75;
+        y = y * y;
+        ans = 1.0 + y * (3.5156229 + y * (3.0899424 + y * (1.2067492 + y * (0.2659732 +
+              y * (0.360768e-1 + y * 0.45813e-2)))));
+    } else {
+        y = 3.75 / ax;
+        ans = (std::exp(ax) / std::sqrt(ax)) * (0.39894228 + y * (0.1328592e-1 + y * (0.225319e-2 +
+              y * (-0.157565e-2 + y * (0.916281e-2 + y * (-0.2057706e-1 + y * (0.2635537e-1 +
+              y * (-0.1647633e-1 + y * 0.392377e-2))))))));
+    }
+    return ans;
+}
+
+static const float kPiF = 3.1415926535897932384626433832795029f;  // MRI_PI (PGIncludes.h:69)
+
+// KB transform at u, float arithmetic of griddingSupport.cpp:249-260
+static float kb_ft(float W, float beta, float u)
+{
+    float e = kPiF * kPiF * W * W * u * u - beta * beta;
+    if (e >= 0) return std::sin(std::sqrt(e)) / std::sqrt(e);
+    return std::sinh(std::sqrt((float)-1.0 * e)) / std::sqrt((float)-1.0 * e);
+}
+
+static void host_window(float k, int N, float os, float halfw, int& lo, int& lo_c, int& hi)
+{
+    const float s = os * (k + (float)N / 2.0f);
+    lo = (int)std::ceil(s - halfw);
+    lo_c = std::max(0, lo);
+    hi = (int)std::fmin(os * (float)N - 1.0f, std::floor(s + halfw));
+}
+
+void Gnufft::ensure_batch(int B)
+{
+    if (B <= cap_batch) return;
+    dev_free(grids);
+    dev_free(tbuf);
+    dev_free(tbuf2);
+    const Geometry& g = geo;
+    grids = dev_alloc<float2>((size_t)B * g.G());
+    tbuf = dev_alloc<float2>((size_t)B * g.Nz * g.Nx * g.Gy);
+    if (g.dims == 3) tbuf2 = dev_alloc<float2>((size_t)B * g.Nz * g.Gx * g.Gy);
+    cap_batch = B;
+}
+
+void Gnufft::release()
+{
+    dev_free(kx);
+    dev_free(ky);
+    dev_free(kz);
+    dev_free(lut);
+    dev_free(dfx);
+    dev_free(dfy);
+    dev_free(dfz);
+    free_fft_plan(px);
+    free_fft_plan(py);
+    free_fft_plan(pz);
+    dev_free(tile_off);
+    dev_free(tile_idx);
+    dev_free(grids);
+    dev_free(tbuf);
+    dev_free(tbuf2);
+    dev_free(img_tmp);
+    dev_free(img_tmp2);
+    dev_free(dat_tmp);
+    dev_free(dat_tmp2);
+    cap_batch = 0;
+}
+
+// Gnufft.cpp:34-122
+static void gnufft_setup(Gnufft& G, size_t M, float os, size_t Nx, size_t Ny, size_t Nz,
+                         const float* kx, const float* ky, const float* kz, float W)
+{
+    PG_REQUIRE(M > 0 && Nx > 0 && Ny > 0 && Nz > 0, PG_ERR_ARG, "empty Gnufft");
+    PG_REQUIRE(kx && ky, PG_ERR_ARG, "kx/ky required");
+    PG_REQUIRE(Nz == 1 || kz, PG_ERR_ARG, "kz required for 3-D");
+    PG_REQUIRE(M < (1u << 31), PG_ERR_ARG, "too many samples");
+    PG_REQUIRE(Nx == Ny, PG_ERR_UNSUPPORTED,
+               "Nx != Ny: the reference's zero_pad2d/circshift2 index x-fastest and are only "
+               "consistent for square images");
+    Geometry& g = G.geo;
+    g.Nx = (int)Nx;
+    g.Ny = (int)Ny;
+    g.Nz = (int)Nz;
+    g.dims = Nz > 1 ? 3 : 2;
+    g.os = os;
+    // gridding.cpp:440-446
+    g.Gx = (int)std::ceil(os * (float)Nx);
+    if (g.Gx % 2) g.Gx += 1;
+    g.Gy = (int)std::ceil(os * (float)Ny);
+    if (g.Gy % 2) g.Gy += 1;
+    g.Gz = (Nz == 1) ? 1 : (int)std::ceil(os * (float)Nz);
+    PG_REQUIRE(std::fabs(os * (float)Nx - g.Gx) < 1e-6f && std::fabs(os * (float)Ny - g.Gy) < 1e-6f,
+               PG_ERR_UNSUPPORTED, "gridos*N must be an even integer");
+    PG_REQUIRE(Nz == 1 || (std::fabs(os * (float)Nz - g.Gz) < 1e-6f && g.Gz % 2 == 0),
+               PG_ERR_UNSUPPORTED, "gridos*Nz must be an even integer");
+    g.W = W;
+    g.ntaps = (int)std::ceil(W * os - 1e-6f);
+    PG_REQUIRE(g.ntaps == 8 || g.ntaps == 6, PG_ERR_UNSUPPORTED,
+               "kernelWidth*gridos must be 6 or 8 taps");
+    // Gnufft.cpp:85-88
+    g.beta = (float)((double)kPiF * std::sqrt((os - 0.5) * (os - 0.5) * (W * W * 4.0) /
+                                              (os * os) - 0.8));
+    // griddingSupport.cpp:469-470, 516-518
+    g.offX = (int)(((float)Nx * os / 2.0f) - ((float)Nx / 2.0f));
+    g.offY = (int)(((float)Ny * os / 2.0f) - ((float)Ny / 2.0f));
+    g.offZ = (Nz > 1) ? (int)(((float)Nz * os / 2.0f) - ((float)Nz / 2.0f)) : 0;
+    PG_REQUIRE(g.Ny <= 1024, PG_ERR_UNSUPPORTED, "Ny > 1024");
+    G.M = (int)M;
+
+    // range warning of Gnufft.cpp:67-81 is reported through pg_last_error only
+    G.kx = dev_copy(kx, M);
+    G.ky = dev_copy(ky, M);
+    std::vector<float> zero;
+    if (!kz) zero.assign(M, 0.0f);
+    G.kz = dev_copy(kz ? kz : zero.data(), M);
+
+    // LUT (griddingSupport.cpp:52-76)
+    G.S = (int)(size_t)(10000 * W);
+    std::vector<float> lut(G.S);
+    for (int k = 0; k < G.S; ++k) {
+        const float v = (float)k / (float)G.S;
+        lut[k] = (float)(bessi0((double)g.beta * std::sqrt(1.0 - (double)v)) / (double)W);
+    }
+    G.lut = dev_copy(lut.data(), lut.size());
+
+    // per-axis deapodization factors (griddingSupport.cpp:246-260, 327-348)
+    std::vector<float> fx(Nx), fy(Ny), fz(Nz, 1.0f);
+    for (size_t X = 0; X < Nx; ++X) {
+        const float u = (g.dims == 2) ? (float)((int)X - (int)(Nx / 2)) / (float)Nx
+                                      : (float)((double)X - ((float)Nx / 2.0)) / (float)Nx;
+        fx[X] = kb_ft(W, g.beta, u);
+    }
+    for (size_t Y = 0; Y < Ny; ++Y) {
+        const float u = (g.dims == 2) ? (float)((int)Y - (int)(Ny / 2)) / (float)Ny
+                                      : (float)((double)Y - ((float)Ny / 2.0)) / (float)Ny;
+        fy[Y] = kb_ft(W, g.beta, u);
+    }
+    if (g.dims == 3)
+        for (size_t Z = 0; Z < Nz; ++Z) {
+            const float u = (float)((double)Z - ((float)Nz / 2.0)) / (float)Nz;
+            fz[Z] = kb_ft(W, g.beta, u);
+        }
+    G.dfx = dev_copy(fx.data(), Nx);
+    G.dfy = dev_copy(fy.data(), Ny);
+    G.dfz = dev_copy(fz.data(), Nz);
+    G.inv_osd = (g.dims == 2) ? 1.0f / (os * os) : (float)(1.0 / (double)(os * os * os));
+
+    G.px = make_fft_plan(g.Gx);
+    G.py = make_fft_plan(g.Gy);
+    if (g.dims == 3) G.pz = make_fft_plan(g.Gz);
+
+    // spread tiles
+    const int TX = g.dims == 2 ? 16 : 8, TY = g.dims == 2 ? 16 : 8, TZ = g.dims == 2 ? 1 : 4;
+    G.ntx = (g.Gx + TX - 1) / TX;
+    G.nty = (g.Gy + TY - 1) / TY;
+    G.ntz = (g.Gz + TZ - 1) / TZ;
+    const long long ntiles = (long long)G.ntx * G.nty * G.ntz;
+    const float halfw = W * os / 2.0f;
+    std::vector<int> cnt(ntiles + 1, 0);
+    struct R3 {
+        int x0, x1, y0, y1, z0, z1;
+    };
+    std::vector<R3> rng(M);
+    for (size_t i = 0; i < M; ++i) {
+        int lo, lc, hi;
+        R3 r;
+        host_window(kx[i], g.Nx, os, halfw, lo, lc, hi);
+        r.x0 = lc;
+        r.x1 = std::min(hi, lo + g.ntaps - 1);
+        host_window(ky[i], g.Ny, os, halfw, lo, lc, hi);
+        r.y0 = lc;
+        r.y1 = std::min(hi, lo + g.ntaps - 1);
+        if (g.dims == 3) {
+            host_window(kz[i], g.Nz, os, halfw, lo, lc, hi);
+            r.z0 = lc;
+            r.z1 = std::min(hi, lo + g.ntaps - 1);
+        } else {
+            r.z0 = r.z1 = 0;
+        }
+        rng[i] = r;
+        if (r.x1 < r.x0 || r.y1 < r.y0 || r.z1 < r.z0) continue;
+        for (int tz = r.z0 / TZ; tz <= r.z1 / TZ; ++tz)
+            for (int ty = r.y0 / TY; ty <= r.y1 / TY; ++ty)
+                for (int tx = r.x0 / TX; tx <= r.x1 / TX; ++tx)
+                    cnt[((long long)tz * G.nty + ty) * G.ntx + tx + 1]++;
+    }
+    for (long long t = 0; t < ntiles; ++t) cnt[t + 1] += cnt[t];
+    G.tile_entries = cnt[ntiles];
+    PG_REQUIRE(G.tile_entries < (1ll << 31), PG_ERR_UNSUPPORTED, "too many tile entries");
+    std::vector<int> idx(std::max<long long>(1, G.tile_entries));
+    std::vector<int> pos(cnt.begin(), cnt.end() - 1);
+    for (size_t i = 0; i < M; ++i) {
+        const R3& r = rng[i];
+        if (r.x1 < r.x0 || r.y1 < r.y0 || r.z1 < r.z0) continue;
+        for (int tz = r.z0 / TZ; tz <= r.z1 / TZ; ++tz)
+            for (int ty = r.y0 / TY; ty <= r.y1 / TY; ++ty)
+                for (int tx = r.x0 / TX; tx <= r.x1 / TX; ++tx)
+                    idx[pos[((long long)tz * G.nty + ty) * G.ntx + tx]++] = (int)i;
+    }
+    G.max_tile = 0;
+    for (long long t = 0; t < ntiles; ++t) G.max_tile = std::max(G.max_tile, cnt[t + 1] - cnt[t]);
+    G.tile_off = dev_copy(cnt.data(), cnt.size());
+    G.tile_idx = dev_copy(idx.data(), idx.size());
+
+    G.img_tmp = dev_alloc<float2>(g.N());
+    G.img_tmp2 = dev_alloc<float2>(g.N());
+    G.dat_tmp = dev_alloc<float2>(M);
+    G.dat_tmp2 = dev_alloc<float2>(M);
+}
+
+// ----------------------------------------------------------- operators --
+enum Kind { K_NUFFT = 1, K_DFT = 2, K_TS = 3, K_SENSE = 4 };
+
+struct Op {
+    int magic = 0x50474f50;
+    Kind kind;
+    size_t n_in = 0, n_out = 0;
+    int mode = PG_MODE_BATCHED;
+    virtual ~Op() {}
+};
+
+struct NufftOp : Op {
+    Gnufft G;
+    ~NufftOp() override { G.release(); }
+};
+
+struct DftOp : Op {
+    pgd::DftData d{};
+    float2* part = nullptr;
+    size_t part_cap = 0;
+    float2* tmp = nullptr;  // N (plain SENSE)
+    void ensure_part(size_t n)
+    {
+        if (n <= part_cap) return;
+        dev_free(part);
+        part = dev_alloc<float2>(n);
+        part_cap = n;
+    }
+    ~DftOp() override
+    {
+        dev_free(d.kx);
+        dev_free(d.ky);
+        dev_free(d.kz);
+        dev_free(d.t);
+        dev_free(d.ix);
+        dev_free(d.iy);
+        dev_free(d.iz);
+        dev_free(d.fm);
+        dev_free(part);
+        dev_free(tmp);
+    }
+};
+
+struct TsOp : Op {
+    Op* inner = nullptr;
+    int L = 1;
+    float tau = 0, tmin = 0;
+    float* AA = nullptr;    // L x M
+    float2* Wo = nullptr;   // L x N
+    float2* WoH = nullptr;  // L x N
+    float2* tmp_n = nullptr;
+    float2* tmp_n2 = nullptr;
+    float2* tmp_m = nullptr;
+    float2* tmp_m2 = nullptr;
+    ~TsOp() override
+    {
+        dev_free(AA);
+        dev_free(Wo);
+        dev_free(WoH);
+        dev_free(tmp_n);
+        dev_free(tmp_n2);
+        dev_free(tmp_m);
+        dev_free(tmp_m2);
+    }
+};
+
+struct SenseOp : Op {
+    Op* inner = nullptr;
+    int nc = 1;
+    size_t M = 0, N = 0;
+    float2* S = nullptr;    // nc x N
+    float2* P = nullptr;    // nc x N partial images
+    float2* tmp_n = nullptr;
+    float2* tmp_n2 = nullptr;
+    ~SenseOp() override
+    {
+        dev_free(S);
+        dev_free(P);
+        dev_free(tmp_n);
+        dev_free(tmp_n2);
+    }
+};
+
+static Op* as_op(pg_op h)
+{
+    Op* op = reinterpret_cast<Op*>(h);
+    PG_REQUIRE(op && op->magic == 0x50474f50, PG_ERR_ARG, "invalid operator handle");
+    return op;
+}
+
+// Description of a composite that maps onto the fused batched kernels.
+struct Fused {
+    Gnufft* G = nullptr;
+    int nc = 1, L = 1;
+    const float2* S = nullptr;
+    const float2* Wo = nullptr;
+    const float2* WoH = nullptr;
+    const float* AA = nullptr;
+};
+
+static bool fusable(Op* op, Fused& f)
+{
+    if (op->mode != PG_MODE_BATCHED) return false;
+    Op* in = op;
+    if (in->kind == K_SENSE) {
+        SenseOp* s = static_cast<SenseOp*>(in);
+        f.nc = s->nc;
+        f.S = s->S;
+        in = s->inner;
+        if (in->mode != PG_MODE_BATCHED) return false;
+    }
+    if (in->kind == K_TS) {
+        TsOp* t = static_cast<TsOp*>(in);
+        f.L = t->L;
+        f.Wo = t->Wo;
+        f.WoH = t->WoH;
+        f.AA = t->AA;
+        in = t->inner;
+        if (in->mode != PG_MODE_BATCHED) return false;
+    }
+    if (in->kind != K_NUFFT) return false;
+    f.G = &static_cast<NufftOp*>(in)->G;
+    return true;
+}
+
+static void op_forward(Op* op, const float2* in, float2* out, hipStream_t s);
+static void op_adjoint(Op* op, const float2* in, float2* out, hipStream_t s);
+
+static void fused_forward(const Fused& f, const float2* x, float2* y, hipStream_t s)
+{
+    Gnufft& G = *f.G;
+    const int B = f.nc * f.L;
+    G.ensure_batch(B);
+    pgk::WeightArgs w{f.S, f.Wo, f.WoH, f.AA};
+    {
+        Prof p("fwd_prep_fft", s);
+        pgk::launch_fwd_prep_fft(G, x, 0, w, f.nc, f.L, G.tbuf, G.tbuf2, G.grids, s);
+    }
+    {
+        Prof p("interp", s);
+        pgk::launch_interp(G, G.grids, f.nc, f.L, f.AA, y, G.M, s);
+    }
+}
+
+// adjoint into per-coil partial images P (nc x N), optionally of the CG
+// residual W (y - Ax) formed on the fly
+static void fused_adjoint_partial(const Fused& f, const float2* in, const float2* resid_y,
+                                  const float* Wd, float2* P, hipStream_t s)
+{
+    Gnufft& G = *f.G;
+    const int B = f.nc * f.L;
+    G.ensure_batch(B);
+    pgk::WeightArgs w{f.S, f.Wo, f.WoH, f.AA};
+    {
+        Prof p("spread", s);
+        pgk::launch_spread(G, in, G.M, f.nc, f.L, f.AA, resid_y, Wd, G.grids, s);
+    }
+    {
+        Prof p("adj_fft_post", s);
+        pgk::launch_adj_fft_post(G, G.grids, G.tbuf, G.tbuf2, w, f.nc, f.L, P, G.geo.N(), s);
+    }
+}
+
+// allocate every workspace a forward/adjoint pair can touch (so that a
+// captured graph never allocates)
+static void op_prepare(Op* op)
+{
+    Fused f;
+    if (fusable(op, f)) {
+        f.G->ensure_batch(f.nc * f.L);
+        return;
+    }
+    switch (op->kind) {
+    case K_NUFFT:
+        static_cast<NufftOp*>(op)->G.ensure_batch(1);
+        return;
+    case K_DFT: {
+        DftOp* d = static_cast<DftOp*>(op);
+        d->ensure_part(std::max(pgd::part_elems(d->d.M, d->d.N, 1, false),
+                                pgd::part_elems(d->d.M, d->d.N, 1, true)));
+        return;
+    }
+    case K_TS:
+        op_prepare(static_cast<TsOp*>(op)->inner);
+        return;
+    case K_SENSE: {
+        SenseOp* se = static_cast<SenseOp*>(op);
+        if (se->inner->kind == K_DFT) {
+            DftOp* d = static_cast<DftOp*>(se->inner);
+            d->ensure_part(std::max(pgd::part_elems(d->d.M, d->d.N, se->nc, false),
+                                    pgd::part_elems(d->d.M, d->d.N, se->nc, true)));
+        }
+        op_prepare(se->inner);
+        return;
+    }
+    }
+}
+
+static void op_forward(Op* op, const float2* in, float2* out, hipStream_t s)
+{
+    Fused f;
+    if (fusable(op, f)) {
+        fused_forward(f, in, out, s);
+        return;
+    }
+    switch (op->kind) {
+    case K_NUFFT: {
+        Fused g;
+        g.G = &static_cast<NufftOp*>(op)->G;
+        fused_forward(g, in, out, s);
+        return;
+    }
+    case K_DFT: {
+        DftOp* d = static_cast<DftOp*>(op);
+        d->ensure_part(pgd::part_elems(d->d.M, d->d.N, 1, false));
+        Prof p("gdft_fwd", s);
+        pgd::forward(d->d, in, nullptr, 1, out, d->part, s);
+        return;
+    }
+    case K_TS: {
+        // TimeSegmentation.cpp:252-296, one segment at a time
+        TsOp* t = static_cast<TsOp*>(op);
+        const long long N = (long long)t->n_in, M = (long long)t->n_out;
+        for (int l = 0; l < t->L; ++l) {
+            pgv::cmul(t->tmp_n, t->Wo + (size_t)l * N, in, N, s);
+            op_forward(t->inner, t->tmp_n, t->tmp_m, s);
+            pgv::rmul_acc(out, t->AA + (size_t)l * M, t->tmp_m, M, l == 0, s);
+        }
+        return;
+    }
+    case K_SENSE: {
+        SenseOp* se = static_cast<SenseOp*>(op);
+        const long long N = (long long)se->N, M = (long long)se->M;
+        if (se->inner->kind == K_DFT && op->mode == PG_MODE_BATCHED) {
+            DftOp* d = static_cast<DftOp*>(se->inner);
+            d->ensure_part(pgd::part_elems(d->d.M, d->d.N, se->nc, false));
+            Prof p("gdft_fwd", s);
+            pgd::forward(d->d, in, se->S, se->nc, out, d->part, s);
+            return;
+        }
+        // SENSE.cpp:59-67, one coil at a time
+        for (int c = 0; c < se->nc; ++c) {
+            pgv::cmul(se->tmp_n, se->S + (size_t)c * N, in, N, s);
+            op_forward(se->inner, se->tmp_n, out + (size_t)c * M, s);
+        }
+        return;
+    }
+    }
+}
+
+static void op_adjoint(Op* op, const float2* in, float2* out, hipStream_t s)
+{
+    Fused f;
+    if (fusable(op, f)) {
+        if (f.nc == 1) {
+            fused_adjoint_partial(f, in, nullptr, nullptr, out, s);
+        } else {
+            SenseOp* se = static_cast<SenseOp*>(op);
+            fused_adjoint_partial(f, in, nullptr, nullptr, se->P, s);
+            pgv::coil_sum(out, se->P, se->nc, (long long)se->N, s);
+        }
+        return;
+    }
+    switch (op->kind) {
+    case K_NUFFT: {
+        Fused g;
+        g.G = &static_cast<NufftOp*>(op)->G;
+        fused_adjoint_partial(g, in, nullptr, nullptr, out, s);
+        return;
+    }
+    case K_DFT: {
+        DftOp* d = static_cast<DftOp*>(op);
+        d->ensure_part(pgd::part_elems(d->d.M, d->d.N, 1, true));
+        Prof p("gdft_adj", s);
+        pgd::adjoint(d->d, in, nullptr, 1, out, d->part, s);
+        return;
+    }
+    case K_TS: {
+        // TimeSegmentation.cpp:297-324
+        TsOp* t = static_cast<TsOp*>(op);
+        const long long N = (long long)t->n_in, M = (long long)t->n_out;
+        for (int l = 0; l < t->L; ++l) {
+            pgv::rmul_acc(t->tmp_m, t->AA + (size_t)l * M, in, M, true, s);
+            op_adjoint(t->inner, t->tmp_m, t->tmp_n, s);
+            pgv::cmul_acc(out, t->tmp_n, t->WoH + (size_t)l * N, N, l == 0, false, s);
+        }
+        return;
+    }
+    case K_SENSE: {
+        SenseOp* se = static_cast<SenseOp*>(op);
+        const long long N = (long long)se->N, M = (long long)se->M;
+        if (se->inner->kind == K_DFT && op->mode == PG_MODE_BATCHED) {
+            DftOp* d = static_cast<DftOp*>(se->inner);
+            d->ensure_part(pgd::part_elems(d->d.M, d->d.N, se->nc, true));
+            Prof p("gdft_adj", s);
+            pgd::adjoint(d->d, in, se->S, se->nc, out, d->part, s);
+            return;
+        }
+        // SENSE.cpp:84-96
+        for (int c = 0; c < se->nc; ++c) {
+            op_adjoint(se->inner, in + (size_t)c * M, se->tmp_n, s);
+            pgv::cmul_acc(out, se->tmp_n, se->S + (size_t)c * N, N, c == 0, true, s);
+        }
+        return;
+    }
+    }
+}
+
+// ----------------------------------------------------------- penalty ----
+struct Penalty {
+    int magic = 0x5047504e;
+    pgv::PenSpec spec;
+    double* part = nullptr;
+    double* out = nullptr;
+    ~Penalty()
+    {
+        dev_free(part);
+        dev_free(out);
+    }
+};
+
+static Penalty* as_pen(pg_penalty h)
+{
+    Penalty* p = reinterpret_cast<Penalty*>(h);
+    PG_REQUIRE(p && p->magic == 0x5047504e, PG_ERR_ARG, "invalid penalty handle");
+    return p;
+}
+
+// --------------------------------------------------------------- PCG ----
+struct CgWork {
+    float2 *x = nullptr, *Ax = nullptr, *ngrad = nullptr, *ddir = nullptr, *Adir = nullptr;
+    float2 *g = nullptr, *r = nullptr;
+    double* part = nullptr;
+    double* resid = nullptr;
+    CgScal* sc = nullptr;
+    ~CgWork()
+    {
+        dev_free(x);
+        dev_free(Ax);
+        dev_free(ngrad);
+        dev_free(ddir);
+        dev_free(Adir);
+        dev_free(g);
+        dev_free(r);
+        dev_free(part);
+        dev_free(resid);
+        dev_free(sc);
+    }
+};
+
+// one PCG iteration (solve_pwls_pcg.hpp:80-164) as a launch sequence
+static void cg_iteration(Op* A, Penalty* R, const float2* y, const float* W, CgWork& w,
+                         hipStream_t s, const Fused* f)
+{
+    const long long N = (long long)A->n_in, M = (long long)A->n_out;
+    const pgv::PenSpec* ps = R ? &R->spec : nullptr;
+    // ngrad = A / (W (y - Ax))
+    if (f) {
+        SenseOp* se = (A->kind == K_SENSE) ? static_cast<SenseOp*>(A) : nullptr;
+        float2* P = se ? se->P : w.g;
+        fused_adjoint_partial(*f, w.Ax, y, W, P, s);
+        pgv::cg_grad(P, f->nc, N, ps, w.x, w.ngrad, w.part, w.sc, s);
+    } else {
+        pgv::resid(w.r, y, w.Ax, W, M, s);
+        op_adjoint(A, w.r, w.g, s);
+        pgv::cg_grad(w.g, 1, N, ps, w.x, w.ngrad, w.part, w.sc, s);
+    }
+    pgv::cg_dir(w.ngrad, w.ddir, N, w.part, w.sc, s);
+    op_forward(A, w.ddir, w.Adir, s);
+    pgv::cg_adir(w.Adir, y, w.Ax, W, M, w.part, w.sc, s);
+    pgv::cg_linesearch(ps, w.ddir, w.x, w.part, w.sc, s);
+    pgv::cg_update(w.Ax, w.Adir, y, M, w.x, w.ddir, N, w.part, w.sc, w.resid, s);
+}
+
+static void solve_pcg(Op* A, Penalty* R, const float2* x0, const float* W, const float2* y,
+                      size_t niter, int mode, float2* x_out, double* resid_h, pg_cg_info* info,
+                      hipStream_t user_stream)
+{
+    const long long N = (long long)A->n_in, M = (long long)A->n_out;
+    CgWork w;
+    w.x = dev_alloc<float2>(N);
+    w.Ax = dev_alloc<float2>(M);
+    w.ngrad = dev_alloc<float2>(N);
+    w.ddir = dev_alloc<float2>(N);
+    w.Adir = dev_alloc<float2>(M);
+    w.g = dev_alloc<float2>(N);
+    w.r = dev_alloc<float2>(M);
+    w.part = dev_alloc<double>(2 * pgv::kMaxParts);
+    w.resid = dev_alloc<double>(std::max<size_t>(niter, 1));
+    w.sc = dev_alloc<CgScal>(1);
+
+    Fused fz;
+    const bool fused = fusable(A, fz);
+    op_prepare(A);
+
+    hipStream_t s = user_stream;
+    hipStream_t own = nullptr;
+    if (mode == PG_CG_GRAPH && s == nullptr) {
+        PG_HIP(hipStreamSynchronize(nullptr));
+        PG_HIP(hipStreamCreateWithFlags(&own, hipStreamNonBlocking));
+        s = own;
+    }
+    struct StreamGuard {
+        hipStream_t s;
+        ~StreamGuard()
+        {
+            if (s) hipStreamDestroy(s);
+        }
+    } guard{own};
+
+    if (x0) PG_HIP(hipMemcpyAsync(w.x, x0, sizeof(float2) * N, hipMemcpyDeviceToDevice, s));
+    else pgv::fill(w.x, N, make_float2(0.f, 0.f), s);
+    op_forward(A, w.x, w.Ax, s);  // Ax = A * xInitial (:55)
+    pgv::cg_init(y, W, M, w.part, w.sc, s);
+
+    CgScal hs{};
+    if (mode == PG_CG_GRAPH) {
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        PG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        try {
+            cg_iteration(A, R, y, W, w, s, fused ? &fz : nullptr);
+        } catch (...) {
+            hipStreamEndCapture(s, &graph);
+            if (graph) hipGraphDestroy(graph);
+            throw;
+        }
+        PG_HIP(hipStreamEndCapture(s, &graph));
+        PG_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        for (size_t it = 0; it < niter; ++it) PG_HIP(hipGraphLaunch(exec, s));
+        PG_HIP(hipMemcpyAsync(&hs, w.sc, sizeof(CgScal), hipMemcpyDeviceToHost, s));
+        PG_HIP(hipStreamSynchronize(s));
+        hipGraphExecDestroy(exec);
+        hipGraphDestroy(graph);
+    } else {
+        for (size_t it = 0; it < niter; ++it) {
+            cg_iteration(A, R, y, W, w, s, fused ? &fz : nullptr);
+            // host-side branch on the device scalars, as the reference loop does
+            PG_HIP(hipMemcpyAsync(&hs, w.sc, sizeof(CgScal), hipMemcpyDeviceToHost, s));
+            PG_HIP(hipStreamSynchronize(s));
+            if (hs.stop) break;
+        }
+        if (niter == 0) {
+            PG_HIP(hipMemcpyAsync(&hs, w.sc, sizeof(CgScal), hipMemcpyDeviceToHost, s));
+            PG_HIP(hipStreamSynchronize(s));
+        }
+    }
+    PG_HIP(hipMemcpyAsync(x_out, w.x, sizeof(float2) * N, hipMemcpyDeviceToDevice, s));
+    std::vector<double> rh(std::max<size_t>(niter, 1), 0.0);
+    PG_HIP(hipMemcpyAsync(rh.data(), w.resid, sizeof(double) * rh.size(), hipMemcpyDeviceToHost, s));
+    PG_HIP(hipStreamSynchronize(s));
+    if (resid_h)
+        for (int i = 0; i < hs.iter && (size_t)i < niter; ++i) resid_h[i] = rh[i];
+    if (info) {
+        info->iterations = hs.iter;
+        info->exit_reason = hs.stop;
+        info->final_resid = hs.iter > 0 ? rh[hs.iter - 1] : 0.0;
+    }
+}
+
+}  // namespace pg
+
+// ===================================================================== ABI
+using namespace pg;
+
+#define PG_TRY try {
+#define PG_CATCH                                                     \
+    }                                                                \
+    catch (const pg::Error& e)                                       \
+    {                                                                \
+        pg::set_last_error(e.msg);                                   \
+        return e.code;                                               \
+    }                                                                \
+    catch (const std::bad_alloc&)                                    \
+    {                                                                \
+        pg::set_last_error("host allocation failed");                \
+        return PG_ERR_OOM;                                           \
+    }                                                                \
+    catch (...)                                                      \
+    {                                                                \
+        pg::set_last_error("unknown error");                         \
+        return PG_ERR_STATE;                                         \
+    }                                                                \
+    return PG_OK;
+
+extern "C" {
+
+const char* pg_status_string(int st)
+{
+    switch (st) {
+    case PG_OK: return "ok";
+    case PG_ERR_ARG: return "invalid argument";
+    case PG_ERR_UNSUPPORTED: return "unsupported";
+    case PG_ERR_HIP: return "HIP runtime error";
+    case PG_ERR_OOM: return "out of device memory";
+    case PG_ERR_STATE: return "invalid state";
+    case PG_ERR_NUMERIC: return "numerical error";
+    }
+    return "unknown status";
+}
+
+const char* pg_last_error(void) { return pg::g_last_error.c_str(); }
+const char* pg_version(void) { return "0.1.0"; }
+
+int pg_set_device(int device)
+{
+    PG_TRY PG_HIP(hipSetDevice(device));
+    PG_CATCH
+}
+int pg_malloc(void** ptr, size_t bytes)
+{
+    PG_TRY PG_REQUIRE(ptr, PG_ERR_ARG, "null ptr");
+    *ptr = nullptr;
+    if (hipMalloc(ptr, bytes ? bytes : 1) != hipSuccess) throw Error{PG_ERR_OOM, "hipMalloc failed"};
+    PG_CATCH
+}
+int pg_free(void* ptr)
+{
+    PG_TRY if (ptr) PG_HIP(hipFree(ptr));
+    PG_CATCH
+}
+int pg_memcpy_htod(void* d, const void* h, size_t n, void* s)
+{
+    PG_TRY PG_HIP(hipMemcpyAsync(d, h, n, hipMemcpyHostToDevice, (hipStream_t)s));
+    PG_HIP(hipStreamSynchronize((hipStream_t)s));
+    PG_CATCH
+}
+int pg_memcpy_dtoh(void* h, const void* d, size_t n, void* s)
+{
+    PG_TRY PG_HIP(hipMemcpyAsync(h, d, n, hipMemcpyDeviceToHost, (hipStream_t)s));
+    PG_HIP(hipStreamSynchronize((hipStream_t)s));
+    PG_CATCH
+}
+int pg_memcpy_dtod(void* d, const void* src, size_t n, void* s)
+{
+    PG_TRY PG_HIP(hipMemcpyAsync(d, src, n, hipMemcpyDeviceToDevice, (hipStream_t)s));
+    PG_CATCH
+}
+int pg_stream_create(void** s)
+{
+    PG_TRY PG_REQUIRE(s, PG_ERR_ARG, "null ptr");
+    hipStream_t st;
+    PG_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    *s = (void*)st;
+    PG_CATCH
+}
+int pg_stream_destroy(void* s)
+{
+    PG_TRY if (s) PG_HIP(hipStreamDestroy((hipStream_t)s));
+    PG_CATCH
+}
+int pg_stream_synchronize(void* s)
+{
+    PG_TRY PG_HIP(hipStreamSynchronize((hipStream_t)s));
+    PG_CATCH
+}
+
+int pg_gnufft_create(pg_op* out, size_t M, float gridos, size_t Nx, size_t Ny, size_t Nz,
+                     const float* kx, const float* ky, const float* kz, const float* ix,
+                     const float* iy, const float* iz, float kernelWidth)
+{
+    (void)ix;
+    (void)iy;
+    (void)iz;
+    PG_TRY PG_REQUIRE(out, PG_ERR_ARG, "null out");
+    *out = nullptr;
+    std::unique_ptr<NufftOp> op(new NufftOp());
+    op->kind = K_NUFFT;
+    try {
+        gnufft_setup(op->G, M, gridos, Nx, Ny, Nz, kx, ky, kz, kernelWidth);
+    } catch (...) {
+        op->G.release();
+        throw;
+    }
+    op->n_in = Nx * Ny * Nz;
+    op->n_out = M;
+    *out = reinterpret_cast<pg_op>(op.release());
+    PG_CATCH
+}
+
+int pg_gdft_create(pg_op* out, size_t M, size_t N, const float* kx, const float* ky,
+                   const float* kz, const float* ix, const float* iy, const float* iz,
+                   const float* fm, const float* t, int precision)
+{
+    PG_TRY PG_REQUIRE(out && kx && ky && ix && iy, PG_ERR_ARG, "null argument");
+    PG_REQUIRE(M > 0 && N > 0 && M < (1u << 31) && N < (1u << 31), PG_ERR_ARG, "bad sizes");
+    PG_REQUIRE(precision == 0 || precision == 1, PG_ERR_ARG, "precision must be 0 or 1");
+    *out = nullptr;
+    std::unique_ptr<DftOp> op(new DftOp());
+    op->kind = K_DFT;
+    op->n_in = N;
+    op->n_out = M;
+    std::vector<float> zm(M, 0.0f), zn(N, 0.0f);
+    op->d.M = (int)M;
+    op->d.N = (int)N;
+    op->d.precision = precision;
+    op->d.kx = dev_copy(kx, M);
+    op->d.ky = dev_copy(ky, M);
+    op->d.kz = dev_copy(kz ? kz : zm.data(), M);
+    op->d.t = dev_copy(t ? t : zm.data(), M);
+    op->d.ix = dev_copy(ix, N);
+    op->d.iy = dev_copy(iy, N);
+    op->d.iz = dev_copy(iz ? iz : zn.data(), N);
+    op->d.fm = dev_copy(fm ? fm : zn.data(), N);
+    op->tmp = dev_alloc<float2>(N);
+    *out = reinterpret_cast<pg_op>(op.release());
+    PG_CATCH
+}
+
+int pg_ts_create(pg_op* out, pg_op Gh, const float* fieldmap, const float* tvec, size_t M,
+                 size_t N, size_t L, size_t interptype, size_t shots)
+{
+    PG_TRY PG_REQUIRE(out && fieldmap && tvec, PG_ERR_ARG, "null argument");
+    *out = nullptr;
+    Op* G = as_op(Gh);
+    PG_REQUIRE(G->n_in == N && G->n_out == M, PG_ERR_ARG, "TimeSegmentation sizes do not match G");
+    PG_REQUIRE(interptype == 1, PG_ERR_UNSUPPORTED,
+               "only the Hanning interpolator (interptype 1) is implemented");
+    if (shots < 1) shots = 1;
+    PG_REQUIRE(M % shots == 0, PG_ERR_ARG, "M must be a multiple of shots");
+    std::unique_ptr<TsOp> op(new TsOp());
+    op->kind = K_TS;
+    op->inner = G;
+    op->n_in = N;
+    op->n_out = M;
+    // TimeSegmentation.cpp:46-93, 213-218
+    const int Lc = (L == 0 || L == 1) ? 1 : (int)L;
+    op->L = Lc;
+    std::vector<float> tv(tvec, tvec + M);
+    float tmin = tv[0], tmax = tv[0];
+    for (size_t j = 1; j < M; ++j) {
+        tmin = std::min(tmin, tv[j]);
+        tmax = std::max(tmax, tv[j]);
+    }
+    const float rangt = tmax - tmin;
+    float tau = (float)(((double)rangt + 2.220446049250313e-16) / (double)Lc);
+    for (size_t j = 0; j < M; ++j) tv[j] = tv[j] - tmin;
+    std::vector<float> AA((size_t)Lc * M, 0.0f);
+    std::vector<float2> Wo((size_t)Lc * N), WoH((size_t)Lc * N);
+    if (Lc <= 1) {
+        tau = 0;
+        std::fill(AA.begin(), AA.end(), 1.0f);
+        std::fill(Wo.begin(), Wo.end(), make_float2(1.f, 0.f));
+        std::fill(WoH.begin(), WoH.end(), make_float2(1.f, 0.f));
+    } else {
+        const size_t none = M / shots;
+        for (int l = 0; l < Lc; ++l)
+            for (size_t j = 0; j < none; ++j) {
+                const float d = tv[j] - ((float)(unsigned)l * tau);
+                float a = 0.0f;
+                if (std::fabs(d) <= tau) a = (float)(0.5 + 0.5 * std::cos(3.141592653589793 * (double)d / (double)tau));
+                for (size_t sh = 0; sh < shots; ++sh) AA[(size_t)l * M + j + sh * none] = a;
+            }
+        for (int l = 0; l < Lc; ++l) {
+            float sl = (float)(unsigned)l * tau;
+            sl = sl + tmin;
+            for (size_t n = 0; n < N; ++n) {
+                const float ph = (-fieldmap[n]) * sl;
+                Wo[(size_t)l * N + n] = make_float2(std::cos(ph), std::sin(ph));
+                const float phh = fieldmap[n] * sl;
+                WoH[(size_t)l * N + n] = make_float2(std::cos(phh), std::sin(phh));
+            }
+        }
+    }
+    op->tau = tau;
+    op->tmin = tmin;
+    op->AA = dev_copy(AA.data(), AA.size());
+    op->Wo = dev_copy(Wo.data(), Wo.size());
+    op->WoH = dev_copy(WoH.data(), WoH.size());
+    op->tmp_n = dev_alloc<float2>(N);
+    op->tmp_n2 = dev_alloc<float2>(N);
+    op->tmp_m = dev_alloc<float2>(M);
+    op->tmp_m2 = dev_alloc<float2>(M);
+    *out = reinterpret_cast<pg_op>(op.release());
+    PG_CATCH
+}
+
+int pg_sense_create(pg_op* out, pg_op Gh, const float* sens, size_t M, size_t N, size_t nc)
+{
+    PG_TRY PG_REQUIRE(out && sens && nc > 0, PG_ERR_ARG, "null argument");
+    *out = nullptr;
+    Op* G = as_op(Gh);
+    PG_REQUIRE(G->n_in == N && G->n_out == M, PG_ERR_ARG, "SENSE sizes do not match G");
+    std::unique_ptr<SenseOp> op(new SenseOp());
+    op->kind = K_SENSE;
+    op->inner = G;
+    op->nc = (int)nc;
+    op->M = M;
+    op->N = N;
+    op->n_in = N;
+    op->n_out = M * nc;
+    op->S = dev_copy(reinterpret_cast<const float2*>(sens), N * nc);
+    op->P = dev_alloc<float2>(N * nc);
+    op->tmp_n = dev_alloc<float2>(N);
+    op->tmp_n2 = dev_alloc<float2>(N);
+    *out = reinterpret_cast<pg_op>(op.release());
+    PG_CATCH
+}
+
+int pg_op_destroy(pg_op h)
+{
+    PG_TRY if (!h) return PG_OK;
+    Op* op = as_op(h);
+    op->magic = 0;
+    delete op;
+    PG_CATCH
+}
+
+int pg_op_shape(pg_op h, size_t* n_in, size_t* n_out)
+{
+    PG_TRY Op* op = as_op(h);
+    if (n_in) *n_in = op->n_in;
+    if (n_out) *n_out = op->n_out;
+    PG_CATCH
+}
+
+int pg_op_set_mode(pg_op h, int mode)
+{
+    PG_TRY Op* op = as_op(h);
+    PG_REQUIRE(mode == PG_MODE_BATCHED || mode == PG_MODE_PLAIN, PG_ERR_ARG, "bad mode");
+    op->mode = mode;
+    PG_CATCH
+}
+
+int pg_op_forward(pg_op h, const void* in, void* out, void* stream)
+{
+    PG_TRY Op* op = as_op(h);
+    PG_REQUIRE(in && out && in != out, PG_ERR_ARG, "bad buffers");
+    op_forward(op, (const float2*)in, (float2*)out, (hipStream_t)stream);
+    PG_HIP(hipGetLastError());
+    PG_CATCH
+}
+
+int pg_op_adjoint(pg_op h, const void* in, void* out, void* stream)
+{
+    PG_TRY Op* op = as_op(h);
+    PG_REQUIRE(in && out && in != out, PG_ERR_ARG, "bad buffers");
+    op_adjoint(op, (const float2*)in, (float2*)out, (hipStream_t)stream);
+    PG_HIP(hipGetLastError());
+    PG_CATCH
+}
+
+static int host_apply(pg_op h, const void* in, void* out, bool fwd)
+{
+    PG_TRY Op* op = as_op(h);
+    PG_REQUIRE(in && out, PG_ERR_ARG, "bad buffers");
+    const size_t ni = fwd ? op->n_in : op->n_out, no = fwd ? op->n_out : op->n_in;
+    float2* di = dev_copy((const float2*)in, ni);
+    float2* dout = nullptr;
+    try {
+        dout = dev_alloc<float2>(no);
+        if (fwd) op_forward(op, di, dout, nullptr);
+        else op_adjoint(op, di, dout, nullptr);
+        PG_HIP(hipGetLastError());
+        PG_HIP(hipMemcpy(out, dout, sizeof(float2) * no, hipMemcpyDeviceToHost));
+    } catch (...) {
+        dev_free(di);
+        dev_free(dout);
+        throw;
+    }
+    dev_free(di);
+    dev_free(dout);
+    PG_CATCH
+}
+
+int pg_op_forward_host(pg_op h, const void* in, void* out) { return host_apply(h, in, out, true); }
+int pg_op_adjoint_host(pg_op h, const void* in, void* out) { return host_apply(h, in, out, false); }
+
+int pg_penalty_create(pg_penalty* out, int kind, size_t Nx, size_t Ny, size_t Nz, float beta,
+                      float delta, size_t dims2penalize)
+{
+    PG_TRY PG_REQUIRE(out, PG_ERR_ARG, "null out");
+    PG_REQUIRE(kind == PG_PENALTY_QUAD || kind == PG_PENALTY_TV, PG_ERR_ARG, "bad penalty kind");
+    PG_REQUIRE(Nx > 0 && Ny > 0 && Nz > 0, PG_ERR_ARG, "bad sizes");
+    PG_REQUIRE(kind == PG_PENALTY_QUAD || delta != 0.0f, PG_ERR_ARG, "TV needs delta != 0");
+    *out = nullptr;
+    std::unique_ptr<Penalty> p(new Penalty());
+    p->spec.kind = kind;
+    p->spec.nd = ((Nz == 1) || (dims2penalize == 2)) ? 2 : 3;
+    p->spec.n = (long long)(Nx * Ny * Nz);
+    p->spec.off[0] = 1;
+    p->spec.off[1] = (long long)Ny;
+    p->spec.off[2] = (long long)(Nx * Ny);
+    p->spec.beta = beta;
+    p->spec.delta = delta;
+    PG_REQUIRE(p->spec.n > p->spec.off[p->spec.nd - 1], PG_ERR_ARG, "image too small for the penalty");
+    p->part = dev_alloc<double>(2 * pgv::kMaxParts);
+    p->out = dev_alloc<double>(2);
+    *out = reinterpret_cast<pg_penalty>(p.release());
+    PG_CATCH
+}
+
+int pg_penalty_destroy(pg_penalty h)
+{
+    PG_TRY if (!h) return PG_OK;
+    Penalty* p = as_pen(h);
+    p->magic = 0;
+    delete p;
+    PG_CATCH
+}
+
+int pg_penalty_gradient(pg_penalty h, const void* x, void* g, void* stream)
+{
+    PG_TRY Penalty* p = as_pen(h);
+    pgv::pen_gradient(p->spec, (const float2*)x, (float2*)g, (hipStream_t)stream);
+    PG_HIP(hipGetLastError());
+    PG_CATCH
+}
+
+int pg_penalty_denom(pg_penalty h, const void* ddir, const void* x, double* out2, void* stream)
+{
+    PG_TRY Penalty* p = as_pen(h);
+    PG_REQUIRE(out2, PG_ERR_ARG, "null out2");
+    hipStream_t s = (hipStream_t)stream;
+    pgv::pen_denom_pdot(p->spec, (const float2*)ddir, (const float2*)x, nullptr, false, p->part,
+                        p->out, nullptr, s);
+    double h2[2];
+    PG_HIP(hipMemcpyAsync(h2, p->out, sizeof(h2), hipMemcpyDeviceToHost, s));
+    PG_HIP(hipStreamSynchronize(s));
+    out2[0] = h2[0];
+    out2[1] = 0.0;
+    PG_CATCH
+}
+
+int pg_penalty_value(pg_penalty h, const void* x, double* out, void* stream)
+{
+    PG_TRY Penalty* p = as_pen(h);
+    PG_REQUIRE(out, PG_ERR_ARG, "null out");
+    hipStream_t s = (hipStream_t)stream;
+    pgv::pen_value(p->spec, (const float2*)x, p->part, p->out, s);
+    double v;
+    PG_HIP(hipMemcpyAsync(&v, p->out, sizeof(double), hipMemcpyDeviceToHost, s));
+    PG_HIP(hipStreamSynchronize(s));
+    *out = v * p->spec.beta;
+    PG_CATCH
+}
+
+int pg_solve_pwls_pcg(pg_op Ah, pg_penalty Rh, const void* x0, const float* W, const void* y,
+                      size_t niter, int mode, void* x_out, double* resid, pg_cg_info* info,
+                      void* stream)
+{
+    PG_TRY Op* A = as_op(Ah);
+    Penalty* R = Rh ? as_pen(Rh) : nullptr;
+    PG_REQUIRE(y && x_out, PG_ERR_ARG, "null y or x_out");
+    PG_REQUIRE(mode == PG_CG_GRAPH || mode == PG_CG_HOST, PG_ERR_ARG, "bad CG mode");
+    if (R) PG_REQUIRE((size_t)R->spec.n == A->n_in, PG_ERR_ARG, "penalty size != image size");
+    solve_pcg(A, R, (const float2*)x0, W, (const float2*)y, niter, mode, (float2*)x_out, resid, info,
+              (hipStream_t)stream);
+    PG_CATCH
+}
+
+int pg_recon_solve(pg_op A, pg_penalty R, const void* y, size_t niter, int mode, void* x_out,
+                   double* resid, pg_cg_info* info, void* stream)
+{
+    return pg_solve_pwls_pcg(A, R, nullptr, nullptr, y, niter, mode, x_out, resid, info, stream);
+}
+
+int pg_profile_enable(int on)
+{
+    g_prof = on != 0;
+    return PG_OK;
+}
+
+int pg_profile_reset(void)
+{
+    PG_TRY std::lock_guard<std::mutex> lk(g_prof_mu);
+    for (auto& r : g_prof_recs) {
+        hipEventSynchronize(r.b);
+        hipEventDestroy(r.a);
+        hipEventDestroy(r.b);
+    }
+    g_prof_recs.clear();
+    PG_CATCH
+}
+
+int pg_profile_query(const char* name, int* count, double* total_ms)
+{
+    PG_TRY PG_REQUIRE(name, PG_ERR_ARG, "null name");
+    std::lock_guard<std::mutex> lk(g_prof_mu);
+    int n = 0;
+    double tot = 0;
+    for (auto& r : g_prof_recs) {
+        if (r.name != name) continue;
+        PG_HIP(hipEventSynchronize(r.b));
+        float ms = 0;
+        PG_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+        tot += ms;
+        ++n;
+    }
+    if (count) *count = n;
+    if (total_ms) *total_ms = tot;
+    PG_CATCH
+}
+
+}  // extern "C"

@@ -1,0 +1,23 @@
+// pg_dft.h -- exact field-corrected DFT launchers (pg_dft.hip).
+#pragma once
+
+#include "pg_internal.h"
+
+namespace pgd {
+
+struct DftData {
+    int M, N, precision;
+    float *kx, *ky, *kz, *t;   // M
+    float *ix, *iy, *iz, *fm;  // N (fm in rad/s)
+};
+
+// partial-sum buffer size (float2 elements) needed by forward/adjoint
+size_t part_elems(int M, int N, int nc, bool adjoint);
+// out[c*M + i] for c < nc; S may be null (nc must then be 1)
+void forward(const DftData& d, const float2* x, const float2* S, int nc, float2* out, float2* part,
+             hipStream_t s);
+// out[j] = sum_c conj(S_c[j]) (E^H y_c)[j]
+void adjoint(const DftData& d, const float2* y, const float2* S, int nc, float2* out, float2* part,
+             hipStream_t s);
+
+}  // namespace pgd

@@ -1,0 +1,130 @@
+// pg_internal.h -- shared definitions for the gfx950 implementation of the
+// field-corrected SENSE path (Gnufft / TimeSegmentation / SENSE / Gdft and
+// the PWLS-PCG loop around them).
+//
+// Layout conventions (identical to the reference, PowerGrid/):
+//   image  : y-fastest, idx = y + x*Ny + z*Nx*Ny        (reconSolve.cpp:40-53)
+//   grid   : ny + nx*Gy + nz*Gx*Gy                       (gridding.cpp:100,202)
+//   k-space: coil-major blocks, sample i of coil c at c*M + i (SENSE.cpp:70)
+//   complex: interleaved float2
+// Batched grids are stored grid-major: grid b = c*L + l occupies
+// [b*G, (b+1)*G).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/powergrid_amd.h"
+
+namespace pg {
+
+// ------------------------------------------------------------ errors ----
+struct Error {
+    int code;
+    std::string msg;
+};
+
+#define PG_HIP(call)                                                        \
+    do {                                                                    \
+        hipError_t e_ = (call);                                             \
+        if (e_ != hipSuccess)                                               \
+            throw ::pg::Error{PG_ERR_HIP, std::string(#call) + ": " +       \
+                                              hipGetErrorString(e_)};       \
+    } while (0)
+
+#define PG_REQUIRE(cond, code, text)                                        \
+    do {                                                                    \
+        if (!(cond)) throw ::pg::Error{(code), (text)};                     \
+    } while (0)
+
+void set_last_error(const std::string& s);
+
+// ------------------------------------------------------- FFT plans ----
+// A 1-D length is factored into radix-4 / radix-2 / radix-3 stages.
+constexpr int kMaxStages = 16;
+struct FftPlan1D {
+    int n = 1;
+    int nst = 0;
+    int radix[kMaxStages];
+    float2* tw = nullptr;  // device: exp(-2 pi i k / n), k < n
+};
+
+bool fft_length_supported(int n);
+FftPlan1D make_fft_plan(int n);
+void free_fft_plan(FftPlan1D& p);
+
+// Kernel-side copy of a 1-D plan (passed by value).
+struct FftArgs {
+    int n;
+    int nst;
+    int radix[kMaxStages];
+    const float2* tw;
+};
+inline FftArgs fft_args(const FftPlan1D& p)
+{
+    FftArgs a;
+    a.n = p.n;
+    a.nst = p.nst;
+    for (int i = 0; i < kMaxStages; ++i) a.radix[i] = i < p.nst ? p.radix[i] : 1;
+    a.tw = p.tw;
+    return a;
+}
+
+// ------------------------------------------------------- geometry ----
+struct Geometry {
+    int Nx, Ny, Nz;     // image
+    int Gx, Gy, Gz;     // oversampled grid
+    int dims;           // 2 or 3
+    float os;           // grid oversampling (gridOS)
+    float W;            // kernel width in original-grid units (reference: 4.0)
+    float beta;         // KB shape (Gnufft.cpp:85-88)
+    int ntaps;          // taps per dimension, ceil(W*os)
+    int offX, offY, offZ;  // zero_pad offsets
+    long long N() const { return (long long)Nx * Ny * Nz; }
+    long long G() const { return (long long)Gx * Gy * Gz; }
+};
+
+// Spread binning: for each grid tile, the samples whose window intersects it.
+constexpr int kSpreadTX = 16;  // tile extent along x (grid points)
+constexpr int kSpreadTY = 16;  // tile extent along y
+constexpr int kSpreadTZ = 4;   // tile extent along z (3-D)
+
+struct Gnufft;
+
+}  // namespace pg
+
+// --------------------------------------------------- kernel launchers ----
+// (defined in the .hip files; all asynchronous on `stream`)
+namespace pgk {
+
+struct WeightArgs {
+    // image-side weights applied before the forward NUFFT and after the
+    // adjoint one.  Any pointer may be null (= 1).
+    const float2* S;    // coil maps, nc x N (SENSE.cpp:59-63 / :89-92)
+    const float2* Wo;   // segment phases exp(-i fm (l tau + Tmin)), L x N
+    const float2* WoH;  // conj(Wo), L x N
+    // data-side weights: AA (L x M, real) -- TimeSegmentation.cpp:291-293
+    const float* AA;
+};
+
+// forward: image(s) -> batched grids  (deapod, pad, fftshift, FFT, ifftshift)
+void launch_fwd_prep_fft(const pg::Gnufft& G, const float2* img, long long img_cstride,
+                         WeightArgs w, int nc, int L, float2* tbuf, float2* tbuf2, float2* grids,
+                         hipStream_t s);
+// forward: batched grids -> samples (KB interpolation, sum over segments)
+void launch_interp(const pg::Gnufft& G, const float2* grids, int nc, int L, const float* AA,
+                   float2* out, long long out_cstride, hipStream_t s);
+// adjoint: samples -> batched grids (KB spread, deterministic tiles)
+void launch_spread(const pg::Gnufft& G, const float2* in, long long in_cstride, int nc, int L,
+                   const float* AA, const float2* resid_y, const float* Wd, float2* grids,
+                   hipStream_t s);
+// adjoint: batched grids -> image(s) (ifftshift, IFFT, fftshift, crop,
+// deapod, x WoH, sum over segments, x conj(S))
+void launch_adj_fft_post(const pg::Gnufft& G, float2* grids, float2* tbuf, float2* tbuf2,
+                         WeightArgs w, int nc, int L, float2* out, long long out_cstride,
+                         hipStream_t s);
+
+}  // namespace pgk

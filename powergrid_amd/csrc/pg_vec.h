@@ -1,0 +1,57 @@
+// pg_vec.h -- elementwise, penalty and PWLS-PCG launchers (pg_vec.hip).
+#pragma once
+
+#include "pg_internal.h"
+
+// device-resident PCG scalars (solve_pwls_pcg.hpp:60-77, kept in FP64)
+struct CgScal {
+    double ywy;        // yi' W yi
+    double oldinprod;  // previous real(cdot(ngrad, ngrad))
+    double gamma;
+    double dAWAd, dAWr;
+    double2 step;      // complex step, as the reference's CxT1 step
+    int ii;            // loop index
+    int iter;          // completed updates
+    int stop;          // 0 or PG_EXIT_*
+    int first;         // ddir = ngrad on the first iteration
+};
+
+namespace pgv {
+
+constexpr int kMaxParts = 1024;  // partial sums per reduction
+
+struct PenSpec {
+    int kind;  // 0 quad, 1 TV
+    int nd;    // 2 or 3 penalised dimensions (Robject.cpp:128-133)
+    long long n;
+    long long off[3];  // 1, Ny, Nx*Ny (Robject.cpp:76)
+    float beta, delta;
+};
+
+void cmul(float2* out, const float2* a, const float2* b, long long n, hipStream_t s);
+void cmul_acc(float2* out, const float2* a, const float2* b, long long n, bool first, bool conj_b,
+              hipStream_t s);
+void rmul_acc(float2* out, const float* r, const float2* a, long long n, bool first, hipStream_t s);
+void coil_sum(float2* out, const float2* P, int nc, long long N, hipStream_t s);
+void fill(float2* out, long long n, float2 v, hipStream_t s);
+void resid(float2* r, const float2* y, const float2* Ax, const float* W, long long n, hipStream_t s);
+
+void pen_gradient(const PenSpec& p, const float2* x, float2* g, hipStream_t s);
+void pen_denom_pdot(const PenSpec& p, const float2* ddir, const float2* x, const double2* step,
+                    bool want_pdot, double* part, double* out2, const int* stop, hipStream_t s);
+void pen_value(const PenSpec& p, const float2* x, double* part, double* out, hipStream_t s);
+
+void cg_init(const float2* y, const float* W, long long n, double* part, CgScal* sc, hipStream_t s);
+void cg_grad(const float2* P, int nc, long long N, const PenSpec* p, const float2* x,
+             float2* ngrad, double* part, CgScal* sc, hipStream_t s);
+void cg_dir(const float2* ngrad, float2* ddir, long long N, double* part, CgScal* sc,
+            hipStream_t s);
+void cg_adir(const float2* Adir, const float2* y, const float2* Ax, const float* W, long long n,
+             double* part, CgScal* sc, hipStream_t s);
+void cg_linesearch(const PenSpec* p, const float2* ddir, const float2* x, double* part, CgScal* sc,
+                   hipStream_t s);
+void cg_update(float2* Ax, const float2* Adir, const float2* y, long long M, float2* x,
+               const float2* ddir, long long N, double* part, CgScal* sc, double* resid,
+               hipStream_t s);
+
+}  // namespace pgv
