@@ -1,0 +1,250 @@
+"""Benchmark of the field-corrected SENSE PWLS-PCG reconstruction on one MI355X.
+
+Workload (the metric's configuration): 2-D field-corrected SENSE, 128x128
+image, spiral 16 interleaves x 2048 samples (25.6 ms readout, 12.5 us dwell),
+8 Gaussian coils, field map of 4 Gaussian bumps (peak +/-120 Hz), time
+segmentation L=8 (Hanning), 20 PCG iterations, QuadPenalty beta=1e-3,
+kernel width 4.0 (the reference's Gnufft.cpp:84), gridos 2.  k-space is the
+FP64 exact field-corrected NUDFT of a 10-ellipse phantom plus 40 dB noise.
+
+A step is one reconstruction (20 PCG iterations) with every input already
+resident in HBM; `value` is PCG iterations per second of the batched path
+(fused coil x segment kernels, one HIP graph per iteration).  The plain path
+(one NUFFT at a time, host-driven loop, the reference's structure) is timed
+on the same inputs.  Per-kernel durations come from HIP events recorded by
+the library on its launch stream; the CPU baseline is the oracle's
+restatement of the reference CPU path on this host's cores.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def synth_data(P, prob):
+    """k-space = exact field-corrected NUDFT (FP64 trig and accumulation, on
+    the GPU) of the phantom through each coil map, plus complex noise."""
+    from powergrid_amd import synth as S
+    M, N, nc = prob.M, prob.N, prob.nc
+    D = P.Gdft(M, N, prob.kx, prob.ky, prob.kz, prob.ix, prob.iy, prob.iz, prob.fm, prob.t,
+               precision=1)
+    Sd = P.SENSE(D, prob.sens, M, N, nc)
+    y = Sd * prob.x_true.astype(np.complex64)
+    return S.add_noise(y.astype(np.complex128), 40.0, seed=11).astype(np.complex64)
+
+
+def build_ops(P, prob, mode):
+    G = P.Gnufft(prob.M, 2.0, prob.Nx, prob.Ny, prob.Nz, prob.kx, prob.ky, prob.kz,
+                 kernel_width=prob.kernel_width)
+    T = P.TimeSegmentation(G, prob.fm, prob.t, prob.M, prob.N, prob.L)
+    Sg = P.SENSE(T, prob.sens, prob.M, prob.N, prob.nc)
+    for op in (G, T, Sg):
+        op.set_mode(mode)
+    R = P.QuadPenalty(prob.Nx, prob.Ny, prob.Nz, prob.beta)
+    return G, T, Sg, R
+
+
+def gridding_bytes(M, G, B, d):
+    """algorithmic bytes of one gridding launch: coordinates once, every grid
+    once, the samples of every grid once: 4 d M + B (8 M + 8 G)"""
+    return 4 * d * M + B * (8 * M + 8 * G)
+
+
+def fft_bytes(G, B):
+    return 16 * B * G
+
+
+def cpu_baseline(prob, y, budget_s=20.0):
+    """The oracle's restatement of the reference CPU path (Gnufft/TS/SENSE +
+    solve_pwls_pcg, FP32, OpenMP over samples/pixels), timed on this host."""
+    from oracle import oracle as O
+    threads = min(16, os.cpu_count() or 1)
+    O.set_threads(threads)
+    G = O.Gnufft(prob.M, 2.0, prob.Nx, prob.Ny, prob.Nz, prob.kx, prob.ky, prob.kz,
+                 prob.kernel_width)
+    T = O.TimeSegmentation(G, prob.fm, prob.t, prob.M, prob.N, prob.L)
+    Sg = O.SENSE(T, prob.sens.ravel(), prob.M, prob.N, prob.nc)
+    R = O.QuadPenalty(prob.Nx, prob.Ny, prob.Nz, prob.beta)
+    # one warm-up iteration, then as many single-iteration solves as fit the budget
+    O.reconSolve(y, Sg, R, prob.Nx, prob.Ny, prob.Nz, 1)
+    n, t0 = 0, time.perf_counter()
+    while True:
+        O.reconSolve(y, Sg, R, prob.Nx, prob.Ny, prob.Nz, 2)
+        n += 2
+        if time.perf_counter() - t0 > budget_s / 2 or n >= 6:
+            break
+    dt = time.perf_counter() - t0
+    # the timed solves include one A*x0 forward each; report iterations/s of
+    # the whole solve call (reference loop structure)
+    return dict(value=n / dt, unit="CG iterations/s", cores=threads, kind="port",
+                sample=f"{n} PCG iterations in {n // 2} solves of 2 iterations "
+                       f"(same inputs, FP32, {threads} OpenMP threads)")
+
+
+def fp64_oracle_recon(prob, y, niter):
+    """FP64 evaluation of the same discretised operator (same L, Hanning, LUT,
+    W) -- the NRMSE gate of the reconstruction."""
+    from oracle import oracle as O
+    O.set_threads(min(16, os.cpu_count() or 1))
+    G = O.Gnufft(prob.M, 2.0, prob.Nx, prob.Ny, prob.Nz, prob.kx, prob.ky, prob.kz,
+                 prob.kernel_width, prec="d")
+    T = O.TimeSegmentation(G, prob.fm, prob.t, prob.M, prob.N, prob.L)
+    Sg = O.SENSE(T, prob.sens.ravel(), prob.M, prob.N, prob.nc)
+    R = O.QuadPenalty(prob.Nx, prob.Ny, prob.Nz, prob.beta, prec="d")
+    x, n, _ = O.reconSolve(y.astype(np.complex128), Sg, R, prob.Nx, prob.Ny, prob.Nz, niter)
+    return x
+
+
+def nrmse(a, b):
+    return float(np.linalg.norm(np.ravel(a) - np.ravel(b)) / np.linalg.norm(np.ravel(b)))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-oracle", action="store_true", help="skip the FP64 oracle recon")
+    ap.add_argument("--plain-steps", type=int, default=2)
+    ap.add_argument("--beta", type=float, default=None, help="QuadPenalty weight override")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path "
+              f"(asked for --gpus {args.gpus}).", file=sys.stderr)
+        sys.exit(2)
+
+    import powergrid_amd as P
+    from powergrid_amd import synth as S
+
+    prob = S.config_sense2d(seed=0)
+    if args.beta is not None:
+        prob.beta = args.beta
+    niter = prob.niter
+    log(f"[bench] workload 128x128 spiral 16x2048 nc={prob.nc} L={prob.L} niter={niter}")
+    y = synth_data(P, prob)
+    yd = P.DeviceArray.from_host(y)
+    stream = P.Stream()
+
+    # ---------------- batched path: fused kernels + graph-captured PCG -----
+    G, T, Sg, R = build_ops(P, prob, P.MODE_BATCHED)
+    for _ in range(args.warmup):
+        x, info = P.reconSolve(yd, Sg, R, niter=niter, mode=P.CG_GRAPH, stream=stream)
+    stream.synchronize()
+    lat = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ts = time.perf_counter()
+        x, info = P.reconSolve(yd, Sg, R, niter=niter, mode=P.CG_GRAPH, stream=stream)
+        lat.append(time.perf_counter() - ts)
+    stream.synchronize()
+    dt = time.perf_counter() - t0
+    iters_done = info["iterations"]
+    value = args.steps * iters_done / dt
+    x_gpu = x.to_host()
+    log(f"[bench] batched: {value:.1f} it/s, {1e3 * dt / args.steps:.2f} ms/recon, "
+        f"exit={info['exit_reason']} iters={iters_done}")
+
+    # ---------------- per-kernel durations (HIP events on the launch stream) --
+    P.profile.enable(True)
+    P.profile.reset()
+    xd = P.DeviceArray.from_host(x_gpu)
+    nrep = 20
+    for _ in range(nrep):
+        ad = Sg.forward(xd, stream=stream)
+        Sg.adjoint(ad, stream=stream)
+    stream.synchronize()
+    kern = {}
+    for name in ("fwd_prep_fft", "interp", "spread", "adj_fft_post"):
+        n, ms = P.profile.query(name)
+        kern[name] = ms / max(n, 1)
+    P.profile.enable(False)
+    M, Gsz, B = prob.M, 4 * prob.N, prob.nc * prob.L
+    gb = gridding_bytes(M, Gsz, B, 2)
+    fb = fft_bytes(Gsz, B)
+    kinfo = {
+        "interp": dict(ms=kern["interp"], bytes=gb),
+        "spread": dict(ms=kern["spread"], bytes=gb),
+        "fwd_prep_fft": dict(ms=kern["fwd_prep_fft"], bytes=fb),
+        "adj_fft_post": dict(ms=kern["adj_fft_post"], bytes=fb),
+    }
+    for k, v in kinfo.items():
+        v["GBs"] = v["bytes"] / (v["ms"] * 1e-3) / 1e9 if v["ms"] > 0 else 0.0
+        v["frac"] = v["GBs"] / HBM_PEAK_GBS
+        log(f"[bench] {k:13s} {v['ms'] * 1e3:8.1f} us  {v['GBs']:8.1f} GB/s  ({100 * v['frac']:.1f}% of HBM)")
+    dom = max(kinfo, key=lambda k: kinfo[k]["ms"])
+    roof = dict(bound="hbm", achieved=round(kinfo[dom]["GBs"], 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=round(kinfo[dom]["frac"], 4), traffic=None, kernel=dom,
+                algorithmic_bytes=kinfo[dom]["bytes"], avg_launch_ms=round(kinfo[dom]["ms"], 5))
+
+    # ---------------- plain path: one NUFFT at a time, host-driven PCG ------
+    G2, T2, Sg2, R2 = build_ops(P, prob, P.MODE_PLAIN)
+    P.reconSolve(yd, Sg2, R2, niter=2, mode=P.CG_HOST, stream=stream)
+    t0 = time.perf_counter()
+    for _ in range(args.plain_steps):
+        xp, infop = P.reconSolve(yd, Sg2, R2, niter=niter, mode=P.CG_HOST, stream=stream)
+    stream.synchronize()
+    dtp = time.perf_counter() - t0
+    plain = args.plain_steps * infop["iterations"] / dtp
+    log(f"[bench] plain: {plain:.1f} it/s")
+    x_plain = xp.to_host()
+
+    out = {
+        "metric": "SENSE-TS CG iters/s batched vs plain at NRMSE<=1e-3; p50/p99 recon latency",
+        "value": round(value, 2),
+        "unit": "CG iterations/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * dt / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "2D field-corrected SENSE 128x128, spiral 16x2048, 8 coils, "
+                        "L=8 Hanning, 20 PCG iterations, QuadPenalty 1e-3, W=4, os=2",
+            "step": "one reconstruction (20 PCG iterations), inputs resident in HBM",
+            "cg_scalars": "f64", "grids_per_launch": B,
+        },
+        "plain_iters_per_s": round(plain, 2),
+        "speedup_vs_plain": round(value / plain, 2),
+        "recon_latency_ms": {"p50": round(1e3 * float(np.percentile(lat, 50)), 3),
+                             "p99": round(1e3 * float(np.percentile(lat, 99)), 3)},
+        "nrmse_plain_vs_batched": nrmse(x_plain, x_gpu),
+        "kernels_us": {k: round(v["ms"] * 1e3, 2) for k, v in kinfo.items()},
+        "roofline": roof,
+    }
+    if not args.no_oracle:
+        t0 = time.perf_counter()
+        x64 = fp64_oracle_recon(prob, y, niter)
+        out["nrmse_vs_fp64_oracle"] = nrmse(x_gpu, x64)
+        out["nrmse_vs_phantom"] = nrmse(x_gpu, prob.x_true)
+        log(f"[bench] fp64 oracle recon {time.perf_counter() - t0:.1f}s, nrmse {out['nrmse_vs_fp64_oracle']:.2e}")
+    if not args.no_cpu:
+        cb = cpu_baseline(prob, y)
+        out["cpu_baseline"] = cb
+        out["speedup_vs_cpu"] = round(value / cb["value"], 1)
+        log(f"[bench] cpu baseline {cb['value']:.3f} it/s on {cb['cores']} threads")
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -142,22 +142,12 @@ static double bessi0(double x)
     return ans;
 }
 
-static const float kPiF = 3.1415926535897932384626433832795029f;  // MRI_PI (PGIncludes.h:69)
-
-// KB transform at u, float arithmetic of griddingSupport.cpp:249-260
-static float kb_ft(float W, float beta, float u)
+// KB transform psi(u) of griddingSupport.cpp:249-260, evaluated in double
+static double kb_ft(double W, double beta, double u)
 {
-    float e = kPiF * kPiF * W * W * u * u - beta * beta;
+    const double e = M_PI * M_PI * W * W * u * u - beta * beta;
     if (e >= 0) return std::sin(std::sqrt(e)) / std::sqrt(e);
-    return std::sinh(std::sqrt((float)-1.0 * e)) / std::sqrt((float)-1.0 * e);
-}
-
-static void host_window(float k, int N, float os, float halfw, int& lo, int& lo_c, int& hi)
-{
-    const float s = os * (k + (float)N / 2.0f);
-    lo = (int)std::ceil(s - halfw);
-    lo_c = std::max(0, lo);
-    hi = (int)std::fmin(os * (float)N - 1.0f, std::floor(s + halfw));
+    return std::sinh(std::sqrt(-e)) / std::sqrt(-e);
 }
 
 void Gnufft::ensure_batch(int B)
@@ -166,35 +156,71 @@ void Gnufft::ensure_batch(int B)
     dev_free(grids);
     dev_free(tbuf);
     dev_free(tbuf2);
+    dev_free(partial);
     const Geometry& g = geo;
     grids = dev_alloc<float2>((size_t)B * g.G());
     tbuf = dev_alloc<float2>((size_t)B * g.Nz * g.Nx * g.Gy);
     if (g.dims == 3) tbuf2 = dev_alloc<float2>((size_t)B * g.Nz * g.Gx * g.Gy);
+    if (n_slots > 0) partial = dev_alloc<float2>((size_t)((B + 63) / 64) * n_slots * 16 * 64);
     cap_batch = B;
 }
 
 void Gnufft::release()
 {
-    dev_free(kx);
-    dev_free(ky);
-    dev_free(kz);
-    dev_free(lut);
+    dev_free(lo);
+    dev_free(w);
+    dev_free(wpad);
     dev_free(dfx);
     dev_free(dfy);
     dev_free(dfz);
     free_fft_plan(px);
     free_fft_plan(py);
     free_fft_plan(pz);
+    dev_free(items);
+    dev_free(entries);
+    dev_free(combine);
     dev_free(tile_off);
     dev_free(tile_idx);
     dev_free(grids);
     dev_free(tbuf);
     dev_free(tbuf2);
+    dev_free(partial);
     dev_free(img_tmp);
-    dev_free(img_tmp2);
     dev_free(dat_tmp);
-    dev_free(dat_tmp2);
     cap_batch = 0;
+}
+
+// Per-dimension window of one sample: unclamped start and the NT weights.
+// gridding.cpp:64-97 evaluated in double: s = os (k + N/2); taps
+// [max(0, ceil(s - W os/2)), min(os N - 1, floor(s + W os/2))]; weight
+// LUT[(int)(d^2 4/W^2 S)], d = |s - n|/os, 0 past the end of the table.
+struct Win1 {
+    int lo;       // first tap (unclamped)
+    int vlo, vhi; // valid tap range (inclusive; empty if vhi < vlo)
+};
+
+static Win1 window1(double k, int N, double os, double W, int NT, const std::vector<double>& lut,
+                    float* wout)
+{
+    const double s = os * (k + (double)N / 2.0);
+    const double hw = W * os / 2.0;
+    Win1 r;
+    r.lo = (int)std::ceil(s - hw);
+    const int hi = (int)std::min(os * (double)N - 1.0, std::floor(s + hw));
+    r.vlo = std::max(0, r.lo);
+    r.vhi = std::min(hi, r.lo + NT - 1);
+    const int S = (int)lut.size();
+    for (int j = 0; j < NT; ++j) {
+        const int n = r.lo + j;
+        double wt = 0.0;
+        if (n >= r.vlo && n <= r.vhi) {
+            const double d = std::fabs(s - (double)n) / os;
+            const long long k0 = (long long)((d * d * 4.0 / (W * W)) * (double)S);
+            wt = (k0 >= S) ? 0.0 : lut[k0];
+        }
+        wout[j] = (float)wt;
+    }
+    return r;
 }
 
 // Gnufft.cpp:34-122
@@ -204,7 +230,7 @@ static void gnufft_setup(Gnufft& G, size_t M, float os, size_t Nx, size_t Ny, si
     PG_REQUIRE(M > 0 && Nx > 0 && Ny > 0 && Nz > 0, PG_ERR_ARG, "empty Gnufft");
     PG_REQUIRE(kx && ky, PG_ERR_ARG, "kx/ky required");
     PG_REQUIRE(Nz == 1 || kz, PG_ERR_ARG, "kz required for 3-D");
-    PG_REQUIRE(M < (1u << 31), PG_ERR_ARG, "too many samples");
+    PG_REQUIRE(M < (1u << 30), PG_ERR_ARG, "too many samples");
     PG_REQUIRE(Nx == Ny, PG_ERR_UNSUPPORTED,
                "Nx != Ny: the reference's zero_pad2d/circshift2 index x-fastest and are only "
                "consistent for square images");
@@ -222,121 +248,176 @@ static void gnufft_setup(Gnufft& G, size_t M, float os, size_t Nx, size_t Ny, si
     g.Gz = (Nz == 1) ? 1 : (int)std::ceil(os * (float)Nz);
     PG_REQUIRE(std::fabs(os * (float)Nx - g.Gx) < 1e-6f && std::fabs(os * (float)Ny - g.Gy) < 1e-6f,
                PG_ERR_UNSUPPORTED, "gridos*N must be an even integer");
-    PG_REQUIRE(Nz == 1 || (std::fabs(os * (float)Nz - g.Gz) < 1e-6f && g.Gz % 2 == 0),
-               PG_ERR_UNSUPPORTED, "gridos*Nz must be an even integer");
+    PG_REQUIRE(g.Gx % 4 == 0, PG_ERR_UNSUPPORTED, "gridos*N must be a multiple of 4");
+    PG_REQUIRE(Nz == 1 || (std::fabs(os * (float)Nz - g.Gz) < 1e-6f && g.Gz % 4 == 0),
+               PG_ERR_UNSUPPORTED, "gridos*Nz must be a multiple of 4");
     g.W = W;
     g.ntaps = (int)std::ceil(W * os - 1e-6f);
     PG_REQUIRE(g.ntaps == 8 || g.ntaps == 6, PG_ERR_UNSUPPORTED,
                "kernelWidth*gridos must be 6 or 8 taps");
-    // Gnufft.cpp:85-88
-    g.beta = (float)((double)kPiF * std::sqrt((os - 0.5) * (os - 0.5) * (W * W * 4.0) /
-                                              (os * os) - 0.8));
+    // Gnufft.cpp:85-88 (beta kept in double for the tables below)
+    const double beta_d = M_PI * std::sqrt(((double)os - 0.5) * ((double)os - 0.5) *
+                                           ((double)W * W * 4.0) / ((double)os * os) - 0.8);
+    g.beta = (float)beta_d;
     // griddingSupport.cpp:469-470, 516-518
     g.offX = (int)(((float)Nx * os / 2.0f) - ((float)Nx / 2.0f));
     g.offY = (int)(((float)Ny * os / 2.0f) - ((float)Ny / 2.0f));
     g.offZ = (Nz > 1) ? (int)(((float)Nz * os / 2.0f) - ((float)Nz / 2.0f)) : 0;
     PG_REQUIRE(g.Ny <= 1024, PG_ERR_UNSUPPORTED, "Ny > 1024");
     G.M = (int)M;
+    const int NT = g.ntaps;
+    const int D = g.dims;
 
-    // range warning of Gnufft.cpp:67-81 is reported through pg_last_error only
-    G.kx = dev_copy(kx, M);
-    G.ky = dev_copy(ky, M);
-    std::vector<float> zero;
-    if (!kz) zero.assign(M, 0.0f);
-    G.kz = dev_copy(kz ? kz : zero.data(), M);
+    // KB look-up table (griddingSupport.cpp:52-76), S = 10000 W, in double
+    const int S = (int)(size_t)(10000 * W);
+    std::vector<double> lut(S);
+    for (int k = 0; k < S; ++k)
+        lut[k] = bessi0(beta_d * std::sqrt(1.0 - (double)k / (double)S)) / (double)W;
 
-    // LUT (griddingSupport.cpp:52-76)
-    G.S = (int)(size_t)(10000 * W);
-    std::vector<float> lut(G.S);
-    for (int k = 0; k < G.S; ++k) {
-        const float v = (float)k / (float)G.S;
-        lut[k] = (float)(bessi0((double)g.beta * std::sqrt(1.0 - (double)v)) / (double)W);
-    }
-    G.lut = dev_copy(lut.data(), lut.size());
+    // per-sample windows and weights
+    std::vector<int> lo((size_t)D * M);
+    std::vector<float> w((size_t)D * NT * M);
+    std::vector<int> vlo((size_t)D * M), vhi((size_t)D * M);
+    std::vector<float> tmp(NT);
+    const float* kk[3] = {kx, ky, kz};
+    const int Nd[3] = {g.Nx, g.Ny, g.Nz};
+    for (int d = 0; d < D; ++d)
+        for (size_t i = 0; i < M; ++i) {
+            Win1 r = window1((double)kk[d][i], Nd[d], os, W, NT, lut, tmp.data());
+            lo[(size_t)d * M + i] = r.lo;
+            vlo[(size_t)d * M + i] = r.vlo;
+            vhi[(size_t)d * M + i] = r.vhi;
+            for (int j = 0; j < NT; ++j) w[((size_t)d * NT + j) * M + i] = tmp[j];
+        }
+    G.lo = dev_copy(lo.data(), lo.size());
+    G.w = dev_copy(w.data(), w.size());
 
     // per-axis deapodization factors (griddingSupport.cpp:246-260, 327-348)
     std::vector<float> fx(Nx), fy(Ny), fz(Nz, 1.0f);
-    for (size_t X = 0; X < Nx; ++X) {
-        const float u = (g.dims == 2) ? (float)((int)X - (int)(Nx / 2)) / (float)Nx
-                                      : (float)((double)X - ((float)Nx / 2.0)) / (float)Nx;
-        fx[X] = kb_ft(W, g.beta, u);
-    }
-    for (size_t Y = 0; Y < Ny; ++Y) {
-        const float u = (g.dims == 2) ? (float)((int)Y - (int)(Ny / 2)) / (float)Ny
-                                      : (float)((double)Y - ((float)Ny / 2.0)) / (float)Ny;
-        fy[Y] = kb_ft(W, g.beta, u);
-    }
-    if (g.dims == 3)
-        for (size_t Z = 0; Z < Nz; ++Z) {
-            const float u = (float)((double)Z - ((float)Nz / 2.0)) / (float)Nz;
-            fz[Z] = kb_ft(W, g.beta, u);
-        }
+    for (size_t X = 0; X < Nx; ++X)
+        fx[X] = (float)kb_ft(W, beta_d, ((double)X - (double)(Nx / 2)) / (double)Nx);
+    for (size_t Y = 0; Y < Ny; ++Y)
+        fy[Y] = (float)kb_ft(W, beta_d, ((double)Y - (double)(Ny / 2)) / (double)Ny);
+    if (D == 3)
+        for (size_t Z = 0; Z < Nz; ++Z)
+            fz[Z] = (float)kb_ft(W, beta_d, ((double)Z - (double)Nz / 2.0) / (double)Nz);
     G.dfx = dev_copy(fx.data(), Nx);
     G.dfy = dev_copy(fy.data(), Ny);
     G.dfz = dev_copy(fz.data(), Nz);
-    G.inv_osd = (g.dims == 2) ? 1.0f / (os * os) : (float)(1.0 / (double)(os * os * os));
+    G.inv_osd = (D == 2) ? (float)(1.0 / ((double)os * os)) : (float)(1.0 / ((double)os * os * os));
 
     G.px = make_fft_plan(g.Gx);
     G.py = make_fft_plan(g.Gy);
-    if (g.dims == 3) G.pz = make_fft_plan(g.Gz);
+    if (D == 3) G.pz = make_fft_plan(g.Gz);
 
-    // spread tiles
-    const int TX = g.dims == 2 ? 16 : 8, TY = g.dims == 2 ? 16 : 8, TZ = g.dims == 2 ? 1 : 4;
-    G.ntx = (g.Gx + TX - 1) / TX;
-    G.nty = (g.Gy + TY - 1) / TY;
-    G.ntz = (g.Gz + TZ - 1) / TZ;
-    const long long ntiles = (long long)G.ntx * G.nty * G.ntz;
-    const float halfw = W * os / 2.0f;
-    std::vector<int> cnt(ntiles + 1, 0);
-    struct R3 {
-        int x0, x1, y0, y1, z0, z1;
+    auto empty = [&](size_t i) {
+        for (int d = 0; d < D; ++d)
+            if (vhi[(size_t)d * M + i] < vlo[(size_t)d * M + i]) return true;
+        return false;
     };
-    std::vector<R3> rng(M);
-    for (size_t i = 0; i < M; ++i) {
-        int lo, lc, hi;
-        R3 r;
-        host_window(kx[i], g.Nx, os, halfw, lo, lc, hi);
-        r.x0 = lc;
-        r.x1 = std::min(hi, lo + g.ntaps - 1);
-        host_window(ky[i], g.Ny, os, halfw, lo, lc, hi);
-        r.y0 = lc;
-        r.y1 = std::min(hi, lo + g.ntaps - 1);
-        if (g.dims == 3) {
-            host_window(kz[i], g.Nz, os, halfw, lo, lc, hi);
-            r.z0 = lc;
-            r.z1 = std::min(hi, lo + g.ntaps - 1);
-        } else {
-            r.z0 = r.z1 = 0;
+
+    if (D == 2) {
+        // padded weights and 4x4 patch work items
+        std::vector<float> wpad((size_t)2 * M * 16, 0.0f);
+        for (int d = 0; d < 2; ++d)
+            for (size_t i = 0; i < M; ++i)
+                for (int j = 0; j < NT; ++j)
+                    wpad[((size_t)d * M + i) * 16 + 3 + j] = w[((size_t)d * NT + j) * M + i];
+        G.wpad = dev_copy(wpad.data(), wpad.size());
+        const int npx = g.Gx / kPatch, npy = g.Gy / kPatch;
+        const long long np = (long long)npx * npy;
+        std::vector<long long> cnt(np + 1, 0);
+        for (int pass = 0; pass < 2; ++pass) {
+            std::vector<long long> pos;
+            std::vector<int> ent;
+            if (pass == 1) {
+                for (long long q = 0; q < np; ++q) cnt[q + 1] += cnt[q];
+                PG_REQUIRE(cnt[np] < (1ll << 31), PG_ERR_UNSUPPORTED, "too many patch entries");
+                pos.assign(cnt.begin(), cnt.end() - 1);
+                ent.resize(std::max<long long>(1, cnt[np]));
+            }
+            for (size_t i = 0; i < M; ++i) {
+                if (empty(i)) continue;
+                const int x0 = vlo[i] / kPatch, x1 = vhi[i] / kPatch;
+                const int y0 = vlo[M + i] / kPatch, y1 = vhi[M + i] / kPatch;
+                for (int a = x0; a <= x1; ++a)
+                    for (int c = y0; c <= y1; ++c) {
+                        const long long q = (long long)a * npy + c;
+                        if (pass == 0) cnt[q + 1]++;
+                        else ent[pos[q]++] = (int)i;
+                    }
+            }
+            if (pass == 1) {
+                G.entries = dev_copy(ent.data(), ent.size());
+                G.tile_entries = cnt[np];
+            }
         }
-        rng[i] = r;
-        if (r.x1 < r.x0 || r.y1 < r.y0 || r.z1 < r.z0) continue;
-        for (int tz = r.z0 / TZ; tz <= r.z1 / TZ; ++tz)
-            for (int ty = r.y0 / TY; ty <= r.y1 / TY; ++ty)
-                for (int tx = r.x0 / TX; tx <= r.x1 / TX; ++tx)
-                    cnt[((long long)tz * G.nty + ty) * G.ntx + tx + 1]++;
+        // split patches with more than kMaxPatchEntries samples
+        const int kMaxPatchEntries = 48;
+        std::vector<int4> items, comb;
+        int slots = 0;
+        for (long long q = 0; q < np; ++q) {
+            const int b = (int)cnt[q], e = (int)cnt[q + 1];
+            const int n = e - b;
+            if (n <= kMaxPatchEntries) {
+                items.push_back(make_int4((int)q, b, e, -1));
+            } else {
+                const int parts = (n + kMaxPatchEntries - 1) / kMaxPatchEntries;
+                comb.push_back(make_int4((int)q, slots, parts, 0));
+                for (int k = 0; k < parts; ++k) {
+                    const int pb = b + (int)((long long)n * k / parts);
+                    const int pe = b + (int)((long long)n * (k + 1) / parts);
+                    items.push_back(make_int4((int)q, pb, pe, slots + k));
+                }
+                slots += parts;
+            }
+        }
+        // heaviest items first, so that the long ones start early
+        std::stable_sort(items.begin(), items.end(),
+                         [](const int4& a, const int4& b) { return (a.z - a.y) > (b.z - b.y); });
+        G.n_items = (int)items.size();
+        G.n_combine = (int)comb.size();
+        G.n_slots = slots;
+        G.items = dev_copy(items.data(), items.size());
+        if (!comb.empty()) G.combine = dev_copy(comb.data(), comb.size());
+    } else {
+        // 3-D: samples whose window meets each 8x8x4 tile
+        const int TX = 8, TY = 8, TZ = 4;
+        G.ntx = (g.Gx + TX - 1) / TX;
+        G.nty = (g.Gy + TY - 1) / TY;
+        G.ntz = (g.Gz + TZ - 1) / TZ;
+        const long long ntiles = (long long)G.ntx * G.nty * G.ntz;
+        std::vector<long long> cnt(ntiles + 1, 0);
+        for (int pass = 0; pass < 2; ++pass) {
+            std::vector<long long> pos;
+            std::vector<int> idx;
+            if (pass == 1) {
+                for (long long t = 0; t < ntiles; ++t) cnt[t + 1] += cnt[t];
+                PG_REQUIRE(cnt[ntiles] < (1ll << 31), PG_ERR_UNSUPPORTED, "too many tile entries");
+                pos.assign(cnt.begin(), cnt.end() - 1);
+                idx.resize(std::max<long long>(1, cnt[ntiles]));
+            }
+            for (size_t i = 0; i < M; ++i) {
+                if (empty(i)) continue;
+                for (int tz = vlo[2 * M + i] / TZ; tz <= vhi[2 * M + i] / TZ; ++tz)
+                    for (int ty = vlo[M + i] / TY; ty <= vhi[M + i] / TY; ++ty)
+                        for (int tx = vlo[i] / TX; tx <= vhi[i] / TX; ++tx) {
+                            const long long t = ((long long)tz * G.nty + ty) * G.ntx + tx;
+                            if (pass == 0) cnt[t + 1]++;
+                            else idx[pos[t]++] = (int)i;
+                        }
+            }
+            if (pass == 1) {
+                G.tile_entries = cnt[ntiles];
+                std::vector<int> off(cnt.begin(), cnt.end());
+                G.tile_off = dev_copy(off.data(), off.size());
+                G.tile_idx = dev_copy(idx.data(), idx.size());
+            }
+        }
     }
-    for (long long t = 0; t < ntiles; ++t) cnt[t + 1] += cnt[t];
-    G.tile_entries = cnt[ntiles];
-    PG_REQUIRE(G.tile_entries < (1ll << 31), PG_ERR_UNSUPPORTED, "too many tile entries");
-    std::vector<int> idx(std::max<long long>(1, G.tile_entries));
-    std::vector<int> pos(cnt.begin(), cnt.end() - 1);
-    for (size_t i = 0; i < M; ++i) {
-        const R3& r = rng[i];
-        if (r.x1 < r.x0 || r.y1 < r.y0 || r.z1 < r.z0) continue;
-        for (int tz = r.z0 / TZ; tz <= r.z1 / TZ; ++tz)
-            for (int ty = r.y0 / TY; ty <= r.y1 / TY; ++ty)
-                for (int tx = r.x0 / TX; tx <= r.x1 / TX; ++tx)
-                    idx[pos[((long long)tz * G.nty + ty) * G.ntx + tx]++] = (int)i;
-    }
-    G.max_tile = 0;
-    for (long long t = 0; t < ntiles; ++t) G.max_tile = std::max(G.max_tile, cnt[t + 1] - cnt[t]);
-    G.tile_off = dev_copy(cnt.data(), cnt.size());
-    G.tile_idx = dev_copy(idx.data(), idx.size());
 
     G.img_tmp = dev_alloc<float2>(g.N());
-    G.img_tmp2 = dev_alloc<float2>(g.N());
     G.dat_tmp = dev_alloc<float2>(M);
-    G.dat_tmp2 = dev_alloc<float2>(M);
 }
 
 // ----------------------------------------------------------- operators --
@@ -486,7 +567,7 @@ static void fused_forward(const Fused& f, const float2* x, float2* y, hipStream_
 // adjoint into per-coil partial images P (nc x N), optionally of the CG
 // residual W (y - Ax) formed on the fly
 static void fused_adjoint_partial(const Fused& f, const float2* in, const float2* resid_y,
-                                  const float* Wd, float2* P, hipStream_t s)
+                                  const double2* ax64, const float* Wd, float2* P, hipStream_t s)
 {
     Gnufft& G = *f.G;
     const int B = f.nc * f.L;
@@ -494,7 +575,7 @@ static void fused_adjoint_partial(const Fused& f, const float2* in, const float2
     pgk::WeightArgs w{f.S, f.Wo, f.WoH, f.AA};
     {
         Prof p("spread", s);
-        pgk::launch_spread(G, in, G.M, f.nc, f.L, f.AA, resid_y, Wd, G.grids, s);
+        pgk::launch_spread(G, in, G.M, f.nc, f.L, f.AA, resid_y, ax64, Wd, G.grids, s);
     }
     {
         Prof p("adj_fft_post", s);
@@ -594,10 +675,10 @@ static void op_adjoint(Op* op, const float2* in, float2* out, hipStream_t s)
     Fused f;
     if (fusable(op, f)) {
         if (f.nc == 1) {
-            fused_adjoint_partial(f, in, nullptr, nullptr, out, s);
+            fused_adjoint_partial(f, in, nullptr, nullptr, nullptr, out, s);
         } else {
             SenseOp* se = static_cast<SenseOp*>(op);
-            fused_adjoint_partial(f, in, nullptr, nullptr, se->P, s);
+            fused_adjoint_partial(f, in, nullptr, nullptr, nullptr, se->P, s);
             pgv::coil_sum(out, se->P, se->nc, (long long)se->N, s);
         }
         return;
@@ -606,7 +687,7 @@ static void op_adjoint(Op* op, const float2* in, float2* out, hipStream_t s)
     case K_NUFFT: {
         Fused g;
         g.G = &static_cast<NufftOp*>(op)->G;
-        fused_adjoint_partial(g, in, nullptr, nullptr, out, s);
+        fused_adjoint_partial(g, in, nullptr, nullptr, nullptr, out, s);
         return;
     }
     case K_DFT: {
@@ -669,8 +750,10 @@ static Penalty* as_pen(pg_penalty h)
 
 // --------------------------------------------------------------- PCG ----
 struct CgWork {
-    float2 *x = nullptr, *Ax = nullptr, *ngrad = nullptr, *ddir = nullptr, *Adir = nullptr;
-    float2 *g = nullptr, *r = nullptr;
+    double2 *x = nullptr, *Ax = nullptr;  // FP64 CG state (solution, A x)
+    float2* x32 = nullptr;                // FP32 copy of x (penalty stencils, output)
+    float2 *ngrad = nullptr, *ddir = nullptr, *Adir = nullptr;
+    float2 *g = nullptr, *r = nullptr, *tmp = nullptr;
     double* part = nullptr;
     double* resid = nullptr;
     CgScal* sc = nullptr;
@@ -678,11 +761,13 @@ struct CgWork {
     {
         dev_free(x);
         dev_free(Ax);
+        dev_free(x32);
         dev_free(ngrad);
         dev_free(ddir);
         dev_free(Adir);
         dev_free(g);
         dev_free(r);
+        dev_free(tmp);
         dev_free(part);
         dev_free(resid);
         dev_free(sc);
@@ -695,22 +780,22 @@ static void cg_iteration(Op* A, Penalty* R, const float2* y, const float* W, CgW
 {
     const long long N = (long long)A->n_in, M = (long long)A->n_out;
     const pgv::PenSpec* ps = R ? &R->spec : nullptr;
-    // ngrad = A / (W (y - Ax))
+    // ngrad = A / (W (y - Ax)) - R.Gradient(x)
     if (f) {
         SenseOp* se = (A->kind == K_SENSE) ? static_cast<SenseOp*>(A) : nullptr;
         float2* P = se ? se->P : w.g;
-        fused_adjoint_partial(*f, w.Ax, y, W, P, s);
-        pgv::cg_grad(P, f->nc, N, ps, w.x, w.ngrad, w.part, w.sc, s);
+        fused_adjoint_partial(*f, nullptr, y, w.Ax, W, P, s);
+        pgv::cg_grad(P, f->nc, N, ps, w.x32, w.ngrad, w.part, w.sc, s);
     } else {
         pgv::resid(w.r, y, w.Ax, W, M, s);
         op_adjoint(A, w.r, w.g, s);
-        pgv::cg_grad(w.g, 1, N, ps, w.x, w.ngrad, w.part, w.sc, s);
+        pgv::cg_grad(w.g, 1, N, ps, w.x32, w.ngrad, w.part, w.sc, s);
     }
     pgv::cg_dir(w.ngrad, w.ddir, N, w.part, w.sc, s);
     op_forward(A, w.ddir, w.Adir, s);
     pgv::cg_adir(w.Adir, y, w.Ax, W, M, w.part, w.sc, s);
-    pgv::cg_linesearch(ps, w.ddir, w.x, w.part, w.sc, s);
-    pgv::cg_update(w.Ax, w.Adir, y, M, w.x, w.ddir, N, w.part, w.sc, w.resid, s);
+    pgv::cg_linesearch(ps, w.ddir, w.x32, w.part, w.sc, s);
+    pgv::cg_update(w.Ax, w.Adir, y, M, w.x, w.x32, w.ddir, N, w.part, w.sc, w.resid, s);
 }
 
 static void solve_pcg(Op* A, Penalty* R, const float2* x0, const float* W, const float2* y,
@@ -719,8 +804,10 @@ static void solve_pcg(Op* A, Penalty* R, const float2* x0, const float* W, const
 {
     const long long N = (long long)A->n_in, M = (long long)A->n_out;
     CgWork w;
-    w.x = dev_alloc<float2>(N);
-    w.Ax = dev_alloc<float2>(M);
+    w.x = dev_alloc<double2>(N);
+    w.Ax = dev_alloc<double2>(M);
+    w.x32 = dev_alloc<float2>(N);
+    w.tmp = dev_alloc<float2>(M);
     w.ngrad = dev_alloc<float2>(N);
     w.ddir = dev_alloc<float2>(N);
     w.Adir = dev_alloc<float2>(M);
@@ -749,9 +836,11 @@ static void solve_pcg(Op* A, Penalty* R, const float2* x0, const float* W, const
         }
     } guard{own};
 
-    if (x0) PG_HIP(hipMemcpyAsync(w.x, x0, sizeof(float2) * N, hipMemcpyDeviceToDevice, s));
-    else pgv::fill(w.x, N, make_float2(0.f, 0.f), s);
-    op_forward(A, w.x, w.Ax, s);  // Ax = A * xInitial (:55)
+    if (x0) PG_HIP(hipMemcpyAsync(w.x32, x0, sizeof(float2) * N, hipMemcpyDeviceToDevice, s));
+    else pgv::fill(w.x32, N, make_float2(0.f, 0.f), s);
+    pgv::f2d(w.x, w.x32, N, s);
+    op_forward(A, w.x32, w.tmp, s);  // Ax = A * xInitial (:55)
+    pgv::f2d(w.Ax, w.tmp, M, s);
     pgv::cg_init(y, W, M, w.part, w.sc, s);
 
     CgScal hs{};
@@ -786,7 +875,7 @@ static void solve_pcg(Op* A, Penalty* R, const float2* x0, const float* W, const
             PG_HIP(hipStreamSynchronize(s));
         }
     }
-    PG_HIP(hipMemcpyAsync(x_out, w.x, sizeof(float2) * N, hipMemcpyDeviceToDevice, s));
+    PG_HIP(hipMemcpyAsync(x_out, w.x32, sizeof(float2) * N, hipMemcpyDeviceToDevice, s));
     std::vector<double> rh(std::max<size_t>(niter, 1), 0.0);
     PG_HIP(hipMemcpyAsync(rh.data(), w.resid, sizeof(double) * rh.size(), hipMemcpyDeviceToHost, s));
     PG_HIP(hipStreamSynchronize(s));
@@ -964,18 +1053,17 @@ int pg_ts_create(pg_op* out, pg_op Gh, const float* fieldmap, const float* tvec,
     op->inner = G;
     op->n_in = N;
     op->n_out = M;
-    // TimeSegmentation.cpp:46-93, 213-218
+    // TimeSegmentation.cpp:46-93, 213-218, evaluated in double and stored
+    // as FP32 tables
     const int Lc = (L == 0 || L == 1) ? 1 : (int)L;
     op->L = Lc;
-    std::vector<float> tv(tvec, tvec + M);
-    float tmin = tv[0], tmax = tv[0];
+    double tmin = tvec[0], tmax = tvec[0];
     for (size_t j = 1; j < M; ++j) {
-        tmin = std::min(tmin, tv[j]);
-        tmax = std::max(tmax, tv[j]);
+        tmin = std::min(tmin, (double)tvec[j]);
+        tmax = std::max(tmax, (double)tvec[j]);
     }
-    const float rangt = tmax - tmin;
-    float tau = (float)(((double)rangt + 2.220446049250313e-16) / (double)Lc);
-    for (size_t j = 0; j < M; ++j) tv[j] = tv[j] - tmin;
+    const double rangt = tmax - tmin;
+    double tau = (rangt + 2.220446049250313e-16) / (double)Lc;
     std::vector<float> AA((size_t)Lc * M, 0.0f);
     std::vector<float2> Wo((size_t)Lc * N), WoH((size_t)Lc * N);
     if (Lc <= 1) {
@@ -987,24 +1075,23 @@ int pg_ts_create(pg_op* out, pg_op Gh, const float* fieldmap, const float* tvec,
         const size_t none = M / shots;
         for (int l = 0; l < Lc; ++l)
             for (size_t j = 0; j < none; ++j) {
-                const float d = tv[j] - ((float)(unsigned)l * tau);
+                const double d = ((double)tvec[j] - tmin) - (double)l * tau;
                 float a = 0.0f;
-                if (std::fabs(d) <= tau) a = (float)(0.5 + 0.5 * std::cos(3.141592653589793 * (double)d / (double)tau));
+                if (std::fabs(d) <= tau) a = (float)(0.5 + 0.5 * std::cos(M_PI * d / tau));
                 for (size_t sh = 0; sh < shots; ++sh) AA[(size_t)l * M + j + sh * none] = a;
             }
         for (int l = 0; l < Lc; ++l) {
-            float sl = (float)(unsigned)l * tau;
-            sl = sl + tmin;
+            const double sl = (double)l * tau + tmin;
             for (size_t n = 0; n < N; ++n) {
-                const float ph = (-fieldmap[n]) * sl;
-                Wo[(size_t)l * N + n] = make_float2(std::cos(ph), std::sin(ph));
-                const float phh = fieldmap[n] * sl;
-                WoH[(size_t)l * N + n] = make_float2(std::cos(phh), std::sin(phh));
+                const double ph = (double)fieldmap[n] * sl;
+                const float c = (float)std::cos(ph), sn = (float)std::sin(ph);
+                Wo[(size_t)l * N + n] = make_float2(c, -sn);
+                WoH[(size_t)l * N + n] = make_float2(c, sn);
             }
         }
     }
-    op->tau = tau;
-    op->tmin = tmin;
+    op->tau = (float)tau;
+    op->tmin = (float)tmin;
     op->AA = dev_copy(AA.data(), AA.size());
     op->Wo = dev_copy(Wo.data(), Wo.size());
     op->WoH = dev_copy(WoH.data(), WoH.size());

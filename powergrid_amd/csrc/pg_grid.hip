@@ -1,129 +1,233 @@
 // pg_grid.hip -- Kaiser-Bessel interpolation (forward gather) and spreading
 // (adjoint scatter) on the oversampled grid, batched over coils x segments.
 //
-// Weights follow the reference exactly (gridding.cpp:64-97):
-//   s  = os * (k + N/2)
-//   taps n in [max(0, ceil(s - W*os/2)), min(os*N - 1, floor(s + W*os/2))]
-//        (clamped, no wrap-around)
-//   w  = LUT[(int)(d*d*4/W^2 * S)], d = |s - n| / os, 0 if the index >= S
-// A window holds ceil(W*os) taps; the extra tap that appears when s - W*os/2
-// is an integer sits at distance exactly W/2 and has weight 0, so it is not
-// visited.
+// Tap weights are precomputed per sample (pg_gnufft.h); taps outside the
+// clamped window of gridding.cpp:67-73 carry weight 0 and their addresses
+// are clamped into the grid, so no tap wraps around.
 //
-// Spreading is output-stationary: each block owns a tile of grid points
-// (16x16 in 2-D, 8x8x4 in 3-D) for a chunk of BC grids, walks the samples
-// whose window meets the tile (built once per trajectory on the host, in
-// sample order), and accumulates in registers.  No atomics, no zero-fill of
-// the grid, and the per-point summation order is the reference's serial
-// sample order.
+// interp  -- one thread per (sample, coil); the L segment grids of the coil
+//            are read with the same 64 (2-D) / 512 (3-D) weights and summed
+//            with the AA weights (TimeSegmentation.cpp:279-295).  Coils are
+//            mapped to XCDs so that each XCD's L2 holds its coils' grids.
+// spread  -- 2-D, >= 16 grids: one wavefront per 4x4 grid patch, lane = grid
+//            (coil x segment), accumulating in registers over the samples
+//            whose window meets the patch; heavy patches are split into
+//            chunks whose partial sums are added in a fixed order.  No
+//            atomics, no zero-fill, deterministic.
+//            3-D, >= 16 grids: one block per 8x8x4 tile, thread per point.
+//            < 16 grids (one NUFFT at a time): one thread per sample,
+//            hardware float atomics into a zeroed grid (the reference's
+//            OpenACC spread, gridding.cpp:102-106).
 #include "pg_gnufft.h"
 #include "pg_internal.h"
 
 namespace {
 
-struct Win {
-    float s;
-    int lo, hi;
-};
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return min(max(v, lo), hi); }
 
-__device__ __forceinline__ Win window(float k, int N, float os, float halfw)
-{
-    Win w;
-    w.s = os * (k + (float)N / 2.0f);
-    w.lo = (int)ceilf(w.s - halfw);  // unclamped start; taps < 0 are skipped
-    w.hi = (int)fminf(os * (float)N - 1.0f, floorf(w.s + halfw));
-    return w;
-}
-
-__device__ __forceinline__ float lut_w(float s, int n, float os, float W2, const float* lut, int S)
-{
-    float d = fabsf(s - (float)n) / os;
-    int k0 = (int)((d * d * 4.0f / W2) * (float)S);
-    return (k0 >= S) ? 0.0f : lut[k0];
-}
-
-template <int NT>
-__device__ __forceinline__ void taps(const Win& w, float os, float W2, const float* lut, int S,
-                                     float (&wt)[NT])
-{
-#pragma unroll
-    for (int j = 0; j < NT; ++j) {
-        const int n = w.lo + j;
-        wt[j] = (n >= 0 && n <= w.hi) ? lut_w(w.s, n, os, W2, lut, S) : 0.0f;
-    }
-}
-
-// Forward interpolation, one thread per (sample i, coil c); sums the L
-// segments with the AA weights (TimeSegmentation.cpp:279-295).
+// ---------------------------------------------------------------- interp --
 template <int NT, bool D3>
 __global__ void __launch_bounds__(256) k_interp(
-    const float* __restrict__ kx, const float* __restrict__ ky, const float* __restrict__ kz,
-    int M, int Nx, int Ny, int Nz, int Gx, int Gy, float os, float halfw, float W2,
-    const float* __restrict__ lut, int S, const float2* __restrict__ grids, long long G, int L,
-    const float* __restrict__ AA, float2* __restrict__ out, long long out_cstride)
+    const int* __restrict__ lo, const float* __restrict__ w, int M, int Gx, int Gy, int Gz,
+    const float2* __restrict__ grids, long long G, int nc, int L, const float* __restrict__ AA,
+    float2* __restrict__ out, long long out_cstride, int nsb, int xcd_map)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int c = blockIdx.y;
-    if (i >= M) return;
-    const Win wx_ = window(kx[i], Nx, os, halfw);
-    const Win wy_ = window(ky[i], Ny, os, halfw);
-    float wx[NT], wy[NT], wz[D3 ? NT : 1];
-    taps<NT>(wx_, os, W2, lut, S, wx);
-    taps<NT>(wy_, os, W2, lut, S, wy);
-    Win wz_{0.f, 0, 0};
-    if (D3) {
-        wz_ = window(kz[i], Nz, os, halfw);
-        float t[NT];
-        taps<NT>(wz_, os, W2, lut, S, t);
-#pragma unroll
-        for (int j = 0; j < NT; ++j) wz[j % (D3 ? NT : 1)] = t[j];
+    int c, sb;
+    const int p = blockIdx.x;
+    if (xcd_map) {
+        // consecutive block ids go to consecutive XCDs: keep every coil on
+        // one XCD (nc is a multiple of 8 here)
+        const int xcd = p & 7, r = p >> 3;
+        const int cl = r / nsb;
+        sb = r - cl * nsb;
+        c = cl * 8 + xcd;
+    } else {
+        c = p / nsb;
+        sb = p - c * nsb;
     }
-    float2 tot = make_float2(0.f, 0.f);
+    const int i = sb * blockDim.x + threadIdx.x;
+    if (i >= M || c >= nc) return;
+    const int lx = lo[i], ly = lo[M + i];
+    const int lz = D3 ? lo[2 * M + i] : 0;
+    float wx[NT], wy[NT], wz[D3 ? NT : 1];
+    int ox[NT], oy[NT];
+#pragma unroll
+    for (int j = 0; j < NT; ++j) {
+        wx[j] = w[(long long)j * M + i];
+        wy[j] = w[(long long)(NT + j) * M + i];
+        ox[j] = clampi(lx + j, 0, Gx - 1) * Gy;
+        oy[j] = clampi(ly + j, 0, Gy - 1);
+    }
+    long long oz[D3 ? NT : 1];
+    if (D3) {
+#pragma unroll
+        for (int j = 0; j < NT; ++j) {
+            wz[j] = w[(long long)(2 * NT + j) * M + i];
+            oz[j] = (long long)clampi(lz + j, 0, Gz - 1) * Gx * Gy;
+        }
+    } else {
+        wz[0] = 1.f;
+        oz[0] = 0;
+    }
+    float tr = 0.f, ti = 0.f;
+#pragma unroll 1
     for (int l = 0; l < L; ++l) {
         const float2* g = grids + ((long long)c * L + l) * G;
         float ar = 0.f, ai = 0.f;
 #pragma unroll
         for (int jz = 0; jz < (D3 ? NT : 1); ++jz) {
-            const int nz = D3 ? wz_.lo + jz : 0;
-            if (D3 && (nz < 0 || nz > wz_.hi)) continue;
-            const float2* gz = g + (long long)nz * Gx * Gy;
+            const float2* gz = g + oz[jz];
 #pragma unroll
             for (int jx = 0; jx < NT; ++jx) {
-                const int nx = wx_.lo + jx;
-                if (nx < 0 || nx > wx_.hi) continue;
-                const float2* row = gz + (long long)nx * Gy;
+                const float2* row = gz + ox[jx];
+                float rr = 0.f, ri = 0.f;
 #pragma unroll
                 for (int jy = 0; jy < NT; ++jy) {
-                    const int ny = wy_.lo + jy;
-                    if (ny < 0 || ny > wy_.hi) continue;
-                    float w = D3 ? wx[jx] * wy[jy] * wz[jz] : wx[jx] * wy[jy];
-                    const float2 v = row[ny];
-                    ar += w * v.x;
-                    ai += w * v.y;
+                    const float2 v = row[oy[jy]];
+                    rr += wy[jy] * v.x;
+                    ri += wy[jy] * v.y;
                 }
+                const float wxz = D3 ? wx[jx] * wz[jz] : wx[jx];
+                ar += wxz * rr;
+                ai += wxz * ri;
             }
         }
         if (AA) {
             const float a = AA[(long long)l * M + i];
-            ar = a * ar;
-            ai = a * ai;
+            ar *= a;
+            ai *= a;
         }
-        tot.x += ar;
-        tot.y += ai;
+        tr += ar;
+        ti += ai;
     }
-    out[c * out_cstride + i] = tot;
+    out[c * out_cstride + i] = make_float2(tr, ti);
 }
 
-constexpr int kChunk = 64;  // samples staged in LDS per pass
+// value of sample i in grid b: AA_l * (in_c[i] or W (y_c[i] - Ax_c[i])),
+// the residual formed in FP64 from the FP64 Ax of the CG
+__device__ __forceinline__ float2 spread_value(const float2* __restrict__ in, long long in_cstride,
+                                               const float2* __restrict__ resid_y,
+                                               const double2* __restrict__ ax64,
+                                               const float* __restrict__ Wd,
+                                               const float* __restrict__ AA, int L, int M, int b,
+                                               int i)
+{
+    const int c = b / L, l = b - c * L;
+    const long long di = c * in_cstride + i;
+    float2 v;
+    if (resid_y) {
+        // W % (yi - Ax)   (solve_pwls_pcg.hpp:83)
+        const float2 yv = resid_y[di];
+        const double2 a = ax64[di];
+        double rr = (double)yv.x - a.x, ri = (double)yv.y - a.y;
+        if (Wd) {
+            const double wv = Wd[di];
+            rr *= wv;
+            ri *= wv;
+        }
+        v = make_float2((float)rr, (float)ri);
+    } else {
+        v = in[di];
+    }
+    if (AA) {
+        // conj(AA) % d with AA real  (TimeSegmentation.cpp:304-310)
+        const float a = AA[(long long)l * M + i];
+        v = make_float2(a * v.x, a * v.y);
+    }
+    return v;
+}
 
-template <int NT, bool D3, int BC>
-__global__ void __launch_bounds__(256) k_spread(
-    const float* __restrict__ kx, const float* __restrict__ ky, const float* __restrict__ kz,
-    int M, int Nx, int Ny, int Nz, int Gx, int Gy, int Gz, float os, float halfw, float W2,
-    const float* __restrict__ lut, int S, const int* __restrict__ tile_off,
-    const int* __restrict__ tile_idx, int ntx, int nty, const float2* __restrict__ in,
-    long long in_cstride, const float2* __restrict__ resid_y, const float* __restrict__ Wd,
-    int L, const float* __restrict__ AA, int B, float2* __restrict__ grids, long long G)
+// ------------------------------------------------- 2-D patch spread (B>=16)
+__global__ void __launch_bounds__(64) k_spread_patch(
+    const int4* __restrict__ items, const int* __restrict__ entries, const int* __restrict__ lo,
+    const float* __restrict__ wpad, int M, int npy, const float2* __restrict__ in,
+    long long in_cstride, const float2* __restrict__ resid_y, const double2* __restrict__ ax64,
+    const float* __restrict__ Wd, int L,
+    const float* __restrict__ AA, int B, float2* __restrict__ grids, long long G, int Gy,
+    float2* __restrict__ partial, int n_slots)
+{
+    const int4 it = items[blockIdx.x];
+    const int b = blockIdx.y * 64 + threadIdx.x;
+    const bool active = b < B;
+    const int px0 = (it.x / npy) * pg::kPatch, py0 = (it.x % npy) * pg::kPatch;
+    float2 acc[pg::kPatch][pg::kPatch];
+#pragma unroll
+    for (int a = 0; a < pg::kPatch; ++a)
+#pragma unroll
+        for (int q = 0; q < pg::kPatch; ++q) acc[a][q] = make_float2(0.f, 0.f);
+
+    for (int e = it.y; e < it.z; ++e) {
+        const int i = entries[e];
+        const int dx = px0 - lo[i] + 3, dy = py0 - lo[M + i] + 3;  // in [0, 10]
+        const float* wxp = wpad + (long long)i * 16 + dx;
+        const float* wyp = wpad + ((long long)M + i) * 16 + dy;
+        float wx[pg::kPatch], wy[pg::kPatch];
+#pragma unroll
+        for (int a = 0; a < pg::kPatch; ++a) {
+            wx[a] = wxp[a];
+            wy[a] = wyp[a];
+        }
+        const float2 v = active ? spread_value(in, in_cstride, resid_y, ax64, Wd, AA, L, M, b, i)
+                                : make_float2(0.f, 0.f);
+#pragma unroll
+        for (int a = 0; a < pg::kPatch; ++a)
+#pragma unroll
+            for (int q = 0; q < pg::kPatch; ++q) {
+                const float ww = wx[a] * wy[q];
+                acc[a][q].x += ww * v.x;
+                acc[a][q].y += ww * v.y;
+            }
+    }
+    if (!active) return;
+    if (it.w < 0) {
+        float2* g = grids + (long long)b * G + (long long)px0 * Gy + py0;
+#pragma unroll
+        for (int a = 0; a < pg::kPatch; ++a)
+#pragma unroll
+            for (int q = 0; q < pg::kPatch; ++q) g[(long long)a * Gy + q] = acc[a][q];
+    } else {
+        float2* pp = partial + ((long long)blockIdx.y * n_slots + it.w) * 16 * 64 + threadIdx.x;
+#pragma unroll
+        for (int a = 0; a < pg::kPatch; ++a)
+#pragma unroll
+            for (int q = 0; q < pg::kPatch; ++q) pp[(a * pg::kPatch + q) * 64] = acc[a][q];
+    }
+}
+
+// sum the partial patches of split patches, in slot order
+__global__ void __launch_bounds__(64) k_spread_combine(const int4* __restrict__ comb, int npy,
+                                                       const float2* __restrict__ partial,
+                                                       int n_slots, int B,
+                                                       float2* __restrict__ grids, long long G,
+                                                       int Gy)
+{
+    const int4 cb = comb[blockIdx.x];
+    const int b = blockIdx.y * 64 + threadIdx.x;
+    if (b >= B) return;
+    const int px0 = (cb.x / npy) * pg::kPatch, py0 = (cb.x % npy) * pg::kPatch;
+    const float2* pp = partial + ((long long)blockIdx.y * n_slots + cb.y) * 16 * 64 + threadIdx.x;
+    float2* g = grids + (long long)b * G + (long long)px0 * Gy + py0;
+    for (int p = 0; p < 16; ++p) {
+        float2 s = pp[p * 64];
+        for (int k = 1; k < cb.z; ++k) {
+            const float2 v = pp[((long long)k * 16 + p) * 64];
+            s.x += v.x;
+            s.y += v.y;
+        }
+        g[(long long)(p / pg::kPatch) * Gy + (p % pg::kPatch)] = s;
+    }
+}
+
+// ------------------------------------------- 3-D tile spread (B >= 16) ----
+constexpr int kChunk = 64;
+
+template <int NT, int BC>
+__global__ void __launch_bounds__(256) k_spread_tile3(
+    const int* __restrict__ lo, const float* __restrict__ w, int M, int Gx, int Gy, int Gz,
+    const int* __restrict__ tile_off, const int* __restrict__ tile_idx, int ntx, int nty,
+    const float2* __restrict__ in, long long in_cstride, const float2* __restrict__ resid_y,
+    const double2* __restrict__ ax64, const float* __restrict__ Wd, int L,
+    const float* __restrict__ AA, int B, float2* __restrict__ grids, long long G)
 {
     __shared__ int s_lo[kChunk][3];
     __shared__ float s_w[kChunk][3][NT];
@@ -132,19 +236,12 @@ __global__ void __launch_bounds__(256) k_spread(
     const int tile = blockIdx.x;
     const int b0 = blockIdx.y * BC;
     const int tid = threadIdx.x;
-    int px, py, pz = 0;
-    if (!D3) {
-        const int tx = tile % ntx, ty = tile / ntx;
-        px = tx * 16 + (tid >> 4);
-        py = ty * 16 + (tid & 15);
-    } else {
-        const int txy = ntx * nty;
-        const int tz = tile / txy, r = tile - tz * txy;
-        const int tx = r % ntx, ty = r / ntx;
-        px = tx * 8 + ((tid >> 3) & 7);
-        py = ty * 8 + (tid & 7);
-        pz = tz * 4 + (tid >> 6);
-    }
+    const int txy = ntx * nty;
+    const int tz = tile / txy, r = tile - tz * txy;
+    const int tx = r % ntx, ty = r / ntx;
+    const int px = tx * 8 + ((tid >> 3) & 7);
+    const int py = ty * 8 + (tid & 7);
+    const int pz = tz * 4 + (tid >> 6);
     float2 acc[BC];
 #pragma unroll
     for (int bb = 0; bb < BC; ++bb) acc[bb] = make_float2(0.f, 0.f);
@@ -155,71 +252,37 @@ __global__ void __launch_bounds__(256) k_spread(
         __syncthreads();
         if (tid < cnt) {
             const int i = tile_idx[base + tid];
-            const Win a = window(kx[i], Nx, os, halfw);
-            const Win b = window(ky[i], Ny, os, halfw);
-            float t[NT];
-            taps<NT>(a, os, W2, lut, S, t);
 #pragma unroll
-            for (int j = 0; j < NT; ++j) s_w[tid][0][j] = t[j];
-            taps<NT>(b, os, W2, lut, S, t);
+            for (int d = 0; d < 3; ++d) {
+                s_lo[tid][d] = lo[(long long)d * M + i];
 #pragma unroll
-            for (int j = 0; j < NT; ++j) s_w[tid][1][j] = t[j];
-            s_lo[tid][0] = a.lo;
-            s_lo[tid][1] = b.lo;
-            if (D3) {
-                const Win z = window(kz[i], Nz, os, halfw);
-                taps<NT>(z, os, W2, lut, S, t);
-#pragma unroll
-                for (int j = 0; j < NT; ++j) s_w[tid][2][j] = t[j];
-                s_lo[tid][2] = z.lo;
-            } else {
-                s_lo[tid][2] = 0;
+                for (int j = 0; j < NT; ++j) s_w[tid][d][j] = w[(long long)(d * NT + j) * M + i];
             }
         }
         for (int t = tid; t < cnt * BC; t += blockDim.x) {
             const int sl = t / BC, bb = t - sl * BC;
             const int b = b0 + bb;
-            float2 v = make_float2(0.f, 0.f);
-            if (b < B) {
-                const int i = tile_idx[base + sl];
-                const int c = b / L, l = b - c * L;
-                const long long di = c * in_cstride + i;
-                v = in[di];
-                if (resid_y) {
-                    // W % (yi - Ax)   (solve_pwls_pcg.hpp:83)
-                    const float2 yv = resid_y[di];
-                    v = make_float2(yv.x - v.x, yv.y - v.y);
-                    if (Wd) {
-                        const float wv = Wd[di];
-                        v = make_float2(wv * v.x, wv * v.y);
-                    }
-                }
-                if (AA) {
-                    // conj(AA) % d with AA real  (TimeSegmentation.cpp:304-310)
-                    const float a = AA[(long long)l * M + i];
-                    v = make_float2(a * v.x, a * v.y);
-                }
-            }
-            s_v[sl][bb] = v;
+            s_v[sl][bb] = (b < B) ? spread_value(in, in_cstride, resid_y, ax64, Wd, AA, L, M, b,
+                                                 tile_idx[base + sl])
+                                  : make_float2(0.f, 0.f);
         }
         __syncthreads();
         for (int sl = 0; sl < cnt; ++sl) {
             const unsigned dx = (unsigned)(px - s_lo[sl][0]);
             const unsigned dy = (unsigned)(py - s_lo[sl][1]);
-            const unsigned dz = D3 ? (unsigned)(pz - s_lo[sl][2]) : 0u;
-            if (dx < (unsigned)NT && dy < (unsigned)NT && dz < (unsigned)(D3 ? NT : 1)) {
-                float w = s_w[sl][0][dx] * s_w[sl][1][dy];
-                if (D3) w = w * s_w[sl][2][dz];
+            const unsigned dz = (unsigned)(pz - s_lo[sl][2]);
+            if (dx < (unsigned)NT && dy < (unsigned)NT && dz < (unsigned)NT) {
+                const float ww = s_w[sl][0][dx] * s_w[sl][1][dy] * s_w[sl][2][dz];
 #pragma unroll
                 for (int bb = 0; bb < BC; ++bb) {
                     const float2 v = s_v[sl][bb];
-                    acc[bb].x += w * v.x;
-                    acc[bb].y += w * v.y;
+                    acc[bb].x += ww * v.x;
+                    acc[bb].y += ww * v.y;
                 }
             }
         }
     }
-    if (px < Gx && py < Gy && pz < (D3 ? Gz : 1)) {
+    if (px < Gx && py < Gy && pz < Gz) {
         const long long p = (long long)pz * Gx * Gy + (long long)px * Gy + py;
 #pragma unroll
         for (int bb = 0; bb < BC; ++bb)
@@ -227,27 +290,44 @@ __global__ void __launch_bounds__(256) k_spread(
     }
 }
 
-template <int NT, bool D3>
-void spread_dispatch(const pg::Gnufft& G, const float2* in, long long in_cstride, int nc, int L,
-                     const float* AA, const float2* resid_y, const float* Wd, float2* grids,
-                     hipStream_t s)
+// --------------------------------------- atomic spread (one NUFFT at a time)
+__global__ void k_zero(float2* __restrict__ p, long long n)
 {
-    const pg::Geometry& g = G.geo;
-    const int B = nc * L;
-    const int ntiles = G.ntx * G.nty * G.ntz;
-    const float halfw = g.W * g.os / 2.0f;
-    const float W2 = g.W * g.W;
-    if (B == 1) {
-        hipLaunchKernelGGL((k_spread<NT, D3, 1>), dim3(ntiles, 1), dim3(256), 0, s, G.kx, G.ky,
-                           G.kz, G.M, g.Nx, g.Ny, g.Nz, g.Gx, g.Gy, g.Gz, g.os, halfw, W2, G.lut,
-                           G.S, G.tile_off, G.tile_idx, G.ntx, G.nty, in, in_cstride, resid_y, Wd,
-                           L, AA, B, grids, g.G());
-    } else {
-        constexpr int BC = 16;
-        hipLaunchKernelGGL((k_spread<NT, D3, BC>), dim3(ntiles, (B + BC - 1) / BC), dim3(256), 0,
-                           s, G.kx, G.ky, G.kz, G.M, g.Nx, g.Ny, g.Nz, g.Gx, g.Gy, g.Gz, g.os,
-                           halfw, W2, G.lut, G.S, G.tile_off, G.tile_idx, G.ntx, G.nty, in,
-                           in_cstride, resid_y, Wd, L, AA, B, grids, g.G());
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = make_float2(0.f, 0.f);
+}
+
+template <int NT, bool D3>
+__global__ void __launch_bounds__(256) k_spread_atomic(
+    const int* __restrict__ lo, const float* __restrict__ w, int M, int Gx, int Gy, int Gz,
+    const float2* __restrict__ in, long long in_cstride, const float2* __restrict__ resid_y,
+    const double2* __restrict__ ax64, const float* __restrict__ Wd, int L,
+    const float* __restrict__ AA, int B, float2* __restrict__ grids, long long G)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int b = blockIdx.y;
+    if (i >= M) return;
+    const float2 v = spread_value(in, in_cstride, resid_y, ax64, Wd, AA, L, M, b, i);
+    const int lx = lo[i], ly = lo[M + i], lz = D3 ? lo[2 * M + i] : 0;
+    float* g = reinterpret_cast<float*>(grids + (long long)b * G);
+    for (int jz = 0; jz < (D3 ? NT : 1); ++jz) {
+        const int nz = lz + jz;
+        const float wz = D3 ? w[(long long)(2 * NT + jz) * M + i] : 1.f;
+        if (D3 && (wz == 0.f || nz < 0 || nz >= Gz)) continue;
+        for (int jx = 0; jx < NT; ++jx) {
+            const int nx = lx + jx;
+            const float wx = w[(long long)jx * M + i];
+            if (wx == 0.f || nx < 0 || nx >= Gx) continue;
+            for (int jy = 0; jy < NT; ++jy) {
+                const int ny = ly + jy;
+                const float wy = w[(long long)(NT + jy) * M + i];
+                if (wy == 0.f || ny < 0 || ny >= Gy) continue;
+                const float ww = D3 ? wx * wy * wz : wx * wy;
+                const long long idx = (long long)nz * Gx * Gy + (long long)nx * Gy + ny;
+                unsafeAtomicAdd(g + 2 * idx, ww * v.x);
+                unsafeAtomicAdd(g + 2 * idx + 1, ww * v.y);
+            }
+        }
     }
 }
 
@@ -256,10 +336,43 @@ void interp_dispatch(const pg::Gnufft& G, const float2* grids, int nc, int L, co
                      float2* out, long long out_cstride, hipStream_t s)
 {
     const pg::Geometry& g = G.geo;
-    const float halfw = g.W * g.os / 2.0f;
-    hipLaunchKernelGGL((k_interp<NT, D3>), dim3((G.M + 255) / 256, nc), dim3(256), 0, s, G.kx,
-                       G.ky, G.kz, G.M, g.Nx, g.Ny, g.Nz, g.Gx, g.Gy, g.os, halfw, g.W * g.W,
-                       G.lut, G.S, grids, g.G(), L, AA, out, out_cstride);
+    const int nsb = (G.M + 255) / 256;
+    const int xmap = (nc % 8 == 0) ? 1 : 0;
+    hipLaunchKernelGGL((k_interp<NT, D3>), dim3(nsb * nc), dim3(256), 0, s, G.lo, G.w, G.M, g.Gx,
+                       g.Gy, g.Gz, grids, g.G(), nc, L, AA, out, out_cstride, nsb, xmap);
+}
+
+template <int NT, bool D3>
+void spread_dispatch(const pg::Gnufft& G, const float2* in, long long in_cstride, int nc, int L,
+                     const float* AA, const float2* resid_y, const double2* ax64, const float* Wd,
+                     float2* grids, hipStream_t s)
+{
+    const pg::Geometry& g = G.geo;
+    const int B = nc * L;
+    if (B < 16) {
+        const long long n = (long long)B * g.G();
+        hipLaunchKernelGGL(k_zero, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, grids, n);
+        hipLaunchKernelGGL((k_spread_atomic<NT, D3>), dim3((G.M + 255) / 256, B), dim3(256), 0, s,
+                           G.lo, G.w, G.M, g.Gx, g.Gy, g.Gz, in, in_cstride, resid_y, ax64, Wd, L,
+                           AA, B, grids, g.G());
+        return;
+    }
+    if (!D3) {
+        const int nch = (B + 63) / 64;
+        const int npy = g.Gy / pg::kPatch;
+        hipLaunchKernelGGL(k_spread_patch, dim3(G.n_items, nch), dim3(64), 0, s, G.items, G.entries,
+                           G.lo, G.wpad, G.M, npy, in, in_cstride, resid_y, ax64, Wd, L, AA, B, grids,
+                           g.G(), g.Gy, G.partial, G.n_slots);
+        if (G.n_combine > 0)
+            hipLaunchKernelGGL(k_spread_combine, dim3(G.n_combine, nch), dim3(64), 0, s, G.combine,
+                               npy, G.partial, G.n_slots, B, grids, g.G(), g.Gy);
+        return;
+    }
+    constexpr int BC = 16;
+    const int ntiles = G.ntx * G.nty * G.ntz;
+    hipLaunchKernelGGL((k_spread_tile3<NT, BC>), dim3(ntiles, (B + BC - 1) / BC), dim3(256), 0, s,
+                       G.lo, G.w, G.M, g.Gx, g.Gy, g.Gz, G.tile_off, G.tile_idx, G.ntx, G.nty, in,
+                       in_cstride, resid_y, ax64, Wd, L, AA, B, grids, g.G());
 }
 
 }  // namespace
@@ -285,18 +398,18 @@ void launch_interp(const pg::Gnufft& G, const float2* grids, int nc, int L, cons
 }
 
 void launch_spread(const pg::Gnufft& G, const float2* in, long long in_cstride, int nc, int L,
-                   const float* AA, const float2* resid_y, const float* Wd, float2* grids,
-                   hipStream_t s)
+                   const float* AA, const float2* resid_y, const double2* ax64, const float* Wd,
+                   float2* grids, hipStream_t s)
 {
     const bool d3 = G.geo.dims == 3;
     switch (G.geo.ntaps) {
     case 8:
-        d3 ? spread_dispatch<8, true>(G, in, in_cstride, nc, L, AA, resid_y, Wd, grids, s)
-           : spread_dispatch<8, false>(G, in, in_cstride, nc, L, AA, resid_y, Wd, grids, s);
+        d3 ? spread_dispatch<8, true>(G, in, in_cstride, nc, L, AA, resid_y, ax64, Wd, grids, s)
+           : spread_dispatch<8, false>(G, in, in_cstride, nc, L, AA, resid_y, ax64, Wd, grids, s);
         break;
     case 6:
-        d3 ? spread_dispatch<6, true>(G, in, in_cstride, nc, L, AA, resid_y, Wd, grids, s)
-           : spread_dispatch<6, false>(G, in, in_cstride, nc, L, AA, resid_y, Wd, grids, s);
+        d3 ? spread_dispatch<6, true>(G, in, in_cstride, nc, L, AA, resid_y, ax64, Wd, grids, s)
+           : spread_dispatch<6, false>(G, in, in_cstride, nc, L, AA, resid_y, ax64, Wd, grids, s);
         break;
     default:
         throw pg::Error{PG_ERR_UNSUPPORTED, "kernel width * gridos must give 6 or 8 taps"};

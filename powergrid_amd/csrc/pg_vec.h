@@ -34,7 +34,8 @@ void cmul_acc(float2* out, const float2* a, const float2* b, long long n, bool f
 void rmul_acc(float2* out, const float* r, const float2* a, long long n, bool first, hipStream_t s);
 void coil_sum(float2* out, const float2* P, int nc, long long N, hipStream_t s);
 void fill(float2* out, long long n, float2 v, hipStream_t s);
-void resid(float2* r, const float2* y, const float2* Ax, const float* W, long long n, hipStream_t s);
+void resid(float2* r, const float2* y, const double2* Ax, const float* W, long long n, hipStream_t s);
+void f2d(double2* out, const float2* in, long long n, hipStream_t s);
 
 void pen_gradient(const PenSpec& p, const float2* x, float2* g, hipStream_t s);
 void pen_denom_pdot(const PenSpec& p, const float2* ddir, const float2* x, const double2* step,
@@ -46,12 +47,12 @@ void cg_grad(const float2* P, int nc, long long N, const PenSpec* p, const float
              float2* ngrad, double* part, CgScal* sc, hipStream_t s);
 void cg_dir(const float2* ngrad, float2* ddir, long long N, double* part, CgScal* sc,
             hipStream_t s);
-void cg_adir(const float2* Adir, const float2* y, const float2* Ax, const float* W, long long n,
+void cg_adir(const float2* Adir, const float2* y, const double2* Ax, const float* W, long long n,
              double* part, CgScal* sc, hipStream_t s);
 void cg_linesearch(const PenSpec* p, const float2* ddir, const float2* x, double* part, CgScal* sc,
                    hipStream_t s);
-void cg_update(float2* Ax, const float2* Adir, const float2* y, long long M, float2* x,
-               const float2* ddir, long long N, double* part, CgScal* sc, double* resid,
-               hipStream_t s);
+void cg_update(double2* Ax, const float2* Adir, const float2* y, long long M, double2* x,
+               float2* x32, const float2* ddir, long long N, double* part, CgScal* sc,
+               double* resid, hipStream_t s);
 
 }  // namespace pgv

@@ -89,14 +89,23 @@ __global__ void k_fill(float2* out, long long n, float2 v)
     if (i < n) out[i] = v;
 }
 
-// r = W .* (y - Ax)
-__global__ void k_resid(float2* r, const float2* y, const float2* Ax, const float* W, long long n)
+// r = W .* (y - Ax), formed in FP64 from the FP64 Ax
+__global__ void k_resid(float2* r, const float2* y, const double2* Ax, const float* W, long long n)
 {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    float2 v = make_float2(y[i].x - Ax[i].x, y[i].y - Ax[i].y);
-    if (W) v = make_float2(W[i] * v.x, W[i] * v.y);
-    r[i] = v;
+    double vr = (double)y[i].x - Ax[i].x, vi = (double)y[i].y - Ax[i].y;
+    if (W) {
+        vr *= W[i];
+        vi *= W[i];
+    }
+    r[i] = make_float2((float)vr, (float)vi);
+}
+
+__global__ void k_f2d(double2* out, const float2* in, long long n)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = make_double2(in[i].x, in[i].y);
 }
 
 // ------------------------------------------------------------ penalties --
@@ -346,7 +355,7 @@ __global__ void k_cg_ctl_dir(const double* part, int nparts, CgScal* sc)
 
 // partials of dAWAd = real(cdot(Adir, W Adir)) and
 // dAWr = real(Adir' (W (y - Ax)))   (solve_pwls_pcg.hpp:123-125)
-__global__ void k_cg_adir_dots(const float2* Adir, const float2* y, const float2* Ax,
+__global__ void k_cg_adir_dots(const float2* Adir, const float2* y, const double2* Ax,
                                const float* W, long long n, double* part, const CgScal* sc)
 {
     if (sc->stop) return;
@@ -403,20 +412,21 @@ __global__ void k_cg_ctl_ls(const double* part, int nparts, int have_pen, CgScal
     sc->step = make_double2(s.x - numr / dr, s.y - numi / dr);
 }
 
-// Ax += step*Adir, x += step*ddir; partial ||y - Ax||^2 (solve_pwls_pcg.hpp:160-162)
-__global__ void k_cg_update(float2* Ax, const float2* Adir, const float2* y, long long M,
-                            float2* x, const float2* ddir, long long N, double* part,
+// Ax += step*Adir, x += step*ddir in FP64 (x32 keeps an FP32 copy of x for
+// the penalty stencils); partial ||y - Ax||^2 (solve_pwls_pcg.hpp:160-162)
+__global__ void k_cg_update(double2* Ax, const float2* Adir, const float2* y, long long M,
+                            double2* x, float2* x32, const float2* ddir, long long N, double* part,
                             const CgScal* sc)
 {
     if (sc->stop) return;
-    const float sr = (float)sc->step.x, si = (float)sc->step.y;
+    const double sr = sc->step.x, si = sc->step.y;
     double v[1] = {0};
     const long long n = M > N ? M : N;
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (long long)gridDim.x * blockDim.x) {
         if (i < M) {
             const float2 a = Adir[i];
-            float2 o = Ax[i];
+            double2 o = Ax[i];
             o.x += sr * a.x - si * a.y;
             o.y += sr * a.y + si * a.x;
             Ax[i] = o;
@@ -425,10 +435,11 @@ __global__ void k_cg_update(float2* Ax, const float2* Adir, const float2* y, lon
         }
         if (i < N) {
             const float2 d = ddir[i];
-            float2 o = x[i];
+            double2 o = x[i];
             o.x += sr * d.x - si * d.y;
             o.y += sr * d.y + si * d.x;
             x[i] = o;
+            x32[i] = make_float2((float)o.x, (float)o.y);
         }
     }
     block_sum<1>(v);
@@ -519,9 +530,13 @@ void fill(float2* out, long long n, float2 v, hipStream_t s)
 {
     hipLaunchKernelGGL(k_fill, dim3(nblk(n)), dim3(kThreads), 0, s, out, n, v);
 }
-void resid(float2* r, const float2* y, const float2* Ax, const float* W, long long n, hipStream_t s)
+void resid(float2* r, const float2* y, const double2* Ax, const float* W, long long n, hipStream_t s)
 {
     hipLaunchKernelGGL(k_resid, dim3(nblk(n)), dim3(kThreads), 0, s, r, y, Ax, W, n);
+}
+void f2d(double2* out, const float2* in, long long n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_f2d, dim3(nblk(n)), dim3(kThreads), 0, s, out, in, n);
 }
 
 void pen_gradient(const PenSpec& ps, const float2* x, float2* g, hipStream_t s)
@@ -568,7 +583,7 @@ void cg_dir(const float2* ngrad, float2* ddir, long long N, double* part, CgScal
     hipLaunchKernelGGL(k_cg_ctl_dir, dim3(1), dim3(kThreads), 0, s, part, (int)nb, sc);
 }
 
-void cg_adir(const float2* Adir, const float2* y, const float2* Ax, const float* W, long long n,
+void cg_adir(const float2* Adir, const float2* y, const double2* Ax, const float* W, long long n,
              double* part, CgScal* sc, hipStream_t s)
 {
     const unsigned nb = nred(n);
@@ -590,13 +605,13 @@ void cg_linesearch(const PenSpec* ps, const float2* ddir, const float2* x, doubl
     }
 }
 
-void cg_update(float2* Ax, const float2* Adir, const float2* y, long long M, float2* x,
-               const float2* ddir, long long N, double* part, CgScal* sc, double* resid,
-               hipStream_t s)
+void cg_update(double2* Ax, const float2* Adir, const float2* y, long long M, double2* x,
+               float2* x32, const float2* ddir, long long N, double* part, CgScal* sc,
+               double* resid, hipStream_t s)
 {
     const unsigned nb = nred(M > N ? M : N);
-    hipLaunchKernelGGL(k_cg_update, dim3(nb), dim3(kThreads), 0, s, Ax, Adir, y, M, x, ddir, N, part,
-                       sc);
+    hipLaunchKernelGGL(k_cg_update, dim3(nb), dim3(kThreads), 0, s, Ax, Adir, y, M, x, x32, ddir, N,
+                       part, sc);
     hipLaunchKernelGGL(k_cg_ctl_update, dim3(1), dim3(kThreads), 0, s, part, (int)nb, sc, resid);
 }
 

@@ -1,11 +1,13 @@
 """GPU parity: every operator of the HIP path against the CPU restatement
 (oracle/) on the same seeded inputs, through the C ABI.
 
-Tolerances: operators within 1e-4 relative L2 of the FP32 restatement
-(the stated bar); the gridding path is measured at ~1e-7 here because the
-kernels reproduce the reference's arithmetic (LUT index, window, deapod
-order).  Gdft precision 0 uses hardware sin/cos on revolutions, bounded at
-1e-4.  CG results within 1e-4 NRMSE of the restatement's PCG.
+Tolerances: every operator within 1e-4 relative L2 of the FP32
+restatement (the stated bar; measured ~2e-5, the FP32 reference's own
+distance from FP64, dominated by forming k + N/2 in float).  The HIP path
+computes its tables and tap weights in FP64 and does FP32 arithmetic, so it
+is also checked against the FP64 restatement of the same operator at 5e-6.
+Gdft precision 0 uses hardware sin/cos on revolutions, bounded at 1e-4.
+CG results within 1e-3 NRMSE of the FP64 restatement's PCG.
 """
 import numpy as np
 import pytest
@@ -20,6 +22,18 @@ def rel(a, b):
     a = np.asarray(a).ravel()
     b = np.asarray(b).ravel()
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
+
+
+def pair(orc, make):
+    """the same operator from the FP32 and the FP64 restatement"""
+    return make("f"), make("d")
+
+
+def check(gpu, ref32, ref64, tol64=5e-6):
+    e32, e64 = rel(gpu, ref32), rel(gpu, ref64)
+    assert e32 < 1e-4, (e32, e64)
+    assert e64 < tol64, (e32, e64)
+    return e32, e64
 
 
 def traj(N, nshot, ns, Nz=1, edge=True):
@@ -40,10 +54,10 @@ def test_gnufft2d_matches_oracle(orc, N, W):
     M = len(kx)
     x = S.ellipse_phantom(N, seed=1).astype(np.complex64)
     Gg = P.Gnufft(M, 2.0, N, N, 1, kx, ky, kz, kernel_width=W)
-    Go = orc.Gnufft(M, 2.0, N, N, 1, kx, ky, kz, kernel_width=W)
-    yo = Go * x
-    assert rel(Gg * x, yo) < 2e-6
-    assert rel(Gg / yo, Go / yo) < 2e-6
+    Gf, Gd = pair(orc, lambda p: orc.Gnufft(M, 2.0, N, N, 1, kx, ky, kz, kernel_width=W, prec=p))
+    check(Gg * x, Gf * x, Gd * x)
+    yo = Gd * x
+    check(Gg / yo, Gf / yo, Gd / yo)
 
 
 def test_gnufft3d_matches_oracle(orc):
@@ -53,10 +67,17 @@ def test_gnufft3d_matches_oracle(orc):
     M = len(kx)
     x = S.ellipse_phantom(N, Nz, seed=2).astype(np.complex64)
     Gg = P.Gnufft(M, 2.0, N, N, Nz, kx, ky, kz)
-    Go = orc.Gnufft(M, 2.0, N, N, Nz, kx, ky, kz)
-    yo = Go * x
-    assert rel(Gg * x, yo) < 2e-6
-    assert rel(Gg / yo, Go / yo) < 2e-6
+    Gf, Gd = pair(orc, lambda p: orc.Gnufft(M, 2.0, N, N, Nz, kx, ky, kz, prec=p))
+    check(Gg * x, Gf * x, Gd * x)
+    yo = Gd * x
+    check(Gg / yo, Gf / yo, Gd / yo)
+    # 16 grids at once take the tiled spread
+    sens = S.coil_maps(N, 16, Nz).astype(np.complex64)
+    Sg = P.SENSE(Gg, sens, M, x.size, 16)
+    Sf, Sd = orc.SENSE(Gf, sens.ravel(), M, x.size, 16), orc.SENSE(Gd, sens.ravel(), M, x.size, 16)
+    check(Sg * x, Sf * x, Sd * x)
+    ys = Sd * x
+    check(Sg / ys, Sf / ys, Sd / ys)
 
 
 def _ts_case(N=32, nshot=4, ns=512, L=4, peak=120.0):
@@ -75,15 +96,15 @@ def test_timesegmentation_matches_oracle(orc, mode):
     kx, ky, kz, t, fm, x, M = _ts_case(N, L=L)
     Gg = P.Gnufft(M, 2.0, N, N, 1, kx, ky, kz)
     Tg = P.TimeSegmentation(Gg, fm, t, M, N * N, L).set_mode(mode)
-    Go = orc.Gnufft(M, 2.0, N, N, 1, kx, ky, kz)
-    To = orc.TimeSegmentation(Go, fm, t, M, N * N, L)
-    yo = To * x
-    assert rel(Tg * x, yo) < 5e-6
-    assert rel(Tg / yo, To / yo) < 5e-6
+    Tf, Td = pair(orc, lambda p: orc.TimeSegmentation(orc.Gnufft(M, 2.0, N, N, 1, kx, ky, kz, prec=p),
+                                                      fm, t, M, N * N, L))
+    check(Tg * x, Tf * x, Td * x)
+    yo = Td * x
+    check(Tg / yo, Tf / yo, Td / yo)
 
 
 @pytest.mark.parametrize("mode", [P.MODE_BATCHED, P.MODE_PLAIN])
-@pytest.mark.parametrize("L", [1, 5])
+@pytest.mark.parametrize("L", [1, 5, 8])
 def test_sense_ts_matches_oracle(orc, mode, L):
     print(f"[sense-ts mode={mode} L={L}]", flush=True)
     N, nc = 32, 3
@@ -93,12 +114,15 @@ def test_sense_ts_matches_oracle(orc, mode, L):
     Tg = P.TimeSegmentation(Gg, fm, t, M, N * N, L)
     Sg = P.SENSE(Tg, sens, M, N * N, nc).set_mode(mode)
     Tg.set_mode(mode)
-    Go = orc.Gnufft(M, 2.0, N, N, 1, kx, ky, kz)
-    To = orc.TimeSegmentation(Go, fm, t, M, N * N, L)
-    So = orc.SENSE(To, sens.ravel(), M, N * N, nc)
-    yo = So * x
-    assert rel(Sg * x, yo) < 5e-6
-    assert rel(Sg / yo, So / yo) < 5e-6
+
+    def mk(p):
+        Go = orc.Gnufft(M, 2.0, N, N, 1, kx, ky, kz, prec=p)
+        To = orc.TimeSegmentation(Go, fm, t, M, N * N, L)
+        return orc.SENSE(To, sens.ravel(), M, N * N, nc), To, Go
+    (Sf, *kf), (Sd, *kd) = mk("f"), mk("d")
+    check(Sg * x, Sf * x, Sd * x)
+    yo = Sd * x
+    check(Sg / yo, Sf / yo, Sd / yo)
 
 
 def test_sense_gnufft_direct(orc):
@@ -110,11 +134,30 @@ def test_sense_gnufft_direct(orc):
     sens = S.coil_maps(N, nc).astype(np.complex64)
     Gg = P.Gnufft(M, 2.0, N, N, 1, kx, ky, kz)
     Sg = P.SENSE(Gg, sens, M, N * N, nc)
-    Go = orc.Gnufft(M, 2.0, N, N, 1, kx, ky, kz)
-    So = orc.SENSE(Go, sens.ravel(), M, N * N, nc)
-    yo = So * x
+
+    def mk(p):
+        Go = orc.Gnufft(M, 2.0, N, N, 1, kx, ky, kz, prec=p)
+        return orc.SENSE(Go, sens.ravel(), M, N * N, nc), Go
+    (Sf, gf), (Sd, gd) = mk("f"), mk("d")
+    check(Sg * x, Sf * x, Sd * x)
+    yo = Sd * x
+    check(Sg / yo, Sf / yo, Sd / yo)
+
+
+def test_spread_patch_split_and_many_grids(orc):
+    # 64+ grids (two lane chunks), a dense k-space centre (split patches)
+    print("[spread 72 grids]", flush=True)
+    N, nc, L = 32, 9, 8
+    kx, ky, kz, t, fm, x, M = _ts_case(N, nshot=8, ns=512, L=L)
+    sens = S.coil_maps(N, nc).astype(np.complex64)
+    Gg = P.Gnufft(M, 2.0, N, N, 1, kx, ky, kz)
+    Sg = P.SENSE(P.TimeSegmentation(Gg, fm, t, M, N * N, L), sens, M, N * N, nc)
+    Gd = orc.Gnufft(M, 2.0, N, N, 1, kx, ky, kz, prec="d")
+    Td = orc.TimeSegmentation(Gd, fm, t, M, N * N, L)
+    Sd = orc.SENSE(Td, sens.ravel(), M, N * N, nc)
+    yo = Sd * x
+    assert rel(Sg / yo, Sd / yo) < 5e-6
     assert rel(Sg * x, yo) < 5e-6
-    assert rel(Sg / yo, So / yo) < 5e-6
 
 
 def test_gdft_matches_oracle(orc):
@@ -243,7 +286,8 @@ def test_device_arrays_and_stream():
     xa = G.adjoint(yd, stream=s)
     s.synchronize()
     assert rel(yd.to_host(s), G * x) < 1e-7
-    assert rel(xa.to_host(s), G / (G * x)) < 1e-7
+    # one grid at a time spreads with float atomics: run-to-run rounding only
+    assert rel(xa.to_host(s), G / (G * x)) < 2e-6
     # a second call reuses the workspace and gives the same bits
     yd2 = G.forward(xd, stream=s)
     s.synchronize()
