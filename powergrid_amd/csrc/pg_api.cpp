@@ -162,6 +162,8 @@ void Gnufft::ensure_batch(int B)
     tbuf = dev_alloc<float2>((size_t)B * g.Nz * g.Nx * g.Gy);
     if (g.dims == 3) tbuf2 = dev_alloc<float2>((size_t)B * g.Nz * g.Gx * g.Gy);
     if (n_slots > 0) partial = dev_alloc<float2>((size_t)((B + 63) / 64) * n_slots * 16 * 64);
+    dev_free(vals);
+    vals = dev_alloc<float2>((size_t)M * B);
     cap_batch = B;
 }
 
@@ -185,6 +187,7 @@ void Gnufft::release()
     dev_free(tbuf);
     dev_free(tbuf2);
     dev_free(partial);
+    dev_free(vals);
     dev_free(img_tmp);
     dev_free(dat_tmp);
     cap_batch = 0;
@@ -353,7 +356,7 @@ static void gnufft_setup(Gnufft& G, size_t M, float os, size_t Nx, size_t Ny, si
             }
         }
         // split patches with more than kMaxPatchEntries samples
-        const int kMaxPatchEntries = 48;
+        const int kMaxPatchEntries = 64;
         std::vector<int4> items, comb;
         int slots = 0;
         for (long long q = 0; q < np; ++q) {

@@ -53,6 +53,7 @@ struct Gnufft {
     float2* tbuf = nullptr;     // cap_batch * Nx*Nz*Gy
     float2* tbuf2 = nullptr;    // 3-D only: cap_batch * Nz*Gx*Gy
     float2* partial = nullptr;  // ceil(cap_batch/64) * n_slots * 16 * 64
+    float2* vals = nullptr;     // M * cap_batch per-sample spread values
     float2* img_tmp = nullptr;
     float2* dat_tmp = nullptr;
 

@@ -9,15 +9,17 @@
 //            are read with the same 64 (2-D) / 512 (3-D) weights and summed
 //            with the AA weights (TimeSegmentation.cpp:279-295).  Coils are
 //            mapped to XCDs so that each XCD's L2 holds its coils' grids.
-// spread  -- 2-D, >= 16 grids: one wavefront per 4x4 grid patch, lane = grid
+// spread  -- first the value of every (sample, grid) pair is formed once
+//            (AA weight, CG residual), sample-major.
+//            2-D: one wavefront per 4x4 grid patch, lane = grid
 //            (coil x segment), accumulating in registers over the samples
 //            whose window meets the patch; heavy patches are split into
 //            chunks whose partial sums are added in a fixed order.  No
 //            atomics, no zero-fill, deterministic.
 //            3-D, >= 16 grids: one block per 8x8x4 tile, thread per point.
-//            < 16 grids (one NUFFT at a time): one thread per sample,
-//            hardware float atomics into a zeroed grid (the reference's
-//            OpenACC spread, gridding.cpp:102-106).
+//            3-D, < 16 grids: one thread per sample, hardware float atomics
+//            into a zeroed grid (the reference's OpenACC spread,
+//            gridding.cpp:102-106).
 #include "pg_gnufft.h"
 #include "pg_internal.h"
 
@@ -26,6 +28,9 @@ namespace {
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return min(max(v, lo), hi); }
 
 // ---------------------------------------------------------------- interp --
+// Four lanes per sample: lane q of a sample reads the taps 2q, 2q+1 along y
+// of every row, so one load instruction covers 16 samples x 64 contiguous
+// bytes; the four partial sums are combined with two lane shuffles.
 template <int NT, bool D3>
 __global__ void __launch_bounds__(256) k_interp(
     const int* __restrict__ lo, const float* __restrict__ w, int M, int Gx, int Gy, int Gz,
@@ -45,24 +50,29 @@ __global__ void __launch_bounds__(256) k_interp(
         c = p / nsb;
         sb = p - c * nsb;
     }
-    const int i = sb * blockDim.x + threadIdx.x;
-    if (i >= M || c >= nc) return;
-    const int lx = lo[i], ly = lo[M + i];
-    const int lz = D3 ? lo[2 * M + i] : 0;
-    float wx[NT], wy[NT], wz[D3 ? NT : 1];
-    int ox[NT], oy[NT];
+    if (c >= nc) return;
+    const int q = threadIdx.x & 3;
+    const int i = sb * (int)(blockDim.x >> 2) + (int)(threadIdx.x >> 2);
+    const bool valid = i < M;
+    const int ii = valid ? i : M - 1;
+    const int lx = lo[ii], ly = lo[M + ii];
+    const int lz = D3 ? lo[2 * M + ii] : 0;
+    const int j0 = 2 * q, j1 = 2 * q + 1;
+    const float wy0 = (j0 < NT) ? w[(long long)(NT + j0) * M + ii] : 0.f;
+    const float wy1 = (j1 < NT) ? w[(long long)(NT + j1) * M + ii] : 0.f;
+    const int oy0 = clampi(ly + j0, 0, Gy - 1), oy1 = clampi(ly + j1, 0, Gy - 1);
+    float wx[NT], wz[D3 ? NT : 1];
+    int ox[NT];
 #pragma unroll
     for (int j = 0; j < NT; ++j) {
-        wx[j] = w[(long long)j * M + i];
-        wy[j] = w[(long long)(NT + j) * M + i];
+        wx[j] = w[(long long)j * M + ii];
         ox[j] = clampi(lx + j, 0, Gx - 1) * Gy;
-        oy[j] = clampi(ly + j, 0, Gy - 1);
     }
     long long oz[D3 ? NT : 1];
     if (D3) {
 #pragma unroll
         for (int j = 0; j < NT; ++j) {
-            wz[j] = w[(long long)(2 * NT + j) * M + i];
+            wz[j] = w[(long long)(2 * NT + j) * M + ii];
             oz[j] = (long long)clampi(lz + j, 0, Gz - 1) * Gx * Gy;
         }
     } else {
@@ -80,27 +90,27 @@ __global__ void __launch_bounds__(256) k_interp(
 #pragma unroll
             for (int jx = 0; jx < NT; ++jx) {
                 const float2* row = gz + ox[jx];
-                float rr = 0.f, ri = 0.f;
-#pragma unroll
-                for (int jy = 0; jy < NT; ++jy) {
-                    const float2 v = row[oy[jy]];
-                    rr += wy[jy] * v.x;
-                    ri += wy[jy] * v.y;
-                }
+                const float2 v0 = row[oy0], v1 = row[oy1];
+                const float rr = wy0 * v0.x + wy1 * v1.x;
+                const float ri = wy0 * v0.y + wy1 * v1.y;
                 const float wxz = D3 ? wx[jx] * wz[jz] : wx[jx];
                 ar += wxz * rr;
                 ai += wxz * ri;
             }
         }
         if (AA) {
-            const float a = AA[(long long)l * M + i];
+            const float a = AA[(long long)l * M + ii];
             ar *= a;
             ai *= a;
         }
         tr += ar;
         ti += ai;
     }
-    out[c * out_cstride + i] = make_float2(tr, ti);
+    tr += __shfl_xor(tr, 1);
+    ti += __shfl_xor(ti, 1);
+    tr += __shfl_xor(tr, 2);
+    ti += __shfl_xor(ti, 2);
+    if (q == 0 && valid) out[c * out_cstride + i] = make_float2(tr, ti);
 }
 
 // value of sample i in grid b: AA_l * (in_c[i] or W (y_c[i] - Ax_c[i])),
@@ -137,18 +147,29 @@ __device__ __forceinline__ float2 spread_value(const float2* __restrict__ in, lo
     return v;
 }
 
+// V[i*B + b] = value of sample i in grid b, formed once per spread so that the
+// spread kernels read one contiguous row of B values per sample
+__global__ void __launch_bounds__(256) k_spread_values(
+    const float2* __restrict__ in, long long in_cstride, const float2* __restrict__ resid_y,
+    const double2* __restrict__ ax64, const float* __restrict__ Wd, int L,
+    const float* __restrict__ AA, int M, int B, float2* __restrict__ V)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)M * B) return;
+    const int i = (int)(t / B), b = (int)(t - (long long)i * B);
+    V[t] = spread_value(in, in_cstride, resid_y, ax64, Wd, AA, L, M, b, i);
+}
+
 // ------------------------------------------------- 2-D patch spread (B>=16)
 __global__ void __launch_bounds__(64) k_spread_patch(
     const int4* __restrict__ items, const int* __restrict__ entries, const int* __restrict__ lo,
-    const float* __restrict__ wpad, int M, int npy, const float2* __restrict__ in,
-    long long in_cstride, const float2* __restrict__ resid_y, const double2* __restrict__ ax64,
-    const float* __restrict__ Wd, int L,
-    const float* __restrict__ AA, int B, float2* __restrict__ grids, long long G, int Gy,
-    float2* __restrict__ partial, int n_slots)
+    const float* __restrict__ wpad, int M, int npy, const float2* __restrict__ V, int B,
+    float2* __restrict__ grids, long long G, int Gy, float2* __restrict__ partial, int n_slots)
 {
     const int4 it = items[blockIdx.x];
     const int b = blockIdx.y * 64 + threadIdx.x;
     const bool active = b < B;
+    const int bb = active ? b : B - 1;
     const int px0 = (it.x / npy) * pg::kPatch, py0 = (it.x % npy) * pg::kPatch;
     float2 acc[pg::kPatch][pg::kPatch];
 #pragma unroll
@@ -156,27 +177,36 @@ __global__ void __launch_bounds__(64) k_spread_patch(
 #pragma unroll
         for (int q = 0; q < pg::kPatch; ++q) acc[a][q] = make_float2(0.f, 0.f);
 
-    for (int e = it.y; e < it.z; ++e) {
-        const int i = entries[e];
-        const int dx = px0 - lo[i] + 3, dy = py0 - lo[M + i] + 3;  // in [0, 10]
-        const float* wxp = wpad + (long long)i * 16 + dx;
-        const float* wyp = wpad + ((long long)M + i) * 16 + dy;
-        float wx[pg::kPatch], wy[pg::kPatch];
+    // entries are processed in groups of EB: all loads of a group are issued
+    // before its FMAs, so several memory round trips overlap
+    constexpr int EB = 8;
+    for (int e0 = it.y; e0 < it.z; e0 += EB) {
+        const int n = min(EB, it.z - e0);
+        float wx[EB][pg::kPatch], wy[EB][pg::kPatch];
+        float2 v[EB];
 #pragma unroll
-        for (int a = 0; a < pg::kPatch; ++a) {
-            wx[a] = wxp[a];
-            wy[a] = wyp[a];
-        }
-        const float2 v = active ? spread_value(in, in_cstride, resid_y, ax64, Wd, AA, L, M, b, i)
-                                : make_float2(0.f, 0.f);
+        for (int k = 0; k < EB; ++k) {
+            const int i = entries[e0 + min(k, n - 1)];
+            const int dx = px0 - lo[i] + 3, dy = py0 - lo[M + i] + 3;  // in [0, 10]
+            const float* wxp = wpad + (long long)i * 16 + dx;
+            const float* wyp = wpad + ((long long)M + i) * 16 + dy;
 #pragma unroll
-        for (int a = 0; a < pg::kPatch; ++a)
-#pragma unroll
-            for (int q = 0; q < pg::kPatch; ++q) {
-                const float ww = wx[a] * wy[q];
-                acc[a][q].x += ww * v.x;
-                acc[a][q].y += ww * v.y;
+            for (int a = 0; a < pg::kPatch; ++a) {
+                wx[k][a] = (k < n) ? wxp[a] : 0.f;
+                wy[k][a] = wyp[a];
             }
+            v[k] = V[(long long)i * B + bb];
+        }
+#pragma unroll
+        for (int k = 0; k < EB; ++k)
+#pragma unroll
+            for (int a = 0; a < pg::kPatch; ++a)
+#pragma unroll
+                for (int q = 0; q < pg::kPatch; ++q) {
+                    const float ww = wx[k][a] * wy[k][q];
+                    acc[a][q].x += ww * v[k].x;
+                    acc[a][q].y += ww * v[k].y;
+                }
     }
     if (!active) return;
     if (it.w < 0) {
@@ -194,27 +224,38 @@ __global__ void __launch_bounds__(64) k_spread_patch(
     }
 }
 
-// sum the partial patches of split patches, in slot order
-__global__ void __launch_bounds__(64) k_spread_combine(const int4* __restrict__ comb, int npy,
-                                                       const float2* __restrict__ partial,
-                                                       int n_slots, int B,
-                                                       float2* __restrict__ grids, long long G,
-                                                       int Gy)
+// sum the partial patches of split patches, in slot order; 4 wavefronts per
+// patch, each summing 4 of its 16 points for 64 grids
+__global__ void __launch_bounds__(256) k_spread_combine(const int4* __restrict__ comb, int npy,
+                                                        const float2* __restrict__ partial,
+                                                        int n_slots, int B,
+                                                        float2* __restrict__ grids, long long G,
+                                                        int Gy)
 {
     const int4 cb = comb[blockIdx.x];
-    const int b = blockIdx.y * 64 + threadIdx.x;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int b = blockIdx.y * 64 + lane;
     if (b >= B) return;
     const int px0 = (cb.x / npy) * pg::kPatch, py0 = (cb.x % npy) * pg::kPatch;
-    const float2* pp = partial + ((long long)blockIdx.y * n_slots + cb.y) * 16 * 64 + threadIdx.x;
+    const float2* pp = partial + ((long long)blockIdx.y * n_slots + cb.y) * 16 * 64 + lane;
     float2* g = grids + (long long)b * G + (long long)px0 * Gy + py0;
-    for (int p = 0; p < 16; ++p) {
-        float2 s = pp[p * 64];
-        for (int k = 1; k < cb.z; ++k) {
-            const float2 v = pp[((long long)k * 16 + p) * 64];
-            s.x += v.x;
-            s.y += v.y;
+    float2 s[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) s[r] = pp[(wv * 4 + r) * 64];
+    for (int k = 1; k < cb.z; ++k) {
+        float2 v[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) v[r] = pp[((long long)k * 16 + wv * 4 + r) * 64];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            s[r].x += v[r].x;
+            s[r].y += v[r].y;
         }
-        g[(long long)(p / pg::kPatch) * Gy + (p % pg::kPatch)] = s;
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int p = wv * 4 + r;
+        g[(long long)(p / pg::kPatch) * Gy + (p % pg::kPatch)] = s[r];
     }
 }
 
@@ -225,9 +266,7 @@ template <int NT, int BC>
 __global__ void __launch_bounds__(256) k_spread_tile3(
     const int* __restrict__ lo, const float* __restrict__ w, int M, int Gx, int Gy, int Gz,
     const int* __restrict__ tile_off, const int* __restrict__ tile_idx, int ntx, int nty,
-    const float2* __restrict__ in, long long in_cstride, const float2* __restrict__ resid_y,
-    const double2* __restrict__ ax64, const float* __restrict__ Wd, int L,
-    const float* __restrict__ AA, int B, float2* __restrict__ grids, long long G)
+    const float2* __restrict__ V, int B, float2* __restrict__ grids, long long G)
 {
     __shared__ int s_lo[kChunk][3];
     __shared__ float s_w[kChunk][3][NT];
@@ -262,9 +301,7 @@ __global__ void __launch_bounds__(256) k_spread_tile3(
         for (int t = tid; t < cnt * BC; t += blockDim.x) {
             const int sl = t / BC, bb = t - sl * BC;
             const int b = b0 + bb;
-            s_v[sl][bb] = (b < B) ? spread_value(in, in_cstride, resid_y, ax64, Wd, AA, L, M, b,
-                                                 tile_idx[base + sl])
-                                  : make_float2(0.f, 0.f);
+            s_v[sl][bb] = (b < B) ? V[(long long)tile_idx[base + sl] * B + b] : make_float2(0.f, 0.f);
         }
         __syncthreads();
         for (int sl = 0; sl < cnt; ++sl) {
@@ -336,7 +373,7 @@ void interp_dispatch(const pg::Gnufft& G, const float2* grids, int nc, int L, co
                      float2* out, long long out_cstride, hipStream_t s)
 {
     const pg::Geometry& g = G.geo;
-    const int nsb = (G.M + 255) / 256;
+    const int nsb = (G.M + 63) / 64;  // 64 samples (4 lanes each) per block
     const int xmap = (nc % 8 == 0) ? 1 : 0;
     hipLaunchKernelGGL((k_interp<NT, D3>), dim3(nsb * nc), dim3(256), 0, s, G.lo, G.w, G.M, g.Gx,
                        g.Gy, g.Gz, grids, g.G(), nc, L, AA, out, out_cstride, nsb, xmap);
@@ -349,7 +386,7 @@ void spread_dispatch(const pg::Gnufft& G, const float2* in, long long in_cstride
 {
     const pg::Geometry& g = G.geo;
     const int B = nc * L;
-    if (B < 16) {
+    if (D3 && B < 16) {
         const long long n = (long long)B * g.G();
         hipLaunchKernelGGL(k_zero, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, grids, n);
         hipLaunchKernelGGL((k_spread_atomic<NT, D3>), dim3((G.M + 255) / 256, B), dim3(256), 0, s,
@@ -357,22 +394,26 @@ void spread_dispatch(const pg::Gnufft& G, const float2* in, long long in_cstride
                            AA, B, grids, g.G());
         return;
     }
+    // per-sample values of every grid, once
+    const long long nv = (long long)G.M * B;
+    hipLaunchKernelGGL(k_spread_values, dim3((unsigned)((nv + 255) / 256)), dim3(256), 0, s, in,
+                       in_cstride, resid_y, ax64, Wd, L, AA, G.M, B, G.vals);
     if (!D3) {
         const int nch = (B + 63) / 64;
         const int npy = g.Gy / pg::kPatch;
         hipLaunchKernelGGL(k_spread_patch, dim3(G.n_items, nch), dim3(64), 0, s, G.items, G.entries,
-                           G.lo, G.wpad, G.M, npy, in, in_cstride, resid_y, ax64, Wd, L, AA, B, grids,
-                           g.G(), g.Gy, G.partial, G.n_slots);
+                           G.lo, G.wpad, G.M, npy, G.vals, B, grids, g.G(), g.Gy, G.partial,
+                           G.n_slots);
         if (G.n_combine > 0)
-            hipLaunchKernelGGL(k_spread_combine, dim3(G.n_combine, nch), dim3(64), 0, s, G.combine,
+            hipLaunchKernelGGL(k_spread_combine, dim3(G.n_combine, nch), dim3(256), 0, s, G.combine,
                                npy, G.partial, G.n_slots, B, grids, g.G(), g.Gy);
         return;
     }
     constexpr int BC = 16;
     const int ntiles = G.ntx * G.nty * G.ntz;
     hipLaunchKernelGGL((k_spread_tile3<NT, BC>), dim3(ntiles, (B + BC - 1) / BC), dim3(256), 0, s,
-                       G.lo, G.w, G.M, g.Gx, g.Gy, g.Gz, G.tile_off, G.tile_idx, G.ntx, G.nty, in,
-                       in_cstride, resid_y, ax64, Wd, L, AA, B, grids, g.G());
+                       G.lo, G.w, G.M, g.Gx, g.Gy, g.Gz, G.tile_off, G.tile_idx, G.ntx, G.nty,
+                       G.vals, B, grids, g.G());
 }
 
 }  // namespace

@@ -45,3 +45,19 @@ def test_header_compiles_as_c_and_cxx(tmp_path):
                            "-o", str(tmp_path / "t.o")])
     subprocess.check_call(["g++", "-std=c++17", "-Wall", "-Werror", "-I", inc, "-x", "c++",
                            "-c", str(src), "-o", str(tmp_path / "t2.o")])
+
+
+def test_cpp_api_header_compiles(tmp_path):
+    inc = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include")
+    src = tmp_path / "t.cpp"
+    src.write_text("""
+#include "powergrid/powergrid.hpp"
+using namespace powergrid;
+int main() {
+    Col<float> kx(4), ky(4), kz(4), ix, iy, iz;
+    initImageSpaceCoords(ix, iy, iz, 2, 2, 1);
+    return (int)ix.n_rows - 4;
+}
+""")
+    subprocess.check_call(["g++", "-std=c++17", "-Wall", "-Werror", "-I", inc, "-fsyntax-only",
+                           str(src)])
