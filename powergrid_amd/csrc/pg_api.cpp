@@ -156,12 +156,10 @@ void Gnufft::ensure_batch(int B)
     dev_free(grids);
     dev_free(tbuf);
     dev_free(tbuf2);
-    dev_free(partial);
     const Geometry& g = geo;
     grids = dev_alloc<float2>((size_t)B * g.G());
     tbuf = dev_alloc<float2>((size_t)B * g.Nz * g.Nx * g.Gy);
     if (g.dims == 3) tbuf2 = dev_alloc<float2>((size_t)B * g.Nz * g.Gx * g.Gy);
-    if (n_slots > 0) partial = dev_alloc<float2>((size_t)((B + 63) / 64) * n_slots * 16 * 64);
     dev_free(vals);
     vals = dev_alloc<float2>((size_t)M * B);
     cap_batch = B;
@@ -171,26 +169,35 @@ void Gnufft::release()
 {
     dev_free(lo);
     dev_free(w);
-    dev_free(wpad);
     dev_free(dfx);
     dev_free(dfy);
     dev_free(dfz);
+    dev_free(dap);
     free_fft_plan(px);
     free_fft_plan(py);
     free_fft_plan(pz);
-    dev_free(items);
+    dev_free(witems);
+    dev_free(bgs);
     dev_free(entries);
-    dev_free(combine);
+    dev_free(iitems);
+    dev_free(irec);
+    dev_free(ent_w);
     dev_free(tile_off);
     dev_free(tile_idx);
     dev_free(grids);
     dev_free(tbuf);
     dev_free(tbuf2);
-    dev_free(partial);
     dev_free(vals);
     dev_free(img_tmp);
     dev_free(dat_tmp);
     cap_batch = 0;
+}
+
+static float __int_as_float_host(int v)
+{
+    float f;
+    std::memcpy(&f, &v, sizeof f);
+    return f;
 }
 
 // Per-dimension window of one sample: unclamped start and the NT weights.
@@ -308,6 +315,26 @@ static void gnufft_setup(Gnufft& G, size_t M, float os, size_t Nx, size_t Ny, si
     G.dfy = dev_copy(fy.data(), Ny);
     G.dfz = dev_copy(fz.data(), Nz);
     G.inv_osd = (D == 2) ? (float)(1.0 / ((double)os * os)) : (float)(1.0 / ((double)os * os * os));
+    {
+        // the same factors in FP64, combined per voxel (y fastest)
+        std::vector<double> ax(Nx), ay(Ny), az(Nz, 1.0);
+        for (size_t X = 0; X < Nx; ++X)
+            ax[X] = kb_ft(W, beta_d, ((double)X - (double)(Nx / 2)) / (double)Nx);
+        for (size_t Y = 0; Y < Ny; ++Y)
+            ay[Y] = kb_ft(W, beta_d, ((double)Y - (double)(Ny / 2)) / (double)Ny);
+        if (D == 3)
+            for (size_t Z = 0; Z < Nz; ++Z)
+                az[Z] = kb_ft(W, beta_d, ((double)Z - (double)Nz / 2.0) / (double)Nz);
+        const double osd = (D == 2) ? (double)os * os : (double)os * os * os;
+        std::vector<float> dap(Nx * Ny * Nz);
+        for (size_t Z = 0; Z < Nz; ++Z)
+            for (size_t X = 0; X < Nx; ++X)
+                for (size_t Y = 0; Y < Ny; ++Y) {
+                    const double k = ax[X] * ay[Y] * az[Z];
+                    dap[Y + X * Ny + Z * Nx * Ny] = std::isnan(k) ? 0.0f : (float)(1.0 / (osd * k));
+                }
+        G.dap = dev_copy(dap.data(), dap.size());
+    }
 
     G.px = make_fft_plan(g.Gx);
     G.py = make_fft_plan(g.Gy);
@@ -320,13 +347,76 @@ static void gnufft_setup(Gnufft& G, size_t M, float os, size_t Nx, size_t Ny, si
     };
 
     if (D == 2) {
-        // padded weights and 4x4 patch work items
-        std::vector<float> wpad((size_t)2 * M * 16, 0.0f);
-        for (int d = 0; d < 2; ++d)
-            for (size_t i = 0; i < M; ++i)
-                for (int j = 0; j < NT; ++j)
-                    wpad[((size_t)d * M + i) * 16 + 3 + j] = w[((size_t)d * NT + j) * M + i];
-        G.wpad = dev_copy(wpad.data(), wpad.size());
+        // interp tiles: the window start is re-based into [-padl, G-1] (taps
+        // moved off the start carry zero weight), binned by kIT x kIT tiles
+        const int padl = NT / 2;
+        G.ipadl = padl;
+        G.itx = (g.Gx + padl + kIT - 1) / kIT;
+        G.ity = (g.Gy + padl + kIT - 1) / kIT;
+        const int nt = G.itx * G.ity;
+        std::vector<int> rlo(2 * M);
+        std::vector<float> rw((size_t)2 * 8 * M, 0.0f);
+        std::vector<int> tcnt(nt + 1, 0), tof(M);
+        const int Gd[2] = {g.Gx, g.Gy};
+        for (size_t i = 0; i < M; ++i) {
+            int tt[2];
+            for (int d = 0; d < 2; ++d) {
+                int l0 = lo[(size_t)d * M + i];
+                int sh = 0;
+                if (empty(i)) {
+                    l0 = 0;
+                    sh = NT;  // all weights zero
+                } else if (l0 < -padl) {
+                    sh = -padl - l0;
+                    l0 = -padl;
+                }
+                PG_REQUIRE(l0 <= Gd[d] - 1, PG_ERR_ARG, "window start past the grid");
+                rlo[(size_t)d * M + i] = l0;
+                for (int j = 0; j < 8; ++j) {
+                    const int src = j + sh;
+                    rw[((size_t)d * M + i) * 8 + j] = (src < NT) ? w[((size_t)d * NT + src) * M + i] : 0.0f;
+                }
+                tt[d] = (l0 + padl) / kIT;
+            }
+            tof[i] = tt[0] * G.ity + tt[1];
+            tcnt[tof[i] + 1]++;
+        }
+        for (int t = 0; t < nt; ++t) tcnt[t + 1] += tcnt[t];
+        std::vector<int> pos(tcnt.begin(), tcnt.end() - 1), order(M);
+        for (size_t i = 0; i < M; ++i) order[pos[tof[i]]++] = (int)i;
+        std::vector<float4> rec((size_t)5 * M);
+        std::vector<int4> its;
+        for (int t = 0; t < nt; ++t) {
+            const int tx = t / G.ity, ty = t % G.ity;
+            const int rx0 = tx * kIT - padl, ry0 = ty * kIT - padl;
+            for (int k = tcnt[t]; k < tcnt[t + 1]; ++k) {
+                const int i = order[k];
+                const int ox = rlo[i] - rx0, oy = rlo[M + i] - ry0;
+                PG_REQUIRE(ox >= 0 && ox < kIT && oy >= 0 && oy < kIT, PG_ERR_ARG, "interp binning");
+                float4 h;
+                h.x = __int_as_float_host(i);
+                h.y = __int_as_float_host(ox);
+                h.z = __int_as_float_host(oy);
+                h.w = 0.0f;
+                const float* wx = &rw[(size_t)i * 8];
+                const float* wy = &rw[((size_t)M + i) * 8];
+                rec[(size_t)5 * k + 0] = h;
+                rec[(size_t)5 * k + 1] = make_float4(wx[0], wx[1], wx[2], wx[3]);
+                rec[(size_t)5 * k + 2] = make_float4(wx[4], wx[5], wx[6], wx[7]);
+                rec[(size_t)5 * k + 3] = make_float4(wy[0], wy[1], wy[2], wy[3]);
+                rec[(size_t)5 * k + 4] = make_float4(wy[4], wy[5], wy[6], wy[7]);
+            }
+            for (int k = tcnt[t]; k < tcnt[t + 1]; k += kIChunk)
+                its.push_back(make_int4(t, k, std::min(k + kIChunk, tcnt[t + 1]), 0));
+        }
+        // heaviest first
+        std::stable_sort(its.begin(), its.end(),
+                         [](const int4& a, const int4& b) { return (a.z - a.y) > (b.z - b.y); });
+        G.n_iitems = (int)its.size();
+        G.iitems = dev_copy(its.data(), its.size());
+        G.irec = dev_copy(rec.data(), rec.size());
+
+        // 4x4 patch work items with per-entry patch weights
         const int npx = g.Gx / kPatch, npy = g.Gy / kPatch;
         const long long np = (long long)npx * npy;
         std::vector<long long> cnt(np + 1, 0);
@@ -351,38 +441,145 @@ static void gnufft_setup(Gnufft& G, size_t M, float os, size_t Nx, size_t Ny, si
                     }
             }
             if (pass == 1) {
+                // weights of entry k (patch q, sample i) on the patch's 4 x and 4 y points
+                std::vector<float> ew((size_t)8 * std::max<long long>(1, cnt[np]), 0.0f);
+                auto tapw = [&](int d, size_t i, int j) {
+                    return (j >= 0 && j < NT) ? w[((size_t)d * NT + j) * M + i] : 0.0f;
+                };
+                for (long long q = 0; q < np; ++q) {
+                    const int px0 = (int)(q / npy) * kPatch, py0 = (int)(q % npy) * kPatch;
+                    for (long long k = cnt[q]; k < cnt[q + 1]; ++k) {
+                        const size_t i = (size_t)ent[k];
+                        for (int u = 0; u < kPatch; ++u) {
+                            ew[8 * k + u] = tapw(0, i, px0 + u - lo[i]);
+                            ew[8 * k + 4 + u] = tapw(1, i, py0 + u - lo[M + i]);
+                        }
+                    }
+                }
                 G.entries = dev_copy(ent.data(), ent.size());
+                G.ent_w = dev_copy(ew.data(), ew.size());
                 G.tile_entries = cnt[np];
             }
         }
-        // split patches with more than kMaxPatchEntries samples
-        const int kMaxPatchEntries = 64;
-        std::vector<int4> items, comb;
-        int slots = 0;
+        // XCD groups: the patches in Hilbert order over the npx x npy patch
+        // grid, cut into 8 runs of equal work; group g's blocks get ids
+        // congruent to g mod 8 (workgroups are dealt to the 8 XCDs
+        // round-robin), so a sample's row of values is read into one L2
+        const int kXcd = 8;
+        int hn = 1;
+        while (hn < std::max(npx, npy)) hn <<= 1;
+        std::vector<long long> hkey(np);
         for (long long q = 0; q < np; ++q) {
-            const int b = (int)cnt[q], e = (int)cnt[q + 1];
-            const int n = e - b;
-            if (n <= kMaxPatchEntries) {
-                items.push_back(make_int4((int)q, b, e, -1));
-            } else {
-                const int parts = (n + kMaxPatchEntries - 1) / kMaxPatchEntries;
-                comb.push_back(make_int4((int)q, slots, parts, 0));
-                for (int k = 0; k < parts; ++k) {
-                    const int pb = b + (int)((long long)n * k / parts);
-                    const int pe = b + (int)((long long)n * (k + 1) / parts);
-                    items.push_back(make_int4((int)q, pb, pe, slots + k));
+            int x = (int)(q / npy), y = (int)(q % npy);
+            long long d = 0;
+            for (int sq = hn / 2; sq > 0; sq /= 2) {
+                const int rx = (x & sq) > 0, ry = (y & sq) > 0;
+                d += (long long)sq * sq * ((3 * rx) ^ ry);
+                if (ry == 0) {
+                    if (rx == 1) {
+                        x = hn - 1 - x;
+                        y = hn - 1 - y;
+                    }
+                    std::swap(x, y);
                 }
-                slots += parts;
+            }
+            hkey[q] = d;
+        }
+        std::vector<int> horder(np);
+        for (long long q = 0; q < np; ++q) horder[q] = (int)q;
+        std::stable_sort(horder.begin(), horder.end(),
+                         [&](int u, int v) { return hkey[u] < hkey[v]; });
+        auto pweight = [&](long long q) { return (double)(cnt[q + 1] - cnt[q]) + 16.0; };
+        double wtot = 0;
+        for (long long q = 0; q < np; ++q) wtot += pweight(q);
+        std::vector<std::vector<int>> grp(kXcd);
+        {
+            double run = 0;
+            for (int q : horder) {
+                const double mid = run + 0.5 * pweight(q);
+                grp[std::min(kXcd - 1, (int)(mid * kXcd / wtot))].push_back(q);
+                run += pweight(q);
             }
         }
-        // heaviest items first, so that the long ones start early
-        std::stable_sort(items.begin(), items.end(),
-                         [](const int4& a, const int4& b) { return (a.z - a.y) > (b.z - b.y); });
-        G.n_items = (int)items.size();
-        G.n_combine = (int)comb.size();
-        G.n_slots = slots;
-        G.items = dev_copy(items.data(), items.size());
-        if (!comb.empty()) G.combine = dev_copy(comb.data(), comb.size());
+        // per group: blocks of 16 wavefront items; a patch of n entries takes
+        // gs = 1 (n <= 64) or the power of two >= n/64 (at most 16) wavefronts
+        struct SBlock {
+            int gs;
+            int cost;
+            int4 w[16];
+        };
+        const int kW = 16;
+        std::vector<std::vector<SBlock>> gblk(kXcd);
+        for (int gi = 0; gi < kXcd; ++gi) {
+            std::vector<std::vector<int>> byg(5);  // gs = 1, 2, 4, 8, 16
+            for (int q : grp[gi]) {
+                const long long n = cnt[q + 1] - cnt[q];
+                int gs = 1, lg = 0;
+                while (gs < kW && (long long)gs * 64 < n) {
+                    gs *= 2;
+                    ++lg;
+                }
+                byg[lg].push_back(q);
+            }
+            for (int lg = 0; lg < 5; ++lg) {
+                const int gs = 1 << lg, per = kW / gs;
+                auto& v = byg[lg];
+                std::stable_sort(v.begin(), v.end(), [&](int u, int w) {
+                    return cnt[u + 1] - cnt[u] > cnt[w + 1] - cnt[w];
+                });
+                for (size_t k0 = 0; k0 < v.size(); k0 += per) {
+                    SBlock sb;
+                    sb.gs = gs;
+                    sb.cost = 0;
+                    for (int w = 0; w < kW; ++w) sb.w[w] = make_int4(-1, 0, 0, 0);
+                    for (int pk = 0; pk < per && k0 + pk < v.size(); ++pk) {
+                        const int q = v[k0 + pk];
+                        const long long b0 = cnt[q], n = cnt[q + 1] - cnt[q];
+                        for (int w = 0; w < gs; ++w) {
+                            const int e0 = (int)(b0 + n * w / gs), e1 = (int)(b0 + n * (w + 1) / gs);
+                            sb.w[pk * gs + w] = make_int4(q, e0, e1, 0);
+                            sb.cost = std::max(sb.cost, e1 - e0);
+                        }
+                    }
+                    gblk[gi].push_back(sb);
+                }
+            }
+            // the longest blocks first
+            std::stable_sort(gblk[gi].begin(), gblk[gi].end(),
+                             [](const SBlock& u, const SBlock& w) { return u.cost > w.cost; });
+        }
+        size_t nmax = 0;
+        for (int gi = 0; gi < kXcd; ++gi) nmax = std::max(nmax, gblk[gi].size());
+        std::vector<int4> witems(nmax * kXcd * kW);
+        std::vector<int> bgs(nmax * kXcd, 1);
+        for (size_t k = 0; k < nmax; ++k)
+            for (int gi = 0; gi < kXcd; ++gi) {
+                const size_t blk = k * kXcd + gi;
+                for (int w = 0; w < kW; ++w)
+                    witems[blk * kW + w] = (k < gblk[gi].size()) ? gblk[gi][k].w[w] : make_int4(-1, 0, 0, 0);
+                if (k < gblk[gi].size()) bgs[blk] = gblk[gi][k].gs;
+            }
+        {
+            // every patch is one group of gs wavefronts whose ranges tile its entries
+            std::vector<long long> next(np);
+            for (long long q = 0; q < np; ++q) next[q] = cnt[q];
+            for (size_t blk = 0; blk < nmax * kXcd; ++blk) {
+                const int gs = bgs[blk];
+                for (int w = 0; w < kW; ++w) {
+                    const int4 it = witems[blk * kW + w];
+                    if (it.x < 0) continue;
+                    PG_REQUIRE(it.x < np && next[it.x] == it.y && it.z >= it.y &&
+                                   ((w % gs == 0) == (it.y == cnt[it.x])),
+                               PG_ERR_STATE, "spread work list");
+                    next[it.x] = it.z;
+                }
+            }
+            for (long long q = 0; q < np; ++q)
+                PG_REQUIRE(next[q] == cnt[q + 1], PG_ERR_STATE, "spread work list coverage");
+        }
+        G.n_sblocks = (int)(nmax * kXcd);
+        G.witems = dev_copy(witems.data(), witems.size());
+        G.bgs = dev_copy(bgs.data(), bgs.size());
     } else {
         // 3-D: samples whose window meets each 8x8x4 tile
         const int TX = 8, TY = 8, TZ = 4;
@@ -758,6 +955,7 @@ struct CgWork {
     float2 *ngrad = nullptr, *ddir = nullptr, *Adir = nullptr;
     float2 *g = nullptr, *r = nullptr, *tmp = nullptr;
     double* part = nullptr;
+    pgv::CgParts cp{};
     double* resid = nullptr;
     CgScal* sc = nullptr;
     ~CgWork()
@@ -788,17 +986,17 @@ static void cg_iteration(Op* A, Penalty* R, const float2* y, const float* W, CgW
         SenseOp* se = (A->kind == K_SENSE) ? static_cast<SenseOp*>(A) : nullptr;
         float2* P = se ? se->P : w.g;
         fused_adjoint_partial(*f, nullptr, y, w.Ax, W, P, s);
-        pgv::cg_grad(P, f->nc, N, ps, w.x32, w.ngrad, w.part, w.sc, s);
+        pgv::cg_grad(P, f->nc, N, ps, w.x32, w.ngrad, w.cp, w.sc, w.resid, s);
     } else {
         pgv::resid(w.r, y, w.Ax, W, M, s);
         op_adjoint(A, w.r, w.g, s);
-        pgv::cg_grad(w.g, 1, N, ps, w.x32, w.ngrad, w.part, w.sc, s);
+        pgv::cg_grad(w.g, 1, N, ps, w.x32, w.ngrad, w.cp, w.sc, w.resid, s);
     }
-    pgv::cg_dir(w.ngrad, w.ddir, N, w.part, w.sc, s);
+    pgv::cg_dir(w.ngrad, w.ddir, N, w.cp, w.sc, s);
     op_forward(A, w.ddir, w.Adir, s);
-    pgv::cg_adir(w.Adir, y, w.Ax, W, M, w.part, w.sc, s);
-    pgv::cg_linesearch(ps, w.ddir, w.x32, w.part, w.sc, s);
-    pgv::cg_update(w.Ax, w.Adir, y, M, w.x, w.x32, w.ddir, N, w.part, w.sc, w.resid, s);
+    pgv::cg_adir(w.Adir, y, w.Ax, W, M, ps, w.ddir, w.x32, w.cp, w.sc, s);
+    pgv::cg_linesearch(ps, w.ddir, w.x32, w.cp, w.sc, s);
+    pgv::cg_update(w.Ax, w.Adir, y, M, w.x, w.x32, w.ddir, N, ps != nullptr, w.cp, w.sc, s);
 }
 
 static void solve_pcg(Op* A, Penalty* R, const float2* x0, const float* W, const float2* y,
@@ -816,7 +1014,8 @@ static void solve_pcg(Op* A, Penalty* R, const float2* x0, const float* W, const
     w.Adir = dev_alloc<float2>(M);
     w.g = dev_alloc<float2>(N);
     w.r = dev_alloc<float2>(M);
-    w.part = dev_alloc<double>(2 * pgv::kMaxParts);
+    w.part = dev_alloc<double>(std::max<size_t>(pgv::cg_parts_doubles(N, M), 2 * pgv::kMaxParts));
+    w.cp = pgv::cg_parts(w.part, N, M);
     w.resid = dev_alloc<double>(std::max<size_t>(niter, 1));
     w.sc = dev_alloc<CgScal>(1);
 
@@ -861,6 +1060,7 @@ static void solve_pcg(Op* A, Penalty* R, const float2* x0, const float* W, const
         PG_HIP(hipStreamEndCapture(s, &graph));
         PG_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
         for (size_t it = 0; it < niter; ++it) PG_HIP(hipGraphLaunch(exec, s));
+        pgv::cg_finish(w.cp, w.sc, w.resid, s);
         PG_HIP(hipMemcpyAsync(&hs, w.sc, sizeof(CgScal), hipMemcpyDeviceToHost, s));
         PG_HIP(hipStreamSynchronize(s));
         hipGraphExecDestroy(exec);
@@ -873,10 +1073,9 @@ static void solve_pcg(Op* A, Penalty* R, const float2* x0, const float* W, const
             PG_HIP(hipStreamSynchronize(s));
             if (hs.stop) break;
         }
-        if (niter == 0) {
-            PG_HIP(hipMemcpyAsync(&hs, w.sc, sizeof(CgScal), hipMemcpyDeviceToHost, s));
-            PG_HIP(hipStreamSynchronize(s));
-        }
+        pgv::cg_finish(w.cp, w.sc, w.resid, s);
+        PG_HIP(hipMemcpyAsync(&hs, w.sc, sizeof(CgScal), hipMemcpyDeviceToHost, s));
+        PG_HIP(hipStreamSynchronize(s));
     }
     PG_HIP(hipMemcpyAsync(x_out, w.x32, sizeof(float2) * N, hipMemcpyDeviceToDevice, s));
     std::vector<double> rh(std::max<size_t>(niter, 1), 0.0);

@@ -12,9 +12,12 @@
 //            (3-D: FFT_x -> U, then FFT_z (Nz non-zero inputs) -> grid)
 //   adjoint: [IFFT_x (keep the Nx cropped outputs)]        -> T   (3-D: IFFT_z first)
 //            [IFFT_y + crop + deapod + WoH + sum_l + conj(S)] -> image
-// Each 1-D transform is a mixed radix-4/2/3 Stockham FFT staged in LDS.
+// Power-of-two lengths 16..1024 use the register-resident two-level FFT of
+// pg_fftreg.h (one LDS exchange per line); other 2^a 3^b lengths use a mixed
+// radix-4/2/3 Stockham FFT staged in LDS.
 #include "pg_internal.h"
 #include "pg_gnufft.h"
+#include "pg_fftreg.h"
 
 namespace {
 
@@ -285,6 +288,294 @@ int pick_cpw(int Gy, int Ny)
 
 int pick_rpw(int n) { return n <= 256 ? 16 : 8; }
 
+// ------------------------------------------------------------------------
+// Register-FFT kernels for power-of-two grid lengths (pg_fftreg.h).  Same
+// index conventions as the LDS Stockham kernels above.
+
+// Forward prep + FFT_y: one line per R2 threads (a wavefront holds 64/R2
+// whole columns, so the exchange is wavefront-local).  Grid: (column groups,
+// B); 64*wpb threads per block (wpb <= 4).  Every load is issued
+// unconditionally at a clamped in-bounds address and masked afterwards, so
+// that all loads of a thread are in flight together.  dap[n] =
+// 1/(os^d psi(n)), 0 where psi is NaN (griddingSupport.cpp:262-274).
+// HAS: bit 0 = coil maps S, bit 1 = segment phases Wo.
+template <int R1, int R2, int HAS>
+__global__ void __launch_bounds__(256) k_fwd_col_reg(
+    const float2* __restrict__ img, long long img_cstride, const float2* __restrict__ S,
+    const float2* __restrict__ Wo, const float* __restrict__ dap, int Nx, int Ny, int Nz,
+    int offY, int L, const float2* __restrict__ tw, float2* __restrict__ T)
+{
+    constexpr int N = R1 * R2;            // Gy
+    constexpr int LPW = 64 / R2;          // columns per wavefront
+    constexpr int PITCH = R1 * (R2 + 1);  // LDS float2 per column
+    __shared__ float2 lds[4 * LPW * PITCH];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int t = lane % R2, jl = lane / R2;
+    const int b = blockIdx.y;
+    const int c = b / L, l = b - c * L;
+    const int ncol = Nx * Nz;
+    const int col = (blockIdx.x * (blockDim.x >> 6) + wv) * LPW + jl;
+    const bool valid = col < ncol;
+    const long long Nimg = (long long)Nx * Ny * Nz;
+    const int shift = (offY + N / 2) % N;  // image row y sits at (y + shift) % N
+    const long long cofs = (long long)(valid ? col : 0) * Ny;
+    const float2* im = img + c * img_cstride + cofs;
+    const float2* Sc = S + (long long)c * Nimg + cofs;
+    const float2* Wl = Wo + (long long)l * Nimg + cofs;
+    const float* dp = dap + cofs;
+
+    float2 v[R1];
+#pragma unroll
+    for (int m = 0; m < R1; ++m) {
+        const int y = (t + R2 * m - shift) & (N - 1);
+        const bool ok = valid && y < Ny;
+        const int yc = ok ? y : 0;
+        float2 a = im[yc];
+        const float d = dp[yc];
+        if (HAS & 1) a = pgf::cmul(Sc[yc], a);
+        if (HAS & 2) a = pgf::cmul(Wl[yc], a);
+        v[m] = ok ? make_float2(a.x * d, a.y * d) : make_float2(0.f, 0.f);
+    }
+    float2* my = lds + (wv * LPW + jl) * PITCH;
+    pgf::line_fft<R1, R2, -1, false>(v, t, my, [](int k1, int q) { return k1 * (R2 + 1) + q; },
+                                     tw);
+    if (!valid) return;
+    float2* dst = T + ((long long)b * ncol + col) * N;
+#pragma unroll
+    for (int r = 0; r < R1 / R2; ++r)
+#pragma unroll
+        for (int k2 = 0; k2 < R2; ++k2) {
+            const int k = t + R2 * r + R1 * k2;
+            dst[(k + N / 2) & (N - 1)] = v[r * R2 + k2];
+        }
+}
+
+// FFT along a strided axis (same arguments as k_line_fft).  A block holds
+// LPB consecutive inner lines x R2 threads, inner index fastest across lanes
+// so that every global access is a run of LPB consecutive elements.
+template <int R1, int R2>
+constexpr int line_lpb() { return R1 * R2 >= 1024 ? 8 : 16; }
+
+template <int R1, int R2, int SIGN>
+__global__ void __launch_bounds__(512) k_line_reg(const float2* __restrict__ src,
+                                                  float2* __restrict__ dst, long long inner_n,
+                                                  long long src_os, long long dst_os, int nin,
+                                                  int in_off, int nout, int out_off,
+                                                  const float2* __restrict__ tw)
+{
+    constexpr int N = R1 * R2;
+    constexpr int LPB = line_lpb<R1, R2>();
+    constexpr int KST = R2 * LPB + 16;  // LDS stride between k1 rows (padded)
+    __shared__ float2 lds[R1 * KST];
+    const int j = threadIdx.x % LPB, t = threadIdx.x / LPB;
+    const long long outer = blockIdx.y;
+    const long long inner = (long long)blockIdx.x * LPB + j;
+    const bool valid = inner < inner_n;
+    const float2* s = src + outer * src_os + (valid ? inner : 0);
+    float2* d = dst + outer * dst_os + inner;
+
+    float2 v[R1];
+#pragma unroll
+    for (int m = 0; m < R1; ++m) {
+        const int i = (t + R2 * m - in_off) & (N - 1);  // input entry at position p
+        const bool ok = valid && i < nin;
+        const float2 a = s[(long long)(ok ? i : 0) * inner_n];
+        v[m] = ok ? a : make_float2(0.f, 0.f);
+    }
+    pgf::line_fft<R1, R2, SIGN, true>(
+        v, t, lds, [j](int k1, int q) { return k1 * KST + q * LPB + j; }, tw);
+    if (!valid) return;
+#pragma unroll
+    for (int r = 0; r < R1 / R2; ++r)
+#pragma unroll
+        for (int k2 = 0; k2 < R2; ++k2) {
+            const int k = t + R2 * r + R1 * k2;
+            const int o = (k - out_off) & (N - 1);  // output entry taken from position k
+            if (o < nout) d[(long long)o * inner_n] = v[r * R2 + k2];
+        }
+}
+
+// Adjoint tail for coil c: IFFT_y of the L segment columns, crop, deapod,
+// x WoH_l, sum over l, x conj(S_c).  Block = NW wavefronts over the same
+// LPW columns; wavefront w takes segments l = w, w+NW, ...; the NW partial
+// sums are added in a fixed order through LDS (deterministic).
+template <int R1>
+constexpr int post_waves() { return R1 >= 32 ? 4 : 8; }
+
+template <int R1, int R2, int HAS>
+__global__ void __launch_bounds__(64 * post_waves<R1>()) k_adj_post_reg(
+    const float2* __restrict__ T, const float2* __restrict__ S, const float2* __restrict__ WoH,
+    const float* __restrict__ dap, int Nx, int Ny, int Nz, int offY, int L,
+    const float2* __restrict__ tw, float2* __restrict__ out, long long out_cstride)
+{
+    constexpr int N = R1 * R2;  // Gy
+    constexpr int LPW = 64 / R2;
+    constexpr int PITCH = R1 * (R2 + 1);
+    constexpr int NW = post_waves<R1>();
+    __shared__ float2 lds[NW * LPW * PITCH];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int t = lane % R2, jl = lane / R2;
+    const int c = blockIdx.y;
+    const int ncol = Nx * Nz;
+    const int col = blockIdx.x * LPW + jl;
+    const bool valid = col < ncol;
+    const long long Nimg = (long long)Nx * Ny * Nz;
+    const int shift = (offY + N / 2) % N;  // image row y sits at (y + shift) % N
+    const long long cofs = (long long)(valid ? col : 0) * Ny;
+    const float* dp = dap + cofs;
+
+    // the image rows held by this thread after the transform, and their factors
+    int yk[R1];
+    float dk[R1];
+#pragma unroll
+    for (int r = 0; r < R1 / R2; ++r)
+#pragma unroll
+        for (int k2 = 0; k2 < R2; ++k2) {
+            const int y = (t + R2 * r + R1 * k2 - shift) & (N - 1);
+            const bool ok = valid && y < Ny;
+            yk[r * R2 + k2] = ok ? y : -1;
+            const float d = dp[ok ? y : 0];
+            dk[r * R2 + k2] = ok ? d : 0.f;
+        }
+
+    float2 acc[R1];
+#pragma unroll
+    for (int i = 0; i < R1; ++i) acc[i] = make_float2(0.f, 0.f);
+    float2* my = lds + (wv * LPW + jl) * PITCH;
+    for (int l = wv; l < L; l += NW) {
+        const float2* src = T + (((long long)c * L + l) * ncol + (valid ? col : 0)) * N;
+        float2 v[R1];
+#pragma unroll
+        for (int m = 0; m < R1; ++m) v[m] = src[(t + R2 * m + N / 2) & (N - 1)];
+        pgf::line_fft<R1, R2, 1, false>(v, t, my, [](int k1, int q) { return k1 * (R2 + 1) + q; },
+                                        tw);
+        const float2* Wl = WoH + (long long)l * Nimg + cofs;
+#pragma unroll
+        for (int i = 0; i < R1; ++i) {
+            float2 q = make_float2(v[i].x * dk[i], v[i].y * dk[i]);
+            if (HAS & 2) q = pgf::cmul(q, Wl[yk[i] < 0 ? 0 : yk[i]]);
+            acc[i].x += q.x;
+            acc[i].y += q.y;
+        }
+    }
+    // cross-wavefront sum: red[w][jl][y]
+    __syncthreads();
+    float2* red = lds;
+#pragma unroll
+    for (int i = 0; i < R1; ++i)
+        if (yk[i] >= 0) red[(wv * LPW + jl) * Ny + yk[i]] = acc[i];
+    __syncthreads();
+    for (int e = threadIdx.x; e < LPW * Ny; e += blockDim.x) {
+        const int jj = e / Ny, y = e - jj * Ny;
+        const int cc = blockIdx.x * LPW + jj;
+        if (cc >= ncol) continue;
+        float2 sum = red[jj * Ny + y];
+        for (int w = 1; w < NW; ++w) {
+            const float2 a = red[(w * LPW + jj) * Ny + y];
+            sum.x += a.x;
+            sum.y += a.y;
+        }
+        const long long nidx = (long long)y + (long long)cc * Ny;
+        if (HAS & 1) {
+            float2 sc = S[(long long)c * Nimg + nidx];
+            sc.y = -sc.y;
+            sum = pgf::cmul(sum, sc);
+        }
+        out[c * out_cstride + nidx] = sum;
+    }
+}
+
+bool reg_len(int n) { return n >= 16 && n <= 1024 && (n & (n - 1)) == 0; }
+
+template <int N>
+void fwd_col_reg_n(const pg::Gnufft& G, const float2* img, long long img_cstride,
+                   const pgk::WeightArgs& w, int B, int L, float2* tbuf, hipStream_t s)
+{
+    constexpr int R1 = pgf::Split<N>::R1, R2 = pgf::Split<N>::R2;
+    const pg::Geometry& g = G.geo;
+    const int ncol = g.Nx * g.Nz;
+    const int lpw = 64 / R2;
+    const int waves = (ncol + lpw - 1) / lpw;
+    const int wpb = (waves * B >= 1024) ? 4 : 1;
+    const int has = (w.S ? 1 : 0) | (w.Wo ? 2 : 0);
+    const dim3 grid((waves + wpb - 1) / wpb, B), block(64 * wpb);
+#define PG_COL(H)                                                                                 \
+    hipLaunchKernelGGL((k_fwd_col_reg<R1, R2, H>), grid, block, 0, s, img, img_cstride, w.S, w.Wo, \
+                       G.dap, g.Nx, g.Ny, g.Nz, g.offY, L, (const float2*)G.py.tw, tbuf)
+    switch (has) {
+    case 0: PG_COL(0); break;
+    case 1: PG_COL(1); break;
+    case 2: PG_COL(2); break;
+    default: PG_COL(3); break;
+    }
+#undef PG_COL
+}
+
+template <int N, int SIGN>
+void line_reg_n(const float2* src, float2* dst, long long inner_n, long long src_os,
+                long long dst_os, int nin, int in_off, int nout, int out_off, unsigned outer,
+                const pg::FftPlan1D& p, hipStream_t s)
+{
+    constexpr int R1 = pgf::Split<N>::R1, R2 = pgf::Split<N>::R2;
+    constexpr int LPB = line_lpb<R1, R2>();
+    hipLaunchKernelGGL((k_line_reg<R1, R2, SIGN>),
+                       dim3((unsigned)((inner_n + LPB - 1) / LPB), outer), dim3(LPB * R2), 0, s, src, dst, inner_n, src_os, dst_os, nin, in_off % N,
+                       nout, out_off % N, (const float2*)p.tw);
+}
+
+template <int N>
+void adj_post_reg_n(const pg::Gnufft& G, const float2* tbuf, const pgk::WeightArgs& w, int nc,
+                    int L, float2* out, long long out_cstride, hipStream_t s)
+{
+    constexpr int R1 = pgf::Split<N>::R1, R2 = pgf::Split<N>::R2;
+    const pg::Geometry& g = G.geo;
+    const int ncol = g.Nx * g.Nz;
+    const int lpw = 64 / R2;
+    const int has = (w.S ? 1 : 0) | (w.WoH ? 2 : 0);
+    const dim3 grid((ncol + lpw - 1) / lpw, nc), block(64 * post_waves<R1>());
+#define PG_POST(H)                                                                               \
+    hipLaunchKernelGGL((k_adj_post_reg<R1, R2, H>), grid, block, 0, s, tbuf, w.S, w.WoH, G.dap,   \
+                       g.Nx, g.Ny, g.Nz, g.offY, L, (const float2*)G.py.tw, out, out_cstride)
+    switch (has) {
+    case 0: PG_POST(0); break;
+    case 1: PG_POST(1); break;
+    case 2: PG_POST(2); break;
+    default: PG_POST(3); break;
+    }
+#undef PG_POST
+}
+
+#define PG_REG_SWITCH(n, CALL) \
+    switch (n) {               \
+    case 16: CALL(16); break;  \
+    case 32: CALL(32); break;  \
+    case 64: CALL(64); break;  \
+    case 128: CALL(128); break; \
+    case 256: CALL(256); break; \
+    case 512: CALL(512); break; \
+    default: CALL(1024); break; \
+    }
+
+// strided line FFT: register kernel for power-of-two lengths, Stockham otherwise
+template <int SIGN>
+void line_fft_any(const float2* src, float2* dst, long long inner_n, long long src_os,
+                  long long dst_os, int nin, int in_off, int nout, int out_off, unsigned outer,
+                  const pg::FftPlan1D& p, hipStream_t s)
+{
+    if (reg_len(p.n)) {
+#define PG_CALL_LINE(NN) \
+    line_reg_n<NN, SIGN>(src, dst, inner_n, src_os, dst_os, nin, in_off, nout, out_off, outer, p, s)
+        PG_REG_SWITCH(p.n, PG_CALL_LINE)
+#undef PG_CALL_LINE
+        return;
+    }
+    const int rpw = pick_rpw(p.n);
+    const size_t lds = sizeof(float2) * 2 * rpw * (p.n + 1);
+    hipLaunchKernelGGL(k_line_fft<SIGN>, dim3((unsigned)((inner_n + rpw - 1) / rpw), outer),
+                       dim3(256), lds, s, src, dst, inner_n, src_os, dst_os, nin, in_off, nout,
+                       out_off, rpw, pg::fft_args(p));
+}
+
 }  // namespace
 
 namespace pgk {
@@ -296,28 +587,26 @@ void launch_fwd_prep_fft(const pg::Gnufft& G, const float2* img, long long img_c
     const pg::Geometry& g = G.geo;
     const int B = nc * L;
     const int ncol = g.Nx * g.Nz;
-    const int cpw = pick_cpw(g.Gy, g.Ny);
-    const size_t lds1 = sizeof(float2) * 2 * cpw * (g.Gy + 1);
-    hipLaunchKernelGGL(k_fwd_prep_colfft, dim3((ncol + cpw - 1) / cpw, B), dim3(256), lds1, s, img,
-                       img_cstride, w.S, w.Wo, G.dfx, G.dfy, G.dfz, G.inv_osd, g.Nx, g.Ny, g.Nz,
-                       g.offY, L, cpw, pg::fft_args(G.py), tbuf);
+    if (reg_len(g.Gy)) {
+#define PG_CALL_COL(NN) fwd_col_reg_n<NN>(G, img, img_cstride, w, B, L, tbuf, s)
+        PG_REG_SWITCH(g.Gy, PG_CALL_COL)
+#undef PG_CALL_COL
+    } else {
+        const int cpw = pick_cpw(g.Gy, g.Ny);
+        const size_t lds1 = sizeof(float2) * 2 * cpw * (g.Gy + 1);
+        hipLaunchKernelGGL(k_fwd_prep_colfft, dim3((ncol + cpw - 1) / cpw, B), dim3(256), lds1, s,
+                           img, img_cstride, w.S, w.Wo, G.dfx, G.dfy, G.dfz, G.inv_osd, g.Nx, g.Ny,
+                           g.Nz, g.offY, L, cpw, pg::fft_args(G.py), tbuf);
+    }
     // FFT along x: rows (b, z, gy), Nx non-zero inputs at (x + offX + Gx/2) % Gx
-    const int rpw = pick_rpw(g.Gx);
-    const size_t ldsx = sizeof(float2) * 2 * rpw * (g.Gx + 1);
     float2* xdst = (g.dims == 2) ? grids : tbuf2;
-    hipLaunchKernelGGL(k_line_fft<-1>, dim3((g.Gy + rpw - 1) / rpw, (unsigned)B * g.Nz), dim3(256),
-                       ldsx, s, tbuf, xdst, (long long)g.Gy, (long long)g.Nx * g.Gy,
-                       (long long)g.Gx * g.Gy, g.Nx, g.offX + g.Gx / 2, g.Gx, g.Gx / 2, rpw,
-                       pg::fft_args(G.px));
+    line_fft_any<-1>(tbuf, xdst, (long long)g.Gy, (long long)g.Nx * g.Gy, (long long)g.Gx * g.Gy,
+                     g.Nx, (g.offX + g.Gx / 2) % g.Gx, g.Gx, g.Gx / 2, (unsigned)B * g.Nz, G.px, s);
     if (g.dims == 3) {
         // FFT along z: lines (b, gx, gy), Nz non-zero inputs at (z + offZ + Gz/2) % Gz
-        const int rpz = pick_rpw(g.Gz);
-        const size_t ldsz = sizeof(float2) * 2 * rpz * (g.Gz + 1);
         const long long inner = (long long)g.Gx * g.Gy;
-        hipLaunchKernelGGL(k_line_fft<-1>, dim3((unsigned)((inner + rpz - 1) / rpz), B), dim3(256),
-                           ldsz, s, tbuf2, grids, inner, (long long)g.Nz * inner,
-                           (long long)g.Gz * inner, g.Nz, g.offZ + g.Gz / 2, g.Gz, g.Gz / 2, rpz,
-                           pg::fft_args(G.pz));
+        line_fft_any<-1>(tbuf2, grids, inner, (long long)g.Nz * inner, (long long)g.Gz * inner,
+                         g.Nz, (g.offZ + g.Gz / 2) % g.Gz, g.Gz, g.Gz / 2, (unsigned)B, G.pz, s);
     }
 }
 
@@ -330,22 +619,20 @@ void launch_adj_fft_post(const pg::Gnufft& G, float2* grids, float2* tbuf, float
     const float2* xsrc = grids;
     if (g.dims == 3) {
         // IFFT along z on the spread grid (ifftshift on load), keep Nz planes
-        const int rpz = pick_rpw(g.Gz);
-        const size_t ldsz = sizeof(float2) * 2 * rpz * (g.Gz + 1);
         const long long inner = (long long)g.Gx * g.Gy;
-        hipLaunchKernelGGL(k_line_fft<+1>, dim3((unsigned)((inner + rpz - 1) / rpz), B), dim3(256),
-                           ldsz, s, grids, tbuf2, inner, (long long)g.Gz * inner,
-                           (long long)g.Nz * inner, g.Gz, g.Gz / 2, g.Nz, g.offZ + g.Gz / 2, rpz,
-                           pg::fft_args(G.pz));
+        line_fft_any<+1>(grids, tbuf2, inner, (long long)g.Gz * inner, (long long)g.Nz * inner,
+                         g.Gz, g.Gz / 2, g.Nz, (g.offZ + g.Gz / 2) % g.Gz, (unsigned)B, G.pz, s);
         xsrc = tbuf2;
     }
     // IFFT along x (ifftshift on load), keep the Nx cropped columns
-    const int rpw = pick_rpw(g.Gx);
-    const size_t ldsx = sizeof(float2) * 2 * rpw * (g.Gx + 1);
-    hipLaunchKernelGGL(k_line_fft<+1>, dim3((g.Gy + rpw - 1) / rpw, (unsigned)B * g.Nz), dim3(256),
-                       ldsx, s, xsrc, tbuf, (long long)g.Gy, (long long)g.Gx * g.Gy,
-                       (long long)g.Nx * g.Gy, g.Gx, g.Gx / 2, g.Nx, g.offX + g.Gx / 2, rpw,
-                       pg::fft_args(G.px));
+    line_fft_any<+1>(xsrc, tbuf, (long long)g.Gy, (long long)g.Gx * g.Gy, (long long)g.Nx * g.Gy,
+                     g.Gx, g.Gx / 2, g.Nx, (g.offX + g.Gx / 2) % g.Gx, (unsigned)B * g.Nz, G.px, s);
+    if (reg_len(g.Gy)) {
+#define PG_CALL_POST(NN) adj_post_reg_n<NN>(G, tbuf, w, nc, L, out, out_cstride, s)
+        PG_REG_SWITCH(g.Gy, PG_CALL_POST)
+#undef PG_CALL_POST
+        return;
+    }
     const int cpw = pick_cpw(g.Gy, g.Ny);
     const size_t lds1 = sizeof(float2) * 2 * cpw * (g.Gy + 1);
     const int ncol = g.Nx * g.Nz;

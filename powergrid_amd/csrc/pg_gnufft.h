@@ -13,6 +13,9 @@
 namespace pg {
 
 constexpr int kPatch = 4;  // 2-D spread patch edge (grid points)
+constexpr int kIT = 16;    // 2-D interp: tile edge in window starts
+constexpr int kIR = kIT + 7;  // staged grid region edge (8 taps)
+constexpr int kIChunk = 128;  // samples per interp work item
 
 struct Gnufft {
     Geometry geo;
@@ -22,26 +25,38 @@ struct Gnufft {
     // tap weights, structure-of-arrays: w[(d*NT + j)*M + i] (0 for taps that
     // fall outside the clamped window)
     float* w = nullptr;
-    // 2-D spread: weights padded to 16 entries per sample and dimension,
-    // wpad[(d*M + i)*16 + 3 + j] = weight of tap j (zeros around), so that a
-    // 4-point patch reads 4 consecutive entries at offset (patch - lo + 3)
-    float* wpad = nullptr;
     // per-axis KB transforms psi(u) (griddingSupport.cpp:246-260); the
     // deapodization divides by dfx[x]*dfy[y](*dfz[z])
     float* dfx = nullptr;
     float* dfy = nullptr;
     float* dfz = nullptr;
     float inv_osd = 0.25f;  // 1/os^d
+    // full deapodization factor per image voxel, computed in FP64:
+    // dap[n] = 1/(os^d psi_x psi_y psi_z), 0 where psi is NaN
+    float* dap = nullptr;
     // FFT plans for the three grid lengths
     FftPlan1D px, py, pz;
-    // 2-D batched spread: work items over 4x4 patches
-    //   items[k] = {patch, entry begin, entry end, partial slot (-1: direct)}
-    int4* items = nullptr;
-    int n_items = 0;
+    // 2-D batched spread over 4x4 patches: blocks of 16 wavefront items
+    //   witems[16*k + w] = {patch or -1, entry begin, entry end, 0}
+    // and the per-block group size bgs[k] (wavefronts sharing a patch)
+    int4* witems = nullptr;
+    int* bgs = nullptr;
+    int n_sblocks = 0;
     int* entries = nullptr;  // sample indices, grouped by patch
-    int4* combine = nullptr; // {patch, first slot, slot count, 0} of split patches
-    int n_combine = 0;
-    int n_slots = 0;
+    // per entry k: weights of the patch's 4 x-points (ent_w[8k..8k+3]) and
+    // 4 y-points (ent_w[8k+4..8k+7]) for that entry's sample, 0 outside its
+    // clamped window
+    float* ent_w = nullptr;
+    // 2-D interp through LDS: samples binned by the kIT x kIT tile of their
+    // (re-based) window start; work items {tile, rec begin, rec end, 0} of at
+    // most kIChunk samples; per binned sample a record of 5 float4:
+    //   {i, ox, oy, 0 (ints)}, wx[0..7], wy[0..7]
+    // with (ox, oy) the window start inside the tile's staged region
+    // (origin tile*kIT - ipadl) and zero weights past the taps
+    int4* iitems = nullptr;
+    int n_iitems = 0;
+    float4* irec = nullptr;
+    int itx = 0, ity = 0, ipadl = 0;
     // 3-D spread (and B < 16): point-stationary tiles
     int ntx = 0, nty = 0, ntz = 0;
     int* tile_off = nullptr;  // ntiles + 1
@@ -52,7 +67,6 @@ struct Gnufft {
     float2* grids = nullptr;    // cap_batch * G
     float2* tbuf = nullptr;     // cap_batch * Nx*Nz*Gy
     float2* tbuf2 = nullptr;    // 3-D only: cap_batch * Nz*Gx*Gy
-    float2* partial = nullptr;  // ceil(cap_batch/64) * n_slots * 16 * 64
     float2* vals = nullptr;     // M * cap_batch per-sample spread values
     float2* img_tmp = nullptr;
     float2* dat_tmp = nullptr;

@@ -5,16 +5,21 @@
 // clamped window of gridding.cpp:67-73 carry weight 0 and their addresses
 // are clamped into the grid, so no tap wraps around.
 //
-// interp  -- one thread per (sample, coil); the L segment grids of the coil
-//            are read with the same 64 (2-D) / 512 (3-D) weights and summed
-//            with the AA weights (TimeSegmentation.cpp:279-295).  Coils are
-//            mapped to XCDs so that each XCD's L2 holds its coils' grids.
+// interp  -- 2-D with 2..16 segments: the grid region under a tile of
+//            samples is staged in LDS for all segments of a coil and each
+//            sample is one wavefront (lane = segment x y-tap).  Otherwise
+//            four lanes per (sample, coil) gather from global memory.  In
+//            both, the L segment grids of a coil are read with the same
+//            weights and summed with the AA weights (TimeSegmentation.cpp:
+//            279-295), and coils are mapped to XCDs so that each XCD's L2
+//            holds its coils' grids.
 // spread  -- first the value of every (sample, grid) pair is formed once
 //            (AA weight, CG residual), sample-major.
 //            2-D: one wavefront per 4x4 grid patch, lane = grid
 //            (coil x segment), accumulating in registers over the samples
-//            whose window meets the patch; heavy patches are split into
-//            chunks whose partial sums are added in a fixed order.  No
+//            whose window meets the patch (per-entry patch weights are
+//            precomputed); heavy patches are split into chunks of <= 64
+//            entries whose partial sums are added in a fixed order.  No
 //            atomics, no zero-fill, deterministic.
 //            3-D, >= 16 grids: one block per 8x8x4 tile, thread per point.
 //            3-D, < 16 grids: one thread per sample, hardware float atomics
@@ -113,6 +118,120 @@ __global__ void __launch_bounds__(256) k_interp(
     if (q == 0 && valid) out[c * out_cstride + i] = make_float2(tr, ti);
 }
 
+// ------------------------------------------------- 2-D interp through LDS --
+// Sum of a float over the 64 lanes of a wavefront with DPP row operations;
+// the total ends in lane 63.
+#define PG_DPP_ADD(v, CTRL, ROWMASK)                                                        \
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), (CTRL), (ROWMASK), \
+                                                    0xF, false))
+__device__ __forceinline__ float wave_sum_to_63(float v)
+{
+    PG_DPP_ADD(v, 0xB1, 0xF);   // quad_perm [1,0,3,2]
+    PG_DPP_ADD(v, 0x4E, 0xF);   // quad_perm [2,3,0,1]
+    PG_DPP_ADD(v, 0x141, 0xF);  // row_half_mirror
+    PG_DPP_ADD(v, 0x140, 0xF);  // row_mirror
+    PG_DPP_ADD(v, 0x142, 0xA);  // row_bcast:15 into rows 1 and 3
+    PG_DPP_ADD(v, 0x143, 0xC);  // row_bcast:31 into rows 2 and 3
+    return v;
+}
+#undef PG_DPP_ADD
+
+// One block per (work item, coil): the kIR x kIR grid region under the
+// item's tile is staged in LDS for all L segments of the coil, segment
+// fastest ([x][y][l], LP slots per point), together with the item's sample
+// records and their AA weights.  Each wavefront then takes two samples at a
+// time with lane = (segment l8, y tap j):
+//   acc = sum_x wx[x] * g_l(ox + x, oy + j),  value = sum_lanes acc * wy[j] * AA_l
+// (the 8 grid reads of a sample are each 512 contiguous bytes of LDS).
+// Blocks are ordered coil-fastest so that coil c stays on XCD c % 8.
+template <int LP>
+__global__ void __launch_bounds__(256) k_interp_lds(
+    const int4* __restrict__ items, const float4* __restrict__ rec, int ity, int padl,
+    const float2* __restrict__ grids, long long G, int Gx, int Gy, int nc, int L,
+    const float* __restrict__ AA, int M, float2* __restrict__ out, long long out_cstride)
+{
+    constexpr int R = pg::kIR;
+    constexpr int CH = pg::kIChunk;
+    __shared__ float2 tile[R * R * LP];
+    __shared__ float4 srec[CH * 5];
+    __shared__ float saa[CH * LP];
+    const int c = blockIdx.x % nc;
+    const int4 it = items[blockIdx.x / nc];
+    const int n = it.z - it.y;  // <= CH
+    const int tx = it.x / ity, ty = it.x - (it.x / ity) * ity;
+    const int rx0 = tx * pg::kIT - padl, ry0 = ty * pg::kIT - padl;
+    const float2* gc = grids + (long long)c * L * G;
+    for (int e = threadIdx.x; e < R * R * LP; e += 256) {
+        const int l = e % LP, r = e / LP;
+        const int xr = r / R, yr = r - (r / R) * R;
+        const int x = rx0 + xr, y = ry0 + yr;
+        const bool ok = l < L && x >= 0 && x < Gx && y >= 0 && y < Gy;
+        const int xc = min(max(x, 0), Gx - 1), yc = min(max(y, 0), Gy - 1);
+        const float2 v = gc[(long long)min(l, L - 1) * G + (long long)xc * Gy + yc];
+        tile[e] = ok ? v : make_float2(0.f, 0.f);
+    }
+    for (int e = threadIdx.x; e < n * 5; e += 256) srec[e] = rec[(long long)it.y * 5 + e];
+    __syncthreads();
+    for (int e = threadIdx.x; e < n * LP; e += 256) {
+        const int sl = e / LP, l = e % LP;
+        const int i = __float_as_int(srec[sl * 5].x);
+        const float a = AA[(long long)min(l, L - 1) * M + i];
+        saa[e] = (l < L) ? a : 0.f;
+    }
+    __syncthreads();
+    const int wv = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int l8 = lane & 7, j = lane >> 3;
+
+    // value of staged sample sl, summed into lane 63
+    auto one = [&](int sl, float& tr, float& ti) {
+        const float4* rp = srec + sl * 5;
+        const float4 h = rp[0];
+        const int ox = __float_as_int(h.y), oy = __float_as_int(h.z);
+        const float4 xa = rp[1], xb = rp[2];
+        const float wx[8] = {xa.x, xa.y, xa.z, xa.w, xb.x, xb.y, xb.z, xb.w};
+        const float wyj = reinterpret_cast<const float*>(rp + 3)[j];
+        tr = 0.f;
+        ti = 0.f;
+#pragma unroll
+        for (int g = 0; g < LP / 8; ++g) {
+            const int l = g * 8 + l8;
+            const float2* tp = tile + ((ox * R + oy + j) * LP + l);
+            float ar = 0.f, ai = 0.f;
+#pragma unroll
+            for (int x = 0; x < 8; ++x) {
+                const float2 v = tp[x * R * LP];
+                ar += wx[x] * v.x;
+                ai += wx[x] * v.y;
+            }
+            const float wa = wyj * saa[sl * LP + l];
+            tr += wa * ar;
+            ti += wa * ai;
+        }
+    };
+    int sl = wv;
+    for (; sl + 4 < n; sl += 8) {
+        float ar, ai, br, bi;
+        one(sl, ar, ai);
+        one(sl + 4, br, bi);
+        ar = wave_sum_to_63(ar);
+        ai = wave_sum_to_63(ai);
+        br = wave_sum_to_63(br);
+        bi = wave_sum_to_63(bi);
+        if (lane == 63) {
+            out[c * out_cstride + __float_as_int(srec[sl * 5].x)] = make_float2(ar, ai);
+            out[c * out_cstride + __float_as_int(srec[(sl + 4) * 5].x)] = make_float2(br, bi);
+        }
+    }
+    if (sl < n) {
+        float ar, ai;
+        one(sl, ar, ai);
+        ar = wave_sum_to_63(ar);
+        ai = wave_sum_to_63(ai);
+        if (lane == 63) out[c * out_cstride + __float_as_int(srec[sl * 5].x)] = make_float2(ar, ai);
+    }
+}
+
 // value of sample i in grid b: AA_l * (in_c[i] or W (y_c[i] - Ax_c[i])),
 // the residual formed in FP64 from the FP64 Ax of the CG
 __device__ __forceinline__ float2 spread_value(const float2* __restrict__ in, long long in_cstride,
@@ -148,115 +267,115 @@ __device__ __forceinline__ float2 spread_value(const float2* __restrict__ in, lo
 }
 
 // V[i*B + b] = value of sample i in grid b, formed once per spread so that the
-// spread kernels read one contiguous row of B values per sample
+// spread kernels read one contiguous row of B values per sample.  Block:
+// 4 samples x 64 lanes over b.
 __global__ void __launch_bounds__(256) k_spread_values(
     const float2* __restrict__ in, long long in_cstride, const float2* __restrict__ resid_y,
     const double2* __restrict__ ax64, const float* __restrict__ Wd, int L,
     const float* __restrict__ AA, int M, int B, float2* __restrict__ V)
 {
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (long long)M * B) return;
-    const int i = (int)(t / B), b = (int)(t - (long long)i * B);
-    V[t] = spread_value(in, in_cstride, resid_y, ax64, Wd, AA, L, M, b, i);
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= M) return;
+    for (int b = threadIdx.x & 63; b < B; b += 64)
+        V[(long long)i * B + b] = spread_value(in, in_cstride, resid_y, ax64, Wd, AA, L, M, b, i);
 }
 
-// ------------------------------------------------- 2-D patch spread (B>=16)
-__global__ void __launch_bounds__(64) k_spread_patch(
-    const int4* __restrict__ items, const int* __restrict__ entries, const int* __restrict__ lo,
-    const float* __restrict__ wpad, int M, int npy, const float2* __restrict__ V, int B,
-    float2* __restrict__ grids, long long G, int Gy, float2* __restrict__ partial, int n_slots)
+// ------------------------------------------------- 2-D patch spread --------
+// A block of 16 wavefronts; lane = grid (coil x segment).  Wavefront w runs
+// the work item witems[16 * block + w] = {patch (-1: none), entry begin,
+// entry end, 0}: it accumulates the 4x4 patch in registers over its entries.
+// Entry k carries its sample index and the weights of the patch's 4 x-points
+// and 4 y-points (zero outside the sample's clamped window), precomputed
+// with the trajectory; entries are staged 64 at a time one per lane and
+// broadcast with readlane, so the only load per entry is the sample's
+// contiguous row of grid values.  A patch too heavy for one wavefront is
+// split over gs = 2..16 consecutive wavefronts of a block (bgs[block] = gs,
+// the same for all groups of a block), whose sums are added through LDS in
+// a fixed tree order; the group's first wavefront writes the patch.  No
+// atomics, no zero-fill (every patch has an item), deterministic.
+constexpr int kSpreadWaves = 16;
+
+__global__ void __launch_bounds__(1024) k_spread_patch(
+    const int4* __restrict__ witems, const int* __restrict__ bgs, const int* __restrict__ entries,
+    const float4* __restrict__ ew, int npy, const float2* __restrict__ V, int B,
+    float2* __restrict__ grids, long long G, int Gy)
 {
-    const int4 it = items[blockIdx.x];
-    const int b = blockIdx.y * 64 + threadIdx.x;
+    __shared__ float2 red[8][16][64];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int4 it = witems[blockIdx.x * kSpreadWaves + wv];
+    const int gs = bgs[blockIdx.x];
+    const int b = blockIdx.y * 64 + lane;
     const bool active = b < B;
     const int bb = active ? b : B - 1;
+
+    float2 acc[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) acc[q] = make_float2(0.f, 0.f);
+    const float2* Vb = V + bb;
+    for (int s0 = it.y; s0 < it.z; s0 += 64) {
+        const int n = min(64, it.z - s0);
+        int ei = 0;
+        float4 ex = make_float4(0.f, 0.f, 0.f, 0.f), ey = ex;
+        if (lane < n) {
+            ei = entries[s0 + lane];
+            ex = ew[2 * (s0 + lane)];
+            ey = ew[2 * (s0 + lane) + 1];
+        }
+        constexpr int U = 8;  // entries in flight
+        for (int k0 = 0; k0 < n; k0 += U) {
+            float2 v[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int i = __builtin_amdgcn_readlane(ei, min(k0 + u, 63));
+                v[u] = (k0 + u < n) ? Vb[(long long)i * B] : make_float2(0.f, 0.f);
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int k = min(k0 + u, 63);  // entries past n have v = 0
+                float wx[4], wy[4];
+                wx[0] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(ex.x), k));
+                wx[1] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(ex.y), k));
+                wx[2] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(ex.z), k));
+                wx[3] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(ex.w), k));
+                wy[0] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(ey.x), k));
+                wy[1] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(ey.y), k));
+                wy[2] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(ey.z), k));
+                wy[3] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(ey.w), k));
+                float2 tq[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) tq[q] = make_float2(wy[q] * v[u].x, wy[q] * v[u].y);
+#pragma unroll
+                for (int a = 0; a < 4; ++a)
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        acc[a * 4 + q].x += wx[a] * tq[q].x;
+                        acc[a * 4 + q].y += wx[a] * tq[q].y;
+                    }
+            }
+        }
+    }
+    // groups of gs wavefronts: tree sum, wavefront w + r into w
+    for (int r = 1; r < gs; r *= 2) {
+        __syncthreads();
+        if ((wv & (2 * r - 1)) == r)
+#pragma unroll
+            for (int q = 0; q < 16; ++q) red[wv >> 1][q][lane] = acc[q];
+        __syncthreads();
+        if ((wv & (2 * r - 1)) == 0)
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                const float2 o = red[(wv + r) >> 1][q][lane];
+                acc[q].x += o.x;
+                acc[q].y += o.y;
+            }
+    }
+    if (!active || it.x < 0 || (wv & (gs - 1)) != 0) return;
     const int px0 = (it.x / npy) * pg::kPatch, py0 = (it.x % npy) * pg::kPatch;
-    float2 acc[pg::kPatch][pg::kPatch];
+    float2* g = grids + (long long)b * G + (long long)px0 * Gy + py0;
 #pragma unroll
     for (int a = 0; a < pg::kPatch; ++a)
 #pragma unroll
-        for (int q = 0; q < pg::kPatch; ++q) acc[a][q] = make_float2(0.f, 0.f);
-
-    // entries are processed in groups of EB: all loads of a group are issued
-    // before its FMAs, so several memory round trips overlap
-    constexpr int EB = 8;
-    for (int e0 = it.y; e0 < it.z; e0 += EB) {
-        const int n = min(EB, it.z - e0);
-        float wx[EB][pg::kPatch], wy[EB][pg::kPatch];
-        float2 v[EB];
-#pragma unroll
-        for (int k = 0; k < EB; ++k) {
-            const int i = entries[e0 + min(k, n - 1)];
-            const int dx = px0 - lo[i] + 3, dy = py0 - lo[M + i] + 3;  // in [0, 10]
-            const float* wxp = wpad + (long long)i * 16 + dx;
-            const float* wyp = wpad + ((long long)M + i) * 16 + dy;
-#pragma unroll
-            for (int a = 0; a < pg::kPatch; ++a) {
-                wx[k][a] = (k < n) ? wxp[a] : 0.f;
-                wy[k][a] = wyp[a];
-            }
-            v[k] = V[(long long)i * B + bb];
-        }
-#pragma unroll
-        for (int k = 0; k < EB; ++k)
-#pragma unroll
-            for (int a = 0; a < pg::kPatch; ++a)
-#pragma unroll
-                for (int q = 0; q < pg::kPatch; ++q) {
-                    const float ww = wx[k][a] * wy[k][q];
-                    acc[a][q].x += ww * v[k].x;
-                    acc[a][q].y += ww * v[k].y;
-                }
-    }
-    if (!active) return;
-    if (it.w < 0) {
-        float2* g = grids + (long long)b * G + (long long)px0 * Gy + py0;
-#pragma unroll
-        for (int a = 0; a < pg::kPatch; ++a)
-#pragma unroll
-            for (int q = 0; q < pg::kPatch; ++q) g[(long long)a * Gy + q] = acc[a][q];
-    } else {
-        float2* pp = partial + ((long long)blockIdx.y * n_slots + it.w) * 16 * 64 + threadIdx.x;
-#pragma unroll
-        for (int a = 0; a < pg::kPatch; ++a)
-#pragma unroll
-            for (int q = 0; q < pg::kPatch; ++q) pp[(a * pg::kPatch + q) * 64] = acc[a][q];
-    }
-}
-
-// sum the partial patches of split patches, in slot order; 4 wavefronts per
-// patch, each summing 4 of its 16 points for 64 grids
-__global__ void __launch_bounds__(256) k_spread_combine(const int4* __restrict__ comb, int npy,
-                                                        const float2* __restrict__ partial,
-                                                        int n_slots, int B,
-                                                        float2* __restrict__ grids, long long G,
-                                                        int Gy)
-{
-    const int4 cb = comb[blockIdx.x];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int b = blockIdx.y * 64 + lane;
-    if (b >= B) return;
-    const int px0 = (cb.x / npy) * pg::kPatch, py0 = (cb.x % npy) * pg::kPatch;
-    const float2* pp = partial + ((long long)blockIdx.y * n_slots + cb.y) * 16 * 64 + lane;
-    float2* g = grids + (long long)b * G + (long long)px0 * Gy + py0;
-    float2 s[4];
-#pragma unroll
-    for (int r = 0; r < 4; ++r) s[r] = pp[(wv * 4 + r) * 64];
-    for (int k = 1; k < cb.z; ++k) {
-        float2 v[4];
-#pragma unroll
-        for (int r = 0; r < 4; ++r) v[r] = pp[((long long)k * 16 + wv * 4 + r) * 64];
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            s[r].x += v[r].x;
-            s[r].y += v[r].y;
-        }
-    }
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        const int p = wv * 4 + r;
-        g[(long long)(p / pg::kPatch) * Gy + (p % pg::kPatch)] = s[r];
-    }
+        for (int q = 0; q < pg::kPatch; ++q) g[(long long)a * Gy + q] = acc[a * 4 + q];
 }
 
 // ------------------------------------------- 3-D tile spread (B >= 16) ----
@@ -373,6 +492,16 @@ void interp_dispatch(const pg::Gnufft& G, const float2* grids, int nc, int L, co
                      float2* out, long long out_cstride, hipStream_t s)
 {
     const pg::Geometry& g = G.geo;
+    if (!D3 && AA && L >= 2 && L <= 16 && G.n_iitems > 0) {
+        const dim3 grid((unsigned)G.n_iitems * nc), block(256);
+        if (L <= 8)
+            hipLaunchKernelGGL(k_interp_lds<8>, grid, block, 0, s, G.iitems, G.irec, G.ity, G.ipadl,
+                               grids, g.G(), g.Gx, g.Gy, nc, L, AA, G.M, out, out_cstride);
+        else
+            hipLaunchKernelGGL(k_interp_lds<16>, grid, block, 0, s, G.iitems, G.irec, G.ity,
+                               G.ipadl, grids, g.G(), g.Gx, g.Gy, nc, L, AA, G.M, out, out_cstride);
+        return;
+    }
     const int nsb = (G.M + 63) / 64;  // 64 samples (4 lanes each) per block
     const int xmap = (nc % 8 == 0) ? 1 : 0;
     hipLaunchKernelGGL((k_interp<NT, D3>), dim3(nsb * nc), dim3(256), 0, s, G.lo, G.w, G.M, g.Gx,
@@ -395,18 +524,14 @@ void spread_dispatch(const pg::Gnufft& G, const float2* in, long long in_cstride
         return;
     }
     // per-sample values of every grid, once
-    const long long nv = (long long)G.M * B;
-    hipLaunchKernelGGL(k_spread_values, dim3((unsigned)((nv + 255) / 256)), dim3(256), 0, s, in,
+    hipLaunchKernelGGL(k_spread_values, dim3((unsigned)((G.M + 3) / 4)), dim3(256), 0, s, in,
                        in_cstride, resid_y, ax64, Wd, L, AA, G.M, B, G.vals);
     if (!D3) {
         const int nch = (B + 63) / 64;
         const int npy = g.Gy / pg::kPatch;
-        hipLaunchKernelGGL(k_spread_patch, dim3(G.n_items, nch), dim3(64), 0, s, G.items, G.entries,
-                           G.lo, G.wpad, G.M, npy, G.vals, B, grids, g.G(), g.Gy, G.partial,
-                           G.n_slots);
-        if (G.n_combine > 0)
-            hipLaunchKernelGGL(k_spread_combine, dim3(G.n_combine, nch), dim3(256), 0, s, G.combine,
-                               npy, G.partial, G.n_slots, B, grids, g.G(), g.Gy);
+        hipLaunchKernelGGL(k_spread_patch, dim3(G.n_sblocks, nch), dim3(64 * kSpreadWaves), 0, s,
+                           G.witems, G.bgs, G.entries, (const float4*)G.ent_w, npy, G.vals, B,
+                           grids, g.G(), g.Gy);
         return;
     }
     constexpr int BC = 16;

@@ -7,13 +7,16 @@
 struct CgScal {
     double ywy;        // yi' W yi
     double oldinprod;  // previous real(cdot(ngrad, ngrad))
+    double newinprod;  // this iteration's (moved to oldinprod by the adir kernel)
     double gamma;
     double dAWAd, dAWr;
-    double2 step;      // complex step, as the reference's CxT1 step
+    double2 steps[4];  // line-search steps 0..3 (steps[0] = 0)
+    double2 step;      // the final complex step, as the reference's CxT1 step
     int ii;            // loop index
     int iter;          // completed updates
     int stop;          // 0 or PG_EXIT_*
     int first;         // ddir = ngrad on the first iteration
+    int pending;       // an update's residual norm still to be recorded
 };
 
 namespace pgv {
@@ -42,17 +45,31 @@ void pen_denom_pdot(const PenSpec& p, const float2* ddir, const float2* x, const
                     bool want_pdot, double* part, double* out2, const int* stop, hipStream_t s);
 void pen_value(const PenSpec& p, const float2* x, double* part, double* out, hipStream_t s);
 
+// partial-sum buffers of one PCG iteration (per-block FP64 partials)
+struct CgParts {
+    double* g;     // grad: 2 x ng
+    double* d;     // dir: nd
+    double* a;     // adir: 2 x na
+    double* l[3];  // penalty {denom, pdot} at line-search steps 0..2: 2 x nl each
+    double* u;     // update: nu
+    int ng, nd, na, nl, nu;
+};
+size_t cg_parts_doubles(long long N, long long M);
+CgParts cg_parts(double* base, long long N, long long M);
+
 void cg_init(const float2* y, const float* W, long long n, double* part, CgScal* sc, hipStream_t s);
 void cg_grad(const float2* P, int nc, long long N, const PenSpec* p, const float2* x,
-             float2* ngrad, double* part, CgScal* sc, hipStream_t s);
-void cg_dir(const float2* ngrad, float2* ddir, long long N, double* part, CgScal* sc,
+             float2* ngrad, const CgParts& cp, CgScal* sc, double* resid, hipStream_t s);
+void cg_dir(const float2* ngrad, float2* ddir, long long N, const CgParts& cp, CgScal* sc,
             hipStream_t s);
 void cg_adir(const float2* Adir, const float2* y, const double2* Ax, const float* W, long long n,
-             double* part, CgScal* sc, hipStream_t s);
-void cg_linesearch(const PenSpec* p, const float2* ddir, const float2* x, double* part, CgScal* sc,
-                   hipStream_t s);
+             const PenSpec* p, const float2* ddir, const float2* x, const CgParts& cp,
+             CgScal* sc, hipStream_t s);
+void cg_linesearch(const PenSpec* p, const float2* ddir, const float2* x, const CgParts& cp,
+                   CgScal* sc, hipStream_t s);
 void cg_update(double2* Ax, const float2* Adir, const float2* y, long long M, double2* x,
-               float2* x32, const float2* ddir, long long N, double* part, CgScal* sc,
-               double* resid, hipStream_t s);
+               float2* x32, const float2* ddir, long long N, bool have_pen, const CgParts& cp,
+               CgScal* sc, hipStream_t s);
+void cg_finish(const CgParts& cp, CgScal* sc, double* resid, hipStream_t s);
 
 }  // namespace pgv

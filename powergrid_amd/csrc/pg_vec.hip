@@ -3,8 +3,8 @@
 // Vectors are FP32 complex; every inner product, norm, step size and the
 // line-search scalars are FP64 and live on the device (struct CgScal), so a
 // whole iteration can be captured in a HIP graph.  Reductions are two-stage
-// (per-block partials, then one block summing them in a fixed order), so
-// results do not depend on scheduling.
+// (per-block partials, then summed in a fixed order by the kernel that
+// needs them), so results do not depend on scheduling.
 #include "pg_vec.h"
 
 namespace {
@@ -260,13 +260,96 @@ __global__ void k_reduce_parts(const double* part, int nparts, double* out)
 }
 
 // ------------------------------------------------------------------ CG ---
+// One PCG iteration (solve_pwls_pcg.hpp:83-162) is, after the adjoint:
+//   grad   ngrad = sum_c P_c - R.Gradient(x)           partials |g_data|^2, |ngrad|^2
+//   dir    zero-gradient exit, gamma; ddir             partial  <ddir, ngrad>
+//   (forward A ddir)
+//   adir   descent check; dAWAd, dAWr partials; penalty partials at step 0
+//   ls1    Newton step 1; penalty partials at step 1
+//   ls2    Newton step 2; penalty partials at step 2
+//   update Newton step 3; x += step ddir, Ax += step Adir; partial |y - Ax|^2
+// Each kernel first sums the previous kernel's partials itself, in every
+// block and in the same fixed order, so that every block takes the same
+// (deterministic) decision and no single-block control kernel is needed.
+// Block 0 records the scalars; a kernel never writes a field it reads.
 
-// gradient assembly: g_data = sum_c P[c] (the SENSE coil sum), partial
-// ||g_data||^2; ngrad = g_data - R.Gradient(x); partial ||ngrad||^2
-// (solve_pwls_pcg.hpp:83-95)
-__global__ void k_cg_grad(const float2* P, int nc, long long N, Pen p, int have_pen,
-                          const float2* x, float2* ngrad, double* part, const CgScal* sc)
+// sum of nparts x NV partials in a fixed order, result in every thread
+template <int NV>
+__device__ void all_sum_parts(const double* __restrict__ part, int nparts, double (&out)[NV])
 {
+    __shared__ double bc[NV];
+    double v[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) v[k] = 0;
+    for (int b = threadIdx.x; b < nparts; b += blockDim.x)
+#pragma unroll
+        for (int k = 0; k < NV; ++k) v[k] += part[(long long)b * NV + k];
+    block_sum<NV>(v);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int k = 0; k < NV; ++k) bc[k] = v[k];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NV; ++k) out[k] = bc[k];
+}
+
+// One Newton step of the line search (solve_pwls_pcg.hpp:129-152) from the
+// penalty sums {pdenom, pdot} at the current step; returns false and sets
+// stop on a bad denominator
+__device__ __forceinline__ bool newton_step(double dAWAd, double dAWr, double pdenom, double pdot,
+                                            double2& step, int& stop)
+{
+    const double dr = dAWAd + pdenom;  // Denom is real-valued here
+    if (!(fabs(dr) >= 1e-20) || fabs(dr) > 1e25) {
+        stop = PG_EXIT_BAD_DENOM;
+        return false;
+    }
+    const double2 s = step;
+    const double numr = -dAWr + s.x * dAWAd + pdot;
+    const double numi = s.y * dAWAd;
+    step = make_double2(s.x - numr / dr, s.y - numi / dr);
+    return true;
+}
+
+// penalty partials {beta * Denom, Re<ddir, Gradient(x + step ddir)>} of the
+// elements i = i0, i0 + stride, ... (block-summed into part[2 * slot])
+__device__ void pen_partials(const Pen& p, const float2* ddir, const float2* x, double2 step,
+                             long long i0, long long stride, double* part, int slot)
+{
+    double v[2] = {0, 0};
+    for (long long i = i0; i < p.n; i += stride) {
+        for (int k = 0; k < p.nd; ++k) {
+            const float2 cdir = cd_at(ddir, i, p.off[k]);
+            const float w = wpot(p, cd_xs(x, ddir, step, i, p.off[k]));
+            v[0] += (double)w * ((double)cdir.x * cdir.x + (double)cdir.y * cdir.y);
+        }
+        const float2 g = pen_grad_at(p, x, ddir, step, i);
+        const float2 d = ddir[i];
+        v[1] += (double)d.x * g.x + (double)d.y * g.y;
+    }
+    block_sum<2>(v);
+    if (threadIdx.x == 0) {
+        part[2 * slot] = v[0] * p.beta;
+        part[2 * slot + 1] = v[1];
+    }
+}
+
+// grad: also closes the previous iteration's bookkeeping (residual norm,
+// iteration count) from its update partials
+__global__ void k_cg_grad(const float2* P, int nc, long long N, Pen p, int have_pen,
+                          const float2* x, float2* ngrad, pgv::CgParts cp, CgScal* sc,
+                          double* resid)
+{
+    if (blockIdx.x == 0 && sc->pending) {
+        double u[1];
+        all_sum_parts<1>(cp.u, cp.nu, u);
+        if (threadIdx.x == 0) {
+            resid[sc->ii] = sqrt(u[0]);
+            sc->iter += 1;
+            sc->ii += 1;
+            sc->pending = 0;
+        }
+    }
     if (sc->stop) return;
     double v[2] = {0, 0};
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < N;
@@ -288,45 +371,34 @@ __global__ void k_cg_grad(const float2* P, int nc, long long N, Pen p, int have_
     }
     block_sum<2>(v);
     if (threadIdx.x == 0) {
-        part[2 * blockIdx.x] = v[0];
-        part[2 * blockIdx.x + 1] = v[1];
+        cp.g[2 * blockIdx.x] = v[0];
+        cp.g[2 * blockIdx.x + 1] = v[1];
     }
 }
 
-// control 1: zero-gradient exit, gamma (solve_pwls_pcg.hpp:88-107)
-__global__ void k_cg_ctl_grad(const double* part, int nparts, CgScal* sc)
-{
-    double v[2] = {0, 0};
-    for (int b = threadIdx.x; b < nparts; b += blockDim.x) {
-        v[0] += part[2 * b];
-        v[1] += part[2 * b + 1];
-    }
-    block_sum<2>(v);
-    if (threadIdx.x != 0 || sc->stop) return;
-    const double gnorm = sqrt(v[0]);
-    if (gnorm / sc->ywy < 1e-10) {  // norm_grad (:41-46); NaN compares false
-        sc->stop = PG_EXIT_ZERO_GRAD;
-        return;
-    }
-    const double newinprod = v[1];
-    if (sc->ii == 0) {
-        sc->gamma = 0;
-        sc->first = 1;
-    } else {
-        sc->first = 0;
-        sc->gamma = (fabs(sc->oldinprod) < 1e-10) ? 0.0 : newinprod / sc->oldinprod;
-    }
-    sc->oldinprod = newinprod;
-}
-
-// ddir = ngrad + gamma*ddir (ddir = ngrad on the first iteration); partial
+// dir: zero-gradient exit and gamma (solve_pwls_pcg.hpp:88-107), then
+// ddir = ngrad + gamma ddir (ngrad on the first iteration); partial
 // real(cdot(ddir, ngrad))
-__global__ void k_cg_dir(const float2* ngrad, float2* ddir, long long N, double* part,
-                         const CgScal* sc)
+__global__ void k_cg_dir(const float2* ngrad, float2* ddir, long long N, pgv::CgParts cp,
+                         CgScal* sc)
 {
     if (sc->stop) return;
-    const float gam = (float)sc->gamma;
-    const int first = sc->first;
+    double gs[2];
+    all_sum_parts<2>(cp.g, cp.ng, gs);
+    const double gnorm = sqrt(gs[0]);
+    if (gnorm / sc->ywy < 1e-10) {  // norm_grad (:41-46); NaN compares false
+        if (blockIdx.x == 0 && threadIdx.x == 0) sc->stop = PG_EXIT_ZERO_GRAD;
+        return;
+    }
+    const double newinprod = gs[1];
+    const int first = (sc->ii == 0);
+    const double gam = first ? 0.0 : ((fabs(sc->oldinprod) < 1e-10) ? 0.0 : newinprod / sc->oldinprod);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        sc->newinprod = newinprod;
+        sc->gamma = gam;
+        sc->first = first;
+    }
+    const float gamf = (float)gam;
     double v[1] = {0};
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < N;
          i += (long long)gridDim.x * blockDim.x) {
@@ -334,34 +406,40 @@ __global__ void k_cg_dir(const float2* ngrad, float2* ddir, long long N, double*
         float2 d = g;
         if (!first) {
             const float2 o = ddir[i];
-            d = make_float2(g.x + gam * o.x, g.y + gam * o.y);
+            d = make_float2(g.x + gamf * o.x, g.y + gamf * o.y);
         }
         ddir[i] = d;
         v[0] += (double)d.x * g.x + (double)d.y * g.y;
     }
     block_sum<1>(v);
-    if (threadIdx.x == 0) part[blockIdx.x] = v[0];
+    if (threadIdx.x == 0) cp.d[blockIdx.x] = v[0];
 }
 
-// control 2: descent check (solve_pwls_pcg.hpp:113-116)
-__global__ void k_cg_ctl_dir(const double* part, int nparts, CgScal* sc)
-{
-    double v[1] = {0};
-    for (int b = threadIdx.x; b < nparts; b += blockDim.x) v[0] += part[b];
-    block_sum<1>(v);
-    if (threadIdx.x != 0 || sc->stop) return;
-    if (v[0] < 0) sc->stop = PG_EXIT_NOT_DESCENT;
-}
-
-// partials of dAWAd = real(cdot(Adir, W Adir)) and
-// dAWr = real(Adir' (W (y - Ax)))   (solve_pwls_pcg.hpp:123-125)
-__global__ void k_cg_adir_dots(const float2* Adir, const float2* y, const double2* Ax,
-                               const float* W, long long n, double* part, const CgScal* sc)
+// adir: descent check (:113-116); blocks [0, na) form the partials of
+// dAWAd = real(cdot(Adir, W Adir)) and dAWr = real(Adir' W (y - Ax))
+// (:123-125); blocks [na, na + np) the penalty partials at step 0
+__global__ void k_cg_adir(const float2* Adir, const float2* y, const double2* Ax, const float* W,
+                          long long n, int na, Pen p, int have_pen, const float2* ddir,
+                          const float2* x, pgv::CgParts cp, CgScal* sc)
 {
     if (sc->stop) return;
+    double dd[1];
+    all_sum_parts<1>(cp.d, cp.nd, dd);
+    if (dd[0] < 0) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) sc->stop = PG_EXIT_NOT_DESCENT;
+        return;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) sc->oldinprod = sc->newinprod;
+    if ((int)blockIdx.x >= na) {
+        if (have_pen)
+            pen_partials(p, ddir, x, make_double2(0, 0),
+                         (long long)(blockIdx.x - na) * blockDim.x + threadIdx.x,
+                         (long long)(gridDim.x - na) * blockDim.x, cp.l[0], blockIdx.x - na);
+        return;
+    }
     double v[2] = {0, 0};
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (long long)gridDim.x * blockDim.x) {
+         i += (long long)na * blockDim.x) {
         const float2 a = Adir[i];
         const double w = W ? (double)W[i] : 1.0;
         const double rr = (double)y[i].x - Ax[i].x, ri = (double)y[i].y - Ax[i].y;
@@ -370,56 +448,87 @@ __global__ void k_cg_adir_dots(const float2* Adir, const float2* y, const double
     }
     block_sum<2>(v);
     if (threadIdx.x == 0) {
-        part[2 * blockIdx.x] = v[0];
-        part[2 * blockIdx.x + 1] = v[1];
+        cp.a[2 * blockIdx.x] = v[0];
+        cp.a[2 * blockIdx.x + 1] = v[1];
     }
 }
 
-__global__ void k_cg_ctl_adir(const double* part, int nparts, CgScal* sc)
-{
-    double v[2] = {0, 0};
-    for (int b = threadIdx.x; b < nparts; b += blockDim.x) {
-        v[0] += part[2 * b];
-        v[1] += part[2 * b + 1];
-    }
-    block_sum<2>(v);
-    if (threadIdx.x != 0 || sc->stop) return;
-    sc->dAWAd = v[0];
-    sc->dAWr = v[1];
-    sc->step = make_double2(0, 0);
-}
-
-// one Newton step of the line search (solve_pwls_pcg.hpp:129-152):
-// denom = dAWAd + pdenom; step -= (-dAWr + step*dAWAd + pdot) / denom
-__global__ void k_cg_ctl_ls(const double* part, int nparts, int have_pen, CgScal* sc)
-{
-    double v[2] = {0, 0};
-    if (have_pen)
-        for (int b = threadIdx.x; b < nparts; b += blockDim.x) {
-            v[0] += part[2 * b];
-            v[1] += part[2 * b + 1];
-        }
-    block_sum<2>(v);
-    if (threadIdx.x != 0 || sc->stop) return;
-    const double dr = sc->dAWAd + v[0];  // Denom is real-valued here
-    if (!(fabs(dr) >= 1e-20) || fabs(dr) > 1e25) {
-        sc->stop = PG_EXIT_BAD_DENOM;
-        return;
-    }
-    const double2 s = sc->step;
-    const double numr = -sc->dAWr + s.x * sc->dAWAd + v[1];
-    const double numi = s.y * sc->dAWAd;
-    sc->step = make_double2(s.x - numr / dr, s.y - numi / dr);
-}
-
-// Ax += step*Adir, x += step*ddir in FP64 (x32 keeps an FP32 copy of x for
-// the penalty stencils); partial ||y - Ax||^2 (solve_pwls_pcg.hpp:160-162)
-__global__ void k_cg_update(double2* Ax, const float2* Adir, const float2* y, long long M,
-                            double2* x, float2* x32, const float2* ddir, long long N, double* part,
-                            const CgScal* sc)
+// ls j (j = 1, 2): Newton step j from the penalty partials at steps[j-1],
+// then the penalty partials at steps[j]
+__global__ void k_cg_ls(int j, Pen p, const float2* ddir, const float2* x, pgv::CgParts cp,
+                        CgScal* sc)
 {
     if (sc->stop) return;
-    const double sr = sc->step.x, si = sc->step.y;
+    double dAWAd, dAWr;
+    if (j == 1) {
+        double a[2];
+        all_sum_parts<2>(cp.a, cp.na, a);
+        dAWAd = a[0];
+        dAWr = a[1];
+    } else {
+        dAWAd = sc->dAWAd;
+        dAWr = sc->dAWr;
+    }
+    double pl[2];
+    all_sum_parts<2>(cp.l[j - 1], cp.nl, pl);
+    double2 step = (j == 1) ? make_double2(0, 0) : sc->steps[j - 1];
+    int stop = 0;
+    const bool ok = newton_step(dAWAd, dAWr, pl[0], pl[1], step, stop);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (j == 1) {
+            sc->dAWAd = dAWAd;
+            sc->dAWr = dAWr;
+        }
+        if (ok) sc->steps[j] = step;
+        else sc->stop = stop;
+    }
+    if (!ok) return;
+    pen_partials(p, ddir, x, step, (long long)blockIdx.x * blockDim.x + threadIdx.x,
+                 (long long)gridDim.x * blockDim.x, cp.l[j], blockIdx.x);
+}
+
+// update: the last Newton step (all three without a penalty), then
+// Ax += step Adir, x += step ddir in FP64 (x32: the FP32 copy for the
+// penalty stencils); partial ||y - Ax||^2 (:155-162)
+__global__ void k_cg_update(double2* Ax, const float2* Adir, const float2* y, long long M,
+                            double2* x, float2* x32, const float2* ddir, long long N, int have_pen,
+                            pgv::CgParts cp, CgScal* sc)
+{
+    if (sc->stop) return;
+    double2 step;
+    int stop = 0;
+    bool ok = true;
+    double dAWAd = 0, dAWr = 0;
+    if (have_pen) {
+        double pl[2];
+        all_sum_parts<2>(cp.l[2], cp.nl, pl);
+        dAWAd = sc->dAWAd;
+        dAWr = sc->dAWr;
+        step = sc->steps[2];
+        ok = newton_step(dAWAd, dAWr, pl[0], pl[1], step, stop);
+    } else {
+        double a[2];
+        all_sum_parts<2>(cp.a, cp.na, a);
+        dAWAd = a[0];
+        dAWr = a[1];
+        step = make_double2(0, 0);
+        for (int j = 1; j <= 3 && ok; ++j) ok = newton_step(dAWAd, dAWr, 0.0, 0.0, step, stop);
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (!have_pen) {
+            sc->dAWAd = dAWAd;
+            sc->dAWr = dAWr;
+        }
+        if (ok) {
+            sc->steps[3] = step;
+            sc->step = step;
+            sc->pending = 1;
+        } else {
+            sc->stop = stop;
+        }
+    }
+    if (!ok) return;
+    const double sr = step.x, si = step.y;
     double v[1] = {0};
     const long long n = M > N ? M : N;
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
@@ -443,19 +552,20 @@ __global__ void k_cg_update(double2* Ax, const float2* Adir, const float2* y, lo
         }
     }
     block_sum<1>(v);
-    if (threadIdx.x == 0) part[blockIdx.x] = v[0];
+    if (threadIdx.x == 0) cp.u[blockIdx.x] = v[0];
 }
 
-__global__ void k_cg_ctl_update(const double* part, int nparts, CgScal* sc, double* resid)
+// closes the bookkeeping of the last completed iteration
+__global__ void k_cg_finish(pgv::CgParts cp, CgScal* sc, double* resid)
 {
-    double v[1] = {0};
-    for (int b = threadIdx.x; b < nparts; b += blockDim.x) v[0] += part[b];
-    block_sum<1>(v);
-    if (threadIdx.x != 0) return;
-    if (!sc->stop) {
-        resid[sc->ii] = sqrt(v[0]);
+    if (!sc->pending) return;
+    double u[1];
+    all_sum_parts<1>(cp.u, cp.nu, u);
+    if (threadIdx.x == 0) {
+        resid[sc->ii] = sqrt(u[0]);
         sc->iter += 1;
         sc->ii += 1;
+        sc->pending = 0;
     }
 }
 
@@ -558,6 +668,44 @@ void pen_value(const PenSpec& ps, const float2* x, double* part, double* out, hi
     hipLaunchKernelGGL(k_reduce_parts<1>, dim3(1), dim3(kThreads), 0, s, part, (int)nb, out);
 }
 
+CgParts cg_parts_alloc_sizes(long long N, long long M, int* total)
+{
+    CgParts c{};
+    c.ng = (int)nred(N);
+    c.nd = (int)nred(N);
+    c.na = (int)nred(M);
+    c.nl = (int)nred(N);
+    c.nu = (int)nred(M > N ? M : N);
+    *total = 2 * c.ng + c.nd + 2 * c.na + 3 * 2 * c.nl + c.nu;
+    return c;
+}
+
+CgParts cg_parts(double* base, long long N, long long M)
+{
+    int total = 0;
+    CgParts c = cg_parts_alloc_sizes(N, M, &total);
+    double* q = base;
+    c.g = q;
+    q += 2 * c.ng;
+    c.d = q;
+    q += c.nd;
+    c.a = q;
+    q += 2 * c.na;
+    for (int j = 0; j < 3; ++j) {
+        c.l[j] = q;
+        q += 2 * c.nl;
+    }
+    c.u = q;
+    return c;
+}
+
+size_t cg_parts_doubles(long long N, long long M)
+{
+    int total = 0;
+    cg_parts_alloc_sizes(N, M, &total);
+    return (size_t)total;
+}
+
 void cg_init(const float2* y, const float* W, long long n, double* part, CgScal* sc, hipStream_t s)
 {
     const unsigned nb = nred(n);
@@ -566,53 +714,49 @@ void cg_init(const float2* y, const float* W, long long n, double* part, CgScal*
 }
 
 void cg_grad(const float2* P, int nc, long long N, const PenSpec* ps, const float2* x,
-             float2* ngrad, double* part, CgScal* sc, hipStream_t s)
+             float2* ngrad, const CgParts& cp, CgScal* sc, double* resid, hipStream_t s)
 {
-    const unsigned nb = nred(N);
     Pen p = ps ? to_pen(*ps) : Pen{};
-    hipLaunchKernelGGL(k_cg_grad, dim3(nb), dim3(kThreads), 0, s, P, nc, N, p, ps ? 1 : 0, x, ngrad,
-                       part, sc);
-    hipLaunchKernelGGL(k_cg_ctl_grad, dim3(1), dim3(kThreads), 0, s, part, (int)nb, sc);
+    hipLaunchKernelGGL(k_cg_grad, dim3(cp.ng), dim3(kThreads), 0, s, P, nc, N, p, ps ? 1 : 0, x,
+                       ngrad, cp, sc, resid);
 }
 
-void cg_dir(const float2* ngrad, float2* ddir, long long N, double* part, CgScal* sc,
+void cg_dir(const float2* ngrad, float2* ddir, long long N, const CgParts& cp, CgScal* sc,
             hipStream_t s)
 {
-    const unsigned nb = nred(N);
-    hipLaunchKernelGGL(k_cg_dir, dim3(nb), dim3(kThreads), 0, s, ngrad, ddir, N, part, sc);
-    hipLaunchKernelGGL(k_cg_ctl_dir, dim3(1), dim3(kThreads), 0, s, part, (int)nb, sc);
+    hipLaunchKernelGGL(k_cg_dir, dim3(cp.nd), dim3(kThreads), 0, s, ngrad, ddir, N, cp, sc);
 }
 
 void cg_adir(const float2* Adir, const float2* y, const double2* Ax, const float* W, long long n,
-             double* part, CgScal* sc, hipStream_t s)
+             const PenSpec* ps, const float2* ddir, const float2* x, const CgParts& cp,
+             CgScal* sc, hipStream_t s)
 {
-    const unsigned nb = nred(n);
-    hipLaunchKernelGGL(k_cg_adir_dots, dim3(nb), dim3(kThreads), 0, s, Adir, y, Ax, W, n, part, sc);
-    hipLaunchKernelGGL(k_cg_ctl_adir, dim3(1), dim3(kThreads), 0, s, part, (int)nb, sc);
+    Pen p = ps ? to_pen(*ps) : Pen{};
+    const int np = ps ? cp.nl : 0;
+    hipLaunchKernelGGL(k_cg_adir, dim3(cp.na + np), dim3(kThreads), 0, s, Adir, y, Ax, W, n, cp.na,
+                       p, ps ? 1 : 0, ddir, x, cp, sc);
 }
 
-void cg_linesearch(const PenSpec* ps, const float2* ddir, const float2* x, double* part,
+void cg_linesearch(const PenSpec* ps, const float2* ddir, const float2* x, const CgParts& cp,
                    CgScal* sc, hipStream_t s)
 {
-    for (int j = 0; j < 3; ++j) {
-        unsigned nb = 1;
-        if (ps) {
-            nb = nred(ps->n);
-            hipLaunchKernelGGL(k_pen_denom_pdot, dim3(nb), dim3(kThreads), 0, s, to_pen(*ps), ddir, x,
-                               (const double2*)&sc->step, 1, part, &sc->stop);
-        }
-        hipLaunchKernelGGL(k_cg_ctl_ls, dim3(1), dim3(kThreads), 0, s, part, (int)nb, ps ? 1 : 0, sc);
-    }
+    if (!ps) return;  // without a penalty the update kernel takes all three steps
+    for (int j = 1; j <= 2; ++j)
+        hipLaunchKernelGGL(k_cg_ls, dim3(cp.nl), dim3(kThreads), 0, s, j, to_pen(*ps), ddir, x, cp,
+                           sc);
 }
 
 void cg_update(double2* Ax, const float2* Adir, const float2* y, long long M, double2* x,
-               float2* x32, const float2* ddir, long long N, double* part, CgScal* sc,
-               double* resid, hipStream_t s)
+               float2* x32, const float2* ddir, long long N, bool have_pen, const CgParts& cp,
+               CgScal* sc, hipStream_t s)
 {
-    const unsigned nb = nred(M > N ? M : N);
-    hipLaunchKernelGGL(k_cg_update, dim3(nb), dim3(kThreads), 0, s, Ax, Adir, y, M, x, x32, ddir, N,
-                       part, sc);
-    hipLaunchKernelGGL(k_cg_ctl_update, dim3(1), dim3(kThreads), 0, s, part, (int)nb, sc, resid);
+    hipLaunchKernelGGL(k_cg_update, dim3(cp.nu), dim3(kThreads), 0, s, Ax, Adir, y, M, x, x32, ddir,
+                       N, (int)have_pen, cp, sc);
+}
+
+void cg_finish(const CgParts& cp, CgScal* sc, double* resid, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_cg_finish, dim3(1), dim3(kThreads), 0, s, cp, sc, resid);
 }
 
 }  // namespace pgv

@@ -47,7 +47,8 @@ def traj(N, nshot, ns, Nz=1, edge=True):
     return kx.astype(np.float32), ky.astype(np.float32), kz.astype(np.float32)
 
 
-@pytest.mark.parametrize("N,W", [(32, 4.0), (96, 4.0), (128, 4.0), (64, 3.0)])
+@pytest.mark.parametrize("N,W", [(32, 4.0), (96, 4.0), (128, 4.0), (64, 3.0), (256, 4.0),
+                                 (512, 4.0)])
 def test_gnufft2d_matches_oracle(orc, N, W):
     print(f"[gnufft2d N={N} W={W}]", flush=True)
     kx, ky, kz = traj(N, 8, 1024)
@@ -60,9 +61,9 @@ def test_gnufft2d_matches_oracle(orc, N, W):
     check(Gg / yo, Gf / yo, Gd / yo)
 
 
-def test_gnufft3d_matches_oracle(orc):
-    print("[gnufft3d 16x16x8]", flush=True)
-    N, Nz = 16, 8
+@pytest.mark.parametrize("N,Nz", [(16, 8), (16, 32)])
+def test_gnufft3d_matches_oracle(orc, N, Nz):
+    print(f"[gnufft3d {N}x{N}x{Nz}]", flush=True)
     kx, ky, kz = traj(N, 4, 256, Nz=Nz)
     M = len(kx)
     x = S.ellipse_phantom(N, Nz, seed=2).astype(np.complex64)
