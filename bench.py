@@ -124,6 +124,10 @@ def main():
     ap.add_argument("--no-oracle", action="store_true", help="skip the FP64 oracle recon")
     ap.add_argument("--plain-steps", type=int, default=2)
     ap.add_argument("--beta", type=float, default=None, help="QuadPenalty weight override")
+    ap.add_argument("--counters", type=int, default=0, metavar="REPS",
+                    help="hardware-counter pass (for rocprofv3 --pmc): one 256 MiB device-to-"
+                         "device copy (byte-counter calibration), then REPS forward+adjoint "
+                         "pairs of the batched operator; no timing, no JSON line")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path "
@@ -144,6 +148,19 @@ def main():
 
     # ---------------- batched path: fused kernels + graph-captured PCG -----
     G, T, Sg, R = build_ops(P, prob, P.MODE_BATCHED)
+    if args.counters:
+        nbytes = 1 << 28
+        src = P.DeviceArray(nbytes // 8)
+        dst = P.DeviceArray(nbytes // 8)
+        dst.copy_from(src, stream=stream)
+        stream.synchronize()
+        xd = P.DeviceArray.from_host(prob.x_true.astype(np.complex64))
+        for _ in range(args.counters):
+            ad = Sg.forward(xd, stream=stream)
+            Sg.adjoint(ad, stream=stream)
+        stream.synchronize()
+        log(f"[bench] counter pass: copy {nbytes} B, {args.counters} forward+adjoint pairs")
+        return
     for _ in range(args.warmup):
         x, info = P.reconSolve(yd, Sg, R, niter=niter, mode=P.CG_GRAPH, stream=stream)
     stream.synchronize()

@@ -409,9 +409,8 @@ static void gnufft_setup(Gnufft& G, size_t M, float os, size_t Nx, size_t Ny, si
             for (int k = tcnt[t]; k < tcnt[t + 1]; k += kIChunk)
                 its.push_back(make_int4(t, k, std::min(k + kIChunk, tcnt[t + 1]), 0));
         }
-        // heaviest first
-        std::stable_sort(its.begin(), its.end(),
-                         [](const int4& a, const int4& b) { return (a.z - a.y) > (b.z - b.y); });
+        // items stay in tile raster order: neighbouring tiles, whose staged
+        // regions overlap, run at about the same time and share L2 lines
         G.n_iitems = (int)its.size();
         G.iitems = dev_copy(its.data(), its.size());
         G.irec = dev_copy(rec.data(), rec.size());
@@ -501,17 +500,17 @@ static void gnufft_setup(Gnufft& G, size_t M, float os, size_t Nx, size_t Ny, si
                 run += pweight(q);
             }
         }
-        // per group: blocks of 16 wavefront items; a patch of n entries takes
-        // gs = 1 (n <= 64) or the power of two >= n/64 (at most 16) wavefronts
+        // per group: blocks of 8 wavefront items; a patch of n entries takes
+        // gs = 1 (n <= 64) or the power of two >= n/64 (at most 8) wavefronts
         struct SBlock {
             int gs;
             int cost;
-            int4 w[16];
+            int4 w[8];
         };
-        const int kW = 16;
+        const int kW = 8;
         std::vector<std::vector<SBlock>> gblk(kXcd);
         for (int gi = 0; gi < kXcd; ++gi) {
-            std::vector<std::vector<int>> byg(5);  // gs = 1, 2, 4, 8, 16
+            std::vector<std::vector<int>> byg(4);  // gs = 1, 2, 4, 8
             for (int q : grp[gi]) {
                 const long long n = cnt[q + 1] - cnt[q];
                 int gs = 1, lg = 0;
@@ -521,7 +520,7 @@ static void gnufft_setup(Gnufft& G, size_t M, float os, size_t Nx, size_t Ny, si
                 }
                 byg[lg].push_back(q);
             }
-            for (int lg = 0; lg < 5; ++lg) {
+            for (int lg = 0; lg < 4; ++lg) {
                 const int gs = 1 << lg, per = kW / gs;
                 auto& v = byg[lg];
                 std::stable_sort(v.begin(), v.end(), [&](int u, int w) {

@@ -413,7 +413,8 @@ __global__ void __launch_bounds__(64 * post_waves<R1>()) k_adj_post_reg(
     constexpr int PITCH = R1 * (R2 + 1);
     constexpr int NW = post_waves<R1>();
     __shared__ float2 lds[NW * LPW * PITCH];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    // wavefront index made scalar: the segment loop below is uniform
+    const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int t = lane % R2, jl = lane / R2;
     const int c = blockIdx.y;
     const int ncol = Nx * Nz;

@@ -139,13 +139,15 @@ __device__ __forceinline__ float wave_sum_to_63(float v)
 // One block per (work item, coil): the kIR x kIR grid region under the
 // item's tile is staged in LDS for all L segments of the coil, segment
 // fastest ([x][y][l], LP slots per point), together with the item's sample
-// records and their AA weights.  Each wavefront then takes two samples at a
-// time with lane = (segment l8, y tap j):
+// records and their AA weights.  Each of the 8 wavefronts then takes two
+// samples at a time with lane = (segment l8, y tap j):
 //   acc = sum_x wx[x] * g_l(ox + x, oy + j),  value = sum_lanes acc * wy[j] * AA_l
 // (the 8 grid reads of a sample are each 512 contiguous bytes of LDS).
 // Blocks are ordered coil-fastest so that coil c stays on XCD c % 8.
+constexpr int kInterpWaves = 8;
+
 template <int LP>
-__global__ void __launch_bounds__(256) k_interp_lds(
+__global__ void __launch_bounds__(64 * kInterpWaves) k_interp_lds(
     const int4* __restrict__ items, const float4* __restrict__ rec, int ity, int padl,
     const float2* __restrict__ grids, long long G, int Gx, int Gy, int nc, int L,
     const float* __restrict__ AA, int M, float2* __restrict__ out, long long out_cstride)
@@ -161,7 +163,7 @@ __global__ void __launch_bounds__(256) k_interp_lds(
     const int tx = it.x / ity, ty = it.x - (it.x / ity) * ity;
     const int rx0 = tx * pg::kIT - padl, ry0 = ty * pg::kIT - padl;
     const float2* gc = grids + (long long)c * L * G;
-    for (int e = threadIdx.x; e < R * R * LP; e += 256) {
+    for (int e = threadIdx.x; e < R * R * LP; e += blockDim.x) {
         const int l = e % LP, r = e / LP;
         const int xr = r / R, yr = r - (r / R) * R;
         const int x = rx0 + xr, y = ry0 + yr;
@@ -170,16 +172,16 @@ __global__ void __launch_bounds__(256) k_interp_lds(
         const float2 v = gc[(long long)min(l, L - 1) * G + (long long)xc * Gy + yc];
         tile[e] = ok ? v : make_float2(0.f, 0.f);
     }
-    for (int e = threadIdx.x; e < n * 5; e += 256) srec[e] = rec[(long long)it.y * 5 + e];
+    for (int e = threadIdx.x; e < n * 5; e += blockDim.x) srec[e] = rec[(long long)it.y * 5 + e];
     __syncthreads();
-    for (int e = threadIdx.x; e < n * LP; e += 256) {
+    for (int e = threadIdx.x; e < n * LP; e += blockDim.x) {
         const int sl = e / LP, l = e % LP;
         const int i = __float_as_int(srec[sl * 5].x);
         const float a = AA[(long long)min(l, L - 1) * M + i];
         saa[e] = (l < L) ? a : 0.f;
     }
     __syncthreads();
-    const int wv = threadIdx.x >> 6;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // scalar sample loop
     const int lane = threadIdx.x & 63;
     const int l8 = lane & 7, j = lane >> 3;
 
@@ -209,18 +211,19 @@ __global__ void __launch_bounds__(256) k_interp_lds(
             ti += wa * ai;
         }
     };
+    constexpr int NW = kInterpWaves;
     int sl = wv;
-    for (; sl + 4 < n; sl += 8) {
+    for (; sl + NW < n; sl += 2 * NW) {
         float ar, ai, br, bi;
         one(sl, ar, ai);
-        one(sl + 4, br, bi);
+        one(sl + NW, br, bi);
         ar = wave_sum_to_63(ar);
         ai = wave_sum_to_63(ai);
         br = wave_sum_to_63(br);
         bi = wave_sum_to_63(bi);
         if (lane == 63) {
             out[c * out_cstride + __float_as_int(srec[sl * 5].x)] = make_float2(ar, ai);
-            out[c * out_cstride + __float_as_int(srec[(sl + 4) * 5].x)] = make_float2(br, bi);
+            out[c * out_cstride + __float_as_int(srec[(sl + NW) * 5].x)] = make_float2(br, bi);
         }
     }
     if (sl < n) {
@@ -281,27 +284,29 @@ __global__ void __launch_bounds__(256) k_spread_values(
 }
 
 // ------------------------------------------------- 2-D patch spread --------
-// A block of 16 wavefronts; lane = grid (coil x segment).  Wavefront w runs
-// the work item witems[16 * block + w] = {patch (-1: none), entry begin,
+// A block of 8 wavefronts; lane = grid (coil x segment).  Wavefront w runs
+// the work item witems[8 * block + w] = {patch (-1: none), entry begin,
 // entry end, 0}: it accumulates the 4x4 patch in registers over its entries.
 // Entry k carries its sample index and the weights of the patch's 4 x-points
 // and 4 y-points (zero outside the sample's clamped window), precomputed
 // with the trajectory; entries are staged 64 at a time one per lane and
 // broadcast with readlane, so the only load per entry is the sample's
-// contiguous row of grid values.  A patch too heavy for one wavefront is
-// split over gs = 2..16 consecutive wavefronts of a block (bgs[block] = gs,
+// contiguous row of grid values (the next 8 rows are loaded while the
+// current 8 are accumulated).  A patch too heavy for one wavefront is
+// split over gs = 2..8 consecutive wavefronts of a block (bgs[block] = gs,
 // the same for all groups of a block), whose sums are added through LDS in
 // a fixed tree order; the group's first wavefront writes the patch.  No
 // atomics, no zero-fill (every patch has an item), deterministic.
-constexpr int kSpreadWaves = 16;
+constexpr int kSpreadWaves = 8;
 
-__global__ void __launch_bounds__(1024) k_spread_patch(
+__global__ void __launch_bounds__(64 * kSpreadWaves) k_spread_patch(
     const int4* __restrict__ witems, const int* __restrict__ bgs, const int* __restrict__ entries,
     const float4* __restrict__ ew, int npy, const float2* __restrict__ V, int B,
     float2* __restrict__ grids, long long G, int Gy)
 {
-    __shared__ float2 red[8][16][64];
-    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    __shared__ float2 red[kSpreadWaves / 2][16][64];
+    // wave-uniform (scalar) work item: its loop bounds live in SGPRs
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int4 it = witems[blockIdx.x * kSpreadWaves + wv];
     const int gs = bgs[blockIdx.x];
     const int b = blockIdx.y * 64 + lane;
@@ -312,6 +317,7 @@ __global__ void __launch_bounds__(1024) k_spread_patch(
 #pragma unroll
     for (int q = 0; q < 16; ++q) acc[q] = make_float2(0.f, 0.f);
     const float2* Vb = V + bb;
+    constexpr int U = 8;  // entries per group
     for (int s0 = it.y; s0 < it.z; s0 += 64) {
         const int n = min(64, it.z - s0);
         int ei = 0;
@@ -321,16 +327,24 @@ __global__ void __launch_bounds__(1024) k_spread_patch(
             ex = ew[2 * (s0 + lane)];
             ey = ew[2 * (s0 + lane) + 1];
         }
-        constexpr int U = 8;  // entries in flight
-        for (int k0 = 0; k0 < n; k0 += U) {
-            float2 v[U];
+        // rows of group k0; entries past n load the last entry's row, which
+        // is zeroed with a bit mask where it is used (no branch around the
+        // loads, so that the next group's loads stay in flight while this
+        // group is summed)
+        auto load_group = [&](int k0, float2 (&v)[U]) {
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                const int i = __builtin_amdgcn_readlane(ei, min(k0 + u, 63));
-                v[u] = (k0 + u < n) ? Vb[(long long)i * B] : make_float2(0.f, 0.f);
+                const int i = __builtin_amdgcn_readlane(ei, min(k0 + u, n - 1));
+                v[u] = Vb[(long long)i * B];
             }
+        };
+        // sum group k0 (rows in v) into the patch
+        auto sum_group = [&](int k0, float2 (&v)[U]) {
 #pragma unroll
             for (int u = 0; u < U; ++u) {
+                const int m = (k0 + u < n) ? -1 : 0;
+                const float2 vu = make_float2(__int_as_float(__float_as_int(v[u].x) & m),
+                                              __int_as_float(__float_as_int(v[u].y) & m));
                 const int k = min(k0 + u, 63);  // entries past n have v = 0
                 float wx[4], wy[4];
                 wx[0] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(ex.x), k));
@@ -343,7 +357,7 @@ __global__ void __launch_bounds__(1024) k_spread_patch(
                 wy[3] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(ey.w), k));
                 float2 tq[4];
 #pragma unroll
-                for (int q = 0; q < 4; ++q) tq[q] = make_float2(wy[q] * v[u].x, wy[q] * v[u].y);
+                for (int q = 0; q < 4; ++q) tq[q] = make_float2(wy[q] * vu.x, wy[q] * vu.y);
 #pragma unroll
                 for (int a = 0; a < 4; ++a)
 #pragma unroll
@@ -352,6 +366,20 @@ __global__ void __launch_bounds__(1024) k_spread_patch(
                         acc[a * 4 + q].y += wx[a] * tq[q].y;
                     }
             }
+        };
+        // two row buffers in turn: one group in flight while the other is summed
+        float2 va[U], vb[U];
+        load_group(0, va);
+        // (the scheduling barriers keep each group's loads ahead of the sums;
+        // the second group is summed unconditionally, masked past n, so that
+        // its loads are not sunk next to their use)
+        for (int k0 = 0; k0 < n; k0 += 2 * U) {
+            load_group(k0 + U, vb);
+            __builtin_amdgcn_sched_barrier(0);
+            sum_group(k0, va);
+            load_group(k0 + 2 * U, va);
+            __builtin_amdgcn_sched_barrier(0);
+            sum_group(k0 + U, vb);
         }
     }
     // groups of gs wavefronts: tree sum, wavefront w + r into w
@@ -493,7 +521,7 @@ void interp_dispatch(const pg::Gnufft& G, const float2* grids, int nc, int L, co
 {
     const pg::Geometry& g = G.geo;
     if (!D3 && AA && L >= 2 && L <= 16 && G.n_iitems > 0) {
-        const dim3 grid((unsigned)G.n_iitems * nc), block(256);
+        const dim3 grid((unsigned)G.n_iitems * nc), block(64 * kInterpWaves);
         if (L <= 8)
             hipLaunchKernelGGL(k_interp_lds<8>, grid, block, 0, s, G.iitems, G.irec, G.ity, G.ipadl,
                                grids, g.G(), g.Gx, g.Gy, nc, L, AA, G.M, out, out_cstride);
