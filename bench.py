@@ -31,6 +31,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3  # MI355X published FP32 vector peak
 
 
 def log(*a):
@@ -90,11 +91,20 @@ def cpu_baseline(prob, y, budget_s=20.0):
         if time.perf_counter() - t0 > budget_s / 2 or n >= 6:
             break
     dt = time.perf_counter() - t0
+    # the same loops on one core (the reference's serial GCC build): one solve
+    # of 2 iterations
+    O.set_threads(1)
+    t1 = time.perf_counter()
+    O.reconSolve(y, Sg, R, prob.Nx, prob.Ny, prob.Nz, 2)
+    serial = 2 / (time.perf_counter() - t1)
+    O.set_threads(threads)
     # the timed solves include one A*x0 forward each; report iterations/s of
     # the whole solve call (reference loop structure)
     return dict(value=n / dt, unit="CG iterations/s", cores=threads, kind="port",
                 sample=f"{n} PCG iterations in {n // 2} solves of 2 iterations "
-                       f"(same inputs, FP32, {threads} OpenMP threads)")
+                       f"(same inputs, FP32, {threads} OpenMP threads)",
+                serial_value=serial,
+                serial_sample="one solve of 2 PCG iterations on 1 thread")
 
 
 def fp64_oracle_recon(prob, y, niter):
@@ -111,6 +121,68 @@ def fp64_oracle_recon(prob, y, niter):
     return x
 
 
+def copy_bandwidth(P, stream, nbytes=1 << 30, reps=5):
+    """device-to-device copy (read + write) in GB/s: the measured HBM ceiling"""
+    src = P.DeviceArray(nbytes // 8)
+    dst = P.DeviceArray(nbytes // 8)
+    dst.copy_from(src, stream=stream)
+    stream.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        dst.copy_from(src, stream=stream)
+    stream.synchronize()
+    return 2 * nbytes * reps / (time.perf_counter() - t0) / 1e9
+
+
+def gdft_roofline(P, prob, stream, reps=3):
+    """The exact field-corrected DFT (Gdft, FP32 phase and trig) through SENSE
+    on the workload: flops = 16 per (sample, voxel) pair per coil and
+    direction, against the FP32 vector peak."""
+    M, N, nc = prob.M, prob.N, prob.nc
+    D = P.Gdft(M, N, prob.kx, prob.ky, prob.kz, prob.ix, prob.iy, prob.iz, prob.fm, prob.t,
+               precision=0)
+    Sd = P.SENSE(D, prob.sens, M, N, nc)
+    xd = P.DeviceArray.from_host(prob.x_true.astype(np.complex64))
+    yd = Sd.forward(xd, stream=stream)
+    Sd.adjoint(yd, stream=stream)
+    stream.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        yd = Sd.forward(xd, stream=stream)
+        Sd.adjoint(yd, stream=stream)
+    stream.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    pairs = 2.0 * M * N * nc
+    tf = 16.0 * pairs / dt / 1e12
+    return dict(ms_per_pair_of_ops=round(1e3 * dt, 3), pairs=pairs, achieved=round(tf, 2),
+                peak=FP32_PEAK_TFLOPS, unit="TFLOP/s", frac=round(tf / FP32_PEAK_TFLOPS, 4),
+                bound="fp32 valu")
+
+
+def c3_rate(P, S, iters, stream):
+    """PCG iterations/s of the 3-D stack-of-spirals workload (128x128x32,
+    32 x 8 x 2048 samples, 32 coils, L=12, TVPenalty), data = the operator's
+    own forward of the phantom plus noise (timing only)."""
+    prob = S.config_sos3d(seed=0)
+    G = P.Gnufft(prob.M, 2.0, prob.Nx, prob.Ny, prob.Nz, prob.kx, prob.ky, prob.kz)
+    T = P.TimeSegmentation(G, prob.fm, prob.t, prob.M, prob.N, prob.L)
+    Sg = P.SENSE(T, prob.sens, prob.M, prob.N, prob.nc)
+    R = P.TVPenalty(prob.Nx, prob.Ny, prob.Nz, prob.beta, prob.delta)
+    xd = P.DeviceArray.from_host(prob.x_true.astype(np.complex64))
+    yd = Sg.forward(xd, stream=stream)
+    y = S.add_noise(yd.to_host().astype(np.complex128), 40.0, seed=11).astype(np.complex64)
+    yd = P.DeviceArray.from_host(y)
+    P.reconSolve(yd, Sg, R, niter=1, mode=P.CG_GRAPH, stream=stream)
+    stream.synchronize()
+    t0 = time.perf_counter()
+    _, info = P.reconSolve(yd, Sg, R, niter=iters, mode=P.CG_GRAPH, stream=stream)
+    stream.synchronize()
+    dt = time.perf_counter() - t0
+    return dict(workload="3D SENSE 128x128x32, 32x8x2048 stack of spirals, 32 coils, L=12, TV",
+                iters=info["iterations"], s_per_iter=round(dt / max(info["iterations"], 1), 4),
+                iters_per_s=round(info["iterations"] / dt, 3), grids_per_launch=prob.nc * prob.L)
+
+
 def nrmse(a, b):
     return float(np.linalg.norm(np.ravel(a) - np.ravel(b)) / np.linalg.norm(np.ravel(b)))
 
@@ -124,6 +196,9 @@ def main():
     ap.add_argument("--no-oracle", action="store_true", help="skip the FP64 oracle recon")
     ap.add_argument("--plain-steps", type=int, default=2)
     ap.add_argument("--beta", type=float, default=None, help="QuadPenalty weight override")
+    ap.add_argument("--c3-iters", type=int, default=0, metavar="K",
+                    help="also time K PCG iterations of the 3-D stack-of-spirals workload")
+    ap.add_argument("--no-gdft", action="store_true", help="skip the Gdft roofline")
     ap.add_argument("--counters", type=int, default=0, metavar="REPS",
                     help="hardware-counter pass (for rocprofv3 --pmc): one 256 MiB device-to-"
                          "device copy (byte-counter calibration), then REPS forward+adjoint "
@@ -249,6 +324,14 @@ def main():
         "kernels_us": {k: round(v["ms"] * 1e3, 2) for k, v in kinfo.items()},
         "roofline": roof,
     }
+    out["hbm_copy_GBs"] = round(copy_bandwidth(P, stream), 1)
+    log(f"[bench] device copy {out['hbm_copy_GBs']} GB/s")
+    if not args.no_gdft:
+        out["gdft_roofline"] = gdft_roofline(P, prob, stream)
+        log(f"[bench] gdft {out['gdft_roofline']}")
+    if args.c3_iters:
+        out["c3"] = c3_rate(P, S, args.c3_iters, stream)
+        log(f"[bench] c3 {out['c3']}")
     if not args.no_oracle:
         t0 = time.perf_counter()
         x64 = fp64_oracle_recon(prob, y, niter)
@@ -269,7 +352,8 @@ def main():
         cb = cpu_baseline(prob, y)
         out["cpu_baseline"] = cb
         out["speedup_vs_cpu"] = round(value / cb["value"], 1)
-        log(f"[bench] cpu baseline {cb['value']:.3f} it/s on {cb['cores']} threads")
+        log(f"[bench] cpu baseline {cb['value']:.3f} it/s on {cb['cores']} threads, "
+            f"{cb['serial_value']:.3f} it/s serial")
     print(json.dumps(out), flush=True)
 
 

@@ -273,6 +273,34 @@ public:
     uword n1 = 0, n2 = 0, nc = 0;
 };
 
+// pcSenseTimeSeg.h:48-50: kx, ky, kz and t hold the shots' samples
+// shot-major; ShotPhaseMap holds one real phase map per shot (its length sets
+// the shot count, as in pcSenseTimeSeg.cpp:45-46); the per-shot Gnufft and
+// TimeSegmentation objects are owned by the device handle.
+template <typename T1>
+class pcSenseTimeSeg : public detail::OpBase<pcSenseTimeSeg<T1>> {
+public:
+    pcSenseTimeSeg(Col<T1> kx, Col<T1> ky, Col<T1> kz, uword nx, uword ny, uword nz, uword nc,
+                   Col<T1> t, uword lSegs, uword intType, Col<complex<T1>> SENSEmap,
+                   Col<T1> FieldMap, Col<T1> ShotPhaseMap, float kernelWidth = 4.0f)
+    {
+        detail::require_float<T1>();
+        Ni = nx * ny * nz;
+        Nc = nc;
+        Ns = Ni ? ShotPhaseMap.n_rows / Ni : 0;
+        Nd = Ns ? kx.n_rows / Ns : 0;
+        if (Ns * Ni != ShotPhaseMap.n_rows || Nd * Ns != kx.n_rows)
+            throw Error(PG_ERR_ARG, "pcSenseTimeSeg: array lengths do not give whole shots");
+        pg_op h = nullptr;
+        check(pg_pcsense_create(&h, kx.memptr(), ky.memptr(), kz.n_rows ? kz.memptr() : nullptr, nx,
+                                ny, nz, nc, t.memptr(), lSegs, intType,
+                                reinterpret_cast<const float*>(SENSEmap.memptr()),
+                                FieldMap.memptr(), ShotPhaseMap.memptr(), Ns, Nd, kernelWidth));
+        this->h_ = detail::OpHandle(h);
+    }
+    uword Ni = 0, Nc = 0, Ns = 0, Nd = 0;
+};
+
 // ------------------------------------------------------------ penalties ---
 namespace detail {
 struct PenHandle {

@@ -93,6 +93,21 @@ int pg_ts_create(pg_op* out, pg_op G, const float* fieldmap, const float* tvec, 
  * interleaved complex, coil-major (SENSE.cpp:40).  G is not owned. */
 int pg_sense_create(pg_op* out, pg_op G, const float* sens, size_t M, size_t N, size_t nc);
 
+/* replaces pcSenseTimeSeg<float>::pcSenseTimeSeg (pcSenseTimeSeg.cpp:40-124,
+ * pcSenseTimeSeg.h:48-50): shot-phase-corrected, field-corrected SENSE.
+ * kx, ky, kz (kz may be null for 2-D) and tvec hold n_shots shots of Nd
+ * samples, shot-major; sens is Nx*Ny*Nz*nc interleaved complex, coil-major;
+ * fieldmap is Nx*Ny*Nz (rad/s); shotphase is Nx*Ny*Nz*n_shots real phases,
+ * shot-major.  The reference derives n_shots and Nd from the array lengths;
+ * here they are passed.  kernelWidth: 4.0 is the reference's (Gnufft.cpp:84).
+ * Forward data is Nd x (n_shots*nc), column s + c*n_shots
+ * (pcSenseTimeSeg.cpp:129-148); the adjoint sums over coils and shots
+ * (:152-171).  The per-shot operators are owned by the handle. */
+int pg_pcsense_create(pg_op* out, const float* kx, const float* ky, const float* kz, size_t Nx,
+                      size_t Ny, size_t Nz, size_t nc, const float* tvec, size_t L,
+                      size_t interptype, const float* sens, const float* fieldmap,
+                      const float* shotphase, size_t n_shots, size_t Nd, float kernelWidth);
+
 int pg_op_destroy(pg_op op);
 /* n_in = image length (N), n_out = data length (M, or M*nc for SENSE) */
 int pg_op_shape(pg_op op, size_t* n_in, size_t* n_out);

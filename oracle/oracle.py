@@ -67,6 +67,8 @@ def _declare(L):
         f("orc_ts_create", _vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int)
         f("orc_sense_create", _vp, _vp, _vp, C.c_int, C.c_int, C.c_int)
         f("orc_dense_create", _vp, _vp, C.c_int, C.c_int)
+        f("orc_pcsense_create", _vp, C.c_int, C.c_int, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int,
+          C.c_int, _vp, C.c_int, C.c_int, _vp, _vp, _vp, R)
         f("orc_op_forward", None, _vp, _vp, _vp)
         f("orc_op_adjoint", None, _vp, _vp, _vp)
         f("orc_op_n_in", C.c_int, _vp)
@@ -292,6 +294,25 @@ class SENSE(_Op):
         h = _fn("orc_sense_create", prec)(G.h, _p(_c(sens, prec)), M, N, nc)
         super().__init__(h, prec)
         self.G = G
+
+
+class PcSenseTimeSeg(_Op):
+    """pcSenseTimeSeg.cpp:40-171: shot-phase-corrected, field-corrected SENSE.
+    kx, ky, kz, t hold Ns shots of Nd samples, shot-major; sens is Nc maps
+    (coil-major); shot_phase is Ns maps of Ni real phases (rad); the data is
+    Nd x (Ns*Nc), column s + c*Ns."""
+
+    def __init__(self, kx, ky, kz, Nx, Ny, Nz, nc, t, L, interptype, sens, fieldmap, shot_phase,
+                 kernel_width=4.0, prec="f"):
+        Ni = Nx * Ny * Nz
+        Ns = len(np.ravel(shot_phase)) // Ni
+        Nd = len(np.ravel(kx)) // Ns
+        h = _fn("orc_pcsense_create", prec)(
+            Nd, Ns, _p(_r(kx, prec)), _p(_r(ky, prec)), _p(_r(kz, prec)), Nx, Ny, Nz, nc,
+            _p(_r(t, prec)), L, interptype, _p(_c(sens, prec)), _p(_r(fieldmap, prec)),
+            _p(_r(shot_phase, prec)), kernel_width)
+        super().__init__(h, prec)
+        self.Nd, self.Ns, self.Nc = Nd, Ns, nc
 
 
 class Dense(_Op):

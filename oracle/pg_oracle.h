@@ -83,6 +83,12 @@ int orc_get_threads(void);
                           int L, int interptype, int shots);                  \
   void* orc_sense_create_##S(void* G, const R* sens, int M, int N, int nc);   \
   void* orc_dense_create_##S(const R* A, int rows, int cols);                 \
+  /* pcSenseTimeSeg.cpp:40-171 */                                             \
+  void* orc_pcsense_create_##S(int Nd, int Ns, const R* kx, const R* ky,      \
+                               const R* kz, int Nx, int Ny, int Nz, int Nc,   \
+                               const R* t, int L, int interptype,             \
+                               const R* sens, const R* fm, const R* phase,    \
+                               R kernelWidth);                                \
   void orc_op_forward_##S(void* op, const R* in, R* out);                     \
   void orc_op_adjoint_##S(void* op, const R* in, R* out);                     \
   int orc_op_n_in_##S(void* op);                                              \

@@ -537,7 +537,7 @@ void FN(orc_ift)(R* out, const R* in, const R* kx, const R* ky, const R* kz,
 /* -------------------------------------------------------- operators -- */
 
 typedef struct FN(orc_op_s) {
-    int kind; /* 0 nufft, 1 dft, 2 ts, 3 sense, 4 dense */
+    int kind; /* 0 nufft, 1 dft, 2 ts, 3 sense, 4 dense, 5 pcsense */
     int n_in, n_out;
     void* ctx;
 } FN(orc_op);
@@ -576,6 +576,18 @@ typedef struct {
     int rows, cols;
     R* A;
 } FN(dense_ctx);
+
+/* pcSenseTimeSeg.cpp:40-124: Ns shots, each with its own Gnufft and
+ * TimeSegmentation over the shot's Nd samples; shot-specific maps
+ * SS[c*Ns + s] = S_c exp(-i P_s), CS[c*Ns + s] = conj(S_c) conj(exp(-i P_s)) */
+typedef struct {
+    int Ns, Nc, Nd, Ni;
+    FN(orc_op) * *G;
+    FN(orc_op) * *A;
+    R* SS;  /* Ni x (Nc*Ns) complex */
+    R* CS;
+    R *tmp_in, *tmp_out;
+} FN(pcsense_ctx);
 
 static R* FN(dup)(const R* p, size_t n)
 {
@@ -792,6 +804,56 @@ void* FN(orc_dense_create)(const R* A, int rows, int cols)
     return op;
 }
 
+/* pcSenseTimeSeg.cpp:40-124.  kx, ky, kz, t: Nd*Ns, shot-major (sample d of
+ * shot s at s*Nd + d); sens: Ni*Nc complex coil-major; fm: Ni; phase: Ni*Ns
+ * real, shot-major.  Gnufft per shot with gridos 2 (:98), TimeSegmentation
+ * per shot with shots = 1 (:99-100).  The shot maps (:104-115) are formed in
+ * double and rounded once. */
+void* FN(orc_pcsense_create)(int Nd, int Ns, const R* kx, const R* ky, const R* kz, int Nx, int Ny,
+                             int Nz, int Nc, const R* t, int L, int interptype, const R* sens,
+                             const R* fm, const R* phase, R kernelWidth)
+{
+    FN(pcsense_ctx)* c = (FN(pcsense_ctx)*)calloc(1, sizeof(*c));
+    const int Ni = Nx * Ny * Nz;
+    c->Ns = Ns;
+    c->Nc = Nc;
+    c->Nd = Nd;
+    c->Ni = Ni;
+    c->G = (FN(orc_op)**)calloc((size_t)Ns, sizeof(void*));
+    c->A = (FN(orc_op)**)calloc((size_t)Ns, sizeof(void*));
+    c->tmp_in = FN(dup)(NULL, 2 * (size_t)Ni);
+    c->tmp_out = FN(dup)(NULL, 2 * (size_t)Ni);
+    for (int s = 0; s < Ns; ++s) {
+        const size_t o = (size_t)s * Nd;
+        c->G[s] = (FN(orc_op)*)FN(orc_gnufft_create)(Nd, (R)2.0, Nx, Ny, Nz, kx + o, ky + o,
+                                                      kz + o, kernelWidth);
+        c->A[s] = (FN(orc_op)*)FN(orc_ts_create)(c->G[s], fm, t + o, Nd, Ni, L, interptype, 1);
+    }
+    c->SS = FN(dup)(NULL, (size_t)2 * Ni * Nc * Ns);
+    c->CS = FN(dup)(NULL, (size_t)2 * Ni * Nc * Ns);
+    for (int cc = 0; cc < Nc; ++cc)
+        for (int s = 0; s < Ns; ++s) {
+            const size_t col = (size_t)s + (size_t)cc * Ns;
+            for (int n = 0; n < Ni; ++n) {
+                const double sr = sens[2 * ((size_t)cc * Ni + n)],
+                             si = sens[2 * ((size_t)cc * Ni + n) + 1];
+                const double ph = phase[(size_t)s * Ni + n];
+                const double er = cos(ph), ei = -sin(ph); /* exp(-i P) */
+                c->SS[2 * (col * Ni + n)] = (R)(sr * er - si * ei);
+                c->SS[2 * (col * Ni + n) + 1] = (R)(sr * ei + si * er);
+                /* conj(S) conj(exp(-i P)) */
+                c->CS[2 * (col * Ni + n)] = (R)(sr * er - si * ei);
+                c->CS[2 * (col * Ni + n) + 1] = (R)(-(sr * ei + si * er));
+            }
+        }
+    FN(orc_op)* op = (FN(orc_op)*)calloc(1, sizeof(*op));
+    op->kind = 5;
+    op->n_in = Ni;
+    op->n_out = Nd * Ns * Nc;
+    op->ctx = c;
+    return op;
+}
+
 static inline void FN(cmul)(R ar, R ai, R br, R bi, R* cr, R* ci)
 {
     *cr = ar * br - ai * bi;
@@ -838,6 +900,20 @@ void FN(orc_op_forward)(void* opv, const R* in, R* out)
                          &c->tmp_in[2 * n + 1]);
             FN(orc_op_forward)(c->G, c->tmp_in, out + 2 * (size_t)cc * c->M);
         }
+        break;
+    }
+    case 5: {
+        /* pcSenseTimeSeg.cpp:129-148: column s + c*Ns of an Nd x (Ns*Nc) array */
+        FN(pcsense_ctx)* c = (FN(pcsense_ctx)*)op->ctx;
+        for (int cc = 0; cc < c->Nc; ++cc)
+            for (int s = 0; s < c->Ns; ++s) {
+                const size_t col = (size_t)s + (size_t)cc * c->Ns;
+                const R* m = c->SS + 2 * col * c->Ni;
+                for (int n = 0; n < c->Ni; ++n)
+                    FN(cmul)(in[2 * n], in[2 * n + 1], m[2 * n], m[2 * n + 1], &c->tmp_in[2 * n],
+                             &c->tmp_in[2 * n + 1]);
+                FN(orc_op_forward)(c->A[s], c->tmp_in, out + 2 * col * c->Nd);
+            }
         break;
     }
     case 4: {
@@ -911,6 +987,26 @@ void FN(orc_op_adjoint)(void* opv, const R* in, R* out)
         }
         break;
     }
+    case 5: {
+        /* pcSenseTimeSeg.cpp:152-171: sum over coils (outer) and shots (inner) of
+           CS[c*Ns + s] (x) A_s' d[:, s + c*Ns] */
+        FN(pcsense_ctx)* c = (FN(pcsense_ctx)*)op->ctx;
+        memset(out, 0, sizeof(R) * 2 * c->Ni);
+        for (int cc = 0; cc < c->Nc; ++cc)
+            for (int s = 0; s < c->Ns; ++s) {
+                const size_t col = (size_t)s + (size_t)cc * c->Ns;
+                FN(orc_op_adjoint)(c->A[s], in + 2 * col * c->Nd, c->tmp_out);
+                const R* m = c->CS + 2 * col * c->Ni;
+                for (int n = 0; n < c->Ni; ++n) {
+                    R cr, ci;
+                    FN(cmul)(m[2 * n], m[2 * n + 1], c->tmp_out[2 * n], c->tmp_out[2 * n + 1], &cr,
+                             &ci);
+                    out[2 * n] += cr;
+                    out[2 * n + 1] += ci;
+                }
+            }
+        break;
+    }
     case 4: {
         FN(dense_ctx)* c = (FN(dense_ctx)*)op->ctx;
         for (int q = 0; q < c->cols; ++q) {
@@ -964,6 +1060,15 @@ void FN(orc_op_destroy)(void* opv)
     case 4: {
         FN(dense_ctx)* c = (FN(dense_ctx)*)op->ctx;
         free(c->A);
+        break;
+    }
+    case 5: {
+        FN(pcsense_ctx)* c = (FN(pcsense_ctx)*)op->ctx;
+        for (int s = 0; s < c->Ns; ++s) {
+            FN(orc_op_destroy)(c->A[s]);
+            FN(orc_op_destroy)(c->G[s]);
+        }
+        free(c->A); free(c->G); free(c->SS); free(c->CS); free(c->tmp_in); free(c->tmp_out);
         break;
     }
     }
@@ -1221,17 +1326,19 @@ int FN(orc_pcg)(void* opv, void* Rv, const R* x0, const R* W, const R* yi, int n
             R pden[2] = {0, 0};
             if (Rv) FN(orc_penalty_denom)(Rv, ddir, xs, pden);
             R dr = dAWAd + pden[0], di = pden[1];
-            R ad = (R)sqrt(dr * dr + di * di);
+            /* std::abs of the complex denom: hypot, no overflow of dr^2 in float */
+            R ad = (R)hypot(dr, di);
             if (ad < 1e-20 || ad > 1e25) { bad = 1; break; } /* :137-145 */
             R pdot = 0;
             if (Rv) {
                 FN(orc_penalty_gradient)(Rv, xs, pg);
                 FN(cdot)(ddir, pg, N, &pdot, &dummy);
             }
-            /* step -= (-dAWr + step*dAWAd + pdot) / denom (complex denom) */
+            /* step -= (-dAWr + step*dAWAd + pdot) / denom (complex denom),
+               divided with scaling as complex division does */
             R num = -dAWr + step * dAWAd + pdot;
-            R den2 = dr * dr + di * di;
-            step -= num * dr / den2;
+            R sdr = dr / ad, sdi = di / ad;
+            step -= (num / ad) * sdr / (sdr * sdr + sdi * sdi);
         }
         if (bad) break;
         for (long i = 0; i < 2 * M; ++i) Ax[i] += step * Adir[i];

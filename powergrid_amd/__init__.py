@@ -81,6 +81,14 @@ class DeviceArray:
             check(lib().pg_memcpy_htod(d.ptr, _p(a), a.nbytes, _sh(stream)))
         return d
 
+    def upload(self, a, stream=None):
+        """copy host data of the same size into this buffer"""
+        a = np.ascontiguousarray(np.asarray(a, dtype=self.dtype).ravel())
+        assert a.nbytes == self.nbytes
+        if a.nbytes:
+            check(lib().pg_memcpy_htod(self.ptr, _p(a), a.nbytes, _sh(stream)))
+        return self
+
     def to_host(self, stream=None):
         out = np.empty(self.size, self.dtype)
         if self.nbytes:
@@ -217,6 +225,32 @@ class TimeSegmentation(_Op):
         self.L = max(1, int(L))
 
 
+class PcSenseTimeSeg(_Op):
+    """pcSenseTimeSeg<float>(kx, ky, kz, nx, ny, nz, nc, t, L, intType,
+    SENSEmap, FieldMap, ShotPhaseMap) (pcSenseTimeSeg.cpp:40-124): kx, ky, kz
+    and t hold the shots' samples shot-major; ShotPhaseMap holds one real
+    phase map (rad) per shot; the data is Nd x (Ns*nc), column s + c*Ns."""
+
+    def __init__(self, kx, ky, kz, nx, ny, nz, nc, t, L, interptype, sens, fieldmap,
+                 shot_phase, kernel_width=4.0):
+        Ni = nx * ny * nz
+        ph = np.ascontiguousarray(np.ravel(shot_phase), dtype=np.float32)
+        Ns = ph.size // Ni
+        assert Ns >= 1 and Ns * Ni == ph.size
+        Mt = len(np.ravel(kx))
+        assert Mt % Ns == 0
+        Nd = Mt // Ns
+        arrs = [_f32(kx, Mt), _f32(ky, Mt), None if kz is None else _f32(kz, Mt), _f32(t, Mt)]
+        s = np.ascontiguousarray(np.ravel(sens), dtype=np.complex64)
+        assert s.size == Ni * nc
+        fm = _f32(fieldmap, Ni)
+        h = _create(lib().pg_pcsense_create, _p(arrs[0]), _p(arrs[1]), _p(arrs[2]), nx, ny, nz,
+                    nc, _p(arrs[3]), L, interptype, _p(s), _p(fm), _p(ph), Ns, Nd,
+                    float(kernel_width))
+        super().__init__(h)
+        self.Ns, self.Nd, self.nc = Ns, Nd, nc
+
+
 class SENSE(_Op):
     """SENSE<float, G>(G, sens, M, N, nc) (SENSE.cpp:32-44); sens holds nc
     coil maps of N values, coil-major."""
@@ -328,6 +362,7 @@ class profile:
         return n.value, ms.value
 
 
-__all__ = ["DeviceArray", "Stream", "Gnufft", "Gdft", "TimeSegmentation", "SENSE", "QuadPenalty", "TVPenalty",
+__all__ = ["DeviceArray", "Stream", "Gnufft", "Gdft", "TimeSegmentation", "SENSE", "PcSenseTimeSeg",
+           "QuadPenalty", "TVPenalty",
            "solve_pwls_pcg", "reconSolve", "profile", "PgError", "MODE_BATCHED", "MODE_PLAIN",
            "CG_GRAPH", "CG_HOST"]

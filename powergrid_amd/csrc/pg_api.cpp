@@ -150,9 +150,14 @@ static double kb_ft(double W, double beta, double u)
     return std::sinh(std::sqrt(-e)) / std::sqrt(-e);
 }
 
+// bumped whenever an operator workspace is (re)allocated: a cached solver
+// graph that captured workspace addresses is rebuilt after that
+static uint64_t g_ws_gen = 1;
+
 void Gnufft::ensure_batch(int B)
 {
     if (B <= cap_batch) return;
+    ++g_ws_gen;
     dev_free(grids);
     dev_free(tbuf);
     dev_free(tbuf2);
@@ -162,6 +167,8 @@ void Gnufft::ensure_batch(int B)
     if (g.dims == 3) tbuf2 = dev_alloc<float2>((size_t)B * g.Nz * g.Gx * g.Gy);
     dev_free(vals);
     vals = dev_alloc<float2>((size_t)M * B);
+    dev_free(partial);
+    if (n_slots > 0) partial = dev_alloc<float2>((size_t)((B + 63) / 64) * n_slots * 16 * 64);
     cap_batch = B;
 }
 
@@ -178,6 +185,8 @@ void Gnufft::release()
     free_fft_plan(pz);
     dev_free(witems);
     dev_free(bgs);
+    dev_free(comb);
+    dev_free(partial);
     dev_free(entries);
     dev_free(iitems);
     dev_free(irec);
@@ -210,7 +219,7 @@ struct Win1 {
 };
 
 static Win1 window1(double k, int N, double os, double W, int NT, const std::vector<double>& lut,
-                    float* wout)
+                    double* wout)
 {
     const double s = os * (k + (double)N / 2.0);
     const double hw = W * os / 2.0;
@@ -228,7 +237,7 @@ static Win1 window1(double k, int N, double os, double W, int NT, const std::vec
             const long long k0 = (long long)((d * d * 4.0 / (W * W)) * (double)S);
             wt = (k0 >= S) ? 0.0 : lut[k0];
         }
-        wout[j] = (float)wt;
+        wout[j] = wt;
     }
     return r;
 }
@@ -287,8 +296,9 @@ static void gnufft_setup(Gnufft& G, size_t M, float os, size_t Nx, size_t Ny, si
     // per-sample windows and weights
     std::vector<int> lo((size_t)D * M);
     std::vector<float> w((size_t)D * NT * M);
+    std::vector<double> wd((size_t)D * NT * M);  // the same weights in FP64
     std::vector<int> vlo((size_t)D * M), vhi((size_t)D * M);
-    std::vector<float> tmp(NT);
+    std::vector<double> tmp(NT);
     const float* kk[3] = {kx, ky, kz};
     const int Nd[3] = {g.Nx, g.Ny, g.Nz};
     for (int d = 0; d < D; ++d)
@@ -297,7 +307,10 @@ static void gnufft_setup(Gnufft& G, size_t M, float os, size_t Nx, size_t Ny, si
             lo[(size_t)d * M + i] = r.lo;
             vlo[(size_t)d * M + i] = r.vlo;
             vhi[(size_t)d * M + i] = r.vhi;
-            for (int j = 0; j < NT; ++j) w[((size_t)d * NT + j) * M + i] = tmp[j];
+            for (int j = 0; j < NT; ++j) {
+                w[((size_t)d * NT + j) * M + i] = (float)tmp[j];
+                wd[((size_t)d * NT + j) * M + i] = tmp[j];
+            }
         }
     G.lo = dev_copy(lo.data(), lo.size());
     G.w = dev_copy(w.data(), w.size());
@@ -440,19 +453,20 @@ static void gnufft_setup(Gnufft& G, size_t M, float os, size_t Nx, size_t Ny, si
                     }
             }
             if (pass == 1) {
-                // weights of entry k (patch q, sample i) on the patch's 4 x and 4 y points
-                std::vector<float> ew((size_t)8 * std::max<long long>(1, cnt[np]), 0.0f);
+                // weights of entry k (patch q, sample i) on the patch's 16 points,
+                // wx * wy formed in FP64 and rounded once
+                std::vector<float> ew((size_t)16 * std::max<long long>(1, cnt[np]), 0.0f);
                 auto tapw = [&](int d, size_t i, int j) {
-                    return (j >= 0 && j < NT) ? w[((size_t)d * NT + j) * M + i] : 0.0f;
+                    return (j >= 0 && j < NT) ? wd[((size_t)d * NT + j) * M + i] : 0.0;
                 };
                 for (long long q = 0; q < np; ++q) {
                     const int px0 = (int)(q / npy) * kPatch, py0 = (int)(q % npy) * kPatch;
                     for (long long k = cnt[q]; k < cnt[q + 1]; ++k) {
                         const size_t i = (size_t)ent[k];
-                        for (int u = 0; u < kPatch; ++u) {
-                            ew[8 * k + u] = tapw(0, i, px0 + u - lo[i]);
-                            ew[8 * k + 4 + u] = tapw(1, i, py0 + u - lo[M + i]);
-                        }
+                        for (int u = 0; u < kPatch; ++u)
+                            for (int v = 0; v < kPatch; ++v)
+                                ew[16 * k + 4 * u + v] = (float)(tapw(0, i, px0 + u - lo[i]) *
+                                                                 tapw(1, i, py0 + v - lo[M + i]));
                     }
                 }
                 G.entries = dev_copy(ent.data(), ent.size());
@@ -500,43 +514,60 @@ static void gnufft_setup(Gnufft& G, size_t M, float os, size_t Nx, size_t Ny, si
                 run += pweight(q);
             }
         }
-        // per group: blocks of 8 wavefront items; a patch of n entries takes
-        // gs = 1 (n <= 64) or the power of two >= n/64 (at most 8) wavefronts
+        // per group: blocks of 8 wavefront items {patch, entry begin, entry end,
+        // slot}.  A patch of n <= 512 entries is one unit taking gs = 1
+        // (n <= 64) or the power of two >= n/64 wavefronts and writes the
+        // grid (slot -1); a heavier patch is cut into ceil(n/512) units of 8
+        // wavefronts, each writing a partial slot that a second pass adds up
+        // in slot order.
+        struct Unit {
+            int q, e0, e1, slot, gs;
+        };
         struct SBlock {
             int gs;
             int cost;
             int4 w[8];
         };
-        const int kW = 8;
+        const int kW = 8, kUnitMax = kW * 64;
+        std::vector<int4> comb;
+        int nslots = 0;
         std::vector<std::vector<SBlock>> gblk(kXcd);
         for (int gi = 0; gi < kXcd; ++gi) {
-            std::vector<std::vector<int>> byg(4);  // gs = 1, 2, 4, 8
+            std::vector<std::vector<Unit>> byg(4);  // gs = 1, 2, 4, 8
             for (int q : grp[gi]) {
-                const long long n = cnt[q + 1] - cnt[q];
+                const int b0 = (int)cnt[q], n = (int)(cnt[q + 1] - cnt[q]);
+                if (n > kUnitMax) {
+                    const int parts = (n + kUnitMax - 1) / kUnitMax;
+                    comb.push_back(make_int4(q, nslots, parts, 0));
+                    for (int k = 0; k < parts; ++k)
+                        byg[3].push_back(Unit{q, b0 + (int)((long long)n * k / parts),
+                                              b0 + (int)((long long)n * (k + 1) / parts), nslots + k, 8});
+                    nslots += parts;
+                    continue;
+                }
                 int gs = 1, lg = 0;
-                while (gs < kW && (long long)gs * 64 < n) {
+                while ((long long)gs * 64 < n) {
                     gs *= 2;
                     ++lg;
                 }
-                byg[lg].push_back(q);
+                byg[lg].push_back(Unit{q, b0, b0 + n, -1, gs});
             }
             for (int lg = 0; lg < 4; ++lg) {
                 const int gs = 1 << lg, per = kW / gs;
                 auto& v = byg[lg];
-                std::stable_sort(v.begin(), v.end(), [&](int u, int w) {
-                    return cnt[u + 1] - cnt[u] > cnt[w + 1] - cnt[w];
-                });
+                std::stable_sort(v.begin(), v.end(),
+                                 [](const Unit& u, const Unit& w) { return u.e1 - u.e0 > w.e1 - w.e0; });
                 for (size_t k0 = 0; k0 < v.size(); k0 += per) {
                     SBlock sb;
                     sb.gs = gs;
                     sb.cost = 0;
-                    for (int w = 0; w < kW; ++w) sb.w[w] = make_int4(-1, 0, 0, 0);
+                    for (int w = 0; w < kW; ++w) sb.w[w] = make_int4(-1, 0, 0, -1);
                     for (int pk = 0; pk < per && k0 + pk < v.size(); ++pk) {
-                        const int q = v[k0 + pk];
-                        const long long b0 = cnt[q], n = cnt[q + 1] - cnt[q];
+                        const Unit& un = v[k0 + pk];
+                        const long long n = un.e1 - un.e0;
                         for (int w = 0; w < gs; ++w) {
-                            const int e0 = (int)(b0 + n * w / gs), e1 = (int)(b0 + n * (w + 1) / gs);
-                            sb.w[pk * gs + w] = make_int4(q, e0, e1, 0);
+                            const int e0 = (int)(un.e0 + n * w / gs), e1 = (int)(un.e0 + n * (w + 1) / gs);
+                            sb.w[pk * gs + w] = make_int4(un.q, e0, e1, un.slot);
                             sb.cost = std::max(sb.cost, e1 - e0);
                         }
                     }
@@ -555,27 +586,39 @@ static void gnufft_setup(Gnufft& G, size_t M, float os, size_t Nx, size_t Ny, si
             for (int gi = 0; gi < kXcd; ++gi) {
                 const size_t blk = k * kXcd + gi;
                 for (int w = 0; w < kW; ++w)
-                    witems[blk * kW + w] = (k < gblk[gi].size()) ? gblk[gi][k].w[w] : make_int4(-1, 0, 0, 0);
+                    witems[blk * kW + w] = (k < gblk[gi].size()) ? gblk[gi][k].w[w] : make_int4(-1, 0, 0, -1);
                 if (k < gblk[gi].size()) bgs[blk] = gblk[gi][k].gs;
             }
         {
-            // every patch is one group of gs wavefronts whose ranges tile its entries
-            std::vector<long long> next(np);
-            for (long long q = 0; q < np; ++q) next[q] = cnt[q];
+            // groups are aligned runs of gs wavefronts with one patch and slot
+            // and consecutive ranges; all ranges of a patch tile its entries
+            std::vector<std::vector<std::pair<int, int>>> rng(np);
             for (size_t blk = 0; blk < nmax * kXcd; ++blk) {
                 const int gs = bgs[blk];
                 for (int w = 0; w < kW; ++w) {
                     const int4 it = witems[blk * kW + w];
+                    const int4 ld = witems[blk * kW + (w / gs) * gs];
+                    PG_REQUIRE(it.x == ld.x && it.w == ld.w, PG_ERR_STATE, "spread work groups");
+                    if (w % gs) PG_REQUIRE(witems[blk * kW + w - 1].z == it.y, PG_ERR_STATE, "spread work groups");
                     if (it.x < 0) continue;
-                    PG_REQUIRE(it.x < np && next[it.x] == it.y && it.z >= it.y &&
-                                   ((w % gs == 0) == (it.y == cnt[it.x])),
-                               PG_ERR_STATE, "spread work list");
-                    next[it.x] = it.z;
+                    PG_REQUIRE(it.x < np && it.z >= it.y, PG_ERR_STATE, "spread work list");
+                    rng[it.x].push_back({it.y, it.z});
                 }
             }
-            for (long long q = 0; q < np; ++q)
-                PG_REQUIRE(next[q] == cnt[q + 1], PG_ERR_STATE, "spread work list coverage");
+            for (long long q = 0; q < np; ++q) {
+                auto& r = rng[q];
+                std::sort(r.begin(), r.end());
+                long long at = cnt[q];
+                for (auto& pr : r) {
+                    PG_REQUIRE(pr.first == at, PG_ERR_STATE, "spread work list coverage");
+                    at = pr.second;
+                }
+                PG_REQUIRE(at == cnt[q + 1] && !r.empty(), PG_ERR_STATE, "spread work list coverage");
+            }
         }
+        G.n_comb = (int)comb.size();
+        G.n_slots = nslots;
+        if (!comb.empty()) G.comb = dev_copy(comb.data(), comb.size());
         G.n_sblocks = (int)(nmax * kXcd);
         G.witems = dev_copy(witems.data(), witems.size());
         G.bgs = dev_copy(bgs.data(), bgs.size());
@@ -620,14 +663,16 @@ static void gnufft_setup(Gnufft& G, size_t M, float os, size_t Nx, size_t Ny, si
 }
 
 // ----------------------------------------------------------- operators --
-enum Kind { K_NUFFT = 1, K_DFT = 2, K_TS = 3, K_SENSE = 4 };
+enum Kind { K_NUFFT = 1, K_DFT = 2, K_TS = 3, K_SENSE = 4, K_PCSENSE = 5 };
 
+struct SolveCache;
 struct Op {
     int magic = 0x50474f50;
     Kind kind;
     size_t n_in = 0, n_out = 0;
     int mode = PG_MODE_BATCHED;
-    virtual ~Op() {}
+    SolveCache* scache = nullptr;  // PCG work buffers and graph of the last solve
+    virtual ~Op();
 };
 
 struct NufftOp : Op {
@@ -643,6 +688,7 @@ struct DftOp : Op {
     void ensure_part(size_t n)
     {
         if (n <= part_cap) return;
+        ++g_ws_gen;
         dev_free(part);
         part = dev_alloc<float2>(n);
         part_cap = n;
@@ -702,6 +748,28 @@ struct SenseOp : Op {
     }
 };
 
+// pcSenseTimeSeg.cpp:40-171: Ns shots, each with its own NUFFT plan and
+// time segmentation over its Nd samples; the shot maps S_c exp(-i P_s) are
+// stored per shot, coil-major ([s][c][n]).  Data: Nd x (Ns*Nc), column
+// s + c*Ns.
+struct PcSenseOp : Op {
+    int Ns = 1, Nc = 1, L = 1;
+    size_t Nd = 0, Ni = 0;
+    std::vector<Op*> G;   // NufftOp per shot (owned)
+    std::vector<Op*> A;   // TsOp per shot (owned)
+    float2* SS = nullptr;  // Ns x Nc x Ni
+    float2* P = nullptr;   // Ns x Nc x Ni partial images
+    float2* tmp_n = nullptr;
+    ~PcSenseOp() override
+    {
+        for (Op* o : A) delete o;
+        for (Op* o : G) delete o;
+        dev_free(SS);
+        dev_free(P);
+        dev_free(tmp_n);
+    }
+};
+
 static Op* as_op(pg_op h)
 {
     Op* op = reinterpret_cast<Op*>(h);
@@ -747,7 +815,8 @@ static bool fusable(Op* op, Fused& f)
 static void op_forward(Op* op, const float2* in, float2* out, hipStream_t s);
 static void op_adjoint(Op* op, const float2* in, float2* out, hipStream_t s);
 
-static void fused_forward(const Fused& f, const float2* x, float2* y, hipStream_t s)
+static void fused_forward(const Fused& f, const float2* x, float2* y, hipStream_t s,
+                          long long y_cstride = -1)
 {
     Gnufft& G = *f.G;
     const int B = f.nc * f.L;
@@ -759,14 +828,15 @@ static void fused_forward(const Fused& f, const float2* x, float2* y, hipStream_
     }
     {
         Prof p("interp", s);
-        pgk::launch_interp(G, G.grids, f.nc, f.L, f.AA, y, G.M, s);
+        pgk::launch_interp(G, G.grids, f.nc, f.L, f.AA, y, y_cstride < 0 ? G.M : y_cstride, s);
     }
 }
 
 // adjoint into per-coil partial images P (nc x N), optionally of the CG
 // residual W (y - Ax) formed on the fly
 static void fused_adjoint_partial(const Fused& f, const float2* in, const float2* resid_y,
-                                  const double2* ax64, const float* Wd, float2* P, hipStream_t s)
+                                  const double2* ax64, const float* Wd, float2* P, hipStream_t s,
+                                  bool vals_ready = false, long long in_cstride = -1)
 {
     Gnufft& G = *f.G;
     const int B = f.nc * f.L;
@@ -774,7 +844,8 @@ static void fused_adjoint_partial(const Fused& f, const float2* in, const float2
     pgk::WeightArgs w{f.S, f.Wo, f.WoH, f.AA};
     {
         Prof p("spread", s);
-        pgk::launch_spread(G, in, G.M, f.nc, f.L, f.AA, resid_y, ax64, Wd, G.grids, s);
+        pgk::launch_spread(G, in, in_cstride < 0 ? G.M : in_cstride, f.nc, f.L, f.AA, resid_y, ax64,
+                           Wd, G.grids, vals_ready, s);
     }
     {
         Prof p("adj_fft_post", s);
@@ -804,6 +875,15 @@ static void op_prepare(Op* op)
     case K_TS:
         op_prepare(static_cast<TsOp*>(op)->inner);
         return;
+    case K_PCSENSE: {
+        PcSenseOp* pc = static_cast<PcSenseOp*>(op);
+        for (Op* ts : pc->A) {
+            if (pc->mode == PG_MODE_BATCHED)
+                static_cast<NufftOp*>(static_cast<TsOp*>(ts)->inner)->G.ensure_batch(pc->Nc * pc->L);
+            op_prepare(ts);
+        }
+        return;
+    }
     case K_SENSE: {
         SenseOp* se = static_cast<SenseOp*>(op);
         if (se->inner->kind == K_DFT) {
@@ -866,6 +946,34 @@ static void op_forward(Op* op, const float2* in, float2* out, hipStream_t s)
         }
         return;
     }
+    case K_PCSENSE: {
+        // pcSenseTimeSeg.cpp:129-148
+        PcSenseOp* pc = static_cast<PcSenseOp*>(op);
+        const long long Ni = (long long)pc->Ni, Nd = (long long)pc->Nd;
+        for (int sh = 0; sh < pc->Ns; ++sh) {
+            TsOp* ts = static_cast<TsOp*>(pc->A[sh]);
+            const float2* SSs = pc->SS + (size_t)sh * pc->Nc * Ni;
+            if (op->mode == PG_MODE_BATCHED) {
+                // every coil x segment of the shot in one fused launch; the
+                // shot's coil blocks are Ns*Nd apart
+                Fused f;
+                f.G = &static_cast<NufftOp*>(ts->inner)->G;
+                f.nc = pc->Nc;
+                f.L = ts->L;
+                f.S = SSs;
+                f.Wo = ts->Wo;
+                f.WoH = ts->WoH;
+                f.AA = ts->AA;
+                fused_forward(f, in, out + sh * Nd, s, (long long)pc->Ns * Nd);
+            } else {
+                for (int c = 0; c < pc->Nc; ++c) {
+                    pgv::cmul(pc->tmp_n, SSs + (size_t)c * Ni, in, Ni, s);
+                    op_forward(ts, pc->tmp_n, out + ((size_t)sh + (size_t)c * pc->Ns) * Nd, s);
+                }
+            }
+        }
+        return;
+    }
     }
 }
 
@@ -924,6 +1032,38 @@ static void op_adjoint(Op* op, const float2* in, float2* out, hipStream_t s)
         }
         return;
     }
+    case K_PCSENSE: {
+        // pcSenseTimeSeg.cpp:152-171
+        PcSenseOp* pc = static_cast<PcSenseOp*>(op);
+        const long long Ni = (long long)pc->Ni, Nd = (long long)pc->Nd;
+        if (op->mode == PG_MODE_BATCHED) {
+            // per shot, every coil x segment in one fused launch, into that
+            // shot's block of Nc partial images; then one sum over the Ns*Nc
+            for (int sh = 0; sh < pc->Ns; ++sh) {
+                TsOp* ts = static_cast<TsOp*>(pc->A[sh]);
+                Fused f;
+                f.G = &static_cast<NufftOp*>(ts->inner)->G;
+                f.nc = pc->Nc;
+                f.L = ts->L;
+                f.S = pc->SS + (size_t)sh * pc->Nc * Ni;
+                f.Wo = ts->Wo;
+                f.WoH = ts->WoH;
+                f.AA = ts->AA;
+                fused_adjoint_partial(f, in + sh * Nd, nullptr, nullptr, nullptr,
+                                      pc->P + (size_t)sh * pc->Nc * Ni, s, false,
+                                      (long long)pc->Ns * Nd);
+            }
+            pgv::coil_sum(out, pc->P, pc->Ns * pc->Nc, Ni, s);
+            return;
+        }
+        for (int c = 0; c < pc->Nc; ++c)
+            for (int sh = 0; sh < pc->Ns; ++sh) {
+                op_adjoint(pc->A[sh], in + ((size_t)sh + (size_t)c * pc->Ns) * Nd, pc->tmp_n, s);
+                pgv::cmul_acc(out, pc->tmp_n, pc->SS + ((size_t)sh * pc->Nc + c) * Ni, Ni,
+                              c == 0 && sh == 0, true, s);
+            }
+        return;
+    }
     }
 }
 
@@ -975,6 +1115,38 @@ struct CgWork {
 };
 
 // one PCG iteration (solve_pwls_pcg.hpp:80-164) as a launch sequence
+// Work buffers and the instantiated graph of a PCG solve, kept on the
+// operator and reused while the solve's key is unchanged (the graph bakes in
+// the addresses of y, W and the operator workspaces, and the penalty's
+// parameters).
+struct SolveCache {
+    const Penalty* R = nullptr;
+    pgv::PenSpec spec{};
+    const float2* y = nullptr;
+    const float* W = nullptr;
+    size_t niter = 0;
+    uint64_t ws_gen = 0;
+    CgWork w;
+    hipStream_t own = nullptr;  // stream for graph solves without a user stream
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    ~SolveCache()
+    {
+        if (exec) hipGraphExecDestroy(exec);
+        if (graph) hipGraphDestroy(graph);
+        if (own) hipStreamDestroy(own);
+    }
+    void drop_graph()
+    {
+        if (exec) hipGraphExecDestroy(exec);
+        if (graph) hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+    }
+};
+
+Op::~Op() { delete scache; }
+
 static void cg_iteration(Op* A, Penalty* R, const float2* y, const float* W, CgWork& w,
                          hipStream_t s, const Fused* f)
 {
@@ -984,7 +1156,8 @@ static void cg_iteration(Op* A, Penalty* R, const float2* y, const float* W, CgW
     if (f) {
         SenseOp* se = (A->kind == K_SENSE) ? static_cast<SenseOp*>(A) : nullptr;
         float2* P = se ? se->P : w.g;
-        fused_adjoint_partial(*f, nullptr, y, w.Ax, W, P, s);
+        // the spread values come from the previous update (or the set-up)
+        fused_adjoint_partial(*f, nullptr, y, w.Ax, W, P, s, true);
         pgv::cg_grad(P, f->nc, N, ps, w.x32, w.ngrad, w.cp, w.sc, w.resid, s);
     } else {
         pgv::resid(w.r, y, w.Ax, W, M, s);
@@ -995,7 +1168,29 @@ static void cg_iteration(Op* A, Penalty* R, const float2* y, const float* W, CgW
     op_forward(A, w.ddir, w.Adir, s);
     pgv::cg_adir(w.Adir, y, w.Ax, W, M, ps, w.ddir, w.x32, w.cp, w.sc, s);
     pgv::cg_linesearch(ps, w.ddir, w.x32, w.cp, w.sc, s);
-    pgv::cg_update(w.Ax, w.Adir, y, M, w.x, w.x32, w.ddir, N, ps != nullptr, w.cp, w.sc, s);
+    pgv::VFill vf{};
+    if (f) vf = pgv::VFill{f->G->vals, f->AA, f->L, f->nc, (long long)f->G->M};
+    pgv::cg_update(w.Ax, w.Adir, y, W, M, w.x, w.x32, w.ddir, N, ps, w.cp, vf, w.sc, s);
+}
+
+static bool same_spec(const pgv::PenSpec& a, const pgv::PenSpec& b)
+{
+    return a.kind == b.kind && a.nd == b.nd && a.n == b.n && a.off[0] == b.off[0] &&
+           a.off[1] == b.off[1] && a.off[2] == b.off[2] && a.beta == b.beta && a.delta == b.delta;
+}
+
+// x = x0 (or 0), Ax = A x (:55), the CG scalars, and the first adjoint's
+// spread values
+static void cg_setup(Op* A, const float2* y, const float* W, CgWork& w, const Fused* f,
+                     hipStream_t s)
+{
+    const long long N = (long long)A->n_in, M = (long long)A->n_out;
+    pgv::f2d(w.x, w.x32, N, s);
+    op_forward(A, w.x32, w.tmp, s);
+    pgv::f2d(w.Ax, w.tmp, M, s);
+    pgv::cg_init(y, W, M, w.part, w.sc, s);
+    if (f)  // spread values of the first adjoint, from the initial Ax
+        pgk::launch_spread_values(*f->G, nullptr, f->G->M, f->nc, f->L, f->AA, y, w.Ax, W, s);
 }
 
 static void solve_pcg(Op* A, Penalty* R, const float2* x0, const float* W, const float2* y,
@@ -1003,68 +1198,80 @@ static void solve_pcg(Op* A, Penalty* R, const float2* x0, const float* W, const
                       hipStream_t user_stream)
 {
     const long long N = (long long)A->n_in, M = (long long)A->n_out;
-    CgWork w;
-    w.x = dev_alloc<double2>(N);
-    w.Ax = dev_alloc<double2>(M);
-    w.x32 = dev_alloc<float2>(N);
-    w.tmp = dev_alloc<float2>(M);
-    w.ngrad = dev_alloc<float2>(N);
-    w.ddir = dev_alloc<float2>(N);
-    w.Adir = dev_alloc<float2>(M);
-    w.g = dev_alloc<float2>(N);
-    w.r = dev_alloc<float2>(M);
-    w.part = dev_alloc<double>(std::max<size_t>(pgv::cg_parts_doubles(N, M), 2 * pgv::kMaxParts));
-    w.cp = pgv::cg_parts(w.part, N, M);
-    w.resid = dev_alloc<double>(std::max<size_t>(niter, 1));
-    w.sc = dev_alloc<CgScal>(1);
-
     Fused fz;
     const bool fused = fusable(A, fz);
-    op_prepare(A);
+    op_prepare(A);  // may grow workspaces (and bump g_ws_gen)
+
+    pgv::PenSpec spec{};
+    if (R) spec = R->spec;
+    SolveCache* c = A->scache;
+    const bool same = c && c->R == R && (!R || same_spec(c->spec, spec)) && c->y == y && c->W == W &&
+                      c->niter == niter;
+    if (!same) {
+        delete c;
+        A->scache = nullptr;
+        c = new SolveCache();
+        A->scache = c;
+        c->R = R;
+        c->spec = spec;
+        c->y = y;
+        c->W = W;
+        c->niter = niter;
+        CgWork& w = c->w;
+        w.x = dev_alloc<double2>(N);
+        w.Ax = dev_alloc<double2>(M);
+        w.x32 = dev_alloc<float2>(N);
+        w.tmp = dev_alloc<float2>(M);
+        w.ngrad = dev_alloc<float2>(N);
+        w.ddir = dev_alloc<float2>(N);
+        w.Adir = dev_alloc<float2>(M);
+        w.g = dev_alloc<float2>(N);
+        w.r = dev_alloc<float2>(M);
+        w.part = dev_alloc<double>(std::max<size_t>(pgv::cg_parts_doubles(N, M), 2 * pgv::kMaxParts));
+        w.cp = pgv::cg_parts(w.part, N, M);
+        w.resid = dev_alloc<double>(std::max<size_t>(niter, 1));
+        w.sc = dev_alloc<CgScal>(1);
+    }
+    CgWork& w = c->w;
+    if (c->ws_gen != g_ws_gen) c->drop_graph();
+    c->ws_gen = g_ws_gen;
 
     hipStream_t s = user_stream;
-    hipStream_t own = nullptr;
     if (mode == PG_CG_GRAPH && s == nullptr) {
         PG_HIP(hipStreamSynchronize(nullptr));
-        PG_HIP(hipStreamCreateWithFlags(&own, hipStreamNonBlocking));
-        s = own;
+        if (!c->own) PG_HIP(hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking));
+        s = c->own;
     }
-    struct StreamGuard {
-        hipStream_t s;
-        ~StreamGuard()
-        {
-            if (s) hipStreamDestroy(s);
-        }
-    } guard{own};
 
     if (x0) PG_HIP(hipMemcpyAsync(w.x32, x0, sizeof(float2) * N, hipMemcpyDeviceToDevice, s));
     else pgv::fill(w.x32, N, make_float2(0.f, 0.f), s);
-    pgv::f2d(w.x, w.x32, N, s);
-    op_forward(A, w.x32, w.tmp, s);  // Ax = A * xInitial (:55)
-    pgv::f2d(w.Ax, w.tmp, M, s);
-    pgv::cg_init(y, W, M, w.part, w.sc, s);
 
     CgScal hs{};
     if (mode == PG_CG_GRAPH) {
-        hipGraph_t graph = nullptr;
-        hipGraphExec_t exec = nullptr;
-        PG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        try {
-            cg_iteration(A, R, y, W, w, s, fused ? &fz : nullptr);
-        } catch (...) {
-            hipStreamEndCapture(s, &graph);
-            if (graph) hipGraphDestroy(graph);
-            throw;
+        // the whole solve (set-up, every iteration, bookkeeping) is one graph;
+        // every kernel checks the device-side stop flag, so an early exit
+        // leaves the remaining iterations as no-ops
+        if (!c->exec) {
+            PG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            try {
+                cg_setup(A, y, W, w, fused ? &fz : nullptr, s);
+                for (size_t it = 0; it < niter; ++it)
+                    cg_iteration(A, R, y, W, w, s, fused ? &fz : nullptr);
+                pgv::cg_finish(w.cp, w.sc, w.resid, s);
+            } catch (...) {
+                hipGraph_t g = nullptr;
+                hipStreamEndCapture(s, &g);
+                if (g) hipGraphDestroy(g);
+                throw;
+            }
+            PG_HIP(hipStreamEndCapture(s, &c->graph));
+            PG_HIP(hipGraphInstantiate(&c->exec, c->graph, nullptr, nullptr, 0));
         }
-        PG_HIP(hipStreamEndCapture(s, &graph));
-        PG_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-        for (size_t it = 0; it < niter; ++it) PG_HIP(hipGraphLaunch(exec, s));
-        pgv::cg_finish(w.cp, w.sc, w.resid, s);
+        PG_HIP(hipGraphLaunch(c->exec, s));
         PG_HIP(hipMemcpyAsync(&hs, w.sc, sizeof(CgScal), hipMemcpyDeviceToHost, s));
         PG_HIP(hipStreamSynchronize(s));
-        hipGraphExecDestroy(exec);
-        hipGraphDestroy(graph);
     } else {
+        cg_setup(A, y, W, w, fused ? &fz : nullptr, s);
         for (size_t it = 0; it < niter; ++it) {
             cg_iteration(A, R, y, W, w, s, fused ? &fz : nullptr);
             // host-side branch on the device scalars, as the reference loop does
@@ -1087,6 +1294,69 @@ static void solve_pcg(Op* A, Penalty* R, const float2* x0, const float* W, const
         info->exit_reason = hs.stop;
         info->final_resid = hs.iter > 0 ? rh[hs.iter - 1] : 0.0;
     }
+}
+
+// TimeSegmentation.cpp:36-221 over the operator G (not owned)
+static TsOp* ts_build(Op* G, const float* fieldmap, const float* tvec, size_t M, size_t N, size_t L,
+                      size_t interptype, size_t shots)
+{
+    PG_REQUIRE(G->n_in == N && G->n_out == M, PG_ERR_ARG, "TimeSegmentation sizes do not match G");
+    PG_REQUIRE(interptype == 1, PG_ERR_UNSUPPORTED,
+               "only the Hanning interpolator (interptype 1) is implemented");
+    if (shots < 1) shots = 1;
+    PG_REQUIRE(M % shots == 0, PG_ERR_ARG, "M must be a multiple of shots");
+    std::unique_ptr<TsOp> op(new TsOp());
+    op->kind = K_TS;
+    op->inner = G;
+    op->n_in = N;
+    op->n_out = M;
+    // TimeSegmentation.cpp:46-93, 213-218, evaluated in double and stored
+    // as FP32 tables
+    const int Lc = (L == 0 || L == 1) ? 1 : (int)L;
+    op->L = Lc;
+    double tmin = tvec[0], tmax = tvec[0];
+    for (size_t j = 1; j < M; ++j) {
+        tmin = std::min(tmin, (double)tvec[j]);
+        tmax = std::max(tmax, (double)tvec[j]);
+    }
+    const double rangt = tmax - tmin;
+    double tau = (rangt + 2.220446049250313e-16) / (double)Lc;
+    std::vector<float> AA((size_t)Lc * M, 0.0f);
+    std::vector<float2> Wo((size_t)Lc * N), WoH((size_t)Lc * N);
+    if (Lc <= 1) {
+        tau = 0;
+        std::fill(AA.begin(), AA.end(), 1.0f);
+        std::fill(Wo.begin(), Wo.end(), make_float2(1.f, 0.f));
+        std::fill(WoH.begin(), WoH.end(), make_float2(1.f, 0.f));
+    } else {
+        const size_t none = M / shots;
+        for (int l = 0; l < Lc; ++l)
+            for (size_t j = 0; j < none; ++j) {
+                const double d = ((double)tvec[j] - tmin) - (double)l * tau;
+                float a = 0.0f;
+                if (std::fabs(d) <= tau) a = (float)(0.5 + 0.5 * std::cos(M_PI * d / tau));
+                for (size_t sh = 0; sh < shots; ++sh) AA[(size_t)l * M + j + sh * none] = a;
+            }
+        for (int l = 0; l < Lc; ++l) {
+            const double sl = (double)l * tau + tmin;
+            for (size_t n = 0; n < N; ++n) {
+                const double ph = (double)fieldmap[n] * sl;
+                const float c = (float)std::cos(ph), sn = (float)std::sin(ph);
+                Wo[(size_t)l * N + n] = make_float2(c, -sn);
+                WoH[(size_t)l * N + n] = make_float2(c, sn);
+            }
+        }
+    }
+    op->tau = (float)tau;
+    op->tmin = (float)tmin;
+    op->AA = dev_copy(AA.data(), AA.size());
+    op->Wo = dev_copy(Wo.data(), Wo.size());
+    op->WoH = dev_copy(WoH.data(), WoH.size());
+    op->tmp_n = dev_alloc<float2>(N);
+    op->tmp_n2 = dev_alloc<float2>(N);
+    op->tmp_m = dev_alloc<float2>(M);
+    op->tmp_m2 = dev_alloc<float2>(M);
+    return op.release();
 }
 
 }  // namespace pg
@@ -1244,62 +1514,64 @@ int pg_ts_create(pg_op* out, pg_op Gh, const float* fieldmap, const float* tvec,
     PG_TRY PG_REQUIRE(out && fieldmap && tvec, PG_ERR_ARG, "null argument");
     *out = nullptr;
     Op* G = as_op(Gh);
-    PG_REQUIRE(G->n_in == N && G->n_out == M, PG_ERR_ARG, "TimeSegmentation sizes do not match G");
-    PG_REQUIRE(interptype == 1, PG_ERR_UNSUPPORTED,
-               "only the Hanning interpolator (interptype 1) is implemented");
-    if (shots < 1) shots = 1;
-    PG_REQUIRE(M % shots == 0, PG_ERR_ARG, "M must be a multiple of shots");
-    std::unique_ptr<TsOp> op(new TsOp());
-    op->kind = K_TS;
-    op->inner = G;
-    op->n_in = N;
-    op->n_out = M;
-    // TimeSegmentation.cpp:46-93, 213-218, evaluated in double and stored
-    // as FP32 tables
-    const int Lc = (L == 0 || L == 1) ? 1 : (int)L;
-    op->L = Lc;
-    double tmin = tvec[0], tmax = tvec[0];
-    for (size_t j = 1; j < M; ++j) {
-        tmin = std::min(tmin, (double)tvec[j]);
-        tmax = std::max(tmax, (double)tvec[j]);
-    }
-    const double rangt = tmax - tmin;
-    double tau = (rangt + 2.220446049250313e-16) / (double)Lc;
-    std::vector<float> AA((size_t)Lc * M, 0.0f);
-    std::vector<float2> Wo((size_t)Lc * N), WoH((size_t)Lc * N);
-    if (Lc <= 1) {
-        tau = 0;
-        std::fill(AA.begin(), AA.end(), 1.0f);
-        std::fill(Wo.begin(), Wo.end(), make_float2(1.f, 0.f));
-        std::fill(WoH.begin(), WoH.end(), make_float2(1.f, 0.f));
-    } else {
-        const size_t none = M / shots;
-        for (int l = 0; l < Lc; ++l)
-            for (size_t j = 0; j < none; ++j) {
-                const double d = ((double)tvec[j] - tmin) - (double)l * tau;
-                float a = 0.0f;
-                if (std::fabs(d) <= tau) a = (float)(0.5 + 0.5 * std::cos(M_PI * d / tau));
-                for (size_t sh = 0; sh < shots; ++sh) AA[(size_t)l * M + j + sh * none] = a;
-            }
-        for (int l = 0; l < Lc; ++l) {
-            const double sl = (double)l * tau + tmin;
-            for (size_t n = 0; n < N; ++n) {
-                const double ph = (double)fieldmap[n] * sl;
-                const float c = (float)std::cos(ph), sn = (float)std::sin(ph);
-                Wo[(size_t)l * N + n] = make_float2(c, -sn);
-                WoH[(size_t)l * N + n] = make_float2(c, sn);
-            }
+    *out = reinterpret_cast<pg_op>(ts_build(G, fieldmap, tvec, M, N, L, interptype, shots));
+    PG_CATCH
+}
+
+
+int pg_pcsense_create(pg_op* out, const float* kx, const float* ky, const float* kz, size_t Nx,
+                      size_t Ny, size_t Nz, size_t nc, const float* tvec, size_t L,
+                      size_t interptype, const float* sens, const float* fieldmap,
+                      const float* shotphase, size_t n_shots, size_t Nd, float kernelWidth)
+{
+    PG_TRY PG_REQUIRE(out && kx && ky && tvec && sens && fieldmap && shotphase, PG_ERR_ARG,
+                      "null argument");
+    PG_REQUIRE(Nz == 1 || kz, PG_ERR_ARG, "kz required for 3-D");
+    PG_REQUIRE(n_shots > 0 && Nd > 0 && nc > 0, PG_ERR_ARG, "empty pcSenseTimeSeg");
+    *out = nullptr;
+    const size_t Ni = Nx * Ny * Nz;
+    std::unique_ptr<PcSenseOp> op(new PcSenseOp());
+    op->kind = K_PCSENSE;
+    op->Ns = (int)n_shots;
+    op->Nc = (int)nc;
+    op->Nd = Nd;
+    op->Ni = Ni;
+    op->n_in = Ni;
+    op->n_out = Nd * n_shots * nc;
+    std::vector<float> zk(Nd, 0.0f);
+    for (size_t sh = 0; sh < n_shots; ++sh) {
+        // pcSenseTimeSeg.cpp:96-101: Gnufft(Nd, 2.0, ...) and TimeSegmentation
+        // over the shot's samples, shots = 1
+        std::unique_ptr<NufftOp> g(new NufftOp());
+        g->kind = K_NUFFT;
+        const size_t o = sh * Nd;
+        try {
+            gnufft_setup(g->G, Nd, 2.0f, Nx, Ny, Nz, kx + o, ky + o, kz ? kz + o : zk.data(),
+                         kernelWidth);
+        } catch (...) {
+            g->G.release();
+            throw;
         }
+        g->n_in = Ni;
+        g->n_out = Nd;
+        op->G.push_back(g.release());
+        op->A.push_back(ts_build(op->G.back(), fieldmap, tvec + o, Nd, Ni, L, interptype, 1));
     }
-    op->tau = (float)tau;
-    op->tmin = (float)tmin;
-    op->AA = dev_copy(AA.data(), AA.size());
-    op->Wo = dev_copy(Wo.data(), Wo.size());
-    op->WoH = dev_copy(WoH.data(), WoH.size());
-    op->tmp_n = dev_alloc<float2>(N);
-    op->tmp_n2 = dev_alloc<float2>(N);
-    op->tmp_m = dev_alloc<float2>(M);
-    op->tmp_m2 = dev_alloc<float2>(M);
+    op->L = static_cast<TsOp*>(op->A[0])->L;
+    // pcSenseTimeSeg.cpp:104-115, in double: S_c exp(-i P_s), stored [s][c][n]
+    std::vector<float2> ss(n_shots * nc * Ni);
+    for (size_t sh = 0; sh < n_shots; ++sh)
+        for (size_t c = 0; c < nc; ++c)
+            for (size_t n = 0; n < Ni; ++n) {
+                const double sr = sens[2 * (c * Ni + n)], si = sens[2 * (c * Ni + n) + 1];
+                const double ph = shotphase[sh * Ni + n];
+                const double er = std::cos(ph), ei = -std::sin(ph);
+                ss[(sh * nc + c) * Ni + n] = make_float2((float)(sr * er - si * ei),
+                                                         (float)(sr * ei + si * er));
+            }
+    op->SS = dev_copy(ss.data(), ss.size());
+    op->P = dev_alloc<float2>(n_shots * nc * Ni);
+    op->tmp_n = dev_alloc<float2>(Ni);
     *out = reinterpret_cast<pg_op>(op.release());
     PG_CATCH
 }
@@ -1348,6 +1620,12 @@ int pg_op_set_mode(pg_op h, int mode)
     PG_TRY Op* op = as_op(h);
     PG_REQUIRE(mode == PG_MODE_BATCHED || mode == PG_MODE_PLAIN, PG_ERR_ARG, "bad mode");
     op->mode = mode;
+    if (op->kind == K_PCSENSE) {
+        // the per-shot operators are internal: they follow the composite
+        PcSenseOp* pc = static_cast<PcSenseOp*>(op);
+        for (Op* o : pc->A) o->mode = mode;
+        for (Op* o : pc->G) o->mode = mode;
+    }
     PG_CATCH
 }
 

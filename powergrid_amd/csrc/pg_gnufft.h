@@ -36,15 +36,21 @@ struct Gnufft {
     float* dap = nullptr;
     // FFT plans for the three grid lengths
     FftPlan1D px, py, pz;
-    // 2-D batched spread over 4x4 patches: blocks of 16 wavefront items
-    //   witems[16*k + w] = {patch or -1, entry begin, entry end, 0}
-    // and the per-block group size bgs[k] (wavefronts sharing a patch)
+    // 2-D batched spread over 4x4 patches: blocks of 8 wavefront items
+    //   witems[8*k + w] = {patch or -1, entry begin, entry end, slot or -1}
+    // and the per-block group size bgs[k] (wavefronts sharing a patch);
+    // patches with more than 512 entries are summed in parts into partial
+    // slots and combined by comb[] = {patch, first slot, parts, 0}
     int4* witems = nullptr;
     int* bgs = nullptr;
     int n_sblocks = 0;
+    int4* comb = nullptr;
+    int n_comb = 0;
+    int n_slots = 0;
+    float2* partial = nullptr;  // ceil(cap_batch/64) * n_slots * 16 * 64
     int* entries = nullptr;  // sample indices, grouped by patch
-    // per entry k: weights of the patch's 4 x-points (ent_w[8k..8k+3]) and
-    // 4 y-points (ent_w[8k+4..8k+7]) for that entry's sample, 0 outside its
+    // per entry k: the 16 weights wx*wy of the patch's points for that
+    // entry's sample (ent_w[16k + 4a + q], point (a, q)), 0 outside its
     // clamped window
     float* ent_w = nullptr;
     // 2-D interp through LDS: samples binned by the kIT x kIT tile of their

@@ -27,6 +27,7 @@
 //            gridding.cpp:102-106).
 #include "pg_gnufft.h"
 #include "pg_internal.h"
+#include "pg_fftreg.h"
 
 namespace {
 
@@ -119,30 +120,27 @@ __global__ void __launch_bounds__(256) k_interp(
 }
 
 // ------------------------------------------------- 2-D interp through LDS --
-// Sum of a float over the 64 lanes of a wavefront with DPP row operations;
-// the total ends in lane 63.
-#define PG_DPP_ADD(v, CTRL, ROWMASK)                                                        \
-    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), (CTRL), (ROWMASK), \
-                                                    0xF, false))
-__device__ __forceinline__ float wave_sum_to_63(float v)
+// Sum over the 8 lanes of each aligned 8-lane group with DPP; the total ends
+// in the group's lane 7 (row_half_mirror pairs lane k with lane 7 - k).
+__device__ __forceinline__ float group8_sum_to_7(float v)
 {
-    PG_DPP_ADD(v, 0xB1, 0xF);   // quad_perm [1,0,3,2]
-    PG_DPP_ADD(v, 0x4E, 0xF);   // quad_perm [2,3,0,1]
-    PG_DPP_ADD(v, 0x141, 0xF);  // row_half_mirror
-    PG_DPP_ADD(v, 0x140, 0xF);  // row_mirror
-    PG_DPP_ADD(v, 0x142, 0xA);  // row_bcast:15 into rows 1 and 3
-    PG_DPP_ADD(v, 0x143, 0xC);  // row_bcast:31 into rows 2 and 3
+#define PG_DPP_ADD8(v, CTRL)                                                                 \
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), (CTRL), 0xF, 0xF, \
+                                                    false))
+    PG_DPP_ADD8(v, 0xB1);   // quad_perm [1,0,3,2]
+    PG_DPP_ADD8(v, 0x4E);   // quad_perm [2,3,0,1]
+    PG_DPP_ADD8(v, 0x141);  // row_half_mirror
+#undef PG_DPP_ADD8
     return v;
 }
-#undef PG_DPP_ADD
 
 // One block per (work item, coil): the kIR x kIR grid region under the
-// item's tile is staged in LDS for all L segments of the coil, segment
-// fastest ([x][y][l], LP slots per point), together with the item's sample
-// records and their AA weights.  Each of the 8 wavefronts then takes two
-// samples at a time with lane = (segment l8, y tap j):
-//   acc = sum_x wx[x] * g_l(ox + x, oy + j),  value = sum_lanes acc * wy[j] * AA_l
-// (the 8 grid reads of a sample are each 512 contiguous bytes of LDS).
+// item's tile is staged in LDS for all L segments of the coil ([x][l][y],
+// y fastest, LP segment slots), together with the item's sample records and
+// their AA weights.  Each wavefront then takes 8 samples at a time with
+// lane = (sample s, segment l):
+//   acc_l = sum_x wx[x] sum_y wy[y] g_l(ox + x, oy + y),  value = sum_l AA_l acc_l
+// the segment sum being a 3-step DPP reduction inside each 8-lane group.
 // Blocks are ordered coil-fastest so that coil c stays on XCD c % 8.
 constexpr int kInterpWaves = 8;
 
@@ -154,18 +152,18 @@ __global__ void __launch_bounds__(64 * kInterpWaves) k_interp_lds(
 {
     constexpr int R = pg::kIR;
     constexpr int CH = pg::kIChunk;
-    __shared__ float2 tile[R * R * LP];
+    __shared__ float2 tile[R * LP * R];
     __shared__ float4 srec[CH * 5];
     __shared__ float saa[CH * LP];
     const int c = blockIdx.x % nc;
     const int4 it = items[blockIdx.x / nc];
-    const int n = it.z - it.y;  // <= CH
+    const int n = it.z - it.y;  // 1..CH
     const int tx = it.x / ity, ty = it.x - (it.x / ity) * ity;
     const int rx0 = tx * pg::kIT - padl, ry0 = ty * pg::kIT - padl;
     const float2* gc = grids + (long long)c * L * G;
-    for (int e = threadIdx.x; e < R * R * LP; e += blockDim.x) {
-        const int l = e % LP, r = e / LP;
-        const int xr = r / R, yr = r - (r / R) * R;
+    for (int e = threadIdx.x; e < R * LP * R; e += blockDim.x) {
+        const int yr = e % R, t = e / R;
+        const int l = t % LP, xr = t / LP;
         const int x = rx0 + xr, y = ry0 + yr;
         const bool ok = l < L && x >= 0 && x < Gx && y >= 0 && y < Gy;
         const int xc = min(max(x, 0), Gx - 1), yc = min(max(y, 0), Gy - 1);
@@ -183,55 +181,42 @@ __global__ void __launch_bounds__(64 * kInterpWaves) k_interp_lds(
     __syncthreads();
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // scalar sample loop
     const int lane = threadIdx.x & 63;
-    const int l8 = lane & 7, j = lane >> 3;
-
-    // value of staged sample sl, summed into lane 63
-    auto one = [&](int sl, float& tr, float& ti) {
+    const int s8 = lane >> 3, l8 = lane & 7;
+    for (int base = wv * 8; base < n; base += kInterpWaves * 8) {
+        const int sl = min(base + s8, n - 1);
         const float4* rp = srec + sl * 5;
         const float4 h = rp[0];
         const int ox = __float_as_int(h.y), oy = __float_as_int(h.z);
-        const float4 xa = rp[1], xb = rp[2];
+        const float4 xa = rp[1], xb = rp[2], ya = rp[3], yb = rp[4];
         const float wx[8] = {xa.x, xa.y, xa.z, xa.w, xb.x, xb.y, xb.z, xb.w};
-        const float wyj = reinterpret_cast<const float*>(rp + 3)[j];
-        tr = 0.f;
-        ti = 0.f;
+        const float wy[8] = {ya.x, ya.y, ya.z, ya.w, yb.x, yb.y, yb.z, yb.w};
+        float tr = 0.f, ti = 0.f;
 #pragma unroll
         for (int g = 0; g < LP / 8; ++g) {
             const int l = g * 8 + l8;
-            const float2* tp = tile + ((ox * R + oy + j) * LP + l);
+            const float2* tp = tile + ((ox * LP + l) * R + oy);
             float ar = 0.f, ai = 0.f;
 #pragma unroll
             for (int x = 0; x < 8; ++x) {
-                const float2 v = tp[x * R * LP];
-                ar += wx[x] * v.x;
-                ai += wx[x] * v.y;
+                const float2* row = tp + x * LP * R;
+                float rr = 0.f, ri = 0.f;
+#pragma unroll
+                for (int y = 0; y < 8; ++y) {
+                    const float2 v = row[y];
+                    rr += wy[y] * v.x;
+                    ri += wy[y] * v.y;
+                }
+                ar += wx[x] * rr;
+                ai += wx[x] * ri;
             }
-            const float wa = wyj * saa[sl * LP + l];
-            tr += wa * ar;
-            ti += wa * ai;
+            const float a = saa[sl * LP + l];
+            tr += a * ar;
+            ti += a * ai;
         }
-    };
-    constexpr int NW = kInterpWaves;
-    int sl = wv;
-    for (; sl + NW < n; sl += 2 * NW) {
-        float ar, ai, br, bi;
-        one(sl, ar, ai);
-        one(sl + NW, br, bi);
-        ar = wave_sum_to_63(ar);
-        ai = wave_sum_to_63(ai);
-        br = wave_sum_to_63(br);
-        bi = wave_sum_to_63(bi);
-        if (lane == 63) {
-            out[c * out_cstride + __float_as_int(srec[sl * 5].x)] = make_float2(ar, ai);
-            out[c * out_cstride + __float_as_int(srec[(sl + NW) * 5].x)] = make_float2(br, bi);
-        }
-    }
-    if (sl < n) {
-        float ar, ai;
-        one(sl, ar, ai);
-        ar = wave_sum_to_63(ar);
-        ai = wave_sum_to_63(ai);
-        if (lane == 63) out[c * out_cstride + __float_as_int(srec[sl * 5].x)] = make_float2(ar, ai);
+        tr = group8_sum_to_7(tr);
+        ti = group8_sum_to_7(ti);
+        if (l8 == 7 && base + s8 < n)
+            out[c * out_cstride + __float_as_int(h.x)] = make_float2(tr, ti);
     }
 }
 
@@ -271,7 +256,8 @@ __device__ __forceinline__ float2 spread_value(const float2* __restrict__ in, lo
 
 // V[i*B + b] = value of sample i in grid b, formed once per spread so that the
 // spread kernels read one contiguous row of B values per sample.  Block:
-// 4 samples x 64 lanes over b.
+// 4 samples x 64 lanes over b.  (Inside the PCG the update kernel forms the
+// next iteration's values itself, with the same arithmetic.)
 __global__ void __launch_bounds__(256) k_spread_values(
     const float2* __restrict__ in, long long in_cstride, const float2* __restrict__ resid_y,
     const double2* __restrict__ ax64, const float* __restrict__ Wd, int L,
@@ -286,25 +272,28 @@ __global__ void __launch_bounds__(256) k_spread_values(
 // ------------------------------------------------- 2-D patch spread --------
 // A block of 8 wavefronts; lane = grid (coil x segment).  Wavefront w runs
 // the work item witems[8 * block + w] = {patch (-1: none), entry begin,
-// entry end, 0}: it accumulates the 4x4 patch in registers over its entries.
-// Entry k carries its sample index and the weights of the patch's 4 x-points
-// and 4 y-points (zero outside the sample's clamped window), precomputed
-// with the trajectory; entries are staged 64 at a time one per lane and
-// broadcast with readlane, so the only load per entry is the sample's
-// contiguous row of grid values (the next 8 rows are loaded while the
-// current 8 are accumulated).  A patch too heavy for one wavefront is
-// split over gs = 2..8 consecutive wavefronts of a block (bgs[block] = gs,
-// the same for all groups of a block), whose sums are added through LDS in
-// a fixed tree order; the group's first wavefront writes the patch.  No
-// atomics, no zero-fill (every patch has an item), deterministic.
+// entry end, slot}: it accumulates the 4x4 patch in registers over its
+// entries.  Entry k carries its sample index and the 16 weights wx*wy of the
+// patch's points (zero outside the sample's clamped window), precomputed
+// with the trajectory.  Per stage of up to 64 entries a wavefront stages the
+// weights in LDS (read back as broadcasts) and the indices one per lane
+// (readlane); the only global load per entry is the sample's contiguous row
+// of grid values, and the next 8 rows are in flight while the current 8 are
+// summed.  A patch of 65..512 entries is split over gs = 2..8 consecutive
+// wavefronts of a block (bgs[block] = gs, the same for all groups of a
+// block), whose sums are added through LDS in a fixed tree order; the
+// group's first wavefront writes the patch, or its partial slot when the
+// patch is one of the parts of a heavier patch (added by
+// k_spread_heavy_combine).  No atomics, no zero-fill, deterministic.
 constexpr int kSpreadWaves = 8;
 
 __global__ void __launch_bounds__(64 * kSpreadWaves) k_spread_patch(
     const int4* __restrict__ witems, const int* __restrict__ bgs, const int* __restrict__ entries,
     const float4* __restrict__ ew, int npy, const float2* __restrict__ V, int B,
-    float2* __restrict__ grids, long long G, int Gy)
+    float2* __restrict__ grids, long long G, int Gy, float2* __restrict__ partial, int n_slots)
 {
     __shared__ float2 red[kSpreadWaves / 2][16][64];
+    __shared__ float4 sw[kSpreadWaves][64][4];
     // wave-uniform (scalar) work item: its loop bounds live in SGPRs
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int4 it = witems[blockIdx.x * kSpreadWaves + wv];
@@ -318,15 +307,16 @@ __global__ void __launch_bounds__(64 * kSpreadWaves) k_spread_patch(
     for (int q = 0; q < 16; ++q) acc[q] = make_float2(0.f, 0.f);
     const float2* Vb = V + bb;
     constexpr int U = 8;  // entries per group
+    float4(*mw)[4] = sw[wv];
     for (int s0 = it.y; s0 < it.z; s0 += 64) {
         const int n = min(64, it.z - s0);
         int ei = 0;
-        float4 ex = make_float4(0.f, 0.f, 0.f, 0.f), ey = ex;
         if (lane < n) {
             ei = entries[s0 + lane];
-            ex = ew[2 * (s0 + lane)];
-            ey = ew[2 * (s0 + lane) + 1];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) mw[lane][r] = ew[4 * (s0 + lane) + r];
         }
+        pgf::wave_lds_sync();
         // rows of group k0; entries past n load the last entry's row, which
         // is zeroed with a bit mask where it is used (no branch around the
         // loads, so that the next group's loads stay in flight while this
@@ -345,34 +335,29 @@ __global__ void __launch_bounds__(64 * kSpreadWaves) k_spread_patch(
                 const int m = (k0 + u < n) ? -1 : 0;
                 const float2 vu = make_float2(__int_as_float(__float_as_int(v[u].x) & m),
                                               __int_as_float(__float_as_int(v[u].y) & m));
-                const int k = min(k0 + u, 63);  // entries past n have v = 0
-                float wx[4], wy[4];
-                wx[0] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(ex.x), k));
-                wx[1] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(ex.y), k));
-                wx[2] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(ex.z), k));
-                wx[3] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(ex.w), k));
-                wy[0] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(ey.x), k));
-                wy[1] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(ey.y), k));
-                wy[2] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(ey.z), k));
-                wy[3] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(ey.w), k));
-                float2 tq[4];
+                const int k = min(k0 + u, 63);
+                float w[16];
 #pragma unroll
-                for (int q = 0; q < 4; ++q) tq[q] = make_float2(wy[q] * vu.x, wy[q] * vu.y);
+                for (int r = 0; r < 4; ++r) {
+                    const float4 q4 = mw[k][r];
+                    w[4 * r] = q4.x;
+                    w[4 * r + 1] = q4.y;
+                    w[4 * r + 2] = q4.z;
+                    w[4 * r + 3] = q4.w;
+                }
 #pragma unroll
-                for (int a = 0; a < 4; ++a)
-#pragma unroll
-                    for (int q = 0; q < 4; ++q) {
-                        acc[a * 4 + q].x += wx[a] * tq[q].x;
-                        acc[a * 4 + q].y += wx[a] * tq[q].y;
-                    }
+                for (int q = 0; q < 16; ++q) {
+                    acc[q].x += w[q] * vu.x;
+                    acc[q].y += w[q] * vu.y;
+                }
             }
         };
-        // two row buffers in turn: one group in flight while the other is summed
+        // two row buffers in turn (the scheduling barriers keep each group's
+        // loads ahead of the sums; the second group is summed
+        // unconditionally, masked past n, so that its loads are not sunk next
+        // to their use)
         float2 va[U], vb[U];
         load_group(0, va);
-        // (the scheduling barriers keep each group's loads ahead of the sums;
-        // the second group is summed unconditionally, masked past n, so that
-        // its loads are not sunk next to their use)
         for (int k0 = 0; k0 < n; k0 += 2 * U) {
             load_group(k0 + U, vb);
             __builtin_amdgcn_sched_barrier(0);
@@ -381,6 +366,7 @@ __global__ void __launch_bounds__(64 * kSpreadWaves) k_spread_patch(
             __builtin_amdgcn_sched_barrier(0);
             sum_group(k0 + U, vb);
         }
+        pgf::wave_lds_sync();  // the weights of the next stage replace these
     }
     // groups of gs wavefronts: tree sum, wavefront w + r into w
     for (int r = 1; r < gs; r *= 2) {
@@ -398,12 +384,161 @@ __global__ void __launch_bounds__(64 * kSpreadWaves) k_spread_patch(
             }
     }
     if (!active || it.x < 0 || (wv & (gs - 1)) != 0) return;
+    if (it.w >= 0) {
+        float2* pp = partial + ((long long)blockIdx.y * n_slots + it.w) * 16 * 64 + lane;
+#pragma unroll
+        for (int q = 0; q < 16; ++q) pp[q * 64] = acc[q];
+        return;
+    }
     const int px0 = (it.x / npy) * pg::kPatch, py0 = (it.x % npy) * pg::kPatch;
     float2* g = grids + (long long)b * G + (long long)px0 * Gy + py0;
 #pragma unroll
     for (int a = 0; a < pg::kPatch; ++a)
 #pragma unroll
         for (int q = 0; q < pg::kPatch; ++q) g[(long long)a * Gy + q] = acc[a * 4 + q];
+}
+
+// The same work list on the matrix cores (B a multiple of 4).  For a chunk of
+// 4 entries the patch update is a product
+//   acc[16 points][128 floats] += W[16 points][4 entries] * V[4 entries][128 floats]
+// of the entries' weights and their rows of grid values (re and im as
+// separate columns), done as 8 v_mfma_f32_16x16x4f32, one per 16-column
+// tile.  Tile t, column j holds row float 8j + t, so that each lane's share
+// of a row is 8 consecutive floats (two 16-byte loads).  Lane layouts of the
+// 16x16x4 f32 product (one block): A[i][k] in lane i + 16k, B[k][j] in lane
+// j + 16k, D[i][j] in lane j + 16(i/4), register i % 4.
+typedef float pg_f4 __attribute__((ext_vector_type(4)));
+
+__global__ void __launch_bounds__(64 * kSpreadWaves) k_spread_patch_mfma(
+    const int4* __restrict__ witems, const int* __restrict__ bgs, const int* __restrict__ entries,
+    const float4* __restrict__ ew, int npy, const float2* __restrict__ V, int B,
+    float2* __restrict__ grids, long long G, int Gy, float2* __restrict__ partial, int n_slots)
+{
+    __shared__ float red[kSpreadWaves / 2][32][64];
+    __shared__ float sw[kSpreadWaves][64][16];
+    __shared__ int si[kSpreadWaves][64];
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int4 it = witems[blockIdx.x * kSpreadWaves + wv];
+    const int gs = bgs[blockIdx.x];
+    const int cw = min(64, B - 64 * (int)blockIdx.y);  // grids in this chunk (multiple of 4)
+    const int kq = lane >> 4, jq = lane & 15;          // MFMA k (entry) and j (column) of this lane
+    const bool colok = 4 * jq < cw;                     // this lane's 4 grids exist
+    // this lane's 8 floats of a row: grids 4jq .. 4jq+3 of the chunk
+    const float* Vc = reinterpret_cast<const float*>(V) + 2 * 64 * (long long)blockIdx.y +
+                      8 * (colok ? jq : 0);
+    const long long rowf = 2LL * B;  // floats per sample row
+
+    pg_f4 acc[8];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) acc[t] = pg_f4{0.f, 0.f, 0.f, 0.f};
+    float(*mw)[16] = sw[wv];
+    int* mi = si[wv];
+    for (int s0 = it.y; s0 < it.z; s0 += 64) {
+        const int n = min(64, it.z - s0);
+        {
+            // stage weights and indices; entries past n get zero weights
+            const bool ok = lane < n;
+            const int e = ok ? s0 + lane : s0;
+            mi[lane] = entries[e];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const float4 q = ew[4 * e + r];
+                mw[lane][4 * r] = ok ? q.x : 0.f;
+                mw[lane][4 * r + 1] = ok ? q.y : 0.f;
+                mw[lane][4 * r + 2] = ok ? q.z : 0.f;
+                mw[lane][4 * r + 3] = ok ? q.w : 0.f;
+            }
+        }
+        pgf::wave_lds_sync();
+        // rows of chunk k0 for this lane (entry k0 + kq; past n: zeroed)
+        auto load_chunk = [&](int k0, float4 (&v)[2]) {
+            const int k = min(k0 + kq, 63);
+            const float4* src = reinterpret_cast<const float4*>(Vc + rowf * mi[k]);
+            v[0] = src[0];
+            v[1] = src[1];
+        };
+        auto mma_chunk = [&](int k0, float4 (&v)[2]) {
+            const int m = (k0 + kq < n && colok) ? -1 : 0;
+            const float a = mw[min(k0 + kq, 63)][jq];  // A[point jq][entry kq]
+            float b[8] = {v[0].x, v[0].y, v[0].z, v[0].w, v[1].x, v[1].y, v[1].z, v[1].w};
+#pragma unroll
+            for (int t = 0; t < 8; ++t) {
+                const float bt = __int_as_float(__float_as_int(b[t]) & m);
+                acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bt, acc[t], 0, 0, 0);
+            }
+        };
+        float4 va[2], vb[2];
+        load_chunk(0, va);
+        for (int k0 = 0; k0 < n; k0 += 8) {
+            load_chunk(k0 + 4, vb);
+            __builtin_amdgcn_sched_barrier(0);
+            mma_chunk(k0, va);
+            load_chunk(k0 + 8, va);
+            __builtin_amdgcn_sched_barrier(0);
+            mma_chunk(k0 + 4, vb);
+        }
+        pgf::wave_lds_sync();  // the next stage's weights replace these
+    }
+    // groups of gs wavefronts: tree sum, wavefront w + r into w
+    for (int r = 1; r < gs; r *= 2) {
+        __syncthreads();
+        if ((wv & (2 * r - 1)) == r)
+#pragma unroll
+            for (int t = 0; t < 8; ++t)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) red[wv >> 1][4 * t + q][lane] = acc[t][q];
+        __syncthreads();
+        if ((wv & (2 * r - 1)) == 0)
+#pragma unroll
+            for (int t = 0; t < 8; ++t)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) acc[t][q] += red[(wv + r) >> 1][4 * t + q][lane];
+    }
+    if (it.x < 0 || (wv & (gs - 1)) != 0 || !colok) return;
+    // lane holds points p = 4 kq + q and grids b = 4 jq + u (re: tile 2u, im: tile 2u+1)
+    const int b0 = 64 * blockIdx.y + 4 * jq;
+    if (it.w >= 0) {
+        float2* pp = partial + ((long long)blockIdx.y * n_slots + it.w) * 16 * 64;
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                pp[(4 * kq + q) * 64 + 4 * jq + u] = make_float2(acc[2 * u][q], acc[2 * u + 1][q]);
+        return;
+    }
+    const int px0 = (it.x / npy) * pg::kPatch, py0 = (it.x % npy) * pg::kPatch;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int p = 4 * kq + q;
+        const long long off = (long long)(px0 + p / pg::kPatch) * Gy + py0 + p % pg::kPatch;
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            grids[(long long)(b0 + u) * G + off] = make_float2(acc[2 * u][q], acc[2 * u + 1][q]);
+    }
+}
+
+// patches summed in parts: add the parts' slots in slot order; block
+// (patch, grid chunk, point)
+__global__ void __launch_bounds__(64) k_spread_heavy_combine(const int4* __restrict__ comb, int npy,
+                                                             const float2* __restrict__ partial,
+                                                             int n_slots, int B,
+                                                             float2* __restrict__ grids,
+                                                             long long G, int Gy)
+{
+    const int4 cb = comb[blockIdx.x];
+    const int q = blockIdx.z;
+    const int b = blockIdx.y * 64 + threadIdx.x;
+    if (b >= B) return;
+    const float2* pp = partial + ((long long)blockIdx.y * n_slots + cb.y) * 16 * 64 + q * 64 +
+                       threadIdx.x;
+    float2 t = pp[0];
+    for (int k = 1; k < cb.z; ++k) {
+        const float2 v = pp[(long long)k * 16 * 64];
+        t.x += v.x;
+        t.y += v.y;
+    }
+    const int px0 = (cb.x / npy) * pg::kPatch, py0 = (cb.x % npy) * pg::kPatch;
+    grids[(long long)b * G + (long long)(px0 + q / pg::kPatch) * Gy + py0 + q % pg::kPatch] = t;
 }
 
 // ------------------------------------------- 3-D tile spread (B >= 16) ----
@@ -539,7 +674,7 @@ void interp_dispatch(const pg::Gnufft& G, const float2* grids, int nc, int L, co
 template <int NT, bool D3>
 void spread_dispatch(const pg::Gnufft& G, const float2* in, long long in_cstride, int nc, int L,
                      const float* AA, const float2* resid_y, const double2* ax64, const float* Wd,
-                     float2* grids, hipStream_t s)
+                     float2* grids, bool vals_ready, hipStream_t s)
 {
     const pg::Geometry& g = G.geo;
     const int B = nc * L;
@@ -552,14 +687,23 @@ void spread_dispatch(const pg::Gnufft& G, const float2* in, long long in_cstride
         return;
     }
     // per-sample values of every grid, once
-    hipLaunchKernelGGL(k_spread_values, dim3((unsigned)((G.M + 3) / 4)), dim3(256), 0, s, in,
-                       in_cstride, resid_y, ax64, Wd, L, AA, G.M, B, G.vals);
+    if (!vals_ready)
+        hipLaunchKernelGGL(k_spread_values, dim3((unsigned)((G.M + 3) / 4)), dim3(256), 0, s, in,
+                           in_cstride, resid_y, ax64, Wd, L, AA, G.M, B, G.vals);
     if (!D3) {
         const int nch = (B + 63) / 64;
         const int npy = g.Gy / pg::kPatch;
-        hipLaunchKernelGGL(k_spread_patch, dim3(G.n_sblocks, nch), dim3(64 * kSpreadWaves), 0, s,
-                           G.witems, G.bgs, G.entries, (const float4*)G.ent_w, npy, G.vals, B,
-                           grids, g.G(), g.Gy);
+        if (B % 4 == 0)
+            hipLaunchKernelGGL(k_spread_patch_mfma, dim3(G.n_sblocks, nch), dim3(64 * kSpreadWaves),
+                               0, s, G.witems, G.bgs, G.entries, (const float4*)G.ent_w, npy, G.vals,
+                               B, grids, g.G(), g.Gy, G.partial, G.n_slots);
+        else
+            hipLaunchKernelGGL(k_spread_patch, dim3(G.n_sblocks, nch), dim3(64 * kSpreadWaves), 0, s,
+                               G.witems, G.bgs, G.entries, (const float4*)G.ent_w, npy, G.vals, B,
+                               grids, g.G(), g.Gy, G.partial, G.n_slots);
+        if (G.n_comb > 0)
+            hipLaunchKernelGGL(k_spread_heavy_combine, dim3(G.n_comb, nch, 16), dim3(64), 0, s, G.comb,
+                               npy, G.partial, G.n_slots, B, grids, g.G(), g.Gy);
         return;
     }
     constexpr int BC = 16;
@@ -591,19 +735,27 @@ void launch_interp(const pg::Gnufft& G, const float2* grids, int nc, int L, cons
     }
 }
 
+void launch_spread_values(const pg::Gnufft& G, const float2* in, long long in_cstride, int nc,
+                          int L, const float* AA, const float2* resid_y, const double2* ax64,
+                          const float* Wd, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_spread_values, dim3((unsigned)((G.M + 3) / 4)), dim3(256), 0, s, in,
+                       in_cstride, resid_y, ax64, Wd, L, AA, G.M, nc * L, G.vals);
+}
+
 void launch_spread(const pg::Gnufft& G, const float2* in, long long in_cstride, int nc, int L,
                    const float* AA, const float2* resid_y, const double2* ax64, const float* Wd,
-                   float2* grids, hipStream_t s)
+                   float2* grids, bool vals_ready, hipStream_t s)
 {
     const bool d3 = G.geo.dims == 3;
     switch (G.geo.ntaps) {
     case 8:
-        d3 ? spread_dispatch<8, true>(G, in, in_cstride, nc, L, AA, resid_y, ax64, Wd, grids, s)
-           : spread_dispatch<8, false>(G, in, in_cstride, nc, L, AA, resid_y, ax64, Wd, grids, s);
+        d3 ? spread_dispatch<8, true>(G, in, in_cstride, nc, L, AA, resid_y, ax64, Wd, grids, vals_ready, s)
+           : spread_dispatch<8, false>(G, in, in_cstride, nc, L, AA, resid_y, ax64, Wd, grids, vals_ready, s);
         break;
     case 6:
-        d3 ? spread_dispatch<6, true>(G, in, in_cstride, nc, L, AA, resid_y, ax64, Wd, grids, s)
-           : spread_dispatch<6, false>(G, in, in_cstride, nc, L, AA, resid_y, ax64, Wd, grids, s);
+        d3 ? spread_dispatch<6, true>(G, in, in_cstride, nc, L, AA, resid_y, ax64, Wd, grids, vals_ready, s)
+           : spread_dispatch<6, false>(G, in, in_cstride, nc, L, AA, resid_y, ax64, Wd, grids, vals_ready, s);
         break;
     default:
         throw pg::Error{PG_ERR_UNSUPPORTED, "kernel width * gridos must give 6 or 8 taps"};

@@ -487,19 +487,35 @@ __global__ void k_cg_ls(int j, Pen p, const float2* ddir, const float2* x, pgv::
                  (long long)gridDim.x * blockDim.x, cp.l[j], blockIdx.x);
 }
 
-// update: the last Newton step (all three without a penalty), then
-// Ax += step Adir, x += step ddir in FP64 (x32: the FP32 copy for the
-// penalty stencils); partial ||y - Ax||^2 (:155-162)
-__global__ void k_cg_update(double2* Ax, const float2* Adir, const float2* y, long long M,
-                            double2* x, float2* x32, const float2* ddir, long long N, int have_pen,
-                            pgv::CgParts cp, CgScal* sc)
+// update: the last Newton step, then Ax += step Adir, x += step ddir in FP64
+// (x32: the FP32 copy for the penalty stencils); partial ||y - Ax||^2
+// (:155-162).  pen_mode 0 (no penalty) and 2 (quadratic penalty) take all
+// three Newton steps here: for a quadratic penalty Denom(ddir, x + s ddir)
+// is beta |C ddir|^2 for every s and Re<ddir, Gradient(x + s ddir)> =
+// pdot(0) + Re(s) beta |C ddir|^2, both from the step-0 penalty partials.
+// pen_mode 1 (TV) takes the third step from the partials at steps[2].  With
+// vf.V set, also the next adjoint's spread values W (y - Ax) * AA_l from the
+// new Ax (the arithmetic of the spread's own value pass).
+__global__ void k_cg_update(double2* Ax, const float2* Adir, const float2* y, const float* W,
+                            long long M, double2* x, float2* x32, const float2* ddir, long long N,
+                            int pen_mode, pgv::CgParts cp, pgv::VFill vf, CgScal* sc)
 {
     if (sc->stop) return;
     double2 step;
     int stop = 0;
     bool ok = true;
     double dAWAd = 0, dAWr = 0;
-    if (have_pen) {
+    const int have_pen = (pen_mode == 1);
+    if (pen_mode == 2) {
+        double a[2], pl[2];
+        all_sum_parts<2>(cp.a, cp.na, a);
+        all_sum_parts<2>(cp.l[0], cp.nl, pl);
+        dAWAd = a[0];
+        dAWr = a[1];
+        step = make_double2(0, 0);
+        for (int j = 1; j <= 3 && ok; ++j)
+            ok = newton_step(dAWAd, dAWr, pl[0], pl[1] + step.x * pl[0], step, stop);
+    } else if (have_pen) {
         double pl[2];
         all_sum_parts<2>(cp.l[2], cp.nl, pl);
         dAWAd = sc->dAWAd;
@@ -541,6 +557,25 @@ __global__ void k_cg_update(double2* Ax, const float2* Adir, const float2* y, lo
             Ax[i] = o;
             const double rr = (double)y[i].x - o.x, ri = (double)y[i].y - o.y;
             v[0] += rr * rr + ri * ri;
+            if (vf.V) {
+                double wr = rr, wi = ri;
+                if (W) {
+                    const double wv = W[i];
+                    wr *= wv;
+                    wi *= wv;
+                }
+                const float2 r32 = make_float2((float)wr, (float)wi);
+                const long long c = i / vf.Mc, ii = i - c * vf.Mc;
+                float2* row = vf.V + ii * ((long long)vf.nc * vf.L) + c * vf.L;
+                for (int l = 0; l < vf.L; ++l) {
+                    if (vf.AA) {
+                        const float a = vf.AA[l * vf.Mc + ii];
+                        row[l] = make_float2(a * r32.x, a * r32.y);
+                    } else {
+                        row[l] = r32;
+                    }
+                }
+            }
         }
         if (i < N) {
             const float2 d = ddir[i];
@@ -740,18 +775,21 @@ void cg_adir(const float2* Adir, const float2* y, const double2* Ax, const float
 void cg_linesearch(const PenSpec* ps, const float2* ddir, const float2* x, const CgParts& cp,
                    CgScal* sc, hipStream_t s)
 {
-    if (!ps) return;  // without a penalty the update kernel takes all three steps
+    // without a penalty, or with a quadratic one, the update kernel takes all
+    // three steps
+    if (!ps || ps->kind == 0) return;
     for (int j = 1; j <= 2; ++j)
         hipLaunchKernelGGL(k_cg_ls, dim3(cp.nl), dim3(kThreads), 0, s, j, to_pen(*ps), ddir, x, cp,
                            sc);
 }
 
-void cg_update(double2* Ax, const float2* Adir, const float2* y, long long M, double2* x,
-               float2* x32, const float2* ddir, long long N, bool have_pen, const CgParts& cp,
-               CgScal* sc, hipStream_t s)
+void cg_update(double2* Ax, const float2* Adir, const float2* y, const float* W, long long M,
+               double2* x, float2* x32, const float2* ddir, long long N, const PenSpec* ps,
+               const CgParts& cp, const VFill& vf, CgScal* sc, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_cg_update, dim3(cp.nu), dim3(kThreads), 0, s, Ax, Adir, y, M, x, x32, ddir,
-                       N, (int)have_pen, cp, sc);
+    const int pen_mode = !ps ? 0 : (ps->kind == 0 ? 2 : 1);
+    hipLaunchKernelGGL(k_cg_update, dim3(cp.nu), dim3(kThreads), 0, s, Ax, Adir, y, W, M, x, x32,
+                       ddir, N, pen_mode, cp, vf, sc);
 }
 
 void cg_finish(const CgParts& cp, CgScal* sc, double* resid, hipStream_t s)

@@ -161,6 +161,23 @@ def test_spread_patch_split_and_many_grids(orc):
     assert rel(Sg * x, yo) < 5e-6
 
 
+def test_spread_dense_centre_parts(orc):
+    # 32x32 image under a 16 x 2048 spiral: the central 4x4 patches see up to
+    # ~6000 samples and are summed in up to 12 parts
+    print("[spread dense centre]", flush=True)
+    N, nc, L = 32, 4, 4
+    kx, ky, kz, t, fm, x, M = _ts_case(N, nshot=16, ns=2048, L=L)
+    sens = S.coil_maps(N, nc).astype(np.complex64)
+    Gg = P.Gnufft(M, 2.0, N, N, 1, kx, ky, kz)
+    Sg = P.SENSE(P.TimeSegmentation(Gg, fm, t, M, N * N, L), sens, M, N * N, nc)
+    Gd = orc.Gnufft(M, 2.0, N, N, 1, kx, ky, kz, prec="d")
+    Sd = orc.SENSE(orc.TimeSegmentation(Gd, fm, t, M, N * N, L), sens.ravel(), M, N * N, nc)
+    rng = np.random.default_rng(5)
+    yo = (rng.standard_normal(nc * M) + 1j * rng.standard_normal(nc * M)).astype(np.complex64)
+    assert rel(Sg / yo, Sd / yo) < 5e-6
+    assert rel(Gg / yo[:M], Gd / yo[:M]) < 5e-6
+
+
 def test_gdft_matches_oracle(orc):
     print("[gdft]", flush=True)
     N, nc = 32, 3
@@ -257,6 +274,51 @@ def test_pcg_plain_ops_match_batched():
     xp, ip = P.reconSolve(y, Sg, R, niter=niter, mode=P.CG_HOST)
     assert ib["iterations"] == ip["iterations"] == niter
     assert rel(xp, xb) < 1e-5
+
+
+def test_pcg_solve_cache_reuse_and_rebuild():
+    # a solve keeps its work buffers and graph on the operator; the results of
+    # reused, re-keyed and rebuilt solves equal those of fresh operators
+    print("[pcg solve cache]", flush=True)
+    N, nc, L, niter = 32, 2, 3, 6
+    kx, ky, kz, t, fm, x, M, sens = _recon_case(N, nc, L, niter)
+
+    def ops(nc_=nc):
+        G = P.Gnufft(M, 2.0, N, N, 1, kx, ky, kz)
+        T = P.TimeSegmentation(G, fm, t, M, N * N, L)
+        s_ = S.coil_maps(N, nc_).astype(np.complex64)
+        return G, T, P.SENSE(T, s_, M, N * N, nc_)
+
+    G, T, Sg = ops()
+    R = P.QuadPenalty(N, N, 1, 1e-2)
+    y = Sg * x
+    yd = P.DeviceArray.from_host(y)
+    x1, _ = P.reconSolve(yd, Sg, R, niter=niter)
+    x2, _ = P.reconSolve(yd, Sg, R, niter=niter)  # cached graph
+    np.testing.assert_array_equal(x1.to_host(), x2.to_host())
+
+    def fresh(ydata, R_, n_):
+        _, _, Sf = ops()
+        xf, _ = P.reconSolve(ydata, Sf, R_, niter=n_)
+        return xf.to_host() if hasattr(xf, "to_host") else xf
+
+    y2 = (y * np.exp(0.3j)).astype(np.complex64)
+    yd.upload(y2)  # same buffer, new data: the cached graph reads it
+    x3, _ = P.reconSolve(yd, Sg, R, niter=niter)
+    np.testing.assert_array_equal(x3.to_host(), fresh(y2, R, niter))
+    x4, i4 = P.reconSolve(yd, Sg, R, niter=4)  # new key: iterations
+    assert i4["iterations"] == 4
+    np.testing.assert_array_equal(x4.to_host(), fresh(y2, R, 4))
+    R2 = P.QuadPenalty(N, N, 1, 3e-2)  # new key: penalty
+    x5, _ = P.reconSolve(yd, Sg, R2, niter=niter)
+    np.testing.assert_array_equal(x5.to_host(), fresh(y2, R2, niter))
+    # a wider composite over the same NUFFT grows its workspaces: the cached
+    # graph of Sg is rebuilt on its next solve
+    s4 = S.coil_maps(N, 4).astype(np.complex64)
+    Sbig = P.SENSE(T, s4, M, N * N, 4)
+    _ = Sbig * x
+    x6, _ = P.reconSolve(yd, Sg, R, niter=niter)
+    np.testing.assert_array_equal(x6.to_host(), x3.to_host())
 
 
 def test_pcg_zero_data_exits_early():

@@ -257,3 +257,71 @@ def test_recon_small_sense_ts_reduces_residual(orc, prec):
     xr, n, res = orc.reconSolve(y, Sg, R, 16, 16, 1, 15)
     assert n == 15
     assert res[-1] < 0.05 * np.linalg.norm(y)
+
+
+# ------------------------------------------------------------- pcSENSE --
+def _pcs_case(Ns=3, Nd=384, N=16, nc=2, L=3, seed=0):
+    rng = np.random.default_rng(seed)
+    kx, ky, kz = S.spiral(N, Ns, Nd)
+    t = S.readout_times(Ns, Nd, dwell=2.5e-5, te=5e-4)
+    fm = S.field_map(N, peak_hz=100.0).astype(np.float32)
+    sens = S.coil_maps(N, nc).astype(np.complex64)
+    phase = rng.uniform(-np.pi, np.pi, (Ns, N * N)).astype(np.float32)
+    return kx, ky, kz, t, fm, sens, phase
+
+
+def test_pcsense_one_shot_zero_phase_is_sense_ts(orc):
+    # pcSenseTimeSeg.cpp:129-171 with Ns = 1 and P = 0 is SENSE(TimeSegmentation)
+    kx, ky, kz, t, fm, sens, phase = _pcs_case(Ns=1)
+    N, nc, L, M = 16, 2, 3, len(kx)
+    for prec in "fd":
+        A = orc.PcSenseTimeSeg(kx, ky, kz, N, N, 1, nc, t, L, 1, sens.ravel(), fm,
+                               np.zeros(N * N, np.float32), prec=prec)
+        G = orc.Gnufft(M, 2.0, N, N, 1, kx, ky, kz, prec=prec)
+        B = orc.SENSE(orc.TimeSegmentation(G, fm, t, M, N * N, L), sens.ravel(), M, N * N, nc)
+        x = S.ellipse_phantom(N, seed=2).astype(np.complex64)
+        np.testing.assert_array_equal(A * x, B * x)
+        y = B * x
+        np.testing.assert_array_equal(A / y, B / y)
+
+
+def test_pcsense_columns_are_shot_sense_blocks(orc):
+    # column s + c*Ns of the Nd x (Ns*Nc) data is TS_s((S_c exp(-i P_s)) x), each
+    # shot with its own Gnufft/TimeSegmentation (pcSenseTimeSeg.cpp:96-115, 129-148)
+    Ns, Nd, N, nc, L = 3, 384, 16, 2, 3
+    kx, ky, kz, t, fm, sens, phase = _pcs_case(Ns, Nd, N, nc, L)
+    A = orc.PcSenseTimeSeg(kx, ky, kz, N, N, 1, nc, t, L, 1, sens.ravel(), fm, phase.ravel(),
+                           prec="d")
+    x = S.ellipse_phantom(N, seed=4)
+    y = A * x
+    assert len(y) == Nd * Ns * nc
+    rng = np.random.default_rng(9)
+    d = rng.standard_normal(len(y)) + 1j * rng.standard_normal(len(y))
+    xa = A / d
+    xa_ref = np.zeros(N * N, complex)
+    for s in range(Ns):
+        sl = slice(s * Nd, (s + 1) * Nd)
+        G = orc.Gnufft(Nd, 2.0, N, N, 1, kx[sl], ky[sl], kz[sl], prec="d")
+        T = orc.TimeSegmentation(G, fm, t[sl], Nd, N * N, L)
+        ss = (sens.astype(complex) * np.exp(-1j * phase[s].astype(float))[None, :])
+        Bs = orc.SENSE(T, ss.ravel(), Nd, N * N, nc)
+        ys = Bs * x
+        for c in range(nc):
+            col = s + c * Ns
+            np.testing.assert_allclose(y[col * Nd:(col + 1) * Nd], ys[c * Nd:(c + 1) * Nd],
+                                       rtol=0, atol=1e-12 * np.abs(ys).max())
+        ds = np.concatenate([d[(s + c * Ns) * Nd:(s + c * Ns + 1) * Nd] for c in range(nc)])
+        xa_ref += Bs / ds
+    np.testing.assert_allclose(xa, xa_ref, rtol=0, atol=1e-11 * np.abs(xa_ref).max())
+
+
+def test_pcsense_adjointness(orc):
+    Ns, Nd, N, nc, L = 2, 512, 16, 3, 4
+    kx, ky, kz, t, fm, sens, phase = _pcs_case(Ns, Nd, N, nc, L, seed=1)
+    A = orc.PcSenseTimeSeg(kx, ky, kz, N, N, 1, nc, t, L, 1, sens.ravel(), fm, phase.ravel(),
+                           prec="d")
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal(N * N) + 1j * rng.standard_normal(N * N)
+    y = rng.standard_normal(A.n_out) + 1j * rng.standard_normal(A.n_out)
+    lhs, rhs = np.vdot(y, A * x), np.vdot(A / y, x)
+    assert abs(lhs - rhs) < 1e-10 * abs(lhs)
