@@ -183,6 +183,49 @@ def c3_rate(P, S, iters, stream):
                 iters_per_s=round(info["iterations"] / dt, 3), grids_per_launch=prob.nc * prob.L)
 
 
+def direct_recon_rates(P, prob, stream, reps=5):
+    """The direct-reconstruction support on the workload's trajectory:
+    density compensation (one call, including its 3x-grid plan set-up), the
+    dcf-weighted adjoint of every coil, and the coil-combined magnitude
+    image over a 128x128x32 x 32-coil volume against the HBM peak (bytes:
+    8 per complex read, 4 per voxel written)."""
+    t0 = time.perf_counter()
+    dcf = P.calc3DDensityCompensation(prob.kx, prob.ky, prob.kz, prob.Nx, prob.Nz)
+    t_dcf = time.perf_counter() - t0
+    G = P.Gnufft(prob.M, 2.0, prob.Nx, prob.Ny, prob.Nz, prob.kx, prob.ky, prob.kz)
+    rng = np.random.default_rng(3)
+    n = prob.M * prob.nc
+    dd = P.DeviceArray.from_host(dcf)
+    yd = P.DeviceArray.from_host((rng.standard_normal(n) + 1j * rng.standard_normal(n)).astype(np.complex64))
+    P.grid_coil_images(G, dd, yd, prob.nc, stream=stream)
+    stream.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        im = P.grid_coil_images(G, dd, yd, prob.nc, stream=stream)
+    stream.synchronize()
+    t_grid = (time.perf_counter() - t0) / reps
+    nvox, nc = 128 * 128 * 32, 32
+    xd = P.DeviceArray(nvox * nc)
+    xd.upload(np.ones(nvox * nc, np.complex64))
+    P.calcSumOfSquaresImage(xd, 128, 32, 1, nc, stream=stream)
+    stream.synchronize()
+    P.profile.enable(True)
+    P.profile.reset()
+    for _ in range(reps):
+        P.calcSumOfSquaresImage(xd, 128, 32, 1, nc, stream=stream)
+    stream.synchronize()
+    cnt, ms = P.profile.query("sum_of_squares")
+    P.profile.enable(False)
+    t_sos = ms / 1e3 / max(cnt, 1)
+    gbs = (8.0 * nvox * nc + 4.0 * nvox) / t_sos / 1e9
+    del im
+    return dict(dcf_ms=round(1e3 * t_dcf, 2), grid_coil_images_ms=round(1e3 * t_grid, 3),
+                sum_of_squares={"us": round(1e6 * t_sos, 2), "achieved": round(gbs, 1),
+                                "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                "frac": round(gbs / HBM_PEAK_GBS, 3),
+                                "volume": "128x128x32 x 32 coils"})
+
+
 def nrmse(a, b):
     return float(np.linalg.norm(np.ravel(a) - np.ravel(b)) / np.linalg.norm(np.ravel(b)))
 
@@ -199,6 +242,8 @@ def main():
     ap.add_argument("--c3-iters", type=int, default=0, metavar="K",
                     help="also time K PCG iterations of the 3-D stack-of-spirals workload")
     ap.add_argument("--no-gdft", action="store_true", help="skip the Gdft roofline")
+    ap.add_argument("--no-direct", action="store_true",
+                    help="skip the density-compensation / gridded-coil-image timings")
     ap.add_argument("--counters", type=int, default=0, metavar="REPS",
                     help="hardware-counter pass (for rocprofv3 --pmc): one 256 MiB device-to-"
                          "device copy (byte-counter calibration), then REPS forward+adjoint "
@@ -329,6 +374,9 @@ def main():
     if not args.no_gdft:
         out["gdft_roofline"] = gdft_roofline(P, prob, stream)
         log(f"[bench] gdft {out['gdft_roofline']}")
+    if not args.no_direct:
+        out["direct_recon"] = direct_recon_rates(P, prob, stream)
+        log(f"[bench] direct recon {out['direct_recon']}")
     if args.c3_iters:
         out["c3"] = c3_rate(P, S, args.c3_iters, stream)
         log(f"[bench] c3 {out['c3']}")

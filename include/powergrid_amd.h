@@ -65,10 +65,12 @@ int pg_stream_synchronize(void* stream);
 /* ---- operators ---------------------------------------------------------
  * pg_gnufft_create replaces Gnufft<float>::Gnufft (Gnufft.cpp:34-122,
  * Gnufft.h:51-53): same arguments plus kernelWidth (original-grid units; the
- * reference hard-codes 4.0 at Gnufft.cpp:84; W*gridos must be 6 or 8).
- * Requirements: gridos*Nx integer and even, Nx == Ny (the reference's
- * zero_pad2d/circshift2 are x-fastest and only consistent for square
- * images, griddingSupport.cpp:488,565).  ix/iy/iz are accepted for
+ * reference hard-codes 4.0 at Gnufft.cpp:84; W*gridos must be 6, 8 or 12).
+ * Requirements: gridos*Nx integer and a multiple of 4, Nx == Ny (the
+ * reference's zero_pad2d/circshift2 are x-fastest and only consistent for
+ * square images, griddingSupport.cpp:488,565).  Grid lengths that are not
+ * 2^a 3^b leave only the spatial interpolation available (operator* and
+ * operator/ then return PG_ERR_UNSUPPORTED).  ix/iy/iz are accepted for
  * signature parity and may be NULL (Gnufft does not use them). */
 int pg_gnufft_create(pg_op* out, size_t M, float gridos, size_t Nx, size_t Ny, size_t Nz,
                      const float* kx, const float* ky, const float* kz, const float* ix,
@@ -130,6 +132,40 @@ int pg_op_adjoint(pg_op op, const void* in_dev, void* out_dev, void* stream);
 /* the same on host buffers (copies in, runs, copies out, synchronizes) */
 int pg_op_forward_host(pg_op op, const void* in_host, void* out_host);
 int pg_op_adjoint_host(pg_op op, const void* in_host, void* out_host);
+
+/* ---- gridding alone and direct reconstruction ---------------------------
+ * The oversampled grid is Gx x Gy (x Gz) complex, y fastest, then x, then z
+ * (gridding.cpp:100, idx = ny + nx*Gy + nz*Gx*Gy), with no FFT shift:
+ * Gx = ceil(gridos*Nx) rounded up to even, Gz = ceil(gridos*Nz) in 3-D and
+ * 1 in 2-D (Gnufft.cpp:228-234).  pg_gnufft_grid_size returns them. */
+int pg_gnufft_grid_size(pg_op G, size_t* Gx, size_t* Gy, size_t* Gz);
+/* Gnufft<float>::forwardSpatialInterp (Gnufft.cpp:214-263): the grid
+ * interpolated at the trajectory with the Kaiser-Bessel taps
+ * (gridding_forward_2D/3D, gridding.cpp:217-415); samples: M complex. */
+int pg_gnufft_forward_spatial(pg_op G, const void* grid_dev, void* samples_dev, void* stream);
+/* Gnufft<float>::adjointSpatialInterp (Gnufft.cpp:266-337): the samples
+ * spread onto the zeroed grid (gridding_adjoint_2D/3D, gridding.cpp:31-214,
+ * sdc 1); grid: Gx*Gy*Gz complex, fully overwritten. */
+int pg_gnufft_adjoint_spatial(pg_op G, const void* samples_dev, void* grid_dev, void* stream);
+/* calc3DDensityCompensation (directRecon.cpp:28-60): the Pipe-Menon
+ * iteration dcf <- dcf / real(interp(spread(dcf))), twice, from
+ * dcf = 1/M^2, with a Gnufft on a 3x oversampled grid (kernel width 4), then
+ * |dcf| / ||dcf||_2.  kx, ky, kz: host arrays of M (kz ignored when
+ * Nz == 1); dcf_dev: M floats on the device.  3*Ninplane must be a multiple
+ * of 4 (and 3*Nz in 3-D). */
+int pg_density_compensation(const float* kx, const float* ky, const float* kz, size_t M,
+                            size_t Ninplane, size_t Nz, float* dcf_dev, void* stream);
+/* the coil loop of gridCoilImages (directRecon.cpp:118-127): image c =
+ * G / (dcf % data_c) for c < ncoils, all coils in one batched adjoint.
+ * G: a Gnufft handle; dcf: M floats; data: ncoils x M complex (coil-major);
+ * images: ncoils x N complex (coil-major). */
+int pg_grid_coil_images(pg_op G, const float* dcf_dev, const void* data_dev, size_t ncoils,
+                        void* images_dev, void* stream);
+/* calcSumOfSquaresImage (directRecon.cpp:63-73): out[v] =
+ * sqrt(sum_c |x[v + c*nvox]|) over nvox voxels x ncoils coils.  As in the
+ * reference the coil magnitudes are summed, not their squares. */
+int pg_sum_of_squares(const void* coil_images_dev, size_t nvox, size_t ncoils, float* out_dev,
+                      void* stream);
 
 /* ---- penalties ----------------------------------------------------------
  * kind 0: QuadPenalty (QuadPenalty.cpp:32-64), kind 1: TVPenalty

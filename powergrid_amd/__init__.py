@@ -182,6 +182,20 @@ class _Op:
             pass
 
 
+def _unary(fn, h, x, n_in, dt_in, n_out, dt_out, out, stream):
+    """fn(h, in_dev, out_dev, stream): DeviceArrays in -> DeviceArray out,
+    host arrays in -> numpy out (synchronised)."""
+    dev = isinstance(x, DeviceArray)
+    xd = _as_device(x, dt_in)
+    if xd.size != n_in:
+        raise ValueError(f"expected {n_in} values, got {xd.size}")
+    od = out if out is not None else DeviceArray(n_out, dt_out)
+    if od.size != n_out or od.dtype != np.dtype(dt_out):
+        raise ValueError(f"expected a {np.dtype(dt_out)} output of {n_out} elements")
+    check(fn(h, xd.ptr, od.ptr, _sh(stream)))
+    return od if dev else od.to_host(stream)
+
+
 def _create(fn, *args):
     h = C.c_void_p()
     check(fn(C.byref(h), *args))
@@ -201,6 +215,22 @@ class Gnufft(_Op):
         super().__init__(h)
         self.Nx, self.Ny, self.Nz = Nx, Ny, Nz
         self.kernel_width = kernel_width
+        g = [C.c_size_t() for _ in range(3)]
+        check(lib().pg_gnufft_grid_size(h, *[C.byref(v) for v in g]))
+        self.grid_shape = tuple(v.value for v in g)  # (Gx, Gy, Gz); y fastest in memory
+        self.grid_size = int(np.prod(self.grid_shape))
+
+    def forwardSpatialInterp(self, grid, out=None, stream=None):
+        """Gnufft::forwardSpatialInterp (Gnufft.cpp:214-263): the oversampled
+        grid (Gx*Gy*Gz, y fastest, unshifted) interpolated at the samples."""
+        return _unary(lib().pg_gnufft_forward_spatial, self._h, grid, self.grid_size,
+                      np.complex64, self.n_out, np.complex64, out, stream)
+
+    def adjointSpatialInterp(self, samples, out=None, stream=None):
+        """Gnufft::adjointSpatialInterp (Gnufft.cpp:266-337): the samples
+        spread onto the zeroed oversampled grid."""
+        return _unary(lib().pg_gnufft_adjoint_spatial, self._h, samples, self.n_out,
+                      np.complex64, self.grid_size, np.complex64, out, stream)
 
 
 class Gdft(_Op):
@@ -262,6 +292,48 @@ class SENSE(_Op):
         h = _create(lib().pg_sense_create, G.handle, _p(s), M, N, nc)
         super().__init__(h, (G,))
         self.nc = nc
+
+
+def calc3DDensityCompensation(kx, ky, kz, Ninplane, Nz, stream=None):
+    """calc3DDensityCompensation(kx, ky, kz, Ninplane, Nz)
+    (directRecon.cpp:28-60): Pipe-Menon density compensation, two
+    iterations of spread and interpolation on a 3x oversampled grid,
+    normalised to unit 2-norm.  Returns M float32 weights (numpy)."""
+    M = len(np.ravel(kx))
+    kx, ky = _f32(kx, M), _f32(ky, M)
+    kz = _f32(kz, M) if kz is not None else np.zeros(M, np.float32)
+    out = DeviceArray(M, np.float32)
+    check(lib().pg_density_compensation(_p(kx), _p(ky), _p(kz), M, Ninplane, Nz, out.ptr,
+                                        _sh(stream)))
+    return out.to_host(stream)
+
+
+def grid_coil_images(G, dcf, data, ncoils, stream=None):
+    """The coil loop of gridCoilImages (directRecon.cpp:118-127): image c =
+    G / (dcf % data_c).  data: ncoils x M (coil-major); returns ncoils x N
+    (coil-major), a DeviceArray when data is one."""
+    dev = isinstance(data, DeviceArray)
+    dd = _as_device(dcf, np.float32)
+    xd = _as_device(data, np.complex64)
+    if dd.size != G.n_out or xd.size != G.n_out * ncoils:
+        raise ValueError("dcf must hold M values and data M*ncoils")
+    out = DeviceArray(G.n_in * ncoils, np.complex64)
+    check(lib().pg_grid_coil_images(G.handle, dd.ptr, xd.ptr, ncoils, out.ptr, _sh(stream)))
+    return out if dev else out.to_host(stream)
+
+
+def calcSumOfSquaresImage(coilImages, Nx, Nz, NSliceMax, Ncoils, stream=None):
+    """calcSumOfSquaresImage(coilImages, Nx, Nz, NSliceMax, Ncoils)
+    (directRecon.cpp:63-73): sqrt of the sum over coils of the magnitudes
+    (the reference sums |x|, not |x|^2) over Nx*Nx*Nz*NSliceMax voxels."""
+    nvox = Nx * Nx * Nz * NSliceMax
+    dev = isinstance(coilImages, DeviceArray)
+    xd = _as_device(coilImages, np.complex64)
+    if xd.size != nvox * Ncoils:
+        raise ValueError(f"expected {nvox * Ncoils} values, got {xd.size}")
+    out = DeviceArray(nvox, np.float32)
+    check(lib().pg_sum_of_squares(xd.ptr, nvox, Ncoils, out.ptr, _sh(stream)))
+    return out if dev else out.to_host(stream)
 
 
 class _Penalty:
@@ -363,6 +435,6 @@ class profile:
 
 
 __all__ = ["DeviceArray", "Stream", "Gnufft", "Gdft", "TimeSegmentation", "SENSE", "PcSenseTimeSeg",
-           "QuadPenalty", "TVPenalty",
-           "solve_pwls_pcg", "reconSolve", "profile", "PgError", "MODE_BATCHED", "MODE_PLAIN",
+           "QuadPenalty", "TVPenalty", "calc3DDensityCompensation", "grid_coil_images",
+           "calcSumOfSquaresImage", "solve_pwls_pcg", "reconSolve", "profile", "PgError", "MODE_BATCHED", "MODE_PLAIN",
            "CG_GRAPH", "CG_HOST"]

@@ -52,6 +52,12 @@ _SIGS = {
     "pg_op_adjoint": (i32, [vp, vp, vp, vp]),
     "pg_op_forward_host": (i32, [vp, vp, vp]),
     "pg_op_adjoint_host": (i32, [vp, vp, vp]),
+    "pg_gnufft_grid_size": (i32, [vp, C.POINTER(sz), C.POINTER(sz), C.POINTER(sz)]),
+    "pg_gnufft_forward_spatial": (i32, [vp, vp, vp, vp]),
+    "pg_gnufft_adjoint_spatial": (i32, [vp, vp, vp, vp]),
+    "pg_density_compensation": (i32, [vp, vp, vp, sz, sz, sz, vp, vp]),
+    "pg_grid_coil_images": (i32, [vp, vp, vp, sz, vp, vp]),
+    "pg_sum_of_squares": (i32, [vp, sz, sz, vp, vp]),
     "pg_penalty_create": (i32, [C.POINTER(vp), i32, sz, sz, sz, f32, f32, sz]),
     "pg_penalty_destroy": (i32, [vp]),
     "pg_penalty_gradient": (i32, [vp, vp, vp, vp]),

@@ -17,6 +17,7 @@
 #include "pg_dft.h"
 #include "pg_gnufft.h"
 #include "pg_internal.h"
+#include "pg_recon.h"
 #include "pg_vec.h"
 
 namespace pg {
@@ -242,9 +243,14 @@ static Win1 window1(double k, int N, double os, double W, int NT, const std::vec
     return r;
 }
 
-// Gnufft.cpp:34-122
+// Gnufft.cpp:34-122.  unit_taps: every tap weight is scaled by the power of
+// two nearest 1/LUT[0] (exact in binary floating point, so every rounding is
+// that of the unscaled weights, while sums of products of up to three taps,
+// ~LUT[0]^6 ~ 1e44, stay inside the FP32 range); the deapodization is scaled
+// back, so the operator is unchanged.
 static void gnufft_setup(Gnufft& G, size_t M, float os, size_t Nx, size_t Ny, size_t Nz,
-                         const float* kx, const float* ky, const float* kz, float W)
+                         const float* kx, const float* ky, const float* kz, float W,
+                         bool unit_taps = false)
 {
     PG_REQUIRE(M > 0 && Nx > 0 && Ny > 0 && Nz > 0, PG_ERR_ARG, "empty Gnufft");
     PG_REQUIRE(kx && ky, PG_ERR_ARG, "kx/ky required");
@@ -272,8 +278,8 @@ static void gnufft_setup(Gnufft& G, size_t M, float os, size_t Nx, size_t Ny, si
                PG_ERR_UNSUPPORTED, "gridos*Nz must be a multiple of 4");
     g.W = W;
     g.ntaps = (int)std::ceil(W * os - 1e-6f);
-    PG_REQUIRE(g.ntaps == 8 || g.ntaps == 6, PG_ERR_UNSUPPORTED,
-               "kernelWidth*gridos must be 6 or 8 taps");
+    PG_REQUIRE(g.ntaps == 8 || g.ntaps == 6 || g.ntaps == 12, PG_ERR_UNSUPPORTED,
+               "kernelWidth*gridos must be 6, 8 or 12 taps");
     // Gnufft.cpp:85-88 (beta kept in double for the tables below)
     const double beta_d = M_PI * std::sqrt(((double)os - 0.5) * ((double)os - 0.5) *
                                            ((double)W * W * 4.0) / ((double)os * os) - 0.8);
@@ -292,6 +298,8 @@ static void gnufft_setup(Gnufft& G, size_t M, float os, size_t Nx, size_t Ny, si
     std::vector<double> lut(S);
     for (int k = 0; k < S; ++k)
         lut[k] = bessi0(beta_d * std::sqrt(1.0 - (double)k / (double)S)) / (double)W;
+    const int wexp = unit_taps ? -(int)std::lround(std::log2(lut[0])) : 0;
+    for (int k = 0; k < S && wexp != 0; ++k) lut[k] = std::ldexp(lut[k], wexp);
 
     // per-sample windows and weights
     std::vector<int> lo((size_t)D * M);
@@ -327,7 +335,8 @@ static void gnufft_setup(Gnufft& G, size_t M, float os, size_t Nx, size_t Ny, si
     G.dfx = dev_copy(fx.data(), Nx);
     G.dfy = dev_copy(fy.data(), Ny);
     G.dfz = dev_copy(fz.data(), Nz);
-    G.inv_osd = (D == 2) ? (float)(1.0 / ((double)os * os)) : (float)(1.0 / ((double)os * os * os));
+    G.inv_osd = (float)(std::ldexp(1.0, -wexp * D) /
+                        ((D == 2) ? (double)os * os : (double)os * os * os));
     {
         // the same factors in FP64, combined per voxel (y fastest)
         std::vector<double> ax(Nx), ay(Ny), az(Nz, 1.0);
@@ -344,14 +353,19 @@ static void gnufft_setup(Gnufft& G, size_t M, float os, size_t Nx, size_t Ny, si
             for (size_t X = 0; X < Nx; ++X)
                 for (size_t Y = 0; Y < Ny; ++Y) {
                     const double k = ax[X] * ay[Y] * az[Z];
-                    dap[Y + X * Ny + Z * Nx * Ny] = std::isnan(k) ? 0.0f : (float)(1.0 / (osd * k));
+                    dap[Y + X * Ny + Z * Nx * Ny] =
+                        std::isnan(k) ? 0.0f : (float)(std::ldexp(1.0, -wexp * D) / (osd * k));
                 }
         G.dap = dev_copy(dap.data(), dap.size());
     }
 
-    G.px = make_fft_plan(g.Gx);
-    G.py = make_fft_plan(g.Gy);
-    if (D == 3) G.pz = make_fft_plan(g.Gz);
+    G.fft_ok = fft_length_supported(g.Gx) && fft_length_supported(g.Gy) &&
+               (D == 2 || fft_length_supported(g.Gz));
+    if (G.fft_ok) {
+        G.px = make_fft_plan(g.Gx);
+        G.py = make_fft_plan(g.Gy);
+        if (D == 3) G.pz = make_fft_plan(g.Gz);
+    }
 
     auto empty = [&](size_t i) {
         for (int d = 0; d < D; ++d)
@@ -360,73 +374,76 @@ static void gnufft_setup(Gnufft& G, size_t M, float os, size_t Nx, size_t Ny, si
     };
 
     if (D == 2) {
-        // interp tiles: the window start is re-based into [-padl, G-1] (taps
-        // moved off the start carry zero weight), binned by kIT x kIT tiles
-        const int padl = NT / 2;
-        G.ipadl = padl;
-        G.itx = (g.Gx + padl + kIT - 1) / kIT;
-        G.ity = (g.Gy + padl + kIT - 1) / kIT;
-        const int nt = G.itx * G.ity;
-        std::vector<int> rlo(2 * M);
-        std::vector<float> rw((size_t)2 * 8 * M, 0.0f);
-        std::vector<int> tcnt(nt + 1, 0), tof(M);
-        const int Gd[2] = {g.Gx, g.Gy};
-        for (size_t i = 0; i < M; ++i) {
-            int tt[2];
-            for (int d = 0; d < 2; ++d) {
-                int l0 = lo[(size_t)d * M + i];
-                int sh = 0;
-                if (empty(i)) {
-                    l0 = 0;
-                    sh = NT;  // all weights zero
-                } else if (l0 < -padl) {
-                    sh = -padl - l0;
-                    l0 = -padl;
+        if (NT <= 8) {
+            // interp tiles: the window start is re-based into [-padl, G-1] (taps
+            // moved off the start carry zero weight), binned by kIT x kIT tiles
+            const int padl = NT / 2;
+            G.ipadl = padl;
+            G.itx = (g.Gx + padl + kIT - 1) / kIT;
+            G.ity = (g.Gy + padl + kIT - 1) / kIT;
+            const int nt = G.itx * G.ity;
+            std::vector<int> rlo(2 * M);
+            std::vector<float> rw((size_t)2 * 8 * M, 0.0f);
+            std::vector<int> tcnt(nt + 1, 0), tof(M);
+            const int Gd[2] = {g.Gx, g.Gy};
+            for (size_t i = 0; i < M; ++i) {
+                int tt[2];
+                for (int d = 0; d < 2; ++d) {
+                    int l0 = lo[(size_t)d * M + i];
+                    int sh = 0;
+                    if (empty(i)) {
+                        l0 = 0;
+                        sh = NT;  // all weights zero
+                    } else if (l0 < -padl) {
+                        sh = -padl - l0;
+                        l0 = -padl;
+                    }
+                    PG_REQUIRE(l0 <= Gd[d] - 1, PG_ERR_ARG, "window start past the grid");
+                    rlo[(size_t)d * M + i] = l0;
+                    for (int j = 0; j < 8; ++j) {
+                        const int src = j + sh;
+                        rw[((size_t)d * M + i) * 8 + j] =
+                            (src < NT) ? w[((size_t)d * NT + src) * M + i] : 0.0f;
+                    }
+                    tt[d] = (l0 + padl) / kIT;
                 }
-                PG_REQUIRE(l0 <= Gd[d] - 1, PG_ERR_ARG, "window start past the grid");
-                rlo[(size_t)d * M + i] = l0;
-                for (int j = 0; j < 8; ++j) {
-                    const int src = j + sh;
-                    rw[((size_t)d * M + i) * 8 + j] = (src < NT) ? w[((size_t)d * NT + src) * M + i] : 0.0f;
+                tof[i] = tt[0] * G.ity + tt[1];
+                tcnt[tof[i] + 1]++;
+            }
+            for (int t = 0; t < nt; ++t) tcnt[t + 1] += tcnt[t];
+            std::vector<int> pos(tcnt.begin(), tcnt.end() - 1), order(M);
+            for (size_t i = 0; i < M; ++i) order[pos[tof[i]]++] = (int)i;
+            std::vector<float4> rec((size_t)5 * M);
+            std::vector<int4> its;
+            for (int t = 0; t < nt; ++t) {
+                const int tx = t / G.ity, ty = t % G.ity;
+                const int rx0 = tx * kIT - padl, ry0 = ty * kIT - padl;
+                for (int k = tcnt[t]; k < tcnt[t + 1]; ++k) {
+                    const int i = order[k];
+                    const int ox = rlo[i] - rx0, oy = rlo[M + i] - ry0;
+                    PG_REQUIRE(ox >= 0 && ox < kIT && oy >= 0 && oy < kIT, PG_ERR_ARG, "interp binning");
+                    float4 h;
+                    h.x = __int_as_float_host(i);
+                    h.y = __int_as_float_host(ox);
+                    h.z = __int_as_float_host(oy);
+                    h.w = 0.0f;
+                    const float* wx = &rw[(size_t)i * 8];
+                    const float* wy = &rw[((size_t)M + i) * 8];
+                    rec[(size_t)5 * k + 0] = h;
+                    rec[(size_t)5 * k + 1] = make_float4(wx[0], wx[1], wx[2], wx[3]);
+                    rec[(size_t)5 * k + 2] = make_float4(wx[4], wx[5], wx[6], wx[7]);
+                    rec[(size_t)5 * k + 3] = make_float4(wy[0], wy[1], wy[2], wy[3]);
+                    rec[(size_t)5 * k + 4] = make_float4(wy[4], wy[5], wy[6], wy[7]);
                 }
-                tt[d] = (l0 + padl) / kIT;
+                for (int k = tcnt[t]; k < tcnt[t + 1]; k += kIChunk)
+                    its.push_back(make_int4(t, k, std::min(k + kIChunk, tcnt[t + 1]), 0));
             }
-            tof[i] = tt[0] * G.ity + tt[1];
-            tcnt[tof[i] + 1]++;
-        }
-        for (int t = 0; t < nt; ++t) tcnt[t + 1] += tcnt[t];
-        std::vector<int> pos(tcnt.begin(), tcnt.end() - 1), order(M);
-        for (size_t i = 0; i < M; ++i) order[pos[tof[i]]++] = (int)i;
-        std::vector<float4> rec((size_t)5 * M);
-        std::vector<int4> its;
-        for (int t = 0; t < nt; ++t) {
-            const int tx = t / G.ity, ty = t % G.ity;
-            const int rx0 = tx * kIT - padl, ry0 = ty * kIT - padl;
-            for (int k = tcnt[t]; k < tcnt[t + 1]; ++k) {
-                const int i = order[k];
-                const int ox = rlo[i] - rx0, oy = rlo[M + i] - ry0;
-                PG_REQUIRE(ox >= 0 && ox < kIT && oy >= 0 && oy < kIT, PG_ERR_ARG, "interp binning");
-                float4 h;
-                h.x = __int_as_float_host(i);
-                h.y = __int_as_float_host(ox);
-                h.z = __int_as_float_host(oy);
-                h.w = 0.0f;
-                const float* wx = &rw[(size_t)i * 8];
-                const float* wy = &rw[((size_t)M + i) * 8];
-                rec[(size_t)5 * k + 0] = h;
-                rec[(size_t)5 * k + 1] = make_float4(wx[0], wx[1], wx[2], wx[3]);
-                rec[(size_t)5 * k + 2] = make_float4(wx[4], wx[5], wx[6], wx[7]);
-                rec[(size_t)5 * k + 3] = make_float4(wy[0], wy[1], wy[2], wy[3]);
-                rec[(size_t)5 * k + 4] = make_float4(wy[4], wy[5], wy[6], wy[7]);
-            }
-            for (int k = tcnt[t]; k < tcnt[t + 1]; k += kIChunk)
-                its.push_back(make_int4(t, k, std::min(k + kIChunk, tcnt[t + 1]), 0));
-        }
-        // items stay in tile raster order: neighbouring tiles, whose staged
-        // regions overlap, run at about the same time and share L2 lines
-        G.n_iitems = (int)its.size();
-        G.iitems = dev_copy(its.data(), its.size());
-        G.irec = dev_copy(rec.data(), rec.size());
+            // items stay in tile raster order: neighbouring tiles, whose staged
+            // regions overlap, run at about the same time and share L2 lines
+            G.n_iitems = (int)its.size();
+            G.iitems = dev_copy(its.data(), its.size());
+            G.irec = dev_copy(rec.data(), rec.size());
+        }  // wider windows interpolate with the gather kernel
 
         // 4x4 patch work items with per-entry patch weights
         const int npx = g.Gx / kPatch, npy = g.Gy / kPatch;
@@ -677,7 +694,13 @@ struct Op {
 
 struct NufftOp : Op {
     Gnufft G;
-    ~NufftOp() override { G.release(); }
+    float2* wtmp = nullptr;  // dcf-weighted coil data of pg_grid_coil_images
+    size_t wtmp_cap = 0;
+    ~NufftOp() override
+    {
+        G.release();
+        dev_free(wtmp);
+    }
 };
 
 struct DftOp : Op {
@@ -815,10 +838,19 @@ static bool fusable(Op* op, Fused& f)
 static void op_forward(Op* op, const float2* in, float2* out, hipStream_t s);
 static void op_adjoint(Op* op, const float2* in, float2* out, hipStream_t s);
 
+static void require_fft(const Gnufft& G)
+{
+    PG_REQUIRE(G.fft_ok, PG_ERR_UNSUPPORTED,
+               "grid lengths " + std::to_string(G.geo.Gx) + "x" + std::to_string(G.geo.Gy) + "x" +
+                   std::to_string(G.geo.Gz) +
+                   " are not of the form 2^a 3^b: only the spatial interpolation is available");
+}
+
 static void fused_forward(const Fused& f, const float2* x, float2* y, hipStream_t s,
                           long long y_cstride = -1)
 {
     Gnufft& G = *f.G;
+    require_fft(G);
     const int B = f.nc * f.L;
     G.ensure_batch(B);
     pgk::WeightArgs w{f.S, f.Wo, f.WoH, f.AA};
@@ -839,6 +871,7 @@ static void fused_adjoint_partial(const Fused& f, const float2* in, const float2
                                   bool vals_ready = false, long long in_cstride = -1)
 {
     Gnufft& G = *f.G;
+    require_fft(G);
     const int B = f.nc * f.L;
     G.ensure_batch(B);
     pgk::WeightArgs w{f.S, f.Wo, f.WoH, f.AA};
@@ -1672,6 +1705,153 @@ static int host_apply(pg_op h, const void* in, void* out, bool fwd)
 
 int pg_op_forward_host(pg_op h, const void* in, void* out) { return host_apply(h, in, out, true); }
 int pg_op_adjoint_host(pg_op h, const void* in, void* out) { return host_apply(h, in, out, false); }
+
+// ------------------------------------- gridding alone, direct recon ----
+
+static NufftOp* as_nufft(pg_op h)
+{
+    Op* op = as_op(h);
+    PG_REQUIRE(op->kind == K_NUFFT, PG_ERR_ARG, "not a Gnufft handle");
+    return static_cast<NufftOp*>(op);
+}
+
+int pg_gnufft_grid_size(pg_op h, size_t* Gx, size_t* Gy, size_t* Gz)
+{
+    PG_TRY const pg::Geometry& g = as_nufft(h)->G.geo;
+    if (Gx) *Gx = (size_t)g.Gx;
+    if (Gy) *Gy = (size_t)g.Gy;
+    if (Gz) *Gz = (size_t)g.Gz;
+    PG_CATCH
+}
+
+int pg_gnufft_forward_spatial(pg_op h, const void* grid, void* samples, void* stream)
+{
+    PG_TRY Gnufft& G = as_nufft(h)->G;
+    PG_REQUIRE(grid && samples && grid != samples, PG_ERR_ARG, "bad buffers");
+    hipStream_t s = (hipStream_t)stream;
+    {
+        Prof p("interp", s);
+        pgk::launch_interp(G, (const float2*)grid, 1, 1, nullptr, (float2*)samples, G.M, s);
+    }
+    PG_HIP(hipGetLastError());
+    PG_CATCH
+}
+
+int pg_gnufft_adjoint_spatial(pg_op h, const void* samples, void* grid, void* stream)
+{
+    PG_TRY Gnufft& G = as_nufft(h)->G;
+    PG_REQUIRE(grid && samples && grid != samples, PG_ERR_ARG, "bad buffers");
+    hipStream_t s = (hipStream_t)stream;
+    G.ensure_batch(1);
+    {
+        Prof p("spread", s);
+        pgk::launch_spread(G, (const float2*)samples, G.M, 1, 1, nullptr, nullptr, nullptr, nullptr,
+                           (float2*)grid, false, s);
+    }
+    PG_HIP(hipGetLastError());
+    PG_CATCH
+}
+
+int pg_density_compensation(const float* kx, const float* ky, const float* kz, size_t M,
+                            size_t Ninplane, size_t Nz, float* dcf, void* stream)
+{
+    PG_TRY PG_REQUIRE(kx && ky && dcf && M > 0 && Ninplane > 0 && Nz > 0, PG_ERR_ARG,
+                      "bad arguments");
+    hipStream_t s = (hipStream_t)stream;
+    // directRecon.cpp:31-35: a Gnufft on a 3x oversampled grid; its kernel
+    // width is the reference Gnufft's fixed 4.0 (Gnufft.cpp:84)
+    std::unique_ptr<NufftOp> op(new NufftOp);
+    op->kind = K_NUFFT;
+    // (the taps scaled by a power of two: the result is invariant to the
+    // kernel's scale, and the float instantiation's sums, ~1e40 in 3-D,
+    // would overflow unscaled)
+    gnufft_setup(op->G, M, 3.0f, Ninplane, Ninplane, Nz, kx, ky, kz, 4.0f, true);
+    Gnufft& G = op->G;
+    G.ensure_batch(1);
+    struct Tmp {
+        float2* d = nullptr;
+        float2* c1 = nullptr;
+        float2* c2 = nullptr;
+        double* part = nullptr;
+        ~Tmp()
+        {
+            dev_free(d);
+            dev_free(c1);
+            dev_free(c2);
+            dev_free(part);
+        }
+    } t;
+    const long long Mm = (long long)M;
+    t.d = dev_alloc<float2>(M);
+    t.c1 = dev_alloc<float2>((size_t)G.geo.G());
+    t.c2 = dev_alloc<float2>(M);
+    t.part = dev_alloc<double>(pgr::kMaxParts);
+    // directRecon.cpp:38-39: ones / (M * M), in float
+    const float den = (float)M * (float)M;
+    pgv::fill(t.d, Mm, make_float2(1.0f / den, 0.0f), s);
+    // directRecon.cpp:46-52, Niter = 2
+    for (int it = 0; it < 2; ++it) {
+        {
+            Prof p("spread", s);
+            pgk::launch_spread(G, t.d, Mm, 1, 1, nullptr, nullptr, nullptr, nullptr, t.c1, false, s);
+        }
+        {
+            Prof p("interp", s);
+            pgk::launch_interp(G, t.c1, 1, 1, nullptr, t.c2, Mm, s);
+        }
+        pgr::dcf_update(t.d, t.c2, Mm, s);
+    }
+    // directRecon.cpp:54-56
+    pgr::dcf_finish(t.d, Mm, t.part, dcf, s);
+    PG_HIP(hipGetLastError());
+    // the temporaries are freed on return: wait for the kernels that use them
+    PG_HIP(hipStreamSynchronize(s));
+    PG_CATCH
+}
+
+int pg_grid_coil_images(pg_op h, const float* dcf, const void* data, size_t ncoils, void* images,
+                        void* stream)
+{
+    PG_TRY NufftOp* no = as_nufft(h);
+    PG_REQUIRE(dcf && data && images && ncoils > 0 && ncoils <= 4096, PG_ERR_ARG, "bad arguments");
+    hipStream_t s = (hipStream_t)stream;
+    Gnufft& G = no->G;
+    const long long M = G.M, N = G.geo.N();
+    const size_t need = ncoils * (size_t)M;
+    if (need > no->wtmp_cap) {
+        ++g_ws_gen;
+        dev_free(no->wtmp);
+        no->wtmp = dev_alloc<float2>(need);
+        no->wtmp_cap = need;
+    }
+    // directRecon.cpp:123: G / (dcf % data_c) for every coil
+    pgr::weight_columns(no->wtmp, dcf, (const float2*)data, M, (int)ncoils, s);
+    Fused f;
+    f.G = &G;
+    if (no->mode == PG_MODE_BATCHED) {
+        f.nc = (int)ncoils;
+        fused_adjoint_partial(f, no->wtmp, nullptr, nullptr, nullptr, (float2*)images, s);
+    } else {
+        for (size_t c = 0; c < ncoils; ++c)
+            fused_adjoint_partial(f, no->wtmp + c * (size_t)M, nullptr, nullptr, nullptr,
+                                  (float2*)images + c * (size_t)N, s);
+    }
+    PG_HIP(hipGetLastError());
+    PG_CATCH
+}
+
+int pg_sum_of_squares(const void* x, size_t nvox, size_t ncoils, float* out, void* stream)
+{
+    PG_TRY PG_REQUIRE(x && out && nvox > 0 && ncoils > 0 && ncoils < (1u << 20), PG_ERR_ARG,
+                      "bad arguments");
+    hipStream_t s = (hipStream_t)stream;
+    {
+        Prof p("sum_of_squares", s);
+        pgr::sum_of_magnitudes((const float2*)x, (long long)nvox, (int)ncoils, out, s);
+    }
+    PG_HIP(hipGetLastError());
+    PG_CATCH
+}
 
 int pg_penalty_create(pg_penalty* out, int kind, size_t Nx, size_t Ny, size_t Nz, float beta,
                       float delta, size_t dims2penalize)

@@ -34,8 +34,10 @@ struct Gnufft {
     // full deapodization factor per image voxel, computed in FP64:
     // dap[n] = 1/(os^d psi_x psi_y psi_z), 0 where psi is NaN
     float* dap = nullptr;
-    // FFT plans for the three grid lengths
+    // FFT plans for the three grid lengths; fft_ok is false when a grid
+    // length is not 2^a 3^b (only the spatial interpolation is available)
     FftPlan1D px, py, pz;
+    bool fft_ok = false;
     // 2-D batched spread over 4x4 patches: blocks of 8 wavefront items
     //   witems[8*k + w] = {patch or -1, entry begin, entry end, slot or -1}
     // and the per-block group size bgs[k] (wavefronts sharing a patch);

@@ -34,9 +34,10 @@ namespace {
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return min(max(v, lo), hi); }
 
 // ---------------------------------------------------------------- interp --
-// Four lanes per sample: lane q of a sample reads the taps 2q, 2q+1 along y
-// of every row, so one load instruction covers 16 samples x 64 contiguous
-// bytes; the four partial sums are combined with two lane shuffles.
+// Four lanes per sample: lane q of a sample reads the taps q, q+4, q+8 (as
+// many as NT has) along y of every row, so one load instruction covers 16
+// samples x 32 contiguous bytes; the four partial sums are combined with two
+// lane shuffles.
 template <int NT, bool D3>
 __global__ void __launch_bounds__(256) k_interp(
     const int* __restrict__ lo, const float* __restrict__ w, int M, int Gx, int Gy, int Gz,
@@ -63,10 +64,16 @@ __global__ void __launch_bounds__(256) k_interp(
     const int ii = valid ? i : M - 1;
     const int lx = lo[ii], ly = lo[M + ii];
     const int lz = D3 ? lo[2 * M + ii] : 0;
-    const int j0 = 2 * q, j1 = 2 * q + 1;
-    const float wy0 = (j0 < NT) ? w[(long long)(NT + j0) * M + ii] : 0.f;
-    const float wy1 = (j1 < NT) ? w[(long long)(NT + j1) * M + ii] : 0.f;
-    const int oy0 = clampi(ly + j0, 0, Gy - 1), oy1 = clampi(ly + j1, 0, Gy - 1);
+    // lane q takes the y taps q, q + 4, q + 8 (zero weight past NT)
+    constexpr int JY = (NT + 3) / 4;
+    float wy[JY];
+    int oy[JY];
+#pragma unroll
+    for (int j = 0; j < JY; ++j) {
+        const int jj = q + 4 * j;
+        wy[j] = (jj < NT) ? w[(long long)(NT + jj) * M + ii] : 0.f;
+        oy[j] = clampi(ly + jj, 0, Gy - 1);
+    }
     float wx[NT], wz[D3 ? NT : 1];
     int ox[NT];
 #pragma unroll
@@ -96,9 +103,13 @@ __global__ void __launch_bounds__(256) k_interp(
 #pragma unroll
             for (int jx = 0; jx < NT; ++jx) {
                 const float2* row = gz + ox[jx];
-                const float2 v0 = row[oy0], v1 = row[oy1];
-                const float rr = wy0 * v0.x + wy1 * v1.x;
-                const float ri = wy0 * v0.y + wy1 * v1.y;
+                float rr = 0.f, ri = 0.f;
+#pragma unroll
+                for (int j = 0; j < JY; ++j) {
+                    const float2 v = row[oy[j]];
+                    rr += wy[j] * v.x;
+                    ri += wy[j] * v.y;
+                }
                 const float wxz = D3 ? wx[jx] * wz[jz] : wx[jx];
                 ar += wxz * rr;
                 ai += wxz * ri;
@@ -655,7 +666,7 @@ void interp_dispatch(const pg::Gnufft& G, const float2* grids, int nc, int L, co
                      float2* out, long long out_cstride, hipStream_t s)
 {
     const pg::Geometry& g = G.geo;
-    if (!D3 && AA && L >= 2 && L <= 16 && G.n_iitems > 0) {
+    if (!D3 && NT <= 8 && AA && L >= 2 && L <= 16 && G.n_iitems > 0) {
         const dim3 grid((unsigned)G.n_iitems * nc), block(64 * kInterpWaves);
         if (L <= 8)
             hipLaunchKernelGGL(k_interp_lds<8>, grid, block, 0, s, G.iitems, G.irec, G.ity, G.ipadl,
@@ -730,8 +741,12 @@ void launch_interp(const pg::Gnufft& G, const float2* grids, int nc, int L, cons
         d3 ? interp_dispatch<6, true>(G, grids, nc, L, AA, out, out_cstride, s)
            : interp_dispatch<6, false>(G, grids, nc, L, AA, out, out_cstride, s);
         break;
+    case 12:
+        d3 ? interp_dispatch<12, true>(G, grids, nc, L, AA, out, out_cstride, s)
+           : interp_dispatch<12, false>(G, grids, nc, L, AA, out, out_cstride, s);
+        break;
     default:
-        throw pg::Error{PG_ERR_UNSUPPORTED, "kernel width * gridos must give 6 or 8 taps"};
+        throw pg::Error{PG_ERR_UNSUPPORTED, "kernel width * gridos must give 6, 8 or 12 taps"};
     }
 }
 
@@ -757,8 +772,12 @@ void launch_spread(const pg::Gnufft& G, const float2* in, long long in_cstride, 
         d3 ? spread_dispatch<6, true>(G, in, in_cstride, nc, L, AA, resid_y, ax64, Wd, grids, vals_ready, s)
            : spread_dispatch<6, false>(G, in, in_cstride, nc, L, AA, resid_y, ax64, Wd, grids, vals_ready, s);
         break;
+    case 12:
+        d3 ? spread_dispatch<12, true>(G, in, in_cstride, nc, L, AA, resid_y, ax64, Wd, grids, vals_ready, s)
+           : spread_dispatch<12, false>(G, in, in_cstride, nc, L, AA, resid_y, ax64, Wd, grids, vals_ready, s);
+        break;
     default:
-        throw pg::Error{PG_ERR_UNSUPPORTED, "kernel width * gridos must give 6 or 8 taps"};
+        throw pg::Error{PG_ERR_UNSUPPORTED, "kernel width * gridos must give 6, 8 or 12 taps"};
     }
 }
 
