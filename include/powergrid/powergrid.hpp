@@ -122,6 +122,22 @@ public:
 // ------------------------------------------------------------ operators ---
 namespace detail {
 
+// a device buffer owned by the library's runtime (pg_malloc / pg_free)
+struct DevBuf {
+    void* p = nullptr;
+    explicit DevBuf(size_t bytes) { check(pg_malloc(&p, bytes ? bytes : 1)); }
+    DevBuf(const void* host, size_t bytes) : DevBuf(bytes)
+    {
+        if (bytes) check(pg_memcpy_htod(p, host, bytes, nullptr));
+    }
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf()
+    {
+        if (p) pg_free(p);
+    }
+};
+
 class OpHandle {
 public:
     OpHandle() = default;
@@ -210,8 +226,35 @@ public:
         Nz = nz;
         n1 = nx * ny * nz;
         n2 = dataLength;
+        size_t gx = 0, gy = 0, gz = 0;
+        check(pg_gnufft_grid_size(h, &gx, &gy, &gz));
+        gridNumElems = gx * gy * gz;
     }
-    uword n1 = 0, n2 = 0, Nx = 0, Ny = 0, Nz = 0;
+    // Gnufft.cpp:214-263: the oversampled grid (gridNumElems, y fastest,
+    // unshifted) interpolated at the samples
+    Col<complex<T1>> forwardSpatialInterp(const Col<complex<T1>>& d) const
+    {
+        return spatial(d, gridNumElems, n2, true);
+    }
+    // Gnufft.cpp:266-337: the samples spread onto the zeroed grid
+    Col<complex<T1>> adjointSpatialInterp(const Col<complex<T1>>& d) const
+    {
+        return spatial(d, n2, gridNumElems, false);
+    }
+    uword n1 = 0, n2 = 0, Nx = 0, Ny = 0, Nz = 0, gridNumElems = 0;
+
+private:
+    Col<complex<T1>> spatial(const Col<complex<T1>>& d, uword nin, uword nout, bool fwd) const
+    {
+        if (d.size() != nin) throw Error(PG_ERR_ARG, "spatial interpolation: wrong input length");
+        detail::DevBuf in(d.memptr(), sizeof(complex<T1>) * nin);
+        detail::DevBuf out(sizeof(complex<T1>) * nout);
+        check(fwd ? pg_gnufft_forward_spatial(this->handle(), in.p, out.p, nullptr)
+                  : pg_gnufft_adjoint_spatial(this->handle(), in.p, out.p, nullptr));
+        Col<complex<T1>> r(nout);
+        check(pg_memcpy_dtoh(r.memptr(), out.p, sizeof(complex<T1>) * nout, nullptr));
+        return r;
+    }
 };
 
 template <typename T1>
@@ -363,23 +406,6 @@ private:
 };
 
 // --------------------------------------------------------------- solver ---
-namespace detail {
-struct DevBuf {
-    void* p = nullptr;
-    explicit DevBuf(size_t bytes) { check(pg_malloc(&p, bytes)); }
-    DevBuf(const void* host, size_t bytes) : DevBuf(bytes)
-    {
-        check(pg_memcpy_htod(p, host, bytes, nullptr));
-    }
-    DevBuf(const DevBuf&) = delete;
-    DevBuf& operator=(const DevBuf&) = delete;
-    ~DevBuf()
-    {
-        if (p) pg_free(p);
-    }
-};
-}  // namespace detail
-
 // solve_pwls_pcg.hpp:48-167, run on the device.  info (optional) receives
 // the completed iterations and the exit reason.
 template <typename T1, typename Tobj, typename Robj>
@@ -422,6 +448,59 @@ Col<complex<T1>> reconSolve(Col<complex<T1>> data, TObj& Sg, RObj R, Col<T1> kx,
     Col<complex<T1>> xinit(Nx * Ny * Nz, complex<T1>(0, 0));
     Col<T1> W;  // empty = ones (reconSolve.h:72-73)
     return solve_pwls_pcg<T1, TObj, RObj>(xinit, Sg, W, data, R, niter);
+}
+
+// ------------------------------------------------- direct reconstruction ---
+// directRecon.cpp:28-60 (Pipe-Menon density compensation on a 3x grid)
+template <typename T>
+Col<T> calc3DDensityCompensation(Col<T> kx, Col<T> ky, Col<T> kz, uword Ninplane, uword Nz)
+{
+    detail::require_float<T>();
+    const uword M = kx.size();
+    if (ky.size() != M || (Nz > 1 && kz.size() != M))
+        throw Error(PG_ERR_ARG, "calc3DDensityCompensation: trajectory lengths differ");
+    detail::DevBuf d(sizeof(T) * M);
+    check(pg_density_compensation(kx.memptr(), ky.memptr(), kz.size() ? kz.memptr() : nullptr, M,
+                                  Ninplane, Nz, (float*)d.p, nullptr));
+    Col<T> out(M);
+    check(pg_memcpy_dtoh(out.memptr(), d.p, sizeof(T) * M, nullptr));
+    return out;
+}
+
+// directRecon.cpp:63-73: sqrt of the coil sum of magnitudes over
+// Nx*Nx*Nz*NSliceMax voxels
+template <typename T>
+Col<T> calcSumOfSquaresImage(Col<complex<T>> coilImages, uword Nx, uword Nz, uword NSliceMax,
+                             uword Ncoils)
+{
+    detail::require_float<T>();
+    const uword nvox = Nx * Nx * Nz * NSliceMax;
+    if (coilImages.size() != nvox * Ncoils)
+        throw Error(PG_ERR_ARG, "calcSumOfSquaresImage: wrong input length");
+    detail::DevBuf x(coilImages.memptr(), sizeof(complex<T>) * nvox * Ncoils);
+    detail::DevBuf o(sizeof(T) * nvox);
+    check(pg_sum_of_squares(x.p, nvox, Ncoils, (float*)o.p, nullptr));
+    Col<T> out(nvox);
+    check(pg_memcpy_dtoh(out.memptr(), o.p, sizeof(T) * nvox, nullptr));
+    return out;
+}
+
+// the coil loop of gridCoilImages (directRecon.cpp:118-127): image c =
+// G / (dcf % data_c), coil-major
+template <typename T>
+Col<complex<T>> gridCoilImages(const Gnufft<T>& G, const Col<T>& dcf, const Col<complex<T>>& data,
+                               uword Ncoils)
+{
+    const uword M = G.n2, N = G.n1;
+    if (dcf.size() != M || data.size() != M * Ncoils)
+        throw Error(PG_ERR_ARG, "gridCoilImages: wrong input lengths");
+    detail::DevBuf w(dcf.memptr(), sizeof(T) * M);
+    detail::DevBuf y(data.memptr(), sizeof(complex<T>) * M * Ncoils);
+    detail::DevBuf im(sizeof(complex<T>) * N * Ncoils);
+    check(pg_grid_coil_images(G.handle(), (const float*)w.p, y.p, Ncoils, im.p, nullptr));
+    Col<complex<T>> out(N * Ncoils);
+    check(pg_memcpy_dtoh(out.memptr(), im.p, sizeof(complex<T>) * N * Ncoils, nullptr));
+    return out;
 }
 
 // reconSolve.cpp:30-54: y-fastest image-space coordinates in [-1/2, 1/2)

@@ -110,6 +110,17 @@ int pg_pcsense_create(pg_op* out, const float* kx, const float* ky, const float*
                       size_t interptype, const float* sens, const float* fieldmap,
                       const float* shotphase, size_t n_shots, size_t Nd, float kernelWidth);
 
+/* replaces pcSENSE<float>::pcSENSE (pcSENSE.cpp:39-102, pcSENSE.h:47-49):
+ * shot-phase-corrected SENSE over an exact DFT per shot (Gdft, FP32 trig).
+ * Arguments as pg_pcsense_create, without L/interptype/kernelWidth.  As in the
+ * reference, the forward applies S_c exp(+i P_s) (pcSENSE.cpp:117) and the
+ * adjoint conj(S_c) exp(+i P_s) (:142): the pair is not adjoint unless the
+ * shot phases are zero. */
+int pg_pcsense_gdft_create(pg_op* out, const float* kx, const float* ky, const float* kz,
+                           size_t Nx, size_t Ny, size_t Nz, size_t nc, const float* tvec,
+                           const float* sens, const float* fieldmap, const float* shotphase,
+                           size_t n_shots, size_t Nd);
+
 int pg_op_destroy(pg_op op);
 /* n_in = image length (N), n_out = data length (M, or M*nc for SENSE) */
 int pg_op_shape(pg_op op, size_t* n_in, size_t* n_out);

@@ -69,6 +69,8 @@ def _declare(L):
         f("orc_dense_create", _vp, _vp, C.c_int, C.c_int)
         f("orc_pcsense_create", _vp, C.c_int, C.c_int, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int,
           C.c_int, _vp, C.c_int, C.c_int, _vp, _vp, _vp, R)
+        f("orc_pcsense_dft_create", _vp, C.c_int, C.c_int, _vp, _vp, _vp, C.c_int, C.c_int,
+          C.c_int, C.c_int, _vp, _vp, _vp, _vp)
         f("orc_op_forward", None, _vp, _vp, _vp)
         f("orc_op_adjoint", None, _vp, _vp, _vp)
         f("orc_op_n_in", C.c_int, _vp)
@@ -311,6 +313,22 @@ class PcSenseTimeSeg(_Op):
             Nd, Ns, _p(_r(kx, prec)), _p(_r(ky, prec)), _p(_r(kz, prec)), Nx, Ny, Nz, nc,
             _p(_r(t, prec)), L, interptype, _p(_c(sens, prec)), _p(_r(fieldmap, prec)),
             _p(_r(shot_phase, prec)), kernel_width)
+        super().__init__(h, prec)
+        self.Nd, self.Ns, self.Nc = Nd, Ns, nc
+
+
+class PcSENSE(_Op):
+    """pcSENSE.cpp:39-153: shot-phase-corrected SENSE over an exact DFT per
+    shot (Gdft).  As in the reference, the forward applies S_c exp(+i P_s)
+    and the adjoint conj(S_c) exp(+i P_s)."""
+
+    def __init__(self, kx, ky, kz, Nx, Ny, Nz, nc, t, sens, fieldmap, shot_phase, prec="f"):
+        Ni = Nx * Ny * Nz
+        Ns = len(np.ravel(shot_phase)) // Ni
+        Nd = len(np.ravel(kx)) // Ns
+        h = _fn("orc_pcsense_dft_create", prec)(
+            Nd, Ns, _p(_r(kx, prec)), _p(_r(ky, prec)), _p(_r(kz, prec)), Nx, Ny, Nz, nc,
+            _p(_r(t, prec)), _p(_c(sens, prec)), _p(_r(fieldmap, prec)), _p(_r(shot_phase, prec)))
         super().__init__(h, prec)
         self.Nd, self.Ns, self.Nc = Nd, Ns, nc
 

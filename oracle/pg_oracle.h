@@ -89,6 +89,11 @@ int orc_get_threads(void);
                                const R* t, int L, int interptype,             \
                                const R* sens, const R* fm, const R* phase,    \
                                R kernelWidth);                                \
+  /* pcSENSE.cpp:39-153 (an exact DFT per shot) */                            \
+  void* orc_pcsense_dft_create_##S(int Nd, int Ns, const R* kx, const R* ky,  \
+                                   const R* kz, int Nx, int Ny, int Nz,       \
+                                   int Nc, const R* t, const R* sens,         \
+                                   const R* fm, const R* phase);              \
   void orc_op_forward_##S(void* op, const R* in, R* out);                     \
   void orc_op_adjoint_##S(void* op, const R* in, R* out);                     \
   int orc_op_n_in_##S(void* op);                                              \

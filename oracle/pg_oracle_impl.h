@@ -854,6 +854,70 @@ void* FN(orc_pcsense_create)(int Nd, int Ns, const R* kx, const R* ky, const R* 
     return op;
 }
 
+/* pcSENSE.cpp:39-102: the same shot/coil structure over an exact DFT per
+ * shot (Gdft with the shot's samples and times, the field map, image
+ * coordinates (x - N/2)/N formed in double, y fastest, pcSENSE.cpp:62-80).
+ * The reference's maps: forward S_c exp(+i P_s) (expiPMap = conj(exp(-i P)),
+ * :97, used at :117), adjoint conj(S_c) exp(+i P_s) (:98, :142), so the two
+ * operators are not each other's adjoint; restated as written. */
+void* FN(orc_pcsense_dft_create)(int Nd, int Ns, const R* kx, const R* ky, const R* kz, int Nx,
+                                 int Ny, int Nz, int Nc, const R* t, const R* sens, const R* fm,
+                                 const R* phase)
+{
+    FN(pcsense_ctx)* c = (FN(pcsense_ctx)*)calloc(1, sizeof(*c));
+    const int Ni = Nx * Ny * Nz;
+    c->Ns = Ns;
+    c->Nc = Nc;
+    c->Nd = Nd;
+    c->Ni = Ni;
+    c->G = (FN(orc_op)**)calloc((size_t)Ns, sizeof(void*));
+    c->A = (FN(orc_op)**)calloc((size_t)Ns, sizeof(void*));
+    c->tmp_in = FN(dup)(NULL, 2 * (size_t)Ni);
+    c->tmp_out = FN(dup)(NULL, 2 * (size_t)Ni);
+    R* ix = FN(dup)(NULL, (size_t)Ni);
+    R* iy = FN(dup)(NULL, (size_t)Ni);
+    R* iz = FN(dup)(NULL, (size_t)Ni);
+    for (int z = 0; z < Nz; ++z)
+        for (int x = 0; x < Nx; ++x)
+            for (int y = 0; y < Ny; ++y) {
+                const size_t n = (size_t)y + (size_t)x * Ny + (size_t)z * Nx * Ny;
+                ix[n] = (R)(((double)(R)x - (double)(R)Nx / 2.0) / (double)(R)Nx);
+                iy[n] = (R)(((double)(R)y - (double)(R)Ny / 2.0) / (double)(R)Ny);
+                iz[n] = (R)(((double)(R)z - (double)(R)Nz / 2.0) / (double)(R)Nz);
+            }
+    for (int s = 0; s < Ns; ++s) {
+        const size_t o = (size_t)s * Nd;
+        c->A[s] = (FN(orc_op)*)FN(orc_gdft_create)(Nd, Ni, kx + o, ky + o, kz + o, ix, iy, iz,
+                                                    fm, t + o);
+    }
+    free(ix);
+    free(iy);
+    free(iz);
+    c->SS = FN(dup)(NULL, (size_t)2 * Ni * Nc * Ns);
+    c->CS = FN(dup)(NULL, (size_t)2 * Ni * Nc * Ns);
+    for (int cc = 0; cc < Nc; ++cc)
+        for (int s = 0; s < Ns; ++s) {
+            const size_t col = (size_t)s + (size_t)cc * Ns;
+            for (int n = 0; n < Ni; ++n) {
+                const double sr = sens[2 * ((size_t)cc * Ni + n)],
+                             si = sens[2 * ((size_t)cc * Ni + n) + 1];
+                const double ph = phase[(size_t)s * Ni + n];
+                const double er = cos(ph), ei = sin(ph); /* exp(+i P) */
+                c->SS[2 * (col * Ni + n)] = (R)(sr * er - si * ei);
+                c->SS[2 * (col * Ni + n) + 1] = (R)(sr * ei + si * er);
+                /* conj(S) exp(+i P) */
+                c->CS[2 * (col * Ni + n)] = (R)(sr * er + si * ei);
+                c->CS[2 * (col * Ni + n) + 1] = (R)(sr * ei - si * er);
+            }
+        }
+    FN(orc_op)* op = (FN(orc_op)*)calloc(1, sizeof(*op));
+    op->kind = 5;
+    op->n_in = Ni;
+    op->n_out = Nd * Ns * Nc;
+    op->ctx = c;
+    return op;
+}
+
 static inline void FN(cmul)(R ar, R ai, R br, R bi, R* cr, R* ci)
 {
     *cr = ar * br - ai * bi;

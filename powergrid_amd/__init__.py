@@ -281,6 +281,30 @@ class PcSenseTimeSeg(_Op):
         self.Ns, self.Nd, self.nc = Ns, Nd, nc
 
 
+class PcSENSE(_Op):
+    """pcSENSE<float>(kx, ky, kz, nx, ny, nz, nc, t, SENSEmap, FieldMap,
+    ShotPhaseMap) (pcSENSE.cpp:39-102): an exact DFT per shot.  The forward
+    applies S_c exp(+i P_s) and the adjoint conj(S_c) exp(+i P_s), as in the
+    reference (pcSENSE.cpp:97-98)."""
+
+    def __init__(self, kx, ky, kz, nx, ny, nz, nc, t, sens, fieldmap, shot_phase):
+        Ni = nx * ny * nz
+        ph = np.ascontiguousarray(np.ravel(shot_phase), dtype=np.float32)
+        Ns = ph.size // Ni
+        assert Ns >= 1 and Ns * Ni == ph.size
+        Mt = len(np.ravel(kx))
+        assert Mt % Ns == 0
+        Nd = Mt // Ns
+        arrs = [_f32(kx, Mt), _f32(ky, Mt), None if kz is None else _f32(kz, Mt), _f32(t, Mt)]
+        s = np.ascontiguousarray(np.ravel(sens), dtype=np.complex64)
+        assert s.size == Ni * nc
+        fm = _f32(fieldmap, Ni)
+        h = _create(lib().pg_pcsense_gdft_create, _p(arrs[0]), _p(arrs[1]), _p(arrs[2]), nx, ny,
+                    nz, nc, _p(arrs[3]), _p(s), _p(fm), _p(ph), Ns, Nd)
+        super().__init__(h)
+        self.Ns, self.Nd, self.nc = Ns, Nd, nc
+
+
 class SENSE(_Op):
     """SENSE<float, G>(G, sens, M, N, nc) (SENSE.cpp:32-44); sens holds nc
     coil maps of N values, coil-major."""
@@ -434,7 +458,7 @@ class profile:
         return n.value, ms.value
 
 
-__all__ = ["DeviceArray", "Stream", "Gnufft", "Gdft", "TimeSegmentation", "SENSE", "PcSenseTimeSeg",
+__all__ = ["DeviceArray", "Stream", "Gnufft", "Gdft", "TimeSegmentation", "SENSE", "PcSenseTimeSeg", "PcSENSE",
            "QuadPenalty", "TVPenalty", "calc3DDensityCompensation", "grid_coil_images",
            "calcSumOfSquaresImage", "solve_pwls_pcg", "reconSolve", "profile", "PgError", "MODE_BATCHED", "MODE_PLAIN",
            "CG_GRAPH", "CG_HOST"]

@@ -680,7 +680,7 @@ static void gnufft_setup(Gnufft& G, size_t M, float os, size_t Nx, size_t Ny, si
 }
 
 // ----------------------------------------------------------- operators --
-enum Kind { K_NUFFT = 1, K_DFT = 2, K_TS = 3, K_SENSE = 4, K_PCSENSE = 5 };
+enum Kind { K_NUFFT = 1, K_DFT = 2, K_TS = 3, K_SENSE = 4, K_PCSENSE = 5, K_PCSENSE_DFT = 6 };
 
 struct SolveCache;
 struct Op {
@@ -789,6 +789,27 @@ struct PcSenseOp : Op {
         for (Op* o : G) delete o;
         dev_free(SS);
         dev_free(P);
+        dev_free(tmp_n);
+    }
+};
+
+// pcSENSE.cpp:39-153: Ns shots, each an exact DFT (Gdft) over its Nd
+// samples.  The reference's maps are S_c exp(+i P_s) in the forward and
+// conj(S_c) exp(+i P_s) in the adjoint (expiPMap = conj(exp(-i P)),
+// pcSENSE.cpp:97-98,117,142).  Stored [s][c][n]: SF = S_c exp(+i P_s), and
+// SA = S_c exp(-i P_s), whose conjugate the DFT adjoint kernel applies.
+struct PcDftOp : Op {
+    int Ns = 1, Nc = 1;
+    size_t Nd = 0, Ni = 0;
+    std::vector<DftOp*> D;  // per shot (owned)
+    float2* SF = nullptr;
+    float2* SA = nullptr;
+    float2* tmp_n = nullptr;
+    ~PcDftOp() override
+    {
+        for (DftOp* d : D) delete d;
+        dev_free(SF);
+        dev_free(SA);
         dev_free(tmp_n);
     }
 };
@@ -917,6 +938,13 @@ static void op_prepare(Op* op)
         }
         return;
     }
+    case K_PCSENSE_DFT: {
+        PcDftOp* pc = static_cast<PcDftOp*>(op);
+        for (DftOp* d : pc->D)
+            d->ensure_part(std::max(pgd::part_elems(d->d.M, d->d.N, pc->Nc, false),
+                                    pgd::part_elems(d->d.M, d->d.N, pc->Nc, true)));
+        return;
+    }
     case K_SENSE: {
         SenseOp* se = static_cast<SenseOp*>(op);
         if (se->inner->kind == K_DFT) {
@@ -1002,6 +1030,29 @@ static void op_forward(Op* op, const float2* in, float2* out, hipStream_t s)
                 for (int c = 0; c < pc->Nc; ++c) {
                     pgv::cmul(pc->tmp_n, SSs + (size_t)c * Ni, in, Ni, s);
                     op_forward(ts, pc->tmp_n, out + ((size_t)sh + (size_t)c * pc->Ns) * Nd, s);
+                }
+            }
+        }
+        return;
+    }
+    case K_PCSENSE_DFT: {
+        // pcSENSE.cpp:108-127
+        PcDftOp* pc = static_cast<PcDftOp*>(op);
+        const long long Ni = (long long)pc->Ni, Nd = (long long)pc->Nd;
+        for (int sh = 0; sh < pc->Ns; ++sh) {
+            DftOp* d = pc->D[sh];
+            const float2* SFs = pc->SF + (size_t)sh * pc->Nc * Ni;
+            Prof p("gdft_fwd", s);
+            if (op->mode == PG_MODE_BATCHED) {
+                // every coil of the shot in one launch; its columns are Ns*Nd apart
+                d->ensure_part(pgd::part_elems(d->d.M, d->d.N, pc->Nc, false));
+                pgd::forward(d->d, in, SFs, pc->Nc, out + sh * Nd, d->part, s, pc->Ns * Nd);
+            } else {
+                d->ensure_part(pgd::part_elems(d->d.M, d->d.N, 1, false));
+                for (int c = 0; c < pc->Nc; ++c) {
+                    pgv::cmul(pc->tmp_n, SFs + (size_t)c * Ni, in, Ni, s);
+                    pgd::forward(d->d, pc->tmp_n, nullptr, 1,
+                                 out + ((size_t)sh + (size_t)c * pc->Ns) * Nd, d->part, s);
                 }
             }
         }
@@ -1093,6 +1144,33 @@ static void op_adjoint(Op* op, const float2* in, float2* out, hipStream_t s)
             for (int sh = 0; sh < pc->Ns; ++sh) {
                 op_adjoint(pc->A[sh], in + ((size_t)sh + (size_t)c * pc->Ns) * Nd, pc->tmp_n, s);
                 pgv::cmul_acc(out, pc->tmp_n, pc->SS + ((size_t)sh * pc->Nc + c) * Ni, Ni,
+                              c == 0 && sh == 0, true, s);
+            }
+        return;
+    }
+    case K_PCSENSE_DFT: {
+        // pcSENSE.cpp:131-153
+        PcDftOp* pc = static_cast<PcDftOp*>(op);
+        const long long Ni = (long long)pc->Ni, Nd = (long long)pc->Nd;
+        Prof p("gdft_adj", s);
+        if (op->mode == PG_MODE_BATCHED) {
+            // per shot, every coil in one launch, accumulated over shots
+            for (int sh = 0; sh < pc->Ns; ++sh) {
+                DftOp* d = pc->D[sh];
+                d->ensure_part(pgd::part_elems(d->d.M, d->d.N, pc->Nc, true));
+                pgd::adjoint(d->d, in + sh * Nd, pc->SA + (size_t)sh * pc->Nc * Ni, pc->Nc, out,
+                             d->part, s, pc->Ns * Nd, sh > 0);
+            }
+            return;
+        }
+        // the reference's order: coils outer, shots inner
+        for (int c = 0; c < pc->Nc; ++c)
+            for (int sh = 0; sh < pc->Ns; ++sh) {
+                DftOp* d = pc->D[sh];
+                d->ensure_part(pgd::part_elems(d->d.M, d->d.N, 1, true));
+                pgd::adjoint(d->d, in + ((size_t)sh + (size_t)c * pc->Ns) * Nd, nullptr, 1,
+                             pc->tmp_n, d->part, s);
+                pgv::cmul_acc(out, pc->tmp_n, pc->SA + ((size_t)sh * pc->Nc + c) * Ni, Ni,
                               c == 0 && sh == 0, true, s);
             }
         return;
@@ -1604,6 +1682,62 @@ int pg_pcsense_create(pg_op* out, const float* kx, const float* ky, const float*
             }
     op->SS = dev_copy(ss.data(), ss.size());
     op->P = dev_alloc<float2>(n_shots * nc * Ni);
+    op->tmp_n = dev_alloc<float2>(Ni);
+    *out = reinterpret_cast<pg_op>(op.release());
+    PG_CATCH
+}
+
+int pg_pcsense_gdft_create(pg_op* out, const float* kx, const float* ky, const float* kz,
+                           size_t Nx, size_t Ny, size_t Nz, size_t nc, const float* tvec,
+                           const float* sens, const float* fieldmap, const float* shotphase,
+                           size_t n_shots, size_t Nd)
+{
+    PG_TRY PG_REQUIRE(out && kx && ky && tvec && sens && fieldmap && shotphase, PG_ERR_ARG,
+                      "null argument");
+    PG_REQUIRE(n_shots > 0 && Nd > 0 && nc > 0, PG_ERR_ARG, "empty pcSENSE");
+    const size_t Ni = Nx * Ny * Nz;
+    PG_REQUIRE(Ni > 0 && Ni < (1u << 31) && Nd < (1u << 31), PG_ERR_ARG, "bad sizes");
+    *out = nullptr;
+    std::unique_ptr<PcDftOp> op(new PcDftOp());
+    op->kind = K_PCSENSE_DFT;
+    op->Ns = (int)n_shots;
+    op->Nc = (int)nc;
+    op->Nd = Nd;
+    op->Ni = Ni;
+    op->n_in = Ni;
+    op->n_out = Nd * n_shots * nc;
+    // pcSENSE.cpp:62-84: (x - N/2)/N in double, rounded to float, y fastest
+    std::vector<float> ix(Ni), iy(Ni), iz(Ni), zk(Nd, 0.0f);
+    for (size_t z = 0; z < Nz; ++z)
+        for (size_t x = 0; x < Nx; ++x)
+            for (size_t y = 0; y < Ny; ++y) {
+                const size_t n = y + x * Ny + z * Nx * Ny;
+                ix[n] = (float)(((double)(float)x - (double)(float)Nx / 2.0) / (double)(float)Nx);
+                iy[n] = (float)(((double)(float)y - (double)(float)Ny / 2.0) / (double)(float)Ny);
+                iz[n] = (float)(((double)(float)z - (double)(float)Nz / 2.0) / (double)(float)Nz);
+            }
+    // pcSENSE.cpp:88-93: Gdft(Nd, Ni, Kx.col(s), ..., FMap, Tvec.col(s))
+    for (size_t sh = 0; sh < n_shots; ++sh) {
+        const size_t o = sh * Nd;
+        pg_op h = nullptr;
+        const int st = pg_gdft_create(&h, Nd, Ni, kx + o, ky + o, kz ? kz + o : zk.data(), ix.data(),
+                                      iy.data(), iz.data(), fieldmap, tvec + o, 0);
+        if (st != PG_OK) throw pg::Error{st, pg::g_last_error};
+        op->D.push_back(reinterpret_cast<DftOp*>(h));
+    }
+    std::vector<float2> sf(n_shots * nc * Ni), sa(n_shots * nc * Ni);
+    for (size_t sh = 0; sh < n_shots; ++sh)
+        for (size_t c = 0; c < nc; ++c)
+            for (size_t n = 0; n < Ni; ++n) {
+                const double sr = sens[2 * (c * Ni + n)], si = sens[2 * (c * Ni + n) + 1];
+                const double ph = shotphase[sh * Ni + n];
+                const double er = std::cos(ph), ei = std::sin(ph);
+                const size_t k = (sh * nc + c) * Ni + n;
+                sf[k] = make_float2((float)(sr * er - si * ei), (float)(sr * ei + si * er));
+                sa[k] = make_float2((float)(sr * er + si * ei), (float)(si * er - sr * ei));
+            }
+    op->SF = dev_copy(sf.data(), sf.size());
+    op->SA = dev_copy(sa.data(), sa.size());
     op->tmp_n = dev_alloc<float2>(Ni);
     *out = reinterpret_cast<pg_op>(op.release());
     PG_CATCH

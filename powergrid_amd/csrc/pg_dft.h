@@ -13,11 +13,12 @@ struct DftData {
 
 // partial-sum buffer size (float2 elements) needed by forward/adjoint
 size_t part_elems(int M, int N, int nc, bool adjoint);
-// out[c*M + i] for c < nc; S may be null (nc must then be 1)
+// out[c*ostride + i] for c < nc (ostride -1: M); S may be null (nc must then be 1)
 void forward(const DftData& d, const float2* x, const float2* S, int nc, float2* out, float2* part,
-             hipStream_t s);
-// out[j] = sum_c conj(S_c[j]) (E^H y_c)[j]
+             hipStream_t s, long long ostride = -1);
+// out[j] (+)= sum_c conj(S_c[j]) (E^H y_c)[j], y_c at y + c*ystride (-1: M);
+// accumulate adds to out
 void adjoint(const DftData& d, const float2* y, const float2* S, int nc, float2* out, float2* part,
-             hipStream_t s);
+             hipStream_t s, long long ystride = -1, bool accumulate = false);
 
 }  // namespace pgd

@@ -112,7 +112,8 @@ __global__ void __launch_bounds__(kT) k_gdft_adj(
     const float* __restrict__ kx, const float* __restrict__ ky, const float* __restrict__ kz,
     const float* __restrict__ t, const float* __restrict__ ix, const float* __restrict__ iy,
     const float* __restrict__ iz, const float* __restrict__ fm, const float2* __restrict__ y,
-    const float2* __restrict__ S, int nc, int M, int N, int slice_len, float2* __restrict__ part)
+    long long ystride, const float2* __restrict__ S, int nc, int M, int N, int slice_len,
+    float2* __restrict__ part)
 {
     __shared__ float s_k[kT][4];
     __shared__ float2 s_v[kT][NCB];
@@ -147,7 +148,7 @@ __global__ void __launch_bounds__(kT) k_gdft_adj(
                 s_k[threadIdx.x][3] = t[i];
 #pragma unroll
                 for (int c = 0; c < NCB; ++c)
-                    s_v[threadIdx.x][c] = (c0 + c < nc) ? y[(long long)(c0 + c) * M + i]
+                    s_v[threadIdx.x][c] = (c0 + c < nc) ? y[(long long)(c0 + c) * ystride + i]
                                                         : make_float2(0.f, 0.f);
             }
             __syncthreads();
@@ -204,7 +205,10 @@ __global__ void __launch_bounds__(kT) k_gdft_adj(
     if (j < N) part[(long long)blockIdx.y * N + j] = F64 ? make_float2((float)xr, (float)xi) : make_float2(gr, gi);
 }
 
-__global__ void k_sum_slices(const float2* part, int nslice, long long n, float2* out)
+// out[(i / row) * ostride + i % row] (+)= sum of the slices of element i, in
+// slice order
+__global__ void k_sum_slices(const float2* part, int nslice, long long n, long long row,
+                             long long ostride, int acc, float2* out)
 {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -214,7 +218,13 @@ __global__ void k_sum_slices(const float2* part, int nslice, long long n, float2
         s.x += p.x;
         s.y += p.y;
     }
-    out[i] = s;
+    const long long r = i / row;
+    const long long o = r * ostride + (i - r * row);
+    if (acc) {
+        const float2 a = out[o];
+        s = make_float2(a.x + s.x, a.y + s.y);
+    }
+    out[o] = s;
 }
 
 int slices_for(long long outputs, long long inputs)
@@ -237,8 +247,9 @@ size_t part_elems(int M, int N, int nc, bool adjoint)
 }
 
 void forward(const DftData& d, const float2* x, const float2* S, int nc, float2* out, float2* part,
-             hipStream_t s)
+             hipStream_t s, long long ostride)
 {
+    if (ostride < 0) ostride = d.M;
     const int ns = slices_for(d.M, d.N);
     const int slen = (d.N + ns - 1) / ns;
     constexpr int NCB = 8;
@@ -250,24 +261,26 @@ void forward(const DftData& d, const float2* x, const float2* S, int nc, float2*
         hipLaunchKernelGGL((k_gdft_fwd<NCB, false>), grid, dim3(kT), 0, s, d.kx, d.ky, d.kz, d.t, d.ix,
                            d.iy, d.iz, d.fm, x, S, nc, d.M, d.N, slen, part);
     const long long n = (long long)nc * d.M;
-    hipLaunchKernelGGL(k_sum_slices, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, part, ns, n, out);
+    hipLaunchKernelGGL(k_sum_slices, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, part, ns, n,
+                       (long long)d.M, ostride, 0, out);
 }
 
 void adjoint(const DftData& d, const float2* y, const float2* S, int nc, float2* out, float2* part,
-             hipStream_t s)
+             hipStream_t s, long long ystride, bool accumulate)
 {
+    if (ystride < 0) ystride = d.M;
     const int ns = slices_for(d.N, d.M);
     const int slen = (d.M + ns - 1) / ns;
     constexpr int NCB = 8;
     dim3 grid((d.N + kT - 1) / kT, ns, 1);
     if (d.precision == 1)
         hipLaunchKernelGGL((k_gdft_adj<NCB, true>), grid, dim3(kT), 0, s, d.kx, d.ky, d.kz, d.t, d.ix,
-                           d.iy, d.iz, d.fm, y, S, nc, d.M, d.N, slen, part);
+                           d.iy, d.iz, d.fm, y, ystride, S, nc, d.M, d.N, slen, part);
     else
         hipLaunchKernelGGL((k_gdft_adj<NCB, false>), grid, dim3(kT), 0, s, d.kx, d.ky, d.kz, d.t, d.ix,
-                           d.iy, d.iz, d.fm, y, S, nc, d.M, d.N, slen, part);
+                           d.iy, d.iz, d.fm, y, ystride, S, nc, d.M, d.N, slen, part);
     hipLaunchKernelGGL(k_sum_slices, dim3((unsigned)((d.N + 255) / 256)), dim3(256), 0, s, part, ns,
-                       (long long)d.N, out);
+                       (long long)d.N, (long long)d.N, (long long)d.N, accumulate ? 1 : 0, out);
 }
 
 }  // namespace pgd

@@ -6,7 +6,11 @@
 //
 // input : int32 N1(=Nx=Ny), M, nc, L, niter; float beta;
 //         float kx[M], ky[M], t[M], fm[N]; complex<float> sens[N*nc], x[N], y[M*nc]
-// output: complex<float> fwd[M*nc], adj[N], recon[N]; int32 iterations
+// output: complex<float> fwd[M*nc], adj[N], recon[N]; int32 iterations;
+//         then the direct-reconstruction calls (directRecon.cpp): float
+//         dcf[M]; complex<float> coil images[N*nc]; float sos[N]; and
+//         G's spatial pair on coil 0's data: complex<float> grid[Gx*Gy],
+//         samples[M]
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
@@ -70,6 +74,19 @@ int main(int argc, char** argv)
         o.write(reinterpret_cast<const char*>(img.memptr()), sizeof(complex<float>) * img.size());
         const int it = info.iterations;
         o.write(reinterpret_cast<const char*>(&it), sizeof(int));
+
+        Col<float> dcf = calc3DDensityCompensation<float>(kx, ky, kz, Nx, Nz);
+        Col<complex<float>> coil = gridCoilImages<float>(G, dcf, y, nc);
+        Col<float> sos = calcSumOfSquaresImage<float>(coil, Nx, Nz, 1, nc);
+        Col<complex<float>> y0(M);
+        for (uword i = 0; i < M; ++i) y0(i) = y(i);
+        Col<complex<float>> grid = G.adjointSpatialInterp(y0);
+        Col<complex<float>> samp = G.forwardSpatialInterp(grid);
+        o.write(reinterpret_cast<const char*>(dcf.memptr()), sizeof(float) * dcf.size());
+        o.write(reinterpret_cast<const char*>(coil.memptr()), sizeof(complex<float>) * coil.size());
+        o.write(reinterpret_cast<const char*>(sos.memptr()), sizeof(float) * sos.size());
+        o.write(reinterpret_cast<const char*>(grid.memptr()), sizeof(complex<float>) * grid.size());
+        o.write(reinterpret_cast<const char*>(samp.memptr()), sizeof(complex<float>) * samp.size());
         std::cout << "operator_api_test ok: " << it << " iterations\n";
     } catch (const std::exception& e) {
         std::cerr << "operator_api_test failed: " << e.what() << "\n";

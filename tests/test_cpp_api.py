@@ -1,7 +1,9 @@
 """The reference-shaped C++ API (include/powergrid/powergrid.hpp) driven by a
 C++ program in the reference's own call pattern (PowerGridIsmrmrd.cpp:242-251):
-Gnufft -> TimeSegmentation -> SENSE -> QuadPenalty -> reconSolve, checked
-against the FP64 restatement."""
+Gnufft -> TimeSegmentation -> SENSE -> QuadPenalty -> reconSolve, then the
+direct-reconstruction calls (density compensation, gridded coil images,
+coil-combined image, the spatial interpolation pair), checked against the
+FP64 restatement."""
 import os
 import subprocess
 
@@ -52,3 +54,24 @@ def test_cpp_operator_api_matches_oracle(orc, tmp_path):
     xo, n, _ = orc.reconSolve(y, Sd, R, N, N, 1, niter)
     assert it == n == niter
     assert rel(img, xo) < 1e-3
+    # direct reconstruction (directRecon.cpp) through the same API
+    Gn = (2 * N) ** 2
+    off = 8 * (M * nc + 2 * N * N) + 4
+    dcf = np.fromfile(fout, np.float32, count=M, offset=off)
+    off += 4 * M
+    coil = np.fromfile(fout, np.complex64, count=N * N * nc, offset=off)
+    off += 8 * N * N * nc
+    sos = np.fromfile(fout, np.float32, count=N * N, offset=off)
+    off += 4 * N * N
+    grid = np.fromfile(fout, np.complex64, count=Gn, offset=off)
+    off += 8 * Gn
+    samp = np.fromfile(fout, np.complex64, count=M, offset=off)
+    assert samp.size == M
+    dcf64 = orc.density_compensation(kx, ky, kz, N, 1, prec="d")
+    assert rel(dcf, dcf64) < 5e-6
+    coil64 = orc.grid_coil_images(Gd, dcf.astype(np.float64), y.astype(np.complex128), nc)
+    assert rel(coil, coil64) < 5e-6
+    assert rel(sos, orc.sum_of_squares(coil64, N * N, nc)) < 5e-6
+    g64 = orc.grid_adjoint(y[:M].astype(np.complex128), kx, ky, kz, N, N, 1, Gn, prec="d")
+    assert rel(grid, g64) < 5e-6
+    assert rel(samp, orc.grid_forward(g64, kx, ky, kz, N, N, 1, prec="d")) < 5e-6

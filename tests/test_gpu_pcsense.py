@@ -1,5 +1,6 @@
-"""GPU parity of the shot-phase-corrected SENSE operator (pcSenseTimeSeg)
-against the oracle's restatement, through the C ABI.
+"""GPU parity of the shot-phase-corrected SENSE operators (pcSenseTimeSeg,
+and pcSENSE over an exact DFT per shot) against the oracle's restatement,
+through the C ABI.
 
 Tolerances as for the other operators: 1e-4 rel-L2 of the FP32 and 5e-6 of
 the FP64 restatement; PCG within 1e-3 NRMSE of the FP64 same-operator PCG.
@@ -83,4 +84,40 @@ def test_pcsense_pcg_matches_oracle(orc):
         xg, info = P.reconSolve(y, A, R, niter=niter, mode=mode)
         print(f"  mode={mode} nrmse vs fp64 oracle {rel(xg, x64):.2e}")
         assert info["iterations"] == n64 == niter
+        assert rel(xg, x64) < 1e-3
+
+
+@pytest.mark.parametrize("mode", [P.MODE_BATCHED, P.MODE_PLAIN])
+def test_pcsense_gdft_matches_oracle(orc, mode):
+    # pcSENSE.cpp:39-153 over an exact DFT per shot; FP32 hardware trig, so
+    # 1e-4 against both restatements (whose sin/cos are FP32 as well)
+    print(f"[pcsense gdft mode={mode}]", flush=True)
+    Ns, Nd, N, nc = 3, 512, 16, 4
+    kx, ky, kz, t, fm, sens, phase = case(Ns, Nd, N, nc, seed=2)
+    A = P.PcSENSE(kx, ky, kz, N, N, 1, nc, t, sens, fm, phase)
+    A.set_mode(mode)
+    assert A.n_in == N * N and A.n_out == Nd * Ns * nc
+    x = S.ellipse_phantom(N, seed=1).astype(np.complex64)
+    for prec in "fd":
+        O = orc.PcSENSE(kx, ky, kz, N, N, 1, nc, t, sens.ravel(), fm, phase.ravel(), prec=prec)
+        y = (O * x).astype(np.complex64)
+        assert rel(A * x, O * x) < 1e-4
+        assert rel(A / y, O / y) < 1e-4
+
+
+def test_pcsense_gdft_pcg_matches_oracle(orc):
+    print("[pcsense gdft pcg]", flush=True)
+    Ns, Nd, N, nc, niter = 2, 512, 16, 3, 5
+    kx, ky, kz, t, fm, sens, phase = case(Ns, Nd, N, nc, seed=7)
+    A = P.PcSENSE(kx, ky, kz, N, N, 1, nc, t, sens, fm, phase)
+    Od = orc.PcSENSE(kx, ky, kz, N, N, 1, nc, t, sens.ravel(), fm, phase.ravel(), prec="d")
+    x = S.ellipse_phantom(N, seed=3)
+    y = S.add_noise(Od * x, 40.0, seed=5).astype(np.complex64)
+    R = P.QuadPenalty(N, N, 1, 1e2)
+    Ro = orc.QuadPenalty(N, N, 1, 1e2, prec="d")
+    x64, n64, _ = orc.reconSolve(y.astype(np.complex128), Od, Ro, N, N, 1, niter)
+    for mode in (P.CG_GRAPH, P.CG_HOST):
+        xg, info = P.reconSolve(y, A, R, niter=niter, mode=mode)
+        print(f"  mode={mode} nrmse vs fp64 oracle {rel(xg, x64):.2e}")
+        assert info["iterations"] == n64
         assert rel(xg, x64) < 1e-3

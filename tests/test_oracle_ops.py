@@ -325,3 +325,58 @@ def test_pcsense_adjointness(orc):
     y = rng.standard_normal(A.n_out) + 1j * rng.standard_normal(A.n_out)
     lhs, rhs = np.vdot(y, A * x), np.vdot(A / y, x)
     assert abs(lhs - rhs) < 1e-10 * abs(lhs)
+
+
+def _dense_dft(kx, ky, kz, t, fm, N):
+    """E[i, j] = exp(-i (2 pi k_i . r_j + fm_j t_i)), r = ((x - N/2)/N, (y - N/2)/N),
+    y fastest (pcSENSE.cpp:62-80, ftCpu.cpp:94-140), FP64."""
+    x, y = np.meshgrid(np.arange(N), np.arange(N), indexing="ij")
+    rx = ((x - N / 2) / N).ravel()
+    ry = ((y - N / 2) / N).ravel()
+    ph = 2 * np.pi * (np.outer(kx, rx) + np.outer(ky, ry)) + np.outer(t, fm)
+    return np.exp(-1j * ph)
+
+
+def test_pcsense_dft_matches_dense_blocks_with_reference_phase_signs(orc):
+    # pcSENSE.cpp:108-153: forward column s + c*Ns = E_s (S_c exp(+i P_s) x);
+    # adjoint sum_c sum_s conj(S_c) exp(+i P_s) E_s^H d_(s,c)
+    Ns, Nd, N, nc = 2, 96, 8, 2
+    kx, ky, kz, t, fm, sens, phase = _pcs_case(Ns, Nd, N, nc, 1, seed=6)
+    A = orc.PcSENSE(kx, ky, kz, N, N, 1, nc, t, sens.ravel(), fm, phase.ravel(), prec="d")
+    rng = np.random.default_rng(8)
+    x = rng.standard_normal(N * N) + 1j * rng.standard_normal(N * N)
+    d = rng.standard_normal(A.n_out) + 1j * rng.standard_normal(A.n_out)
+    y = A * x
+    xa = A / d
+    yr = np.zeros(A.n_out, complex)
+    xr = np.zeros(N * N, complex)
+    S_ = sens.astype(complex).reshape(nc, -1)
+    for s in range(Ns):
+        sl = slice(s * Nd, (s + 1) * Nd)
+        E = _dense_dft(kx[sl].astype(float), ky[sl].astype(float), None, t[sl].astype(float),
+                       fm.astype(float), N)
+        e = np.exp(1j * phase[s].astype(float))
+        for c in range(nc):
+            col = s + c * Ns
+            yr[col * Nd:(col + 1) * Nd] = E @ (S_[c] * e * x)
+            xr += np.conj(S_[c]) * e * (E.conj().T @ d[col * Nd:(col + 1) * Nd])
+    # the restatement's sin/cos are FP32 (ftCpu.cpp:129-130), ~1e-7
+    assert rel(y, yr) < 1e-6
+    assert rel(xa, xr) < 1e-6
+    # with a non-zero shot phase the pair is not adjoint (the reference's signs)
+    lhs, rhs = np.vdot(d, y), np.vdot(xa, x)
+    assert abs(lhs - rhs) > 1e-3 * abs(lhs)
+
+
+def test_pcsense_dft_zero_phase_one_shot_is_sense_gdft(orc):
+    kx, ky, kz, t, fm, sens, _ = _pcs_case(Ns=1)
+    N, nc, M = 16, 2, len(kx)
+    ix, iy, iz = S.image_coords(N, N)
+    A = orc.PcSENSE(kx, ky, kz, N, N, 1, nc, t, sens.ravel(), fm, np.zeros(N * N, np.float32),
+                    prec="f")
+    D = orc.Gdft(M, N * N, kx, ky, kz, ix, iy, iz, fm, t, prec="f")
+    B = orc.SENSE(D, sens.ravel(), M, N * N, nc)
+    x = S.ellipse_phantom(N, seed=2).astype(np.complex64)
+    np.testing.assert_array_equal(A * x, B * x)
+    y = B * x
+    np.testing.assert_array_equal(A / y, B / y)
