@@ -84,6 +84,17 @@ int pg_gdft_create(pg_op* out, size_t M, size_t N, const float* kx, const float*
                    const float* kz, const float* ix, const float* iy, const float* iz,
                    const float* fm, const float* t, int precision);
 
+/* replaces GdftR2<float>::GdftR2 (GdftR2.cpp:32-57, GdftR2.h:45-48): Gdft
+ * (precision 0) whose every (sample, voxel) pair is weighted by
+ * sinc(kx/Nx + Gx t) sinc(ky/Ny + Gy t) sinc(kz/Nz + Gz t)
+ * (ftCpuWithGrads.cpp:98-212), G = the field map's differences along the
+ * image's offsets 1, Ny and Nx*Ny (GdftR2.cpp:67-107).  In 2-D the
+ * reference's z map reads outside the field map; it is zero here.
+ * N must equal Nx*Ny*Nz. */
+int pg_gdftr2_create(pg_op* out, size_t M, size_t N, const float* kx, const float* ky,
+                     const float* kz, const float* ix, const float* iy, const float* iz,
+                     const float* fm, const float* t, size_t Nx, size_t Ny, size_t Nz);
+
 /* replaces TimeSegmentation<float,G>::TimeSegmentation
  * (TimeSegmentation.cpp:36-221, TimeSegmentation.h:70-71).  Only the
  * Hanning interpolator (interptype 1) is implemented; G is not owned and

@@ -69,6 +69,9 @@ def _declare(L):
         f("orc_dense_create", _vp, _vp, C.c_int, C.c_int)
         f("orc_pcsense_create", _vp, C.c_int, C.c_int, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int,
           C.c_int, _vp, C.c_int, C.c_int, _vp, _vp, _vp, R)
+        f("orc_gdftr2_create", _vp, C.c_int, C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+          C.c_int, C.c_int, C.c_int)
+        f("orc_gdftr2_grad", None, _vp, C.c_int, C.c_int, C.c_int, C.c_int, _vp)
         f("orc_pcsense_dft_create", _vp, C.c_int, C.c_int, _vp, _vp, _vp, C.c_int, C.c_int,
           C.c_int, C.c_int, _vp, _vp, _vp, _vp)
         f("orc_op_forward", None, _vp, _vp, _vp)
@@ -265,6 +268,23 @@ class Gdft(_Op):
         arrs = [_r(a, prec) for a in (kx, ky, kz, ix, iy, iz, fm, t)]
         h = _fn("orc_gdft_create", prec)(M, N, *[_p(a) for a in arrs])
         super().__init__(h, prec)
+
+
+class GdftR2(_Op):
+    """GdftR2.cpp:32-207: the field-corrected DFT with the per-pair factor
+    sinc(kx/Nx + Gx t) sinc(ky/Ny + Gy t) sinc(kz/Nz + Gz t), G = Cd(FM, dim)."""
+
+    def __init__(self, M, N, kx, ky, kz, ix, iy, iz, fm, t, nx, ny, nz, prec="f"):
+        args = [_p(_r(a, prec)) for a in (kx, ky, kz, ix, iy, iz, fm, t)]
+        h = _fn("orc_gdftr2_create", prec)(M, N, *args, nx, ny, nz)
+        super().__init__(h, prec)
+
+
+def gdftr2_grad(fm, nx, ny, nz, dim, prec="f"):
+    """GdftR2::Cd (GdftR2.cpp:67-107) of the field map along dim."""
+    out = np.zeros(nx * ny * nz, dtype=_rt(prec))
+    _fn("orc_gdftr2_grad", prec)(_p(_r(fm, prec)), nx, ny, nz, dim, _p(out))
+    return out
 
 
 class TimeSegmentation(_Op):

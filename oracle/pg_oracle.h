@@ -79,6 +79,13 @@ int orc_get_threads(void);
   void* orc_gdft_create_##S(int M, int N, const R* kx, const R* ky,           \
                             const R* kz, const R* ix, const R* iy,            \
                             const R* iz, const R* fm, const R* t);            \
+  /* GdftR2.cpp:32-107, ftCpuWithGrads.cpp:98-212 */                         \
+  void* orc_gdftr2_create_##S(int M, int N, const R* kx, const R* ky,         \
+                              const R* kz, const R* ix, const R* iy,          \
+                              const R* iz, const R* fm, const R* t, int nx,   \
+                              int ny, int nz);                                \
+  void orc_gdftr2_grad_##S(const R* fm, int nx, int ny, int nz, int dim,      \
+                           R* out);                                           \
   void* orc_ts_create_##S(void* G, const R* fm, const R* t, int M, int N,     \
                           int L, int interptype, int shots);                  \
   void* orc_sense_create_##S(void* G, const R* sens, int M, int N, int nc);   \

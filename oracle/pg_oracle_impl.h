@@ -553,6 +553,9 @@ typedef struct {
 typedef struct {
     int M, N;
     R *kx, *ky, *kz, *ix, *iy, *iz, *fm, *t;
+    /* GdftR2 only: field-map difference maps and the image dimensions */
+    R *gx, *gy, *gz;
+    int numx, numy, numz;
 } FN(dft_ctx);
 
 typedef struct {
@@ -666,6 +669,81 @@ static void FN(nufft_adj)(FN(nufft_ctx) * c, const R* in, R* out)
     FN(orc_deapod)(out, c->img_d, c->Nx, c->Ny, c->Nz, c->W, c->beta, c->os);
 }
 
+/* ftCpuWithGrads.cpp:86-96: sin(pi x)/(pi x), 1 below 1e-4, in R with the
+ * float literal pi of the float build */
+static inline R FN(sinc_pg)(R x)
+{
+    if (fabs(x) < 0.0001) return (R)1.0;
+    return (R)(sin(PI_R * x) / (PI_R * x));
+}
+
+/* GdftR2.cpp:60-107, Cd(FM, dim): d[i + o] - d[i - o] for o <= i < n - o,
+ * d[i + o] - d[i] below, d[i] - d[i - o] above, o = 1 (dim 0), Ny (dim 1),
+ * Nx*Ny (dim 2), in R.  With Nz == 1 the dim-2 offset is the whole map and
+ * the reference indexes outside it (undefined); the map is 0 there. */
+void FN(orc_gdftr2_grad)(const R* fm, int nx, int ny, int nz, int dim, R* out)
+{
+    const long n = (long)nx * ny * nz;
+    const long o = dim == 0 ? 1 : dim == 1 ? ny : (long)nx * ny;
+    if (o >= n) {
+        for (long i = 0; i < n; ++i) out[i] = 0;
+        return;
+    }
+    for (long i = o; i < n - o; ++i) out[i] = fm[i + o] - fm[i - o];
+    for (long i = 0; i < o; ++i) out[i] = fm[i + o] - fm[i];
+    for (long i = n - o; i < n; ++i) out[i] = fm[i] - fm[i - o];
+}
+
+/* ftCpuWithGrads.cpp:98-137: ftCpu with the factor
+ * sinc(kx/Nx + Gx t) sinc(ky/Ny + Gy t) sinc(kz/Nz + Gz t) per pair */
+static void FN(ft_grads)(const FN(dft_ctx) * c, const R* in, R* out)
+{
+#pragma omp parallel for num_threads(orc_nthreads()) schedule(static)
+    for (int i = 0; i < c->M; i++) {
+        R sumr = 0, sumi = 0;
+        R kxtpi = c->kx[i] * 2 * PI_R;
+        R kytpi = c->ky[i] * 2 * PI_R;
+        R kztpi = c->kz[i] * 2 * PI_R;
+        R myti = c->t[i];
+        for (int j = 0; j < c->N; j++) {
+            R expr = (kxtpi * c->ix[j] + kytpi * c->iy[j] + kztpi * c->iz[j] + (c->fm[j] * myti));
+            float se = sinf((float)expr);
+            float ce = cosf((float)expr);
+            R b = FN(sinc_pg)(c->kx[i] / c->numx + c->gx[j] * myti) *
+                  FN(sinc_pg)(c->ky[i] / c->numy + c->gy[j] * myti) *
+                  FN(sinc_pg)(c->kz[i] / c->numz + c->gz[j] * myti);
+            sumr += b * ((ce * in[2 * j]) + (se * in[2 * j + 1]));
+            sumi += b * ((-se * in[2 * j]) + (ce * in[2 * j + 1]));
+        }
+        out[2 * i] = sumr;
+        out[2 * i + 1] = sumi;
+    }
+}
+
+/* ftCpuWithGrads.cpp:150-212 */
+static void FN(ift_grads)(const FN(dft_ctx) * c, const R* in, R* out)
+{
+    R tpi = PI_R * 2.0;
+#pragma omp parallel for num_threads(orc_nthreads()) schedule(static)
+    for (int j = 0; j < c->N; j++) {
+        R sumr = 0, sumi = 0;
+        R itx = c->ix[j] * tpi, ity = c->iy[j] * tpi, itz = c->iz[j] * tpi;
+        R myfmj = c->fm[j];
+        for (int i = 0; i < c->M; i++) {
+            R expr = (c->kx[i] * itx + c->ky[i] * ity + c->kz[i] * itz + (myfmj * c->t[i]));
+            R se = (R)sinf((float)expr);
+            R ce = (R)cosf((float)expr);
+            R b = FN(sinc_pg)(c->kx[i] / c->numx + c->gx[j] * c->t[i]) *
+                  FN(sinc_pg)(c->ky[i] / c->numy + c->gy[j] * c->t[i]) *
+                  FN(sinc_pg)(c->kz[i] / c->numz + c->gz[j] * c->t[i]);
+            sumr += b * ((ce * in[2 * i]) - (se * in[2 * i + 1]));
+            sumi += b * ((se * in[2 * i]) + (ce * in[2 * i + 1]));
+        }
+        out[2 * j] = sumr;
+        out[2 * j + 1] = sumi;
+    }
+}
+
 /* Gdft.cpp:32-49 */
 void* FN(orc_gdft_create)(int M, int N, const R* kx, const R* ky, const R* kz,
                           const R* ix, const R* iy, const R* iz, const R* fm, const R* t)
@@ -686,6 +764,25 @@ void* FN(orc_gdft_create)(int M, int N, const R* kx, const R* ky, const R* kz,
     op->n_in = N;
     op->n_out = M;
     op->ctx = c;
+    return op;
+}
+
+/* GdftR2.cpp:32-57 */
+void* FN(orc_gdftr2_create)(int M, int N, const R* kx, const R* ky, const R* kz, const R* ix,
+                            const R* iy, const R* iz, const R* fm, const R* t, int nx, int ny,
+                            int nz)
+{
+    FN(orc_op)* op = (FN(orc_op)*)FN(orc_gdft_create)(M, N, kx, ky, kz, ix, iy, iz, fm, t);
+    FN(dft_ctx)* c = (FN(dft_ctx)*)op->ctx;
+    c->numx = nx;
+    c->numy = ny;
+    c->numz = nz;
+    c->gx = FN(dup)(NULL, (size_t)N);
+    c->gy = FN(dup)(NULL, (size_t)N);
+    c->gz = FN(dup)(NULL, (size_t)N);
+    FN(orc_gdftr2_grad)(fm, nx, ny, nz, 0, c->gx);
+    FN(orc_gdftr2_grad)(fm, nx, ny, nz, 1, c->gy);
+    FN(orc_gdftr2_grad)(fm, nx, ny, nz, 2, c->gz);
     return op;
 }
 
@@ -933,7 +1030,10 @@ void FN(orc_op_forward)(void* opv, const R* in, R* out)
         break;
     case 1: {
         FN(dft_ctx)* c = (FN(dft_ctx)*)op->ctx;
-        FN(orc_ft)(out, in, c->kx, c->ky, c->kz, c->ix, c->iy, c->iz, c->fm, c->t, c->M, c->N);
+        if (c->gx)
+            FN(ft_grads)(c, in, out);
+        else
+            FN(orc_ft)(out, in, c->kx, c->ky, c->kz, c->ix, c->iy, c->iz, c->fm, c->t, c->M, c->N);
         break;
     }
     case 2: {
@@ -1008,7 +1108,10 @@ void FN(orc_op_adjoint)(void* opv, const R* in, R* out)
         break;
     case 1: {
         FN(dft_ctx)* c = (FN(dft_ctx)*)op->ctx;
-        FN(orc_ift)(out, in, c->kx, c->ky, c->kz, c->ix, c->iy, c->iz, c->fm, c->t, c->M, c->N);
+        if (c->gx)
+            FN(ift_grads)(c, in, out);
+        else
+            FN(orc_ift)(out, in, c->kx, c->ky, c->kz, c->ix, c->iy, c->iz, c->fm, c->t, c->M, c->N);
         break;
     }
     case 2: {
@@ -1108,6 +1211,7 @@ void FN(orc_op_destroy)(void* opv)
         FN(dft_ctx)* c = (FN(dft_ctx)*)op->ctx;
         free(c->kx); free(c->ky); free(c->kz); free(c->t);
         free(c->ix); free(c->iy); free(c->iz); free(c->fm);
+        free(c->gx); free(c->gy); free(c->gz);
         break;
     }
     case 2: {

@@ -244,6 +244,18 @@ class Gdft(_Op):
         super().__init__(h)
 
 
+class GdftR2(_Op):
+    """GdftR2<float>(a, b, kx, ky, kz, ix, iy, iz, fm, t, numX, numY, numZ)
+    (GdftR2.cpp:32-57): Gdft with the per-pair field-gradient sinc factors
+    (ftCpuWithGrads.cpp:98-212)."""
+
+    def __init__(self, M, N, kx, ky, kz, ix, iy, iz, fm, t, Nx, Ny, Nz):
+        arrs = [_f32(kx, M), _f32(ky, M), _f32(kz, M), _f32(ix, N), _f32(iy, N), _f32(iz, N),
+                _f32(fm, N), _f32(t, M)]
+        h = _create(lib().pg_gdftr2_create, M, N, *[_p(a) for a in arrs], Nx, Ny, Nz)
+        super().__init__(h)
+
+
 class TimeSegmentation(_Op):
     """TimeSegmentation<float, G>(G, fieldmap, tvec, M, N, L, interptype=1,
     shots=1) (TimeSegmentation.cpp:36-221); Hanning interpolator."""
@@ -458,7 +470,7 @@ class profile:
         return n.value, ms.value
 
 
-__all__ = ["DeviceArray", "Stream", "Gnufft", "Gdft", "TimeSegmentation", "SENSE", "PcSenseTimeSeg", "PcSENSE",
+__all__ = ["DeviceArray", "Stream", "Gnufft", "Gdft", "GdftR2", "TimeSegmentation", "SENSE", "PcSenseTimeSeg", "PcSENSE",
            "QuadPenalty", "TVPenalty", "calc3DDensityCompensation", "grid_coil_images",
            "calcSumOfSquaresImage", "solve_pwls_pcg", "reconSolve", "profile", "PgError", "MODE_BATCHED", "MODE_PLAIN",
            "CG_GRAPH", "CG_HOST"]

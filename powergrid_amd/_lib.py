@@ -41,6 +41,8 @@ _SIGS = {
     "pg_stream_synchronize": (i32, [vp]),
     "pg_gnufft_create": (i32, [C.POINTER(vp), sz, f32, sz, sz, sz, vp, vp, vp, vp, vp, vp, f32]),
     "pg_gdft_create": (i32, [C.POINTER(vp), sz, sz, vp, vp, vp, vp, vp, vp, vp, vp, i32]),
+    "pg_gdftr2_create": (i32, [C.POINTER(vp), sz, sz, vp, vp, vp, vp, vp, vp, vp, vp, sz, sz,
+                               sz]),
     "pg_ts_create": (i32, [C.POINTER(vp), vp, vp, vp, sz, sz, sz, sz, sz]),
     "pg_sense_create": (i32, [C.POINTER(vp), vp, vp, sz, sz, sz]),
     "pg_pcsense_create": (i32, [C.POINTER(vp), vp, vp, vp, sz, sz, sz, sz, vp, sz, sz, vp, vp, vp,

@@ -726,6 +726,9 @@ struct DftOp : Op {
         dev_free(d.iy);
         dev_free(d.iz);
         dev_free(d.fm);
+        dev_free(d.gx);
+        dev_free(d.gy);
+        dev_free(d.gz);
         dev_free(part);
         dev_free(tmp);
     }
@@ -1615,6 +1618,41 @@ int pg_gdft_create(pg_op* out, size_t M, size_t N, const float* kx, const float*
     op->d.iz = dev_copy(iz ? iz : zn.data(), N);
     op->d.fm = dev_copy(fm ? fm : zn.data(), N);
     op->tmp = dev_alloc<float2>(N);
+    *out = reinterpret_cast<pg_op>(op.release());
+    PG_CATCH
+}
+
+int pg_gdftr2_create(pg_op* out, size_t M, size_t N, const float* kx, const float* ky,
+                     const float* kz, const float* ix, const float* iy, const float* iz,
+                     const float* fm, const float* t, size_t Nx, size_t Ny, size_t Nz)
+{
+    PG_TRY PG_REQUIRE(out && fm && Nx > 0 && Ny > 0 && Nz > 0, PG_ERR_ARG, "null argument");
+    PG_REQUIRE(Nx * Ny * Nz == N, PG_ERR_ARG, "Nx*Ny*Nz must equal N");
+    *out = nullptr;
+    pg_op h = nullptr;
+    const int st = pg_gdft_create(&h, M, N, kx, ky, kz, ix, iy, iz, fm, t, 0);
+    if (st != PG_OK) throw pg::Error{st, pg::g_last_error};
+    std::unique_ptr<DftOp> op(reinterpret_cast<DftOp*>(h));
+    // GdftR2.cpp:60-107: Cd(FM, dim) in float, offsets 1, Ny, Nx*Ny of the
+    // y-fastest image.  In 2-D the z offset is the whole map and the
+    // reference indexes outside it; that map is zero here.
+    std::vector<float> g(N);
+    float** dst[3] = {&op->d.gx, &op->d.gy, &op->d.gz};
+    const size_t offs[3] = {1, Ny, Nx * Ny};
+    for (int dim = 0; dim < 3; ++dim) {
+        const size_t o = offs[dim];
+        if (o >= N) {
+            std::fill(g.begin(), g.end(), 0.0f);
+        } else {
+            for (size_t i = o; i < N - o; ++i) g[i] = fm[i + o] - fm[i - o];
+            for (size_t i = 0; i < o; ++i) g[i] = fm[i + o] - fm[i];
+            for (size_t i = N - o; i < N; ++i) g[i] = fm[i] - fm[i - o];
+        }
+        *dst[dim] = dev_copy(g.data(), N);
+    }
+    op->d.numx = (int)Nx;
+    op->d.numy = (int)Ny;
+    op->d.numz = (int)Nz;
     *out = reinterpret_cast<pg_op>(op.release());
     PG_CATCH
 }

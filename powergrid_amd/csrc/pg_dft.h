@@ -9,6 +9,10 @@ struct DftData {
     int M, N, precision;
     float *kx, *ky, *kz, *t;   // M
     float *ix, *iy, *iz, *fm;  // N (fm in rad/s)
+    // GdftR2 (null otherwise): field-map difference maps (N each) and the
+    // image dimensions of the sinc factors
+    float *gx = nullptr, *gy = nullptr, *gz = nullptr;
+    int numx = 1, numy = 1, numz = 1;
 };
 
 // partial-sum buffer size (float2 elements) needed by forward/adjoint

@@ -12,20 +12,47 @@
 //   revolutions); FP32 accumulation.
 // precision 1 -- FP64 phase, FP64 sin/cos, FP64 accumulation (the exact
 //   NUDFT; used to synthesise data and as an accuracy reference).
+// GdftR2 (GdftR2.cpp, ftCpuWithGrads.cpp:98-212) multiplies every pair by
+//   b = sinc(kx/Nx + Gx t) sinc(ky/Ny + Gy t) sinc(kz/Nz + Gz t),
+//   sinc(x) = sin(pi x)/(pi x) (1 below 1e-4), with the field-map difference
+//   maps G staged beside the pixel coordinates.
 #include "pg_dft.h"
 
 namespace {
 
 constexpr int kT = 64;  // threads per block (one wavefront) = tile width
 
-template <int NCB, bool F64>
+// sin(pi x)/(pi x) in FP32 with the hardware sine (which takes revolutions:
+// sin(pi x) = sin(2 pi (x/2)), reduced to [-1/2, 1/2] turns first)
+__device__ __forceinline__ float sinc_f(float x)
+{
+    if (fabsf(x) < 1e-4f) return 1.0f;
+    float r = 0.5f * x;
+    r = r - rintf(r);
+    return __builtin_amdgcn_sinf(r) / (3.14159265358979f * x);
+}
+
+__device__ __forceinline__ double sinc_d(double x)
+{
+    if (fabs(x) < 1e-4) return 1.0;
+    return sin(M_PI * x) / (M_PI * x);
+}
+
+struct R2Args {
+    const float *gx, *gy, *gz;
+    float inx, iny, inz;  // 1/Nx, 1/Ny, 1/Nz
+};
+
+template <int NCB, bool F64, bool R2>
 __global__ void __launch_bounds__(kT) k_gdft_fwd(
     const float* __restrict__ kx, const float* __restrict__ ky, const float* __restrict__ kz,
     const float* __restrict__ t, const float* __restrict__ ix, const float* __restrict__ iy,
     const float* __restrict__ iz, const float* __restrict__ fm, const float2* __restrict__ x,
-    const float2* __restrict__ S, int nc, int M, int N, int slice_len, float2* __restrict__ part)
+    const float2* __restrict__ S, int nc, int M, int N, int slice_len, float2* __restrict__ part,
+    R2Args r2)
 {
     __shared__ float s_r[kT][4];
+    __shared__ float s_g[R2 ? kT : 1][3];
     __shared__ float2 s_v[kT][NCB];
     const int i = blockIdx.x * kT + threadIdx.x;
     const int c0 = blockIdx.z * NCB;
@@ -55,6 +82,11 @@ __global__ void __launch_bounds__(kT) k_gdft_fwd(
             s_r[threadIdx.x][1] = iy[j];
             s_r[threadIdx.x][2] = iz[j];
             s_r[threadIdx.x][3] = F64 ? fm[j] : (float)((double)fm[j] / (2.0 * M_PI));
+            if (R2) {
+                s_g[threadIdx.x][0] = r2.gx[j];
+                s_g[threadIdx.x][1] = r2.gy[j];
+                s_g[threadIdx.x][2] = r2.gz[j];
+            }
             const float2 xv = x[j];
 #pragma unroll
             for (int c = 0; c < NCB; ++c) {
@@ -76,6 +108,13 @@ __global__ void __launch_bounds__(kT) k_gdft_fwd(
                                   (double)s_r[q][3] * ti;
                 double sd, cd;
                 sincos(ph, &sd, &cd);
+                if (R2) {
+                    const double b = sinc_d((double)ki[0] * r2.inx + (double)s_g[q][0] * ti) *
+                                     sinc_d((double)ki[1] * r2.iny + (double)s_g[q][1] * ti) *
+                                     sinc_d((double)ki[2] * r2.inz + (double)s_g[q][2] * ti);
+                    sd *= b;
+                    cd *= b;
+                }
 #pragma unroll
                 for (int c = 0; c < NCB; ++c) {
                     const float2 v = s_v[q][c];
@@ -87,6 +126,13 @@ __global__ void __launch_bounds__(kT) k_gdft_fwd(
                 p = p - rintf(p);
                 sn = __builtin_amdgcn_sinf(p);
                 cs = __builtin_amdgcn_cosf(p);
+                if (R2) {
+                    const float b = sinc_f(ki[0] * r2.inx + s_g[q][0] * ti) *
+                                    sinc_f(ki[1] * r2.iny + s_g[q][1] * ti) *
+                                    sinc_f(ki[2] * r2.inz + s_g[q][2] * ti);
+                    sn *= b;
+                    cs *= b;
+                }
 #pragma unroll
                 for (int c = 0; c < NCB; ++c) {
                     const float2 v = s_v[q][c];
@@ -107,13 +153,13 @@ __global__ void __launch_bounds__(kT) k_gdft_fwd(
     }
 }
 
-template <int NCB, bool F64>
+template <int NCB, bool F64, bool R2>
 __global__ void __launch_bounds__(kT) k_gdft_adj(
     const float* __restrict__ kx, const float* __restrict__ ky, const float* __restrict__ kz,
     const float* __restrict__ t, const float* __restrict__ ix, const float* __restrict__ iy,
     const float* __restrict__ iz, const float* __restrict__ fm, const float2* __restrict__ y,
     long long ystride, const float2* __restrict__ S, int nc, int M, int N, int slice_len,
-    float2* __restrict__ part)
+    float2* __restrict__ part, R2Args r2)
 {
     __shared__ float s_k[kT][4];
     __shared__ float2 s_v[kT][NCB];
@@ -122,11 +168,17 @@ __global__ void __launch_bounds__(kT) k_gdft_adj(
     const int i1 = min(M, i0 + slice_len);
     float rj[3] = {0, 0, 0};
     float fj = 0;
+    float gj[3] = {0, 0, 0};
     if (j < N) {
         rj[0] = ix[j];
         rj[1] = iy[j];
         rj[2] = iz[j];
         fj = F64 ? fm[j] : (float)((double)fm[j] / (2.0 * M_PI));
+        if (R2) {
+            gj[0] = r2.gx[j];
+            gj[1] = r2.gy[j];
+            gj[2] = r2.gz[j];
+        }
     }
     double xr = 0, xi = 0;
     float gr = 0, gi = 0;
@@ -160,6 +212,14 @@ __global__ void __launch_bounds__(kT) k_gdft_adj(
                                       (double)fj * s_k[q][3];
                     double sd, cd;
                     sincos(ph, &sd, &cd);
+                    if (R2) {
+                        const double tq = s_k[q][3];
+                        const double b = sinc_d((double)s_k[q][0] * r2.inx + (double)gj[0] * tq) *
+                                         sinc_d((double)s_k[q][1] * r2.iny + (double)gj[1] * tq) *
+                                         sinc_d((double)s_k[q][2] * r2.inz + (double)gj[2] * tq);
+                        sd *= b;
+                        cd *= b;
+                    }
 #pragma unroll
                     for (int c = 0; c < NCB; ++c) {
                         const float2 v = s_v[q][c];
@@ -169,8 +229,16 @@ __global__ void __launch_bounds__(kT) k_gdft_adj(
                 } else {
                     float p = s_k[q][0] * rj[0] + s_k[q][1] * rj[1] + s_k[q][2] * rj[2] + fj * s_k[q][3];
                     p = p - rintf(p);
-                    const float sn = __builtin_amdgcn_sinf(p);
-                    const float cs = __builtin_amdgcn_cosf(p);
+                    float sn = __builtin_amdgcn_sinf(p);
+                    float cs = __builtin_amdgcn_cosf(p);
+                    if (R2) {
+                        const float tq = s_k[q][3];
+                        const float b = sinc_f(s_k[q][0] * r2.inx + gj[0] * tq) *
+                                        sinc_f(s_k[q][1] * r2.iny + gj[1] * tq) *
+                                        sinc_f(s_k[q][2] * r2.inz + gj[2] * tq);
+                        sn *= b;
+                        cs *= b;
+                    }
 #pragma unroll
                     for (int c = 0; c < NCB; ++c) {
                         const float2 v = s_v[q][c];
@@ -227,6 +295,12 @@ __global__ void k_sum_slices(const float2* part, int nslice, long long n, long l
     out[o] = s;
 }
 
+R2Args r2_args(const pgd::DftData& d)
+{
+    return R2Args{d.gx, d.gy, d.gz, 1.0f / (float)d.numx, 1.0f / (float)d.numy,
+                  1.0f / (float)d.numz};
+}
+
 int slices_for(long long outputs, long long inputs)
 {
     // aim for >= 4096 wavefronts in flight
@@ -254,12 +328,18 @@ void forward(const DftData& d, const float2* x, const float2* S, int nc, float2*
     const int slen = (d.N + ns - 1) / ns;
     constexpr int NCB = 8;
     dim3 grid((d.M + kT - 1) / kT, ns, (nc + NCB - 1) / NCB);
-    if (d.precision == 1)
-        hipLaunchKernelGGL((k_gdft_fwd<NCB, true>), grid, dim3(kT), 0, s, d.kx, d.ky, d.kz, d.t, d.ix,
-                           d.iy, d.iz, d.fm, x, S, nc, d.M, d.N, slen, part);
-    else
-        hipLaunchKernelGGL((k_gdft_fwd<NCB, false>), grid, dim3(kT), 0, s, d.kx, d.ky, d.kz, d.t, d.ix,
-                           d.iy, d.iz, d.fm, x, S, nc, d.M, d.N, slen, part);
+    const R2Args r2 = r2_args(d);
+#define PG_GDFT_FWD(F64, R2)                                                                     \
+    hipLaunchKernelGGL((k_gdft_fwd<NCB, F64, R2>), grid, dim3(kT), 0, s, d.kx, d.ky, d.kz, d.t,  \
+                       d.ix, d.iy, d.iz, d.fm, x, S, nc, d.M, d.N, slen, part, r2)
+    if (d.precision == 1) {
+        if (d.gx) PG_GDFT_FWD(true, true);
+        else PG_GDFT_FWD(true, false);
+    } else {
+        if (d.gx) PG_GDFT_FWD(false, true);
+        else PG_GDFT_FWD(false, false);
+    }
+#undef PG_GDFT_FWD
     const long long n = (long long)nc * d.M;
     hipLaunchKernelGGL(k_sum_slices, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, part, ns, n,
                        (long long)d.M, ostride, 0, out);
@@ -273,12 +353,18 @@ void adjoint(const DftData& d, const float2* y, const float2* S, int nc, float2*
     const int slen = (d.M + ns - 1) / ns;
     constexpr int NCB = 8;
     dim3 grid((d.N + kT - 1) / kT, ns, 1);
-    if (d.precision == 1)
-        hipLaunchKernelGGL((k_gdft_adj<NCB, true>), grid, dim3(kT), 0, s, d.kx, d.ky, d.kz, d.t, d.ix,
-                           d.iy, d.iz, d.fm, y, ystride, S, nc, d.M, d.N, slen, part);
-    else
-        hipLaunchKernelGGL((k_gdft_adj<NCB, false>), grid, dim3(kT), 0, s, d.kx, d.ky, d.kz, d.t, d.ix,
-                           d.iy, d.iz, d.fm, y, ystride, S, nc, d.M, d.N, slen, part);
+    const R2Args r2 = r2_args(d);
+#define PG_GDFT_ADJ(F64, R2)                                                                     \
+    hipLaunchKernelGGL((k_gdft_adj<NCB, F64, R2>), grid, dim3(kT), 0, s, d.kx, d.ky, d.kz, d.t,  \
+                       d.ix, d.iy, d.iz, d.fm, y, ystride, S, nc, d.M, d.N, slen, part, r2)
+    if (d.precision == 1) {
+        if (d.gx) PG_GDFT_ADJ(true, true);
+        else PG_GDFT_ADJ(true, false);
+    } else {
+        if (d.gx) PG_GDFT_ADJ(false, true);
+        else PG_GDFT_ADJ(false, false);
+    }
+#undef PG_GDFT_ADJ
     hipLaunchKernelGGL(k_sum_slices, dim3((unsigned)((d.N + 255) / 256)), dim3(256), 0, s, part, ns,
                        (long long)d.N, (long long)d.N, (long long)d.N, accumulate ? 1 : 0, out);
 }

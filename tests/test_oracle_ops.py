@@ -380,3 +380,47 @@ def test_pcsense_dft_zero_phase_one_shot_is_sense_gdft(orc):
     np.testing.assert_array_equal(A * x, B * x)
     y = B * x
     np.testing.assert_array_equal(A / y, B / y)
+
+
+def test_gdftr2_gradient_maps(orc):
+    # GdftR2.cpp:67-107: offset 1 (dim 0), Ny (dim 1), Nx*Ny (dim 2) in the
+    # y-fastest image; centred in the interior, one-sided at the two ends
+    nx, ny, nz = 4, 5, 3
+    rng = np.random.default_rng(1)
+    fm = rng.standard_normal(nx * ny * nz)
+    n = fm.size
+    for dim, o in ((0, 1), (1, ny), (2, nx * ny)):
+        g = orc.gdftr2_grad(fm, nx, ny, nz, dim, prec="d")
+        ref = np.empty(n)
+        ref[o:n - o] = fm[2 * o:] - fm[:n - 2 * o]
+        ref[:o] = fm[o:2 * o] - fm[:o]
+        ref[n - o:] = fm[n - o:] - fm[n - 2 * o:n - o]
+        np.testing.assert_array_equal(g, ref)
+    # 2-D: the z map is zero (the reference reads outside the map there)
+    assert not np.any(orc.gdftr2_grad(fm[:nx * ny], nx, ny, 1, 2, prec="d"))
+
+
+def test_gdftr2_matches_dense_sinc_weighted_dft(orc):
+    # ftCpuWithGrads.cpp:98-212: forward e^{-i(2 pi k.r + fm t)} b, adjoint
+    # e^{+i(...)} b, b = prod_d sinc(k_d/N_d + G_d t), sinc(x) = sin(pi x)/(pi x)
+    N, Nz = 6, 4
+    ix, iy, iz = S.image_coords(N, N, Nz)
+    rng = np.random.default_rng(2)
+    M = 40
+    kx, ky = rng.uniform(-N / 2, N / 2, (2, M))
+    kz = rng.uniform(-Nz / 2, Nz / 2, M)
+    t = rng.uniform(0, 5e-3, M)
+    fm = rng.uniform(-300, 300, N * N * Nz)
+    A = orc.GdftR2(M, N * N * Nz, kx, ky, kz, ix, iy, iz, fm, t, N, N, Nz, prec="d")
+    g = [orc.gdftr2_grad(fm, N, N, Nz, d, prec="d") for d in range(3)]
+    ph = 2 * np.pi * (np.outer(kx, ix) + np.outer(ky, iy) + np.outer(kz, iz)) + np.outer(t, fm)
+    B = (np.sinc(kx[:, None] / N + g[0][None, :] * t[:, None]) *
+         np.sinc(ky[:, None] / N + g[1][None, :] * t[:, None]) *
+         np.sinc(kz[:, None] / Nz + g[2][None, :] * t[:, None]))
+    E = np.exp(-1j * ph) * B
+    x = rng.standard_normal(N * N * Nz) + 1j * rng.standard_normal(N * N * Nz)
+    y = rng.standard_normal(M) + 1j * rng.standard_normal(M)
+    # FP32 sin/cos of the phase (ftCpuWithGrads.cpp:107-108): ~1e-7
+    assert rel(A * x, E @ x) < 1e-6
+    assert rel(A / y, E.conj().T @ y) < 1e-6
+    assert np.min(np.abs(B)) < 0.99  # the factor is active
