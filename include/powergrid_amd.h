@@ -231,6 +231,13 @@ int pg_solve_pwls_pcg(pg_op A, pg_penalty R, const void* x0_dev, const float* W_
 int pg_recon_solve(pg_op A, pg_penalty R, const void* y_dev, size_t niter, int mode,
                    void* x_out_dev, double* resid, pg_cg_info* info, void* stream);
 
+/* ---- tuning (no counterpart in the reference) ----------------------------
+ * "spread_mfma" 1: the 2-D batched spread runs on the matrix cores when the
+ * grid count is a multiple of 4; 0 (default): on the vector units.  Both
+ * give the same sums up to FP32 rounding order.  Takes effect for launches
+ * (and solve graphs) made after the call. */
+int pg_set_tuning(const char* key, int value);
+
 /* ---- measurement --------------------------------------------------------
  * When enabled, every kernel launch of the library is bracketed by HIP
  * events on its stream and its duration accumulated per kernel name.  Not

@@ -330,6 +330,13 @@ class SENSE(_Op):
         self.nc = nc
 
 
+def set_tuning(key, value):
+    """pg_set_tuning: "spread_mfma" 1 runs the 2-D batched spread on the
+    matrix cores (grid count a multiple of 4), 0 (default) on the vector
+    units."""
+    check(lib().pg_set_tuning(key.encode(), int(value)))
+
+
 def calc3DDensityCompensation(kx, ky, kz, Ninplane, Nz, stream=None):
     """calc3DDensityCompensation(kx, ky, kz, Ninplane, Nz)
     (directRecon.cpp:28-60): Pipe-Menon density compensation, two
@@ -470,7 +477,8 @@ class profile:
         return n.value, ms.value
 
 
-__all__ = ["DeviceArray", "Stream", "Gnufft", "Gdft", "GdftR2", "TimeSegmentation", "SENSE", "PcSenseTimeSeg", "PcSENSE",
-           "QuadPenalty", "TVPenalty", "calc3DDensityCompensation", "grid_coil_images",
-           "calcSumOfSquaresImage", "solve_pwls_pcg", "reconSolve", "profile", "PgError", "MODE_BATCHED", "MODE_PLAIN",
-           "CG_GRAPH", "CG_HOST"]
+__all__ = ["DeviceArray", "Stream", "Gnufft", "Gdft", "GdftR2", "TimeSegmentation", "SENSE",
+           "PcSenseTimeSeg", "PcSENSE", "QuadPenalty", "TVPenalty", "calc3DDensityCompensation",
+           "grid_coil_images", "calcSumOfSquaresImage", "set_tuning", "solve_pwls_pcg",
+           "reconSolve", "profile", "PgError", "MODE_BATCHED", "MODE_PLAIN", "CG_GRAPH",
+           "CG_HOST"]

@@ -69,6 +69,7 @@ _SIGS = {
     "pg_penalty_value": (i32, [vp, vp, dp, vp]),
     "pg_solve_pwls_pcg": (i32, [vp, vp, vp, vp, vp, sz, i32, vp, dp, C.POINTER(CgInfo), vp]),
     "pg_recon_solve": (i32, [vp, vp, vp, sz, i32, vp, dp, C.POINTER(CgInfo), vp]),
+    "pg_set_tuning": (i32, [C.c_char_p, i32]),
     "pg_profile_enable": (i32, [i32]),
     "pg_profile_reset": (i32, []),
     "pg_profile_query": (i32, [C.c_char_p, C.POINTER(i32), dp]),

@@ -2114,6 +2114,17 @@ int pg_recon_solve(pg_op A, pg_penalty R, const void* y, size_t niter, int mode,
     return pg_solve_pwls_pcg(A, R, nullptr, nullptr, y, niter, mode, x_out, resid, info, stream);
 }
 
+int pg_set_tuning(const char* key, int value)
+{
+    PG_TRY PG_REQUIRE(key, PG_ERR_ARG, "null key");
+    const std::string k(key);
+    if (k == "spread_mfma")
+        pgk::g_spread_mfma.store(value ? 1 : 0);
+    else
+        throw pg::Error{PG_ERR_ARG, "unknown tuning key " + k};
+    PG_CATCH
+}
+
 int pg_profile_enable(int on)
 {
     g_prof = on != 0;

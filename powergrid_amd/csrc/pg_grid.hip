@@ -704,7 +704,7 @@ void spread_dispatch(const pg::Gnufft& G, const float2* in, long long in_cstride
     if (!D3) {
         const int nch = (B + 63) / 64;
         const int npy = g.Gy / pg::kPatch;
-        if (B % 4 == 0)
+        if (B % 4 == 0 && pgk::g_spread_mfma.load(std::memory_order_relaxed))
             hipLaunchKernelGGL(k_spread_patch_mfma, dim3(G.n_sblocks, nch), dim3(64 * kSpreadWaves),
                                0, s, G.witems, G.bgs, G.entries, (const float4*)G.ent_w, npy, G.vals,
                                B, grids, g.G(), g.Gy, G.partial, G.n_slots);
@@ -727,6 +727,8 @@ void spread_dispatch(const pg::Gnufft& G, const float2* in, long long in_cstride
 }  // namespace
 
 namespace pgk {
+
+std::atomic<int> g_spread_mfma{0};
 
 void launch_interp(const pg::Gnufft& G, const float2* grids, int nc, int L, const float* AA,
                    float2* out, long long out_cstride, hipStream_t s)

@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <atomic>
 #include <string>
 #include <vector>
 
@@ -99,6 +100,10 @@ struct Gnufft;
 // --------------------------------------------------- kernel launchers ----
 // (defined in the .hip files; all asynchronous on `stream`)
 namespace pgk {
+
+// 2-D batched spread on the matrix cores (k_spread_patch_mfma) when set and
+// the grid count is a multiple of 4; the vector-unit kernel otherwise
+extern std::atomic<int> g_spread_mfma;
 
 struct WeightArgs {
     // image-side weights applied before the forward NUFFT and after the
