@@ -96,9 +96,12 @@ int pg_gdftr2_create(pg_op* out, size_t M, size_t N, const float* kx, const floa
                      const float* fm, const float* t, size_t Nx, size_t Ny, size_t Nz);
 
 /* replaces TimeSegmentation<float,G>::TimeSegmentation
- * (TimeSegmentation.cpp:36-221, TimeSegmentation.h:70-71).  Only the
- * Hanning interpolator (interptype 1) is implemented; G is not owned and
- * must outlive the new handle (TimeSegmentation.h:55). */
+ * (TimeSegmentation.cpp:36-221, TimeSegmentation.h:70-71).  interptype 1:
+ * Hanning (:80-93); 2: the exact least-squares min-max interpolator
+ * (:94-150), its complex weights formed in FP64 (a Jacobi eigen-solve of the
+ * L x L Gram matrix, then one device pass over samples x voxels), applied
+ * one segment at a time.  interptype 3 returns PG_ERR_UNSUPPORTED.  G is
+ * not owned and must outlive the new handle (TimeSegmentation.h:55). */
 int pg_ts_create(pg_op* out, pg_op G, const float* fieldmap, const float* tvec, size_t M,
                  size_t N, size_t L, size_t interptype, size_t shots);
 

@@ -80,6 +80,7 @@ def _declare(L):
         f("orc_op_n_out", C.c_int, _vp)
         f("orc_op_destroy", None, _vp)
         f("orc_ts_tables", C.c_int, _vp, _vp, _vp, _vp)
+        f("orc_ts_set_aa_complex", C.c_int, _vp, _vp)
         f("orc_penalty_create", _vp, C.c_int, C.c_int, C.c_int, C.c_int, R, R, C.c_int)
         f("orc_penalty_destroy", None, _vp)
         f("orc_penalty_cd", None, _vp, _vp, C.c_int, _vp)
@@ -287,8 +288,55 @@ def gdftr2_grad(fm, nx, ny, nz, dim, prec="f"):
     return out
 
 
+def arma_pinv_rcond(rows, cols, prec):
+    """pinv's default tolerance in Armadillo 9.200 (not vendored):
+    max(rows, cols) * max singular value * eps of the element type, i.e. a
+    relative cutoff of max(rows, cols) * eps."""
+    eps = np.finfo(np.float32 if prec == "f" else np.float64).eps
+    return max(rows, cols) * eps
+
+
+def minmax_ls_weights(fm, tv, tau, L, shots, prec="f"):
+    """TimeSegmentation.cpp:94-150, the exact least-squares min-max
+    interpolator, in the element precision of `prec`:
+      g = exp(i fm tau); G = [1, g, ..., g^(L-1)] (N x L);
+      GTG = N on the diagonal, glsum(k) = sum_j conj(g_j^(k+1)) on the k+1-th
+      sub-diagonal and its conjugate above (G^H G, since |g| = 1);
+      iGTGGT = inv(GTG) G^H, or pinv(GTG) G^H when 1/cond(GTG) <= 2e-15;
+      AA[i, :] = conj(iGTGGT exp(i fm tv_i)) for the first shot's samples,
+      repeated over the shots.  tv: sample times minus their minimum."""
+    rt, ct = _rt(prec), _ct(prec)
+    fm = np.asarray(fm, dtype=rt).ravel()
+    tv = np.asarray(tv, dtype=rt).ravel()
+    n2 = fm.size
+    gg = np.exp(ct(1j) * (fm * rt(tau)).astype(rt)).astype(ct)
+    gl = np.stack([gg ** (k + 1) for k in range(L)], axis=1).astype(ct)  # n2 x L
+    G = np.concatenate([np.ones((n2, 1), ct), gl[:, :L - 1]], axis=1)
+    glsum = np.sum(gl.conj().T, axis=1).astype(ct)
+    GTG = np.zeros((L, L), ct)
+    GTG[np.diag_indices(L)] += rt(n2)
+    for k in range(L - 1):
+        d = k + 1
+        for r in range(d, L):
+            GTG[r, r - d] += glsum[k]
+            GTG[r - d, r] += np.conj(glsum[k])
+    rcn = 1.0 / np.linalg.cond(GTG)
+    if rcn > 10 * 2e-16:
+        inv = np.linalg.inv(GTG)
+    else:
+        inv = np.linalg.pinv(GTG, rcond=arma_pinv_rcond(L, L, prec))
+    iGTGGT = (inv.astype(ct) @ G.conj().T).astype(ct)  # L x n2
+    none = tv.size // shots
+    F = np.exp(ct(1j) * np.outer(fm, tv[:none]).astype(rt)).astype(ct)  # n2 x none
+    R = (iGTGGT @ F).astype(ct)  # L x none
+    AA = np.tile(R.conj().T, (shots, 1))  # M x L
+    return AA, float(rcn)
+
+
 class TimeSegmentation(_Op):
-    """TimeSegmentation.cpp:36-221 (Hanning interpolator)."""
+    """TimeSegmentation.cpp:36-221: interptype 1 Hanning (in C), 2 the exact
+    least-squares min-max interpolator (complex weights computed here with
+    numpy in the element precision and handed to the C operator)."""
 
     def __init__(self, G, fm, t, M, N, L, interptype=1, shots=1):
         prec = G.suf
@@ -298,6 +346,15 @@ class TimeSegmentation(_Op):
         self.G = G
         # the composite does not own the inner op (TimeSegmentation.h:55);
         # the C side frees only its own context.
+        self.rcn = None
+        Lc = max(1, int(L)) if int(L) > 1 else 1
+        if interptype == 2 and Lc > 1:
+            _, _, tau, tmin = self.tables()
+            rt = _rt(prec)
+            tv = (np.asarray(t, dtype=rt) - rt(tmin)).astype(rt)
+            AA, self.rcn = minmax_ls_weights(fm, tv, tau, Lc, shots, prec)
+            AAc = np.ascontiguousarray(AA.T).astype(_ct(prec))  # L x M = column-major M x L
+            _fn("orc_ts_set_aa_complex", prec)(self.h, _p(AAc))
 
     def tables(self):
         L = _fn("orc_ts_tables", self.suf)(self.h, None, None, None)

@@ -108,6 +108,7 @@ int orc_get_threads(void);
   void orc_op_destroy_##S(void* op);                                          \
   /* TS interpolator / phase tables (TimeSegmentation.cpp:64-93,213-218) */   \
   int orc_ts_tables_##S(void* ts, R* AA, R* Wo, R* tau_tmin);                 \
+  int orc_ts_set_aa_complex_##S(void* ts, const R* AAc);                      \
   /* Robject.cpp:49-193, QuadPenalty.cpp:45-64, TVPenalty.cpp:43-68 */        \
   void* orc_penalty_create_##S(int kind, int Nx, int Ny, int Nz, R beta,      \
                                R delta, int dims2penalize);                   \

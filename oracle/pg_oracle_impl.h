@@ -563,6 +563,7 @@ typedef struct {
     int M, N, L;
     R tau, tmin;
     R* AA;  /* M x L real weights (column-major) */
+    R* AAc; /* M x L complex weights (type 2), replacing AA when set */
     R* Wo;  /* N x L complex */
     R* WoH; /* N x L complex */
     R *tmp_in, *tmp_out, *acc;
@@ -854,6 +855,18 @@ void* FN(orc_ts_create)(void* Gop, const R* fm, const R* tvec, int M, int N, int
     return op;
 }
 
+/* complex interpolator weights (M x L, column-major) replacing the real
+ * ones: the min-max interpolators (TimeSegmentation.cpp:94-211) */
+int FN(orc_ts_set_aa_complex)(void* tsop, const R* AAc)
+{
+    FN(orc_op)* op = (FN(orc_op)*)tsop;
+    if (op->kind != 2) return -1;
+    FN(ts_ctx)* c = (FN(ts_ctx)*)op->ctx;
+    free(c->AAc);
+    c->AAc = FN(dup)(AAc, 2 * (size_t)c->M * c->L);
+    return 0;
+}
+
 int FN(orc_ts_tables)(void* tsop, R* AA, R* Wo, R* tau_tmin)
 {
     FN(orc_op)* op = (FN(orc_op)*)tsop;
@@ -1046,6 +1059,17 @@ void FN(orc_op_forward)(void* opv, const R* in, R* out)
                 FN(cmul)(wo[2 * n], wo[2 * n + 1], in[2 * n], in[2 * n + 1], &c->tmp_in[2 * n],
                          &c->tmp_in[2 * n + 1]);
             FN(orc_op_forward)(c->G, c->tmp_in, c->tmp_out);
+            if (c->AAc) {
+                const R* ac = c->AAc + 2 * (size_t)l * c->M;
+                for (int j = 0; j < c->M; ++j) {
+                    R cr, ci;
+                    FN(cmul)(c->tmp_out[2 * j], c->tmp_out[2 * j + 1], ac[2 * j], ac[2 * j + 1],
+                             &cr, &ci);
+                    out[2 * j] += cr;
+                    out[2 * j + 1] += ci;
+                }
+                continue;
+            }
             const R* aa = c->AA + (size_t)l * c->M;
             for (int j = 0; j < c->M; ++j) {
                 out[2 * j] += aa[j] * c->tmp_out[2 * j];
@@ -1120,10 +1144,17 @@ void FN(orc_op_adjoint)(void* opv, const R* in, R* out)
         memset(out, 0, sizeof(R) * 2 * c->N);
         for (int l = 0; l < c->L; ++l) {
             const R* aa = c->AA + (size_t)l * c->M;
+            const R* ac = c->AAc ? c->AAc + 2 * (size_t)l * c->M : NULL;
             for (int j = 0; j < c->M; ++j) {
-                /* conj(AA) * d; AA is real-valued */
-                c->tmp_in[2 * j] = aa[j] * in[2 * j];
-                c->tmp_in[2 * j + 1] = aa[j] * in[2 * j + 1];
+                if (ac) {
+                    /* conj(AA) * d */
+                    FN(cmul)(ac[2 * j], -ac[2 * j + 1], in[2 * j], in[2 * j + 1],
+                             &c->tmp_in[2 * j], &c->tmp_in[2 * j + 1]);
+                } else {
+                    /* conj(AA) * d; AA is real-valued */
+                    c->tmp_in[2 * j] = aa[j] * in[2 * j];
+                    c->tmp_in[2 * j + 1] = aa[j] * in[2 * j + 1];
+                }
             }
             FN(orc_op_adjoint)(c->G, c->tmp_in, c->tmp_out);
             const R* woh = c->WoH + 2 * (size_t)l * c->N;
@@ -1216,7 +1247,7 @@ void FN(orc_op_destroy)(void* opv)
     }
     case 2: {
         FN(ts_ctx)* c = (FN(ts_ctx)*)op->ctx;
-        free(c->AA); free(c->Wo); free(c->WoH);
+        free(c->AA); free(c->AAc); free(c->Wo); free(c->WoH);
         free(c->tmp_in); free(c->tmp_out); free(c->acc);
         break;
     }

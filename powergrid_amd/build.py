@@ -20,7 +20,7 @@ LIB = os.path.join(HERE, "libpowergrid_amd.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("PG_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["pg_fft.hip", "pg_grid.hip", "pg_vec.hip", "pg_dft.hip", "pg_recon.hip", "pg_api.cpp"]
+SOURCES = ["pg_fft.hip", "pg_grid.hip", "pg_vec.hip", "pg_dft.hip", "pg_recon.hip", "pg_tsint.hip", "pg_api.cpp"]
 
 FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17",

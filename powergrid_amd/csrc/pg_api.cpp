@@ -18,7 +18,11 @@
 #include "pg_gnufft.h"
 #include "pg_internal.h"
 #include "pg_recon.h"
+#include "pg_herm.h"
+#include "pg_tsint.h"
 #include "pg_vec.h"
+
+#include <complex>
 
 namespace pg {
 
@@ -738,7 +742,8 @@ struct TsOp : Op {
     Op* inner = nullptr;
     int L = 1;
     float tau = 0, tmin = 0;
-    float* AA = nullptr;    // L x M
+    float* AA = nullptr;    // L x M (Hanning)
+    float2* AAc = nullptr;  // L x M complex (min-max interpolator), replaces AA
     float2* Wo = nullptr;   // L x N
     float2* WoH = nullptr;  // L x N
     float2* tmp_n = nullptr;
@@ -748,6 +753,7 @@ struct TsOp : Op {
     ~TsOp() override
     {
         dev_free(AA);
+        dev_free(AAc);
         dev_free(Wo);
         dev_free(WoH);
         dev_free(tmp_n);
@@ -847,6 +853,7 @@ static bool fusable(Op* op, Fused& f)
     }
     if (in->kind == K_TS) {
         TsOp* t = static_cast<TsOp*>(in);
+        if (t->AAc) return false;  // complex interpolator weights: per segment
         f.L = t->L;
         f.Wo = t->Wo;
         f.WoH = t->WoH;
@@ -989,7 +996,10 @@ static void op_forward(Op* op, const float2* in, float2* out, hipStream_t s)
         for (int l = 0; l < t->L; ++l) {
             pgv::cmul(t->tmp_n, t->Wo + (size_t)l * N, in, N, s);
             op_forward(t->inner, t->tmp_n, t->tmp_m, s);
-            pgv::rmul_acc(out, t->AA + (size_t)l * M, t->tmp_m, M, l == 0, s);
+            if (t->AAc)
+                pgv::cmul_acc(out, t->tmp_m, t->AAc + (size_t)l * M, M, l == 0, false, s);
+            else
+                pgv::rmul_acc(out, t->AA + (size_t)l * M, t->tmp_m, M, l == 0, s);
         }
         return;
     }
@@ -1017,7 +1027,7 @@ static void op_forward(Op* op, const float2* in, float2* out, hipStream_t s)
         for (int sh = 0; sh < pc->Ns; ++sh) {
             TsOp* ts = static_cast<TsOp*>(pc->A[sh]);
             const float2* SSs = pc->SS + (size_t)sh * pc->Nc * Ni;
-            if (op->mode == PG_MODE_BATCHED) {
+            if (op->mode == PG_MODE_BATCHED && !ts->AAc) {
                 // every coil x segment of the shot in one fused launch; the
                 // shot's coil blocks are Ns*Nd apart
                 Fused f;
@@ -1096,7 +1106,10 @@ static void op_adjoint(Op* op, const float2* in, float2* out, hipStream_t s)
         TsOp* t = static_cast<TsOp*>(op);
         const long long N = (long long)t->n_in, M = (long long)t->n_out;
         for (int l = 0; l < t->L; ++l) {
-            pgv::rmul_acc(t->tmp_m, t->AA + (size_t)l * M, in, M, true, s);
+            if (t->AAc)  // conj(AA) % d
+                pgv::cmul_acc(t->tmp_m, in, t->AAc + (size_t)l * M, M, true, true, s);
+            else
+                pgv::rmul_acc(t->tmp_m, t->AA + (size_t)l * M, in, M, true, s);
             op_adjoint(t->inner, t->tmp_m, t->tmp_n, s);
             pgv::cmul_acc(out, t->tmp_n, t->WoH + (size_t)l * N, N, l == 0, false, s);
         }
@@ -1123,7 +1136,7 @@ static void op_adjoint(Op* op, const float2* in, float2* out, hipStream_t s)
         // pcSenseTimeSeg.cpp:152-171
         PcSenseOp* pc = static_cast<PcSenseOp*>(op);
         const long long Ni = (long long)pc->Ni, Nd = (long long)pc->Nd;
-        if (op->mode == PG_MODE_BATCHED) {
+        if (op->mode == PG_MODE_BATCHED && !static_cast<TsOp*>(pc->A[0])->AAc) {
             // per shot, every coil x segment in one fused launch, into that
             // shot's block of Nc partial images; then one sum over the Ns*Nc
             for (int sh = 0; sh < pc->Ns; ++sh) {
@@ -1411,12 +1424,95 @@ static void solve_pcg(Op* A, Penalty* R, const float2* x0, const float* W, const
 }
 
 // TimeSegmentation.cpp:36-221 over the operator G (not owned)
+using cd = std::complex<double>;
+
+// TimeSegmentation.cpp:94-150 (interptype 2), in FP64: the exact
+// least-squares min-max interpolator.  g_j = exp(i fm_j tau), G = [1, g,
+// ..., g^(L-1)] (N x L), GTG = G^H G built as the reference does (N on the
+// diagonal, sum_j conj(g_j^k) on the k-th sub-diagonal, conjugates above);
+// W = inv(GTG) G^H, or pinv(GTG) G^H when 1/cond(GTG) <= 2e-15 (pinv
+// cutoff: L * max eigenvalue * double eps); AA[l][i] = conj(W exp(i fm t_i))
+// on the device.  Returns the L x M complex weights.
+static float2* minmax_ls_weights(const float* fieldmap, const float* tvec, size_t M, size_t N,
+                                 int L, size_t shots, double tau, double tmin)
+{
+    std::vector<double> fm(N);
+    for (size_t j = 0; j < N; ++j) fm[j] = fieldmap[j];
+    std::vector<cd> glsum(L, cd(0, 0));
+    for (size_t j = 0; j < N; ++j)
+        for (int k = 0; k < L - 1; ++k) glsum[k] += std::exp(cd(0, -(double)(k + 1) * fm[j] * tau));
+    std::vector<cd> GTG((size_t)L * L, cd(0, 0));
+    for (int r = 0; r < L; ++r) GTG[(size_t)r * L + r] = (double)N;
+    for (int k = 0; k < L - 1; ++k)
+        for (int r = k + 1; r < L; ++r) {
+            GTG[(size_t)r * L + (r - k - 1)] += glsum[k];
+            GTG[(size_t)(r - k - 1) * L + r] += std::conj(glsum[k]);
+        }
+    std::vector<double> ev;
+    std::vector<cd> V;
+    herm_eig(L, GTG, ev, V);
+    double smax = 0, smin = 1e300;
+    for (double e : ev) {
+        smax = std::max(smax, std::fabs(e));
+        smin = std::min(smin, std::fabs(e));
+    }
+    const double rcn = smax > 0 ? smin / smax : 0.0;
+    const double cut = rcn > 10 * 2e-16 ? 0.0 : (double)L * smax * 2.220446049250313e-16;
+    std::vector<cd> inv((size_t)L * L, cd(0, 0));
+    for (int a = 0; a < L; ++a)
+        for (int b = 0; b < L; ++b) {
+            cd acc(0, 0);
+            for (int k = 0; k < L; ++k)
+                if (std::fabs(ev[k]) > cut && ev[k] != 0.0)
+                    acc += V[(size_t)a * L + k] * std::conj(V[(size_t)b * L + k]) / ev[k];
+            inv[(size_t)a * L + b] = acc;
+        }
+    // W[l][j] = sum_c inv[l][c] conj(g_j^c)
+    std::vector<double2> W((size_t)L * N);
+    std::vector<cd> gc(L);
+    for (size_t j = 0; j < N; ++j) {
+        for (int c = 0; c < L; ++c) gc[c] = std::exp(cd(0, -(double)c * fm[j] * tau));
+        for (int l = 0; l < L; ++l) {
+            cd acc(0, 0);
+            for (int c = 0; c < L; ++c) acc += inv[(size_t)l * L + c] * gc[c];
+            W[(size_t)l * N + j] = make_double2(acc.real(), acc.imag());
+        }
+    }
+    const size_t none = M / shots;
+    std::vector<double> tv(none);
+    for (size_t i = 0; i < none; ++i) tv[i] = (double)tvec[i] - tmin;
+    struct Tmp {
+        double* fm = nullptr;
+        double* tv = nullptr;
+        double2* W = nullptr;
+        ~Tmp()
+        {
+            dev_free(fm);
+            dev_free(tv);
+            dev_free(W);
+        }
+    } d;
+    d.fm = dev_copy(fm.data(), N);
+    d.tv = dev_copy(tv.data(), none);
+    d.W = dev_copy(W.data(), W.size());
+    float2* AA = dev_alloc<float2>((size_t)L * M);
+    pgt::minmax_weights(d.fm, d.tv, d.W, (int)N, (int)none, L, (int)shots, AA, nullptr);
+    const hipError_t e1 = hipGetLastError();
+    const hipError_t e2 = hipStreamSynchronize(nullptr);
+    if (e1 != hipSuccess || e2 != hipSuccess) {
+        dev_free(AA);
+        PG_HIP(e1 != hipSuccess ? e1 : e2);
+    }
+    return AA;
+}
+
 static TsOp* ts_build(Op* G, const float* fieldmap, const float* tvec, size_t M, size_t N, size_t L,
                       size_t interptype, size_t shots)
 {
     PG_REQUIRE(G->n_in == N && G->n_out == M, PG_ERR_ARG, "TimeSegmentation sizes do not match G");
-    PG_REQUIRE(interptype == 1, PG_ERR_UNSUPPORTED,
-               "only the Hanning interpolator (interptype 1) is implemented");
+    PG_REQUIRE(interptype == 1 || interptype == 2, PG_ERR_UNSUPPORTED,
+               "interptype 1 (Hanning) and 2 (exact least-squares min-max) are implemented");
+    PG_REQUIRE(N < (1u << 31) && M < (1u << 31), PG_ERR_ARG, "sizes too large");
     if (shots < 1) shots = 1;
     PG_REQUIRE(M % shots == 0, PG_ERR_ARG, "M must be a multiple of shots");
     std::unique_ptr<TsOp> op(new TsOp());
@@ -1463,6 +1559,8 @@ static TsOp* ts_build(Op* G, const float* fieldmap, const float* tvec, size_t M,
     }
     op->tau = (float)tau;
     op->tmin = (float)tmin;
+    if (interptype == 2 && Lc > 1)
+        op->AAc = minmax_ls_weights(fieldmap, tvec, M, N, Lc, shots, tau, tmin);
     op->AA = dev_copy(AA.data(), AA.size());
     op->Wo = dev_copy(Wo.data(), Wo.size());
     op->WoH = dev_copy(WoH.data(), WoH.size());

@@ -424,3 +424,26 @@ def test_gdftr2_matches_dense_sinc_weighted_dft(orc):
     assert rel(A * x, E @ x) < 1e-6
     assert rel(A / y, E.conj().T @ y) < 1e-6
     assert np.min(np.abs(B)) < 0.99  # the factor is active
+
+
+def test_minmax_ls_weights_are_least_squares_fits(orc):
+    # TimeSegmentation.cpp:94-150: conj(AA[i, :]) is the least-squares
+    # coefficient vector a of sum_l a_l exp(i fm l tau) ~ exp(i fm t_i) over
+    # the voxels; checked against numpy's SVD least squares, and exact
+    # (a unit vector) at the segment knots t = l tau
+    N, L = 16, 4
+    fm = S.field_map(N, peak_hz=100.0).astype(np.float64)
+    tau = 2.5e-3
+    knots = np.arange(L) * tau
+    rng = np.random.default_rng(4)
+    tv = np.concatenate([knots, rng.uniform(0, (L - 1) * tau, 20)])
+    AA, rcn = orc.minmax_ls_weights(fm, tv, tau, L, 1, prec="d")
+    assert 0 < rcn < 1
+    G = np.exp(1j * np.outer(fm, np.arange(L)) * tau)
+    for i, t in enumerate(tv):
+        a, *_ = np.linalg.lstsq(G, np.exp(1j * fm * t), rcond=None)
+        assert np.abs(np.conj(AA[i]) - a).max() < 1e-6 * np.abs(a).max()
+    np.testing.assert_allclose(AA[:L], np.eye(L), atol=1e-6)
+    # repeated over shots
+    AA2, _ = orc.minmax_ls_weights(fm, np.tile(tv, 2), tau, L, 2, prec="d")
+    np.testing.assert_array_equal(AA2[:len(tv)], AA2[len(tv):])
