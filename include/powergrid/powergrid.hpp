@@ -503,6 +503,30 @@ Col<complex<T>> gridCoilImages(const Gnufft<T>& G, const Col<T>& dcf, const Col<
     return out;
 }
 
+// ------------------------------------------------------------ image output ---
+// processNIFTI.hpp:41-101: <filename>.nii, float32
+template <typename T1>
+void writeNiftiRealImage(std::string filename, Col<T1> imageData, uword Nx, uword Ny, uword Nz = 1,
+                         uword Nslab = 1, uword N_TR = 1)
+{
+    detail::require_float<T1>();
+    if (imageData.size() != Nx * Ny * Nz * Nslab * N_TR)
+        throw Error(PG_ERR_ARG, "writeNiftiRealImage: image size does not match");
+    check(pg_write_nifti_real(filename.c_str(), imageData.memptr(), Nx, Ny, Nz, Nslab, N_TR));
+}
+
+// processNIFTI.hpp:104-119: <filename>_mag.nii and <filename>_phs.nii
+template <typename T1>
+void writeNiftiMagPhsImage(std::string filename, Col<complex<T1>> imageData, uword Nx, uword Ny,
+                           uword Nz, uword Nslab = 1, uword N_TR = 1)
+{
+    detail::require_float<T1>();
+    const uword n = Nx * Ny * Nz * Nslab * N_TR;
+    if (imageData.size() != n) throw Error(PG_ERR_ARG, "writeNiftiMagPhsImage: size mismatch");
+    detail::DevBuf d(imageData.memptr(), sizeof(complex<T1>) * n);
+    check(pg_write_nifti_mag_phase(filename.c_str(), d.p, Nx, Ny, Nz, Nslab, N_TR, nullptr));
+}
+
 // reconSolve.cpp:30-54: y-fastest image-space coordinates in [-1/2, 1/2)
 template <typename T1>
 void initImageSpaceCoords(Col<T1>& ix, Col<T1>& iy, Col<T1>& iz, uword Nx, uword Ny, uword Nz)

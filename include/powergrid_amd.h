@@ -192,6 +192,20 @@ int pg_grid_coil_images(pg_op G, const float* dcf_dev, const void* data_dev, siz
 int pg_sum_of_squares(const void* coil_images_dev, size_t nvox, size_t ncoils, float* out_dev,
                       void* stream);
 
+/* ---- image output -------------------------------------------------------
+ * writeNiftiRealImage (processNIFTI.hpp:41-101): <filename>.nii, a single-
+ * file NIfTI-1 image of float32 values (host buffer), with the reference's
+ * header: dim = {3, Nx, Ny, Nz, Nslab, N_TR, 1, 1}, pixdim[0..3] = 1,
+ * vox_offset 352, scl_slope 100, mm units, magic "n+1"; the fields the
+ * reference leaves uninitialised are zero. */
+int pg_write_nifti_real(const char* filename, const float* data_host, size_t Nx, size_t Ny,
+                        size_t Nz, size_t Nslab, size_t N_TR);
+/* writeNiftiMagPhsImage (processNIFTI.hpp:104-119): |x| and arg(x) of a
+ * complex device image, formed on the device, to <filename>_mag.nii and
+ * <filename>_phs.nii. */
+int pg_write_nifti_mag_phase(const char* filename, const void* image_dev, size_t Nx, size_t Ny,
+                             size_t Nz, size_t Nslab, size_t N_TR, void* stream);
+
 /* ---- penalties ----------------------------------------------------------
  * kind 0: QuadPenalty (QuadPenalty.cpp:32-64), kind 1: TVPenalty
  * (TVPenalty.cpp:32-68, delta used); the Robject methods Cd / Ctd /

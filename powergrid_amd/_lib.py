@@ -62,6 +62,8 @@ _SIGS = {
     "pg_density_compensation": (i32, [vp, vp, vp, sz, sz, sz, vp, vp]),
     "pg_grid_coil_images": (i32, [vp, vp, vp, sz, vp, vp]),
     "pg_sum_of_squares": (i32, [vp, sz, sz, vp, vp]),
+    "pg_write_nifti_real": (i32, [C.c_char_p, vp, sz, sz, sz, sz, sz]),
+    "pg_write_nifti_mag_phase": (i32, [C.c_char_p, vp, sz, sz, sz, sz, sz, vp]),
     "pg_penalty_create": (i32, [C.POINTER(vp), i32, sz, sz, sz, f32, f32, sz]),
     "pg_penalty_destroy": (i32, [vp]),
     "pg_penalty_gradient": (i32, [vp, vp, vp, vp]),

@@ -2123,6 +2123,76 @@ int pg_sum_of_squares(const void* x, size_t nvox, size_t ncoils, float* out, voi
     PG_CATCH
 }
 
+// processNIFTI.hpp:41-101: a single-file NIfTI-1 image of float32 values.
+// Header fields at the NIfTI-1 offsets; those the reference leaves
+// uninitialised are zero here.
+static void write_nifti_real(const std::string& path, const float* data, size_t Nx, size_t Ny,
+                             size_t Nz, size_t Nslab, size_t NTR)
+{
+    unsigned char h[352];
+    std::memset(h, 0, sizeof h);
+    auto put32 = [&](int off, int32_t v) { std::memcpy(h + off, &v, 4); };
+    auto put16 = [&](int off, int16_t v) { std::memcpy(h + off, &v, 2); };
+    auto putf = [&](int off, float v) { std::memcpy(h + off, &v, 4); };
+    put32(0, 348);  // sizeof_hdr
+    const int16_t dim[8] = {3, (int16_t)Nx, (int16_t)Ny, (int16_t)Nz, (int16_t)Nslab,
+                            (int16_t)NTR, 1, 1};
+    for (int k = 0; k < 8; ++k) put16(40 + 2 * k, dim[k]);
+    put16(70, 16);  // datatype NIFTI_TYPE_FLOAT32
+    put16(72, 32);  // bitpix
+    for (int k = 0; k < 4; ++k) putf(76 + 4 * k, 1.0f);  // pixdim[0..3]
+    putf(108, 352.0f);  // vox_offset
+    putf(112, 100.0f);  // scl_slope (as the reference writes it)
+    h[123] = 2;         // xyzt_units NIFTI_UNITS_MM
+    std::memcpy(h + 344, "n+1\0", 4);  // magic; bytes 348-351: the zero extender
+    const size_t n = Nx * Ny * Nz * Nslab * NTR;
+    FILE* f = std::fopen(path.c_str(), "wb");
+    PG_REQUIRE(f, PG_ERR_ARG, "cannot open " + path + " for writing");
+    const bool ok = std::fwrite(h, 1, sizeof h, f) == sizeof h &&
+                    std::fwrite(data, sizeof(float), n, f) == n;
+    const bool closed = std::fclose(f) == 0;
+    PG_REQUIRE(ok && closed, PG_ERR_STATE, "short write to " + path);
+}
+
+int pg_write_nifti_real(const char* filename, const float* data, size_t Nx, size_t Ny, size_t Nz,
+                        size_t Nslab, size_t NTR)
+{
+    PG_TRY PG_REQUIRE(filename && data, PG_ERR_ARG, "null argument");
+    PG_REQUIRE(Nx && Ny && Nz && Nslab && NTR, PG_ERR_ARG, "empty image");
+    PG_REQUIRE(Nx < 32768 && Ny < 32768 && Nz < 32768 && Nslab < 32768 && NTR < 32768,
+               PG_ERR_ARG, "NIfTI-1 dimensions are 16-bit");
+    write_nifti_real(std::string(filename) + ".nii", data, Nx, Ny, Nz, Nslab, NTR);
+    PG_CATCH
+}
+
+int pg_write_nifti_mag_phase(const char* filename, const void* image_dev, size_t Nx, size_t Ny,
+                             size_t Nz, size_t Nslab, size_t NTR, void* stream)
+{
+    PG_TRY PG_REQUIRE(filename && image_dev, PG_ERR_ARG, "null argument");
+    PG_REQUIRE(Nx && Ny && Nz && Nslab && NTR, PG_ERR_ARG, "empty image");
+    PG_REQUIRE(Nx < 32768 && Ny < 32768 && Nz < 32768 && Nslab < 32768 && NTR < 32768,
+               PG_ERR_ARG, "NIfTI-1 dimensions are 16-bit");
+    hipStream_t s = (hipStream_t)stream;
+    const size_t n = Nx * Ny * Nz * Nslab * NTR;
+    struct Tmp {
+        float* d = nullptr;
+        ~Tmp() { dev_free(d); }
+    } t;
+    t.d = dev_alloc<float>(2 * n);
+    {
+        Prof p("mag_phase", s);
+        pgr::mag_phase((const float2*)image_dev, (long long)n, t.d, t.d + n, s);
+    }
+    PG_HIP(hipGetLastError());
+    std::vector<float> h(2 * n);
+    PG_HIP(hipMemcpyAsync(h.data(), t.d, sizeof(float) * 2 * n, hipMemcpyDeviceToHost, s));
+    PG_HIP(hipStreamSynchronize(s));
+    // processNIFTI.hpp:106-119
+    write_nifti_real(std::string(filename) + "_mag.nii", h.data(), Nx, Ny, Nz, Nslab, NTR);
+    write_nifti_real(std::string(filename) + "_phs.nii", h.data() + n, Nx, Ny, Nz, Nslab, NTR);
+    PG_CATCH
+}
+
 int pg_penalty_create(pg_penalty* out, int kind, size_t Nx, size_t Ny, size_t Nz, float beta,
                       float delta, size_t dims2penalize)
 {

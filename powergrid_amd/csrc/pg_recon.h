@@ -18,6 +18,9 @@ void weight_columns(float2* out, const float* dcf, const float2* data, long long
 // out[v] = sqrt(sum_c |x[v + c*nvox]|)   (directRecon.cpp:63-73)
 void sum_of_magnitudes(const float2* x, long long nvox, int nc, float* out, hipStream_t s);
 
+// mag[i] = |x[i]|, phs[i] = arg(x[i])   (processNIFTI.hpp:110-111)
+void mag_phase(const float2* x, long long n, float* mag, float* phs, hipStream_t s);
+
 constexpr int kMaxParts = 256;
 
 }  // namespace pgr

@@ -106,6 +106,18 @@ __global__ void __launch_bounds__(kT) k_sum_of_magnitudes(const float2* __restri
     }
 }
 
+// magnitude and phase of a complex image (processNIFTI.hpp:110-111: abs, arg)
+__global__ void __launch_bounds__(kT) k_mag_phase(const float2* __restrict__ x, long long n,
+                                                   float* __restrict__ mag,
+                                                   float* __restrict__ phs)
+{
+    const long long i = (long long)blockIdx.x * kT + threadIdx.x;
+    if (i >= n) return;
+    const float2 v = x[i];
+    mag[i] = hypotf(v.x, v.y);
+    phs[i] = atan2f(v.y, v.x);
+}
+
 unsigned nblocks(long long n) { return (unsigned)((n + kT - 1) / kT); }
 
 }  // namespace
@@ -129,6 +141,11 @@ void weight_columns(float2* out, const float* dcf, const float2* data, long long
                     hipStream_t s)
 {
     hipLaunchKernelGGL(k_weight_columns, dim3(nblocks(M)), dim3(kT), 0, s, out, dcf, data, M, nc);
+}
+
+void mag_phase(const float2* x, long long n, float* mag, float* phs, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_mag_phase, dim3(nblocks(n)), dim3(kT), 0, s, x, n, mag, phs);
 }
 
 void sum_of_magnitudes(const float2* x, long long nvox, int nc, float* out, hipStream_t s)

@@ -10,7 +10,7 @@
 //         then the direct-reconstruction calls (directRecon.cpp): float
 //         dcf[M]; complex<float> coil images[N*nc]; float sos[N]; and
 //         G's spatial pair on coil 0's data: complex<float> grid[Gx*Gy],
-//         samples[M]
+//         samples[M]; and <out.bin>_img_mag.nii, <out.bin>_img_phs.nii
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
@@ -87,6 +87,8 @@ int main(int argc, char** argv)
         o.write(reinterpret_cast<const char*>(sos.memptr()), sizeof(float) * sos.size());
         o.write(reinterpret_cast<const char*>(grid.memptr()), sizeof(complex<float>) * grid.size());
         o.write(reinterpret_cast<const char*>(samp.memptr()), sizeof(complex<float>) * samp.size());
+        // the reconstruction as magnitude/phase NIfTI files beside out.bin
+        writeNiftiMagPhsImage<float>(std::string(argv[2]) + "_img", img, Nx, Ny, Nz);
         std::cout << "operator_api_test ok: " << it << " iterations\n";
     } catch (const std::exception& e) {
         std::cerr << "operator_api_test failed: " << e.what() << "\n";

@@ -75,3 +75,8 @@ def test_cpp_operator_api_matches_oracle(orc, tmp_path):
     g64 = orc.grid_adjoint(y[:M].astype(np.complex128), kx, ky, kz, N, N, 1, Gn, prec="d")
     assert rel(grid, g64) < 5e-6
     assert rel(samp, orc.grid_forward(g64, kx, ky, kz, N, N, 1, prec="d")) < 5e-6
+    from powergrid_amd import io as PIO
+    _, mag = PIO.read_nifti(str(fout) + "_img_mag.nii")
+    _, phs = PIO.read_nifti(str(fout) + "_img_phs.nii")
+    np.testing.assert_allclose(mag, np.abs(img), rtol=1e-6)
+    assert np.max(np.abs(np.exp(1j * phs) - np.exp(1j * np.angle(img)))) < 1e-5

@@ -150,3 +150,20 @@ def test_sum_of_squares(orc, nvox, nc):
     assert got.dtype == np.float32
     assert np.max(np.abs(got - ref64) / ref64) < 2e-6
     assert np.max(np.abs(got - ref32) / ref32) < 2e-6
+
+
+def test_nifti_mag_phase_from_device(tmp_path):
+    # processNIFTI.hpp:104-119: |x| and arg(x) formed on the device
+    print("[nifti mag/phase]", flush=True)
+    from powergrid_amd import io as PIO
+    N, Nz = 32, 3
+    rng = np.random.default_rng(12)
+    x = (rng.standard_normal(N * N * Nz) + 1j * rng.standard_normal(N * N * Nz)).astype(np.complex64)
+    xd = P.DeviceArray.from_host(x)
+    PIO.write_nifti_mag_phase(tmp_path / "img", xd, N, N, Nz)
+    hm, mag = PIO.read_nifti(tmp_path / "img_mag.nii")
+    hp, phs = PIO.read_nifti(tmp_path / "img_phs.nii")
+    assert hm["dim"][:4] == [3, N, N, Nz] and hp["magic"] == b"n+1\0"
+    x64 = x.astype(np.complex128)
+    assert np.max(np.abs(mag - np.abs(x64)) / np.abs(x64)) < 2e-7
+    assert np.max(np.abs(phs - np.angle(x64))) < 1e-6
