@@ -334,6 +334,25 @@ __device__ void pen_partials(const Pen& p, const float2* ddir, const float2* x, 
     }
 }
 
+// quadratic penalty: partials of q = Re<ddir, Gradient(ddir)>, the slope of
+// pdot(s) = Re<ddir, Gradient(x + s ddir)> = pdot(0) + s q (the gradient is
+// linear; Ctd is not the exact transpose of Cd, so q differs from Denom)
+__device__ void pen_slope_partials(const Pen& p, const float2* ddir, long long i0,
+                                   long long stride, double* part, int slot)
+{
+    double v[2] = {0, 0};
+    for (long long i = i0; i < p.n; i += stride) {
+        const float2 g = pen_grad_at(p, ddir, nullptr, make_double2(0, 0), i);
+        const float2 d = ddir[i];
+        v[0] += (double)d.x * g.x + (double)d.y * g.y;
+    }
+    block_sum<2>(v);
+    if (threadIdx.x == 0) {
+        part[2 * slot] = v[0];
+        part[2 * slot + 1] = 0.0;
+    }
+}
+
 // grad: also closes the previous iteration's bookkeeping (residual norm,
 // iteration count) from its update partials
 __global__ void k_cg_grad(const float2* P, int nc, long long N, Pen p, int have_pen,
@@ -431,10 +450,14 @@ __global__ void k_cg_adir(const float2* Adir, const float2* y, const double2* Ax
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) sc->oldinprod = sc->newinprod;
     if ((int)blockIdx.x >= na) {
-        if (have_pen)
-            pen_partials(p, ddir, x, make_double2(0, 0),
-                         (long long)(blockIdx.x - na) * blockDim.x + threadIdx.x,
-                         (long long)(gridDim.x - na) * blockDim.x, cp.l[0], blockIdx.x - na);
+        if (have_pen) {
+            const long long i0 = (long long)(blockIdx.x - na) * blockDim.x + threadIdx.x;
+            const long long stride = (long long)(gridDim.x - na) * blockDim.x;
+            pen_partials(p, ddir, x, make_double2(0, 0), i0, stride, cp.l[0], blockIdx.x - na);
+            // the quadratic penalty's line search is closed-form in the update
+            // kernel; the step-1 slots are free for the slope partials
+            if (p.kind == 0) pen_slope_partials(p, ddir, i0, stride, cp.l[1], blockIdx.x - na);
+        }
         return;
     }
     double v[2] = {0, 0};
@@ -491,8 +514,8 @@ __global__ void k_cg_ls(int j, Pen p, const float2* ddir, const float2* x, pgv::
 // (x32: the FP32 copy for the penalty stencils); partial ||y - Ax||^2
 // (:155-162).  pen_mode 0 (no penalty) and 2 (quadratic penalty) take all
 // three Newton steps here: for a quadratic penalty Denom(ddir, x + s ddir)
-// is beta |C ddir|^2 for every s and Re<ddir, Gradient(x + s ddir)> =
-// pdot(0) + Re(s) beta |C ddir|^2, both from the step-0 penalty partials.
+// is the same for every s and Re<ddir, Gradient(x + s ddir)> = pdot(0) +
+// s q (the step stays real), from the step-0 penalty and slope partials.
 // pen_mode 1 (TV) takes the third step from the partials at steps[2].  With
 // vf.V set, also the next adjoint's spread values W (y - Ax) * AA_l from the
 // new Ax (the arithmetic of the spread's own value pass).
@@ -507,14 +530,15 @@ __global__ void k_cg_update(double2* Ax, const float2* Adir, const float2* y, co
     double dAWAd = 0, dAWr = 0;
     const int have_pen = (pen_mode == 1);
     if (pen_mode == 2) {
-        double a[2], pl[2];
+        double a[2], pl[2], ql[2];
         all_sum_parts<2>(cp.a, cp.na, a);
         all_sum_parts<2>(cp.l[0], cp.nl, pl);
+        all_sum_parts<2>(cp.l[1], cp.nl, ql);
         dAWAd = a[0];
         dAWr = a[1];
         step = make_double2(0, 0);
         for (int j = 1; j <= 3 && ok; ++j)
-            ok = newton_step(dAWAd, dAWr, pl[0], pl[1] + step.x * pl[0], step, stop);
+            ok = newton_step(dAWAd, dAWr, pl[0], pl[1] + step.x * ql[0], step, stop);
     } else if (have_pen) {
         double pl[2];
         all_sum_parts<2>(cp.l[2], cp.nl, pl);

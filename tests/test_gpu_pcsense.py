@@ -15,8 +15,9 @@ from powergrid_amd import synth as S
 
 
 def rel(a, b):
-    a = np.asarray(a).ravel()
-    b = np.asarray(b).ravel()
+    # in FP64: 3-D gridding sums reach 1e22, whose squares overflow FP32
+    a = np.asarray(a, dtype=np.complex128).ravel()
+    b = np.asarray(b, dtype=np.complex128).ravel()
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
 
 
