@@ -170,15 +170,19 @@ def c3_rate(P, S, iters, stream):
     R = P.TVPenalty(prob.Nx, prob.Ny, prob.Nz, prob.beta, prob.delta)
     xd = P.DeviceArray.from_host(prob.x_true.astype(np.complex64))
     yd = Sg.forward(xd, stream=stream)
-    y = S.add_noise(yd.to_host().astype(np.complex128), 40.0, seed=11).astype(np.complex64)
-    yd = P.DeviceArray.from_host(y)
+    # in scanner units (x 1e-3): at the operator's own unnormalised scale the
+    # first line-search denominator exceeds 1e25 and the PCG stops at once,
+    # as the reference's does (solve_pwls_pcg.hpp:137)
+    y = 1e-3 * S.add_noise(yd.to_host().astype(np.complex128), 40.0, seed=11)
+    yd = P.DeviceArray.from_host(y.astype(np.complex64))
     P.reconSolve(yd, Sg, R, niter=1, mode=P.CG_GRAPH, stream=stream)
     stream.synchronize()
     t0 = time.perf_counter()
     _, info = P.reconSolve(yd, Sg, R, niter=iters, mode=P.CG_GRAPH, stream=stream)
     stream.synchronize()
     dt = time.perf_counter() - t0
-    return dict(workload="3D SENSE 128x128x32, 32x8x2048 stack of spirals, 32 coils, L=12, TV",
+    return dict(workload="3D SENSE 128x128x32, 32x8x2048 stack of spirals, 32 coils, L=12, TV; "
+                         "data x 1e-3",
                 iters=info["iterations"], s_per_iter=round(dt / max(info["iterations"], 1), 4),
                 iters_per_s=round(info["iterations"] / dt, 3), grids_per_launch=prob.nc * prob.L)
 

@@ -1295,9 +1295,12 @@ static void cg_iteration(Op* A, Penalty* R, const float2* y, const float* W, CgW
     op_forward(A, w.ddir, w.Adir, s);
     pgv::cg_adir(w.Adir, y, w.Ax, W, M, ps, w.ddir, w.x32, w.cp, w.sc, s);
     pgv::cg_linesearch(ps, w.ddir, w.x32, w.cp, w.sc, s);
-    pgv::VFill vf{};
-    if (f) vf = pgv::VFill{f->G->vals, f->AA, f->L, f->nc, (long long)f->G->M};
-    pgv::cg_update(w.Ax, w.Adir, y, W, M, w.x, w.x32, w.ddir, N, ps, w.cp, vf, w.sc, s);
+    // fused: the update also writes W (y - Ax) in FP32, from which the next
+    // adjoint's per-sample spread values are formed (coalesced along grids)
+    pgv::cg_update(w.Ax, w.Adir, y, W, M, w.x, w.x32, w.ddir, N, ps, w.cp, f ? w.r : nullptr,
+                   w.sc, s);
+    if (f) pgk::launch_spread_values(*f->G, w.r, f->G->M, f->nc, f->L, f->AA, nullptr, nullptr,
+                                     nullptr, s);
 }
 
 static bool same_spec(const pgv::PenSpec& a, const pgv::PenSpec& b)

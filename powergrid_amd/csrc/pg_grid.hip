@@ -321,11 +321,16 @@ __global__ void __launch_bounds__(64 * kSpreadWaves) k_spread_patch(
     float4(*mw)[4] = sw[wv];
     for (int s0 = it.y; s0 < it.z; s0 += 64) {
         const int n = min(64, it.z - s0);
-        int ei = 0;
-        if (lane < n) {
-            ei = entries[s0 + lane];
+        // stage weights and indices; the weights of rows past n are zero (the
+        // sums below read every row of a group of 8 and mask only the values,
+        // so a stale row would otherwise enter as 0 * stale)
+        const bool ok = lane < n;
+        const int e = ok ? s0 + lane : s0;
+        const int ei = entries[e];
 #pragma unroll
-            for (int r = 0; r < 4; ++r) mw[lane][r] = ew[4 * (s0 + lane) + r];
+        for (int r = 0; r < 4; ++r) {
+            const float4 q = ew[4 * e + r];
+            mw[lane][r] = ok ? q : make_float4(0.f, 0.f, 0.f, 0.f);
         }
         pgf::wave_lds_sync();
         // rows of group k0; entries past n load the last entry's row, which

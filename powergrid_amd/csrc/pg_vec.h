@@ -54,14 +54,6 @@ struct CgParts {
     double* u;     // update: nu
     int ng, nd, na, nl, nu;
 };
-// the spread values of the next adjoint, formed by the update kernel:
-// V[i*nc*L + c*L + l] = AA[l*Mc + i] * W (y - Ax) of sample i, coil c
-struct VFill {
-    float2* V;         // null: not formed
-    const float* AA;   // L x Mc, or null (L = 1)
-    int L, nc;
-    long long Mc;      // samples per coil
-};
 size_t cg_parts_doubles(long long N, long long M);
 CgParts cg_parts(double* base, long long N, long long M);
 
@@ -77,7 +69,7 @@ void cg_linesearch(const PenSpec* p, const float2* ddir, const float2* x, const 
                    CgScal* sc, hipStream_t s);
 void cg_update(double2* Ax, const float2* Adir, const float2* y, const float* W, long long M,
                double2* x, float2* x32, const float2* ddir, long long N, const PenSpec* p,
-               const CgParts& cp, const VFill& vf, CgScal* sc, hipStream_t s);
+               const CgParts& cp, float2* rout, CgScal* sc, hipStream_t s);
 void cg_finish(const CgParts& cp, CgScal* sc, double* resid, hipStream_t s);
 
 }  // namespace pgv

@@ -517,11 +517,11 @@ __global__ void k_cg_ls(int j, Pen p, const float2* ddir, const float2* x, pgv::
 // is the same for every s and Re<ddir, Gradient(x + s ddir)> = pdot(0) +
 // s q (the step stays real), from the step-0 penalty and slope partials.
 // pen_mode 1 (TV) takes the third step from the partials at steps[2].  With
-// vf.V set, also the next adjoint's spread values W (y - Ax) * AA_l from the
-// new Ax (the arithmetic of the spread's own value pass).
+// rout set, also the next adjoint's input W (y - Ax) from the new Ax, in FP32
+// (the rounding of the spread's own value pass).
 __global__ void k_cg_update(double2* Ax, const float2* Adir, const float2* y, const float* W,
                             long long M, double2* x, float2* x32, const float2* ddir, long long N,
-                            int pen_mode, pgv::CgParts cp, pgv::VFill vf, CgScal* sc)
+                            int pen_mode, pgv::CgParts cp, float2* rout, CgScal* sc)
 {
     if (sc->stop) return;
     double2 step;
@@ -581,24 +581,14 @@ __global__ void k_cg_update(double2* Ax, const float2* Adir, const float2* y, co
             Ax[i] = o;
             const double rr = (double)y[i].x - o.x, ri = (double)y[i].y - o.y;
             v[0] += rr * rr + ri * ri;
-            if (vf.V) {
+            if (rout) {
                 double wr = rr, wi = ri;
                 if (W) {
                     const double wv = W[i];
                     wr *= wv;
                     wi *= wv;
                 }
-                const float2 r32 = make_float2((float)wr, (float)wi);
-                const long long c = i / vf.Mc, ii = i - c * vf.Mc;
-                float2* row = vf.V + ii * ((long long)vf.nc * vf.L) + c * vf.L;
-                for (int l = 0; l < vf.L; ++l) {
-                    if (vf.AA) {
-                        const float a = vf.AA[l * vf.Mc + ii];
-                        row[l] = make_float2(a * r32.x, a * r32.y);
-                    } else {
-                        row[l] = r32;
-                    }
-                }
+                rout[i] = make_float2((float)wr, (float)wi);
             }
         }
         if (i < N) {
@@ -809,11 +799,11 @@ void cg_linesearch(const PenSpec* ps, const float2* ddir, const float2* x, const
 
 void cg_update(double2* Ax, const float2* Adir, const float2* y, const float* W, long long M,
                double2* x, float2* x32, const float2* ddir, long long N, const PenSpec* ps,
-               const CgParts& cp, const VFill& vf, CgScal* sc, hipStream_t s)
+               const CgParts& cp, float2* rout, CgScal* sc, hipStream_t s)
 {
     const int pen_mode = !ps ? 0 : (ps->kind == 0 ? 2 : 1);
     hipLaunchKernelGGL(k_cg_update, dim3(cp.nu), dim3(kThreads), 0, s, Ax, Adir, y, W, M, x, x32,
-                       ddir, N, pen_mode, cp, vf, sc);
+                       ddir, N, pen_mode, cp, rout, sc);
 }
 
 void cg_finish(const CgParts& cp, CgScal* sc, double* resid, hipStream_t s)

@@ -8,7 +8,9 @@
   operator against the FP64 and FP32 restatements.
 * 3-D stack-of-spirals SENSE with TV: a reduced size against the oracle's
   PCG, and at full size (128x128x32, 32 coils, L=12) through size-independent
-  properties (adjointness, linearity) and a residual that decreases.
+  properties (adjointness, linearity) and a residual that decreases (data in
+  scanner units; at the operator's own scale the reference's line search
+  stops at its 1e25 denominator bound, which is checked too).
 * the uneven-batch shapes (96^2 and 192^2 images: radix-3 grids of 192 and
   384 points, 16 coils / L=12): operator against the FP64 restatement.
 
@@ -150,7 +152,12 @@ def test_sos3d_full_size_properties():
     assert rel(lin, a * Ax1 + b * (Sg * x2)) < 1e-5
     y = (Sg * x2).astype(np.complex64)
     R = P.TVPenalty(p.Nx, p.Ny, p.Nz, p.beta, p.delta)
+    # at the operator's own scale the first line-search denominator exceeds
+    # 1e25 and the PCG stops before its first update, as the reference's
+    # (solve_pwls_pcg.hpp:137); in scanner units it runs
     xr, info = P.reconSolve(y, Sg, R, niter=3)
+    assert info["iterations"] == 0 and info["exit_reason"] == "bad_denominator"
+    xr, info = P.reconSolve((1e-3 * y).astype(np.complex64), Sg, R, niter=3)
     assert info["iterations"] == 3
     assert np.all(np.diff(info["resid"]) < 0)
 

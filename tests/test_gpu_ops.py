@@ -162,11 +162,14 @@ def test_spread_patch_split_and_many_grids(orc):
     assert rel(Sg * x, yo) < 5e-6
 
 
-def test_spread_dense_centre_parts(orc):
-    # 32x32 image under a 16 x 2048 spiral: the central 4x4 patches see up to
-    # ~6000 samples and are summed in up to 12 parts
-    print("[spread dense centre]", flush=True)
-    N, nc, L = 32, 4, 4
+@pytest.mark.parametrize("N", [32, 128])
+def test_spread_dense_centre_parts(orc, N):
+    # a 16 x 2048 spiral: at 32x32 the central 4x4 patches see up to ~6000
+    # samples and are summed in up to 12 parts; the single-grid adjoint at
+    # 128x128 (random data) covers work items whose last stage has fewer
+    # than 64 entries after other kernels have used the LDS
+    print(f"[spread dense centre N={N}]", flush=True)
+    nc, L = 4, 4
     kx, ky, kz, t, fm, x, M = _ts_case(N, nshot=16, ns=2048, L=L)
     sens = S.coil_maps(N, nc).astype(np.complex64)
     Gg = P.Gnufft(M, 2.0, N, N, 1, kx, ky, kz)
@@ -175,8 +178,12 @@ def test_spread_dense_centre_parts(orc):
     Sd = orc.SENSE(orc.TimeSegmentation(Gd, fm, t, M, N * N, L), sens.ravel(), M, N * N, nc)
     rng = np.random.default_rng(5)
     yo = (rng.standard_normal(nc * M) + 1j * rng.standard_normal(nc * M)).astype(np.complex64)
-    assert rel(Sg / yo, Sd / yo) < 5e-6
-    assert rel(Gg / yo[:M], Gd / yo[:M]) < 5e-6
+    orc.set_threads(8)
+    try:
+        assert rel(Sg / yo, Sd / yo) < 5e-6
+        assert rel(Gg / yo[:M], Gd / yo[:M]) < 5e-6
+    finally:
+        orc.set_threads(1)
 
 
 def test_gdft_matches_oracle(orc):
