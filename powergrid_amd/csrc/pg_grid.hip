@@ -280,6 +280,37 @@ __global__ void __launch_bounds__(256) k_spread_values(
         V[(long long)i * B + b] = spread_value(in, in_cstride, resid_y, ax64, Wd, AA, L, M, b, i);
 }
 
+// the same for an even grid count, two grids per thread (16-byte stores;
+// consecutive threads fill consecutive pairs of a sample's row)
+__global__ void __launch_bounds__(256) k_spread_values2(
+    const float2* __restrict__ in, long long in_cstride, const float2* __restrict__ resid_y,
+    const double2* __restrict__ ax64, const float* __restrict__ Wd, int L,
+    const float* __restrict__ AA, int M, int B, float2* __restrict__ V)
+{
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int hb = B >> 1;
+    const long long i = p / hb;
+    if (i >= M) return;
+    const int b = 2 * (int)(p - i * hb);
+    const float2 v0 = spread_value(in, in_cstride, resid_y, ax64, Wd, AA, L, M, b, (int)i);
+    const float2 v1 = spread_value(in, in_cstride, resid_y, ax64, Wd, AA, L, M, b + 1, (int)i);
+    *reinterpret_cast<float4*>(V + i * B + b) = make_float4(v0.x, v0.y, v1.x, v1.y);
+}
+
+void launch_values(const float2* in, long long in_cstride, const float2* resid_y,
+                   const double2* ax64, const float* Wd, int L, const float* AA, int M, int B,
+                   float2* V, hipStream_t s)
+{
+    if (B % 2 == 0) {
+        const long long n = (long long)M * (B / 2);
+        hipLaunchKernelGGL(k_spread_values2, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, in,
+                           in_cstride, resid_y, ax64, Wd, L, AA, M, B, V);
+    } else {
+        hipLaunchKernelGGL(k_spread_values, dim3((unsigned)((M + 3) / 4)), dim3(256), 0, s, in,
+                           in_cstride, resid_y, ax64, Wd, L, AA, M, B, V);
+    }
+}
+
 // ------------------------------------------------- 2-D patch spread --------
 // A block of 8 wavefronts; lane = grid (coil x segment).  Wavefront w runs
 // the work item witems[8 * block + w] = {patch (-1: none), entry begin,
@@ -703,9 +734,7 @@ void spread_dispatch(const pg::Gnufft& G, const float2* in, long long in_cstride
         return;
     }
     // per-sample values of every grid, once
-    if (!vals_ready)
-        hipLaunchKernelGGL(k_spread_values, dim3((unsigned)((G.M + 3) / 4)), dim3(256), 0, s, in,
-                           in_cstride, resid_y, ax64, Wd, L, AA, G.M, B, G.vals);
+    if (!vals_ready) launch_values(in, in_cstride, resid_y, ax64, Wd, L, AA, G.M, B, G.vals, s);
     if (!D3) {
         const int nch = (B + 63) / 64;
         const int npy = g.Gy / pg::kPatch;
@@ -761,8 +790,7 @@ void launch_spread_values(const pg::Gnufft& G, const float2* in, long long in_cs
                           int L, const float* AA, const float2* resid_y, const double2* ax64,
                           const float* Wd, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_spread_values, dim3((unsigned)((G.M + 3) / 4)), dim3(256), 0, s, in,
-                       in_cstride, resid_y, ax64, Wd, L, AA, G.M, nc * L, G.vals);
+    launch_values(in, in_cstride, resid_y, ax64, Wd, L, AA, G.M, nc * L, G.vals, s);
 }
 
 void launch_spread(const pg::Gnufft& G, const float2* in, long long in_cstride, int nc, int L,
