@@ -173,7 +173,7 @@ def c3_rate(P, S, iters, stream):
     # in scanner units (x 1e-3): at the operator's own unnormalised scale the
     # first line-search denominator exceeds 1e25 and the PCG stops at once,
     # as the reference's does (solve_pwls_pcg.hpp:137)
-    y = 1e-3 * S.add_noise(yd.to_host().astype(np.complex128), 40.0, seed=11)
+    y = 1e-3 * S.add_noise(yd.to_host(stream).astype(np.complex128), 40.0, seed=11)
     yd = P.DeviceArray.from_host(y.astype(np.complex64))
     P.reconSolve(yd, Sg, R, niter=1, mode=P.CG_GRAPH, stream=stream)
     stream.synchronize()
@@ -228,6 +228,24 @@ def direct_recon_rates(P, prob, stream, reps=5):
                                 "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                 "frac": round(gbs / HBM_PEAK_GBS, 3),
                                 "volume": "128x128x32 x 32 coils"})
+
+
+def pmc_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed FETCH_SIZE/WRITE_SIZE
+    summary (profiles/sense2d_pmc_traffic.md, counters scaled by a device copy
+    of known size), or None when it holds no row for it."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        "sense2d_pmc_traffic.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) >= 7 and cells[0] == kernel:
+            try:
+                return float(cells[6]) * 1e6
+            except ValueError:
+                return None
+    return None
 
 
 def nrmse(a, b):
@@ -312,16 +330,19 @@ def main():
         Sg.adjoint(ad, stream=stream)
     stream.synchronize()
     kern = {}
-    for name in ("fwd_prep_fft", "interp", "spread", "adj_fft_post"):
+    for name in ("fwd_prep_fft", "interp", "spread_values", "spread", "adj_fft_post"):
         n, ms = P.profile.query(name)
         kern[name] = ms / max(n, 1)
     P.profile.enable(False)
     M, Gsz, B = prob.M, 4 * prob.N, prob.nc * prob.L
     gb = gridding_bytes(M, Gsz, B, 2)
     fb = fft_bytes(Gsz, B)
+    # per-sample spread values: B complex written, plus the coil data and AA read
+    vb = 8 * B * M + 8 * prob.nc * M + 4 * prob.L * M
     kinfo = {
         "interp": dict(ms=kern["interp"], bytes=gb),
         "spread": dict(ms=kern["spread"], bytes=gb),
+        "spread_values": dict(ms=kern["spread_values"], bytes=vb),
         "fwd_prep_fft": dict(ms=kern["fwd_prep_fft"], bytes=fb),
         "adj_fft_post": dict(ms=kern["adj_fft_post"], bytes=fb),
     }
@@ -330,9 +351,12 @@ def main():
         v["frac"] = v["GBs"] / HBM_PEAK_GBS
         log(f"[bench] {k:13s} {v['ms'] * 1e3:8.1f} us  {v['GBs']:8.1f} GB/s  ({100 * v['frac']:.1f}% of HBM)")
     dom = max(kinfo, key=lambda k: kinfo[k]["ms"])
+    kname = {"interp": "k_interp_lds<8>", "spread": "k_spread_patch"}.get(dom, dom)
+    tr = pmc_traffic(kname)
     roof = dict(bound="hbm", achieved=round(kinfo[dom]["GBs"], 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                frac=round(kinfo[dom]["frac"], 4), traffic=None, kernel=dom,
-                algorithmic_bytes=kinfo[dom]["bytes"], avg_launch_ms=round(kinfo[dom]["ms"], 5))
+                frac=round(kinfo[dom]["frac"], 4), traffic=None if tr is None else round(tr),
+                kernel=kname, algorithmic_bytes=kinfo[dom]["bytes"],
+                avg_launch_ms=round(kinfo[dom]["ms"], 5))
 
     # ---------------- plain path: one NUFFT at a time, host-driven PCG ------
     G2, T2, Sg2, R2 = build_ops(P, prob, P.MODE_PLAIN)

@@ -55,6 +55,8 @@ const char* pg_version(void);
 int pg_set_device(int device);
 int pg_malloc(void** ptr, size_t bytes);
 int pg_free(void* ptr);
+/* blocking copies; with stream NULL they wait for all work on the device
+ * (the library's streams are non-blocking), pg_memcpy_dtod only enqueues */
 int pg_memcpy_htod(void* dst_dev, const void* src_host, size_t bytes, void* stream);
 int pg_memcpy_dtoh(void* dst_host, const void* src_dev, size_t bytes, void* stream);
 int pg_memcpy_dtod(void* dst_dev, const void* src_dev, size_t bytes, void* stream);

@@ -906,10 +906,15 @@ static void fused_adjoint_partial(const Fused& f, const float2* in, const float2
     const int B = f.nc * f.L;
     G.ensure_batch(B);
     pgk::WeightArgs w{f.S, f.Wo, f.WoH, f.AA};
+    const long long ics = in_cstride < 0 ? G.M : in_cstride;
+    if (!vals_ready && pgk::spread_uses_values(G, B)) {
+        Prof p("spread_values", s);
+        pgk::launch_spread_values(G, in, ics, f.nc, f.L, f.AA, resid_y, ax64, Wd, s);
+        vals_ready = true;
+    }
     {
         Prof p("spread", s);
-        pgk::launch_spread(G, in, in_cstride < 0 ? G.M : in_cstride, f.nc, f.L, f.AA, resid_y, ax64,
-                           Wd, G.grids, vals_ready, s);
+        pgk::launch_spread(G, in, ics, f.nc, f.L, f.AA, resid_y, ax64, Wd, G.grids, vals_ready, s);
     }
     {
         Prof p("adj_fft_post", s);
@@ -1635,21 +1640,27 @@ int pg_free(void* ptr)
     PG_TRY if (ptr) PG_HIP(hipFree(ptr));
     PG_CATCH
 }
+// With stream NULL the copies are ordered after all work on the device: the
+// library's own streams are non-blocking, so the null stream alone would not
+// wait for them.
 int pg_memcpy_htod(void* d, const void* h, size_t n, void* s)
 {
-    PG_TRY PG_HIP(hipMemcpyAsync(d, h, n, hipMemcpyHostToDevice, (hipStream_t)s));
+    PG_TRY if (!s) PG_HIP(hipDeviceSynchronize());
+    PG_HIP(hipMemcpyAsync(d, h, n, hipMemcpyHostToDevice, (hipStream_t)s));
     PG_HIP(hipStreamSynchronize((hipStream_t)s));
     PG_CATCH
 }
 int pg_memcpy_dtoh(void* h, const void* d, size_t n, void* s)
 {
-    PG_TRY PG_HIP(hipMemcpyAsync(h, d, n, hipMemcpyDeviceToHost, (hipStream_t)s));
+    PG_TRY if (!s) PG_HIP(hipDeviceSynchronize());
+    PG_HIP(hipMemcpyAsync(h, d, n, hipMemcpyDeviceToHost, (hipStream_t)s));
     PG_HIP(hipStreamSynchronize((hipStream_t)s));
     PG_CATCH
 }
 int pg_memcpy_dtod(void* d, const void* src, size_t n, void* s)
 {
-    PG_TRY PG_HIP(hipMemcpyAsync(d, src, n, hipMemcpyDeviceToDevice, (hipStream_t)s));
+    PG_TRY if (!s) PG_HIP(hipDeviceSynchronize());
+    PG_HIP(hipMemcpyAsync(d, src, n, hipMemcpyDeviceToDevice, (hipStream_t)s));
     PG_CATCH
 }
 int pg_stream_create(void** s)

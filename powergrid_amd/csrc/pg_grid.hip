@@ -786,6 +786,8 @@ void launch_interp(const pg::Gnufft& G, const float2* grids, int nc, int L, cons
     }
 }
 
+bool spread_uses_values(const pg::Gnufft& G, int B) { return !(G.geo.dims == 3 && B < 16); }
+
 void launch_spread_values(const pg::Gnufft& G, const float2* in, long long in_cstride, int nc,
                           int L, const float* AA, const float2* resid_y, const double2* ax64,
                           const float* Wd, hipStream_t s)

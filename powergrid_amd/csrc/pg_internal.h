@@ -129,6 +129,8 @@ void launch_interp(const pg::Gnufft& G, const float2* grids, int nc, int L, cons
 void launch_spread(const pg::Gnufft& G, const float2* in, long long in_cstride, int nc, int L,
                    const float* AA, const float2* resid_y, const double2* ax64, const float* Wd,
                    float2* grids, bool vals_ready, hipStream_t s);
+// whether launch_spread reads per-sample values (all but the 3-D atomic path)
+bool spread_uses_values(const pg::Gnufft& G, int B);
 // the per-sample values G.vals of a spread (see launch_spread)
 void launch_spread_values(const pg::Gnufft& G, const float2* in, long long in_cstride, int nc,
                           int L, const float* AA, const float2* resid_y, const double2* ax64,
