@@ -299,7 +299,17 @@ int pick_rpw(int n) { return n <= 256 ? 16 : 8; }
 // that all loads of a thread are in flight together.  dap[n] =
 // 1/(os^d psi(n)), 0 where psi is NaN (griddingSupport.cpp:262-274).
 // HAS: bit 0 = coil maps S, bit 1 = segment phases Wo.
-template <int R1, int R2, int HAS>
+// HALF: the zero-padded layout of gridos 2 (Ny = N/2 image rows placed from
+// row 3N/4 on, cyclically): entry m of a thread is an image row exactly when
+// (m - 3 R1/4) mod R1 < R1/2, whatever the thread, so the other half of the
+// loads are dropped at compile time.
+template <int R1>
+__host__ __device__ constexpr bool half_live(int m)
+{
+    return ((m - 3 * R1 / 4 + R1) % R1) < R1 / 2;
+}
+
+template <int R1, int R2, int HAS, bool HALF>
 __global__ void __launch_bounds__(256) k_fwd_col_reg(
     const float2* __restrict__ img, long long img_cstride, const float2* __restrict__ S,
     const float2* __restrict__ Wo, const float* __restrict__ dap, int Nx, int Ny, int Nz,
@@ -327,6 +337,10 @@ __global__ void __launch_bounds__(256) k_fwd_col_reg(
     float2 v[R1];
 #pragma unroll
     for (int m = 0; m < R1; ++m) {
+        if (HALF && !half_live<R1>(m)) {
+            v[m] = make_float2(0.f, 0.f);
+            continue;
+        }
         const int y = (t + R2 * m - shift) & (N - 1);
         const bool ok = valid && y < Ny;
         const int yc = ok ? y : 0;
@@ -356,7 +370,7 @@ __global__ void __launch_bounds__(256) k_fwd_col_reg(
 template <int R1, int R2>
 constexpr int line_lpb() { return R1 * R2 >= 1024 ? 8 : 16; }
 
-template <int R1, int R2, int SIGN>
+template <int R1, int R2, int SIGN, bool HALF>
 __global__ void __launch_bounds__(512) k_line_reg(const float2* __restrict__ src,
                                                   float2* __restrict__ dst, long long inner_n,
                                                   long long src_os, long long dst_os, int nin,
@@ -377,6 +391,10 @@ __global__ void __launch_bounds__(512) k_line_reg(const float2* __restrict__ src
     float2 v[R1];
 #pragma unroll
     for (int m = 0; m < R1; ++m) {
+        if (HALF && !half_live<R1>(m)) {
+            v[m] = make_float2(0.f, 0.f);
+            continue;
+        }
         const int i = (t + R2 * m - in_off) & (N - 1);  // input entry at position p
         const bool ok = valid && i < nin;
         const float2 a = s[(long long)(ok ? i : 0) * inner_n];
@@ -499,15 +517,20 @@ void fwd_col_reg_n(const pg::Gnufft& G, const float2* img, long long img_cstride
     const int waves = (ncol + lpw - 1) / lpw;
     const int wpb = (waves * B >= 1024) ? 4 : 1;
     const int has = (w.S ? 1 : 0) | (w.Wo ? 2 : 0);
+    const bool half = 2 * g.Ny == N && (g.offY + N / 2) % N == 3 * N / 4;
     const dim3 grid((waves + wpb - 1) / wpb, B), block(64 * wpb);
-#define PG_COL(H)                                                                                 \
-    hipLaunchKernelGGL((k_fwd_col_reg<R1, R2, H>), grid, block, 0, s, img, img_cstride, w.S, w.Wo, \
-                       G.dap, g.Nx, g.Ny, g.Nz, g.offY, L, (const float2*)G.py.tw, tbuf)
-    switch (has) {
-    case 0: PG_COL(0); break;
-    case 1: PG_COL(1); break;
-    case 2: PG_COL(2); break;
-    default: PG_COL(3); break;
+#define PG_COL(H, HF)                                                                             \
+    hipLaunchKernelGGL((k_fwd_col_reg<R1, R2, H, HF>), grid, block, 0, s, img, img_cstride, w.S,   \
+                       w.Wo, G.dap, g.Nx, g.Ny, g.Nz, g.offY, L, (const float2*)G.py.tw, tbuf)
+    switch (has + (half ? 4 : 0)) {
+    case 0: PG_COL(0, false); break;
+    case 1: PG_COL(1, false); break;
+    case 2: PG_COL(2, false); break;
+    case 3: PG_COL(3, false); break;
+    case 4: PG_COL(0, true); break;
+    case 5: PG_COL(1, true); break;
+    case 6: PG_COL(2, true); break;
+    default: PG_COL(3, true); break;
     }
 #undef PG_COL
 }
@@ -519,9 +542,15 @@ void line_reg_n(const float2* src, float2* dst, long long inner_n, long long src
 {
     constexpr int R1 = pgf::Split<N>::R1, R2 = pgf::Split<N>::R2;
     constexpr int LPB = line_lpb<R1, R2>();
-    hipLaunchKernelGGL((k_line_reg<R1, R2, SIGN>),
-                       dim3((unsigned)((inner_n + LPB - 1) / LPB), outer), dim3(LPB * R2), 0, s, src, dst, inner_n, src_os, dst_os, nin, in_off % N,
-                       nout, out_off % N, (const float2*)p.tw);
+    const dim3 grid((unsigned)((inner_n + LPB - 1) / LPB), outer), block(LPB * R2);
+    if (2 * nin == N && in_off % N == 3 * N / 4)
+        hipLaunchKernelGGL((k_line_reg<R1, R2, SIGN, true>), grid, block, 0, s, src, dst, inner_n,
+                           src_os, dst_os, nin, in_off % N, nout, out_off % N,
+                           (const float2*)p.tw);
+    else
+        hipLaunchKernelGGL((k_line_reg<R1, R2, SIGN, false>), grid, block, 0, s, src, dst, inner_n,
+                           src_os, dst_os, nin, in_off % N, nout, out_off % N,
+                           (const float2*)p.tw);
 }
 
 template <int N>
