@@ -230,6 +230,133 @@ def direct_recon_rates(P, prob, stream, reps=5):
                                 "volume": "128x128x32 x 32 coils"})
 
 
+def _pair_ms(P, A, x, stream, reps):
+    """device milliseconds of one forward + adjoint pair of operator A (inputs
+    and outputs resident, allocated before the timed pairs)"""
+    xd = P.DeviceArray.from_host(np.asarray(x, dtype=np.complex64).ravel())
+    yd = P.DeviceArray(A.n_out)
+    xo = P.DeviceArray(A.n_in)
+    A.forward(xd, out=yd, stream=stream)
+    A.adjoint(yd, out=xo, stream=stream)
+    stream.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        A.forward(xd, out=yd, stream=stream)
+        A.adjoint(yd, out=xo, stream=stream)
+    stream.synchronize()
+    return 1e3 * (time.perf_counter() - t0) / reps
+
+
+def _cpu_pair_ms(O, A, x, threads):
+    """oracle (CPU restatement, FP32) milliseconds of one forward + adjoint pair"""
+    O.set_threads(threads)
+    t0 = time.perf_counter()
+    y = A * x.astype(np.complex64)
+    A / y
+    return 1e3 * (time.perf_counter() - t0)
+
+
+def rows_rates(P, S, stream, with_cpu=True):
+    """The operators built beyond the headline path, each timed as a forward +
+    adjoint pair on the device, with its roofline figure and the oracle's FP32
+    restatement of the reference's CPU loops timed beside it (one pair, 16
+    threads):
+      pcSenseTimeSeg: 128^2, 4 shots x 8192 samples, 8 coils, L=8 (gridding
+        bytes of the 4 per-shot batched launches, each direction);
+      pcSENSE (exact DFT per shot): 64^2, 4 shots x 1024, 8 coils (16 flops
+        per sample-voxel pair, coil and direction);
+      GdftR2: 32x32x8, 4096 samples, with SENSE over 8 coils (the same count
+        plus the three sinc factors, not counted);
+      min-max TimeSegmentation (interptype 2): the weight set-up at the
+        128^2, 16x2048, L=8 shape (the 16 interleaves share one readout's
+        times, shots=16), and the operator pair on one coil."""
+    from oracle import oracle as O
+    th = min(16, os.cpu_count() or 1)
+    out = {}
+    # pcSenseTimeSeg
+    N, Ns, nd, nc, L = 128, 4, 8192, 8, 8
+    kx, ky, kz = S.spiral(N, 16, 2048)
+    t = S.readout_times(Ns, nd, dwell=12.5e-6 / 4)
+    fm = S.field_map(N, peak_hz=120.0).astype(np.float32)
+    sens = S.coil_maps(N, nc).astype(np.complex64)
+    rng = np.random.default_rng(1)
+    ph = rng.uniform(-0.5, 0.5, (Ns, N * N)).astype(np.float32)
+    x = S.ellipse_phantom(N, seed=1)
+    A = P.PcSenseTimeSeg(kx, ky, kz, N, N, 1, nc, t, L, 1, sens, fm, ph)
+    ms = _pair_ms(P, A, x, stream, 10)
+    A.set_mode(P.MODE_PLAIN)
+    ms_plain = _pair_ms(P, A, x, stream, 2)
+    gb = 2 * Ns * gridding_bytes(nd, 4 * N * N, nc * L, 2)
+    e = dict(workload="128^2, 4 shots x 8192, 8 coils, L=8", ms_per_pair=round(ms, 3),
+             ms_per_pair_plain=round(ms_plain, 3),
+             gridding_GBs=round(gb / (ms * 1e-3) / 1e9, 1))
+    if with_cpu:
+        Oa = O.PcSenseTimeSeg(kx, ky, kz, N, N, 1, nc, t, L, 1, sens.ravel(), fm, ph.ravel())
+        e["cpu_ms_per_pair"] = round(_cpu_pair_ms(O, Oa, x, th), 1)
+        e["cpu_threads"] = th
+    out["pcSenseTimeSeg"] = e
+    # pcSENSE over the exact DFT
+    N, Ns, nd, nc = 64, 4, 1024, 8
+    kx, ky, kz = S.spiral(N, Ns, nd)
+    t = S.readout_times(Ns, nd, dwell=12.5e-6)
+    fm = S.field_map(N, peak_hz=120.0).astype(np.float32)
+    sens = S.coil_maps(N, nc).astype(np.complex64)
+    ph = rng.uniform(-0.5, 0.5, (Ns, N * N)).astype(np.float32)
+    x = S.ellipse_phantom(N, seed=2)
+    A = P.PcSENSE(kx, ky, kz, N, N, 1, nc, t, sens, fm, ph)
+    ms = _pair_ms(P, A, x, stream, 5)
+    tf = 16.0 * 2 * Ns * nd * N * N * nc / (ms * 1e-3) / 1e12
+    e = dict(workload="64^2, 4 shots x 1024, 8 coils", ms_per_pair=round(ms, 3),
+             achieved=round(tf, 2), peak=FP32_PEAK_TFLOPS, unit="TFLOP/s",
+             frac=round(tf / FP32_PEAK_TFLOPS, 4))
+    if with_cpu:
+        Oa = O.PcSENSE(kx, ky, kz, N, N, 1, nc, t, sens.ravel(), fm, ph.ravel())
+        e["cpu_ms_per_pair"] = round(_cpu_pair_ms(O, Oa, x, th), 1)
+    out["pcSENSE_gdft"] = e
+    # GdftR2 through SENSE
+    N, Nz, nc = 32, 8, 8
+    kx, ky, kz = S.spiral(N, 2, 256, Nz=Nz)
+    M = len(kx)
+    t = np.tile(S.readout_times(2, 256, dwell=1e-5), Nz).astype(np.float32)
+    ix, iy, iz = S.image_coords(N, N, Nz)
+    fm = S.field_map(N, Nz=Nz, peak_hz=150.0).astype(np.float32)
+    sens = S.coil_maps(N, nc, Nz=Nz).astype(np.complex64)
+    x = S.ellipse_phantom(N, Nz=Nz, seed=3)
+    D = P.GdftR2(M, N * N * Nz, kx, ky, kz, ix, iy, iz, fm, t, N, N, Nz)
+    A = P.SENSE(D, sens, M, N * N * Nz, nc)
+    ms = _pair_ms(P, A, x, stream, 3)
+    tf = 16.0 * 2 * M * N * N * Nz * nc / (ms * 1e-3) / 1e12
+    e = dict(workload="32x32x8, 4096 samples, 8 coils", ms_per_pair=round(ms, 3),
+             achieved=round(tf, 2), peak=FP32_PEAK_TFLOPS, unit="TFLOP/s",
+             frac=round(tf / FP32_PEAK_TFLOPS, 4))
+    if with_cpu:
+        Od = O.GdftR2(M, N * N * Nz, kx, ky, kz, ix, iy, iz, fm, t, N, N, Nz)
+        Oa = O.SENSE(Od, sens.ravel(), M, N * N * Nz, nc)
+        e["cpu_ms_per_pair"] = round(_cpu_pair_ms(O, Oa, x, th), 1)
+    out["GdftR2_sense"] = e
+    # min-max time segmentation
+    prob = S.config_sense2d(seed=0)
+    G = P.Gnufft(prob.M, 2.0, 128, 128, 1, prob.kx, prob.ky, prob.kz)
+    t0 = time.perf_counter()
+    T = P.TimeSegmentation(G, prob.fm, prob.t, prob.M, prob.N, prob.L, interptype=2,
+                           shots=16)
+    setup = time.perf_counter() - t0
+    ms = _pair_ms(P, T, prob.x_true, stream, 5)
+    e = dict(workload="128^2, 16x2048, L=8 (one coil)", setup_ms=round(1e3 * setup, 1),
+             ms_per_pair=round(ms, 3))
+    if with_cpu:
+        O.set_threads(th)
+        Go = O.Gnufft(prob.M, 2.0, 128, 128, 1, prob.kx, prob.ky, prob.kz)
+        t0 = time.perf_counter()
+        To = O.TimeSegmentation(Go, prob.fm, prob.t, prob.M, prob.N, prob.L, interptype=2,
+                                shots=16)
+        e["cpu_setup_ms"] = round(1e3 * (time.perf_counter() - t0), 1)
+        e["cpu_ms_per_pair"] = round(_cpu_pair_ms(O, To, prob.x_true, th), 1)
+    out["ts_minmax"] = e
+    O.set_threads(1)
+    return out
+
+
 def pmc_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed FETCH_SIZE/WRITE_SIZE
     summary (profiles/sense2d_pmc_traffic.md, counters scaled by a device copy
@@ -266,6 +393,8 @@ def main():
     ap.add_argument("--no-gdft", action="store_true", help="skip the Gdft roofline")
     ap.add_argument("--no-direct", action="store_true",
                     help="skip the density-compensation / gridded-coil-image timings")
+    ap.add_argument("--no-rows", action="store_true",
+                    help="skip the pcSENSE / GdftR2 / min-max interpolator timings")
     ap.add_argument("--counters", type=int, default=0, metavar="REPS",
                     help="hardware-counter pass (for rocprofv3 --pmc): one 256 MiB device-to-"
                          "device copy (byte-counter calibration), then REPS forward+adjoint "
@@ -405,6 +534,9 @@ def main():
     if not args.no_direct:
         out["direct_recon"] = direct_recon_rates(P, prob, stream)
         log(f"[bench] direct recon {out['direct_recon']}")
+    if not args.no_rows:
+        out["rows"] = rows_rates(P, S, stream, with_cpu=not args.no_cpu)
+        log(f"[bench] rows {out['rows']}")
     if args.c3_iters:
         out["c3"] = c3_rate(P, S, args.c3_iters, stream)
         log(f"[bench] c3 {out['c3']}")

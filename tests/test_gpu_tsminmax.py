@@ -24,9 +24,12 @@ def rel(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
 
 
-@pytest.mark.parametrize("N,nshot,ns,L,nc", [(32, 4, 512, 4, 2), (128, 16, 2048, 8, 4)])
-def test_minmax_ts_matches_oracle(orc, N, nshot, ns, L, nc):
-    print(f"[ts minmax N={N} L={L} nc={nc}]", flush=True)
+# shots > 1: the interleaves share one readout's times, so the weights of the
+# first shot's samples are repeated over the others (TimeSegmentation.cpp:71,150-156)
+@pytest.mark.parametrize("N,nshot,ns,L,nc,shots", [(32, 4, 512, 4, 2, 1), (32, 4, 512, 4, 2, 4),
+                                                   (128, 16, 2048, 8, 4, 16)])
+def test_minmax_ts_matches_oracle(orc, N, nshot, ns, L, nc, shots):
+    print(f"[ts minmax N={N} L={L} nc={nc} shots={shots}]", flush=True)
     kx, ky, kz = S.spiral(N, nshot, ns)
     M = len(kx)
     t = S.readout_times(nshot, ns, dwell=12.5e-6)
@@ -34,7 +37,7 @@ def test_minmax_ts_matches_oracle(orc, N, nshot, ns, L, nc):
     sens = S.coil_maps(N, nc).astype(np.complex64)
     x = S.ellipse_phantom(N, seed=1).astype(np.complex64)
     G = P.Gnufft(M, 2.0, N, N, 1, kx, ky, kz)
-    T = P.TimeSegmentation(G, fm, t, M, N * N, L, interptype=2)
+    T = P.TimeSegmentation(G, fm, t, M, N * N, L, interptype=2, shots=shots)
     Sg = P.SENSE(T, sens, M, N * N, nc)
     if N >= 128:
         orc.set_threads(8)
@@ -42,7 +45,7 @@ def test_minmax_ts_matches_oracle(orc, N, nshot, ns, L, nc):
         ops = {}
         for prec in "fd":
             Go = orc.Gnufft(M, 2.0, N, N, 1, kx, ky, kz, prec=prec)
-            To = orc.TimeSegmentation(Go, fm, t, M, N * N, L, interptype=2)
+            To = orc.TimeSegmentation(Go, fm, t, M, N * N, L, interptype=2, shots=shots)
             ops[prec] = (To, orc.SENSE(To, sens.ravel(), M, N * N, nc), Go)
         y64 = ops["d"][0] * x
         e64 = rel(T * x, y64)
