@@ -279,7 +279,7 @@ public:
 template <typename T1, typename Tobj>
 class TimeSegmentation : public detail::OpBase<TimeSegmentation<T1, Tobj>> {
 public:
-    // TimeSegmentation.h:70-71 (Hanning interpolator: interptype 1)
+    // TimeSegmentation.h:70-71 (interptype 1 Hanning, 2 exact min-max, 3 histogram min-max)
     TimeSegmentation(Tobj& G, Col<T1> map_in, Col<T1> timeVec_in, uword a, uword b,
                      uword nTimeSegs, uword interptype = 1, uword shots = 1)
         : obj(&G)

@@ -102,8 +102,11 @@ int pg_gdftr2_create(pg_op* out, size_t M, size_t N, const float* kx, const floa
  * Hanning (:80-93); 2: the exact least-squares min-max interpolator
  * (:94-150), its complex weights formed in FP64 (a Jacobi eigen-solve of the
  * L x L Gram matrix, then one device pass over samples x voxels), applied
- * one segment at a time.  interptype 3 returns PG_ERR_UNSUPPORTED.  G is
- * not owned and must outlive the new handle (TimeSegmentation.h:55). */
+ * one segment at a time; 3: the histogram approximation of 2 (:157-207),
+ * the field map binned to 256 frequencies, formed the same way over the
+ * bins (L <= 16; a field map of zero range returns PG_ERR_UNSUPPORTED, where
+ * the reference's FFT length would be infinite).  G is not owned and must
+ * outlive the new handle (TimeSegmentation.h:55). */
 int pg_ts_create(pg_op* out, pg_op G, const float* fieldmap, const float* tvec, size_t M,
                  size_t N, size_t L, size_t interptype, size_t shots);
 

@@ -333,10 +333,78 @@ def minmax_ls_weights(fm, tv, tau, L, shots, prec="f"):
     return AA, float(rcn)
 
 
+def arma_hist(v, nbins, prec="f"):
+    """Armadillo's hist(X, n_bins) (op_hist, Armadillo 9.x, not vendored):
+    n_bins centres min + (k + 1/2)/n_bins * (max - min), formed in the
+    element type; each value is counted at its nearest centre, the lower
+    one on a tie."""
+    rt = _rt(prec)
+    v = np.asarray(v, dtype=rt).ravel()
+    mn, mx = v.min(), v.max()
+    c = ((0.5 + np.arange(nbins, dtype=np.float64)) / nbins).astype(rt)
+    c = (rt(mx - mn) * c + mn).astype(rt)
+    # first minimum of |c - v| over the (sorted) centres
+    idx = np.argmin(np.abs(c[None, :] - v[:, None]), axis=1)
+    return np.bincount(idx, minlength=nbins)[:nbins]
+
+
+def histogram_ls_weights(fm, tv, tau, L, shots, prec="f"):
+    """TimeSegmentation.cpp:157-207 (interptype 3), the approximate min-max
+    interpolator, in the element precision of `prec`:
+      h = hist(fm, 256); KK = floor(2 pi / ((range/256) tau));
+      dwn = 2 pi / (KK tau);
+      ftwe = fft_KK(h exp(i dwn n tau L)) * exp(-i (min + dwn/2) tau (k - L));
+      GTGap: ftwe(d + L) on the d-th diagonal, d = -L..L, (L+1) x (L+1);
+      iGTGap = inv(GTGap.st()), or pinv when rcond <= 10 * double eps;
+      per sample: ftc = fft_KK(h exp(i n dwn t)) * exp(i (min + dwn/2)(t - tau k)),
+      row = (iGTGap ftc[0..L])^H, of which the L-column row keeps the first
+      L entries (an (L+1)-vector assigned to an L-column row under
+      ARMA_NO_DEBUG).  fft_KK zero-pads or trims to KK (calcFFT1D,
+      TimeSegmentation.cpp:224-245), forward sign.  tv: sample times minus
+      their minimum.  Returns (AA M x L, rcond, KK)."""
+    rt, ct = _rt(prec), _ct(prec)
+    fm = np.asarray(fm, dtype=rt).ravel()
+    tv = np.asarray(tv, dtype=rt).ravel()
+    nbins = 256
+    h = arma_hist(fm, nbins, prec).astype(rt)
+    mn, mx = fm.min(), fm.max()
+    rng = rt(mx - mn)
+    KK = int(np.floor(2.0 * np.pi / float(rt(rng / rt(nbins)) * rt(tau))))
+    dwn = rt(2 * np.pi / float(rt(KK * rt(tau))))
+    n = np.arange(nbins, dtype=rt)
+    temp = (h * np.exp(ct(1j) * (((dwn * n).astype(rt) * rt(tau)).astype(rt) * rt(L)))).astype(ct)
+    ftwe = np.fft.fft(temp, n=KK).astype(ct)
+    k = np.arange(KK, dtype=rt)
+    ftwe = (np.exp(ct(-1j) * (rt(mn + dwn / rt(2)) * rt(tau) * (k - rt(L))).astype(rt)) * ftwe)
+    ftwe = ftwe.astype(ct)
+    GTGap = np.zeros((L + 1, L + 1), ct)
+    for d in range(-L, L + 1):
+        for r in range(L + 1):
+            c = r + d
+            if 0 <= c <= L:
+                GTGap[r, c] = ftwe[d + L]
+    Mst = np.ascontiguousarray(GTGap.T)
+    rcond = 1.0 / float(np.real(np.linalg.cond(Mst, 1)))
+    if rcond > 10 * np.finfo(np.float64).eps:
+        iG = np.linalg.inv(Mst).astype(ct)
+    else:
+        iG = np.linalg.pinv(Mst, rcond=arma_pinv_rcond(L + 1, L + 1, prec)).astype(ct)
+    none = tv.size // shots
+    t1 = tv[:none]
+    temp = (h[None, :] * np.exp(ct(1j) * (n[None, :] * dwn * t1[:, None]).astype(rt))).astype(ct)
+    ftc = np.fft.fft(temp, n=KK, axis=1).astype(ct)
+    ph = (rt(mn + dwn / rt(2)) * (t1[:, None] - rt(tau) * k[None, :])).astype(rt)
+    ftc = (np.exp(ct(1j) * ph) * ftc).astype(ct)
+    res = (ftc[:, :L + 1] @ iG.T).astype(ct)  # none x (L+1): rows (iG GTc)^T
+    AA = np.tile(res.conj()[:, :L], (shots, 1))
+    return AA, rcond, KK
+
+
 class TimeSegmentation(_Op):
     """TimeSegmentation.cpp:36-221: interptype 1 Hanning (in C), 2 the exact
-    least-squares min-max interpolator (complex weights computed here with
-    numpy in the element precision and handed to the C operator)."""
+    least-squares min-max interpolator and 3 its histogram approximation
+    (complex weights computed here with numpy in the element precision and
+    handed to the C operator)."""
 
     def __init__(self, G, fm, t, M, N, L, interptype=1, shots=1):
         prec = G.suf
@@ -348,11 +416,14 @@ class TimeSegmentation(_Op):
         # the C side frees only its own context.
         self.rcn = None
         Lc = max(1, int(L)) if int(L) > 1 else 1
-        if interptype == 2 and Lc > 1:
+        if interptype in (2, 3) and Lc > 1:
             _, _, tau, tmin = self.tables()
             rt = _rt(prec)
             tv = (np.asarray(t, dtype=rt) - rt(tmin)).astype(rt)
-            AA, self.rcn = minmax_ls_weights(fm, tv, tau, Lc, shots, prec)
+            if interptype == 2:
+                AA, self.rcn = minmax_ls_weights(fm, tv, tau, Lc, shots, prec)
+            else:
+                AA, self.rcn, self.KK = histogram_ls_weights(fm, tv, tau, Lc, shots, prec)
             AAc = np.ascontiguousarray(AA.T).astype(_ct(prec))  # L x M = column-major M x L
             _fn("orc_ts_set_aa_complex", prec)(self.h, _p(AAc))
 

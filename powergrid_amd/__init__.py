@@ -258,7 +258,8 @@ class GdftR2(_Op):
 
 class TimeSegmentation(_Op):
     """TimeSegmentation<float, G>(G, fieldmap, tvec, M, N, L, interptype=1,
-    shots=1) (TimeSegmentation.cpp:36-221); Hanning interpolator."""
+    shots=1) (TimeSegmentation.cpp:36-221); interptype 1 Hanning, 2 the exact
+    least-squares min-max interpolator, 3 its histogram approximation."""
 
     def __init__(self, G, fieldmap, tvec, M, N, L, interptype=1, shots=1):
         fm, t = _f32(fieldmap, N), _f32(tvec, M)

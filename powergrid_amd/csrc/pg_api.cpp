@@ -1514,12 +1514,148 @@ static float2* minmax_ls_weights(const float* fieldmap, const float* tvec, size_
     return AA;
 }
 
+// TimeSegmentation.cpp:157-207 (interptype 3), in FP64: the approximate
+// min-max interpolator, which replaces the voxels by the field map's
+// 256-bin histogram (Armadillo hist: centres min + (n + 1/2) range/256, each
+// value at its nearest centre, the lower on a tie).  KK = floor(2 pi /
+// ((range/256) tau)), dwn = 2 pi / (KK tau); bin n stands for the frequency
+// om_n = min + (n + 1/2) dwn, and bins n >= KK drop out (the KK-point FFT
+// input is trimmed).  The (L+1) x (L+1) matrix inverted is GTGap.st(),
+// entry (r, c) = sum_n h_n exp(-i om_n tau (r - c)); inv when its 1-norm
+// reciprocal condition exceeds 10 double eps, else pinv (cut-off (L+1) *
+// max eigenvalue * double eps).  The reference assigns the L+1
+// coefficients of each sample to an L-column row, which keeps the first L
+// (TimeSegmentation.cpp:204 under ARMA_NO_DEBUG).  Returns the L x M
+// complex weights.
+static float2* histogram_ls_weights(const float* fieldmap, const float* tvec, size_t M, size_t N,
+                                    int L, size_t shots, double tau, double tmin)
+{
+    constexpr int nbins = pgt::kHistBins;
+    PG_REQUIRE(L + 1 <= pgt::kHistCoef, PG_ERR_UNSUPPORTED,
+               "interptype 3 supports at most 16 time segments");
+    double fmin = 0, fmax = 0;
+    for (size_t j = 0; j < N; ++j) {
+        const double v = fieldmap[j];
+        PG_REQUIRE(std::isfinite(v), PG_ERR_ARG, "interptype 3 needs a finite field map");
+        if (j == 0 || v < fmin) fmin = v;
+        if (j == 0 || v > fmax) fmax = v;
+    }
+    const double range = fmax - fmin;
+    PG_REQUIRE(range > 0, PG_ERR_UNSUPPORTED, "interptype 3 needs a field map with a nonzero range");
+    std::vector<double> h(nbins, 0.0);
+    auto centre = [&](int n) { return fmin + range * ((0.5 + n) / nbins); };
+    for (size_t j = 0; j < N; ++j) {
+        const double v = fieldmap[j];
+        int k0 = (int)std::floor((v - fmin) / range * nbins);
+        k0 = std::min(std::max(k0, 0), nbins - 1);
+        int best = std::max(k0 - 1, 0);
+        double bd = std::fabs(centre(best) - v);
+        for (int n = best + 1; n <= std::min(k0 + 1, nbins - 1); ++n) {
+            const double d = std::fabs(centre(n) - v);
+            if (d < bd) {
+                bd = d;
+                best = n;
+            }
+        }
+        h[best] += 1.0;
+    }
+    const double kkd = std::floor(2.0 * M_PI / ((range / nbins) * tau));
+    PG_REQUIRE(kkd >= 2.0 * L + 1 && kkd < 1e8, PG_ERR_UNSUPPORTED,
+               "interptype 3: the histogram FFT length is outside [2L+1, 1e8)");
+    const int KK = (int)kkd;
+    const double dwn = 2.0 * M_PI / (KK * tau);
+    const int nb = std::min(nbins, KK);
+    std::vector<double> om(nb);
+    std::vector<double2> z(nb);
+    for (int n = 0; n < nb; ++n) {
+        om[n] = fmin + (n + 0.5) * dwn;
+        z[n] = make_double2(std::cos(om[n] * tau), -std::sin(om[n] * tau));
+    }
+    const int nc = L + 1;
+    // f[d] = sum_n h_n exp(-i om_n tau d), d = 0..L; entry (r, c) is f[r - c],
+    // or conj(f[c - r]) above the diagonal
+    std::vector<cd> f(nc, cd(0, 0));
+    for (int d = 0; d < nc; ++d)
+        for (int n = 0; n < nb; ++n) f[d] += h[n] * std::exp(cd(0, -om[n] * tau * d));
+    std::vector<cd> A((size_t)nc * nc);
+    for (int r = 0; r < nc; ++r)
+        for (int c = 0; c < nc; ++c) A[(size_t)r * nc + c] = r >= c ? f[r - c] : std::conj(f[c - r]);
+    std::vector<double> ev;
+    std::vector<cd> V;
+    herm_eig(nc, A, ev, V);
+    double smax = 0, smin = 1e300;
+    for (double e : ev) {
+        smax = std::max(smax, std::fabs(e));
+        smin = std::min(smin, std::fabs(e));
+    }
+    auto inverse = [&](double cut) {
+        std::vector<cd> inv((size_t)nc * nc, cd(0, 0));
+        for (int a = 0; a < nc; ++a)
+            for (int b = 0; b < nc; ++b) {
+                cd acc(0, 0);
+                for (int k = 0; k < nc; ++k)
+                    if (std::fabs(ev[k]) > cut && ev[k] != 0.0)
+                        acc += V[(size_t)a * nc + k] * std::conj(V[(size_t)b * nc + k]) / ev[k];
+                inv[(size_t)a * nc + b] = acc;
+            }
+        return inv;
+    };
+    auto norm1 = [&](const std::vector<cd>& X) {
+        double m = 0;
+        for (int c = 0; c < nc; ++c) {
+            double sacc = 0;
+            for (int r = 0; r < nc; ++r) sacc += std::abs(X[(size_t)r * nc + c]);
+            m = std::max(m, sacc);
+        }
+        return m;
+    };
+    const double eps = 2.220446049250313e-16;
+    std::vector<cd> inv = inverse(0.0);
+    const double rcond = (smin > 0 && smax > 0) ? 1.0 / (norm1(A) * norm1(inv)) : 0.0;
+    if (!(rcond > 10 * eps)) inv = inverse((double)nc * smax * eps);
+    std::vector<double2> iG((size_t)nc * nc);
+    for (size_t e = 0; e < iG.size(); ++e) iG[e] = make_double2(inv[e].real(), inv[e].imag());
+    const size_t none = M / shots;
+    std::vector<double> tv(none);
+    for (size_t i = 0; i < none; ++i) tv[i] = (double)tvec[i] - tmin;
+    struct Tmp {
+        double* h = nullptr;
+        double* om = nullptr;
+        double2* z = nullptr;
+        double* tv = nullptr;
+        double2* iG = nullptr;
+        ~Tmp()
+        {
+            dev_free(h);
+            dev_free(om);
+            dev_free(z);
+            dev_free(tv);
+            dev_free(iG);
+        }
+    } d;
+    d.h = dev_copy(h.data(), nb);
+    d.om = dev_copy(om.data(), nb);
+    d.z = dev_copy(z.data(), nb);
+    d.tv = dev_copy(tv.data(), none);
+    d.iG = dev_copy(iG.data(), iG.size());
+    float2* AA = dev_alloc<float2>((size_t)L * M);
+    pgt::histogram_weights(d.h, d.om, d.z, nb, d.tv, (int)none, d.iG, L, (int)shots, AA, nullptr);
+    const hipError_t e1 = hipGetLastError();
+    const hipError_t e2 = hipStreamSynchronize(nullptr);
+    if (e1 != hipSuccess || e2 != hipSuccess) {
+        dev_free(AA);
+        PG_HIP(e1 != hipSuccess ? e1 : e2);
+    }
+    return AA;
+}
+
 static TsOp* ts_build(Op* G, const float* fieldmap, const float* tvec, size_t M, size_t N, size_t L,
                       size_t interptype, size_t shots)
 {
     PG_REQUIRE(G->n_in == N && G->n_out == M, PG_ERR_ARG, "TimeSegmentation sizes do not match G");
-    PG_REQUIRE(interptype == 1 || interptype == 2, PG_ERR_UNSUPPORTED,
-               "interptype 1 (Hanning) and 2 (exact least-squares min-max) are implemented");
+    PG_REQUIRE(interptype >= 1 && interptype <= 3, PG_ERR_UNSUPPORTED,
+               "interptype 1 (Hanning), 2 (exact least-squares min-max) and 3 (histogram "
+               "min-max) are implemented");
     PG_REQUIRE(N < (1u << 31) && M < (1u << 31), PG_ERR_ARG, "sizes too large");
     if (shots < 1) shots = 1;
     PG_REQUIRE(M % shots == 0, PG_ERR_ARG, "M must be a multiple of shots");
@@ -1569,6 +1705,8 @@ static TsOp* ts_build(Op* G, const float* fieldmap, const float* tvec, size_t M,
     op->tmin = (float)tmin;
     if (interptype == 2 && Lc > 1)
         op->AAc = minmax_ls_weights(fieldmap, tvec, M, N, Lc, shots, tau, tmin);
+    if (interptype == 3 && Lc > 1)
+        op->AAc = histogram_ls_weights(fieldmap, tvec, M, N, Lc, shots, tau, tmin);
     op->AA = dev_copy(AA.data(), AA.size());
     op->Wo = dev_copy(Wo.data(), Wo.size());
     op->WoH = dev_copy(WoH.data(), WoH.size());

@@ -1,8 +1,9 @@
-"""GPU parity of the exact least-squares min-max time-segmentation
-interpolator (TimeSegmentation.cpp:94-150, interptype 2) against the
-oracle's restatement, through the C ABI.
+"""GPU parity of the min-max time-segmentation interpolators -- the exact
+least-squares one (TimeSegmentation.cpp:94-150, interptype 2) and its
+histogram approximation (TimeSegmentation.cpp:157-207, interptype 3) --
+against the oracle's restatement, through the C ABI.
 
-Its Gram matrix G^H G is ill-conditioned (1/cond ~1e-6 on these maps), so
+Their Gram matrix G^H G is ill-conditioned (1/cond ~1e-6 on these maps), so
 the reference's float instantiation is itself 5e-4..6e-3 from its double
 one.  The device forms the weights in FP64 (stored FP32), so the operator is
 gated at 5e-6 against the FP64 restatement and, against the FP32
@@ -26,10 +27,11 @@ def rel(a, b):
 
 # shots > 1: the interleaves share one readout's times, so the weights of the
 # first shot's samples are repeated over the others (TimeSegmentation.cpp:71,150-156)
+@pytest.mark.parametrize("itype", [2, 3])
 @pytest.mark.parametrize("N,nshot,ns,L,nc,shots", [(32, 4, 512, 4, 2, 1), (32, 4, 512, 4, 2, 4),
                                                    (128, 16, 2048, 8, 4, 16)])
-def test_minmax_ts_matches_oracle(orc, N, nshot, ns, L, nc, shots):
-    print(f"[ts minmax N={N} L={L} nc={nc} shots={shots}]", flush=True)
+def test_minmax_ts_matches_oracle(orc, N, nshot, ns, L, nc, shots, itype):
+    print(f"[ts minmax type {itype} N={N} L={L} nc={nc} shots={shots}]", flush=True)
     kx, ky, kz = S.spiral(N, nshot, ns)
     M = len(kx)
     t = S.readout_times(nshot, ns, dwell=12.5e-6)
@@ -37,7 +39,7 @@ def test_minmax_ts_matches_oracle(orc, N, nshot, ns, L, nc, shots):
     sens = S.coil_maps(N, nc).astype(np.complex64)
     x = S.ellipse_phantom(N, seed=1).astype(np.complex64)
     G = P.Gnufft(M, 2.0, N, N, 1, kx, ky, kz)
-    T = P.TimeSegmentation(G, fm, t, M, N * N, L, interptype=2, shots=shots)
+    T = P.TimeSegmentation(G, fm, t, M, N * N, L, interptype=itype, shots=shots)
     Sg = P.SENSE(T, sens, M, N * N, nc)
     if N >= 128:
         orc.set_threads(8)
@@ -45,7 +47,7 @@ def test_minmax_ts_matches_oracle(orc, N, nshot, ns, L, nc, shots):
         ops = {}
         for prec in "fd":
             Go = orc.Gnufft(M, 2.0, N, N, 1, kx, ky, kz, prec=prec)
-            To = orc.TimeSegmentation(Go, fm, t, M, N * N, L, interptype=2, shots=shots)
+            To = orc.TimeSegmentation(Go, fm, t, M, N * N, L, interptype=itype, shots=shots)
             ops[prec] = (To, orc.SENSE(To, sens.ravel(), M, N * N, nc), Go)
         y64 = ops["d"][0] * x
         e64 = rel(T * x, y64)
@@ -63,8 +65,9 @@ def test_minmax_ts_matches_oracle(orc, N, nshot, ns, L, nc, shots):
         orc.set_threads(1)
 
 
-def test_minmax_ts_pcg(orc):
-    print("[ts minmax pcg]", flush=True)
+@pytest.mark.parametrize("itype", [2, 3])
+def test_minmax_ts_pcg(orc, itype):
+    print(f"[ts minmax pcg type {itype}]", flush=True)
     N, nshot, ns, L, nc, niter = 32, 4, 512, 4, 3, 6
     kx, ky, kz = S.spiral(N, nshot, ns)
     M = len(kx)
@@ -72,10 +75,11 @@ def test_minmax_ts_pcg(orc):
     fm = S.field_map(N, peak_hz=120.0).astype(np.float32)
     sens = S.coil_maps(N, nc).astype(np.complex64)
     G = P.Gnufft(M, 2.0, N, N, 1, kx, ky, kz)
-    Sg = P.SENSE(P.TimeSegmentation(G, fm, t, M, N * N, L, interptype=2), sens, M, N * N, nc)
+    Sg = P.SENSE(P.TimeSegmentation(G, fm, t, M, N * N, L, interptype=itype), sens, M, N * N,
+                 nc)
     Go = orc.Gnufft(M, 2.0, N, N, 1, kx, ky, kz, prec="d")
-    So = orc.SENSE(orc.TimeSegmentation(Go, fm, t, M, N * N, L, interptype=2), sens.ravel(), M,
-                   N * N, nc)
+    So = orc.SENSE(orc.TimeSegmentation(Go, fm, t, M, N * N, L, interptype=itype), sens.ravel(),
+                   M, N * N, nc)
     xt = S.ellipse_phantom(N, seed=2)
     y = S.add_noise(So * xt, 40.0, seed=3).astype(np.complex64)
     R = P.QuadPenalty(N, N, 1, 1e2)

@@ -447,3 +447,57 @@ def test_minmax_ls_weights_are_least_squares_fits(orc):
     # repeated over shots
     AA2, _ = orc.minmax_ls_weights(fm, np.tile(tv, 2), tau, L, 2, prec="d")
     np.testing.assert_array_equal(AA2[:len(tv)], AA2[len(tv):])
+
+
+def test_histogram_rule_matches_uniform_bins(orc):
+    # Armadillo's hist(X, 256): nearest of 256 centres spread evenly over
+    # [min, max] -- the same bins as np.histogram's 256 equal intervals
+    rng = np.random.default_rng(7)
+    for prec in "df":
+        v = rng.normal(0.0, 300.0, 5000)
+        h = orc.arma_hist(v, 256, prec)
+        ref = np.histogram(v.astype(np.float32 if prec == "f" else np.float64), bins=256)[0]
+        np.testing.assert_array_equal(h, ref)
+        assert h.sum() == v.size
+
+
+def _quantised_ls(fm, tv, tau, L, nbins=256):
+    """The histogram interpolator from its definition: the map quantised to
+    nbins equal bins over [min, max], bin n standing for the frequency
+    min + (n + 1/2) dwn, dwn = 2 pi / (KK tau), KK = floor(2 pi nbins /
+    (range tau)); voxels in bins n >= KK left out (the FFT input is trimmed
+    to KK); then the least-squares fit of exp(i w t) by
+    sum_{k<=L} a_k exp(i w k tau) over the remaining voxels.  Returns the
+    first L of conj(a) per sample, and KK."""
+    fm = np.asarray(fm, np.float64)
+    mn, mx = fm.min(), fm.max()
+    KK = int(np.floor(2 * np.pi * nbins / ((mx - mn) * tau)))
+    dwn = 2 * np.pi / (KK * tau)
+    b = np.clip(np.floor((fm - mn) / (mx - mn) * nbins).astype(int), 0, nbins - 1)
+    w = (mn + (b + 0.5) * dwn)[b < KK]
+    G = np.exp(1j * np.outer(w, np.arange(L + 1)) * tau)
+    A = np.array([np.linalg.lstsq(G, np.exp(1j * w * t), rcond=None)[0] for t in tv])
+    return np.conj(A)[:, :L], KK
+
+
+@pytest.mark.parametrize("peak,tau", [(100.0, 2.5e-3), (150.0, 1e-2)])
+def test_histogram_ls_weights_fit_the_quantised_map(orc, peak, tau):
+    # TimeSegmentation.cpp:157-207: the FFT-based construction equals the
+    # least-squares fit on the quantised field map; at 150 Hz x 10 ms the
+    # phase range per segment exceeds one turn, so KK < 256 and the trimmed
+    # bins drop out
+    N, L = 32, 4
+    fm = S.field_map(N, peak_hz=peak).astype(np.float64)
+    rng = np.random.default_rng(5)
+    tv = np.concatenate([np.arange(L) * tau, rng.uniform(0, L * tau, 24)])
+    AA, rcond, KK = orc.histogram_ls_weights(fm, tv, tau, L, 1, prec="d")
+    ref, KK2 = _quantised_ls(fm, tv, tau, L)
+    assert KK == KK2 and (KK < 256) == (tau == 1e-2)
+    assert 10 * np.finfo(np.float64).eps < rcond < 1
+    assert np.abs(AA - ref).max() < 1e-6 * np.abs(ref).max()
+    AA2, _, _ = orc.histogram_ls_weights(fm, np.tile(tv, 3), tau, L, 3, prec="d")
+    np.testing.assert_array_equal(AA2[:len(tv)], AA2[2 * len(tv):])
+    # the FP32 restatement follows at its own precision
+    AAf, _, KKf = orc.histogram_ls_weights(fm, tv, tau, L, 1, prec="f")
+    assert KKf == KK
+    assert np.abs(AAf - ref).max() < 1e-2 * np.abs(ref).max()
