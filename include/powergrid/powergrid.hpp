@@ -344,6 +344,49 @@ public:
     uword Ni = 0, Nc = 0, Ns = 0, Nd = 0;
 };
 
+// MPI/mpipcSENSETimeSeg.h: one rank's share, with the rank and rank count in
+// place of the boost::mpi environment and communicator.  operator* returns
+// the rank's columns in task order and operator/ takes them (the reference's
+// gather, reduce and broadcasts are left to the caller); tasks() gives each
+// column's shot and coil.
+template <typename T1>
+class mpipcSENSETimeSeg : public detail::OpBase<mpipcSENSETimeSeg<T1>> {
+public:
+    mpipcSENSETimeSeg(Col<T1> kx, Col<T1> ky, Col<T1> kz, uword nx, uword ny, uword nz, uword nc,
+                      Col<T1> t, uword l, uword type_interp, Col<complex<T1>> SENSEmap,
+                      Col<T1> FieldMap, Col<T1> ShotPhaseMap, uword rank, uword nranks,
+                      float kernelWidth = 4.0f)
+    {
+        detail::require_float<T1>();
+        Ni = nx * ny * nz;
+        Nc = nc;
+        Ns = Ni ? ShotPhaseMap.n_rows / Ni : 0;
+        Nd = Ns ? kx.n_rows / Ns : 0;
+        if (Ns * Ni != ShotPhaseMap.n_rows || Nd * Ns != kx.n_rows)
+            throw Error(PG_ERR_ARG, "mpipcSENSETimeSeg: array lengths do not give whole shots");
+        pg_op h = nullptr;
+        check(pg_pcsense_rank_create(&h, kx.memptr(), ky.memptr(),
+                                     kz.n_rows ? kz.memptr() : nullptr, nx, ny, nz, nc, t.memptr(),
+                                     l, type_interp,
+                                     reinterpret_cast<const float*>(SENSEmap.memptr()),
+                                     FieldMap.memptr(), ShotPhaseMap.memptr(), Ns, Nd, kernelWidth,
+                                     rank, nranks));
+        this->h_ = detail::OpHandle(h);
+    }
+    // (shot, coil) of each column of operator*'s output
+    std::vector<std::pair<uword, uword>> tasks() const
+    {
+        size_t n = 0;
+        check(pg_pcsense_tasks(this->handle(), &n, nullptr, nullptr, nullptr));
+        std::vector<uint32_t> s(n), c(n);
+        check(pg_pcsense_tasks(this->handle(), &n, s.data(), c.data(), nullptr));
+        std::vector<std::pair<uword, uword>> out(n);
+        for (size_t k = 0; k < n; ++k) out[k] = {s[k], c[k]};
+        return out;
+    }
+    uword Ni = 0, Nc = 0, Ns = 0, Nd = 0;
+};
+
 // ------------------------------------------------------------ penalties ---
 namespace detail {
 struct PenHandle {

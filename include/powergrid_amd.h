@@ -28,6 +28,7 @@
 #define POWERGRID_AMD_H
 
 #include <stddef.h>
+#include <stdint.h>
 
 #ifdef __cplusplus
 extern "C" {
@@ -128,6 +129,32 @@ int pg_pcsense_create(pg_op* out, const float* kx, const float* ky, const float*
                       size_t Ny, size_t Nz, size_t nc, const float* tvec, size_t L,
                       size_t interptype, const float* sens, const float* fieldmap,
                       const float* shotphase, size_t n_shots, size_t Nd, float kernelWidth);
+
+/* replaces one MPI rank's share of mpipcSENSETimeSeg<float>
+ * (MPI/mpipcSENSETimeSeg.cpp:39-150, mpipcSENSETimeSeg.h): the tasks
+ * t = 0..n_shots*nc-1 (shot t / nc, coil t % nc, :106-113) are dealt round
+ * robin, t to rank t mod nranks (:115-124); this handle covers rank `rank`'s
+ * tasks, on one device.  Arguments otherwise as pg_pcsense_create.  As in the
+ * reference, each task's TimeSegmentation is built with shots = n_shots
+ * (:146-148), so sample i of a shot takes the interpolator weights of sample
+ * i mod (Nd/n_shots) (Nd must be a multiple of n_shots).  Forward: the
+ * rank's Nd x ntask columns in task order (tempOutData, :163-171); adjoint:
+ * those columns -- column jj is the full data's column shot + coil*n_shots
+ * (:221-223) -- to the rank's partial image (:218-228), summed in task order.
+ * The gather to rank 0, the cross-rank sum and the broadcasts (:177-212,
+ * :230-260) are the caller's collectives. */
+int pg_pcsense_rank_create(pg_op* out, const float* kx, const float* ky, const float* kz,
+                           size_t Nx, size_t Ny, size_t Nz, size_t nc, const float* tvec, size_t L,
+                           size_t interptype, const float* sens, const float* fieldmap,
+                           const float* shotphase, size_t n_shots, size_t Nd, float kernelWidth,
+                           size_t rank, size_t nranks);
+
+/* the tasks of a pg_pcsense_create / pg_pcsense_rank_create handle in data
+ * column order (the reference's taskList, shotList, coilList,
+ * MPI/mpipcSENSETimeSeg.cpp:104-124): *ntask, and per task (each array may be
+ * null, else holds ntask entries) its shot, coil and column in the handle's
+ * forward output (shot + coil*n_shots for pg_pcsense_create) */
+int pg_pcsense_tasks(pg_op h, size_t* ntask, uint32_t* shot, uint32_t* coil, uint32_t* column);
 
 /* replaces pcSENSE<float>::pcSENSE (pcSENSE.cpp:39-102, pcSENSE.h:47-49):
  * shot-phase-corrected SENSE over an exact DFT per shot (Gdft, FP32 trig).

@@ -72,6 +72,9 @@ def _declare(L):
         f("orc_gdftr2_create", _vp, C.c_int, C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
           C.c_int, C.c_int, C.c_int)
         f("orc_gdftr2_grad", None, _vp, C.c_int, C.c_int, C.c_int, C.c_int, _vp)
+        f("orc_pcsense_rank_create", _vp, C.c_int, C.c_int, _vp, _vp, _vp, C.c_int, C.c_int,
+          C.c_int, C.c_int, _vp, C.c_int, C.c_int, _vp, _vp, _vp, R, C.c_int, C.c_int)
+        f("orc_pcsense_rank_tasks", C.c_int, _vp, _vp, _vp)
         f("orc_pcsense_dft_create", _vp, C.c_int, C.c_int, _vp, _vp, _vp, C.c_int, C.c_int,
           C.c_int, C.c_int, _vp, _vp, _vp, _vp)
         f("orc_op_forward", None, _vp, _vp, _vp)
@@ -462,6 +465,31 @@ class PcSenseTimeSeg(_Op):
             _p(_r(t, prec)), L, interptype, _p(_c(sens, prec)), _p(_r(fieldmap, prec)),
             _p(_r(shot_phase, prec)), kernel_width)
         super().__init__(h, prec)
+        self.Nd, self.Ns, self.Nc = Nd, Ns, nc
+
+
+class PcSenseRank(_Op):
+    """MPI/mpipcSENSETimeSeg.cpp:39-262, the share of rank `rank` of `nranks`:
+    tasks t = rank, rank + nranks, ... of the Ns*Nc (shot t // Nc, coil
+    t % Nc), a TimeSegmentation with shots = Ns per task.  Forward: the
+    rank's Nd x ntask columns in task order; adjoint: those columns to the
+    rank's partial image.  `tasks` lists (shot, coil) per column."""
+
+    def __init__(self, kx, ky, kz, Nx, Ny, Nz, nc, t, L, interptype, sens, fieldmap, shot_phase,
+                 rank, nranks, kernel_width=4.0, prec="f"):
+        Ni = Nx * Ny * Nz
+        Ns = len(np.ravel(shot_phase)) // Ni
+        Nd = len(np.ravel(kx)) // Ns
+        h = _fn("orc_pcsense_rank_create", prec)(
+            Nd, Ns, _p(_r(kx, prec)), _p(_r(ky, prec)), _p(_r(kz, prec)), Nx, Ny, Nz, nc,
+            _p(_r(t, prec)), L, interptype, _p(_c(sens, prec)), _p(_r(fieldmap, prec)),
+            _p(_r(shot_phase, prec)), kernel_width, rank, nranks)
+        super().__init__(h, prec)
+        n = _fn("orc_pcsense_rank_tasks", prec)(h, None, None)
+        sh = np.zeros(max(n, 1), np.int32)
+        co = np.zeros(max(n, 1), np.int32)
+        _fn("orc_pcsense_rank_tasks", prec)(h, _p(sh), _p(co))
+        self.tasks = list(zip(sh[:n].tolist(), co[:n].tolist()))
         self.Nd, self.Ns, self.Nc = Nd, Ns, nc
 
 

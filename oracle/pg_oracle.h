@@ -96,6 +96,14 @@ int orc_get_threads(void);
                                const R* t, int L, int interptype,             \
                                const R* sens, const R* fm, const R* phase,    \
                                R kernelWidth);                                \
+  /* MPI/mpipcSENSETimeSeg.cpp:39-262, one rank's share */                     \
+  void* orc_pcsense_rank_create_##S(int Nd, int Ns, const R* kx,              \
+                                    const R* ky, const R* kz, int Nx, int Ny, \
+                                    int Nz, int Nc, const R* t, int L,        \
+                                    int interptype, const R* sens,            \
+                                    const R* fm, const R* phase,              \
+                                    R kernelWidth, int rank, int nranks);     \
+  int orc_pcsense_rank_tasks_##S(void* op, int* shot, int* coil);             \
   /* pcSENSE.cpp:39-153 (an exact DFT per shot) */                            \
   void* orc_pcsense_dft_create_##S(int Nd, int Ns, const R* kx, const R* ky,  \
                                    const R* kz, int Nx, int Ny, int Nz,       \

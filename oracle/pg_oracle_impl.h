@@ -593,6 +593,19 @@ typedef struct {
     R *tmp_in, *tmp_out;
 } FN(pcsense_ctx);
 
+/* MPI/mpipcSENSETimeSeg.cpp:39-150: the share of one MPI rank -- its tasks
+ * (shot, coil), one Gnufft and TimeSegmentation per task -- with the maps
+ * S_c exp(-i P_s) formed at each call (:170, :223) */
+typedef struct {
+    int ntask, Ni, Nd;
+    int *shot, *coil;
+    FN(orc_op) * *G;
+    FN(orc_op) * *A;
+    R* S; /* Ni x Nc complex */
+    R* P; /* Ni x Ns real */
+    R *tmp_in, *tmp_out;
+} FN(mpipc_ctx);
+
 static R* FN(dup)(const R* p, size_t n)
 {
     R* q = (R*)malloc(sizeof(R) * (n ? n : 1));
@@ -964,6 +977,74 @@ void* FN(orc_pcsense_create)(int Nd, int Ns, const R* kx, const R* ky, const R* 
     return op;
 }
 
+/* MPI/mpipcSENSETimeSeg.cpp:39-150 for rank `rank` of `nranks`: the tasks
+ * t = 0..Ns*Nc-1 with shot t / Nc and coil t % Nc (:106-113) are dealt round
+ * robin, t to rank t mod nranks (:115-124); the rank builds, per task, a
+ * Gnufft(Nd, 2.0, ...) over the task's shot and a TimeSegmentation over the
+ * shot's times with shots = Ns (:143-148).  The forward returns the rank's
+ * Nd x ntask columns in task order (tempOutData, :163-171, before the
+ * gather); the adjoint takes those columns (column jj = the full data's
+ * column shot + coil*Ns, :221-223) and returns the rank's partial image
+ * (:218-228, before the reduce). */
+void* FN(orc_pcsense_rank_create)(int Nd, int Ns, const R* kx, const R* ky, const R* kz, int Nx,
+                                  int Ny, int Nz, int Nc, const R* t, int L, int interptype,
+                                  const R* sens, const R* fm, const R* phase, R kernelWidth,
+                                  int rank, int nranks)
+{
+    FN(mpipc_ctx)* c = (FN(mpipc_ctx)*)calloc(1, sizeof(*c));
+    const int Ni = Nx * Ny * Nz;
+    c->Ni = Ni;
+    c->Nd = Nd;
+    int n = 0;
+    for (int task = rank; task < Ns * Nc; task += nranks) ++n;
+    c->ntask = n;
+    c->shot = (int*)calloc((size_t)n + 1, sizeof(int));
+    c->coil = (int*)calloc((size_t)n + 1, sizeof(int));
+    c->G = (FN(orc_op)**)calloc((size_t)n + 1, sizeof(void*));
+    c->A = (FN(orc_op)**)calloc((size_t)n + 1, sizeof(void*));
+    int jj = 0;
+    for (int task = rank; task < Ns * Nc; task += nranks, ++jj) {
+        c->shot[jj] = task / Nc;
+        c->coil[jj] = task % Nc;
+        const size_t o = (size_t)c->shot[jj] * Nd;
+        c->G[jj] = (FN(orc_op)*)FN(orc_gnufft_create)(Nd, (R)2.0, Nx, Ny, Nz, kx + o, ky + o,
+                                                       kz + o, kernelWidth);
+        c->A[jj] = (FN(orc_op)*)FN(orc_ts_create)(c->G[jj], fm, t + o, Nd, Ni, L, interptype, Ns);
+    }
+    c->S = FN(dup)(sens, (size_t)2 * Ni * Nc);
+    c->P = FN(dup)(phase, (size_t)Ni * Ns);
+    c->tmp_in = FN(dup)(NULL, 2 * (size_t)Ni);
+    c->tmp_out = FN(dup)(NULL, 2 * (size_t)Ni);
+    FN(orc_op)* op = (FN(orc_op)*)calloc(1, sizeof(*op));
+    op->kind = 7;
+    op->n_in = Ni;
+    op->n_out = Nd * n;
+    op->ctx = c;
+    return op;
+}
+
+/* the task's map S_c exp(-i P_s) at voxel v, in R */
+static void FN(mpipc_map)(const FN(mpipc_ctx) * c, int jj, int v, R* mr, R* mi)
+{
+    const R* sp = c->S + 2 * ((size_t)c->coil[jj] * c->Ni + v);
+    const R ph = c->P[(size_t)c->shot[jj] * c->Ni + v];
+    const R er = (R)cos((double)ph), ei = (R)-sin((double)ph);
+    *mr = sp[0] * er - sp[1] * ei;
+    *mi = sp[0] * ei + sp[1] * er;
+}
+
+int FN(orc_pcsense_rank_tasks)(void* op_, int* shot, int* coil)
+{
+    FN(orc_op)* op = (FN(orc_op)*)op_;
+    if (op->kind != 7) return -1;
+    FN(mpipc_ctx)* c = (FN(mpipc_ctx)*)op->ctx;
+    for (int jj = 0; jj < c->ntask; ++jj) {
+        if (shot) shot[jj] = c->shot[jj];
+        if (coil) coil[jj] = c->coil[jj];
+    }
+    return c->ntask;
+}
+
 /* pcSENSE.cpp:39-102: the same shot/coil structure over an exact DFT per
  * shot (Gdft with the shot's samples and times, the field map, image
  * coordinates (x - N/2)/N formed in double, y fastest, pcSENSE.cpp:62-80).
@@ -1090,6 +1171,19 @@ void FN(orc_op_forward)(void* opv, const R* in, R* out)
         }
         break;
     }
+    case 7: {
+        /* MPI/mpipcSENSETimeSeg.cpp:163-171 */
+        FN(mpipc_ctx)* c = (FN(mpipc_ctx)*)op->ctx;
+        for (int jj = 0; jj < c->ntask; ++jj) {
+            for (int n = 0; n < c->Ni; ++n) {
+                R mr, mi;
+                FN(mpipc_map)(c, jj, n, &mr, &mi);
+                FN(cmul)(in[2 * n], in[2 * n + 1], mr, mi, &c->tmp_in[2 * n], &c->tmp_in[2 * n + 1]);
+            }
+            FN(orc_op_forward)(c->A[jj], c->tmp_in, out + 2 * (size_t)jj * c->Nd);
+        }
+        break;
+    }
     case 5: {
         /* pcSenseTimeSeg.cpp:129-148: column s + c*Ns of an Nd x (Ns*Nc) array */
         FN(pcsense_ctx)* c = (FN(pcsense_ctx)*)op->ctx;
@@ -1185,6 +1279,23 @@ void FN(orc_op_adjoint)(void* opv, const R* in, R* out)
         }
         break;
     }
+    case 7: {
+        /* MPI/mpipcSENSETimeSeg.cpp:218-228: sum over the tasks, in task order,
+           of conj(S_c exp(-i P_s)) (x) A_jj' d[:, jj] */
+        FN(mpipc_ctx)* c = (FN(mpipc_ctx)*)op->ctx;
+        memset(out, 0, sizeof(R) * 2 * c->Ni);
+        for (int jj = 0; jj < c->ntask; ++jj) {
+            FN(orc_op_adjoint)(c->A[jj], in + 2 * (size_t)jj * c->Nd, c->tmp_out);
+            for (int n = 0; n < c->Ni; ++n) {
+                R mr, mi, cr, ci;
+                FN(mpipc_map)(c, jj, n, &mr, &mi);
+                FN(cmul)(mr, -mi, c->tmp_out[2 * n], c->tmp_out[2 * n + 1], &cr, &ci);
+                out[2 * n] += cr;
+                out[2 * n + 1] += ci;
+            }
+        }
+        break;
+    }
     case 5: {
         /* pcSenseTimeSeg.cpp:152-171: sum over coils (outer) and shots (inner) of
            CS[c*Ns + s] (x) A_s' d[:, s + c*Ns] */
@@ -1259,6 +1370,16 @@ void FN(orc_op_destroy)(void* opv)
     case 4: {
         FN(dense_ctx)* c = (FN(dense_ctx)*)op->ctx;
         free(c->A);
+        break;
+    }
+    case 7: {
+        FN(mpipc_ctx)* c = (FN(mpipc_ctx)*)op->ctx;
+        for (int jj = 0; jj < c->ntask; ++jj) {
+            FN(orc_op_destroy)(c->A[jj]);
+            FN(orc_op_destroy)(c->G[jj]);
+        }
+        free(c->A); free(c->G); free(c->shot); free(c->coil); free(c->S); free(c->P);
+        free(c->tmp_in); free(c->tmp_out);
         break;
     }
     case 5: {

@@ -294,6 +294,41 @@ class PcSenseTimeSeg(_Op):
         self.Ns, self.Nd, self.nc = Ns, Nd, nc
 
 
+class PcSenseRank(PcSenseTimeSeg):
+    """One MPI rank's share of mpipcSENSETimeSeg<float>
+    (MPI/mpipcSENSETimeSeg.cpp:39-262): the (shot, coil) tasks t = rank,
+    rank + nranks, ... of the Ns*nc (shot t // nc, coil t % nc), each with a
+    TimeSegmentation built with shots = Ns as in the reference.  Forward: the
+    rank's Nd x ntask columns in task order; adjoint: those columns to the
+    rank's partial image.  `tasks` holds (shot, coil, full-data column
+    shot + coil*Ns) per column; the gather and the cross-rank sum are the
+    caller's collectives."""
+
+    def __init__(self, kx, ky, kz, nx, ny, nz, nc, t, L, interptype, sens, fieldmap, shot_phase,
+                 rank, nranks, kernel_width=4.0):
+        Ni = nx * ny * nz
+        ph = np.ascontiguousarray(np.ravel(shot_phase), dtype=np.float32)
+        Ns = ph.size // Ni
+        assert Ns >= 1 and Ns * Ni == ph.size
+        Mt = len(np.ravel(kx))
+        assert Mt % Ns == 0
+        Nd = Mt // Ns
+        arrs = [_f32(kx, Mt), _f32(ky, Mt), None if kz is None else _f32(kz, Mt), _f32(t, Mt)]
+        s = np.ascontiguousarray(np.ravel(sens), dtype=np.complex64)
+        assert s.size == Ni * nc
+        fm = _f32(fieldmap, Ni)
+        h = _create(lib().pg_pcsense_rank_create, _p(arrs[0]), _p(arrs[1]), _p(arrs[2]), nx, ny,
+                    nz, nc, _p(arrs[3]), L, interptype, _p(s), _p(fm), _p(ph), Ns, Nd,
+                    float(kernel_width), rank, nranks)
+        _Op.__init__(self, h)
+        self.Ns, self.Nd, self.nc = Ns, Nd, nc
+        n = C.c_size_t()
+        check(lib().pg_pcsense_tasks(h, C.byref(n), None, None, None))
+        sh, co = np.zeros(n.value, np.uint32), np.zeros(n.value, np.uint32)
+        check(lib().pg_pcsense_tasks(h, C.byref(n), _p(sh), _p(co), None))
+        self.tasks = [(int(a), int(b), int(a) + int(b) * Ns) for a, b in zip(sh, co)]
+
+
 class PcSENSE(_Op):
     """pcSENSE<float>(kx, ky, kz, nx, ny, nz, nc, t, SENSEmap, FieldMap,
     ShotPhaseMap) (pcSENSE.cpp:39-102): an exact DFT per shot.  The forward
@@ -479,7 +514,7 @@ class profile:
 
 
 __all__ = ["DeviceArray", "Stream", "Gnufft", "Gdft", "GdftR2", "TimeSegmentation", "SENSE",
-           "PcSenseTimeSeg", "PcSENSE", "QuadPenalty", "TVPenalty", "calc3DDensityCompensation",
+           "PcSenseTimeSeg", "PcSenseRank", "PcSENSE", "QuadPenalty", "TVPenalty", "calc3DDensityCompensation",
            "grid_coil_images", "calcSumOfSquaresImage", "set_tuning", "solve_pwls_pcg",
            "reconSolve", "profile", "PgError", "MODE_BATCHED", "MODE_PLAIN", "CG_GRAPH",
            "CG_HOST"]

@@ -49,6 +49,9 @@ _SIGS = {
                                 sz, sz, C.c_float]),
     "pg_pcsense_gdft_create": (i32, [C.POINTER(vp), vp, vp, vp, sz, sz, sz, sz, vp, vp, vp, vp,
                                      sz, sz]),
+    "pg_pcsense_rank_create": (i32, [C.POINTER(vp), vp, vp, vp, sz, sz, sz, sz, vp, sz, sz, vp, vp,
+                                     vp, sz, sz, C.c_float, sz, sz]),
+    "pg_pcsense_tasks": (i32, [vp, C.POINTER(sz), vp, vp, vp]),
     "pg_op_destroy": (i32, [vp]),
     "pg_op_shape": (i32, [vp, C.POINTER(sz), C.POINTER(sz)]),
     "pg_op_set_mode": (i32, [vp, i32]),

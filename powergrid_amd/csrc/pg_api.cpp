@@ -781,17 +781,36 @@ struct SenseOp : Op {
 };
 
 // pcSenseTimeSeg.cpp:40-171: Ns shots, each with its own NUFFT plan and
-// time segmentation over its Nd samples; the shot maps S_c exp(-i P_s) are
-// stored per shot, coil-major ([s][c][n]).  Data: Nd x (Ns*Nc), column
-// s + c*Ns.
+// time segmentation over its Nd samples, and the tasks (shot, coil) the
+// operator covers, shot-major.  pcSenseTimeSeg covers all Ns*Nc tasks and
+// its data is Nd x (Ns*Nc), column s + c*Ns; one MPI rank's share of
+// mpipcSENSETimeSeg (MPI/mpipcSENSETimeSeg.cpp:101-262) covers every
+// nranks-th task and its data is Nd x ntask, column = task index.  The task
+// maps S_c exp(-i P_s) are stored in task order ([task][n]), so a shot's
+// tasks are one block of maps and of partial images.
 struct PcSenseOp : Op {
     int Ns = 1, Nc = 1, L = 1;
     size_t Nd = 0, Ni = 0;
-    std::vector<Op*> G;   // NufftOp per shot (owned)
-    std::vector<Op*> A;   // TsOp per shot (owned)
-    float2* SS = nullptr;  // Ns x Nc x Ni
-    float2* P = nullptr;   // Ns x Nc x Ni partial images
+    bool rank_share = false;
+    std::vector<int> tshot, tcoil, tcol;  // per task: shot, coil, data column
+    std::vector<int> task0, ntask;        // per shot: first task, task count
+    std::vector<int> adj_order;           // task order of the adjoint's sum
+    std::vector<Op*> G;   // NufftOp per shot with tasks (owned; null otherwise)
+    std::vector<Op*> A;   // TsOp per shot with tasks (owned; null otherwise)
+    float2* SS = nullptr;  // ntask x Ni
+    float2* P = nullptr;   // ntask x Ni partial images
     float2* tmp_n = nullptr;
+    int tasks() const { return (int)tshot.size(); }
+    // first data column of shot s and the stride between its tasks' columns
+    long long col0(int s) const { return rank_share ? task0[s] : s; }
+    long long col_stride() const { return rank_share ? 1 : Ns; }
+    bool batched() const
+    {
+        if (mode != PG_MODE_BATCHED) return false;
+        for (Op* a : A)
+            if (a && static_cast<TsOp*>(a)->AAc) return false;
+        return true;
+    }
     ~PcSenseOp() override
     {
         for (Op* o : A) delete o;
@@ -946,9 +965,12 @@ static void op_prepare(Op* op)
         return;
     case K_PCSENSE: {
         PcSenseOp* pc = static_cast<PcSenseOp*>(op);
-        for (Op* ts : pc->A) {
+        for (int sh = 0; sh < pc->Ns; ++sh) {
+            Op* ts = pc->A[sh];
+            if (!ts) continue;
             if (pc->mode == PG_MODE_BATCHED)
-                static_cast<NufftOp*>(static_cast<TsOp*>(ts)->inner)->G.ensure_batch(pc->Nc * pc->L);
+                static_cast<NufftOp*>(static_cast<TsOp*>(ts)->inner)
+                    ->G.ensure_batch(pc->ntask[sh] * pc->L);
             op_prepare(ts);
         }
         return;
@@ -1026,30 +1048,30 @@ static void op_forward(Op* op, const float2* in, float2* out, hipStream_t s)
         return;
     }
     case K_PCSENSE: {
-        // pcSenseTimeSeg.cpp:129-148
+        // pcSenseTimeSeg.cpp:129-148, MPI/mpipcSENSETimeSeg.cpp:163-171
         PcSenseOp* pc = static_cast<PcSenseOp*>(op);
         const long long Ni = (long long)pc->Ni, Nd = (long long)pc->Nd;
-        for (int sh = 0; sh < pc->Ns; ++sh) {
-            TsOp* ts = static_cast<TsOp*>(pc->A[sh]);
-            const float2* SSs = pc->SS + (size_t)sh * pc->Nc * Ni;
-            if (op->mode == PG_MODE_BATCHED && !ts->AAc) {
-                // every coil x segment of the shot in one fused launch; the
-                // shot's coil blocks are Ns*Nd apart
+        if (pc->batched()) {
+            // every task x segment of a shot in one fused launch; the shot's
+            // task columns are col_stride*Nd apart
+            for (int sh = 0; sh < pc->Ns; ++sh) {
+                if (!pc->ntask[sh]) continue;
+                TsOp* ts = static_cast<TsOp*>(pc->A[sh]);
                 Fused f;
                 f.G = &static_cast<NufftOp*>(ts->inner)->G;
-                f.nc = pc->Nc;
+                f.nc = pc->ntask[sh];
                 f.L = ts->L;
-                f.S = SSs;
+                f.S = pc->SS + (size_t)pc->task0[sh] * Ni;
                 f.Wo = ts->Wo;
                 f.WoH = ts->WoH;
                 f.AA = ts->AA;
-                fused_forward(f, in, out + sh * Nd, s, (long long)pc->Ns * Nd);
-            } else {
-                for (int c = 0; c < pc->Nc; ++c) {
-                    pgv::cmul(pc->tmp_n, SSs + (size_t)c * Ni, in, Ni, s);
-                    op_forward(ts, pc->tmp_n, out + ((size_t)sh + (size_t)c * pc->Ns) * Nd, s);
-                }
+                fused_forward(f, in, out + pc->col0(sh) * Nd, s, pc->col_stride() * Nd);
             }
+            return;
+        }
+        for (int k = 0; k < pc->tasks(); ++k) {
+            pgv::cmul(pc->tmp_n, pc->SS + (size_t)k * Ni, in, Ni, s);
+            op_forward(pc->A[pc->tshot[k]], pc->tmp_n, out + (size_t)pc->tcol[k] * Nd, s);
         }
         return;
     }
@@ -1138,35 +1160,36 @@ static void op_adjoint(Op* op, const float2* in, float2* out, hipStream_t s)
         return;
     }
     case K_PCSENSE: {
-        // pcSenseTimeSeg.cpp:152-171
+        // pcSenseTimeSeg.cpp:152-171, MPI/mpipcSENSETimeSeg.cpp:218-228
         PcSenseOp* pc = static_cast<PcSenseOp*>(op);
         const long long Ni = (long long)pc->Ni, Nd = (long long)pc->Nd;
-        if (op->mode == PG_MODE_BATCHED && !static_cast<TsOp*>(pc->A[0])->AAc) {
-            // per shot, every coil x segment in one fused launch, into that
-            // shot's block of Nc partial images; then one sum over the Ns*Nc
+        if (pc->batched()) {
+            // per shot, every task x segment in one fused launch, into that
+            // shot's block of partial images; then one sum over the tasks
             for (int sh = 0; sh < pc->Ns; ++sh) {
+                if (!pc->ntask[sh]) continue;
                 TsOp* ts = static_cast<TsOp*>(pc->A[sh]);
                 Fused f;
                 f.G = &static_cast<NufftOp*>(ts->inner)->G;
-                f.nc = pc->Nc;
+                f.nc = pc->ntask[sh];
                 f.L = ts->L;
-                f.S = pc->SS + (size_t)sh * pc->Nc * Ni;
+                f.S = pc->SS + (size_t)pc->task0[sh] * Ni;
                 f.Wo = ts->Wo;
                 f.WoH = ts->WoH;
                 f.AA = ts->AA;
-                fused_adjoint_partial(f, in + sh * Nd, nullptr, nullptr, nullptr,
-                                      pc->P + (size_t)sh * pc->Nc * Ni, s, false,
-                                      (long long)pc->Ns * Nd);
+                fused_adjoint_partial(f, in + pc->col0(sh) * Nd, nullptr, nullptr, nullptr,
+                                      pc->P + (size_t)pc->task0[sh] * Ni, s, false,
+                                      pc->col_stride() * Nd);
             }
-            pgv::coil_sum(out, pc->P, pc->Ns * pc->Nc, Ni, s);
+            pgv::coil_sum(out, pc->P, pc->tasks(), Ni, s);
             return;
         }
-        for (int c = 0; c < pc->Nc; ++c)
-            for (int sh = 0; sh < pc->Ns; ++sh) {
-                op_adjoint(pc->A[sh], in + ((size_t)sh + (size_t)c * pc->Ns) * Nd, pc->tmp_n, s);
-                pgv::cmul_acc(out, pc->tmp_n, pc->SS + ((size_t)sh * pc->Nc + c) * Ni, Ni,
-                              c == 0 && sh == 0, true, s);
-            }
+        bool first = true;
+        for (int k : pc->adj_order) {
+            op_adjoint(pc->A[pc->tshot[k]], in + (size_t)pc->tcol[k] * Nd, pc->tmp_n, s);
+            pgv::cmul_acc(out, pc->tmp_n, pc->SS + (size_t)k * Ni, Ni, first, true, s);
+            first = false;
+        }
         return;
     }
     case K_PCSENSE_DFT: {
@@ -1918,16 +1941,18 @@ int pg_ts_create(pg_op* out, pg_op Gh, const float* fieldmap, const float* tvec,
 }
 
 
-int pg_pcsense_create(pg_op* out, const float* kx, const float* ky, const float* kz, size_t Nx,
-                      size_t Ny, size_t Nz, size_t nc, const float* tvec, size_t L,
-                      size_t interptype, const float* sens, const float* fieldmap,
-                      const float* shotphase, size_t n_shots, size_t Nd, float kernelWidth)
+// pcSenseTimeSeg (all tasks, rank_share false) or one MPI rank's share of
+// mpipcSENSETimeSeg (rank_share true: tasks rank, rank + nranks, ...)
+static PcSenseOp* pcsense_build(const float* kx, const float* ky, const float* kz, size_t Nx,
+                                size_t Ny, size_t Nz, size_t nc, const float* tvec, size_t L,
+                                size_t interptype, const float* sens, const float* fieldmap,
+                                const float* shotphase, size_t n_shots, size_t Nd,
+                                float kernelWidth, bool rank_share, size_t rank, size_t nranks)
 {
-    PG_TRY PG_REQUIRE(out && kx && ky && tvec && sens && fieldmap && shotphase, PG_ERR_ARG,
-                      "null argument");
+    PG_REQUIRE(kx && ky && tvec && sens && fieldmap && shotphase, PG_ERR_ARG, "null argument");
     PG_REQUIRE(Nz == 1 || kz, PG_ERR_ARG, "kz required for 3-D");
     PG_REQUIRE(n_shots > 0 && Nd > 0 && nc > 0, PG_ERR_ARG, "empty pcSenseTimeSeg");
-    *out = nullptr;
+    PG_REQUIRE(n_shots * nc < (1u << 30), PG_ERR_ARG, "too many tasks");
     const size_t Ni = Nx * Ny * Nz;
     std::unique_ptr<PcSenseOp> op(new PcSenseOp());
     op->kind = K_PCSENSE;
@@ -1935,12 +1960,45 @@ int pg_pcsense_create(pg_op* out, const float* kx, const float* ky, const float*
     op->Nc = (int)nc;
     op->Nd = Nd;
     op->Ni = Ni;
+    op->rank_share = rank_share;
+    const size_t ntot = n_shots * nc;
+    // MPI/mpipcSENSETimeSeg.cpp:106-124: task t is shot t / Nc, coil t % Nc,
+    // dealt round robin to the ranks
+    const size_t first = rank_share ? rank : 0, step = rank_share ? nranks : 1;
+    op->task0.assign(n_shots, 0);
+    op->ntask.assign(n_shots, 0);
+    for (size_t t = first; t < ntot; t += step) {
+        const int sh = (int)(t / nc), c = (int)(t % nc);
+        if (op->ntask[sh] == 0) op->task0[sh] = (int)op->tshot.size();
+        ++op->ntask[sh];
+        op->tcol.push_back(rank_share ? (int)op->tshot.size() : sh + c * (int)n_shots);
+        op->tshot.push_back(sh);
+        op->tcoil.push_back(c);
+    }
+    const int nt = op->tasks();
+    PG_REQUIRE(nt > 0, PG_ERR_ARG, "this rank has no tasks (rank >= shots x coils)");
+    // adjoint sum order: coils outer, shots inner (pcSenseTimeSeg.cpp:160-167),
+    // or the rank's task order (MPI/mpipcSENSETimeSeg.cpp:218-228)
+    for (int k = 0; k < nt; ++k) op->adj_order.push_back(k);
+    if (!rank_share)
+        std::sort(op->adj_order.begin(), op->adj_order.end(), [&](int a, int b) {
+            return op->tcoil[a] != op->tcoil[b] ? op->tcoil[a] < op->tcoil[b]
+                                                : op->tshot[a] < op->tshot[b];
+        });
     op->n_in = Ni;
-    op->n_out = Nd * n_shots * nc;
+    op->n_out = Nd * (size_t)nt;
+    // the interpolator's shots argument: 1 in pcSenseTimeSeg.cpp:99-100, Ns in
+    // MPI/mpipcSENSETimeSeg.cpp:146-148 (there sample i of a shot takes the
+    // weights of sample i mod Nd/Ns)
+    const size_t ts_shots = rank_share ? n_shots : 1;
+    PG_REQUIRE(Nd % ts_shots == 0, PG_ERR_ARG,
+               "mpipcSENSETimeSeg needs the samples per shot to be a multiple of the shot count");
+    op->G.assign(n_shots, nullptr);
+    op->A.assign(n_shots, nullptr);
     std::vector<float> zk(Nd, 0.0f);
     for (size_t sh = 0; sh < n_shots; ++sh) {
-        // pcSenseTimeSeg.cpp:96-101: Gnufft(Nd, 2.0, ...) and TimeSegmentation
-        // over the shot's samples, shots = 1
+        if (!op->ntask[sh]) continue;
+        // Gnufft(Nd, 2.0, ...) and TimeSegmentation over the shot's samples
         std::unique_ptr<NufftOp> g(new NufftOp());
         g->kind = K_NUFFT;
         const size_t o = sh * Nd;
@@ -1953,25 +2011,68 @@ int pg_pcsense_create(pg_op* out, const float* kx, const float* ky, const float*
         }
         g->n_in = Ni;
         g->n_out = Nd;
-        op->G.push_back(g.release());
-        op->A.push_back(ts_build(op->G.back(), fieldmap, tvec + o, Nd, Ni, L, interptype, 1));
+        op->G[sh] = g.release();
+        op->A[sh] = ts_build(op->G[sh], fieldmap, tvec + o, Nd, Ni, L, interptype, ts_shots);
+        op->L = static_cast<TsOp*>(op->A[sh])->L;
     }
-    op->L = static_cast<TsOp*>(op->A[0])->L;
-    // pcSenseTimeSeg.cpp:104-115, in double: S_c exp(-i P_s), stored [s][c][n]
-    std::vector<float2> ss(n_shots * nc * Ni);
-    for (size_t sh = 0; sh < n_shots; ++sh)
-        for (size_t c = 0; c < nc; ++c)
-            for (size_t n = 0; n < Ni; ++n) {
-                const double sr = sens[2 * (c * Ni + n)], si = sens[2 * (c * Ni + n) + 1];
-                const double ph = shotphase[sh * Ni + n];
-                const double er = std::cos(ph), ei = -std::sin(ph);
-                ss[(sh * nc + c) * Ni + n] = make_float2((float)(sr * er - si * ei),
-                                                         (float)(sr * ei + si * er));
-            }
+    // pcSenseTimeSeg.cpp:104-115, in double: S_c exp(-i P_s), stored per task
+    std::vector<float2> ss((size_t)nt * Ni);
+    for (int k = 0; k < nt; ++k) {
+        const size_t c = (size_t)op->tcoil[k], sh = (size_t)op->tshot[k];
+        for (size_t n = 0; n < Ni; ++n) {
+            const double sr = sens[2 * (c * Ni + n)], si = sens[2 * (c * Ni + n) + 1];
+            const double ph = shotphase[sh * Ni + n];
+            const double er = std::cos(ph), ei = -std::sin(ph);
+            ss[(size_t)k * Ni + n] = make_float2((float)(sr * er - si * ei),
+                                                 (float)(sr * ei + si * er));
+        }
+    }
     op->SS = dev_copy(ss.data(), ss.size());
-    op->P = dev_alloc<float2>(n_shots * nc * Ni);
+    op->P = dev_alloc<float2>((size_t)nt * Ni);
     op->tmp_n = dev_alloc<float2>(Ni);
-    *out = reinterpret_cast<pg_op>(op.release());
+    return op.release();
+}
+
+int pg_pcsense_create(pg_op* out, const float* kx, const float* ky, const float* kz, size_t Nx,
+                      size_t Ny, size_t Nz, size_t nc, const float* tvec, size_t L,
+                      size_t interptype, const float* sens, const float* fieldmap,
+                      const float* shotphase, size_t n_shots, size_t Nd, float kernelWidth)
+{
+    PG_TRY PG_REQUIRE(out, PG_ERR_ARG, "null argument");
+    *out = nullptr;
+    *out = reinterpret_cast<pg_op>(pcsense_build(kx, ky, kz, Nx, Ny, Nz, nc, tvec, L, interptype,
+                                                 sens, fieldmap, shotphase, n_shots, Nd,
+                                                 kernelWidth, false, 0, 1));
+    PG_CATCH
+}
+
+int pg_pcsense_rank_create(pg_op* out, const float* kx, const float* ky, const float* kz,
+                           size_t Nx, size_t Ny, size_t Nz, size_t nc, const float* tvec, size_t L,
+                           size_t interptype, const float* sens, const float* fieldmap,
+                           const float* shotphase, size_t n_shots, size_t Nd, float kernelWidth,
+                           size_t rank, size_t nranks)
+{
+    PG_TRY PG_REQUIRE(out, PG_ERR_ARG, "null argument");
+    PG_REQUIRE(nranks > 0 && rank < nranks, PG_ERR_ARG, "rank must be below nranks");
+    *out = nullptr;
+    *out = reinterpret_cast<pg_op>(pcsense_build(kx, ky, kz, Nx, Ny, Nz, nc, tvec, L, interptype,
+                                                 sens, fieldmap, shotphase, n_shots, Nd,
+                                                 kernelWidth, true, rank, nranks));
+    PG_CATCH
+}
+
+int pg_pcsense_tasks(pg_op h, size_t* ntask, uint32_t* shot, uint32_t* coil, uint32_t* column)
+{
+    PG_TRY Op* op = as_op(h);
+    PG_REQUIRE(op->kind == K_PCSENSE, PG_ERR_ARG, "not a pcSenseTimeSeg operator");
+    PG_REQUIRE(ntask, PG_ERR_ARG, "null argument");
+    const PcSenseOp* pc = static_cast<const PcSenseOp*>(op);
+    *ntask = (size_t)pc->tasks();
+    for (int k = 0; k < pc->tasks(); ++k) {
+        if (shot) shot[k] = (uint32_t)pc->tshot[k];
+        if (coil) coil[k] = (uint32_t)pc->tcoil[k];
+        if (column) column[k] = (uint32_t)pc->tcol[k];
+    }
     PG_CATCH
 }
 
@@ -2078,8 +2179,10 @@ int pg_op_set_mode(pg_op h, int mode)
     if (op->kind == K_PCSENSE) {
         // the per-shot operators are internal: they follow the composite
         PcSenseOp* pc = static_cast<PcSenseOp*>(op);
-        for (Op* o : pc->A) o->mode = mode;
-        for (Op* o : pc->G) o->mode = mode;
+        for (Op* o : pc->A)
+            if (o) o->mode = mode;
+        for (Op* o : pc->G)
+            if (o) o->mode = mode;
     }
     PG_CATCH
 }

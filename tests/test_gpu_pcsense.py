@@ -122,3 +122,44 @@ def test_pcsense_gdft_pcg_matches_oracle(orc):
         print(f"  mode={mode} nrmse vs fp64 oracle {rel(xg, x64):.2e}")
         assert info["iterations"] == n64
         assert rel(xg, x64) < 1e-3
+
+
+@pytest.mark.parametrize("mode", [P.MODE_BATCHED, P.MODE_PLAIN])
+def test_pcsense_rank_shares_match_oracle(orc, mode):
+    # MPI/mpipcSENSETimeSeg.cpp:101-262: 4 shots x 6 coils dealt to 5 ranks;
+    # each rank's share against the oracle's, and the shares gathered to the
+    # full data layout are a whole, shots = Ns segmentation included
+    print(f"[pcsense rank shares mode={mode}]", flush=True)
+    Ns, Nd, N, nc, L, nr = 4, 1024, 32, 6, 4, 5
+    kx, ky, kz, t, fm, sens, phase = case(Ns, Nd, N, nc, seed=6)
+    x = S.ellipse_phantom(N, seed=7).astype(np.complex64)
+    Y = np.zeros((Ns * nc, Nd), np.complex128)
+    img = np.zeros(N * N, np.complex128)
+    yfull = orc.PcSenseTimeSeg(kx, ky, kz, N, N, 1, nc, t, 1, 1, sens.ravel(), fm,
+                               phase.ravel(), prec="d") * x
+    for r in range(nr):
+        A = P.PcSenseRank(kx, ky, kz, N, N, 1, nc, t, L, 1, sens, fm, phase, r, nr)
+        A.set_mode(mode)
+        Of = orc.PcSenseRank(kx, ky, kz, N, N, 1, nc, t, L, 1, sens.ravel(), fm, phase.ravel(),
+                             r, nr, prec="f")
+        Od = orc.PcSenseRank(kx, ky, kz, N, N, 1, nc, t, L, 1, sens.ravel(), fm, phase.ravel(),
+                             r, nr, prec="d")
+        assert [(s, c) for s, c, _ in A.tasks] == Of.tasks
+        y = A * x
+        assert rel(y, Of * x) < 1e-4
+        assert rel(y, Od * x) < 5e-6
+        yo = (Od * x).astype(np.complex64)
+        xa = A / yo
+        assert rel(xa, Of / yo) < 1e-4
+        assert rel(xa, Od / yo) < 5e-6
+        # at L = 1 the shares gather to pcSenseTimeSeg's data, and their
+        # partial images sum to its adjoint
+        A1 = P.PcSenseRank(kx, ky, kz, N, N, 1, nc, t, 1, 1, sens, fm, phase, r, nr)
+        A1.set_mode(mode)
+        cols = [col for _, _, col in A1.tasks]
+        Y[cols] = (A1 * x).reshape(len(cols), Nd)
+        img += A1 / yfull.reshape(Ns * nc, Nd)[cols].astype(np.complex64).ravel()
+    assert rel(Y.ravel(), yfull) < 5e-6
+    Of1 = orc.PcSenseTimeSeg(kx, ky, kz, N, N, 1, nc, t, 1, 1, sens.ravel(), fm, phase.ravel(),
+                             prec="d")
+    assert rel(img, Of1 / yfull.astype(np.complex64)) < 5e-6

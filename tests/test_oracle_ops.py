@@ -501,3 +501,72 @@ def test_histogram_ls_weights_fit_the_quantised_map(orc, peak, tau):
     AAf, _, KKf = orc.histogram_ls_weights(fm, tv, tau, L, 1, prec="f")
     assert KKf == KK
     assert np.abs(AAf - ref).max() < 1e-2 * np.abs(ref).max()
+
+
+def _pcsense_inputs(N=16, Ns=3, nd=128, nc=4):
+    kx, ky, kz = S.spiral(N, Ns, nd)
+    t = S.readout_times(Ns, nd, dwell=2e-5, te=1e-3)
+    fm = S.field_map(N, peak_hz=80.0)
+    sens = S.coil_maps(N, nc)
+    ph = np.random.default_rng(3).uniform(-1.0, 1.0, (Ns, N * N))
+    x = S.ellipse_phantom(N, seed=2)
+    return kx, ky, kz, t, fm, sens.ravel(), ph.ravel(), x
+
+
+def test_pcsense_rank_task_lists(orc):
+    # MPI/mpipcSENSETimeSeg.cpp:106-124 by hand: 3 shots x 4 coils = 12
+    # tasks (task = shot*4 + coil) dealt round robin to 5 ranks
+    kx, ky, kz, t, fm, sens, ph, x = _pcsense_inputs()
+    want = {0: [(0, 0), (1, 1), (2, 2)], 1: [(0, 1), (1, 2), (2, 3)], 2: [(0, 2), (1, 3)],
+            3: [(0, 3), (2, 0)], 4: [(1, 0), (2, 1)]}
+    for r in range(5):
+        R = orc.PcSenseRank(kx, ky, kz, 16, 16, 1, 4, t, 1, 1, sens, fm, ph, r, 5, prec="d")
+        assert R.tasks == want[r]
+        assert R.n_out == 128 * len(want[r])
+
+
+def test_pcsense_rank_shares_assemble_the_full_operator(orc):
+    # at L = 1 (no segmentation, so the shots argument of the tasks'
+    # TimeSegmentation has no effect) the ranks' columns, gathered to column
+    # shot + coil*Ns, are pcSenseTimeSeg's forward, and the sum of the
+    # ranks' partial images is its adjoint (:183-212, :244-262)
+    N, Ns, nd, nc, nr = 16, 3, 128, 4, 5
+    kx, ky, kz, t, fm, sens, ph, x = _pcsense_inputs(N, Ns, nd, nc)
+    full = orc.PcSenseTimeSeg(kx, ky, kz, N, N, 1, nc, t, 1, 1, sens, fm, ph, prec="d")
+    y = full * x
+    Y = y.reshape(Ns * nc, nd)
+    img = np.zeros(N * N, complex)
+    seen = []
+    for r in range(nr):
+        R = orc.PcSenseRank(kx, ky, kz, N, N, 1, nc, t, 1, 1, sens, fm, ph, r, nr, prec="d")
+        cols = [s + c * Ns for s, c in R.tasks]
+        assert rel((R * x).reshape(len(cols), nd), Y[cols]) < 1e-12
+        img += R / Y[cols].ravel()
+        seen += R.tasks
+    assert sorted(seen) == [(s, c) for s in range(Ns) for c in range(nc)]
+    assert rel(img, full / y) < 1e-12
+
+
+def test_pcsense_rank_segments_repeat_over_Ns_subshots(orc):
+    # :146-148 passes shots = Ns to each task's TimeSegmentation: sample i of
+    # a shot takes the Hanning weights of sample i mod (Nd/Ns), tau from the
+    # whole shot's times (TimeSegmentation.cpp:64-93)
+    N, Ns, nd, nc, L = 16, 2, 128, 2, 3
+    kx, ky, kz, t, fm, sens, ph, x = _pcsense_inputs(N, Ns, nd, nc)
+    G = orc.Gnufft(nd, 2.0, N, N, 1, kx[nd:], ky[nd:], kz[nd:], prec="d")
+    T = orc.TimeSegmentation(G, fm, t[nd:], nd, N * N, L, shots=Ns)
+    AA, _, tau, tmin = T.tables()
+    ts = np.asarray(t[nd:], np.float64)
+    tau_w = (ts.max() - ts.min()) / L
+    t0 = (ts - ts.min())[np.arange(nd) % (nd // Ns)]
+    for l in range(L):
+        d = t0 - l * tau_w
+        want = np.where(np.abs(d) <= tau_w, 0.5 + 0.5 * np.cos(np.pi * d / tau_w), 0.0)
+        np.testing.assert_allclose(AA[l], want, atol=1e-12)
+    # and the rank operator applies exactly that segmentation to its task
+    R = orc.PcSenseRank(kx, ky, kz, N, N, 1, nc, t, L, 1, sens, fm, ph, 1, 1, prec="d")
+    jj = R.tasks.index((1, 1))
+    m = sens.reshape(nc, -1)[1] * np.exp(-1j * ph.reshape(Ns, -1)[1])
+    assert rel((R * x).reshape(-1, nd)[jj], T * (x * m)) < 1e-12
+    T1 = orc.TimeSegmentation(G, fm, t[nd:], nd, N * N, L, shots=1)
+    assert rel(T1 * (x * m), T * (x * m)) > 1e-2
