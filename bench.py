@@ -267,9 +267,10 @@ def rows_rates(P, S, stream, with_cpu=True):
         per sample-voxel pair, coil and direction);
       GdftR2: 32x32x8, 4096 samples, with SENSE over 8 coils (the same count
         plus the three sinc factors, not counted);
-      min-max TimeSegmentation (interptype 2): the weight set-up at the
-        128^2, 16x2048, L=8 shape (the 16 interleaves share one readout's
-        times, shots=16), and the operator pair on one coil."""
+      min-max TimeSegmentation, exact (interptype 2) and histogram (3): the
+        weight set-up at the 128^2, 16x2048, L=8 shape (the 16 interleaves
+        share one readout's times, shots=16), and the SENSE pair over 8 coils,
+        batched (complex weights in the fused kernels) and plain."""
     from oracle import oracle as O
     th = min(16, os.cpu_count() or 1)
     out = {}
@@ -334,25 +335,32 @@ def rows_rates(P, S, stream, with_cpu=True):
         Oa = O.SENSE(Od, sens.ravel(), M, N * N * Nz, nc)
         e["cpu_ms_per_pair"] = round(_cpu_pair_ms(O, Oa, x, th), 1)
     out["GdftR2_sense"] = e
-    # min-max time segmentation
+    # min-max time segmentation, exact (2) and histogram (3)
     prob = S.config_sense2d(seed=0)
     G = P.Gnufft(prob.M, 2.0, 128, 128, 1, prob.kx, prob.ky, prob.kz)
-    t0 = time.perf_counter()
-    T = P.TimeSegmentation(G, prob.fm, prob.t, prob.M, prob.N, prob.L, interptype=2,
-                           shots=16)
-    setup = time.perf_counter() - t0
-    ms = _pair_ms(P, T, prob.x_true, stream, 5)
-    e = dict(workload="128^2, 16x2048, L=8 (one coil)", setup_ms=round(1e3 * setup, 1),
-             ms_per_pair=round(ms, 3))
     if with_cpu:
         O.set_threads(th)
         Go = O.Gnufft(prob.M, 2.0, 128, 128, 1, prob.kx, prob.ky, prob.kz)
+    for itype, key in ((2, "ts_minmax"), (3, "ts_histogram")):
         t0 = time.perf_counter()
-        To = O.TimeSegmentation(Go, prob.fm, prob.t, prob.M, prob.N, prob.L, interptype=2,
-                                shots=16)
-        e["cpu_setup_ms"] = round(1e3 * (time.perf_counter() - t0), 1)
-        e["cpu_ms_per_pair"] = round(_cpu_pair_ms(O, To, prob.x_true, th), 1)
-    out["ts_minmax"] = e
+        T = P.TimeSegmentation(G, prob.fm, prob.t, prob.M, prob.N, prob.L, interptype=itype,
+                               shots=16)
+        setup = time.perf_counter() - t0
+        Sg = P.SENSE(T, prob.sens, prob.M, prob.N, prob.nc)
+        ms = _pair_ms(P, Sg, prob.x_true, stream, 10)
+        T.set_mode(P.MODE_PLAIN)
+        Sg.set_mode(P.MODE_PLAIN)
+        ms_plain = _pair_ms(P, Sg, prob.x_true, stream, 2)
+        e = dict(workload="SENSE 128^2, 16x2048, 8 coils, L=8", setup_ms=round(1e3 * setup, 1),
+                 ms_per_pair=round(ms, 3), ms_per_pair_plain=round(ms_plain, 3))
+        if with_cpu:
+            t0 = time.perf_counter()
+            To = O.TimeSegmentation(Go, prob.fm, prob.t, prob.M, prob.N, prob.L,
+                                    interptype=itype, shots=16)
+            e["cpu_setup_ms"] = round(1e3 * (time.perf_counter() - t0), 1)
+            So = O.SENSE(To, prob.sens.ravel(), prob.M, prob.N, prob.nc)
+            e["cpu_ms_per_pair"] = round(_cpu_pair_ms(O, So, prob.x_true, th), 1)
+        out[key] = e
     O.set_threads(1)
     return out
 
@@ -480,7 +488,7 @@ def main():
         v["frac"] = v["GBs"] / HBM_PEAK_GBS
         log(f"[bench] {k:13s} {v['ms'] * 1e3:8.1f} us  {v['GBs']:8.1f} GB/s  ({100 * v['frac']:.1f}% of HBM)")
     dom = max(kinfo, key=lambda k: kinfo[k]["ms"])
-    kname = {"interp": "k_interp_lds<8>", "spread": "k_spread_patch"}.get(dom, dom)
+    kname = {"interp": "k_interp_lds<8, false>", "spread": "k_spread_patch"}.get(dom, dom)
     tr = pmc_traffic(kname)
     roof = dict(bound="hbm", achieved=round(kinfo[dom]["GBs"], 1), peak=HBM_PEAK_GBS, unit="GB/s",
                 frac=round(kinfo[dom]["frac"], 4), traffic=None if tr is None else round(tr),

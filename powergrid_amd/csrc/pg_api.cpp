@@ -742,8 +742,9 @@ struct TsOp : Op {
     Op* inner = nullptr;
     int L = 1;
     float tau = 0, tmin = 0;
-    float* AA = nullptr;    // L x M (Hanning)
-    float2* AAc = nullptr;  // L x M complex (min-max interpolator), replaces AA
+    float* AA = nullptr;    // L x M (Hanning; the real parts of AAc when that is set)
+    float2* AAc = nullptr;  // L x M complex (min-max interpolators)
+    float* AAi = nullptr;   // L x M, the imaginary parts of AAc, for the fused kernels
     float2* Wo = nullptr;   // L x N
     float2* WoH = nullptr;  // L x N
     float2* tmp_n = nullptr;
@@ -754,6 +755,7 @@ struct TsOp : Op {
     {
         dev_free(AA);
         dev_free(AAc);
+        dev_free(AAi);
         dev_free(Wo);
         dev_free(WoH);
         dev_free(tmp_n);
@@ -804,13 +806,7 @@ struct PcSenseOp : Op {
     // first data column of shot s and the stride between its tasks' columns
     long long col0(int s) const { return rank_share ? task0[s] : s; }
     long long col_stride() const { return rank_share ? 1 : Ns; }
-    bool batched() const
-    {
-        if (mode != PG_MODE_BATCHED) return false;
-        for (Op* a : A)
-            if (a && static_cast<TsOp*>(a)->AAc) return false;
-        return true;
-    }
+    bool batched() const { return mode == PG_MODE_BATCHED; }
     ~PcSenseOp() override
     {
         for (Op* o : A) delete o;
@@ -857,6 +853,7 @@ struct Fused {
     const float2* Wo = nullptr;
     const float2* WoH = nullptr;
     const float* AA = nullptr;
+    const float* AAi = nullptr;  // imaginary parts of complex weights, or null
 };
 
 static bool fusable(Op* op, Fused& f)
@@ -872,11 +869,11 @@ static bool fusable(Op* op, Fused& f)
     }
     if (in->kind == K_TS) {
         TsOp* t = static_cast<TsOp*>(in);
-        if (t->AAc) return false;  // complex interpolator weights: per segment
         f.L = t->L;
         f.Wo = t->Wo;
         f.WoH = t->WoH;
         f.AA = t->AA;
+        f.AAi = t->AAi;
         in = t->inner;
         if (in->mode != PG_MODE_BATCHED) return false;
     }
@@ -910,7 +907,8 @@ static void fused_forward(const Fused& f, const float2* x, float2* y, hipStream_
     }
     {
         Prof p("interp", s);
-        pgk::launch_interp(G, G.grids, f.nc, f.L, f.AA, y, y_cstride < 0 ? G.M : y_cstride, s);
+        pgk::launch_interp(G, G.grids, f.nc, f.L, f.AA, f.AAi, y, y_cstride < 0 ? G.M : y_cstride,
+                           s);
     }
 }
 
@@ -928,12 +926,13 @@ static void fused_adjoint_partial(const Fused& f, const float2* in, const float2
     const long long ics = in_cstride < 0 ? G.M : in_cstride;
     if (!vals_ready && pgk::spread_uses_values(G, B)) {
         Prof p("spread_values", s);
-        pgk::launch_spread_values(G, in, ics, f.nc, f.L, f.AA, resid_y, ax64, Wd, s);
+        pgk::launch_spread_values(G, in, ics, f.nc, f.L, f.AA, f.AAi, resid_y, ax64, Wd, s);
         vals_ready = true;
     }
     {
         Prof p("spread", s);
-        pgk::launch_spread(G, in, ics, f.nc, f.L, f.AA, resid_y, ax64, Wd, G.grids, vals_ready, s);
+        pgk::launch_spread(G, in, ics, f.nc, f.L, f.AA, f.AAi, resid_y, ax64, Wd, G.grids,
+                           vals_ready, s);
     }
     {
         Prof p("adj_fft_post", s);
@@ -1065,6 +1064,7 @@ static void op_forward(Op* op, const float2* in, float2* out, hipStream_t s)
                 f.Wo = ts->Wo;
                 f.WoH = ts->WoH;
                 f.AA = ts->AA;
+                f.AAi = ts->AAi;
                 fused_forward(f, in, out + pc->col0(sh) * Nd, s, pc->col_stride() * Nd);
             }
             return;
@@ -1177,6 +1177,7 @@ static void op_adjoint(Op* op, const float2* in, float2* out, hipStream_t s)
                 f.Wo = ts->Wo;
                 f.WoH = ts->WoH;
                 f.AA = ts->AA;
+                f.AAi = ts->AAi;
                 fused_adjoint_partial(f, in + pc->col0(sh) * Nd, nullptr, nullptr, nullptr,
                                       pc->P + (size_t)pc->task0[sh] * Ni, s, false,
                                       pc->col_stride() * Nd);
@@ -1327,8 +1328,9 @@ static void cg_iteration(Op* A, Penalty* R, const float2* y, const float* W, CgW
     // adjoint's per-sample spread values are formed (coalesced along grids)
     pgv::cg_update(w.Ax, w.Adir, y, W, M, w.x, w.x32, w.ddir, N, ps, w.cp, f ? w.r : nullptr,
                    w.sc, s);
-    if (f) pgk::launch_spread_values(*f->G, w.r, f->G->M, f->nc, f->L, f->AA, nullptr, nullptr,
-                                     nullptr, s);
+    if (f)
+        pgk::launch_spread_values(*f->G, w.r, f->G->M, f->nc, f->L, f->AA, f->AAi, nullptr,
+                                  nullptr, nullptr, s);
 }
 
 static bool same_spec(const pgv::PenSpec& a, const pgv::PenSpec& b)
@@ -1348,7 +1350,8 @@ static void cg_setup(Op* A, const float2* y, const float* W, CgWork& w, const Fu
     pgv::f2d(w.Ax, w.tmp, M, s);
     pgv::cg_init(y, W, M, w.part, w.sc, s);
     if (f)  // spread values of the first adjoint, from the initial Ax
-        pgk::launch_spread_values(*f->G, nullptr, f->G->M, f->nc, f->L, f->AA, y, w.Ax, W, s);
+        pgk::launch_spread_values(*f->G, nullptr, f->G->M, f->nc, f->L, f->AA, f->AAi, y, w.Ax, W,
+                                  s);
 }
 
 static void solve_pcg(Op* A, Penalty* R, const float2* x0, const float* W, const float2* y,
@@ -1730,6 +1733,18 @@ static TsOp* ts_build(Op* G, const float* fieldmap, const float* tvec, size_t M,
         op->AAc = minmax_ls_weights(fieldmap, tvec, M, N, Lc, shots, tau, tmin);
     if (interptype == 3 && Lc > 1)
         op->AAc = histogram_ls_weights(fieldmap, tvec, M, N, Lc, shots, tau, tmin);
+    std::vector<float> AAim;
+    if (op->AAc) {
+        // the fused kernels take complex weights as real and imaginary tables
+        std::vector<float2> c(AA.size());
+        PG_HIP(hipMemcpy(c.data(), op->AAc, c.size() * sizeof(float2), hipMemcpyDeviceToHost));
+        AAim.resize(AA.size());
+        for (size_t e = 0; e < c.size(); ++e) {
+            AA[e] = c[e].x;
+            AAim[e] = c[e].y;
+        }
+        op->AAi = dev_copy(AAim.data(), AAim.size());
+    }
     op->AA = dev_copy(AA.data(), AA.size());
     op->Wo = dev_copy(Wo.data(), Wo.size());
     op->WoH = dev_copy(WoH.data(), WoH.size());
@@ -2256,7 +2271,7 @@ int pg_gnufft_forward_spatial(pg_op h, const void* grid, void* samples, void* st
     hipStream_t s = (hipStream_t)stream;
     {
         Prof p("interp", s);
-        pgk::launch_interp(G, (const float2*)grid, 1, 1, nullptr, (float2*)samples, G.M, s);
+        pgk::launch_interp(G, (const float2*)grid, 1, 1, nullptr, nullptr, (float2*)samples, G.M, s);
     }
     PG_HIP(hipGetLastError());
     PG_CATCH
@@ -2270,8 +2285,8 @@ int pg_gnufft_adjoint_spatial(pg_op h, const void* samples, void* grid, void* st
     G.ensure_batch(1);
     {
         Prof p("spread", s);
-        pgk::launch_spread(G, (const float2*)samples, G.M, 1, 1, nullptr, nullptr, nullptr, nullptr,
-                           (float2*)grid, false, s);
+        pgk::launch_spread(G, (const float2*)samples, G.M, 1, 1, nullptr, nullptr, nullptr,
+                           nullptr, nullptr, (float2*)grid, false, s);
     }
     PG_HIP(hipGetLastError());
     PG_CATCH
@@ -2318,11 +2333,12 @@ int pg_density_compensation(const float* kx, const float* ky, const float* kz, s
     for (int it = 0; it < 2; ++it) {
         {
             Prof p("spread", s);
-            pgk::launch_spread(G, t.d, Mm, 1, 1, nullptr, nullptr, nullptr, nullptr, t.c1, false, s);
+            pgk::launch_spread(G, t.d, Mm, 1, 1, nullptr, nullptr, nullptr, nullptr, nullptr, t.c1,
+                               false, s);
         }
         {
             Prof p("interp", s);
-            pgk::launch_interp(G, t.c1, 1, 1, nullptr, t.c2, Mm, s);
+            pgk::launch_interp(G, t.c1, 1, 1, nullptr, nullptr, t.c2, Mm, s);
         }
         pgr::dcf_update(t.d, t.c2, Mm, s);
     }

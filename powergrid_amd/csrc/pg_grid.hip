@@ -41,7 +41,7 @@ __device__ __forceinline__ int clampi(int v, int lo, int hi) { return min(max(v,
 template <int NT, bool D3>
 __global__ void __launch_bounds__(256) k_interp(
     const int* __restrict__ lo, const float* __restrict__ w, int M, int Gx, int Gy, int Gz,
-    const float2* __restrict__ grids, long long G, int nc, int L, const float* __restrict__ AA,
+    const float2* __restrict__ grids, long long G, int nc, int L, const float* __restrict__ AA, const float* __restrict__ AAi,
     float2* __restrict__ out, long long out_cstride, int nsb, int xcd_map)
 {
     int c, sb;
@@ -117,8 +117,16 @@ __global__ void __launch_bounds__(256) k_interp(
         }
         if (AA) {
             const float a = AA[(long long)l * M + ii];
-            ar *= a;
-            ai *= a;
+            if (AAi) {
+                // complex weights, real and imaginary parts apart
+                const float b = AAi[(long long)l * M + ii];
+                const float nr = a * ar - b * ai;
+                ai = a * ai + b * ar;
+                ar = nr;
+            } else {
+                ar *= a;
+                ai *= a;
+            }
         }
         tr += ar;
         ti += ai;
@@ -155,17 +163,18 @@ __device__ __forceinline__ float group8_sum_to_7(float v)
 // Blocks are ordered coil-fastest so that coil c stays on XCD c % 8.
 constexpr int kInterpWaves = 8;
 
-template <int LP>
+template <int LP, bool CPX>
 __global__ void __launch_bounds__(64 * kInterpWaves) k_interp_lds(
     const int4* __restrict__ items, const float4* __restrict__ rec, int ity, int padl,
     const float2* __restrict__ grids, long long G, int Gx, int Gy, int nc, int L,
-    const float* __restrict__ AA, int M, float2* __restrict__ out, long long out_cstride)
+    const float* __restrict__ AA, const float* __restrict__ AAi, int M, float2* __restrict__ out, long long out_cstride)
 {
     constexpr int R = pg::kIR;
     constexpr int CH = pg::kIChunk;
     __shared__ float2 tile[R * LP * R];
     __shared__ float4 srec[CH * 5];
     __shared__ float saa[CH * LP];
+    __shared__ float saai[CPX ? CH * LP : 1];
     const int c = blockIdx.x % nc;
     const int4 it = items[blockIdx.x / nc];
     const int n = it.z - it.y;  // 1..CH
@@ -188,6 +197,7 @@ __global__ void __launch_bounds__(64 * kInterpWaves) k_interp_lds(
         const int i = __float_as_int(srec[sl * 5].x);
         const float a = AA[(long long)min(l, L - 1) * M + i];
         saa[e] = (l < L) ? a : 0.f;
+        if (CPX) saai[e] = (l < L) ? AAi[(long long)min(l, L - 1) * M + i] : 0.f;
     }
     __syncthreads();
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // scalar sample loop
@@ -221,8 +231,14 @@ __global__ void __launch_bounds__(64 * kInterpWaves) k_interp_lds(
                 ai += wx[x] * ri;
             }
             const float a = saa[sl * LP + l];
-            tr += a * ar;
-            ti += a * ai;
+            if (CPX) {
+                const float b = saai[sl * LP + l];
+                tr += a * ar - b * ai;
+                ti += a * ai + b * ar;
+            } else {
+                tr += a * ar;
+                ti += a * ai;
+            }
         }
         tr = group8_sum_to_7(tr);
         ti = group8_sum_to_7(ti);
@@ -237,7 +253,7 @@ __device__ __forceinline__ float2 spread_value(const float2* __restrict__ in, lo
                                                const float2* __restrict__ resid_y,
                                                const double2* __restrict__ ax64,
                                                const float* __restrict__ Wd,
-                                               const float* __restrict__ AA, int L, int M, int b,
+                                               const float* __restrict__ AA, const float* __restrict__ AAi, int L, int M, int b,
                                                int i)
 {
     const int c = b / L, l = b - c * L;
@@ -258,9 +274,15 @@ __device__ __forceinline__ float2 spread_value(const float2* __restrict__ in, lo
         v = in[di];
     }
     if (AA) {
-        // conj(AA) % d with AA real  (TimeSegmentation.cpp:304-310)
+        // conj(AA) % d  (TimeSegmentation.cpp:304-310); AAi, when set, holds
+        // the imaginary parts of complex weights
         const float a = AA[(long long)l * M + i];
-        v = make_float2(a * v.x, a * v.y);
+        if (AAi) {
+            const float b = AAi[(long long)l * M + i];
+            v = make_float2(a * v.x + b * v.y, a * v.y - b * v.x);
+        } else {
+            v = make_float2(a * v.x, a * v.y);
+        }
     }
     return v;
 }
@@ -272,12 +294,12 @@ __device__ __forceinline__ float2 spread_value(const float2* __restrict__ in, lo
 __global__ void __launch_bounds__(256) k_spread_values(
     const float2* __restrict__ in, long long in_cstride, const float2* __restrict__ resid_y,
     const double2* __restrict__ ax64, const float* __restrict__ Wd, int L,
-    const float* __restrict__ AA, int M, int B, float2* __restrict__ V)
+    const float* __restrict__ AA, const float* __restrict__ AAi, int M, int B, float2* __restrict__ V)
 {
     const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (i >= M) return;
     for (int b = threadIdx.x & 63; b < B; b += 64)
-        V[(long long)i * B + b] = spread_value(in, in_cstride, resid_y, ax64, Wd, AA, L, M, b, i);
+        V[(long long)i * B + b] = spread_value(in, in_cstride, resid_y, ax64, Wd, AA, AAi, L, M, b, i);
 }
 
 // the same for an even grid count, two grids per thread (16-byte stores;
@@ -285,29 +307,29 @@ __global__ void __launch_bounds__(256) k_spread_values(
 __global__ void __launch_bounds__(256) k_spread_values2(
     const float2* __restrict__ in, long long in_cstride, const float2* __restrict__ resid_y,
     const double2* __restrict__ ax64, const float* __restrict__ Wd, int L,
-    const float* __restrict__ AA, int M, int B, float2* __restrict__ V)
+    const float* __restrict__ AA, const float* __restrict__ AAi, int M, int B, float2* __restrict__ V)
 {
     const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const int hb = B >> 1;
     const long long i = p / hb;
     if (i >= M) return;
     const int b = 2 * (int)(p - i * hb);
-    const float2 v0 = spread_value(in, in_cstride, resid_y, ax64, Wd, AA, L, M, b, (int)i);
-    const float2 v1 = spread_value(in, in_cstride, resid_y, ax64, Wd, AA, L, M, b + 1, (int)i);
+    const float2 v0 = spread_value(in, in_cstride, resid_y, ax64, Wd, AA, AAi, L, M, b, (int)i);
+    const float2 v1 = spread_value(in, in_cstride, resid_y, ax64, Wd, AA, AAi, L, M, b + 1, (int)i);
     *reinterpret_cast<float4*>(V + i * B + b) = make_float4(v0.x, v0.y, v1.x, v1.y);
 }
 
 void launch_values(const float2* in, long long in_cstride, const float2* resid_y,
-                   const double2* ax64, const float* Wd, int L, const float* AA, int M, int B,
+                   const double2* ax64, const float* Wd, int L, const float* AA, const float* AAi, int M, int B,
                    float2* V, hipStream_t s)
 {
     if (B % 2 == 0) {
         const long long n = (long long)M * (B / 2);
         hipLaunchKernelGGL(k_spread_values2, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, in,
-                           in_cstride, resid_y, ax64, Wd, L, AA, M, B, V);
+                           in_cstride, resid_y, ax64, Wd, L, AA, AAi, M, B, V);
     } else {
         hipLaunchKernelGGL(k_spread_values, dim3((unsigned)((M + 3) / 4)), dim3(256), 0, s, in,
-                           in_cstride, resid_y, ax64, Wd, L, AA, M, B, V);
+                           in_cstride, resid_y, ax64, Wd, L, AA, AAi, M, B, V);
     }
 }
 
@@ -668,12 +690,12 @@ __global__ void __launch_bounds__(256) k_spread_atomic(
     const int* __restrict__ lo, const float* __restrict__ w, int M, int Gx, int Gy, int Gz,
     const float2* __restrict__ in, long long in_cstride, const float2* __restrict__ resid_y,
     const double2* __restrict__ ax64, const float* __restrict__ Wd, int L,
-    const float* __restrict__ AA, int B, float2* __restrict__ grids, long long G)
+    const float* __restrict__ AA, const float* __restrict__ AAi, int B, float2* __restrict__ grids, long long G)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int b = blockIdx.y;
     if (i >= M) return;
-    const float2 v = spread_value(in, in_cstride, resid_y, ax64, Wd, AA, L, M, b, i);
+    const float2 v = spread_value(in, in_cstride, resid_y, ax64, Wd, AA, AAi, L, M, b, i);
     const int lx = lo[i], ly = lo[M + i], lz = D3 ? lo[2 * M + i] : 0;
     float* g = reinterpret_cast<float*>(grids + (long long)b * G);
     for (int jz = 0; jz < (D3 ? NT : 1); ++jz) {
@@ -698,29 +720,31 @@ __global__ void __launch_bounds__(256) k_spread_atomic(
 }
 
 template <int NT, bool D3>
-void interp_dispatch(const pg::Gnufft& G, const float2* grids, int nc, int L, const float* AA,
+void interp_dispatch(const pg::Gnufft& G, const float2* grids, int nc, int L, const float* AA, const float* AAi,
                      float2* out, long long out_cstride, hipStream_t s)
 {
     const pg::Geometry& g = G.geo;
     if (!D3 && NT <= 8 && AA && L >= 2 && L <= 16 && G.n_iitems > 0) {
         const dim3 grid((unsigned)G.n_iitems * nc), block(64 * kInterpWaves);
+        auto launch = [&](auto kern) {
+            hipLaunchKernelGGL(kern, grid, block, 0, s, G.iitems, G.irec, G.ity, G.ipadl, grids,
+                               g.G(), g.Gx, g.Gy, nc, L, AA, AAi, G.M, out, out_cstride);
+        };
         if (L <= 8)
-            hipLaunchKernelGGL(k_interp_lds<8>, grid, block, 0, s, G.iitems, G.irec, G.ity, G.ipadl,
-                               grids, g.G(), g.Gx, g.Gy, nc, L, AA, G.M, out, out_cstride);
+            AAi ? launch(k_interp_lds<8, true>) : launch(k_interp_lds<8, false>);
         else
-            hipLaunchKernelGGL(k_interp_lds<16>, grid, block, 0, s, G.iitems, G.irec, G.ity,
-                               G.ipadl, grids, g.G(), g.Gx, g.Gy, nc, L, AA, G.M, out, out_cstride);
+            AAi ? launch(k_interp_lds<16, true>) : launch(k_interp_lds<16, false>);
         return;
     }
     const int nsb = (G.M + 63) / 64;  // 64 samples (4 lanes each) per block
     const int xmap = (nc % 8 == 0) ? 1 : 0;
     hipLaunchKernelGGL((k_interp<NT, D3>), dim3(nsb * nc), dim3(256), 0, s, G.lo, G.w, G.M, g.Gx,
-                       g.Gy, g.Gz, grids, g.G(), nc, L, AA, out, out_cstride, nsb, xmap);
+                       g.Gy, g.Gz, grids, g.G(), nc, L, AA, AAi, out, out_cstride, nsb, xmap);
 }
 
 template <int NT, bool D3>
 void spread_dispatch(const pg::Gnufft& G, const float2* in, long long in_cstride, int nc, int L,
-                     const float* AA, const float2* resid_y, const double2* ax64, const float* Wd,
+                     const float* AA, const float* AAi, const float2* resid_y, const double2* ax64, const float* Wd,
                      float2* grids, bool vals_ready, hipStream_t s)
 {
     const pg::Geometry& g = G.geo;
@@ -730,11 +754,11 @@ void spread_dispatch(const pg::Gnufft& G, const float2* in, long long in_cstride
         hipLaunchKernelGGL(k_zero, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, grids, n);
         hipLaunchKernelGGL((k_spread_atomic<NT, D3>), dim3((G.M + 255) / 256, B), dim3(256), 0, s,
                            G.lo, G.w, G.M, g.Gx, g.Gy, g.Gz, in, in_cstride, resid_y, ax64, Wd, L,
-                           AA, B, grids, g.G());
+                           AA, AAi, B, grids, g.G());
         return;
     }
     // per-sample values of every grid, once
-    if (!vals_ready) launch_values(in, in_cstride, resid_y, ax64, Wd, L, AA, G.M, B, G.vals, s);
+    if (!vals_ready) launch_values(in, in_cstride, resid_y, ax64, Wd, L, AA, AAi, G.M, B, G.vals, s);
     if (!D3) {
         const int nch = (B + 63) / 64;
         const int npy = g.Gy / pg::kPatch;
@@ -764,22 +788,22 @@ namespace pgk {
 
 std::atomic<int> g_spread_mfma{0};
 
-void launch_interp(const pg::Gnufft& G, const float2* grids, int nc, int L, const float* AA,
+void launch_interp(const pg::Gnufft& G, const float2* grids, int nc, int L, const float* AA, const float* AAi,
                    float2* out, long long out_cstride, hipStream_t s)
 {
     const bool d3 = G.geo.dims == 3;
     switch (G.geo.ntaps) {
     case 8:
-        d3 ? interp_dispatch<8, true>(G, grids, nc, L, AA, out, out_cstride, s)
-           : interp_dispatch<8, false>(G, grids, nc, L, AA, out, out_cstride, s);
+        d3 ? interp_dispatch<8, true>(G, grids, nc, L, AA, AAi, out, out_cstride, s)
+           : interp_dispatch<8, false>(G, grids, nc, L, AA, AAi, out, out_cstride, s);
         break;
     case 6:
-        d3 ? interp_dispatch<6, true>(G, grids, nc, L, AA, out, out_cstride, s)
-           : interp_dispatch<6, false>(G, grids, nc, L, AA, out, out_cstride, s);
+        d3 ? interp_dispatch<6, true>(G, grids, nc, L, AA, AAi, out, out_cstride, s)
+           : interp_dispatch<6, false>(G, grids, nc, L, AA, AAi, out, out_cstride, s);
         break;
     case 12:
-        d3 ? interp_dispatch<12, true>(G, grids, nc, L, AA, out, out_cstride, s)
-           : interp_dispatch<12, false>(G, grids, nc, L, AA, out, out_cstride, s);
+        d3 ? interp_dispatch<12, true>(G, grids, nc, L, AA, AAi, out, out_cstride, s)
+           : interp_dispatch<12, false>(G, grids, nc, L, AA, AAi, out, out_cstride, s);
         break;
     default:
         throw pg::Error{PG_ERR_UNSUPPORTED, "kernel width * gridos must give 6, 8 or 12 taps"};
@@ -789,29 +813,29 @@ void launch_interp(const pg::Gnufft& G, const float2* grids, int nc, int L, cons
 bool spread_uses_values(const pg::Gnufft& G, int B) { return !(G.geo.dims == 3 && B < 16); }
 
 void launch_spread_values(const pg::Gnufft& G, const float2* in, long long in_cstride, int nc,
-                          int L, const float* AA, const float2* resid_y, const double2* ax64,
+                          int L, const float* AA, const float* AAi, const float2* resid_y, const double2* ax64,
                           const float* Wd, hipStream_t s)
 {
-    launch_values(in, in_cstride, resid_y, ax64, Wd, L, AA, G.M, nc * L, G.vals, s);
+    launch_values(in, in_cstride, resid_y, ax64, Wd, L, AA, AAi, G.M, nc * L, G.vals, s);
 }
 
 void launch_spread(const pg::Gnufft& G, const float2* in, long long in_cstride, int nc, int L,
-                   const float* AA, const float2* resid_y, const double2* ax64, const float* Wd,
+                   const float* AA, const float* AAi, const float2* resid_y, const double2* ax64, const float* Wd,
                    float2* grids, bool vals_ready, hipStream_t s)
 {
     const bool d3 = G.geo.dims == 3;
     switch (G.geo.ntaps) {
     case 8:
-        d3 ? spread_dispatch<8, true>(G, in, in_cstride, nc, L, AA, resid_y, ax64, Wd, grids, vals_ready, s)
-           : spread_dispatch<8, false>(G, in, in_cstride, nc, L, AA, resid_y, ax64, Wd, grids, vals_ready, s);
+        d3 ? spread_dispatch<8, true>(G, in, in_cstride, nc, L, AA, AAi, resid_y, ax64, Wd, grids, vals_ready, s)
+           : spread_dispatch<8, false>(G, in, in_cstride, nc, L, AA, AAi, resid_y, ax64, Wd, grids, vals_ready, s);
         break;
     case 6:
-        d3 ? spread_dispatch<6, true>(G, in, in_cstride, nc, L, AA, resid_y, ax64, Wd, grids, vals_ready, s)
-           : spread_dispatch<6, false>(G, in, in_cstride, nc, L, AA, resid_y, ax64, Wd, grids, vals_ready, s);
+        d3 ? spread_dispatch<6, true>(G, in, in_cstride, nc, L, AA, AAi, resid_y, ax64, Wd, grids, vals_ready, s)
+           : spread_dispatch<6, false>(G, in, in_cstride, nc, L, AA, AAi, resid_y, ax64, Wd, grids, vals_ready, s);
         break;
     case 12:
-        d3 ? spread_dispatch<12, true>(G, in, in_cstride, nc, L, AA, resid_y, ax64, Wd, grids, vals_ready, s)
-           : spread_dispatch<12, false>(G, in, in_cstride, nc, L, AA, resid_y, ax64, Wd, grids, vals_ready, s);
+        d3 ? spread_dispatch<12, true>(G, in, in_cstride, nc, L, AA, AAi, resid_y, ax64, Wd, grids, vals_ready, s)
+           : spread_dispatch<12, false>(G, in, in_cstride, nc, L, AA, AAi, resid_y, ax64, Wd, grids, vals_ready, s);
         break;
     default:
         throw pg::Error{PG_ERR_UNSUPPORTED, "kernel width * gridos must give 6, 8 or 12 taps"};

@@ -120,20 +120,20 @@ void launch_fwd_prep_fft(const pg::Gnufft& G, const float2* img, long long img_c
                          WeightArgs w, int nc, int L, float2* tbuf, float2* tbuf2, float2* grids,
                          hipStream_t s);
 // forward: batched grids -> samples (KB interpolation, sum over segments)
-void launch_interp(const pg::Gnufft& G, const float2* grids, int nc, int L, const float* AA,
+void launch_interp(const pg::Gnufft& G, const float2* grids, int nc, int L, const float* AA, const float* AAi,
                    float2* out, long long out_cstride, hipStream_t s);
 // adjoint: samples -> batched grids (KB spread, deterministic tiles)
 // (with resid_y set, the samples are W (resid_y - ax64), ax64 the FP64 Ax of
 // the CG, and `in` is unused; with vals_ready the per-sample values
 // G.vals have already been formed, by the PCG update kernel)
 void launch_spread(const pg::Gnufft& G, const float2* in, long long in_cstride, int nc, int L,
-                   const float* AA, const float2* resid_y, const double2* ax64, const float* Wd,
+                   const float* AA, const float* AAi, const float2* resid_y, const double2* ax64, const float* Wd,
                    float2* grids, bool vals_ready, hipStream_t s);
 // whether launch_spread reads per-sample values (all but the 3-D atomic path)
 bool spread_uses_values(const pg::Gnufft& G, int B);
 // the per-sample values G.vals of a spread (see launch_spread)
 void launch_spread_values(const pg::Gnufft& G, const float2* in, long long in_cstride, int nc,
-                          int L, const float* AA, const float2* resid_y, const double2* ax64,
+                          int L, const float* AA, const float* AAi, const float2* resid_y, const double2* ax64,
                           const float* Wd, hipStream_t s);
 // adjoint: batched grids -> image(s) (ifftshift, IFFT, fftshift, crop,
 // deapod, x WoH, sum over segments, x conj(S))

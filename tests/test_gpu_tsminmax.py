@@ -50,17 +50,28 @@ def test_minmax_ts_matches_oracle(orc, N, nshot, ns, L, nc, shots, itype):
             To = orc.TimeSegmentation(Go, fm, t, M, N * N, L, interptype=itype, shots=shots)
             ops[prec] = (To, orc.SENSE(To, sens.ravel(), M, N * N, nc), Go)
         y64 = ops["d"][0] * x
-        e64 = rel(T * x, y64)
         d32 = rel(ops["f"][0] * x, y64)
-        e32 = rel(T * x, ops["f"][0] * x)
-        print(f"  fwd: gpu vs fp64 {e64:.2e}, vs fp32 {e32:.2e}; fp32 vs fp64 {d32:.2e}")
-        assert e64 < 5e-6
-        assert e32 < max(1e-4, 2 * d32)
         yo = y64.astype(np.complex64)
-        assert rel(T / yo, ops["d"][0] / y64) < 5e-6
+        xa64 = ops["d"][0] / y64
         ys = ops["d"][1] * x
-        assert rel(Sg * x, ys) < 5e-6
-        assert rel(Sg / ys.astype(np.complex64), ops["d"][1] / ys) < 5e-6
+        xs64 = ops["d"][1] / ys
+        # the fused batched kernels (complex weights as real and imaginary
+        # tables) and the per-segment path
+        outs = {}
+        for mode in (P.MODE_BATCHED, P.MODE_PLAIN):
+            T.set_mode(mode)
+            Sg.set_mode(mode)
+            e64 = rel(T * x, y64)
+            e32 = rel(T * x, ops["f"][0] * x)
+            print(f"  mode {mode} fwd: gpu vs fp64 {e64:.2e}, vs fp32 {e32:.2e}; "
+                  f"fp32 vs fp64 {d32:.2e}")
+            assert e64 < 5e-6
+            assert e32 < max(1e-4, 2 * d32)
+            assert rel(T / yo, xa64) < 5e-6
+            outs[mode] = Sg * x
+            assert rel(outs[mode], ys) < 5e-6
+            assert rel(Sg / ys.astype(np.complex64), xs64) < 5e-6
+        assert rel(outs[P.MODE_BATCHED], outs[P.MODE_PLAIN]) < 2e-6
     finally:
         orc.set_threads(1)
 
