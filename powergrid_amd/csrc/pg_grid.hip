@@ -181,23 +181,70 @@ __global__ void __launch_bounds__(64 * kInterpWaves) k_interp_lds(
     const int tx = it.x / ity, ty = it.x - (it.x / ity) * ity;
     const int rx0 = tx * pg::kIT - padl, ry0 = ty * pg::kIT - padl;
     const float2* gc = grids + (long long)c * L * G;
-    for (int e = threadIdx.x; e < R * LP * R; e += blockDim.x) {
-        const int yr = e % R, t = e / R;
-        const int l = t % LP, xr = t / LP;
-        const int x = rx0 + xr, y = ry0 + yr;
-        const bool ok = l < L && x >= 0 && x < Gx && y >= 0 && y < Gy;
-        const int xc = min(max(x, 0), Gx - 1), yc = min(max(y, 0), Gy - 1);
-        const float2 v = gc[(long long)min(l, L - 1) * G + (long long)xc * Gy + yc];
-        tile[e] = ok ? v : make_float2(0.f, 0.f);
+    // Staging: every global load of the block is issued before any LDS
+    // store, then one barrier.  The region goes as rows (xr, l) of R values
+    // along y, 32 lanes per row (R <= 32), row and segment by shifts, the
+    // grid offset in 32 bits (L*G of one coil); only points inside the grid
+    // are loaded.  The weights are read with the sample indices taken from
+    // the records in global memory.
+    static_assert(R <= 32 && (LP & (LP - 1)) == 0, "staging layout");
+    constexpr int NTH = 64 * kInterpWaves;
+    constexpr int PASS = NTH / 32;                  // rows per pass
+    constexpr int NP = (R * LP + PASS - 1) / PASS;  // passes over the rows
+    constexpr int NR = (CH * 5 + NTH - 1) / NTH;    // record loads per thread
+    constexpr int NA = (CH * LP + NTH - 1) / NTH;   // weight loads per thread
+    const int yr = threadIdx.x & 31;
+    const int y = ry0 + yr;
+    const bool yok = yr < R && y >= 0 && y < Gy;
+    // loads from clamped, in-range addresses with the condition applied as a
+    // select, so that none of them sits behind a branch and all are in flight
+    // together
+    const int yc = min(max(y, 0), Gy - 1);
+    float2 tv[NP];
+#pragma unroll
+    for (int k = 0; k < NP; ++k) {
+        const int row = (int)(threadIdx.x >> 5) + k * PASS;
+        const int l = row & (LP - 1), xr = row / LP;
+        const int x = rx0 + xr;
+        const bool ok = row < R * LP && yok && l < L && x >= 0 && x < Gx;
+        const int lc = min(l, L - 1), xc = min(max(x, 0), Gx - 1);
+        const float2 v = gc[lc * (int)G + xc * Gy + yc];
+        tv[k] = ok ? v : make_float2(0.f, 0.f);
     }
-    for (int e = threadIdx.x; e < n * 5; e += blockDim.x) srec[e] = rec[(long long)it.y * 5 + e];
-    __syncthreads();
-    for (int e = threadIdx.x; e < n * LP; e += blockDim.x) {
-        const int sl = e / LP, l = e % LP;
-        const int i = __float_as_int(srec[sl * 5].x);
-        const float a = AA[(long long)min(l, L - 1) * M + i];
-        saa[e] = (l < L) ? a : 0.f;
-        if (CPX) saai[e] = (l < L) ? AAi[(long long)min(l, L - 1) * M + i] : 0.f;
+    float4 rv[NR];
+#pragma unroll
+    for (int k = 0; k < NR; ++k) {
+        const int e = min((int)threadIdx.x + k * NTH, n * 5 - 1);
+        rv[k] = rec[(long long)it.y * 5 + e];
+    }
+    float av[NA], aiv[CPX ? NA : 1];
+#pragma unroll
+    for (int k = 0; k < NA; ++k) {
+        const int e = (int)threadIdx.x + k * NTH;
+        const int sl = min(e / LP, n - 1), l = e & (LP - 1);
+        const bool ok = e < n * LP && l < L;
+        const int i = __float_as_int(rec[((long long)it.y + sl) * 5].x);
+        const long long o = (long long)min(l, L - 1) * M + i;
+        av[k] = ok ? AA[o] : 0.f;
+        if (CPX) aiv[k] = ok ? AAi[o] : 0.f;
+    }
+#pragma unroll
+    for (int k = 0; k < NP; ++k) {
+        const int row = (int)(threadIdx.x >> 5) + k * PASS;
+        if (row < R * LP && yr < R) tile[row * R + yr] = tv[k];
+    }
+#pragma unroll
+    for (int k = 0; k < NR; ++k) {
+        const int e = (int)threadIdx.x + k * NTH;
+        if (e < n * 5) srec[e] = rv[k];
+    }
+#pragma unroll
+    for (int k = 0; k < NA; ++k) {
+        const int e = (int)threadIdx.x + k * NTH;
+        if (e < n * LP) {
+            saa[e] = av[k];
+            if (CPX) saai[e] = aiv[k];
+        }
     }
     __syncthreads();
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // scalar sample loop
@@ -724,7 +771,8 @@ void interp_dispatch(const pg::Gnufft& G, const float2* grids, int nc, int L, co
                      float2* out, long long out_cstride, hipStream_t s)
 {
     const pg::Geometry& g = G.geo;
-    if (!D3 && NT <= 8 && AA && L >= 2 && L <= 16 && G.n_iitems > 0) {
+    if (!D3 && NT <= 8 && AA && L >= 2 && L <= 16 && G.n_iitems > 0 &&
+        (long long)L * g.G() < (1LL << 31)) {
         const dim3 grid((unsigned)G.n_iitems * nc), block(64 * kInterpWaves);
         auto launch = [&](auto kern) {
             hipLaunchKernelGGL(kern, grid, block, 0, s, G.iitems, G.irec, G.ity, G.ipadl, grids,
