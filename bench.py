@@ -20,6 +20,7 @@ restatement of the reference CPU path on this host's cores.
 from __future__ import annotations
 
 import argparse
+import collections
 import json
 import os
 import sys
@@ -365,6 +366,73 @@ def rows_rates(P, S, stream, with_cpu=True):
     return out
 
 
+def serving_rates(P, S, n=96, rate=20.0, slots=4, seed=11):
+    """A serving run: n requests with Poisson arrivals at `rate` per second
+    into powergrid_amd.serve.ReconServer, each to a relative gradient
+    tolerance of 1e-4 or 30 PCG iterations.  The mix follows the serving
+    workload at sizes that finish in seconds: 70% 2-D 128^2 (spiral 16x2048,
+    L=8, 8/16/32 coils), 25% 3-D 64x64x16 (16 partitions x 4 x 1024 spiral,
+    8 coils, L=8) and 5% the same 3-D with L=16.  The k-space of each request
+    comes from the device forward operator plus 40 dB noise.  Latency runs
+    from submission to the image on the host; the request's operator set-up
+    is inside it."""
+    from powergrid_amd import serve as SV
+    rng = np.random.default_rng(seed)
+    kinds = rng.choice(3, size=n, p=[0.70, 0.25, 0.05])
+    coils2d = rng.choice([8, 16, 32], size=n)
+    reqs = []
+    cache = {}
+    for j in range(n):
+        if kinds[j] == 0:
+            key = ("2d", int(coils2d[j]))
+            N, Nz, nc, L, nsh, ns = 128, 1, int(coils2d[j]), 8, 16, 2048
+        else:
+            key = ("3d", 8 if kinds[j] == 1 else 16)
+            N, Nz, nc, L, nsh, ns = 64, 16, 8, key[1], 4, 1024
+        if key not in cache:
+            pr = S.make_problem(N, nsh, ns, nc, L, Nz=Nz, peak_hz=120.0, seed=seed + len(cache))
+            M = len(pr.kx)
+            G = P.Gnufft(M, 2.0, N, N, Nz, pr.kx, pr.ky, pr.kz)
+            A = P.SENSE(P.TimeSegmentation(G, pr.fm, pr.t, M, pr.N, L), pr.sens, M, pr.N, nc)
+            y = S.add_noise(A * pr.x_true, 40.0, seed=seed + 100 + len(cache))
+            # unit peak data: the solve is invariant to the data scale (x
+            # scales with y), and the reference's 1e25 bound on the line
+            # search denominator then never ends a solve early
+            y = (y / np.abs(y).max()).astype(np.complex64)
+            cache[key] = (pr, y)
+        pr, y = cache[key]
+        reqs.append(SV.ReconRequest(pr.kx, pr.ky, pr.kz, pr.t, pr.Nx, pr.Ny, pr.Nz, pr.nc, pr.L,
+                                    pr.fm, pr.sens, y, beta=1e4, niter=30, tol=1e-4,
+                                    tag=(j, key)))
+    gaps = rng.exponential(1.0 / rate, size=n)
+    t_start = time.perf_counter()
+    with SV.ReconServer(slots=slots) as srv:
+        futs = []
+        t_next = t_start
+        for r, g in zip(reqs, gaps):
+            t_next += g
+            dt = t_next - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+            futs.append(srv.submit(r))
+        outs = [f.result() for f in futs]
+    t_end = max(o.finished for o in outs)
+    lat = np.array([o.latency for o in outs]) * 1e3
+    its = np.array([o.iterations for o in outs])
+    per = {}
+    for o in outs:
+        per.setdefault(str(o.tag[1]), []).append(o.latency * 1e3)
+    return dict(workload=f"{n} requests, Poisson {rate:g}/s, 70% 2-D 128^2 8-32 coils, "
+                         f"25% 3-D 64x64x16 L=8, 5% L=16; to tol 1e-4 or 30 iterations",
+                slots=slots, recons_per_s=round(n / (t_end - t_start), 2),
+                latency_ms_p50=round(float(np.percentile(lat, 50)), 2),
+                latency_ms_p99=round(float(np.percentile(lat, 99)), 2),
+                mean_iterations=round(float(its.mean()), 1),
+                exit_reasons=dict(collections.Counter(o.exit_reason for o in outs)),
+                max_concurrent=srv.max_running,
+                latency_ms_p50_by_kind={k: round(float(np.median(v)), 2) for k, v in per.items()})
+
+
 def pmc_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed FETCH_SIZE/WRITE_SIZE
     summary (profiles/sense2d_pmc_traffic.md, counters scaled by a device copy
@@ -401,6 +469,8 @@ def main():
     ap.add_argument("--no-gdft", action="store_true", help="skip the Gdft roofline")
     ap.add_argument("--no-direct", action="store_true",
                     help="skip the density-compensation / gridded-coil-image timings")
+    ap.add_argument("--no-serve", action="store_true",
+                    help="skip the serving-loop measurement")
     ap.add_argument("--no-rows", action="store_true",
                     help="skip the pcSENSE / GdftR2 / min-max interpolator timings")
     ap.add_argument("--counters", type=int, default=0, metavar="REPS",
@@ -542,6 +612,9 @@ def main():
     if not args.no_direct:
         out["direct_recon"] = direct_recon_rates(P, prob, stream)
         log(f"[bench] direct recon {out['direct_recon']}")
+    if not args.no_serve:
+        out["serving"] = serving_rates(P, S)
+        log(f"[bench] serving {out['serving']}")
     if not args.no_rows:
         out["rows"] = rows_rates(P, S, stream, with_cpu=not args.no_cpu)
         log(f"[bench] rows {out['rows']}")

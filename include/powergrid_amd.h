@@ -56,6 +56,9 @@ const char* pg_version(void);
 int pg_set_device(int device);
 int pg_malloc(void** ptr, size_t bytes);
 int pg_free(void* ptr);
+/* free and total device memory in bytes (hipMemGetInfo), for callers that
+ * admit work by free HBM (no counterpart in the reference) */
+int pg_mem_info(size_t* free_bytes, size_t* total_bytes);
 /* blocking copies; with stream NULL they wait for all work on the device
  * (the library's streams are non-blocking), pg_memcpy_dtod only enqueues */
 int pg_memcpy_htod(void* dst_dev, const void* src_host, size_t bytes, void* stream);
@@ -268,6 +271,7 @@ int pg_penalty_value(pg_penalty R, const void* x_dev, double* out, void* stream)
 #define PG_EXIT_ZERO_GRAD 1   /* solve_pwls_pcg.hpp:88-91 */
 #define PG_EXIT_NOT_DESCENT 2 /* :113-116 */
 #define PG_EXIT_BAD_DENOM 3   /* :137-145 */
+#define PG_EXIT_TOL 4         /* the tolerance of pg_solve_pwls_pcg_tol */
 typedef struct {
     int iterations;   /* completed updates */
     int exit_reason;  /* PG_EXIT_* */
@@ -279,6 +283,18 @@ int pg_solve_pwls_pcg(pg_op A, pg_penalty R, const void* x0_dev, const float* W_
 /* replaces reconSolve (reconSolve.h:63-82): W = ones, x0 = 0 */
 int pg_recon_solve(pg_op A, pg_penalty R, const void* y_dev, size_t niter, int mode,
                    void* x_out_dev, double* resid, pg_cg_info* info, void* stream);
+/* (no counterpart in the reference, which always runs niter iterations) the
+ * same two with a stopping tolerance, for callers that run a reconstruction
+ * "to a relative residual or niter iterations": the loop also ends, with
+ * PG_EXIT_TOL, before the update of the first iteration k >= 1 whose
+ * gradient ngrad = A' W (y - A x) - R'(x) (solve_pwls_pcg.hpp:80-86)
+ * satisfies ||ngrad_k|| <= tol ||ngrad_0||.  tol <= 0 gives the plain
+ * functions' results. */
+int pg_solve_pwls_pcg_tol(pg_op A, pg_penalty R, const void* x0_dev, const float* W_dev,
+                          const void* y_dev, size_t niter, double tol, int mode, void* x_out_dev,
+                          double* resid, pg_cg_info* info, void* stream);
+int pg_recon_solve_tol(pg_op A, pg_penalty R, const void* y_dev, size_t niter, double tol,
+                       int mode, void* x_out_dev, double* resid, pg_cg_info* info, void* stream);
 
 /* ---- tuning (no counterpart in the reference) ----------------------------
  * "spread_mfma" 1: the 2-D batched spread runs on the matrix cores when the

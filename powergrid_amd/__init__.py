@@ -23,7 +23,8 @@ MODE_BATCHED = 0
 MODE_PLAIN = 1
 CG_GRAPH = 0
 CG_HOST = 1
-EXIT_REASONS = {0: "niter", 1: "zero_gradient", 2: "not_descent", 3: "bad_denominator"}
+EXIT_REASONS = {0: "niter", 1: "zero_gradient", 2: "not_descent", 3: "bad_denominator",
+                4: "tolerance"}
 
 
 def _f32(a, n=None):
@@ -467,20 +468,24 @@ class TVPenalty(_Penalty):
         super().__init__(1, Nx, Ny, Nz, beta, delta, dims2penalize)
 
 
-def solve_pwls_pcg(x0, A, W, yi, R, niter, mode=CG_GRAPH, stream=None):
+def solve_pwls_pcg(x0, A, W, yi, R, niter, mode=CG_GRAPH, stream=None, tol=0.0):
     """solve_pwls_pcg(xInitial, A, W, yi, R, niter) (solve_pwls_pcg.hpp:48-167),
     run on the device.  Arguments are numpy arrays or DeviceArrays; the
-    result has the type of yi.  Returns (x, info dict)."""
+    result has the type of yi.  Returns (x, info dict).  tol > 0 (no
+    counterpart in the reference) also stops before the update of the first
+    iteration k >= 1 whose gradient norm is <= tol times the first one's
+    (exit_reason "tolerance")."""
     y = _as_device(yi, np.complex64)
     x0d = None if x0 is None else _as_device(x0, np.complex64)
     Wd = None if W is None else _as_device(W, np.float32)
     out = DeviceArray(A.n_in, np.complex64)
     resid = (C.c_double * max(int(niter), 1))()
     info = CgInfo()
-    check(lib().pg_solve_pwls_pcg(A.handle, R.handle if R is not None else None,
-                                  None if x0d is None else x0d.ptr,
-                                  None if Wd is None else Wd.ptr, y.ptr, int(niter), int(mode),
-                                  out.ptr, resid, C.byref(info), _sh(stream)))
+    check(lib().pg_solve_pwls_pcg_tol(A.handle, R.handle if R is not None else None,
+                                      None if x0d is None else x0d.ptr,
+                                      None if Wd is None else Wd.ptr, y.ptr, int(niter),
+                                      float(tol), int(mode), out.ptr, resid, C.byref(info),
+                                      _sh(stream)))
     res = dict(iterations=info.iterations, exit_reason=EXIT_REASONS.get(info.exit_reason, "?"),
                resid=np.array(resid[:info.iterations]))
     if not isinstance(yi, DeviceArray):
@@ -489,10 +494,10 @@ def solve_pwls_pcg(x0, A, W, yi, R, niter, mode=CG_GRAPH, stream=None):
 
 
 def reconSolve(data, Sg, R, kx=None, ky=None, kz=None, Nx=None, Ny=None, Nz=None, tvec=None,
-               niter=10, mode=CG_GRAPH, stream=None):
+               niter=10, mode=CG_GRAPH, stream=None, tol=0.0):
     """reconSolve(data, Sg, R, kx, ky, kz, Nx, Ny, Nz, tvec, niter)
-    (reconSolve.h:63-82): W = ones, x0 = 0."""
-    return solve_pwls_pcg(None, Sg, None, data, R, niter, mode, stream)
+    (reconSolve.h:63-82): W = ones, x0 = 0; tol as solve_pwls_pcg."""
+    return solve_pwls_pcg(None, Sg, None, data, R, niter, mode, stream, tol)
 
 
 class profile:

@@ -33,6 +33,7 @@ _SIGS = {
     "pg_set_device": (i32, [i32]),
     "pg_malloc": (i32, [C.POINTER(vp), sz]),
     "pg_free": (i32, [vp]),
+    "pg_mem_info": (i32, [C.POINTER(sz), C.POINTER(sz)]),
     "pg_memcpy_htod": (i32, [vp, vp, sz, vp]),
     "pg_memcpy_dtoh": (i32, [vp, vp, sz, vp]),
     "pg_memcpy_dtod": (i32, [vp, vp, sz, vp]),
@@ -74,6 +75,9 @@ _SIGS = {
     "pg_penalty_value": (i32, [vp, vp, dp, vp]),
     "pg_solve_pwls_pcg": (i32, [vp, vp, vp, vp, vp, sz, i32, vp, dp, C.POINTER(CgInfo), vp]),
     "pg_recon_solve": (i32, [vp, vp, vp, sz, i32, vp, dp, C.POINTER(CgInfo), vp]),
+    "pg_solve_pwls_pcg_tol": (i32, [vp, vp, vp, vp, vp, sz, C.c_double, i32, vp, dp,
+                                    C.POINTER(CgInfo), vp]),
+    "pg_recon_solve_tol": (i32, [vp, vp, vp, sz, C.c_double, i32, vp, dp, C.POINTER(CgInfo), vp]),
     "pg_set_tuning": (i32, [C.c_char_p, i32]),
     "pg_profile_enable": (i32, [i32]),
     "pg_profile_reset": (i32, []),

@@ -38,6 +38,15 @@ static bool g_prof = false;
 static std::vector<ProfRec> g_prof_recs;
 static std::mutex g_prof_mu;
 
+// HIP refuses calls that use the legacy stream or synchronise the device
+// (hipMemcpy, hipFree, null-stream launches, ...) from any thread while a
+// stream capture is open.  Graph capture and every such call of the library
+// hold this lock, so requests of different threads may build, solve and
+// free their operators concurrently; the device work on their own streams
+// (graph launches, asynchronous copies) does not take it.
+static std::recursive_mutex g_legacy_mu;
+using LegacyLock = std::lock_guard<std::recursive_mutex>;
+
 struct Prof {
     bool on = false;
     hipStream_t s;
@@ -90,6 +99,7 @@ FftPlan1D make_fft_plan(int n)
         const double a = -2.0 * M_PI * (double)k / (double)n;
         tw[k] = make_float2((float)std::cos(a), (float)std::sin(a));
     }
+    LegacyLock lk(g_legacy_mu);
     PG_HIP(hipMalloc(&p.tw, sizeof(float2) * n));
     PG_HIP(hipMemcpy(p.tw, tw.data(), sizeof(float2) * n, hipMemcpyHostToDevice));
     return p;
@@ -97,6 +107,7 @@ FftPlan1D make_fft_plan(int n)
 
 void free_fft_plan(FftPlan1D& p)
 {
+    LegacyLock lk(g_legacy_mu);
     if (p.tw) hipFree(p.tw);
     p.tw = nullptr;
 }
@@ -106,6 +117,7 @@ static T* dev_alloc(size_t n)
 {
     if (n == 0) n = 1;
     void* p = nullptr;
+    LegacyLock lk(g_legacy_mu);
     hipError_t e = hipMalloc(&p, sizeof(T) * n);
     if (e != hipSuccess)
         throw Error{PG_ERR_OOM, "hipMalloc of " + std::to_string(sizeof(T) * n) + " bytes failed"};
@@ -115,6 +127,7 @@ static T* dev_alloc(size_t n)
 template <class T>
 static T* dev_copy(const T* h, size_t n)
 {
+    LegacyLock lk(g_legacy_mu);
     T* d = dev_alloc<T>(n);
     if (h && n) PG_HIP(hipMemcpy(d, h, sizeof(T) * n, hipMemcpyHostToDevice));
     return d;
@@ -123,7 +136,10 @@ static T* dev_copy(const T* h, size_t n)
 template <class T>
 static void dev_free(T*& p)
 {
-    if (p) hipFree((void*)p);
+    if (p) {
+        LegacyLock lk(g_legacy_mu);
+        hipFree((void*)p);
+    }
     p = nullptr;
 }
 
@@ -156,13 +172,15 @@ static double kb_ft(double W, double beta, double u)
 }
 
 // bumped whenever an operator workspace is (re)allocated: a cached solver
-// graph that captured workspace addresses is rebuilt after that
-static uint64_t g_ws_gen = 1;
+// graph that captured workspace addresses is rebuilt after that.  Atomic:
+// operators of different threads grow their workspaces concurrently, and a
+// lost or reordered update could leave a stale graph in place.
+static std::atomic<uint64_t> g_ws_gen{1};
 
 void Gnufft::ensure_batch(int B)
 {
     if (B <= cap_batch) return;
-    ++g_ws_gen;
+    g_ws_gen.fetch_add(1);
     dev_free(grids);
     dev_free(tbuf);
     dev_free(tbuf2);
@@ -715,7 +733,7 @@ struct DftOp : Op {
     void ensure_part(size_t n)
     {
         if (n <= part_cap) return;
-        ++g_ws_gen;
+        g_ws_gen.fetch_add(1);
         dev_free(part);
         part = dev_alloc<float2>(n);
         part_cap = n;
@@ -1281,6 +1299,7 @@ struct SolveCache {
     const float2* y = nullptr;
     const float* W = nullptr;
     size_t niter = 0;
+    double tol = 0.0;
     uint64_t ws_gen = 0;
     CgWork w;
     hipStream_t own = nullptr;  // stream for graph solves without a user stream
@@ -1288,12 +1307,14 @@ struct SolveCache {
     hipGraphExec_t exec = nullptr;
     ~SolveCache()
     {
+        LegacyLock lk(g_legacy_mu);
         if (exec) hipGraphExecDestroy(exec);
         if (graph) hipGraphDestroy(graph);
         if (own) hipStreamDestroy(own);
     }
     void drop_graph()
     {
+        LegacyLock lk(g_legacy_mu);
         if (exec) hipGraphExecDestroy(exec);
         if (graph) hipGraphDestroy(graph);
         exec = nullptr;
@@ -1342,21 +1363,21 @@ static bool same_spec(const pgv::PenSpec& a, const pgv::PenSpec& b)
 // x = x0 (or 0), Ax = A x (:55), the CG scalars, and the first adjoint's
 // spread values
 static void cg_setup(Op* A, const float2* y, const float* W, CgWork& w, const Fused* f,
-                     hipStream_t s)
+                     double tol, hipStream_t s)
 {
     const long long N = (long long)A->n_in, M = (long long)A->n_out;
     pgv::f2d(w.x, w.x32, N, s);
     op_forward(A, w.x32, w.tmp, s);
     pgv::f2d(w.Ax, w.tmp, M, s);
-    pgv::cg_init(y, W, M, w.part, w.sc, s);
+    pgv::cg_init(y, W, M, w.part, w.sc, tol, s);
     if (f)  // spread values of the first adjoint, from the initial Ax
         pgk::launch_spread_values(*f->G, nullptr, f->G->M, f->nc, f->L, f->AA, f->AAi, y, w.Ax, W,
                                   s);
 }
 
 static void solve_pcg(Op* A, Penalty* R, const float2* x0, const float* W, const float2* y,
-                      size_t niter, int mode, float2* x_out, double* resid_h, pg_cg_info* info,
-                      hipStream_t user_stream)
+                      size_t niter, double tol, int mode, float2* x_out, double* resid_h,
+                      pg_cg_info* info, hipStream_t user_stream)
 {
     const long long N = (long long)A->n_in, M = (long long)A->n_out;
     Fused fz;
@@ -1366,8 +1387,9 @@ static void solve_pcg(Op* A, Penalty* R, const float2* x0, const float* W, const
     pgv::PenSpec spec{};
     if (R) spec = R->spec;
     SolveCache* c = A->scache;
+    if (!(tol > 0.0)) tol = 0.0;
     const bool same = c && c->R == R && (!R || same_spec(c->spec, spec)) && c->y == y && c->W == W &&
-                      c->niter == niter;
+                      c->niter == niter && c->tol == tol;
     if (!same) {
         delete c;
         A->scache = nullptr;
@@ -1378,6 +1400,7 @@ static void solve_pcg(Op* A, Penalty* R, const float2* x0, const float* W, const
         c->y = y;
         c->W = W;
         c->niter = niter;
+        c->tol = tol;
         CgWork& w = c->w;
         w.x = dev_alloc<double2>(N);
         w.Ax = dev_alloc<double2>(M);
@@ -1394,11 +1417,13 @@ static void solve_pcg(Op* A, Penalty* R, const float2* x0, const float* W, const
         w.sc = dev_alloc<CgScal>(1);
     }
     CgWork& w = c->w;
-    if (c->ws_gen != g_ws_gen) c->drop_graph();
-    c->ws_gen = g_ws_gen;
+    const uint64_t gen = g_ws_gen.load();
+    if (c->ws_gen != gen) c->drop_graph();
+    c->ws_gen = gen;
 
     hipStream_t s = user_stream;
     if (mode == PG_CG_GRAPH && s == nullptr) {
+        LegacyLock lk(g_legacy_mu);
         PG_HIP(hipStreamSynchronize(nullptr));
         if (!c->own) PG_HIP(hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking));
         s = c->own;
@@ -1413,9 +1438,10 @@ static void solve_pcg(Op* A, Penalty* R, const float2* x0, const float* W, const
         // every kernel checks the device-side stop flag, so an early exit
         // leaves the remaining iterations as no-ops
         if (!c->exec) {
+            LegacyLock lk(g_legacy_mu);
             PG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
             try {
-                cg_setup(A, y, W, w, fused ? &fz : nullptr, s);
+                cg_setup(A, y, W, w, fused ? &fz : nullptr, tol, s);
                 for (size_t it = 0; it < niter; ++it)
                     cg_iteration(A, R, y, W, w, s, fused ? &fz : nullptr);
                 pgv::cg_finish(w.cp, w.sc, w.resid, s);
@@ -1432,7 +1458,7 @@ static void solve_pcg(Op* A, Penalty* R, const float2* x0, const float* W, const
         PG_HIP(hipMemcpyAsync(&hs, w.sc, sizeof(CgScal), hipMemcpyDeviceToHost, s));
         PG_HIP(hipStreamSynchronize(s));
     } else {
-        cg_setup(A, y, W, w, fused ? &fz : nullptr, s);
+        cg_setup(A, y, W, w, fused ? &fz : nullptr, tol, s);
         for (size_t it = 0; it < niter; ++it) {
             cg_iteration(A, R, y, W, w, s, fused ? &fz : nullptr);
             // host-side branch on the device scalars, as the reference loop does
@@ -1529,6 +1555,7 @@ static float2* minmax_ls_weights(const float* fieldmap, const float* tvec, size_
     d.fm = dev_copy(fm.data(), N);
     d.tv = dev_copy(tv.data(), none);
     d.W = dev_copy(W.data(), W.size());
+    LegacyLock lk(g_legacy_mu);  // null-stream launch and synchronisation
     float2* AA = dev_alloc<float2>((size_t)L * M);
     pgt::minmax_weights(d.fm, d.tv, d.W, (int)N, (int)none, L, (int)shots, AA, nullptr);
     const hipError_t e1 = hipGetLastError();
@@ -1664,6 +1691,7 @@ static float2* histogram_ls_weights(const float* fieldmap, const float* tvec, si
     d.z = dev_copy(z.data(), nb);
     d.tv = dev_copy(tv.data(), none);
     d.iG = dev_copy(iG.data(), iG.size());
+    LegacyLock lk(g_legacy_mu);  // null-stream launch and synchronisation
     float2* AA = dev_alloc<float2>((size_t)L * M);
     pgt::histogram_weights(d.h, d.om, d.z, nb, d.tv, (int)none, d.iG, L, (int)shots, AA, nullptr);
     const hipError_t e1 = hipGetLastError();
@@ -1735,6 +1763,7 @@ static TsOp* ts_build(Op* G, const float* fieldmap, const float* tvec, size_t M,
         op->AAc = histogram_ls_weights(fieldmap, tvec, M, N, Lc, shots, tau, tmin);
     std::vector<float> AAim;
     if (op->AAc) {
+        LegacyLock lk(g_legacy_mu);
         // the fused kernels take complex weights as real and imaginary tables
         std::vector<float2> c(AA.size());
         PG_HIP(hipMemcpy(c.data(), op->AAc, c.size() * sizeof(float2), hipMemcpyDeviceToHost));
@@ -1804,16 +1833,25 @@ int pg_set_device(int device)
     PG_TRY PG_HIP(hipSetDevice(device));
     PG_CATCH
 }
+int pg_mem_info(size_t* free_bytes, size_t* total_bytes)
+{
+    PG_TRY PG_REQUIRE(free_bytes && total_bytes, PG_ERR_ARG, "null argument");
+    PG_HIP(hipMemGetInfo(free_bytes, total_bytes));
+    PG_CATCH
+}
+
 int pg_malloc(void** ptr, size_t bytes)
 {
     PG_TRY PG_REQUIRE(ptr, PG_ERR_ARG, "null ptr");
     *ptr = nullptr;
+    LegacyLock lk(g_legacy_mu);
     if (hipMalloc(ptr, bytes ? bytes : 1) != hipSuccess) throw Error{PG_ERR_OOM, "hipMalloc failed"};
     PG_CATCH
 }
 int pg_free(void* ptr)
 {
-    PG_TRY if (ptr) PG_HIP(hipFree(ptr));
+    PG_TRY LegacyLock lk(g_legacy_mu);
+    if (ptr) PG_HIP(hipFree(ptr));
     PG_CATCH
 }
 // With stream NULL the copies are ordered after all work on the device: the
@@ -1821,21 +1859,30 @@ int pg_free(void* ptr)
 // wait for them.
 int pg_memcpy_htod(void* d, const void* h, size_t n, void* s)
 {
-    PG_TRY if (!s) PG_HIP(hipDeviceSynchronize());
+    PG_TRY if (!s) {
+        LegacyLock lk(g_legacy_mu);
+        PG_HIP(hipDeviceSynchronize());
+    }
     PG_HIP(hipMemcpyAsync(d, h, n, hipMemcpyHostToDevice, (hipStream_t)s));
     PG_HIP(hipStreamSynchronize((hipStream_t)s));
     PG_CATCH
 }
 int pg_memcpy_dtoh(void* h, const void* d, size_t n, void* s)
 {
-    PG_TRY if (!s) PG_HIP(hipDeviceSynchronize());
+    PG_TRY if (!s) {
+        LegacyLock lk(g_legacy_mu);
+        PG_HIP(hipDeviceSynchronize());
+    }
     PG_HIP(hipMemcpyAsync(h, d, n, hipMemcpyDeviceToHost, (hipStream_t)s));
     PG_HIP(hipStreamSynchronize((hipStream_t)s));
     PG_CATCH
 }
 int pg_memcpy_dtod(void* d, const void* src, size_t n, void* s)
 {
-    PG_TRY if (!s) PG_HIP(hipDeviceSynchronize());
+    PG_TRY if (!s) {
+        LegacyLock lk(g_legacy_mu);
+        PG_HIP(hipDeviceSynchronize());
+    }
     PG_HIP(hipMemcpyAsync(d, src, n, hipMemcpyDeviceToDevice, (hipStream_t)s));
     PG_CATCH
 }
@@ -1843,13 +1890,15 @@ int pg_stream_create(void** s)
 {
     PG_TRY PG_REQUIRE(s, PG_ERR_ARG, "null ptr");
     hipStream_t st;
+    LegacyLock lk(g_legacy_mu);
     PG_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
     *s = (void*)st;
     PG_CATCH
 }
 int pg_stream_destroy(void* s)
 {
-    PG_TRY if (s) PG_HIP(hipStreamDestroy((hipStream_t)s));
+    PG_TRY LegacyLock lk(g_legacy_mu);
+    if (s) PG_HIP(hipStreamDestroy((hipStream_t)s));
     PG_CATCH
 }
 int pg_stream_synchronize(void* s)
@@ -2222,7 +2271,8 @@ int pg_op_adjoint(pg_op h, const void* in, void* out, void* stream)
 
 static int host_apply(pg_op h, const void* in, void* out, bool fwd)
 {
-    PG_TRY Op* op = as_op(h);
+    PG_TRY LegacyLock lk(g_legacy_mu);
+    Op* op = as_op(h);
     PG_REQUIRE(in && out, PG_ERR_ARG, "bad buffers");
     const size_t ni = fwd ? op->n_in : op->n_out, no = fwd ? op->n_out : op->n_in;
     float2* di = dev_copy((const float2*)in, ni);
@@ -2360,7 +2410,7 @@ int pg_grid_coil_images(pg_op h, const float* dcf, const void* data, size_t ncoi
     const long long M = G.M, N = G.geo.N();
     const size_t need = ncoils * (size_t)M;
     if (need > no->wtmp_cap) {
-        ++g_ws_gen;
+        g_ws_gen.fetch_add(1);
         dev_free(no->wtmp);
         no->wtmp = dev_alloc<float2>(need);
         no->wtmp_cap = need;
@@ -2533,24 +2583,39 @@ int pg_penalty_value(pg_penalty h, const void* x, double* out, void* stream)
     PG_CATCH
 }
 
-int pg_solve_pwls_pcg(pg_op Ah, pg_penalty Rh, const void* x0, const float* W, const void* y,
-                      size_t niter, int mode, void* x_out, double* resid, pg_cg_info* info,
-                      void* stream)
+int pg_solve_pwls_pcg_tol(pg_op Ah, pg_penalty Rh, const void* x0, const float* W, const void* y,
+                          size_t niter, double tol, int mode, void* x_out, double* resid,
+                          pg_cg_info* info, void* stream)
 {
     PG_TRY Op* A = as_op(Ah);
     Penalty* R = Rh ? as_pen(Rh) : nullptr;
     PG_REQUIRE(y && x_out, PG_ERR_ARG, "null y or x_out");
     PG_REQUIRE(mode == PG_CG_GRAPH || mode == PG_CG_HOST, PG_ERR_ARG, "bad CG mode");
     if (R) PG_REQUIRE((size_t)R->spec.n == A->n_in, PG_ERR_ARG, "penalty size != image size");
-    solve_pcg(A, R, (const float2*)x0, W, (const float2*)y, niter, mode, (float2*)x_out, resid, info,
-              (hipStream_t)stream);
+    solve_pcg(A, R, (const float2*)x0, W, (const float2*)y, niter, tol, mode, (float2*)x_out, resid,
+              info, (hipStream_t)stream);
     PG_CATCH
+}
+
+int pg_solve_pwls_pcg(pg_op A, pg_penalty R, const void* x0, const float* W, const void* y,
+                      size_t niter, int mode, void* x_out, double* resid, pg_cg_info* info,
+                      void* stream)
+{
+    return pg_solve_pwls_pcg_tol(A, R, x0, W, y, niter, 0.0, mode, x_out, resid, info, stream);
 }
 
 int pg_recon_solve(pg_op A, pg_penalty R, const void* y, size_t niter, int mode, void* x_out,
                    double* resid, pg_cg_info* info, void* stream)
 {
-    return pg_solve_pwls_pcg(A, R, nullptr, nullptr, y, niter, mode, x_out, resid, info, stream);
+    return pg_solve_pwls_pcg_tol(A, R, nullptr, nullptr, y, niter, 0.0, mode, x_out, resid, info,
+                                 stream);
+}
+
+int pg_recon_solve_tol(pg_op A, pg_penalty R, const void* y, size_t niter, double tol, int mode,
+                       void* x_out, double* resid, pg_cg_info* info, void* stream)
+{
+    return pg_solve_pwls_pcg_tol(A, R, nullptr, nullptr, y, niter, tol, mode, x_out, resid, info,
+                                 stream);
 }
 
 int pg_set_tuning(const char* key, int value)

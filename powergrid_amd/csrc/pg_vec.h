@@ -17,6 +17,8 @@ struct CgScal {
     int stop;          // 0 or PG_EXIT_*
     int first;         // ddir = ngrad on the first iteration
     int pending;       // an update's residual norm still to be recorded
+    double tol;        // relative gradient tolerance (0: none)
+    double g0;         // real(cdot(ngrad, ngrad)) of the first iteration
 };
 
 namespace pgv {
@@ -57,7 +59,8 @@ struct CgParts {
 size_t cg_parts_doubles(long long N, long long M);
 CgParts cg_parts(double* base, long long N, long long M);
 
-void cg_init(const float2* y, const float* W, long long n, double* part, CgScal* sc, hipStream_t s);
+void cg_init(const float2* y, const float* W, long long n, double* part, CgScal* sc, double tol,
+             hipStream_t s);
 void cg_grad(const float2* P, int nc, long long N, const PenSpec* p, const float2* x,
              float2* ngrad, const CgParts& cp, CgScal* sc, double* resid, hipStream_t s);
 void cg_dir(const float2* ngrad, float2* ddir, long long N, const CgParts& cp, CgScal* sc,

@@ -411,11 +411,18 @@ __global__ void k_cg_dir(const float2* ngrad, float2* ddir, long long N, pgv::Cg
     }
     const double newinprod = gs[1];
     const int first = (sc->ii == 0);
+    // the optional tolerance exit (no counterpart in the reference): the
+    // gradient, penalty term included, down to tol of the first iteration's
+    if (!first && sc->tol > 0.0 && newinprod <= sc->tol * sc->tol * sc->g0) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) sc->stop = PG_EXIT_TOL;
+        return;
+    }
     const double gam = first ? 0.0 : ((fabs(sc->oldinprod) < 1e-10) ? 0.0 : newinprod / sc->oldinprod);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         sc->newinprod = newinprod;
         sc->gamma = gam;
         sc->first = first;
+        if (first) sc->g0 = newinprod;
     }
     const float gamf = (float)gam;
     double v[1] = {0};
@@ -631,7 +638,7 @@ __global__ void k_wnorm2(const float2* y, const float* W, long long n, double* p
     if (threadIdx.x == 0) part[blockIdx.x] = v[0];
 }
 
-__global__ void k_cg_init(const double* part, int nparts, CgScal* sc)
+__global__ void k_cg_init(const double* part, int nparts, CgScal* sc, double tol)
 {
     double v[1] = {0};
     for (int b = threadIdx.x; b < nparts; b += blockDim.x) v[0] += part[b];
@@ -639,6 +646,7 @@ __global__ void k_cg_init(const double* part, int nparts, CgScal* sc)
     if (threadIdx.x != 0) return;
     CgScal z = {};
     z.ywy = v[0];
+    z.tol = tol;
     *sc = z;
 }
 
@@ -755,11 +763,12 @@ size_t cg_parts_doubles(long long N, long long M)
     return (size_t)total;
 }
 
-void cg_init(const float2* y, const float* W, long long n, double* part, CgScal* sc, hipStream_t s)
+void cg_init(const float2* y, const float* W, long long n, double* part, CgScal* sc, double tol,
+             hipStream_t s)
 {
     const unsigned nb = nred(n);
     hipLaunchKernelGGL(k_wnorm2, dim3(nb), dim3(kThreads), 0, s, y, W, n, part);
-    hipLaunchKernelGGL(k_cg_init, dim3(1), dim3(kThreads), 0, s, part, (int)nb, sc);
+    hipLaunchKernelGGL(k_cg_init, dim3(1), dim3(kThreads), 0, s, part, (int)nb, sc, tol);
 }
 
 void cg_grad(const float2* P, int nc, long long N, const PenSpec* ps, const float2* x,

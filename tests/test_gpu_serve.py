@@ -1,0 +1,112 @@
+"""The serving loop (powergrid_amd.serve): a request's image is the same
+alone and in a mix of concurrent requests, admission keeps the reserved
+device memory within the budget, and the PCG's tolerance exit stops at the
+first iteration whose gradient is below the tolerance."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+import powergrid_amd as P
+from powergrid_amd import serve as SV
+from powergrid_amd import synth as S
+
+
+def rel(a, b):
+    a = np.asarray(a, dtype=np.complex128).ravel()
+    b = np.asarray(b, dtype=np.complex128).ravel()
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
+
+
+def make_request(N, nc, L, nshot, ns, Nz=1, seed=0, niter=30, tol=1e-4, beta=1e2, tag=None):
+    pr = S.make_problem(N, nshot, ns, nc, L, Nz=Nz, peak_hz=100.0, seed=seed)
+    M = len(pr.kx)
+    G = P.Gnufft(M, 2.0, N, N, Nz, pr.kx, pr.ky, pr.kz)
+    A = P.SENSE(P.TimeSegmentation(G, pr.fm, pr.t, M, pr.N, L), pr.sens, M, pr.N, nc)
+    y = S.add_noise(A * pr.x_true, 40.0, seed=seed + 2).astype(np.complex64)
+    return SV.ReconRequest(pr.kx, pr.ky, pr.kz, pr.t, N, N, Nz, nc, L, pr.fm, pr.sens, y,
+                           beta=beta, niter=niter, tol=tol, tag=tag)
+
+
+def mix():
+    # matrix sizes 32..96 (a radix-3 grid among them), 3..16 coils, L 1..8,
+    # one 3-D stack (4 coils x 4 segments: the deterministic tiled spread)
+    return [make_request(32, 4, 4, 4, 512, seed=1, tag="a"),
+            make_request(64, 8, 6, 8, 1024, seed=2, tag="b"),
+            make_request(48, 3, 1, 6, 768, seed=3, tag="c"),
+            make_request(32, 4, 4, 4, 512, Nz=8, seed=4, niter=12, tag="d"),
+            make_request(64, 16, 8, 8, 1024, seed=5, tol=1e-3, tag="e"),
+            make_request(96, 8, 8, 12, 1536, seed=6, tag="f")]
+
+
+def test_serve_results_alone_and_in_the_mix():
+    print("[serve alone vs mix]", flush=True)
+    reqs = mix()
+    st = P.Stream()
+    alone = {r.tag: SV.run_request(r, st) for r in reqs}
+    with SV.ReconServer(slots=4) as srv:
+        futs = [srv.submit(r) for r in reqs]
+        res = {f.result().tag: f.result() for f in futs}
+    assert srv.max_running >= 2
+    for r in reqs:
+        img, info = alone[r.tag]
+        out = res[r.tag]
+        print(f"  {r.tag}: {info['iterations']} iterations ({info['exit_reason']}), "
+              f"latency {1e3 * out.latency:.1f} ms")
+        assert out.iterations == info["iterations"] and out.exit_reason == info["exit_reason"]
+        np.testing.assert_array_equal(out.image, img)
+
+
+def test_serve_admission_by_memory():
+    print("[serve admission]", flush=True)
+    reqs = mix()[:4]
+    est = [SV.estimate_bytes(r) for r in reqs]
+    budget = int(1.5 * max(est))
+    with SV.ReconServer(slots=4, budget_bytes=budget) as srv:
+        futs = [srv.submit(r) for r in reqs]
+        outs = [f.result() for f in futs]
+    assert srv.peak_reserved <= budget
+    assert all(o.image.size == r.N for o, r in zip(outs, reqs))
+    # a request larger than the whole budget fails at once
+    with SV.ReconServer(slots=1, budget_bytes=min(est) // 2) as srv:
+        f = srv.submit(reqs[0])
+        assert isinstance(f.exception(), MemoryError)
+
+
+def test_pcg_tolerance_exit(orc):
+    # the loop ends before the update of iteration k, the first with
+    # ||ngrad_k|| <= tol ||ngrad_0||, ngrad = A'(y - Ax) - R'(x)
+    # (solve_pwls_pcg.hpp:80-86); the iterates are those of the plain solve
+    print("[pcg tolerance]", flush=True)
+    N, nc, L, tol = 32, 4, 4, 1e-2
+    pr = S.make_problem(N, 4, 512, nc, L, peak_hz=100.0, seed=7)
+    M = len(pr.kx)
+    G = P.Gnufft(M, 2.0, N, N, 1, pr.kx, pr.ky, pr.kz)
+    A = P.SENSE(P.TimeSegmentation(G, pr.fm, pr.t, M, pr.N, L), pr.sens, M, pr.N, nc)
+    R = P.QuadPenalty(N, N, 1, 1e2)
+    y = S.add_noise(A * pr.x_true, 40.0, seed=8).astype(np.complex64)
+    xk, info = P.reconSolve(y, A, R, niter=40, tol=tol)
+    k = info["iterations"]
+    print(f"  stopped after {k} updates ({info['exit_reason']})")
+    assert info["exit_reason"] == "tolerance" and 2 <= k < 40
+    xp, ip = P.reconSolve(y, A, R, niter=k)
+    assert ip["exit_reason"] == "niter" and ip["iterations"] == k
+    np.testing.assert_array_equal(xk, xp)
+    for mode in (P.CG_GRAPH, P.CG_HOST):
+        xh, ih = P.reconSolve(y, A, R, niter=40, tol=tol, mode=mode)
+        assert ih["iterations"] == k
+    # the FP64 restatement's gradient at the iterates k-1 and k
+    Go = orc.Gnufft(M, 2.0, N, N, 1, pr.kx, pr.ky, pr.kz, prec="d")
+    So = orc.SENSE(orc.TimeSegmentation(Go, pr.fm, pr.t, M, pr.N, L), pr.sens.ravel(), M,
+                   pr.N, nc)
+    Ro = orc.QuadPenalty(N, N, 1, 1e2, prec="d")
+    yd = y.astype(np.complex128)
+
+    def gnorm(x):
+        x = np.asarray(x, np.complex128)
+        return np.linalg.norm(So / (yd - So * x) - Ro.Gradient(x))
+
+    g0 = gnorm(np.zeros(pr.N))
+    xprev, _ = P.reconSolve(y, A, R, niter=k - 1)
+    assert gnorm(xk) <= tol * g0 * 1.05
+    assert gnorm(xprev) > tol * g0 * 0.95

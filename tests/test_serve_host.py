@@ -1,0 +1,90 @@
+"""Host logic of the serving loop (powergrid_amd.serve) with the device work
+replaced by a stand-in: arrival-order admission, the memory budget, the
+immediate failure of a request larger than the budget, and draining on
+close.  CPU only."""
+import threading
+import time
+
+import numpy as np
+import pytest
+
+from powergrid_amd import serve as SV
+
+
+def req(N, nc, L, M, tag):
+    z = np.zeros(M, np.float32)
+    return SV.ReconRequest(z, z, z, z, N, N, 1, nc, L, np.zeros(N * N, np.float32),
+                           np.zeros(N * N * nc, np.complex64), np.zeros(M * nc, np.complex64),
+                           tag=tag)
+
+
+@pytest.fixture
+def fake_device(monkeypatch):
+    log = []
+    lock = threading.Lock()
+
+    def run(r, stream=None):
+        with lock:
+            log.append(("start", r.tag))
+        time.sleep(0.02)
+        with lock:
+            log.append(("end", r.tag))
+        return np.full(r.N, r.nc, np.complex64), {"iterations": 3, "exit_reason": "niter"}
+
+    monkeypatch.setattr(SV, "run_request", run)
+    monkeypatch.setattr(SV, "Stream", lambda: None)
+    monkeypatch.setattr(SV, "mem_info", lambda: (1 << 40, 1 << 40))
+    return log
+
+
+def test_estimate_grows_with_grids_and_samples():
+    a = SV.estimate_bytes(req(64, 8, 8, 8192, "a"))
+    assert SV.estimate_bytes(req(64, 16, 8, 8192, "b")) > a
+    assert SV.estimate_bytes(req(128, 8, 8, 8192, "c")) > a
+    assert SV.estimate_bytes(req(64, 8, 8, 32768, "d")) > a
+    # the batched grids dominate: 64 grids of 128^2 complex64 twice over
+    assert a > 2 * 8 * 64 * 128 * 128
+
+
+def test_admission_order_and_budget(fake_device):
+    reqs = [req(64, 8, 8, 8192, i) for i in range(8)]
+    need = SV.estimate_bytes(reqs[0])
+    with SV.ReconServer(slots=4, budget_bytes=2 * need + need // 2) as srv:
+        futs = [srv.submit(r) for r in reqs]
+        outs = [f.result() for f in futs]
+    assert srv.peak_reserved <= srv.budget
+    assert srv.max_running == 2
+    starts = [t for e, t in fake_device if e == "start"]
+    assert starts == list(range(8))  # arrival order
+    assert all(o.iterations == 3 and o.latency >= o.finished - o.started for o in outs)
+    assert [o.tag for o in outs] == list(range(8))
+
+
+def test_head_of_line_waits_for_memory(fake_device):
+    small = SV.estimate_bytes(req(32, 2, 2, 1024, "s"))
+    big = req(128, 16, 8, 32768, "big")
+    with SV.ReconServer(slots=4, budget_bytes=SV.estimate_bytes(big) + small) as srv:
+        f1 = srv.submit(req(32, 2, 2, 1024, "s1"))
+        f2 = srv.submit(big)
+        f3 = srv.submit(req(32, 2, 2, 1024, "s2"))
+        for f in (f1, f2, f3):
+            f.result()
+    order = [t for e, t in fake_device if e == "start"]
+    assert order.index("big") < order.index("s2")
+
+
+def test_too_large_request_fails_at_once(fake_device):
+    r = req(64, 8, 8, 8192, "x")
+    with SV.ReconServer(slots=2, budget_bytes=SV.estimate_bytes(r) - 1) as srv:
+        f = srv.submit(r)
+        assert isinstance(f.exception(timeout=1), MemoryError)
+    assert fake_device == []
+
+
+def test_close_drains_the_queue(fake_device):
+    srv = SV.ReconServer(slots=1, budget_bytes=1 << 40)
+    futs = [srv.submit(req(32, 2, 2, 1024, i)) for i in range(5)]
+    srv.close()
+    assert all(f.done() and not f.exception() for f in futs)
+    with pytest.raises(RuntimeError):
+        srv.submit(req(32, 2, 2, 1024, "late"))
