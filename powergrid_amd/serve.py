@@ -121,9 +121,11 @@ def estimate_bytes(req: ReconRequest) -> int:
     return int(1.25 * (grids + vals + data + image + tables)) + (32 << 20)
 
 
-def run_request(req: ReconRequest, stream: Optional[Stream] = None):
+def run_request(req: ReconRequest, stream: Optional[Stream] = None, keep: Optional[list] = None):
     """Build the request's operators, solve and return (image, info); every
-    copy and launch goes on `stream`."""
+    copy and launch goes on `stream`.  With `keep` (a list) the device
+    objects are appended to it instead of being released here, so that the
+    caller can release them after handing the image on."""
     M, N = req.M, req.N
     G = Gnufft(M, req.gridos, req.Nx, req.Ny, req.Nz, req.kx, req.ky, req.kz,
                kernel_width=req.kernel_width)
@@ -136,6 +138,8 @@ def run_request(req: ReconRequest, stream: Optional[Stream] = None):
     y = DeviceArray.from_host(np.asarray(req.data, np.complex64), stream=stream)
     x, info = solve_pwls_pcg(None, A, None, y, R, req.niter, CG_GRAPH, stream, req.tol)
     img = x.to_host(stream)
+    if keep is not None:
+        keep.extend((x, y, A, T, G, R))
     del x, y, A, T, G, R
     return img, info
 
@@ -215,13 +219,17 @@ class ReconServer:
                 return
             req, need, fut, t_sub = item
             t0 = time.perf_counter()
+            keep: list = []
             try:
-                img, info = run_request(req, stream)
+                img, info = run_request(req, stream, keep)
                 fut.set_result(ReconResult(img, info["iterations"], info["exit_reason"], t_sub,
                                            t0, time.perf_counter(), need, req.tag))
             except BaseException as e:  # reported through the future
                 fut.set_exception(e)
             finally:
+                # the operators and buffers go after the image is handed on
+                # (their release waits for the device)
+                keep.clear()
                 with self._cv:
                     self.reserved -= need
                     self._running -= 1

@@ -23,7 +23,7 @@ def fake_device(monkeypatch):
     log = []
     lock = threading.Lock()
 
-    def run(r, stream=None):
+    def run(r, stream=None, keep=None):
         with lock:
             log.append(("start", r.tag))
         time.sleep(0.02)
