@@ -38,14 +38,30 @@ static bool g_prof = false;
 static std::vector<ProfRec> g_prof_recs;
 static std::mutex g_prof_mu;
 
-// HIP refuses calls that use the legacy stream or synchronise the device
-// (hipMemcpy, hipFree, null-stream launches, ...) from any thread while a
-// stream capture is open.  Graph capture and every such call of the library
-// hold this lock, so requests of different threads may build, solve and
-// free their operators concurrently; the device work on their own streams
-// (graph launches, asynchronous copies) does not take it.
+// HIP refuses calls that use the legacy stream (hipMemcpy, null-stream
+// launches and synchronisations) from any thread while a stream capture is
+// open.  The library's own set-up copies and kernels therefore go on a
+// per-thread non-blocking stream (setup_stream), and graph capture and the
+// remaining legacy-stream calls (host-buffer operator calls, copies that the
+// caller asks for on the null stream, stream and graph creation and
+// destruction) hold this lock.  Threads may then build, solve and free their
+// operators concurrently; the device work on their own streams (graph
+// launches, asynchronous copies) does not take it.
 static std::recursive_mutex g_legacy_mu;
 using LegacyLock = std::lock_guard<std::recursive_mutex>;
+
+// the calling thread's stream for set-up copies and kernels: non-blocking, so
+// that set-up never touches the legacy stream (created on first use, kept
+// for the thread's life)
+static hipStream_t setup_stream()
+{
+    static thread_local hipStream_t st = nullptr;
+    if (!st) {
+        LegacyLock lk(g_legacy_mu);
+        PG_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    }
+    return st;
+}
 
 struct Prof {
     bool on = false;
@@ -99,15 +115,15 @@ FftPlan1D make_fft_plan(int n)
         const double a = -2.0 * M_PI * (double)k / (double)n;
         tw[k] = make_float2((float)std::cos(a), (float)std::sin(a));
     }
-    LegacyLock lk(g_legacy_mu);
     PG_HIP(hipMalloc(&p.tw, sizeof(float2) * n));
-    PG_HIP(hipMemcpy(p.tw, tw.data(), sizeof(float2) * n, hipMemcpyHostToDevice));
+    const hipStream_t ss = setup_stream();
+    PG_HIP(hipMemcpyAsync(p.tw, tw.data(), sizeof(float2) * n, hipMemcpyHostToDevice, ss));
+    PG_HIP(hipStreamSynchronize(ss));
     return p;
 }
 
 void free_fft_plan(FftPlan1D& p)
 {
-    LegacyLock lk(g_legacy_mu);
     if (p.tw) hipFree(p.tw);
     p.tw = nullptr;
 }
@@ -117,7 +133,6 @@ static T* dev_alloc(size_t n)
 {
     if (n == 0) n = 1;
     void* p = nullptr;
-    LegacyLock lk(g_legacy_mu);
     hipError_t e = hipMalloc(&p, sizeof(T) * n);
     if (e != hipSuccess)
         throw Error{PG_ERR_OOM, "hipMalloc of " + std::to_string(sizeof(T) * n) + " bytes failed"};
@@ -127,19 +142,23 @@ static T* dev_alloc(size_t n)
 template <class T>
 static T* dev_copy(const T* h, size_t n)
 {
-    LegacyLock lk(g_legacy_mu);
     T* d = dev_alloc<T>(n);
-    if (h && n) PG_HIP(hipMemcpy(d, h, sizeof(T) * n, hipMemcpyHostToDevice));
+    if (h && n) {
+        const hipStream_t ss = setup_stream();
+        const hipError_t e1 = hipMemcpyAsync(d, h, sizeof(T) * n, hipMemcpyHostToDevice, ss);
+        const hipError_t e2 = e1 == hipSuccess ? hipStreamSynchronize(ss) : e1;
+        if (e2 != hipSuccess) {
+            hipFree(d);
+            PG_HIP(e2);
+        }
+    }
     return d;
 }
 
 template <class T>
 static void dev_free(T*& p)
 {
-    if (p) {
-        LegacyLock lk(g_legacy_mu);
-        hipFree((void*)p);
-    }
+    if (p) hipFree((void*)p);
     p = nullptr;
 }
 
@@ -1555,11 +1574,11 @@ static float2* minmax_ls_weights(const float* fieldmap, const float* tvec, size_
     d.fm = dev_copy(fm.data(), N);
     d.tv = dev_copy(tv.data(), none);
     d.W = dev_copy(W.data(), W.size());
-    LegacyLock lk(g_legacy_mu);  // null-stream launch and synchronisation
     float2* AA = dev_alloc<float2>((size_t)L * M);
-    pgt::minmax_weights(d.fm, d.tv, d.W, (int)N, (int)none, L, (int)shots, AA, nullptr);
+    pgt::minmax_weights(d.fm, d.tv, d.W, (int)N, (int)none, L, (int)shots, AA,
+                        setup_stream());
     const hipError_t e1 = hipGetLastError();
-    const hipError_t e2 = hipStreamSynchronize(nullptr);
+    const hipError_t e2 = hipStreamSynchronize(setup_stream());
     if (e1 != hipSuccess || e2 != hipSuccess) {
         dev_free(AA);
         PG_HIP(e1 != hipSuccess ? e1 : e2);
@@ -1691,11 +1710,11 @@ static float2* histogram_ls_weights(const float* fieldmap, const float* tvec, si
     d.z = dev_copy(z.data(), nb);
     d.tv = dev_copy(tv.data(), none);
     d.iG = dev_copy(iG.data(), iG.size());
-    LegacyLock lk(g_legacy_mu);  // null-stream launch and synchronisation
     float2* AA = dev_alloc<float2>((size_t)L * M);
-    pgt::histogram_weights(d.h, d.om, d.z, nb, d.tv, (int)none, d.iG, L, (int)shots, AA, nullptr);
+    pgt::histogram_weights(d.h, d.om, d.z, nb, d.tv, (int)none, d.iG, L, (int)shots, AA,
+                        setup_stream());
     const hipError_t e1 = hipGetLastError();
-    const hipError_t e2 = hipStreamSynchronize(nullptr);
+    const hipError_t e2 = hipStreamSynchronize(setup_stream());
     if (e1 != hipSuccess || e2 != hipSuccess) {
         dev_free(AA);
         PG_HIP(e1 != hipSuccess ? e1 : e2);
@@ -1763,10 +1782,12 @@ static TsOp* ts_build(Op* G, const float* fieldmap, const float* tvec, size_t M,
         op->AAc = histogram_ls_weights(fieldmap, tvec, M, N, Lc, shots, tau, tmin);
     std::vector<float> AAim;
     if (op->AAc) {
-        LegacyLock lk(g_legacy_mu);
         // the fused kernels take complex weights as real and imaginary tables
         std::vector<float2> c(AA.size());
-        PG_HIP(hipMemcpy(c.data(), op->AAc, c.size() * sizeof(float2), hipMemcpyDeviceToHost));
+        const hipStream_t ss = setup_stream();
+        PG_HIP(hipMemcpyAsync(c.data(), op->AAc, c.size() * sizeof(float2), hipMemcpyDeviceToHost,
+                              ss));
+        PG_HIP(hipStreamSynchronize(ss));
         AAim.resize(AA.size());
         for (size_t e = 0; e < c.size(); ++e) {
             AA[e] = c[e].x;
@@ -1844,14 +1865,12 @@ int pg_malloc(void** ptr, size_t bytes)
 {
     PG_TRY PG_REQUIRE(ptr, PG_ERR_ARG, "null ptr");
     *ptr = nullptr;
-    LegacyLock lk(g_legacy_mu);
     if (hipMalloc(ptr, bytes ? bytes : 1) != hipSuccess) throw Error{PG_ERR_OOM, "hipMalloc failed"};
     PG_CATCH
 }
 int pg_free(void* ptr)
 {
-    PG_TRY LegacyLock lk(g_legacy_mu);
-    if (ptr) PG_HIP(hipFree(ptr));
+    PG_TRY if (ptr) PG_HIP(hipFree(ptr));
     PG_CATCH
 }
 // With stream NULL the copies are ordered after all work on the device: the
