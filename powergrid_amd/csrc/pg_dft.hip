@@ -303,10 +303,12 @@ R2Args r2_args(const pgd::DftData& d)
 
 int slices_for(long long outputs, long long inputs)
 {
-    // aim for >= 4096 wavefronts in flight
+    // aim for >= 4096 wavefronts in flight, each slice at least 64 inputs
+    // (small problems, such as pcSENSE's per-shot DFTs of ~1k samples, would
+    // otherwise leave most of the 1024 SIMDs idle)
     long long waves = (outputs + kT - 1) / kT;
     int s = 1;
-    while (waves * s < 4096 && inputs / (s * 2) >= 256 && s < 64) s *= 2;
+    while (waves * s < 4096 && inputs / (s * 2) >= 64 && s < 64) s *= 2;
     return s;
 }
 
