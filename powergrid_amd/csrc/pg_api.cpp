@@ -11,6 +11,7 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <thread>
 #include <string>
 #include <vector>
 
@@ -345,7 +346,7 @@ static void gnufft_setup(Gnufft& G, size_t M, float os, size_t Nx, size_t Ny, si
     // per-sample windows and weights
     std::vector<int> lo((size_t)D * M);
     std::vector<float> w((size_t)D * NT * M);
-    std::vector<double> wd((size_t)D * NT * M);  // the same weights in FP64
+    std::vector<double> wd((size_t)D * NT * M);  // the same in FP64, sample-major [i][d][j]
     std::vector<int> vlo((size_t)D * M), vhi((size_t)D * M);
     std::vector<double> tmp(NT);
     const float* kk[3] = {kx, ky, kz};
@@ -358,7 +359,7 @@ static void gnufft_setup(Gnufft& G, size_t M, float os, size_t Nx, size_t Ny, si
             vhi[(size_t)d * M + i] = r.vhi;
             for (int j = 0; j < NT; ++j) {
                 w[((size_t)d * NT + j) * M + i] = (float)tmp[j];
-                wd[((size_t)d * NT + j) * M + i] = tmp[j];
+                wd[((size_t)i * D + d) * NT + j] = tmp[j];
             }
         }
     G.lo = dev_copy(lo.data(), lo.size());
@@ -514,19 +515,40 @@ static void gnufft_setup(Gnufft& G, size_t M, float os, size_t Nx, size_t Ny, si
                 // weights of entry k (patch q, sample i) on the patch's 16 points,
                 // wx * wy formed in FP64 and rounded once
                 std::vector<float> ew((size_t)16 * std::max<long long>(1, cnt[np]), 0.0f);
-                auto tapw = [&](int d, size_t i, int j) {
-                    return (j >= 0 && j < NT) ? wd[((size_t)d * NT + j) * M + i] : 0.0;
-                };
-                for (long long q = 0; q < np; ++q) {
-                    const int px0 = (int)(q / npy) * kPatch, py0 = (int)(q % npy) * kPatch;
-                    for (long long k = cnt[q]; k < cnt[q + 1]; ++k) {
-                        const size_t i = (size_t)ent[k];
-                        for (int u = 0; u < kPatch; ++u)
-                            for (int v = 0; v < kPatch; ++v)
-                                ew[16 * k + 4 * u + v] = (float)(tapw(0, i, px0 + u - lo[i]) *
-                                                                 tapw(1, i, py0 + v - lo[M + i]));
+                auto fill = [&](long long q0, long long q1) {
+                    for (long long q = q0; q < q1; ++q) {
+                        const int px0 = (int)(q / npy) * kPatch, py0 = (int)(q % npy) * kPatch;
+                        for (long long k = cnt[q]; k < cnt[q + 1]; ++k) {
+                            const size_t i = (size_t)ent[k];
+                            const double* wx = &wd[(i * D + 0) * NT];
+                            const double* wy = &wd[(i * D + 1) * NT];
+                            double ax[kPatch], ay[kPatch];
+                            for (int u = 0; u < kPatch; ++u) {
+                                const int jx = px0 + u - lo[i], jy = py0 + u - lo[M + i];
+                                ax[u] = (jx >= 0 && jx < NT) ? wx[jx] : 0.0;
+                                ay[u] = (jy >= 0 && jy < NT) ? wy[jy] : 0.0;
+                            }
+                            for (int u = 0; u < kPatch; ++u)
+                                for (int v = 0; v < kPatch; ++v)
+                                    ew[16 * k + 4 * u + v] = (float)(ax[u] * ay[v]);
+                        }
                     }
+                };
+                // a few host threads over runs of patches of equal entry count
+                const int nth = (int)std::min<long long>(
+                    std::max(1u, std::min(4u, std::thread::hardware_concurrency())),
+                    std::max<long long>(1, cnt[np] / 16384));
+                std::vector<std::thread> th;
+                long long q0 = 0;
+                for (int t = 1; t <= nth; ++t) {
+                    long long q1 = q0;
+                    const long long target = cnt[np] * t / nth;
+                    while (q1 < np && (cnt[q1] < target || t == nth)) ++q1;
+                    if (t < nth) th.emplace_back(fill, q0, q1);
+                    else fill(q0, q1);
+                    q0 = q1;
                 }
+                for (std::thread& x : th) x.join();
                 G.entries = dev_copy(ent.data(), ent.size());
                 G.ent_w = dev_copy(ew.data(), ew.size());
                 G.tile_entries = cnt[np];
