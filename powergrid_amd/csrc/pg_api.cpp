@@ -129,6 +129,19 @@ void free_fft_plan(FftPlan1D& p)
     p.tw = nullptr;
 }
 
+// fn(begin, end) over [0, n) on up to 4 host threads (set-up loops whose
+// iterations are independent), at least `grain` iterations each
+template <class F>
+static void host_parallel(size_t n, size_t grain, F fn)
+{
+    const size_t hw = std::max(1u, std::min(4u, std::thread::hardware_concurrency()));
+    const size_t nth = std::max<size_t>(1, std::min(hw, n / std::max<size_t>(grain, 1)));
+    std::vector<std::thread> th;
+    for (size_t t = 1; t < nth; ++t) th.emplace_back(fn, n * t / nth, n * (t + 1) / nth);
+    fn(0, n / nth);
+    for (std::thread& x : th) x.join();
+}
+
 template <class T>
 static T* dev_alloc(size_t n)
 {
@@ -348,20 +361,22 @@ static void gnufft_setup(Gnufft& G, size_t M, float os, size_t Nx, size_t Ny, si
     std::vector<float> w((size_t)D * NT * M);
     std::vector<double> wd((size_t)D * NT * M);  // the same in FP64, sample-major [i][d][j]
     std::vector<int> vlo((size_t)D * M), vhi((size_t)D * M);
-    std::vector<double> tmp(NT);
     const float* kk[3] = {kx, ky, kz};
     const int Nd[3] = {g.Nx, g.Ny, g.Nz};
-    for (int d = 0; d < D; ++d)
-        for (size_t i = 0; i < M; ++i) {
-            Win1 r = window1((double)kk[d][i], Nd[d], os, W, NT, lut, tmp.data());
-            lo[(size_t)d * M + i] = r.lo;
-            vlo[(size_t)d * M + i] = r.vlo;
-            vhi[(size_t)d * M + i] = r.vhi;
-            for (int j = 0; j < NT; ++j) {
-                w[((size_t)d * NT + j) * M + i] = (float)tmp[j];
-                wd[((size_t)i * D + d) * NT + j] = tmp[j];
+    host_parallel(M, 8192, [&](size_t b, size_t e) {
+        std::vector<double> tmp(NT);
+        for (int d = 0; d < D; ++d)
+            for (size_t i = b; i < e; ++i) {
+                Win1 r = window1((double)kk[d][i], Nd[d], os, W, NT, lut, tmp.data());
+                lo[(size_t)d * M + i] = r.lo;
+                vlo[(size_t)d * M + i] = r.vlo;
+                vhi[(size_t)d * M + i] = r.vhi;
+                for (int j = 0; j < NT; ++j) {
+                    w[((size_t)d * NT + j) * M + i] = (float)tmp[j];
+                    wd[((size_t)i * D + d) * NT + j] = tmp[j];
+                }
             }
-        }
+    });
     G.lo = dev_copy(lo.data(), lo.size());
     G.w = dev_copy(w.data(), w.size());
 
@@ -1786,15 +1801,15 @@ static TsOp* ts_build(Op* G, const float* fieldmap, const float* tvec, size_t M,
                 if (std::fabs(d) <= tau) a = (float)(0.5 + 0.5 * std::cos(M_PI * d / tau));
                 for (size_t sh = 0; sh < shots; ++sh) AA[(size_t)l * M + j + sh * none] = a;
             }
-        for (int l = 0; l < Lc; ++l) {
-            const double sl = (double)l * tau + tmin;
-            for (size_t n = 0; n < N; ++n) {
-                const double ph = (double)fieldmap[n] * sl;
+        host_parallel((size_t)Lc * N, 16384, [&](size_t b, size_t e) {
+            for (size_t k = b; k < e; ++k) {
+                const size_t l = k / N, n = k - l * N;
+                const double ph = (double)fieldmap[n] * ((double)l * tau + tmin);
                 const float c = (float)std::cos(ph), sn = (float)std::sin(ph);
-                Wo[(size_t)l * N + n] = make_float2(c, -sn);
-                WoH[(size_t)l * N + n] = make_float2(c, sn);
+                Wo[k] = make_float2(c, -sn);
+                WoH[k] = make_float2(c, sn);
             }
-        }
+        });
     }
     op->tau = (float)tau;
     op->tmin = (float)tmin;
