@@ -80,3 +80,13 @@ def test_cpp_operator_api_matches_oracle(orc, tmp_path):
     _, phs = PIO.read_nifti(str(fout) + "_img_phs.nii")
     np.testing.assert_allclose(mag, np.abs(img), rtol=1e-6)
     assert np.max(np.abs(np.exp(1j * phs) - np.exp(1j * np.angle(img)))) < 1e-5
+
+
+def test_header_templates_compile():
+    # every class template of include/powergrid/powergrid.hpp instantiated
+    # explicitly (tests/cpp/header_instantiation.cpp), with the host compiler
+    src = os.path.join(ROOT, "tests", "cpp", "header_instantiation.cpp")
+    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall",
+                        "-I" + os.path.join(ROOT, "include"), src],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
