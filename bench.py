@@ -143,16 +143,7 @@ def gdft_roofline(P, prob, stream, reps=3):
     D = P.Gdft(M, N, prob.kx, prob.ky, prob.kz, prob.ix, prob.iy, prob.iz, prob.fm, prob.t,
                precision=0)
     Sd = P.SENSE(D, prob.sens, M, N, nc)
-    xd = P.DeviceArray.from_host(prob.x_true.astype(np.complex64))
-    yd = Sd.forward(xd, stream=stream)
-    Sd.adjoint(yd, stream=stream)
-    stream.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        yd = Sd.forward(xd, stream=stream)
-        Sd.adjoint(yd, stream=stream)
-    stream.synchronize()
-    dt = (time.perf_counter() - t0) / reps
+    dt = _pair_ms(P, Sd, prob.x_true, stream, reps) * 1e-3
     pairs = 2.0 * M * N * nc
     tf = 16.0 * pairs / dt / 1e12
     return dict(ms_per_pair_of_ops=round(1e3 * dt, 3), pairs=pairs, achieved=round(tf, 2),
@@ -433,6 +424,28 @@ def serving_rates(P, S, n=96, rate=20.0, slots=4, seed=11):
                 latency_ms_p50_by_kind={k: round(float(np.median(v)), 2) for k, v in per.items()})
 
 
+def autotune_rate(P, prob, yd, stream, niter, steps=10):
+    """The headline solve with pg_set_tuning("autotune", 1): the plan times its
+    interpolation and spread variants at first use and keeps the faster."""
+    P.set_tuning("autotune", 1)
+    try:
+        G, T, Sg, R = build_ops(P, prob, P.MODE_BATCHED)
+        for _ in range(2):
+            P.reconSolve(yd, Sg, R, niter=niter, mode=P.CG_GRAPH, stream=stream)
+        stream.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            x, info = P.reconSolve(yd, Sg, R, niter=niter, mode=P.CG_GRAPH, stream=stream)
+        stream.synchronize()
+        dt = time.perf_counter() - t0
+        lds, mfma = P.op_tuning(Sg, prob.nc * prob.L)
+    finally:
+        P.set_tuning("autotune", 0)
+    return dict(iters_per_s=round(steps * info["iterations"] / dt, 1),
+                interp="lds tiles" if lds == 1 else "gather",
+                spread="matrix cores" if mfma == 1 else "vector units")
+
+
 def pmc_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed FETCH_SIZE/WRITE_SIZE
     summary (profiles/sense2d_pmc_traffic.md, counters scaled by a device copy
@@ -526,6 +539,8 @@ def main():
     x_gpu = x.to_host()
     log(f"[bench] batched: {value:.1f} it/s, {1e3 * dt / args.steps:.2f} ms/recon, "
         f"exit={info['exit_reason']} iters={iters_done}")
+    tuned = autotune_rate(P, prob, yd, stream, niter)
+    log(f"[bench] autotuned: {tuned}")
 
     # ---------------- per-kernel durations (HIP events on the launch stream) --
     P.profile.enable(True)
@@ -602,6 +617,7 @@ def main():
                              "p99": round(1e3 * float(np.percentile(lat, 99)), 3)},
         "nrmse_plain_vs_batched": nrmse(x_plain, x_gpu),
         "kernels_us": {k: round(v["ms"] * 1e3, 2) for k, v in kinfo.items()},
+        "autotuned": tuned,
         "roofline": roof,
     }
     out["hbm_copy_GBs"] = round(copy_bandwidth(P, stream), 1)

@@ -299,9 +299,21 @@ int pg_recon_solve_tol(pg_op A, pg_penalty R, const void* y_dev, size_t niter, d
 /* ---- tuning (no counterpart in the reference) ----------------------------
  * "spread_mfma" 1: the 2-D batched spread runs on the matrix cores when the
  * grid count is a multiple of 4; 0 (default): on the vector units.  Both
- * give the same sums up to FP32 rounding order.  Takes effect for launches
- * (and solve graphs) made after the call. */
+ * give the same sums up to FP32 rounding order.
+ * "autotune" 1: when an operator is first prepared for a batch of B grids,
+ * its Gnufft plan times each variant of the 2-D batched interpolation
+ * (LDS-staged tiles or gather) and spread (vector units or matrix cores) on
+ * its own workspaces and keeps the faster for that B, overriding
+ * "spread_mfma"; 0 (default): the fixed choice.  The variants differ in
+ * FP32 summation order only, but with autotuning on, which one runs -- and
+ * so the last bits of a result -- depends on the timing.
+ * Settings take effect for launches (and solve graphs) made after the call. */
 int pg_set_tuning(const char* key, int value);
+/* the variants the autotuner chose for the Gnufft plan under h (a Gnufft,
+ * or a TimeSegmentation or SENSE over one) at B grids: interp_lds 1 =
+ * LDS-staged tiles, 0 = gather; spread_mfma 1 = matrix cores, 0 = vector
+ * units; both -1 when the plan was not tuned for B */
+int pg_op_tuning(pg_op h, int B, int* interp_lds, int* spread_mfma);
 
 /* ---- measurement --------------------------------------------------------
  * When enabled, every kernel launch of the library is bracketed by HIP

@@ -367,10 +367,19 @@ class SENSE(_Op):
         self.nc = nc
 
 
+def op_tuning(op, B):
+    """The kernel variants the autotuner chose for the Gnufft plan under op at
+    B grids: (interp_lds, spread_mfma), each 1/0, or (-1, -1) when untuned."""
+    a, b = C.c_int(), C.c_int()
+    check(lib().pg_op_tuning(op.handle, int(B), C.byref(a), C.byref(b)))
+    return a.value, b.value
+
+
 def set_tuning(key, value):
     """pg_set_tuning: "spread_mfma" 1 runs the 2-D batched spread on the
     matrix cores (grid count a multiple of 4), 0 (default) on the vector
-    units."""
+    units; "autotune" 1 lets each plan time its interpolation and spread
+    variants when first prepared for a grid count and keep the faster."""
     check(lib().pg_set_tuning(key.encode(), int(value)))
 
 
@@ -520,6 +529,7 @@ class profile:
 
 __all__ = ["DeviceArray", "Stream", "Gnufft", "Gdft", "GdftR2", "TimeSegmentation", "SENSE",
            "PcSenseTimeSeg", "PcSenseRank", "PcSENSE", "QuadPenalty", "TVPenalty", "calc3DDensityCompensation",
-           "grid_coil_images", "calcSumOfSquaresImage", "set_tuning", "solve_pwls_pcg",
+           "grid_coil_images", "calcSumOfSquaresImage", "set_tuning", "op_tuning",
+           "solve_pwls_pcg",
            "reconSolve", "profile", "PgError", "MODE_BATCHED", "MODE_PLAIN", "CG_GRAPH",
            "CG_HOST"]

@@ -967,6 +967,94 @@ static void require_fft(const Gnufft& G)
                    " are not of the form 2^a 3^b: only the spatial interpolation is available");
 }
 
+// pg_set_tuning("autotune", 1): plans pick their kernel variants by timing
+static std::atomic<int> g_autotune{0};
+
+// Times each variant of plan G's batched interpolation (LDS-staged or
+// gather) and spread (vector units or matrix cores) for nc x L grids, on the
+// plan's own workspaces (their contents do not matter for timing), and keeps
+// the faster of each.  Set-up work on the thread's set-up stream, before any
+// capture; a plan is tuned once per grid count.
+static void autotune(Gnufft& G, int nc, int L, const float* AA, const float* AAi)
+{
+    const int B = nc * L;
+    if (G.geo.dims != 2 || !G.fft_ok || G.tuned_for(B)) return;
+    G.ensure_batch(B);
+    const hipStream_t ss = setup_stream();
+    hipEvent_t e[2] = {nullptr, nullptr};
+    {
+        LegacyLock lk(g_legacy_mu);
+        PG_HIP(hipEventCreate(&e[0]));
+        if (hipEventCreate(&e[1]) != hipSuccess) {
+            hipEventDestroy(e[0]);
+            throw Error{PG_ERR_HIP, "hipEventCreate failed"};
+        }
+    }
+    struct Ev {
+        hipEvent_t* e;
+        ~Ev()
+        {
+            LegacyLock lk(g_legacy_mu);
+            hipEventDestroy(e[0]);
+            hipEventDestroy(e[1]);
+        }
+    } ev{e};
+    PG_HIP(hipMemsetAsync(G.grids, 0, sizeof(float2) * (size_t)B * G.geo.G(), ss));
+    PG_HIP(hipMemsetAsync(G.vals, 0, sizeof(float2) * (size_t)B * G.M, ss));
+    auto timed = [&](auto&& launch) {
+        launch();  // warm-up
+        PG_HIP(hipEventRecord(e[0], ss));
+        for (int r = 0; r < 3; ++r) launch();
+        PG_HIP(hipEventRecord(e[1], ss));
+        PG_HIP(hipEventSynchronize(e[1]));
+        float ms = 0;
+        PG_HIP(hipEventElapsedTime(&ms, e[0], e[1]));
+        return ms;
+    };
+    G.tuned.push_back(Gnufft::Tuned{B, 1, 0});
+    Gnufft::Tuned& t = G.tuned.back();
+    if (pgk::interp_lds_possible(G, L, AA != nullptr)) {
+        auto interp = [&] { pgk::launch_interp(G, G.grids, nc, L, AA, AAi, G.vals, G.M, ss); };
+        t.lds = 1;
+        const float ms_lds = timed(interp);
+        t.lds = 0;
+        const float ms_gather = timed(interp);
+        t.lds = ms_lds <= ms_gather ? 1 : 0;
+    } else {
+        t.lds = 0;
+    }
+    if (B % 4 == 0) {
+        auto spread = [&] {
+            pgk::launch_spread(G, nullptr, G.M, nc, L, AA, AAi, nullptr, nullptr, nullptr, G.grids,
+                               true, ss);
+        };
+        t.mfma = 0;
+        const float ms_vec = timed(spread);
+        t.mfma = 1;
+        const float ms_mfma = timed(spread);
+        t.mfma = ms_mfma < ms_vec ? 1 : 0;
+    }
+    PG_HIP(hipGetLastError());
+}
+
+// autotune the plan of a fused launch on stream s, unless already tuned for
+// its grid count or s is capturing; s is drained first, since the timing
+// runs overwrite the plan's workspaces
+static void maybe_autotune(const Fused& f, hipStream_t s)
+{
+    if (!g_autotune.load() || f.G->tuned_for(f.nc * f.L)) return;
+    if (s) {
+        hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+        PG_HIP(hipStreamIsCapturing(s, &st));
+        if (st != hipStreamCaptureStatusNone) return;
+        PG_HIP(hipStreamSynchronize(s));
+    } else {
+        LegacyLock lk(g_legacy_mu);
+        PG_HIP(hipDeviceSynchronize());
+    }
+    autotune(*f.G, f.nc, f.L, f.AA, f.AAi);
+}
+
 static void fused_forward(const Fused& f, const float2* x, float2* y, hipStream_t s,
                           long long y_cstride = -1)
 {
@@ -974,6 +1062,7 @@ static void fused_forward(const Fused& f, const float2* x, float2* y, hipStream_
     require_fft(G);
     const int B = f.nc * f.L;
     G.ensure_batch(B);
+    maybe_autotune(f, s);
     pgk::WeightArgs w{f.S, f.Wo, f.WoH, f.AA};
     {
         Prof p("fwd_prep_fft", s);
@@ -996,6 +1085,7 @@ static void fused_adjoint_partial(const Fused& f, const float2* in, const float2
     require_fft(G);
     const int B = f.nc * f.L;
     G.ensure_batch(B);
+    maybe_autotune(f, s);
     pgk::WeightArgs w{f.S, f.Wo, f.WoH, f.AA};
     const long long ics = in_cstride < 0 ? G.M : in_cstride;
     if (!vals_ready && pgk::spread_uses_values(G, B)) {
@@ -1485,6 +1575,7 @@ static void solve_pcg(Op* A, Penalty* R, const float2* x0, const float* W, const
         s = c->own;
     }
 
+    if (fused) maybe_autotune(fz, s);  // before any capture
     if (x0) PG_HIP(hipMemcpyAsync(w.x32, x0, sizeof(float2) * N, hipMemcpyDeviceToDevice, s));
     else pgv::fill(w.x32, N, make_float2(0.f, 0.f), s);
 
@@ -2674,12 +2765,27 @@ int pg_recon_solve_tol(pg_op A, pg_penalty R, const void* y, size_t niter, doubl
                                  stream);
 }
 
+int pg_op_tuning(pg_op h, int B, int* interp_lds, int* spread_mfma)
+{
+    PG_TRY Op* op = as_op(h);
+    PG_REQUIRE(interp_lds && spread_mfma, PG_ERR_ARG, "null argument");
+    if (op->kind == K_SENSE) op = static_cast<SenseOp*>(op)->inner;
+    if (op->kind == K_TS) op = static_cast<TsOp*>(op)->inner;
+    PG_REQUIRE(op->kind == K_NUFFT, PG_ERR_ARG, "not a Gnufft or an operator over one");
+    const Gnufft::Tuned* t = static_cast<NufftOp*>(op)->G.tuned_for(B);
+    *interp_lds = t ? t->lds : -1;
+    *spread_mfma = t ? t->mfma : -1;
+    PG_CATCH
+}
+
 int pg_set_tuning(const char* key, int value)
 {
     PG_TRY PG_REQUIRE(key, PG_ERR_ARG, "null key");
     const std::string k(key);
     if (k == "spread_mfma")
         pgk::g_spread_mfma.store(value ? 1 : 0);
+    else if (k == "autotune")
+        g_autotune.store(value ? 1 : 0);
     else
         throw pg::Error{PG_ERR_ARG, "unknown tuning key " + k};
     PG_CATCH

@@ -8,6 +8,8 @@
 // accumulation stays FP32.
 #pragma once
 
+#include <vector>
+
 #include "pg_internal.h"
 
 namespace pg {
@@ -70,6 +72,19 @@ struct Gnufft {
     int* tile_off = nullptr;  // ntiles + 1
     int* tile_idx = nullptr;  // concatenated sample indices
     long long tile_entries = 0;
+    // kernel variants chosen by the autotuner for a batch of B grids: the
+    // LDS-staged (1) or gather (0) interpolation, the matrix-core (1) or
+    // vector-unit (0) spread; no entry: the defaults
+    struct Tuned {
+        int B, lds, mfma;
+    };
+    std::vector<Tuned> tuned;
+    const Tuned* tuned_for(int B) const
+    {
+        for (const Tuned& t : tuned)
+            if (t.B == B) return &t;
+        return nullptr;
+    }
     // workspace for batched transforms (grown on demand)
     int cap_batch = 0;
     float2* grids = nullptr;    // cap_batch * G

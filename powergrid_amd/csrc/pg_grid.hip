@@ -771,8 +771,8 @@ void interp_dispatch(const pg::Gnufft& G, const float2* grids, int nc, int L, co
                      float2* out, long long out_cstride, hipStream_t s)
 {
     const pg::Geometry& g = G.geo;
-    if (!D3 && NT <= 8 && AA && L >= 2 && L <= 16 && G.n_iitems > 0 &&
-        (long long)L * g.G() < (1LL << 31)) {
+    const pg::Gnufft::Tuned* tn = G.tuned_for(nc * L);
+    if (!D3 && NT <= 8 && pgk::interp_lds_possible(G, L, AA != nullptr) && (!tn || tn->lds)) {
         const dim3 grid((unsigned)G.n_iitems * nc), block(64 * kInterpWaves);
         auto launch = [&](auto kern) {
             hipLaunchKernelGGL(kern, grid, block, 0, s, G.iitems, G.irec, G.ity, G.ipadl, grids,
@@ -810,7 +810,9 @@ void spread_dispatch(const pg::Gnufft& G, const float2* in, long long in_cstride
     if (!D3) {
         const int nch = (B + 63) / 64;
         const int npy = g.Gy / pg::kPatch;
-        if (B % 4 == 0 && pgk::g_spread_mfma.load(std::memory_order_relaxed))
+        const pg::Gnufft::Tuned* tn = G.tuned_for(B);
+        if (B % 4 == 0 &&
+            (tn ? tn->mfma != 0 : pgk::g_spread_mfma.load(std::memory_order_relaxed) != 0))
             hipLaunchKernelGGL(k_spread_patch_mfma, dim3(G.n_sblocks, nch), dim3(64 * kSpreadWaves),
                                0, s, G.witems, G.bgs, G.entries, (const float4*)G.ent_w, npy, G.vals,
                                B, grids, g.G(), g.Gy, G.partial, G.n_slots);
@@ -835,6 +837,13 @@ void spread_dispatch(const pg::Gnufft& G, const float2* in, long long in_cstride
 namespace pgk {
 
 std::atomic<int> g_spread_mfma{0};
+
+bool interp_lds_possible(const pg::Gnufft& G, int L, bool aa)
+{
+    const pg::Geometry& g = G.geo;
+    return g.dims == 2 && g.ntaps <= 8 && aa && L >= 2 && L <= 16 && G.n_iitems > 0 &&
+           (long long)L * g.G() < (1LL << 31);
+}
 
 void launch_interp(const pg::Gnufft& G, const float2* grids, int nc, int L, const float* AA, const float* AAi,
                    float2* out, long long out_cstride, hipStream_t s)

@@ -131,6 +131,9 @@ void launch_spread(const pg::Gnufft& G, const float2* in, long long in_cstride, 
                    float2* grids, bool vals_ready, hipStream_t s);
 // whether launch_spread reads per-sample values (all but the 3-D atomic path)
 bool spread_uses_values(const pg::Gnufft& G, int B);
+// whether the LDS-staged interpolation serves L segments of plan G (the
+// gather kernel does otherwise)
+bool interp_lds_possible(const pg::Gnufft& G, int L, bool aa);
 // the per-sample values G.vals of a spread (see launch_spread)
 void launch_spread_values(const pg::Gnufft& G, const float2* in, long long in_cstride, int nc,
                           int L, const float* AA, const float* AAi, const float2* resid_y, const double2* ax64,
