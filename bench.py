@@ -446,6 +446,53 @@ def autotune_rate(P, prob, yd, stream, niter, steps=10):
                 spread="matrix cores" if mfma == 1 else "vector units")
 
 
+def batch_rates(P, S, stream, K=24, niter=20, reps=3, seed=21):
+    """Uneven reconstructions in one graph launch (pg_recon_solve_batch): K
+    slices with matrix sizes from {96, 128, 192}^2, 8/16/32 coils, L in
+    {4, 8, 12} and spirals of 16 interleaves x 512..4096 samples (8k-64k),
+    each solved for niter PCG iterations, against the same solves run one
+    after another.  k-space from the device forward operator plus 40 dB
+    noise, scaled to unit peak."""
+    rng = np.random.default_rng(seed)
+    items, keep, grids = [], [], 0
+    for k in range(K):
+        N = int(rng.choice([96, 128, 192]))
+        nc = int(rng.choice([8, 16, 32]))
+        L = int(rng.choice([4, 8, 12]))
+        ns = int(rng.choice([512, 1024, 2048, 4096]))
+        pr = S.make_problem(N, 16, ns, nc, L, peak_hz=float(rng.uniform(50, 250)), seed=seed + k)
+        M = len(pr.kx)
+        G = P.Gnufft(M, 2.0, N, N, 1, pr.kx, pr.ky, pr.kz)
+        T = P.TimeSegmentation(G, pr.fm, pr.t, M, pr.N, L)
+        A = P.SENSE(T, pr.sens, M, pr.N, nc)
+        y = S.add_noise(A * pr.x_true, 40.0, seed=seed + 100 + k)
+        y = P.DeviceArray.from_host((y / np.abs(y).max()).astype(np.complex64))
+        items.append((A, y, P.QuadPenalty(N, N, 1, 1e4)))
+        keep.append((G, T))
+        grids += nc * L
+    P.recon_solve_batch(items, niter, stream=stream)  # capture
+    for A, y, R in items:
+        P.reconSolve(y, A, R, niter=niter, stream=stream)
+    stream.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        out = P.recon_solve_batch(items, niter, stream=stream)
+    stream.synchronize()
+    t_batch = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        for A, y, R in items:
+            P.reconSolve(y, A, R, niter=niter, stream=stream)
+    stream.synchronize()
+    t_seq = (time.perf_counter() - t0) / reps
+    its = sum(info["iterations"] for _, info in out)
+    return dict(workload=f"{K} slices, 96-192^2, 8-32 coils, L 4-12, 8k-64k samples, "
+                         f"{niter} PCG iterations each ({grids} grids in all)",
+                ms_one_launch=round(1e3 * t_batch, 2), ms_one_by_one=round(1e3 * t_seq, 2),
+                cg_iters_per_s=round(its / t_batch, 1),
+                speedup=round(t_seq / t_batch, 3))
+
+
 def pmc_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed FETCH_SIZE/WRITE_SIZE
     summary (profiles/sense2d_pmc_traffic.md, counters scaled by a device copy
@@ -482,6 +529,8 @@ def main():
     ap.add_argument("--no-gdft", action="store_true", help="skip the Gdft roofline")
     ap.add_argument("--no-direct", action="store_true",
                     help="skip the density-compensation / gridded-coil-image timings")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="skip the uneven one-launch batch measurement")
     ap.add_argument("--no-serve", action="store_true",
                     help="skip the serving-loop measurement")
     ap.add_argument("--no-rows", action="store_true",
@@ -628,6 +677,9 @@ def main():
     if not args.no_direct:
         out["direct_recon"] = direct_recon_rates(P, prob, stream)
         log(f"[bench] direct recon {out['direct_recon']}")
+    if not args.no_batch:
+        out["uneven_batch"] = batch_rates(P, S, stream)
+        log(f"[bench] uneven batch {out['uneven_batch']}")
     if not args.no_serve:
         out["serving"] = serving_rates(P, S)
         log(f"[bench] serving {out['serving']}")

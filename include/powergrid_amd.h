@@ -295,6 +295,17 @@ int pg_solve_pwls_pcg_tol(pg_op A, pg_penalty R, const void* x0_dev, const float
                           double* resid, pg_cg_info* info, void* stream);
 int pg_recon_solve_tol(pg_op A, pg_penalty R, const void* y_dev, size_t niter, double tol,
                        int mode, void* x_out_dev, double* resid, pg_cg_info* info, void* stream);
+/* (no counterpart in the reference, which reconstructs one slice at a time)
+ * K independent reconstructions -- reconSolve on A[k], R[k] (may be null),
+ * y[k], each of its own size -- as one graph launch on `stream` (required):
+ * every solve's captured graph becomes a child of one parent graph with no
+ * edges between them, so the device runs them concurrently, each with its
+ * own CG scalars.  Stops as pg_recon_solve_tol; infos (may be null) receives
+ * K results.  The K operators must not share operator objects (including
+ * inner Gnufft/TimeSegmentation plans): their workspaces are used at once. */
+int pg_recon_solve_batch(size_t K, const pg_op* A, const pg_penalty* R, const void* const* y_dev,
+                         size_t niter, double tol, void* const* x_out_dev, pg_cg_info* infos,
+                         void* stream);
 
 /* ---- tuning (no counterpart in the reference) ----------------------------
  * "spread_mfma" 1: the 2-D batched spread runs on the matrix cores when the

@@ -509,6 +509,37 @@ def reconSolve(data, Sg, R, kx=None, ky=None, kz=None, Nx=None, Ny=None, Nz=None
     return solve_pwls_pcg(None, Sg, None, data, R, niter, mode, stream, tol)
 
 
+def recon_solve_batch(items, niter, tol=0.0, stream=None):
+    """K independent reconstructions (reconSolve each) as one graph launch
+    (pg_recon_solve_batch): items is a list of (A, y, R) with R possibly None
+    and y a numpy array or DeviceArray; the operators must not share inner
+    plans.  Returns a list of (x, info dict), x of y's type."""
+    K = len(items)
+    own = stream is None
+    st = Stream() if own else stream
+    ys, xs, keep = [], [], []
+    for A, y, R in items:
+        yd = _as_device(y, np.complex64)
+        keep.append(yd)
+        ys.append(yd)
+        xs.append(DeviceArray(A.n_in, np.complex64))
+    Ah = (C.c_void_p * K)(*[A.handle for A, _, _ in items])
+    Rh = (C.c_void_p * K)(*[(R.handle if R is not None else None) for _, _, R in items])
+    yh = (C.c_void_p * K)(*[d.ptr for d in ys])
+    xh = (C.c_void_p * K)(*[d.ptr for d in xs])
+    infos = (CgInfo * K)()
+    check(lib().pg_recon_solve_batch(K, Ah, Rh, yh, int(niter), float(tol), xh, infos,
+                                     _sh(st)))
+    out = []
+    for k, (A, y, R) in enumerate(items):
+        res = dict(iterations=infos[k].iterations,
+                   exit_reason=EXIT_REASONS.get(infos[k].exit_reason, "?"))
+        x = xs[k] if isinstance(y, DeviceArray) else xs[k].to_host(st)
+        out.append((x, res))
+    st.synchronize()
+    return out
+
+
 class profile:
     """Per-kernel timing from HIP events recorded by the library."""
 
@@ -530,6 +561,6 @@ class profile:
 __all__ = ["DeviceArray", "Stream", "Gnufft", "Gdft", "GdftR2", "TimeSegmentation", "SENSE",
            "PcSenseTimeSeg", "PcSenseRank", "PcSENSE", "QuadPenalty", "TVPenalty", "calc3DDensityCompensation",
            "grid_coil_images", "calcSumOfSquaresImage", "set_tuning", "op_tuning",
-           "solve_pwls_pcg",
+           "solve_pwls_pcg", "recon_solve_batch",
            "reconSolve", "profile", "PgError", "MODE_BATCHED", "MODE_PLAIN", "CG_GRAPH",
            "CG_HOST"]

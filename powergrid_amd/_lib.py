@@ -79,6 +79,8 @@ _SIGS = {
     "pg_solve_pwls_pcg_tol": (i32, [vp, vp, vp, vp, vp, sz, C.c_double, i32, vp, dp,
                                     C.POINTER(CgInfo), vp]),
     "pg_recon_solve_tol": (i32, [vp, vp, vp, sz, C.c_double, i32, vp, dp, C.POINTER(CgInfo), vp]),
+    "pg_recon_solve_batch": (i32, [sz, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp), sz,
+                                   C.c_double, C.POINTER(vp), C.POINTER(CgInfo), vp]),
     "pg_set_tuning": (i32, [C.c_char_p, i32]),
     "pg_profile_enable": (i32, [i32]),
     "pg_profile_reset": (i32, []),

@@ -1447,6 +1447,7 @@ struct SolveCache {
     size_t niter = 0;
     double tol = 0.0;
     uint64_t ws_gen = 0;
+    uint64_t uid = 0;  // unique per instantiated graph (0: none)
     CgWork w;
     hipStream_t own = nullptr;  // stream for graph solves without a user stream
     hipGraph_t graph = nullptr;
@@ -1521,19 +1522,16 @@ static void cg_setup(Op* A, const float2* y, const float* W, CgWork& w, const Fu
                                   s);
 }
 
-static void solve_pcg(Op* A, Penalty* R, const float2* x0, const float* W, const float2* y,
-                      size_t niter, double tol, int mode, float2* x_out, double* resid_h,
-                      pg_cg_info* info, hipStream_t user_stream)
+// the solve cache of operator A for this key (R, W, y, niter, tol), made
+// anew when the key changed; its graph is dropped when a workspace moved
+static SolveCache* solve_cache(Op* A, Penalty* R, const float* W, const float2* y, size_t niter,
+                               double tol)
 {
     const long long N = (long long)A->n_in, M = (long long)A->n_out;
-    Fused fz;
-    const bool fused = fusable(A, fz);
     op_prepare(A);  // may grow workspaces (and bump g_ws_gen)
-
     pgv::PenSpec spec{};
     if (R) spec = R->spec;
     SolveCache* c = A->scache;
-    if (!(tol > 0.0)) tol = 0.0;
     const bool same = c && c->R == R && (!R || same_spec(c->spec, spec)) && c->y == y && c->W == W &&
                       c->niter == niter && c->tol == tol;
     if (!same) {
@@ -1562,10 +1560,69 @@ static void solve_pcg(Op* A, Penalty* R, const float2* x0, const float* W, const
         w.resid = dev_alloc<double>(std::max<size_t>(niter, 1));
         w.sc = dev_alloc<CgScal>(1);
     }
-    CgWork& w = c->w;
     const uint64_t gen = g_ws_gen.load();
     if (c->ws_gen != gen) c->drop_graph();
     c->ws_gen = gen;
+    return c;
+}
+
+// capture the whole solve (set-up, every iteration, bookkeeping) on s into
+// c->graph and instantiate it, unless that is done; every kernel checks the
+// device-side stop flag, so an early exit leaves the remaining iterations as
+// no-ops
+static void solve_capture(SolveCache* c, Op* A, Penalty* R, const float* W, const float2* y,
+                          const Fused* f, hipStream_t s)
+{
+    if (c->exec) return;
+    CgWork& w = c->w;
+    LegacyLock lk(g_legacy_mu);
+    PG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    try {
+        cg_setup(A, y, W, w, f, c->tol, s);
+        for (size_t it = 0; it < c->niter; ++it) cg_iteration(A, R, y, W, w, s, f);
+        pgv::cg_finish(w.cp, w.sc, w.resid, s);
+    } catch (...) {
+        hipGraph_t g = nullptr;
+        hipStreamEndCapture(s, &g);
+        if (g) hipGraphDestroy(g);
+        throw;
+    }
+    PG_HIP(hipStreamEndCapture(s, &c->graph));
+    PG_HIP(hipGraphInstantiate(&c->exec, c->graph, nullptr, nullptr, 0));
+    static std::atomic<uint64_t> next_uid{1};
+    c->uid = next_uid.fetch_add(1);
+}
+
+// the result of a finished solve: x_out, the residual history and info
+static void solve_results(SolveCache* c, long long N, float2* x_out, double* resid_h,
+                          pg_cg_info* info, hipStream_t s)
+{
+    CgWork& w = c->w;
+    CgScal hs{};
+    PG_HIP(hipMemcpyAsync(&hs, w.sc, sizeof(CgScal), hipMemcpyDeviceToHost, s));
+    PG_HIP(hipMemcpyAsync(x_out, w.x32, sizeof(float2) * N, hipMemcpyDeviceToDevice, s));
+    std::vector<double> rh(std::max<size_t>(c->niter, 1), 0.0);
+    PG_HIP(hipMemcpyAsync(rh.data(), w.resid, sizeof(double) * rh.size(), hipMemcpyDeviceToHost, s));
+    PG_HIP(hipStreamSynchronize(s));
+    if (resid_h)
+        for (int i = 0; i < hs.iter && (size_t)i < c->niter; ++i) resid_h[i] = rh[i];
+    if (info) {
+        info->iterations = hs.iter;
+        info->exit_reason = hs.stop;
+        info->final_resid = hs.iter > 0 ? rh[hs.iter - 1] : 0.0;
+    }
+}
+
+static void solve_pcg(Op* A, Penalty* R, const float2* x0, const float* W, const float2* y,
+                      size_t niter, double tol, int mode, float2* x_out, double* resid_h,
+                      pg_cg_info* info, hipStream_t user_stream)
+{
+    const long long N = (long long)A->n_in;
+    Fused fz;
+    const bool fused = fusable(A, fz);
+    if (!(tol > 0.0)) tol = 0.0;
+    SolveCache* c = solve_cache(A, R, W, y, niter, tol);
+    CgWork& w = c->w;
 
     hipStream_t s = user_stream;
     if (mode == PG_CG_GRAPH && s == nullptr) {
@@ -1579,54 +1636,115 @@ static void solve_pcg(Op* A, Penalty* R, const float2* x0, const float* W, const
     if (x0) PG_HIP(hipMemcpyAsync(w.x32, x0, sizeof(float2) * N, hipMemcpyDeviceToDevice, s));
     else pgv::fill(w.x32, N, make_float2(0.f, 0.f), s);
 
-    CgScal hs{};
     if (mode == PG_CG_GRAPH) {
-        // the whole solve (set-up, every iteration, bookkeeping) is one graph;
-        // every kernel checks the device-side stop flag, so an early exit
-        // leaves the remaining iterations as no-ops
-        if (!c->exec) {
-            LegacyLock lk(g_legacy_mu);
-            PG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-            try {
-                cg_setup(A, y, W, w, fused ? &fz : nullptr, tol, s);
-                for (size_t it = 0; it < niter; ++it)
-                    cg_iteration(A, R, y, W, w, s, fused ? &fz : nullptr);
-                pgv::cg_finish(w.cp, w.sc, w.resid, s);
-            } catch (...) {
-                hipGraph_t g = nullptr;
-                hipStreamEndCapture(s, &g);
-                if (g) hipGraphDestroy(g);
-                throw;
-            }
-            PG_HIP(hipStreamEndCapture(s, &c->graph));
-            PG_HIP(hipGraphInstantiate(&c->exec, c->graph, nullptr, nullptr, 0));
-        }
+        solve_capture(c, A, R, W, y, fused ? &fz : nullptr, s);
         PG_HIP(hipGraphLaunch(c->exec, s));
-        PG_HIP(hipMemcpyAsync(&hs, w.sc, sizeof(CgScal), hipMemcpyDeviceToHost, s));
-        PG_HIP(hipStreamSynchronize(s));
     } else {
         cg_setup(A, y, W, w, fused ? &fz : nullptr, tol, s);
         for (size_t it = 0; it < niter; ++it) {
             cg_iteration(A, R, y, W, w, s, fused ? &fz : nullptr);
             // host-side branch on the device scalars, as the reference loop does
+            CgScal hs{};
             PG_HIP(hipMemcpyAsync(&hs, w.sc, sizeof(CgScal), hipMemcpyDeviceToHost, s));
             PG_HIP(hipStreamSynchronize(s));
             if (hs.stop) break;
         }
         pgv::cg_finish(w.cp, w.sc, w.resid, s);
-        PG_HIP(hipMemcpyAsync(&hs, w.sc, sizeof(CgScal), hipMemcpyDeviceToHost, s));
-        PG_HIP(hipStreamSynchronize(s));
     }
-    PG_HIP(hipMemcpyAsync(x_out, w.x32, sizeof(float2) * N, hipMemcpyDeviceToDevice, s));
-    std::vector<double> rh(std::max<size_t>(niter, 1), 0.0);
-    PG_HIP(hipMemcpyAsync(rh.data(), w.resid, sizeof(double) * rh.size(), hipMemcpyDeviceToHost, s));
-    PG_HIP(hipStreamSynchronize(s));
-    if (resid_h)
-        for (int i = 0; i < hs.iter && (size_t)i < niter; ++i) resid_h[i] = rh[i];
-    if (info) {
-        info->iterations = hs.iter;
-        info->exit_reason = hs.stop;
-        info->final_resid = hs.iter > 0 ? rh[hs.iter - 1] : 0.0;
+    solve_results(c, N, x_out, resid_h, info, s);
+}
+
+// every operator object under A (the composite and its inner operators)
+static void collect_ops(Op* A, std::vector<const Op*>& out)
+{
+    if (!A) return;
+    out.push_back(A);
+    switch (A->kind) {
+    case K_TS:
+        collect_ops(static_cast<TsOp*>(A)->inner, out);
+        break;
+    case K_SENSE:
+        collect_ops(static_cast<SenseOp*>(A)->inner, out);
+        break;
+    case K_PCSENSE: {
+        PcSenseOp* pc = static_cast<PcSenseOp*>(A);
+        for (Op* o : pc->A) collect_ops(o, out);
+        break;
+    }
+    case K_PCSENSE_DFT: {
+        PcDftOp* pc = static_cast<PcDftOp*>(A);
+        for (DftOp* d : pc->D) out.push_back(d);
+        break;
+    }
+    default:
+        break;
+    }
+}
+
+// K independent reconstructions (reconSolve each: W = ones, x0 = 0) as one
+// graph launch: each solve's captured graph is a child of one parent graph,
+// with no edges between them, so the device runs them concurrently
+static void solve_batch(size_t K, Op* const* A, Penalty* const* R, const float2* const* y,
+                        size_t niter, double tol, float2* const* x_out, pg_cg_info* infos,
+                        hipStream_t s)
+{
+    if (!(tol > 0.0)) tol = 0.0;
+    std::vector<SolveCache*> cs(K);
+    for (size_t k = 0; k < K; ++k) {
+        Fused fz;
+        const bool fused = fusable(A[k], fz);
+        cs[k] = solve_cache(A[k], R[k], nullptr, y[k], niter, tol);
+        if (fused) maybe_autotune(fz, s);
+        solve_capture(cs[k], A[k], R[k], nullptr, y[k], fused ? &fz : nullptr, s);
+    }
+    for (size_t k = 0; k < K; ++k)
+        pgv::fill(cs[k]->w.x32, (long long)A[k]->n_in, make_float2(0.f, 0.f), s);
+    // the parent graph of these K solve graphs, from a small cache keyed by
+    // the children's unique ids (a recaptured or destroyed solve never
+    // matches a stale entry)
+    struct Parent {
+        std::vector<uint64_t> key;
+        hipGraph_t g = nullptr;
+        hipGraphExec_t e = nullptr;
+    };
+    static std::mutex mu;
+    static std::vector<Parent>* cache = new std::vector<Parent>();  // kept for the process
+    std::vector<uint64_t> key(K);
+    for (size_t k = 0; k < K; ++k) key[k] = cs[k]->uid;
+    hipGraphExec_t exec = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        for (const Parent& p : *cache)
+            if (p.key == key) exec = p.e;
+        if (!exec) {
+            LegacyLock ll(g_legacy_mu);
+            Parent p;
+            p.key = key;
+            PG_HIP(hipGraphCreate(&p.g, 0));
+            try {
+                for (size_t k = 0; k < K; ++k) {
+                    hipGraphNode_t node;
+                    PG_HIP(hipGraphAddChildGraphNode(&node, p.g, nullptr, 0, cs[k]->graph));
+                }
+                PG_HIP(hipGraphInstantiate(&p.e, p.g, nullptr, nullptr, 0));
+            } catch (...) {
+                hipGraphDestroy(p.g);
+                throw;
+            }
+            if (cache->size() >= 4) {
+                hipGraphExecDestroy(cache->front().e);
+                hipGraphDestroy(cache->front().g);
+                cache->erase(cache->begin());
+            }
+            cache->push_back(p);
+            exec = p.e;
+        }
+        // launched and read back under the cache lock, so that no other
+        // thread evicts this graph while it runs
+        PG_HIP(hipGraphLaunch(exec, s));
+        for (size_t k = 0; k < K; ++k)
+            solve_results(cs[k], (long long)A[k]->n_in, x_out[k], nullptr,
+                          infos ? infos + k : nullptr, s);
     }
 }
 
@@ -2763,6 +2881,39 @@ int pg_recon_solve_tol(pg_op A, pg_penalty R, const void* y, size_t niter, doubl
 {
     return pg_solve_pwls_pcg_tol(A, R, nullptr, nullptr, y, niter, tol, mode, x_out, resid, info,
                                  stream);
+}
+
+int pg_recon_solve_batch(size_t K, const pg_op* A, const pg_penalty* R, const void* const* y,
+                         size_t niter, double tol, void* const* x_out, pg_cg_info* infos,
+                         void* stream)
+{
+    PG_TRY PG_REQUIRE(K > 0 && A && R && y && x_out, PG_ERR_ARG, "null argument");
+    PG_REQUIRE(stream, PG_ERR_ARG, "a batch runs on a stream of the caller (pg_stream_create)");
+    std::vector<Op*> ops(K);
+    std::vector<Penalty*> pens(K);
+    std::vector<const float2*> ys(K);
+    std::vector<float2*> xs(K);
+    std::vector<const Op*> seen;
+    for (size_t k = 0; k < K; ++k) {
+        ops[k] = as_op(A[k]);
+        pens[k] = R[k] ? as_pen(R[k]) : nullptr;
+        PG_REQUIRE(y[k] && x_out[k], PG_ERR_ARG, "null y or x_out");
+        if (pens[k])
+            PG_REQUIRE((size_t)pens[k]->spec.n == ops[k]->n_in, PG_ERR_ARG,
+                       "penalty size != image size");
+        std::vector<const Op*> mine;
+        collect_ops(ops[k], mine);
+        for (const Op* o : mine)
+            PG_REQUIRE(std::find(seen.begin(), seen.end(), o) == seen.end(), PG_ERR_ARG,
+                       "the reconstructions of a batch must not share operators (their "
+                       "workspaces run concurrently)");
+        seen.insert(seen.end(), mine.begin(), mine.end());
+        ys[k] = (const float2*)y[k];
+        xs[k] = (float2*)x_out[k];
+    }
+    solve_batch(K, ops.data(), pens.data(), ys.data(), niter, tol, xs.data(), infos,
+                (hipStream_t)stream);
+    PG_CATCH
 }
 
 int pg_op_tuning(pg_op h, int B, int* interp_lds, int* spread_mfma)

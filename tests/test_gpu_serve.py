@@ -110,3 +110,29 @@ def test_pcg_tolerance_exit(orc):
     xprev, _ = P.reconSolve(y, A, R, niter=k - 1)
     assert gnorm(xk) <= tol * g0 * 1.05
     assert gnorm(xprev) > tol * g0 * 0.95
+
+
+def test_recon_batch_one_launch_matches_single_solves():
+    # uneven reconstructions (matrix size, coils, segments, a 3-D stack) as
+    # one graph launch: each image equals its own solve bit for bit
+    print("[recon batch]", flush=True)
+    reqs = mix()[:4]
+    items, singles = [], []
+    for r in reqs:
+        M, N = r.M, r.N
+        G = P.Gnufft(M, 2.0, r.Nx, r.Ny, r.Nz, r.kx, r.ky, r.kz)
+        T = P.TimeSegmentation(G, r.fieldmap, r.tvec, M, N, r.L)
+        A = P.SENSE(T, r.sens, M, N, r.nc)
+        R = P.QuadPenalty(r.Nx, r.Ny, r.Nz, r.beta)
+        items.append((A, r.data, R))
+        singles.append((G, T))
+    st = P.Stream()
+    out = P.recon_solve_batch(items, niter=12, tol=1e-4, stream=st)
+    for (A, y, R), (x, info) in zip(items, out):
+        xs, i1 = P.reconSolve(y, A, R, niter=12, stream=st, tol=1e-4)
+        assert info["iterations"] == i1["iterations"] and info["exit_reason"] == i1["exit_reason"]
+        np.testing.assert_array_equal(x, xs)
+    # operators that share a plan cannot run at once
+    A0, y0, R0 = items[0]
+    with pytest.raises(P.PgError):
+        P.recon_solve_batch([(A0, y0, R0), (A0, y0, R0)], niter=2, stream=st)
