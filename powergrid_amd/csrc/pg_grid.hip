@@ -41,8 +41,9 @@ __device__ __forceinline__ int clampi(int v, int lo, int hi) { return min(max(v,
 template <int NT, bool D3>
 __global__ void __launch_bounds__(256) k_interp(
     const int* __restrict__ lo, const float* __restrict__ w, int M, int Gx, int Gy, int Gz,
-    const float2* __restrict__ grids, long long G, int nc, int L, const float* __restrict__ AA, const float* __restrict__ AAi,
-    float2* __restrict__ out, long long out_cstride, int nsb, int xcd_map)
+    const float2* __restrict__ grids, long long G, int nc, int L, const float* __restrict__ AA,
+    const float* __restrict__ AAi, float2* __restrict__ out, long long out_cstride, int nsb,
+    int xcd_map)
 {
     int c, sb;
     const int p = blockIdx.x;
@@ -167,7 +168,8 @@ template <int LP, bool CPX>
 __global__ void __launch_bounds__(64 * kInterpWaves) k_interp_lds(
     const int4* __restrict__ items, const float4* __restrict__ rec, int ity, int padl,
     const float2* __restrict__ grids, long long G, int Gx, int Gy, int nc, int L,
-    const float* __restrict__ AA, const float* __restrict__ AAi, int M, float2* __restrict__ out, long long out_cstride)
+    const float* __restrict__ AA, const float* __restrict__ AAi, int M, float2* __restrict__ out,
+    long long out_cstride)
 {
     constexpr int R = pg::kIR;
     constexpr int CH = pg::kIChunk;
@@ -300,7 +302,8 @@ __device__ __forceinline__ float2 spread_value(const float2* __restrict__ in, lo
                                                const float2* __restrict__ resid_y,
                                                const double2* __restrict__ ax64,
                                                const float* __restrict__ Wd,
-                                               const float* __restrict__ AA, const float* __restrict__ AAi, int L, int M, int b,
+                                               const float* __restrict__ AA,
+                                               const float* __restrict__ AAi, int L, int M, int b,
                                                int i)
 {
     const int c = b / L, l = b - c * L;
@@ -341,12 +344,14 @@ __device__ __forceinline__ float2 spread_value(const float2* __restrict__ in, lo
 __global__ void __launch_bounds__(256) k_spread_values(
     const float2* __restrict__ in, long long in_cstride, const float2* __restrict__ resid_y,
     const double2* __restrict__ ax64, const float* __restrict__ Wd, int L,
-    const float* __restrict__ AA, const float* __restrict__ AAi, int M, int B, float2* __restrict__ V)
+    const float* __restrict__ AA, const float* __restrict__ AAi, int M, int B,
+    float2* __restrict__ V)
 {
     const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (i >= M) return;
     for (int b = threadIdx.x & 63; b < B; b += 64)
-        V[(long long)i * B + b] = spread_value(in, in_cstride, resid_y, ax64, Wd, AA, AAi, L, M, b, i);
+        V[(long long)i * B + b] =
+            spread_value(in, in_cstride, resid_y, ax64, Wd, AA, AAi, L, M, b, i);
 }
 
 // the same for an even grid count, two grids per thread (16-byte stores;
@@ -354,7 +359,8 @@ __global__ void __launch_bounds__(256) k_spread_values(
 __global__ void __launch_bounds__(256) k_spread_values2(
     const float2* __restrict__ in, long long in_cstride, const float2* __restrict__ resid_y,
     const double2* __restrict__ ax64, const float* __restrict__ Wd, int L,
-    const float* __restrict__ AA, const float* __restrict__ AAi, int M, int B, float2* __restrict__ V)
+    const float* __restrict__ AA, const float* __restrict__ AAi, int M, int B,
+    float2* __restrict__ V)
 {
     const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const int hb = B >> 1;
@@ -367,8 +373,8 @@ __global__ void __launch_bounds__(256) k_spread_values2(
 }
 
 void launch_values(const float2* in, long long in_cstride, const float2* resid_y,
-                   const double2* ax64, const float* Wd, int L, const float* AA, const float* AAi, int M, int B,
-                   float2* V, hipStream_t s)
+                   const double2* ax64, const float* Wd, int L, const float* AA, const float* AAi,
+                   int M, int B, float2* V, hipStream_t s)
 {
     if (B % 2 == 0) {
         const long long n = (long long)M * (B / 2);
@@ -737,7 +743,8 @@ __global__ void __launch_bounds__(256) k_spread_atomic(
     const int* __restrict__ lo, const float* __restrict__ w, int M, int Gx, int Gy, int Gz,
     const float2* __restrict__ in, long long in_cstride, const float2* __restrict__ resid_y,
     const double2* __restrict__ ax64, const float* __restrict__ Wd, int L,
-    const float* __restrict__ AA, const float* __restrict__ AAi, int B, float2* __restrict__ grids, long long G)
+    const float* __restrict__ AA, const float* __restrict__ AAi, int B, float2* __restrict__ grids,
+    long long G)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int b = blockIdx.y;
@@ -767,8 +774,8 @@ __global__ void __launch_bounds__(256) k_spread_atomic(
 }
 
 template <int NT, bool D3>
-void interp_dispatch(const pg::Gnufft& G, const float2* grids, int nc, int L, const float* AA, const float* AAi,
-                     float2* out, long long out_cstride, hipStream_t s)
+void interp_dispatch(const pg::Gnufft& G, const float2* grids, int nc, int L, const float* AA,
+                     const float* AAi, float2* out, long long out_cstride, hipStream_t s)
 {
     const pg::Geometry& g = G.geo;
     const pg::Gnufft::Tuned* tn = G.tuned_for(nc * L);
@@ -792,8 +799,8 @@ void interp_dispatch(const pg::Gnufft& G, const float2* grids, int nc, int L, co
 
 template <int NT, bool D3>
 void spread_dispatch(const pg::Gnufft& G, const float2* in, long long in_cstride, int nc, int L,
-                     const float* AA, const float* AAi, const float2* resid_y, const double2* ax64, const float* Wd,
-                     float2* grids, bool vals_ready, hipStream_t s)
+                     const float* AA, const float* AAi, const float2* resid_y, const double2* ax64,
+                     const float* Wd, float2* grids, bool vals_ready, hipStream_t s)
 {
     const pg::Geometry& g = G.geo;
     const int B = nc * L;
@@ -806,7 +813,8 @@ void spread_dispatch(const pg::Gnufft& G, const float2* in, long long in_cstride
         return;
     }
     // per-sample values of every grid, once
-    if (!vals_ready) launch_values(in, in_cstride, resid_y, ax64, Wd, L, AA, AAi, G.M, B, G.vals, s);
+    if (!vals_ready)
+        launch_values(in, in_cstride, resid_y, ax64, Wd, L, AA, AAi, G.M, B, G.vals, s);
     if (!D3) {
         const int nch = (B + 63) / 64;
         const int npy = g.Gy / pg::kPatch;
@@ -845,8 +853,8 @@ bool interp_lds_possible(const pg::Gnufft& G, int L, bool aa)
            (long long)L * g.G() < (1LL << 31);
 }
 
-void launch_interp(const pg::Gnufft& G, const float2* grids, int nc, int L, const float* AA, const float* AAi,
-                   float2* out, long long out_cstride, hipStream_t s)
+void launch_interp(const pg::Gnufft& G, const float2* grids, int nc, int L, const float* AA,
+                   const float* AAi, float2* out, long long out_cstride, hipStream_t s)
 {
     const bool d3 = G.geo.dims == 3;
     switch (G.geo.ntaps) {
@@ -870,29 +878,35 @@ void launch_interp(const pg::Gnufft& G, const float2* grids, int nc, int L, cons
 bool spread_uses_values(const pg::Gnufft& G, int B) { return !(G.geo.dims == 3 && B < 16); }
 
 void launch_spread_values(const pg::Gnufft& G, const float2* in, long long in_cstride, int nc,
-                          int L, const float* AA, const float* AAi, const float2* resid_y, const double2* ax64,
-                          const float* Wd, hipStream_t s)
+                          int L, const float* AA, const float* AAi, const float2* resid_y,
+                          const double2* ax64, const float* Wd, hipStream_t s)
 {
     launch_values(in, in_cstride, resid_y, ax64, Wd, L, AA, AAi, G.M, nc * L, G.vals, s);
 }
 
 void launch_spread(const pg::Gnufft& G, const float2* in, long long in_cstride, int nc, int L,
-                   const float* AA, const float* AAi, const float2* resid_y, const double2* ax64, const float* Wd,
-                   float2* grids, bool vals_ready, hipStream_t s)
+                   const float* AA, const float* AAi, const float2* resid_y, const double2* ax64,
+                   const float* Wd, float2* grids, bool vals_ready, hipStream_t s)
 {
     const bool d3 = G.geo.dims == 3;
     switch (G.geo.ntaps) {
     case 8:
-        d3 ? spread_dispatch<8, true>(G, in, in_cstride, nc, L, AA, AAi, resid_y, ax64, Wd, grids, vals_ready, s)
-           : spread_dispatch<8, false>(G, in, in_cstride, nc, L, AA, AAi, resid_y, ax64, Wd, grids, vals_ready, s);
+        d3 ? spread_dispatch<8, true>(G, in, in_cstride, nc, L, AA, AAi, resid_y, ax64, Wd,
+                                      grids, vals_ready, s)
+           : spread_dispatch<8, false>(G, in, in_cstride, nc, L, AA, AAi, resid_y, ax64, Wd,
+                                       grids, vals_ready, s);
         break;
     case 6:
-        d3 ? spread_dispatch<6, true>(G, in, in_cstride, nc, L, AA, AAi, resid_y, ax64, Wd, grids, vals_ready, s)
-           : spread_dispatch<6, false>(G, in, in_cstride, nc, L, AA, AAi, resid_y, ax64, Wd, grids, vals_ready, s);
+        d3 ? spread_dispatch<6, true>(G, in, in_cstride, nc, L, AA, AAi, resid_y, ax64, Wd,
+                                      grids, vals_ready, s)
+           : spread_dispatch<6, false>(G, in, in_cstride, nc, L, AA, AAi, resid_y, ax64, Wd,
+                                       grids, vals_ready, s);
         break;
     case 12:
-        d3 ? spread_dispatch<12, true>(G, in, in_cstride, nc, L, AA, AAi, resid_y, ax64, Wd, grids, vals_ready, s)
-           : spread_dispatch<12, false>(G, in, in_cstride, nc, L, AA, AAi, resid_y, ax64, Wd, grids, vals_ready, s);
+        d3 ? spread_dispatch<12, true>(G, in, in_cstride, nc, L, AA, AAi, resid_y, ax64, Wd,
+                                       grids, vals_ready, s)
+           : spread_dispatch<12, false>(G, in, in_cstride, nc, L, AA, AAi, resid_y, ax64, Wd,
+                                        grids, vals_ready, s);
         break;
     default:
         throw pg::Error{PG_ERR_UNSUPPORTED, "kernel width * gridos must give 6, 8 or 12 taps"};

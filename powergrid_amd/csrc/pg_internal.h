@@ -120,15 +120,15 @@ void launch_fwd_prep_fft(const pg::Gnufft& G, const float2* img, long long img_c
                          WeightArgs w, int nc, int L, float2* tbuf, float2* tbuf2, float2* grids,
                          hipStream_t s);
 // forward: batched grids -> samples (KB interpolation, sum over segments)
-void launch_interp(const pg::Gnufft& G, const float2* grids, int nc, int L, const float* AA, const float* AAi,
-                   float2* out, long long out_cstride, hipStream_t s);
+void launch_interp(const pg::Gnufft& G, const float2* grids, int nc, int L, const float* AA,
+                   const float* AAi, float2* out, long long out_cstride, hipStream_t s);
 // adjoint: samples -> batched grids (KB spread, deterministic tiles)
 // (with resid_y set, the samples are W (resid_y - ax64), ax64 the FP64 Ax of
 // the CG, and `in` is unused; with vals_ready the per-sample values
 // G.vals have already been formed, by the PCG update kernel)
 void launch_spread(const pg::Gnufft& G, const float2* in, long long in_cstride, int nc, int L,
-                   const float* AA, const float* AAi, const float2* resid_y, const double2* ax64, const float* Wd,
-                   float2* grids, bool vals_ready, hipStream_t s);
+                   const float* AA, const float* AAi, const float2* resid_y, const double2* ax64,
+                   const float* Wd, float2* grids, bool vals_ready, hipStream_t s);
 // whether launch_spread reads per-sample values (all but the 3-D atomic path)
 bool spread_uses_values(const pg::Gnufft& G, int B);
 // whether the LDS-staged interpolation serves L segments of plan G (the
@@ -136,8 +136,8 @@ bool spread_uses_values(const pg::Gnufft& G, int B);
 bool interp_lds_possible(const pg::Gnufft& G, int L, bool aa);
 // the per-sample values G.vals of a spread (see launch_spread)
 void launch_spread_values(const pg::Gnufft& G, const float2* in, long long in_cstride, int nc,
-                          int L, const float* AA, const float* AAi, const float2* resid_y, const double2* ax64,
-                          const float* Wd, hipStream_t s);
+                          int L, const float* AA, const float* AAi, const float2* resid_y,
+                          const double2* ax64, const float* Wd, hipStream_t s);
 // adjoint: batched grids -> image(s) (ifftshift, IFFT, fftshift, crop,
 // deapod, x WoH, sum over segments, x conj(S))
 void launch_adj_fft_post(const pg::Gnufft& G, float2* grids, float2* tbuf, float2* tbuf2,
