@@ -900,12 +900,14 @@ struct PcDftOp : Op {
     int Ns = 1, Nc = 1;
     size_t Nd = 0, Ni = 0;
     std::vector<DftOp*> D;  // per shot (owned)
+    DftOp* all = nullptr;   // every shot as one DFT of Ns*Nd samples (owned)
     float2* SF = nullptr;
     float2* SA = nullptr;
     float2* tmp_n = nullptr;
     ~PcDftOp() override
     {
         for (DftOp* d : D) delete d;
+        delete all;
         dev_free(SF);
         dev_free(SA);
         dev_free(tmp_n);
@@ -1140,9 +1142,15 @@ static void op_prepare(Op* op)
     }
     case K_PCSENSE_DFT: {
         PcDftOp* pc = static_cast<PcDftOp*>(op);
+        if (pc->mode == PG_MODE_BATCHED) {
+            DftOp* d = pc->all;
+            d->ensure_part(std::max(pgd::part_elems(d->d.M, d->d.N, pc->Nc, false, pc->Ns),
+                                    pgd::part_elems(d->d.M, d->d.N, pc->Nc, true, pc->Ns)));
+            return;
+        }
         for (DftOp* d : pc->D)
-            d->ensure_part(std::max(pgd::part_elems(d->d.M, d->d.N, pc->Nc, false),
-                                    pgd::part_elems(d->d.M, d->d.N, pc->Nc, true)));
+            d->ensure_part(std::max(pgd::part_elems(d->d.M, d->d.N, 1, false),
+                                    pgd::part_elems(d->d.M, d->d.N, 1, true)));
         return;
     }
     case K_SENSE: {
@@ -1243,21 +1251,23 @@ static void op_forward(Op* op, const float2* in, float2* out, hipStream_t s)
         // pcSENSE.cpp:108-127
         PcDftOp* pc = static_cast<PcDftOp*>(op);
         const long long Ni = (long long)pc->Ni, Nd = (long long)pc->Nd;
+        Prof p("gdft_fwd", s);
+        if (op->mode == PG_MODE_BATCHED) {
+            // every shot and coil in one launch: column s + c*Ns of Nd samples
+            // is sample range [s*Nd, (s+1)*Nd) of coil c's Ns*Nd
+            DftOp* d = pc->all;
+            d->ensure_part(pgd::part_elems(d->d.M, d->d.N, pc->Nc, false, pc->Ns));
+            pgd::forward(d->d, in, pc->SF, pc->Nc, out, d->part, s, -1, pc->Ns, pc->Nc * Ni);
+            return;
+        }
         for (int sh = 0; sh < pc->Ns; ++sh) {
             DftOp* d = pc->D[sh];
             const float2* SFs = pc->SF + (size_t)sh * pc->Nc * Ni;
-            Prof p("gdft_fwd", s);
-            if (op->mode == PG_MODE_BATCHED) {
-                // every coil of the shot in one launch; its columns are Ns*Nd apart
-                d->ensure_part(pgd::part_elems(d->d.M, d->d.N, pc->Nc, false));
-                pgd::forward(d->d, in, SFs, pc->Nc, out + sh * Nd, d->part, s, pc->Ns * Nd);
-            } else {
-                d->ensure_part(pgd::part_elems(d->d.M, d->d.N, 1, false));
-                for (int c = 0; c < pc->Nc; ++c) {
-                    pgv::cmul(pc->tmp_n, SFs + (size_t)c * Ni, in, Ni, s);
-                    pgd::forward(d->d, pc->tmp_n, nullptr, 1,
-                                 out + ((size_t)sh + (size_t)c * pc->Ns) * Nd, d->part, s);
-                }
+            d->ensure_part(pgd::part_elems(d->d.M, d->d.N, 1, false));
+            for (int c = 0; c < pc->Nc; ++c) {
+                pgv::cmul(pc->tmp_n, SFs + (size_t)c * Ni, in, Ni, s);
+                pgd::forward(d->d, pc->tmp_n, nullptr, 1,
+                             out + ((size_t)sh + (size_t)c * pc->Ns) * Nd, d->part, s);
             }
         }
         return;
@@ -1363,13 +1373,12 @@ static void op_adjoint(Op* op, const float2* in, float2* out, hipStream_t s)
         const long long Ni = (long long)pc->Ni, Nd = (long long)pc->Nd;
         Prof p("gdft_adj", s);
         if (op->mode == PG_MODE_BATCHED) {
-            // per shot, every coil in one launch, accumulated over shots
-            for (int sh = 0; sh < pc->Ns; ++sh) {
-                DftOp* d = pc->D[sh];
-                d->ensure_part(pgd::part_elems(d->d.M, d->d.N, pc->Nc, true));
-                pgd::adjoint(d->d, in + sh * Nd, pc->SA + (size_t)sh * pc->Nc * Ni, pc->Nc, out,
-                             d->part, s, pc->Ns * Nd, sh > 0);
-            }
+            // every shot and coil in one launch; the shots' sums are added in
+            // shot order
+            DftOp* d = pc->all;
+            d->ensure_part(pgd::part_elems(d->d.M, d->d.N, pc->Nc, true, pc->Ns));
+            pgd::adjoint(d->d, in, pc->SA, pc->Nc, out, d->part, s, -1, false, pc->Ns,
+                         pc->Nc * Ni);
             return;
         }
         // the reference's order: coils outer, shots inner
@@ -1674,6 +1683,7 @@ static void collect_ops(Op* A, std::vector<const Op*>& out)
     case K_PCSENSE_DFT: {
         PcDftOp* pc = static_cast<PcDftOp*>(A);
         for (DftOp* d : pc->D) out.push_back(d);
+        out.push_back(pc->all);
         break;
     }
     default:
@@ -2442,6 +2452,16 @@ int pg_pcsense_gdft_create(pg_op* out, const float* kx, const float* ky, const f
                                       iy.data(), iz.data(), fieldmap, tvec + o, 0);
         if (st != PG_OK) throw pg::Error{st, pg::g_last_error};
         op->D.push_back(reinterpret_cast<DftOp*>(h));
+    }
+    {
+        // the same shots as one DFT (batched mode)
+        std::vector<float> zall;
+        if (!kz) zall.assign(Nd * n_shots, 0.0f);
+        pg_op h = nullptr;
+        const int st = pg_gdft_create(&h, Nd * n_shots, Ni, kx, ky, kz ? kz : zall.data(), ix.data(),
+                                      iy.data(), iz.data(), fieldmap, tvec, 0);
+        if (st != PG_OK) throw pg::Error{st, pg::g_last_error};
+        op->all = reinterpret_cast<DftOp*>(h);
     }
     std::vector<float2> sf(n_shots * nc * Ni), sa(n_shots * nc * Ni);
     for (size_t sh = 0; sh < n_shots; ++sh)

@@ -5,6 +5,10 @@
 // The phase of a (sample, pixel) pair is computed once and used for every
 // coil.  Work is split over (output block, input slice); slices are summed
 // by a second kernel in a fixed order.
+// pcSENSE's shots (pcSENSE.cpp:108-153) run as one launch: the M samples
+// are nshot shots of M/nshot, shot-major, and shot s has its own nc maps at
+// S + s * s_shot.  Forward sample tiles and adjoint sample slices never
+// straddle a shot, so a block applies one shot's maps.
 //
 // precision 0 -- the reference's arithmetic class (FP32 phase, FP32 trig):
 //   the phase is formed in revolutions, p = k.r + fm t / (2 pi), reduced to
@@ -48,20 +52,26 @@ __global__ void __launch_bounds__(kT) k_gdft_fwd(
     const float* __restrict__ kx, const float* __restrict__ ky, const float* __restrict__ kz,
     const float* __restrict__ t, const float* __restrict__ ix, const float* __restrict__ iy,
     const float* __restrict__ iz, const float* __restrict__ fm, const float2* __restrict__ x,
-    const float2* __restrict__ S, int nc, int M, int N, int slice_len, float2* __restrict__ part,
-    R2Args r2)
+    const float2* __restrict__ S, int nc, int M, int N, int slice_len, int shot_len,
+    long long s_shot, float2* __restrict__ part, R2Args r2)
 {
     __shared__ float s_r[kT][4];
     __shared__ float s_g[R2 ? kT : 1][3];
     __shared__ float2 s_v[kT][NCB];
-    const int i = blockIdx.x * kT + threadIdx.x;
+    // ceil(shot_len / kT) tiles per shot
+    const int tps = (shot_len + kT - 1) / kT;
+    const int sh = blockIdx.x / tps;
+    const int il = (blockIdx.x - sh * tps) * kT + threadIdx.x;
+    const bool live = il < shot_len;
+    const int i = sh * shot_len + il;
+    if (S) S += sh * s_shot;
     const int c0 = blockIdx.z * NCB;
     const int j0 = blockIdx.y * slice_len;
     const int j1 = min(N, j0 + slice_len);
     const float tw = 6.283185307179586f;
     float ki[3] = {0, 0, 0};
     float ti = 0;
-    if (i < M) {
+    if (live) {
         ki[0] = kx[i];
         ki[1] = ky[i];
         ki[2] = kz[i];
@@ -143,7 +153,7 @@ __global__ void __launch_bounds__(kT) k_gdft_fwd(
         }
     }
     (void)tw;
-    if (i < M) {
+    if (live) {
 #pragma unroll
         for (int c = 0; c < NCB; ++c) {
             if (c0 + c >= nc) break;
@@ -159,13 +169,17 @@ __global__ void __launch_bounds__(kT) k_gdft_adj(
     const float* __restrict__ t, const float* __restrict__ ix, const float* __restrict__ iy,
     const float* __restrict__ iz, const float* __restrict__ fm, const float2* __restrict__ y,
     long long ystride, const float2* __restrict__ S, int nc, int M, int N, int slice_len,
-    float2* __restrict__ part, R2Args r2)
+    int shot_len, long long s_shot, float2* __restrict__ part, R2Args r2)
 {
     __shared__ float s_k[kT][4];
     __shared__ float2 s_v[kT][NCB];
     const int j = blockIdx.x * kT + threadIdx.x;
-    const int i0 = blockIdx.y * slice_len;
-    const int i1 = min(M, i0 + slice_len);
+    // gridDim.y / nshot slices per shot, each within its shot
+    const int sps = (int)gridDim.y / (M / shot_len);
+    const int sh = blockIdx.y / sps;
+    const int i0 = sh * shot_len + (blockIdx.y - sh * sps) * slice_len;
+    const int i1 = min(sh * shot_len + shot_len, i0 + slice_len);
+    if (S) S += sh * s_shot;
     float rj[3] = {0, 0, 0};
     float fj = 0;
     float gj[3] = {0, 0, 0};
@@ -273,18 +287,23 @@ __global__ void __launch_bounds__(kT) k_gdft_adj(
     if (j < N) part[(long long)blockIdx.y * N + j] = F64 ? make_float2((float)xr, (float)xi) : make_float2(gr, gi);
 }
 
-// out[(i / row) * ostride + i % row] (+)= sum of the slices of element i, in
-// slice order
-__global__ void k_sum_slices(const float2* part, int nslice, long long n, long long row,
-                             long long ostride, int acc, float2* out)
+// out[(i / row) * ostride + i % row] (+)= sum of the slices of element i:
+// each run of `group` consecutive slices is summed in slice order, and the
+// runs are added in order
+__global__ void k_sum_slices(const float2* part, int nslice, int group, long long n,
+                             long long row, long long ostride, int acc, float2* out)
 {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    float2 s = part[i];
-    for (int k = 1; k < nslice; ++k) {
-        const float2 p = part[(long long)k * n + i];
-        s.x += p.x;
-        s.y += p.y;
+    float2 s = make_float2(0.f, 0.f);
+    for (int g0 = 0; g0 < nslice; g0 += group) {
+        float2 t = part[(long long)g0 * n + i];
+        for (int k = g0 + 1; k < g0 + group; ++k) {
+            const float2 p = part[(long long)k * n + i];
+            t.x += p.x;
+            t.y += p.y;
+        }
+        s = g0 ? make_float2(s.x + t.x, s.y + t.y) : t;
     }
     const long long r = i / row;
     const long long o = r * ostride + (i - r * row);
@@ -301,14 +320,14 @@ R2Args r2_args(const pgd::DftData& d)
                   1.0f / (float)d.numz};
 }
 
-int slices_for(long long outputs, long long inputs)
+// slices per shot: aim for >= 4096 wavefronts in flight, each slice at
+// least 64 inputs (small problems, such as pcSENSE's DFTs of ~1k samples a
+// shot, would otherwise leave most of the 1024 SIMDs idle)
+int slices_for(long long outputs, long long inputs, int nshot = 1)
 {
-    // aim for >= 4096 wavefronts in flight, each slice at least 64 inputs
-    // (small problems, such as pcSENSE's per-shot DFTs of ~1k samples, would
-    // otherwise leave most of the 1024 SIMDs idle)
     long long waves = (outputs + kT - 1) / kT;
     int s = 1;
-    while (waves * s < 4096 && inputs / (s * 2) >= 64 && s < 64) s *= 2;
+    while (waves * nshot * s < 4096 && inputs / (s * 2) >= 64 && nshot * s < 64) s *= 2;
     return s;
 }
 
@@ -316,24 +335,28 @@ int slices_for(long long outputs, long long inputs)
 
 namespace pgd {
 
-size_t part_elems(int M, int N, int nc, bool adjoint)
+size_t part_elems(int M, int N, int nc, bool adjoint, int nshot)
 {
-    if (adjoint) return (size_t)slices_for(N, M) * N;
+    if (adjoint) return (size_t)slices_for(N, M / nshot, nshot) * nshot * N;
     return (size_t)slices_for((long long)M, N) * nc * M;
 }
 
 void forward(const DftData& d, const float2* x, const float2* S, int nc, float2* out, float2* part,
-             hipStream_t s, long long ostride)
+             hipStream_t s, long long ostride, int nshot, long long s_shot)
 {
     if (ostride < 0) ostride = d.M;
+    if (nshot < 1 || d.M % nshot)
+        throw pg::Error{PG_ERR_ARG, "the DFT's shots must split its samples evenly"};
+    const int shot_len = d.M / nshot;
     const int ns = slices_for(d.M, d.N);
     const int slen = (d.N + ns - 1) / ns;
     constexpr int NCB = 8;
-    dim3 grid((d.M + kT - 1) / kT, ns, (nc + NCB - 1) / NCB);
+    dim3 grid(nshot * ((shot_len + kT - 1) / kT), ns, (nc + NCB - 1) / NCB);
     const R2Args r2 = r2_args(d);
 #define PG_GDFT_FWD(F64, R2)                                                                     \
     hipLaunchKernelGGL((k_gdft_fwd<NCB, F64, R2>), grid, dim3(kT), 0, s, d.kx, d.ky, d.kz, d.t,  \
-                       d.ix, d.iy, d.iz, d.fm, x, S, nc, d.M, d.N, slen, part, r2)
+                       d.ix, d.iy, d.iz, d.fm, x, S, nc, d.M, d.N, slen, shot_len, s_shot, part, \
+                       r2)
     if (d.precision == 1) {
         if (d.gx) PG_GDFT_FWD(true, true);
         else PG_GDFT_FWD(true, false);
@@ -343,22 +366,26 @@ void forward(const DftData& d, const float2* x, const float2* S, int nc, float2*
     }
 #undef PG_GDFT_FWD
     const long long n = (long long)nc * d.M;
-    hipLaunchKernelGGL(k_sum_slices, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, part, ns, n,
-                       (long long)d.M, ostride, 0, out);
+    hipLaunchKernelGGL(k_sum_slices, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, part, ns, ns,
+                       n, (long long)d.M, ostride, 0, out);
 }
 
 void adjoint(const DftData& d, const float2* y, const float2* S, int nc, float2* out, float2* part,
-             hipStream_t s, long long ystride, bool accumulate)
+             hipStream_t s, long long ystride, bool accumulate, int nshot, long long s_shot)
 {
     if (ystride < 0) ystride = d.M;
-    const int ns = slices_for(d.N, d.M);
-    const int slen = (d.M + ns - 1) / ns;
+    if (nshot < 1 || d.M % nshot)
+        throw pg::Error{PG_ERR_ARG, "the DFT's shots must split its samples evenly"};
+    const int shot_len = d.M / nshot;
+    const int sps = slices_for(d.N, shot_len, nshot);
+    const int slen = (shot_len + sps - 1) / sps;
     constexpr int NCB = 8;
-    dim3 grid((d.N + kT - 1) / kT, ns, 1);
+    dim3 grid((d.N + kT - 1) / kT, nshot * sps, 1);
     const R2Args r2 = r2_args(d);
 #define PG_GDFT_ADJ(F64, R2)                                                                     \
     hipLaunchKernelGGL((k_gdft_adj<NCB, F64, R2>), grid, dim3(kT), 0, s, d.kx, d.ky, d.kz, d.t,  \
-                       d.ix, d.iy, d.iz, d.fm, y, ystride, S, nc, d.M, d.N, slen, part, r2)
+                       d.ix, d.iy, d.iz, d.fm, y, ystride, S, nc, d.M, d.N, slen, shot_len,      \
+                       s_shot, part, r2)
     if (d.precision == 1) {
         if (d.gx) PG_GDFT_ADJ(true, true);
         else PG_GDFT_ADJ(true, false);
@@ -367,8 +394,9 @@ void adjoint(const DftData& d, const float2* y, const float2* S, int nc, float2*
         else PG_GDFT_ADJ(false, false);
     }
 #undef PG_GDFT_ADJ
-    hipLaunchKernelGGL(k_sum_slices, dim3((unsigned)((d.N + 255) / 256)), dim3(256), 0, s, part, ns,
-                       (long long)d.N, (long long)d.N, (long long)d.N, accumulate ? 1 : 0, out);
+    hipLaunchKernelGGL(k_sum_slices, dim3((unsigned)((d.N + 255) / 256)), dim3(256), 0, s, part,
+                       nshot * sps, sps, (long long)d.N, (long long)d.N, (long long)d.N,
+                       accumulate ? 1 : 0, out);
 }
 
 }  // namespace pgd

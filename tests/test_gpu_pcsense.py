@@ -88,12 +88,16 @@ def test_pcsense_pcg_matches_oracle(orc):
         assert rel(xg, x64) < 1e-3
 
 
-@pytest.mark.parametrize("mode", [P.MODE_BATCHED, P.MODE_PLAIN])
-def test_pcsense_gdft_matches_oracle(orc, mode):
+@pytest.mark.parametrize("mode,Nd", [(P.MODE_BATCHED, 512), (P.MODE_PLAIN, 512),
+                                     (P.MODE_BATCHED, 500)])
+def test_pcsense_gdft_matches_oracle(orc, mode, Nd):
     # pcSENSE.cpp:39-153 over an exact DFT per shot; FP32 hardware trig, so
-    # 1e-4 against both restatements (whose sin/cos are FP32 as well)
-    print(f"[pcsense gdft mode={mode}]", flush=True)
-    Ns, Nd, N, nc = 3, 512, 16, 4
+    # 1e-4 against both restatements (whose sin/cos are FP32 as well).  The
+    # batched mode runs every shot in one launch; a shot of 500 samples ends
+    # inside a 64-sample forward tile, and its adjoint slices of 250 end
+    # inside a 64-sample step.
+    print(f"[pcsense gdft mode={mode} Nd={Nd}]", flush=True)
+    Ns, N, nc = 3, 16, 4
     kx, ky, kz, t, fm, sens, phase = case(Ns, Nd, N, nc, seed=2)
     A = P.PcSENSE(kx, ky, kz, N, N, 1, nc, t, sens, fm, phase)
     A.set_mode(mode)
