@@ -273,24 +273,44 @@ __global__ void k_reduce_parts(const double* part, int nparts, double* out)
 // (deterministic) decision and no single-block control kernel is needed.
 // Block 0 records the scalars; a kernel never writes a field it reads.
 
-// sum of nparts x NV partials in a fixed order, result in every thread
+// this thread's share of the fixed-order sum of nparts x NV partials:
+// entries b = threadIdx.x, + blockDim.x, ...; the first is loaded without
+// waiting for anything (the usual case: nparts <= blockDim.x)
 template <int NV>
-__device__ void all_sum_parts(const double* __restrict__ part, int nparts, double (&out)[NV])
+__device__ __forceinline__ void load_parts(const double* __restrict__ part, int nparts, double* v)
 {
-    __shared__ double bc[NV];
-    double v[NV];
+    const int b = threadIdx.x;
 #pragma unroll
     for (int k = 0; k < NV; ++k) v[k] = 0;
-    for (int b = threadIdx.x; b < nparts; b += blockDim.x)
+    if (b < nparts)
 #pragma unroll
         for (int k = 0; k < NV; ++k) v[k] += part[(long long)b * NV + k];
+    for (int bb = b + blockDim.x; bb < nparts; bb += blockDim.x)
+#pragma unroll
+        for (int k = 0; k < NV; ++k) v[k] += part[(long long)bb * NV + k];
+}
+
+// block-sum the per-thread shares and give the sums to every thread (each
+// value is reduced on its own, so sums fused into one call are the same)
+template <int NV>
+__device__ void bcast_sum(double (&v)[NV])
+{
+    __shared__ double bc[NV];
     block_sum<NV>(v);
     if (threadIdx.x == 0)
 #pragma unroll
         for (int k = 0; k < NV; ++k) bc[k] = v[k];
     __syncthreads();
 #pragma unroll
-    for (int k = 0; k < NV; ++k) out[k] = bc[k];
+    for (int k = 0; k < NV; ++k) v[k] = bc[k];
+}
+
+// sum of nparts x NV partials in a fixed order, result in every thread
+template <int NV>
+__device__ void all_sum_parts(const double* __restrict__ part, int nparts, double (&out)[NV])
+{
+    load_parts<NV>(part, nparts, out);
+    bcast_sum<NV>(out);
 }
 
 // One Newton step of the line search (solve_pwls_pcg.hpp:129-152) from the
@@ -401,9 +421,19 @@ __global__ void k_cg_grad(const float2* P, int nc, long long N, Pen p, int have_
 __global__ void k_cg_dir(const float2* ngrad, float2* ddir, long long N, pgv::CgParts cp,
                          CgScal* sc)
 {
-    if (sc->stop) return;
+    // the first element's inputs, the partials and the stop flag are loaded
+    // before any of them is used
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    float2 g = make_float2(0.f, 0.f), o = g;
+    if (i < N) {
+        g = ngrad[i];
+        o = ddir[i];
+    }
     double gs[2];
-    all_sum_parts<2>(cp.g, cp.ng, gs);
+    load_parts<2>(cp.g, cp.ng, gs);
+    if (sc->stop) return;
+    bcast_sum<2>(gs);
     const double gnorm = sqrt(gs[0]);
     if (gnorm / sc->ywy < 1e-10) {  // norm_grad (:41-46); NaN compares false
         if (blockIdx.x == 0 && threadIdx.x == 0) sc->stop = PG_EXIT_ZERO_GRAD;
@@ -426,16 +456,20 @@ __global__ void k_cg_dir(const float2* ngrad, float2* ddir, long long N, pgv::Cg
     }
     const float gamf = (float)gam;
     double v[1] = {0};
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < N;
-         i += (long long)gridDim.x * blockDim.x) {
-        const float2 g = ngrad[i];
-        float2 d = g;
-        if (!first) {
-            const float2 o = ddir[i];
-            d = make_float2(g.x + gamf * o.x, g.y + gamf * o.y);
+    while (i < N) {
+        const long long j = i + stride;
+        float2 g2 = make_float2(0.f, 0.f), o2 = g2;
+        if (j < N) {
+            g2 = ngrad[j];
+            o2 = ddir[j];
         }
+        float2 d = g;
+        if (!first) d = make_float2(g.x + gamf * o.x, g.y + gamf * o.y);
         ddir[i] = d;
         v[0] += (double)d.x * g.x + (double)d.y * g.y;
+        i = j;
+        g = g2;
+        o = o2;
     }
     block_sum<1>(v);
     if (threadIdx.x == 0) cp.d[blockIdx.x] = v[0];
@@ -448,33 +482,62 @@ __global__ void k_cg_adir(const float2* Adir, const float2* y, const double2* Ax
                           long long n, int na, Pen p, int have_pen, const float2* ddir,
                           const float2* x, pgv::CgParts cp, CgScal* sc)
 {
-    if (sc->stop) return;
+    // the first sample's inputs, the partials and the stop flag are loaded
+    // before any of them is used; each later sample's before the previous
+    // one is summed
+    const long long stride = (long long)na * blockDim.x;
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool data_block = (int)blockIdx.x < na;
+    float2 a = make_float2(0.f, 0.f), yv = a;
+    double2 ax = make_double2(0, 0);
+    float w = 1.f;
+    if (data_block && i < n) {
+        a = Adir[i];
+        yv = y[i];
+        ax = Ax[i];
+        if (W) w = W[i];
+    }
     double dd[1];
-    all_sum_parts<1>(cp.d, cp.nd, dd);
+    load_parts<1>(cp.d, cp.nd, dd);
+    if (sc->stop) return;
+    bcast_sum<1>(dd);
     if (dd[0] < 0) {
         if (blockIdx.x == 0 && threadIdx.x == 0) sc->stop = PG_EXIT_NOT_DESCENT;
         return;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) sc->oldinprod = sc->newinprod;
-    if ((int)blockIdx.x >= na) {
+    if (!data_block) {
         if (have_pen) {
             const long long i0 = (long long)(blockIdx.x - na) * blockDim.x + threadIdx.x;
-            const long long stride = (long long)(gridDim.x - na) * blockDim.x;
-            pen_partials(p, ddir, x, make_double2(0, 0), i0, stride, cp.l[0], blockIdx.x - na);
+            const long long pstride = (long long)(gridDim.x - na) * blockDim.x;
+            pen_partials(p, ddir, x, make_double2(0, 0), i0, pstride, cp.l[0], blockIdx.x - na);
             // the quadratic penalty's line search is closed-form in the update
             // kernel; the step-1 slots are free for the slope partials
-            if (p.kind == 0) pen_slope_partials(p, ddir, i0, stride, cp.l[1], blockIdx.x - na);
+            if (p.kind == 0) pen_slope_partials(p, ddir, i0, pstride, cp.l[1], blockIdx.x - na);
         }
         return;
     }
     double v[2] = {0, 0};
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (long long)na * blockDim.x) {
-        const float2 a = Adir[i];
-        const double w = W ? (double)W[i] : 1.0;
-        const double rr = (double)y[i].x - Ax[i].x, ri = (double)y[i].y - Ax[i].y;
-        v[0] += w * ((double)a.x * a.x + (double)a.y * a.y);
-        v[1] += w * ((double)a.x * rr + (double)a.y * ri);
+    while (i < n) {
+        const long long j = i + stride;
+        float2 a2 = make_float2(0.f, 0.f), y2 = a2;
+        double2 ax2 = make_double2(0, 0);
+        float w2 = 1.f;
+        if (j < n) {
+            a2 = Adir[j];
+            y2 = y[j];
+            ax2 = Ax[j];
+            if (W) w2 = W[j];
+        }
+        const double wd = W ? (double)w : 1.0;
+        const double rr = (double)yv.x - ax.x, ri = (double)yv.y - ax.y;
+        v[0] += wd * ((double)a.x * a.x + (double)a.y * a.y);
+        v[1] += wd * ((double)a.x * rr + (double)a.y * ri);
+        i = j;
+        a = a2;
+        yv = y2;
+        ax = ax2;
+        w = w2;
     }
     block_sum<2>(v);
     if (threadIdx.x == 0) {
@@ -530,34 +593,57 @@ __global__ void k_cg_update(double2* Ax, const float2* Adir, const float2* y, co
                             long long M, double2* x, float2* x32, const float2* ddir, long long N,
                             int pen_mode, pgv::CgParts cp, float2* rout, CgScal* sc)
 {
+    // the first element's inputs, the partials and the stop flag are all
+    // loaded before any of them is used; each later element's inputs are
+    // loaded before the previous element is stored
+    const long long n = M > N ? M : N;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    float2 a = make_float2(0.f, 0.f), yv = a, d = a;
+    double2 ax = make_double2(0, 0), xo = ax;
+    float w = 1.f;
+    if (i < M) {
+        a = Adir[i];
+        ax = Ax[i];
+        yv = y[i];
+        if (W) w = W[i];
+    }
+    if (i < N) {
+        d = ddir[i];
+        xo = x[i];
+    }
+    const int have_pen = (pen_mode == 1);
+    double q[6] = {0, 0, 0, 0, 0, 0};
+    if (pen_mode == 2) {
+        load_parts<2>(cp.a, cp.na, q);
+        load_parts<2>(cp.l[0], cp.nl, q + 2);
+        load_parts<2>(cp.l[1], cp.nl, q + 4);
+    } else if (have_pen) {
+        load_parts<2>(cp.l[2], cp.nl, q);
+    } else {
+        load_parts<2>(cp.a, cp.na, q);
+    }
     if (sc->stop) return;
+    bcast_sum<6>(q);
     double2 step;
     int stop = 0;
     bool ok = true;
     double dAWAd = 0, dAWr = 0;
-    const int have_pen = (pen_mode == 1);
     if (pen_mode == 2) {
-        double a[2], pl[2], ql[2];
-        all_sum_parts<2>(cp.a, cp.na, a);
-        all_sum_parts<2>(cp.l[0], cp.nl, pl);
-        all_sum_parts<2>(cp.l[1], cp.nl, ql);
-        dAWAd = a[0];
-        dAWr = a[1];
+        // q: {dAWAd, dAWr}, penalty {denom, pdot} at step 0, slope {q, -}
+        dAWAd = q[0];
+        dAWr = q[1];
         step = make_double2(0, 0);
         for (int j = 1; j <= 3 && ok; ++j)
-            ok = newton_step(dAWAd, dAWr, pl[0], pl[1] + step.x * ql[0], step, stop);
+            ok = newton_step(dAWAd, dAWr, q[2], q[3] + step.x * q[4], step, stop);
     } else if (have_pen) {
-        double pl[2];
-        all_sum_parts<2>(cp.l[2], cp.nl, pl);
         dAWAd = sc->dAWAd;
         dAWr = sc->dAWr;
         step = sc->steps[2];
-        ok = newton_step(dAWAd, dAWr, pl[0], pl[1], step, stop);
+        ok = newton_step(dAWAd, dAWr, q[0], q[1], step, stop);
     } else {
-        double a[2];
-        all_sum_parts<2>(cp.a, cp.na, a);
-        dAWAd = a[0];
-        dAWr = a[1];
+        dAWAd = q[0];
+        dAWr = q[1];
         step = make_double2(0, 0);
         for (int j = 1; j <= 3 && ok; ++j) ok = newton_step(dAWAd, dAWr, 0.0, 0.0, step, stop);
     }
@@ -577,21 +663,31 @@ __global__ void k_cg_update(double2* Ax, const float2* Adir, const float2* y, co
     if (!ok) return;
     const double sr = step.x, si = step.y;
     double v[1] = {0};
-    const long long n = M > N ? M : N;
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (long long)gridDim.x * blockDim.x) {
+    while (i < n) {
+        const long long j = i + stride;
+        float2 a2 = make_float2(0.f, 0.f), y2 = a2, d2 = a2;
+        double2 ax2 = make_double2(0, 0), x2 = ax2;
+        float w2 = 1.f;
+        if (j < M) {
+            a2 = Adir[j];
+            ax2 = Ax[j];
+            y2 = y[j];
+            if (W) w2 = W[j];
+        }
+        if (j < N) {
+            d2 = ddir[j];
+            x2 = x[j];
+        }
         if (i < M) {
-            const float2 a = Adir[i];
-            double2 o = Ax[i];
-            o.x += sr * a.x - si * a.y;
-            o.y += sr * a.y + si * a.x;
-            Ax[i] = o;
-            const double rr = (double)y[i].x - o.x, ri = (double)y[i].y - o.y;
+            ax.x += sr * a.x - si * a.y;
+            ax.y += sr * a.y + si * a.x;
+            Ax[i] = ax;
+            const double rr = (double)yv.x - ax.x, ri = (double)yv.y - ax.y;
             v[0] += rr * rr + ri * ri;
             if (rout) {
                 double wr = rr, wi = ri;
                 if (W) {
-                    const double wv = W[i];
+                    const double wv = w;
                     wr *= wv;
                     wi *= wv;
                 }
@@ -599,13 +695,18 @@ __global__ void k_cg_update(double2* Ax, const float2* Adir, const float2* y, co
             }
         }
         if (i < N) {
-            const float2 d = ddir[i];
-            double2 o = x[i];
-            o.x += sr * d.x - si * d.y;
-            o.y += sr * d.y + si * d.x;
-            x[i] = o;
-            x32[i] = make_float2((float)o.x, (float)o.y);
+            xo.x += sr * d.x - si * d.y;
+            xo.y += sr * d.y + si * d.x;
+            x[i] = xo;
+            x32[i] = make_float2((float)xo.x, (float)xo.y);
         }
+        i = j;
+        a = a2;
+        ax = ax2;
+        yv = y2;
+        w = w2;
+        d = d2;
+        xo = x2;
     }
     block_sum<1>(v);
     if (threadIdx.x == 0) cp.u[blockIdx.x] = v[0];
