@@ -393,15 +393,25 @@ __global__ void k_cg_grad(const float2* P, int nc, long long N, Pen p, int have_
     double v[2] = {0, 0};
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < N;
          i += (long long)gridDim.x * blockDim.x) {
+        // the penalty stencil's loads and each run of 8 coils' loads are
+        // issued together; the coils are still added in coil order
+        const float2 r = have_pen ? pen_grad_at(p, x, nullptr, make_double2(0, 0), i)
+                                  : make_float2(0.f, 0.f);
         float2 g = P[i];
-        for (int c = 1; c < nc; ++c) {
-            const float2 q = P[(long long)c * N + i];
-            g.x += q.x;
-            g.y += q.y;
+        for (int c0 = 1; c0 < nc; c0 += 8) {
+            float2 q[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k)
+                q[k] = (c0 + k < nc) ? P[(long long)(c0 + k) * N + i] : make_float2(0.f, 0.f);
+#pragma unroll
+            for (int k = 0; k < 8; ++k)
+                if (c0 + k < nc) {
+                    g.x += q[k].x;
+                    g.y += q[k].y;
+                }
         }
         v[0] += (double)g.x * g.x + (double)g.y * g.y;
         if (have_pen) {
-            const float2 r = pen_grad_at(p, x, nullptr, make_double2(0, 0), i);
             g.x -= r.x;
             g.y -= r.y;
         }
