@@ -373,6 +373,34 @@ __device__ void pen_slope_partials(const Pen& p, const float2* ddir, long long i
     }
 }
 
+// pen_partials at step 0 and pen_slope_partials in one pass over the
+// elements: the same sums, each in the same order, with one block reduction
+__device__ void pen_partials_and_slope(const Pen& p, const float2* ddir, const float2* x,
+                                       long long i0, long long stride, double* part0,
+                                       double* part1, int slot)
+{
+    double v[4] = {0, 0, 0, 0};
+    for (long long i = i0; i < p.n; i += stride) {
+        for (int k = 0; k < p.nd; ++k) {
+            const float2 cdir = cd_at(ddir, i, p.off[k]);
+            const float w = wpot(p, cd_xs(x, ddir, make_double2(0, 0), i, p.off[k]));
+            v[0] += (double)w * ((double)cdir.x * cdir.x + (double)cdir.y * cdir.y);
+        }
+        const float2 g = pen_grad_at(p, x, ddir, make_double2(0, 0), i);
+        const float2 gs = pen_grad_at(p, ddir, nullptr, make_double2(0, 0), i);
+        const float2 d = ddir[i];
+        v[1] += (double)d.x * g.x + (double)d.y * g.y;
+        v[2] += (double)d.x * gs.x + (double)d.y * gs.y;
+    }
+    block_sum<4>(v);
+    if (threadIdx.x == 0) {
+        part0[2 * slot] = v[0] * p.beta;
+        part0[2 * slot + 1] = v[1];
+        part1[2 * slot] = v[2];
+        part1[2 * slot + 1] = 0.0;
+    }
+}
+
 // grad: also closes the previous iteration's bookkeeping (residual norm,
 // iteration count) from its update partials
 __global__ void k_cg_grad(const float2* P, int nc, long long N, Pen p, int have_pen,
@@ -485,9 +513,10 @@ __global__ void k_cg_dir(const float2* ngrad, float2* ddir, long long N, pgv::Cg
     if (threadIdx.x == 0) cp.d[blockIdx.x] = v[0];
 }
 
-// adir: descent check (:113-116); blocks [0, na) form the partials of
-// dAWAd = real(cdot(Adir, W Adir)) and dAWr = real(Adir' W (y - Ax))
-// (:123-125); blocks [na, na + np) the penalty partials at step 0
+// adir: descent check (:113-116); blocks [0, np) form the penalty partials
+// at step 0 (first, so that their longer stencil work starts first), blocks
+// [np, np + na) the partials of dAWAd = real(cdot(Adir, W Adir)) and
+// dAWr = real(Adir' W (y - Ax)) (:123-125)
 __global__ void k_cg_adir(const float2* Adir, const float2* y, const double2* Ax, const float* W,
                           long long n, int na, Pen p, int have_pen, const float2* ddir,
                           const float2* x, pgv::CgParts cp, CgScal* sc)
@@ -495,9 +524,11 @@ __global__ void k_cg_adir(const float2* Adir, const float2* y, const double2* Ax
     // the first sample's inputs, the partials and the stop flag are loaded
     // before any of them is used; each later sample's before the previous
     // one is summed
+    const int np = (int)gridDim.x - na;
+    const int db = (int)blockIdx.x - np;  // data block index
+    const bool data_block = db >= 0;
     const long long stride = (long long)na * blockDim.x;
-    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const bool data_block = (int)blockIdx.x < na;
+    long long i = (long long)db * blockDim.x + threadIdx.x;
     float2 a = make_float2(0.f, 0.f), yv = a;
     double2 ax = make_double2(0, 0);
     float w = 1.f;
@@ -518,12 +549,14 @@ __global__ void k_cg_adir(const float2* Adir, const float2* y, const double2* Ax
     if (blockIdx.x == 0 && threadIdx.x == 0) sc->oldinprod = sc->newinprod;
     if (!data_block) {
         if (have_pen) {
-            const long long i0 = (long long)(blockIdx.x - na) * blockDim.x + threadIdx.x;
-            const long long pstride = (long long)(gridDim.x - na) * blockDim.x;
-            pen_partials(p, ddir, x, make_double2(0, 0), i0, pstride, cp.l[0], blockIdx.x - na);
+            const long long i0 = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+            const long long pstride = (long long)np * blockDim.x;
             // the quadratic penalty's line search is closed-form in the update
             // kernel; the step-1 slots are free for the slope partials
-            if (p.kind == 0) pen_slope_partials(p, ddir, i0, pstride, cp.l[1], blockIdx.x - na);
+            if (p.kind == 0)
+                pen_partials_and_slope(p, ddir, x, i0, pstride, cp.l[0], cp.l[1], blockIdx.x);
+            else
+                pen_partials(p, ddir, x, make_double2(0, 0), i0, pstride, cp.l[0], blockIdx.x);
         }
         return;
     }
@@ -551,8 +584,8 @@ __global__ void k_cg_adir(const float2* Adir, const float2* y, const double2* Ax
     }
     block_sum<2>(v);
     if (threadIdx.x == 0) {
-        cp.a[2 * blockIdx.x] = v[0];
-        cp.a[2 * blockIdx.x + 1] = v[1];
+        cp.a[2 * db] = v[0];
+        cp.a[2 * db + 1] = v[1];
     }
 }
 
