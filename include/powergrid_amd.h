@@ -318,8 +318,18 @@ int pg_recon_solve_batch(size_t K, const pg_op* A, const pg_penalty* R, const vo
  * "spread_mfma"; 0 (default): the fixed choice.  The variants differ in
  * FP32 summation order only, but with autotuning on, which one runs -- and
  * so the last bits of a result -- depends on the timing.
- * Settings take effect for launches (and solve graphs) made after the call. */
+ * "zclass" 1 (default): a 3-D Gnufft plan whose samples take at most Gz
+ * distinct z windows (a stack of 2-D trajectories: kz on a Cartesian
+ * raster) is built for z-class planes: batches of 16 grids or more apply
+ * the separable kernel's z taps inside the z FFT pass and grid in x-y on
+ * one plane per class; 0: plans made afterwards grid on the full 3-D grid.
+ * The two differ in FP32 summation order only.
+ * Settings take effect for launches (and solve graphs) made after the call;
+ * "zclass" for plans created after it. */
 int pg_set_tuning(const char* key, int value);
+/* the number of z-class planes of the Gnufft plan under h (a Gnufft, or a
+ * TimeSegmentation or SENSE over one); 0 when it grids on the full grid */
+int pg_gnufft_zclasses(pg_op h, size_t* nzc);
 /* the variants the autotuner chose for the Gnufft plan under h (a Gnufft,
  * or a TimeSegmentation or SENSE over one) at B grids: interp_lds 1 =
  * LDS-staged tiles, 0 = gather; spread_mfma 1 = matrix cores, 0 = vector

@@ -379,8 +379,18 @@ def set_tuning(key, value):
     """pg_set_tuning: "spread_mfma" 1 runs the 2-D batched spread on the
     matrix cores (grid count a multiple of 4), 0 (default) on the vector
     units; "autotune" 1 lets each plan time its interpolation and spread
-    variants when first prepared for a grid count and keep the faster."""
+    variants when first prepared for a grid count and keep the faster;
+    "zclass" 1 (default) builds 3-D plans of stacked trajectories for z-class
+    planes, 0 for the full 3-D grid (plans created afterwards)."""
     check(lib().pg_set_tuning(key.encode(), int(value)))
+
+
+def zclasses(op):
+    """The number of z-class planes of the Gnufft plan under op (0: the plan
+    grids on the full 3-D grid)."""
+    n = C.c_size_t()
+    check(lib().pg_gnufft_zclasses(op.handle, C.byref(n)))
+    return n.value
 
 
 def calc3DDensityCompensation(kx, ky, kz, Ninplane, Nz, stream=None):

@@ -35,6 +35,7 @@ _SIGS = {
     "pg_free": (i32, [vp]),
     "pg_mem_info": (i32, [C.POINTER(sz), C.POINTER(sz)]),
     "pg_op_tuning": (i32, [vp, i32, C.POINTER(i32), C.POINTER(i32)]),
+    "pg_gnufft_zclasses": (i32, [vp, C.POINTER(sz)]),
     "pg_memcpy_htod": (i32, [vp, vp, sz, vp]),
     "pg_memcpy_dtoh": (i32, [vp, vp, sz, vp]),
     "pg_memcpy_dtod": (i32, [vp, vp, sz, vp]),

@@ -249,6 +249,12 @@ void Gnufft::release()
     dev_free(ent_w);
     dev_free(tile_off);
     dev_free(tile_idx);
+    dev_free(zc_lo);
+    dev_free(zc_w);
+    dev_free(zcls);
+    dev_free(zent);
+    dev_free(zoff);
+    nzc = 0;
     dev_free(grids);
     dev_free(tbuf);
     dev_free(tbuf2);
@@ -430,18 +436,100 @@ static void gnufft_setup(Gnufft& G, size_t M, float os, size_t Nx, size_t Ny, si
         return false;
     };
 
-    if (D == 2) {
+    // 3-D: z classes, the distinct (z window start, FP64 z weights) of the
+    // samples with a non-empty window.  A stack of 2-D trajectories (kz on a
+    // Cartesian raster) has one class per partition; a plan uses them when
+    // there are at most Gz and the z FFT runs in registers.
+    std::vector<int> pl;  // plane of every sample (2-D: none, all on plane 0)
+    int npl = 1;
+    G.nzc = 0;
+    if (D == 3 && pgk::g_zclass.load() && (g.Gz & (g.Gz - 1)) == 0 && g.Gz >= 16 &&
+        g.Gz <= 1024 && G.fft_ok) {
+        std::map<std::vector<uint64_t>, int> key2c;
+        std::vector<int> cls(M, -1);
+        std::vector<uint64_t> key(NT + 1), last;
+        int lastc = -1;
+        bool few = true;
+        for (size_t i = 0; i < M && few; ++i) {
+            if (empty(i)) continue;
+            key[0] = (uint64_t)(uint32_t)lo[2 * M + i];
+            for (int j = 0; j < NT; ++j) std::memcpy(&key[j + 1], &wd[(i * D + 2) * NT + j], 8);
+            if (lastc >= 0 && key == last) {
+                cls[i] = lastc;
+                continue;
+            }
+            auto it = key2c.find(key);
+            if (it == key2c.end()) {
+                if ((int)key2c.size() >= g.Gz) {
+                    few = false;
+                    break;
+                }
+                it = key2c.emplace(key, (int)key2c.size()).first;
+            }
+            cls[i] = lastc = it->second;
+            last = key;
+        }
+        if (few && !key2c.empty()) {
+            // classes in order of (window start, weights): std::map's order
+            const int nzc = (int)key2c.size();
+            std::vector<int> remap(nzc), zlo(nzc);
+            std::vector<float> zw((size_t)nzc * NT);
+            int k = 0;
+            for (auto& kv : key2c) {
+                remap[kv.second] = k;
+                zlo[k] = (int)(int32_t)(uint32_t)kv.first[0];
+                for (int j = 0; j < NT; ++j) {
+                    double v;
+                    std::memcpy(&v, &kv.first[j + 1], 8);
+                    zw[(size_t)k * NT + j] = (float)v;
+                }
+                ++k;
+            }
+            pl.assign(M, 0);
+            for (size_t i = 0; i < M; ++i)
+                if (cls[i] >= 0) pl[i] = remap[cls[i]];
+            // for each grid z index, the classes covering it (class order)
+            std::vector<int> zoff(g.Gz + 1, 0);
+            std::vector<int2> zent;
+            for (int n = 0; n < g.Gz; ++n) {
+                for (int c = 0; c < nzc; ++c) {
+                    const int j = n - zlo[c];
+                    if (j < 0 || j >= NT) continue;
+                    const float wv = zw[(size_t)c * NT + j];
+                    if (wv == 0.0f) continue;
+                    int bits;
+                    std::memcpy(&bits, &wv, 4);
+                    zent.push_back(make_int2(c, bits));
+                }
+                zoff[n + 1] = (int)zent.size();
+            }
+            if (zent.empty()) zent.push_back(make_int2(0, 0));
+            G.nzc = nzc;
+            npl = nzc;
+            G.zc_lo = dev_copy(zlo.data(), zlo.size());
+            G.zc_w = dev_copy(zw.data(), zw.size());
+            G.zcls = dev_copy(pl.data(), pl.size());
+            G.zoff = dev_copy(zoff.data(), zoff.size());
+            G.zent = dev_copy(zent.data(), zent.size());
+        }
+    }
+    auto plane = [&](size_t i) { return pl.empty() ? 0 : pl[i]; };
+
+    if (D == 2 || G.nzc > 0) {
+        // x-y work lists over npl planes (2-D: one)
         if (NT <= 8) {
             // interp tiles: the window start is re-based into [-padl, G-1] (taps
-            // moved off the start carry zero weight), binned by kIT x kIT tiles
+            // moved off the start carry zero weight), binned by (plane, kIT x kIT
+            // tile)
             const int padl = NT / 2;
             G.ipadl = padl;
             G.itx = (g.Gx + padl + kIT - 1) / kIT;
             G.ity = (g.Gy + padl + kIT - 1) / kIT;
             const int nt = G.itx * G.ity;
+            const long long nbin = (long long)nt * npl;
             std::vector<int> rlo(2 * M);
             std::vector<float> rw((size_t)2 * 8 * M, 0.0f);
-            std::vector<int> tcnt(nt + 1, 0), tof(M);
+            std::vector<int> tcnt(nbin + 1, 0), tof(M);
             const int Gd[2] = {g.Gx, g.Gy};
             for (size_t i = 0; i < M; ++i) {
                 int tt[2];
@@ -464,18 +552,19 @@ static void gnufft_setup(Gnufft& G, size_t M, float os, size_t Nx, size_t Ny, si
                     }
                     tt[d] = (l0 + padl) / kIT;
                 }
-                tof[i] = tt[0] * G.ity + tt[1];
+                tof[i] = plane(i) * nt + tt[0] * G.ity + tt[1];
                 tcnt[tof[i] + 1]++;
             }
-            for (int t = 0; t < nt; ++t) tcnt[t + 1] += tcnt[t];
+            for (long long t = 0; t < nbin; ++t) tcnt[t + 1] += tcnt[t];
             std::vector<int> pos(tcnt.begin(), tcnt.end() - 1), order(M);
             for (size_t i = 0; i < M; ++i) order[pos[tof[i]]++] = (int)i;
             std::vector<float4> rec((size_t)5 * M);
             std::vector<int4> its;
-            for (int t = 0; t < nt; ++t) {
+            for (long long tb = 0; tb < nbin; ++tb) {
+                const int p = (int)(tb / nt), t = (int)(tb - (long long)p * nt);
                 const int tx = t / G.ity, ty = t % G.ity;
                 const int rx0 = tx * kIT - padl, ry0 = ty * kIT - padl;
-                for (int k = tcnt[t]; k < tcnt[t + 1]; ++k) {
+                for (int k = tcnt[tb]; k < tcnt[tb + 1]; ++k) {
                     const int i = order[k];
                     const int ox = rlo[i] - rx0, oy = rlo[M + i] - ry0;
                     PG_REQUIRE(ox >= 0 && ox < kIT && oy >= 0 && oy < kIT, PG_ERR_ARG, "interp binning");
@@ -492,36 +581,42 @@ static void gnufft_setup(Gnufft& G, size_t M, float os, size_t Nx, size_t Ny, si
                     rec[(size_t)5 * k + 3] = make_float4(wy[0], wy[1], wy[2], wy[3]);
                     rec[(size_t)5 * k + 4] = make_float4(wy[4], wy[5], wy[6], wy[7]);
                 }
-                for (int k = tcnt[t]; k < tcnt[t + 1]; k += kIChunk)
-                    its.push_back(make_int4(t, k, std::min(k + kIChunk, tcnt[t + 1]), 0));
+                for (int k = tcnt[tb]; k < tcnt[tb + 1]; k += kIChunk)
+                    its.push_back(make_int4(t, k, std::min(k + kIChunk, tcnt[tb + 1]), p));
             }
-            // items stay in tile raster order: neighbouring tiles, whose staged
-            // regions overlap, run at about the same time and share L2 lines
+            // items stay in (plane, tile raster) order: neighbouring tiles,
+            // whose staged regions overlap, run at about the same time and
+            // share L2 lines
             G.n_iitems = (int)its.size();
             G.iitems = dev_copy(its.data(), its.size());
             G.irec = dev_copy(rec.data(), rec.size());
         }  // wider windows interpolate with the gather kernel
 
-        // 4x4 patch work items with per-entry patch weights
+        // 4x4 patch work items with per-entry patch weights; patch q of plane
+        // p is p * np + q
         const int npx = g.Gx / kPatch, npy = g.Gy / kPatch;
         const long long np = (long long)npx * npy;
-        std::vector<long long> cnt(np + 1, 0);
+        const long long npall = np * npl;
+        PG_REQUIRE(npall < (1ll << 31), PG_ERR_UNSUPPORTED, "too many patches");
+        G.npp = (int)np;
+        std::vector<long long> cnt(npall + 1, 0);
         for (int pass = 0; pass < 2; ++pass) {
             std::vector<long long> pos;
             std::vector<int> ent;
             if (pass == 1) {
-                for (long long q = 0; q < np; ++q) cnt[q + 1] += cnt[q];
-                PG_REQUIRE(cnt[np] < (1ll << 31), PG_ERR_UNSUPPORTED, "too many patch entries");
+                for (long long q = 0; q < npall; ++q) cnt[q + 1] += cnt[q];
+                PG_REQUIRE(cnt[npall] < (1ll << 31), PG_ERR_UNSUPPORTED, "too many patch entries");
                 pos.assign(cnt.begin(), cnt.end() - 1);
-                ent.resize(std::max<long long>(1, cnt[np]));
+                ent.resize(std::max<long long>(1, cnt[npall]));
             }
             for (size_t i = 0; i < M; ++i) {
                 if (empty(i)) continue;
+                const long long pb = (long long)plane(i) * np;
                 const int x0 = vlo[i] / kPatch, x1 = vhi[i] / kPatch;
                 const int y0 = vlo[M + i] / kPatch, y1 = vhi[M + i] / kPatch;
                 for (int a = x0; a <= x1; ++a)
                     for (int c = y0; c <= y1; ++c) {
-                        const long long q = (long long)a * npy + c;
+                        const long long q = pb + (long long)a * npy + c;
                         if (pass == 0) cnt[q + 1]++;
                         else ent[pos[q]++] = (int)i;
                     }
@@ -529,10 +624,11 @@ static void gnufft_setup(Gnufft& G, size_t M, float os, size_t Nx, size_t Ny, si
             if (pass == 1) {
                 // weights of entry k (patch q, sample i) on the patch's 16 points,
                 // wx * wy formed in FP64 and rounded once
-                std::vector<float> ew((size_t)16 * std::max<long long>(1, cnt[np]), 0.0f);
+                std::vector<float> ew((size_t)16 * std::max<long long>(1, cnt[npall]), 0.0f);
                 auto fill = [&](long long q0, long long q1) {
                     for (long long q = q0; q < q1; ++q) {
-                        const int px0 = (int)(q / npy) * kPatch, py0 = (int)(q % npy) * kPatch;
+                        const long long qq = q % np;
+                        const int px0 = (int)(qq / npy) * kPatch, py0 = (int)(qq % npy) * kPatch;
                         for (long long k = cnt[q]; k < cnt[q + 1]; ++k) {
                             const size_t i = (size_t)ent[k];
                             const double* wx = &wd[(i * D + 0) * NT];
@@ -551,14 +647,14 @@ static void gnufft_setup(Gnufft& G, size_t M, float os, size_t Nx, size_t Ny, si
                 };
                 // a few host threads over runs of patches of equal entry count
                 const int nth = (int)std::min<long long>(
-                    std::max(1u, std::min(4u, std::thread::hardware_concurrency())),
-                    std::max<long long>(1, cnt[np] / 16384));
+                    std::max(1u, std::min(8u, std::thread::hardware_concurrency())),
+                    std::max<long long>(1, cnt[npall] / 16384));
                 std::vector<std::thread> th;
                 long long q0 = 0;
                 for (int t = 1; t <= nth; ++t) {
                     long long q1 = q0;
-                    const long long target = cnt[np] * t / nth;
-                    while (q1 < np && (cnt[q1] < target || t == nth)) ++q1;
+                    const long long target = cnt[npall] * t / nth;
+                    while (q1 < npall && (cnt[q1] < target || t == nth)) ++q1;
                     if (t < nth) th.emplace_back(fill, q0, q1);
                     else fill(q0, q1);
                     q0 = q1;
@@ -566,11 +662,11 @@ static void gnufft_setup(Gnufft& G, size_t M, float os, size_t Nx, size_t Ny, si
                 for (std::thread& x : th) x.join();
                 G.entries = dev_copy(ent.data(), ent.size());
                 G.ent_w = dev_copy(ew.data(), ew.size());
-                G.tile_entries = cnt[np];
+                G.tile_entries = cnt[npall];
             }
         }
-        // XCD groups: the patches in Hilbert order over the npx x npy patch
-        // grid, cut into 8 runs of equal work; group g's blocks get ids
+        // XCD groups: the patches in (plane, Hilbert) order over the npx x npy
+        // patch grid, cut into 8 runs of equal work; group g's blocks get ids
         // congruent to g mod 8 (workgroups are dealt to the 8 XCDs
         // round-robin), so a sample's row of values is read into one L2
         const int kXcd = 8;
@@ -599,15 +695,17 @@ static void gnufft_setup(Gnufft& G, size_t M, float os, size_t Nx, size_t Ny, si
                          [&](int u, int v) { return hkey[u] < hkey[v]; });
         auto pweight = [&](long long q) { return (double)(cnt[q + 1] - cnt[q]) + 16.0; };
         double wtot = 0;
-        for (long long q = 0; q < np; ++q) wtot += pweight(q);
+        for (long long q = 0; q < npall; ++q) wtot += pweight(q);
         std::vector<std::vector<int>> grp(kXcd);
         {
             double run = 0;
-            for (int q : horder) {
-                const double mid = run + 0.5 * pweight(q);
-                grp[std::min(kXcd - 1, (int)(mid * kXcd / wtot))].push_back(q);
-                run += pweight(q);
-            }
+            for (int p = 0; p < npl; ++p)
+                for (int q0 : horder) {
+                    const int q = (int)((long long)p * np + q0);
+                    const double mid = run + 0.5 * pweight(q);
+                    grp[std::min(kXcd - 1, (int)(mid * kXcd / wtot))].push_back(q);
+                    run += pweight(q);
+                }
         }
         // per group: blocks of 8 wavefront items {patch, entry begin, entry end,
         // slot}.  A patch of n <= 512 entries is one unit taking gs = 1
@@ -687,7 +785,7 @@ static void gnufft_setup(Gnufft& G, size_t M, float os, size_t Nx, size_t Ny, si
         {
             // groups are aligned runs of gs wavefronts with one patch and slot
             // and consecutive ranges; all ranges of a patch tile its entries
-            std::vector<std::vector<std::pair<int, int>>> rng(np);
+            std::vector<std::vector<std::pair<int, int>>> rng(npall);
             for (size_t blk = 0; blk < nmax * kXcd; ++blk) {
                 const int gs = bgs[blk];
                 for (int w = 0; w < kW; ++w) {
@@ -696,11 +794,11 @@ static void gnufft_setup(Gnufft& G, size_t M, float os, size_t Nx, size_t Ny, si
                     PG_REQUIRE(it.x == ld.x && it.w == ld.w, PG_ERR_STATE, "spread work groups");
                     if (w % gs) PG_REQUIRE(witems[blk * kW + w - 1].z == it.y, PG_ERR_STATE, "spread work groups");
                     if (it.x < 0) continue;
-                    PG_REQUIRE(it.x < np && it.z >= it.y, PG_ERR_STATE, "spread work list");
+                    PG_REQUIRE(it.x < npall && it.z >= it.y, PG_ERR_STATE, "spread work list");
                     rng[it.x].push_back({it.y, it.z});
                 }
             }
-            for (long long q = 0; q < np; ++q) {
+            for (long long q = 0; q < npall; ++q) {
                 auto& r = rng[q];
                 std::sort(r.begin(), r.end());
                 long long at = cnt[q];
@@ -717,15 +815,17 @@ static void gnufft_setup(Gnufft& G, size_t M, float os, size_t Nx, size_t Ny, si
         G.n_sblocks = (int)(nmax * kXcd);
         G.witems = dev_copy(witems.data(), witems.size());
         G.bgs = dev_copy(bgs.data(), bgs.size());
-    } else {
-        // 3-D: samples whose window meets each 8x8x4 tile
+    }
+    if (D == 3) {
+        // the full 3-D grid (small batches, or no z classes): samples whose
+        // window meets each 8x8x4 tile
         const int TX = 8, TY = 8, TZ = 4;
         G.ntx = (g.Gx + TX - 1) / TX;
         G.nty = (g.Gy + TY - 1) / TY;
         G.ntz = (g.Gz + TZ - 1) / TZ;
         const long long ntiles = (long long)G.ntx * G.nty * G.ntz;
         std::vector<long long> cnt(ntiles + 1, 0);
-        for (int pass = 0; pass < 2; ++pass) {
+        for (int pass = 0; pass < 2 && G.nzc == 0; ++pass) {
             std::vector<long long> pos;
             std::vector<int> idx;
             if (pass == 1) {
@@ -1001,8 +1101,9 @@ static void autotune(Gnufft& G, int nc, int L, const float* AA, const float* AAi
             hipEventDestroy(e[1]);
         }
     } ev{e};
-    PG_HIP(hipMemsetAsync(G.grids, 0, sizeof(float2) * (size_t)B * G.geo.G(), ss));
-    PG_HIP(hipMemsetAsync(G.vals, 0, sizeof(float2) * (size_t)B * G.M, ss));
+    // timing inputs: zeroed workspaces (a small fill kernel, not a memset node)
+    pgv::fill(G.grids, (long long)B * G.geo.G(), make_float2(0.f, 0.f), ss);
+    pgv::fill(G.vals, (long long)B * G.M, make_float2(0.f, 0.f), ss);
     auto timed = [&](auto&& launch) {
         launch();  // warm-up
         PG_HIP(hipEventRecord(e[0], ss));
@@ -1037,6 +1138,11 @@ static void autotune(Gnufft& G, int nc, int L, const float* AA, const float* AAi
         t.mfma = ms_mfma < ms_vec ? 1 : 0;
     }
     PG_HIP(hipGetLastError());
+    // the fills (and any timing launch) are done before the caller's stream
+    // uses the workspaces again, whatever was tunable
+    PG_HIP(hipStreamSynchronize(ss));
+    // solve graphs captured before this plan was tuned ran the defaults
+    g_ws_gen.fetch_add(1);
 }
 
 // autotune the plan of a fused launch on stream s, unless already tuned for
@@ -1066,14 +1172,15 @@ static void fused_forward(const Fused& f, const float2* x, float2* y, hipStream_
     G.ensure_batch(B);
     maybe_autotune(f, s);
     pgk::WeightArgs w{f.S, f.Wo, f.WoH, f.AA};
+    const bool planes = pgk::zclass_active(G, B);
     {
         Prof p("fwd_prep_fft", s);
-        pgk::launch_fwd_prep_fft(G, x, 0, w, f.nc, f.L, G.tbuf, G.tbuf2, G.grids, s);
+        pgk::launch_fwd_prep_fft(G, x, 0, w, f.nc, f.L, G.tbuf, G.tbuf2, G.grids, s, planes);
     }
     {
         Prof p("interp", s);
         pgk::launch_interp(G, G.grids, f.nc, f.L, f.AA, f.AAi, y, y_cstride < 0 ? G.M : y_cstride,
-                           s);
+                           s, planes);
     }
 }
 
@@ -1090,6 +1197,7 @@ static void fused_adjoint_partial(const Fused& f, const float2* in, const float2
     maybe_autotune(f, s);
     pgk::WeightArgs w{f.S, f.Wo, f.WoH, f.AA};
     const long long ics = in_cstride < 0 ? G.M : in_cstride;
+    const bool planes = pgk::zclass_active(G, B);
     if (!vals_ready && pgk::spread_uses_values(G, B)) {
         Prof p("spread_values", s);
         pgk::launch_spread_values(G, in, ics, f.nc, f.L, f.AA, f.AAi, resid_y, ax64, Wd, s);
@@ -1098,11 +1206,12 @@ static void fused_adjoint_partial(const Fused& f, const float2* in, const float2
     {
         Prof p("spread", s);
         pgk::launch_spread(G, in, ics, f.nc, f.L, f.AA, f.AAi, resid_y, ax64, Wd, G.grids,
-                           vals_ready, s);
+                           vals_ready, s, planes);
     }
     {
         Prof p("adj_fft_post", s);
-        pgk::launch_adj_fft_post(G, G.grids, G.tbuf, G.tbuf2, w, f.nc, f.L, P, G.geo.N(), s);
+        pgk::launch_adj_fft_post(G, G.grids, G.tbuf, G.tbuf2, w, f.nc, f.L, P, G.geo.N(), s,
+                                 planes);
     }
 }
 
@@ -2132,33 +2241,32 @@ int pg_free(void* ptr)
 // With stream NULL the copies are ordered after all work on the device: the
 // library's own streams are non-blocking, so the null stream alone would not
 // wait for them.
-int pg_memcpy_htod(void* d, const void* h, size_t n, void* s)
+// A null stream means the legacy stream: the device is drained first, and
+// the copy and its synchronisation hold the legacy lock throughout, so that
+// no other thread's stream capture is open meanwhile.
+static void copy_sync(void* d, const void* src, size_t n, hipMemcpyKind k, void* s, bool sync)
 {
-    PG_TRY if (!s) {
-        LegacyLock lk(g_legacy_mu);
+    std::unique_lock<std::recursive_mutex> lk(g_legacy_mu, std::defer_lock);
+    if (!s) {
+        lk.lock();
         PG_HIP(hipDeviceSynchronize());
     }
-    PG_HIP(hipMemcpyAsync(d, h, n, hipMemcpyHostToDevice, (hipStream_t)s));
-    PG_HIP(hipStreamSynchronize((hipStream_t)s));
+    PG_HIP(hipMemcpyAsync(d, src, n, k, (hipStream_t)s));
+    if (sync) PG_HIP(hipStreamSynchronize((hipStream_t)s));
+}
+int pg_memcpy_htod(void* d, const void* h, size_t n, void* s)
+{
+    PG_TRY copy_sync(d, h, n, hipMemcpyHostToDevice, s, true);
     PG_CATCH
 }
 int pg_memcpy_dtoh(void* h, const void* d, size_t n, void* s)
 {
-    PG_TRY if (!s) {
-        LegacyLock lk(g_legacy_mu);
-        PG_HIP(hipDeviceSynchronize());
-    }
-    PG_HIP(hipMemcpyAsync(h, d, n, hipMemcpyDeviceToHost, (hipStream_t)s));
-    PG_HIP(hipStreamSynchronize((hipStream_t)s));
+    PG_TRY copy_sync(h, d, n, hipMemcpyDeviceToHost, s, true);
     PG_CATCH
 }
 int pg_memcpy_dtod(void* d, const void* src, size_t n, void* s)
 {
-    PG_TRY if (!s) {
-        LegacyLock lk(g_legacy_mu);
-        PG_HIP(hipDeviceSynchronize());
-    }
-    PG_HIP(hipMemcpyAsync(d, src, n, hipMemcpyDeviceToDevice, (hipStream_t)s));
+    PG_TRY copy_sync(d, src, n, hipMemcpyDeviceToDevice, s, s == nullptr);
     PG_CATCH
 }
 int pg_stream_create(void** s)
@@ -2178,7 +2286,9 @@ int pg_stream_destroy(void* s)
 }
 int pg_stream_synchronize(void* s)
 {
-    PG_TRY PG_HIP(hipStreamSynchronize((hipStream_t)s));
+    PG_TRY std::unique_lock<std::recursive_mutex> lk(g_legacy_mu, std::defer_lock);
+    if (!s) lk.lock();
+    PG_HIP(hipStreamSynchronize((hipStream_t)s));
     PG_CATCH
 }
 
@@ -2524,6 +2634,9 @@ int pg_op_set_mode(pg_op h, int mode)
 {
     PG_TRY Op* op = as_op(h);
     PG_REQUIRE(mode == PG_MODE_BATCHED || mode == PG_MODE_PLAIN, PG_ERR_ARG, "bad mode");
+    // solve graphs captured before (on this operator or on a composite over
+    // it) ran the old mode's kernels: every cached graph is recaptured
+    if (op->mode != mode) g_ws_gen.fetch_add(1);
     op->mode = mode;
     if (op->kind == K_PCSENSE) {
         // the per-shot operators are internal: they follow the composite
@@ -2540,6 +2653,9 @@ int pg_op_forward(pg_op h, const void* in, void* out, void* stream)
 {
     PG_TRY Op* op = as_op(h);
     PG_REQUIRE(in && out && in != out, PG_ERR_ARG, "bad buffers");
+    // launches on the legacy (null) stream hold the legacy lock
+    std::unique_lock<std::recursive_mutex> lk(g_legacy_mu, std::defer_lock);
+    if (!stream) lk.lock();
     op_forward(op, (const float2*)in, (float2*)out, (hipStream_t)stream);
     PG_HIP(hipGetLastError());
     PG_CATCH
@@ -2549,6 +2665,9 @@ int pg_op_adjoint(pg_op h, const void* in, void* out, void* stream)
 {
     PG_TRY Op* op = as_op(h);
     PG_REQUIRE(in && out && in != out, PG_ERR_ARG, "bad buffers");
+    // launches on the legacy (null) stream hold the legacy lock
+    std::unique_lock<std::recursive_mutex> lk(g_legacy_mu, std::defer_lock);
+    if (!stream) lk.lock();
     op_adjoint(op, (const float2*)in, (float2*)out, (hipStream_t)stream);
     PG_HIP(hipGetLastError());
     PG_CATCH
@@ -2949,16 +3068,31 @@ int pg_op_tuning(pg_op h, int B, int* interp_lds, int* spread_mfma)
     PG_CATCH
 }
 
+int pg_gnufft_zclasses(pg_op h, size_t* nzc)
+{
+    PG_TRY Op* op = as_op(h);
+    PG_REQUIRE(nzc, PG_ERR_ARG, "null argument");
+    if (op->kind == K_SENSE) op = static_cast<SenseOp*>(op)->inner;
+    if (op->kind == K_TS) op = static_cast<TsOp*>(op)->inner;
+    PG_REQUIRE(op->kind == K_NUFFT, PG_ERR_ARG, "not a Gnufft or an operator over one");
+    *nzc = (size_t)static_cast<NufftOp*>(op)->G.nzc;
+    PG_CATCH
+}
+
 int pg_set_tuning(const char* key, int value)
 {
     PG_TRY PG_REQUIRE(key, PG_ERR_ARG, "null key");
     const std::string k(key);
     if (k == "spread_mfma")
         pgk::g_spread_mfma.store(value ? 1 : 0);
+    else if (k == "zclass")
+        pgk::g_zclass.store(value ? 1 : 0);
     else if (k == "autotune")
         g_autotune.store(value ? 1 : 0);
     else
         throw pg::Error{PG_ERR_ARG, "unknown tuning key " + k};
+    // cached solve graphs captured the previous kernel variants
+    g_ws_gen.fetch_add(1);
     PG_CATCH
 }
 

@@ -413,6 +413,162 @@ __global__ void __launch_bounds__(512) k_line_reg(const float2* __restrict__ src
         }
 }
 
+// z-class planes (3-D plans whose samples take few distinct z windows,
+// pg_gnufft.h).  Forward: FFT along z of lines (b, gx, gy) of U (Nz non-zero
+// inputs, HALF as in k_line_reg), ifftshifted, then the z taps of every
+// class k applied to the line:
+//   H[k*B + b][gx, gy] = sum_j zw[k][j] Z[clamp(zlo[k] + j)]
+// (taps outside a class's clamped window carry weight 0).  A block holds
+// LPB consecutive (gx, gy) lines x R2 threads; the transformed line goes
+// through LDS, then thread t applies the classes t, t + R2, ...
+template <int R1, int R2, bool HALF>
+__global__ void __launch_bounds__(512) k_zfft_planes(
+    const float2* __restrict__ src, float2* __restrict__ dst, long long inner_n, int nin, int in_off,
+    int out_off, int B, int nzc, int NT, const int* __restrict__ zlo, const float* __restrict__ zw,
+    const float2* __restrict__ tw)
+{
+    constexpr int N = R1 * R2;
+    constexpr int LPB = line_lpb<R1, R2>();
+    constexpr int KST = R2 * LPB + 16;
+    static_assert(N * LPB <= R1 * KST, "LDS reuse");
+    __shared__ float2 lds[R1 * KST];
+    const int j = threadIdx.x % LPB, t = threadIdx.x / LPB;
+    const long long b = blockIdx.y;
+    const long long inner = (long long)blockIdx.x * LPB + j;
+    const bool valid = inner < inner_n;
+    const float2* s = src + b * (long long)nin * inner_n + (valid ? inner : 0);
+
+    float2 v[R1];
+#pragma unroll
+    for (int m = 0; m < R1; ++m) {
+        if (HALF && !half_live<R1>(m)) {
+            v[m] = make_float2(0.f, 0.f);
+            continue;
+        }
+        const int i = (t + R2 * m - in_off) & (N - 1);
+        const bool ok = valid && i < nin;
+        const float2 a = s[(long long)(ok ? i : 0) * inner_n];
+        v[m] = ok ? a : make_float2(0.f, 0.f);
+    }
+    pgf::line_fft<R1, R2, -1, true>(
+        v, t, lds, [j](int k1, int q) { return k1 * KST + q * LPB + j; }, tw);
+    // Z[o] of line j at lds[o * LPB + j] (line_fft ends with a barrier)
+#pragma unroll
+    for (int r = 0; r < R1 / R2; ++r)
+#pragma unroll
+        for (int k2 = 0; k2 < R2; ++k2) {
+            const int k = t + R2 * r + R1 * k2;
+            const int o = (k - out_off) & (N - 1);
+            lds[o * LPB + j] = v[r * R2 + k2];
+        }
+    __syncthreads();
+    if (!valid) return;
+    const long long pst = (long long)B * inner_n;  // between class planes
+    float2* d = dst + b * inner_n + inner;
+    for (int k = t; k < nzc; k += R2) {
+        const int l0 = zlo[k];
+        const float* wk = zw + (long long)k * NT;
+        float ar = 0.f, ai = 0.f;
+        for (int q = 0; q < NT; ++q) {
+            const int z = min(max(l0 + q, 0), N - 1);
+            const float wq = wk[q];
+            const float2 zv = lds[z * LPB + j];
+            ar += wq * zv.x;
+            ai += wq * zv.y;
+        }
+        d[(long long)k * pst] = make_float2(ar, ai);
+    }
+}
+
+// Adjoint of the above: the z line of grid b at (gx, gy) is rebuilt from the
+// class planes, Z[n] = sum over the classes covering n of zw * P[k*B + b]
+// (zent[zoff[n] .. zoff[n+1]), class order), ifftshifted on load, IFFT along
+// z, and the Nz cropped outputs (fftshift) are stored into U (b, z, gx, gy).
+template <int R1, int R2>
+__global__ void __launch_bounds__(512) k_zifft_planes(
+    const float2* __restrict__ src, float2* __restrict__ dst, long long inner_n, int nout,
+    int in_off, int out_off, int B, int nzc, const int* __restrict__ zoff,
+    const int2* __restrict__ zent, const float2* __restrict__ tw)
+{
+    constexpr int N = R1 * R2;
+    constexpr int LPB = line_lpb<R1, R2>();
+    constexpr int KST = R2 * LPB + 16;
+    static_assert(N * LPB <= R1 * KST, "LDS reuse");
+    __shared__ float2 lds[R1 * KST];
+    const int j = threadIdx.x % LPB, t = threadIdx.x / LPB;
+    const long long b = blockIdx.y;
+    const long long inner = (long long)blockIdx.x * LPB + j;
+    const bool valid = inner < inner_n;
+    const long long pst = (long long)B * inner_n;
+    const float2* s = src + b * inner_n + (valid ? inner : 0);
+    // class values of the line into LDS: P[k] at lds[k * LPB + j] (nzc <= N)
+    for (int k = t; k < nzc; k += R2) {
+        const float2 a = s[(long long)k * pst];
+        lds[k * LPB + j] = valid ? a : make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+    float2 v[R1];
+#pragma unroll
+    for (int m = 0; m < R1; ++m) {
+        const int n = (t + R2 * m - in_off) & (N - 1);  // grid z index at position t + R2 m
+        float ar = 0.f, ai = 0.f;
+        const int e1 = zoff[n + 1];
+        for (int e = zoff[n]; e < e1; ++e) {
+            const int2 ce = zent[e];
+            const float wq = __int_as_float(ce.y);
+            const float2 pv = lds[ce.x * LPB + j];
+            ar += wq * pv.x;
+            ai += wq * pv.y;
+        }
+        v[m] = make_float2(ar, ai);
+    }
+    __syncthreads();  // the class values are overwritten by the exchange
+    pgf::line_fft<R1, R2, 1, true>(
+        v, t, lds, [j](int k1, int q) { return k1 * KST + q * LPB + j; }, tw);
+    if (!valid) return;
+    float2* d = dst + b * (long long)nout * inner_n + inner;
+#pragma unroll
+    for (int r = 0; r < R1 / R2; ++r)
+#pragma unroll
+        for (int k2 = 0; k2 < R2; ++k2) {
+            const int k = t + R2 * r + R1 * k2;
+            const int o = (k - out_off) & (N - 1);
+            if (o < nout) d[(long long)o * inner_n] = v[r * R2 + k2];
+        }
+}
+
+template <int N>
+void zfft_planes_n(const pg::Gnufft& G, const float2* src, float2* dst, int B, hipStream_t s)
+{
+    constexpr int R1 = pgf::Split<N>::R1, R2 = pgf::Split<N>::R2;
+    constexpr int LPB = line_lpb<R1, R2>();
+    const pg::Geometry& g = G.geo;
+    const long long inner = (long long)g.Gx * g.Gy;
+    const dim3 grid((unsigned)((inner + LPB - 1) / LPB), (unsigned)B), block(LPB * R2);
+    const int in_off = (g.offZ + g.Gz / 2) % g.Gz;
+    if (2 * g.Nz == N && in_off == 3 * N / 4)
+        hipLaunchKernelGGL((k_zfft_planes<R1, R2, true>), grid, block, 0, s, src, dst, inner, g.Nz,
+                           in_off, g.Gz / 2, B, G.nzc, g.ntaps, G.zc_lo, G.zc_w,
+                           (const float2*)G.pz.tw);
+    else
+        hipLaunchKernelGGL((k_zfft_planes<R1, R2, false>), grid, block, 0, s, src, dst, inner, g.Nz,
+                           in_off, g.Gz / 2, B, G.nzc, g.ntaps, G.zc_lo, G.zc_w,
+                           (const float2*)G.pz.tw);
+}
+
+template <int N>
+void zifft_planes_n(const pg::Gnufft& G, const float2* src, float2* dst, int B, hipStream_t s)
+{
+    constexpr int R1 = pgf::Split<N>::R1, R2 = pgf::Split<N>::R2;
+    constexpr int LPB = line_lpb<R1, R2>();
+    const pg::Geometry& g = G.geo;
+    const long long inner = (long long)g.Gx * g.Gy;
+    const dim3 grid((unsigned)((inner + LPB - 1) / LPB), (unsigned)B), block(LPB * R2);
+    hipLaunchKernelGGL((k_zifft_planes<R1, R2>), grid, block, 0, s, src, dst, inner, g.Nz,
+                       g.Gz / 2, (g.offZ + g.Gz / 2) % g.Gz, B, G.nzc, G.zoff, G.zent,
+                       (const float2*)G.pz.tw);
+}
+
 // Adjoint tail for coil c: IFFT_y of the L segment columns, crop, deapod,
 // x WoH_l, sum over l, x conj(S_c).  Block = NW wavefronts over the same
 // LPW columns; wavefront w takes segments l = w, w+NW, ...; the NW partial
@@ -612,7 +768,7 @@ namespace pgk {
 
 void launch_fwd_prep_fft(const pg::Gnufft& G, const float2* img, long long img_cstride,
                          WeightArgs w, int nc, int L, float2* tbuf, float2* tbuf2, float2* grids,
-                         hipStream_t s)
+                         hipStream_t s, bool planes)
 {
     const pg::Geometry& g = G.geo;
     const int B = nc * L;
@@ -632,7 +788,14 @@ void launch_fwd_prep_fft(const pg::Gnufft& G, const float2* img, long long img_c
     float2* xdst = (g.dims == 2) ? grids : tbuf2;
     line_fft_any<-1>(tbuf, xdst, (long long)g.Gy, (long long)g.Nx * g.Gy, (long long)g.Gx * g.Gy,
                      g.Nx, (g.offX + g.Gx / 2) % g.Gx, g.Gx, g.Gx / 2, (unsigned)B * g.Nz, G.px, s);
-    if (g.dims == 3) {
+    if (planes) {
+        // FFT along z and the z taps of every class: the class planes
+        PG_REQUIRE(zclass_active(G, B) && reg_len(g.Gz) && G.nzc <= g.Gz, PG_ERR_STATE,
+                   "no z-class planes for this batch");
+#define PG_CALL_ZP(NN) zfft_planes_n<NN>(G, tbuf2, grids, B, s)
+        PG_REG_SWITCH(g.Gz, PG_CALL_ZP)
+#undef PG_CALL_ZP
+    } else if (g.dims == 3) {
         // FFT along z: lines (b, gx, gy), Nz non-zero inputs at (z + offZ + Gz/2) % Gz
         const long long inner = (long long)g.Gx * g.Gy;
         line_fft_any<-1>(tbuf2, grids, inner, (long long)g.Nz * inner, (long long)g.Gz * inner,
@@ -642,12 +805,20 @@ void launch_fwd_prep_fft(const pg::Gnufft& G, const float2* img, long long img_c
 
 void launch_adj_fft_post(const pg::Gnufft& G, float2* grids, float2* tbuf, float2* tbuf2,
                          WeightArgs w, int nc, int L, float2* out, long long out_cstride,
-                         hipStream_t s)
+                         hipStream_t s, bool planes)
 {
     const pg::Geometry& g = G.geo;
     const int B = nc * L;
     const float2* xsrc = grids;
-    if (g.dims == 3) {
+    if (planes) {
+        // z lines rebuilt from the class planes, IFFT along z, keep Nz planes
+        PG_REQUIRE(zclass_active(G, B) && reg_len(g.Gz) && G.nzc <= g.Gz, PG_ERR_STATE,
+                   "no z-class planes for this batch");
+#define PG_CALL_ZI(NN) zifft_planes_n<NN>(G, grids, tbuf2, B, s)
+        PG_REG_SWITCH(g.Gz, PG_CALL_ZI)
+#undef PG_CALL_ZI
+        xsrc = tbuf2;
+    } else if (g.dims == 3) {
         // IFFT along z on the spread grid (ifftshift on load), keep Nz planes
         const long long inner = (long long)g.Gx * g.Gy;
         line_fft_any<+1>(grids, tbuf2, inner, (long long)g.Gz * inner, (long long)g.Nz * inner,

@@ -67,6 +67,24 @@ struct Gnufft {
     int n_iitems = 0;
     float4* irec = nullptr;
     int itx = 0, ity = 0, ipadl = 0;
+    // 3-D trajectories whose samples take few distinct z windows (a stack of
+    // 2-D trajectories, kz on a Cartesian raster): the separable kernel is
+    // applied along z inside the z FFT pass, and the x-y gridding runs on nzc
+    // planes, one per z class, with the 2-D work lists above (built over
+    // patches and tiles of all planes: plane p, patch q at p*npp + q; interp
+    // items carry their plane in .w).  Class k has z window start zc_lo[k]
+    // and weights zc_w[k*ntaps + j] (0 outside its clamped window); the
+    // planes of B grids are stored plane-major, (k*B + b)*Gx*Gy.  For grid z
+    // index n, zent[zoff[n] .. zoff[n+1]) = {class, weight bits} lists the
+    // classes whose window covers n, in class order.  zcls[i]: sample i's
+    // class.  nzc = 0: the full 3-D grid is used.
+    int nzc = 0;
+    int npp = 0;  // 4x4 patches per plane
+    int* zc_lo = nullptr;
+    float* zc_w = nullptr;
+    int* zcls = nullptr;
+    int2* zent = nullptr;
+    int* zoff = nullptr;
     // 3-D spread (and B < 16): point-stationary tiles
     int ntx = 0, nty = 0, ntz = 0;
     int* tile_off = nullptr;  // ntiles + 1

@@ -43,7 +43,7 @@ __global__ void __launch_bounds__(256) k_interp(
     const int* __restrict__ lo, const float* __restrict__ w, int M, int Gx, int Gy, int Gz,
     const float2* __restrict__ grids, long long G, int nc, int L, const float* __restrict__ AA,
     const float* __restrict__ AAi, float2* __restrict__ out, long long out_cstride, int nsb,
-    int xcd_map)
+    int xcd_map, const int* __restrict__ pl, long long pstride)
 {
     int c, sb;
     const int p = blockIdx.x;
@@ -93,10 +93,12 @@ __global__ void __launch_bounds__(256) k_interp(
         wz[0] = 1.f;
         oz[0] = 0;
     }
+    // z-class planes (2-D gather on the plane of the sample's class)
+    const float2* gp = (!D3 && pl) ? grids + (long long)pl[ii] * pstride : grids;
     float tr = 0.f, ti = 0.f;
 #pragma unroll 1
     for (int l = 0; l < L; ++l) {
-        const float2* g = grids + ((long long)c * L + l) * G;
+        const float2* g = gp + ((long long)c * L + l) * G;
         float ar = 0.f, ai = 0.f;
 #pragma unroll
         for (int jz = 0; jz < (D3 ? NT : 1); ++jz) {
@@ -169,7 +171,7 @@ __global__ void __launch_bounds__(64 * kInterpWaves) k_interp_lds(
     const int4* __restrict__ items, const float4* __restrict__ rec, int ity, int padl,
     const float2* __restrict__ grids, long long G, int Gx, int Gy, int nc, int L,
     const float* __restrict__ AA, const float* __restrict__ AAi, int M, float2* __restrict__ out,
-    long long out_cstride)
+    long long out_cstride, long long pstride)
 {
     constexpr int R = pg::kIR;
     constexpr int CH = pg::kIChunk;
@@ -182,7 +184,8 @@ __global__ void __launch_bounds__(64 * kInterpWaves) k_interp_lds(
     const int n = it.z - it.y;  // 1..CH
     const int tx = it.x / ity, ty = it.x - (it.x / ity) * ity;
     const int rx0 = tx * pg::kIT - padl, ry0 = ty * pg::kIT - padl;
-    const float2* gc = grids + (long long)c * L * G;
+    // item plane it.w (z-class planes; 0 for a 2-D plan)
+    const float2* gc = grids + (long long)it.w * pstride + (long long)c * L * G;
     // Staging: every global load of the block is issued before any LDS
     // store, then one barrier.  The region goes as rows (xr, l) of R values
     // along y, 32 lanes per row (R <= 32), row and segment by shifts, the
@@ -407,7 +410,8 @@ constexpr int kSpreadWaves = 8;
 __global__ void __launch_bounds__(64 * kSpreadWaves) k_spread_patch(
     const int4* __restrict__ witems, const int* __restrict__ bgs, const int* __restrict__ entries,
     const float4* __restrict__ ew, int npy, const float2* __restrict__ V, int B,
-    float2* __restrict__ grids, long long G, int Gy, float2* __restrict__ partial, int n_slots)
+    float2* __restrict__ grids, long long G, int Gy, float2* __restrict__ partial, int n_slots,
+    int npp, long long pstride)
 {
     __shared__ float2 red[kSpreadWaves / 2][16][64];
     __shared__ float4 sw[kSpreadWaves][64][4];
@@ -512,8 +516,10 @@ __global__ void __launch_bounds__(64 * kSpreadWaves) k_spread_patch(
         for (int q = 0; q < 16; ++q) pp[q * 64] = acc[q];
         return;
     }
-    const int px0 = (it.x / npy) * pg::kPatch, py0 = (it.x % npy) * pg::kPatch;
-    float2* g = grids + (long long)b * G + (long long)px0 * Gy + py0;
+    // patch it.x = plane * npp + q (plane 0 for a 2-D plan)
+    const int pln = it.x / npp, q0 = it.x - pln * npp;
+    const int px0 = (q0 / npy) * pg::kPatch, py0 = (q0 % npy) * pg::kPatch;
+    float2* g = grids + pln * pstride + (long long)b * G + (long long)px0 * Gy + py0;
 #pragma unroll
     for (int a = 0; a < pg::kPatch; ++a)
 #pragma unroll
@@ -534,7 +540,8 @@ typedef float pg_f4 __attribute__((ext_vector_type(4)));
 __global__ void __launch_bounds__(64 * kSpreadWaves) k_spread_patch_mfma(
     const int4* __restrict__ witems, const int* __restrict__ bgs, const int* __restrict__ entries,
     const float4* __restrict__ ew, int npy, const float2* __restrict__ V, int B,
-    float2* __restrict__ grids, long long G, int Gy, float2* __restrict__ partial, int n_slots)
+    float2* __restrict__ grids, long long G, int Gy, float2* __restrict__ partial, int n_slots,
+    int npp, long long pstride)
 {
     __shared__ float red[kSpreadWaves / 2][32][64];
     __shared__ float sw[kSpreadWaves][64][16];
@@ -628,11 +635,13 @@ __global__ void __launch_bounds__(64 * kSpreadWaves) k_spread_patch_mfma(
                 pp[(4 * kq + q) * 64 + 4 * jq + u] = make_float2(acc[2 * u][q], acc[2 * u + 1][q]);
         return;
     }
-    const int px0 = (it.x / npy) * pg::kPatch, py0 = (it.x % npy) * pg::kPatch;
+    const int pln = it.x / npp, q0 = it.x - pln * npp;
+    const int px0 = (q0 / npy) * pg::kPatch, py0 = (q0 % npy) * pg::kPatch;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         const int p = 4 * kq + q;
-        const long long off = (long long)(px0 + p / pg::kPatch) * Gy + py0 + p % pg::kPatch;
+        const long long off = pln * pstride + (long long)(px0 + p / pg::kPatch) * Gy + py0 +
+                              p % pg::kPatch;
 #pragma unroll
         for (int u = 0; u < 4; ++u)
             grids[(long long)(b0 + u) * G + off] = make_float2(acc[2 * u][q], acc[2 * u + 1][q]);
@@ -645,7 +654,8 @@ __global__ void __launch_bounds__(64) k_spread_heavy_combine(const int4* __restr
                                                              const float2* __restrict__ partial,
                                                              int n_slots, int B,
                                                              float2* __restrict__ grids,
-                                                             long long G, int Gy)
+                                                             long long G, int Gy, int npp,
+                                                             long long pstride)
 {
     const int4 cb = comb[blockIdx.x];
     const int q = blockIdx.z;
@@ -659,8 +669,10 @@ __global__ void __launch_bounds__(64) k_spread_heavy_combine(const int4* __restr
         t.x += v.x;
         t.y += v.y;
     }
-    const int px0 = (cb.x / npy) * pg::kPatch, py0 = (cb.x % npy) * pg::kPatch;
-    grids[(long long)b * G + (long long)(px0 + q / pg::kPatch) * Gy + py0 + q % pg::kPatch] = t;
+    const int pln = cb.x / npp, q0 = cb.x - pln * npp;
+    const int px0 = (q0 / npy) * pg::kPatch, py0 = (q0 % npy) * pg::kPatch;
+    grids[pln * pstride + (long long)b * G + (long long)(px0 + q / pg::kPatch) * Gy + py0 +
+          q % pg::kPatch] = t;
 }
 
 // ------------------------------------------- 3-D tile spread (B >= 16) ----
@@ -775,15 +787,20 @@ __global__ void __launch_bounds__(256) k_spread_atomic(
 
 template <int NT, bool D3>
 void interp_dispatch(const pg::Gnufft& G, const float2* grids, int nc, int L, const float* AA,
-                     const float* AAi, float2* out, long long out_cstride, hipStream_t s)
+                     const float* AAi, float2* out, long long out_cstride, hipStream_t s,
+                     bool planes)
 {
     const pg::Geometry& g = G.geo;
     const pg::Gnufft::Tuned* tn = G.tuned_for(nc * L);
-    if (!D3 && NT <= 8 && pgk::interp_lds_possible(G, L, AA != nullptr) && (!tn || tn->lds)) {
+    // z-class planes: a 2-D interpolation on the plane of each sample's class
+    const long long gsz = planes ? (long long)g.Gx * g.Gy : g.G();
+    const long long pstride = planes ? (long long)nc * L * gsz : 0;
+    if ((!D3 || planes) && NT <= 8 && pgk::interp_lds_possible(G, L, AA != nullptr) &&
+        (!tn || tn->lds)) {
         const dim3 grid((unsigned)G.n_iitems * nc), block(64 * kInterpWaves);
         auto launch = [&](auto kern) {
             hipLaunchKernelGGL(kern, grid, block, 0, s, G.iitems, G.irec, G.ity, G.ipadl, grids,
-                               g.G(), g.Gx, g.Gy, nc, L, AA, AAi, G.M, out, out_cstride);
+                               gsz, g.Gx, g.Gy, nc, L, AA, AAi, G.M, out, out_cstride, pstride);
         };
         if (L <= 8)
             AAi ? launch(k_interp_lds<8, true>) : launch(k_interp_lds<8, false>);
@@ -793,18 +810,25 @@ void interp_dispatch(const pg::Gnufft& G, const float2* grids, int nc, int L, co
     }
     const int nsb = (G.M + 63) / 64;  // 64 samples (4 lanes each) per block
     const int xmap = (nc % 8 == 0) ? 1 : 0;
+    if (planes) {
+        hipLaunchKernelGGL((k_interp<NT, false>), dim3(nsb * nc), dim3(256), 0, s, G.lo, G.w, G.M,
+                           g.Gx, g.Gy, 1, grids, gsz, nc, L, AA, AAi, out, out_cstride, nsb, xmap,
+                           (const int*)G.zcls, pstride);
+        return;
+    }
     hipLaunchKernelGGL((k_interp<NT, D3>), dim3(nsb * nc), dim3(256), 0, s, G.lo, G.w, G.M, g.Gx,
-                       g.Gy, g.Gz, grids, g.G(), nc, L, AA, AAi, out, out_cstride, nsb, xmap);
+                       g.Gy, g.Gz, grids, g.G(), nc, L, AA, AAi, out, out_cstride, nsb, xmap,
+                       (const int*)nullptr, 0LL);
 }
 
 template <int NT, bool D3>
 void spread_dispatch(const pg::Gnufft& G, const float2* in, long long in_cstride, int nc, int L,
                      const float* AA, const float* AAi, const float2* resid_y, const double2* ax64,
-                     const float* Wd, float2* grids, bool vals_ready, hipStream_t s)
+                     const float* Wd, float2* grids, bool vals_ready, hipStream_t s, bool planes)
 {
     const pg::Geometry& g = G.geo;
     const int B = nc * L;
-    if (D3 && B < 16) {
+    if (D3 && !planes && B < 16) {
         const long long n = (long long)B * g.G();
         hipLaunchKernelGGL(k_zero, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, grids, n);
         hipLaunchKernelGGL((k_spread_atomic<NT, D3>), dim3((G.M + 255) / 256, B), dim3(256), 0, s,
@@ -815,7 +839,11 @@ void spread_dispatch(const pg::Gnufft& G, const float2* in, long long in_cstride
     // per-sample values of every grid, once
     if (!vals_ready)
         launch_values(in, in_cstride, resid_y, ax64, Wd, L, AA, AAi, G.M, B, G.vals, s);
-    if (!D3) {
+    if (!D3 || planes) {
+        // 2-D patches (of every z-class plane); plane p of grid b at
+        // (p*B + b)*Gx*Gy
+        const long long gsz = planes ? (long long)g.Gx * g.Gy : g.G();
+        const long long pstride = planes ? (long long)B * gsz : 0;
         const int nch = (B + 63) / 64;
         const int npy = g.Gy / pg::kPatch;
         const pg::Gnufft::Tuned* tn = G.tuned_for(B);
@@ -823,16 +851,18 @@ void spread_dispatch(const pg::Gnufft& G, const float2* in, long long in_cstride
             (tn ? tn->mfma != 0 : pgk::g_spread_mfma.load(std::memory_order_relaxed) != 0))
             hipLaunchKernelGGL(k_spread_patch_mfma, dim3(G.n_sblocks, nch), dim3(64 * kSpreadWaves),
                                0, s, G.witems, G.bgs, G.entries, (const float4*)G.ent_w, npy, G.vals,
-                               B, grids, g.G(), g.Gy, G.partial, G.n_slots);
+                               B, grids, gsz, g.Gy, G.partial, G.n_slots, G.npp, pstride);
         else
             hipLaunchKernelGGL(k_spread_patch, dim3(G.n_sblocks, nch), dim3(64 * kSpreadWaves), 0, s,
                                G.witems, G.bgs, G.entries, (const float4*)G.ent_w, npy, G.vals, B,
-                               grids, g.G(), g.Gy, G.partial, G.n_slots);
+                               grids, gsz, g.Gy, G.partial, G.n_slots, G.npp, pstride);
         if (G.n_comb > 0)
             hipLaunchKernelGGL(k_spread_heavy_combine, dim3(G.n_comb, nch, 16), dim3(64), 0, s, G.comb,
-                               npy, G.partial, G.n_slots, B, grids, g.G(), g.Gy);
+                               npy, G.partial, G.n_slots, B, grids, gsz, g.Gy, G.npp, pstride);
         return;
     }
+    PG_REQUIRE(G.tile_off && G.tile_idx, PG_ERR_STATE,
+               "plan built for z-class planes: no 3-D tile lists for a batch of this size");
     constexpr int BC = 16;
     const int ntiles = G.ntx * G.nty * G.ntz;
     hipLaunchKernelGGL((k_spread_tile3<NT, BC>), dim3(ntiles, (B + BC - 1) / BC), dim3(256), 0, s,
@@ -845,30 +875,34 @@ void spread_dispatch(const pg::Gnufft& G, const float2* in, long long in_cstride
 namespace pgk {
 
 std::atomic<int> g_spread_mfma{0};
+std::atomic<int> g_zclass{1};
+
+bool zclass_active(const pg::Gnufft& G, int B) { return G.nzc > 0 && B >= 16; }
 
 bool interp_lds_possible(const pg::Gnufft& G, int L, bool aa)
 {
     const pg::Geometry& g = G.geo;
-    return g.dims == 2 && g.ntaps <= 8 && aa && L >= 2 && L <= 16 && G.n_iitems > 0 &&
-           (long long)L * g.G() < (1LL << 31);
+    return (g.dims == 2 || G.nzc > 0) && g.ntaps <= 8 && aa && L >= 2 && L <= 16 &&
+           G.n_iitems > 0 && (long long)L * g.Gx * g.Gy < (1LL << 31);
 }
 
 void launch_interp(const pg::Gnufft& G, const float2* grids, int nc, int L, const float* AA,
-                   const float* AAi, float2* out, long long out_cstride, hipStream_t s)
+                   const float* AAi, float2* out, long long out_cstride, hipStream_t s, bool planes)
 {
+    PG_REQUIRE(!planes || zclass_active(G, nc * L), PG_ERR_STATE, "no z-class planes for this batch");
     const bool d3 = G.geo.dims == 3;
     switch (G.geo.ntaps) {
     case 8:
-        d3 ? interp_dispatch<8, true>(G, grids, nc, L, AA, AAi, out, out_cstride, s)
-           : interp_dispatch<8, false>(G, grids, nc, L, AA, AAi, out, out_cstride, s);
+        d3 ? interp_dispatch<8, true>(G, grids, nc, L, AA, AAi, out, out_cstride, s, planes)
+           : interp_dispatch<8, false>(G, grids, nc, L, AA, AAi, out, out_cstride, s, planes);
         break;
     case 6:
-        d3 ? interp_dispatch<6, true>(G, grids, nc, L, AA, AAi, out, out_cstride, s)
-           : interp_dispatch<6, false>(G, grids, nc, L, AA, AAi, out, out_cstride, s);
+        d3 ? interp_dispatch<6, true>(G, grids, nc, L, AA, AAi, out, out_cstride, s, planes)
+           : interp_dispatch<6, false>(G, grids, nc, L, AA, AAi, out, out_cstride, s, planes);
         break;
     case 12:
-        d3 ? interp_dispatch<12, true>(G, grids, nc, L, AA, AAi, out, out_cstride, s)
-           : interp_dispatch<12, false>(G, grids, nc, L, AA, AAi, out, out_cstride, s);
+        d3 ? interp_dispatch<12, true>(G, grids, nc, L, AA, AAi, out, out_cstride, s, planes)
+           : interp_dispatch<12, false>(G, grids, nc, L, AA, AAi, out, out_cstride, s, planes);
         break;
     default:
         throw pg::Error{PG_ERR_UNSUPPORTED, "kernel width * gridos must give 6, 8 or 12 taps"};
@@ -886,27 +920,28 @@ void launch_spread_values(const pg::Gnufft& G, const float2* in, long long in_cs
 
 void launch_spread(const pg::Gnufft& G, const float2* in, long long in_cstride, int nc, int L,
                    const float* AA, const float* AAi, const float2* resid_y, const double2* ax64,
-                   const float* Wd, float2* grids, bool vals_ready, hipStream_t s)
+                   const float* Wd, float2* grids, bool vals_ready, hipStream_t s, bool planes)
 {
+    PG_REQUIRE(!planes || zclass_active(G, nc * L), PG_ERR_STATE, "no z-class planes for this batch");
     const bool d3 = G.geo.dims == 3;
     switch (G.geo.ntaps) {
     case 8:
         d3 ? spread_dispatch<8, true>(G, in, in_cstride, nc, L, AA, AAi, resid_y, ax64, Wd,
-                                      grids, vals_ready, s)
+                                      grids, vals_ready, s, planes)
            : spread_dispatch<8, false>(G, in, in_cstride, nc, L, AA, AAi, resid_y, ax64, Wd,
-                                       grids, vals_ready, s);
+                                       grids, vals_ready, s, planes);
         break;
     case 6:
         d3 ? spread_dispatch<6, true>(G, in, in_cstride, nc, L, AA, AAi, resid_y, ax64, Wd,
-                                      grids, vals_ready, s)
+                                      grids, vals_ready, s, planes)
            : spread_dispatch<6, false>(G, in, in_cstride, nc, L, AA, AAi, resid_y, ax64, Wd,
-                                       grids, vals_ready, s);
+                                       grids, vals_ready, s, planes);
         break;
     case 12:
         d3 ? spread_dispatch<12, true>(G, in, in_cstride, nc, L, AA, AAi, resid_y, ax64, Wd,
-                                       grids, vals_ready, s)
+                                       grids, vals_ready, s, planes)
            : spread_dispatch<12, false>(G, in, in_cstride, nc, L, AA, AAi, resid_y, ax64, Wd,
-                                        grids, vals_ready, s);
+                                        grids, vals_ready, s, planes);
         break;
     default:
         throw pg::Error{PG_ERR_UNSUPPORTED, "kernel width * gridos must give 6, 8 or 12 taps"};

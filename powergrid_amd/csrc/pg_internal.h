@@ -104,6 +104,13 @@ namespace pgk {
 // 2-D batched spread on the matrix cores (k_spread_patch_mfma) when set and
 // the grid count is a multiple of 4; the vector-unit kernel otherwise
 extern std::atomic<int> g_spread_mfma;
+// plans of 3-D trajectories with few distinct z windows are built for the
+// z-class planes (pg_gnufft.h) when set (the default); 0 builds them for the
+// full 3-D grid
+extern std::atomic<int> g_zclass;
+
+// whether a batched launch of B grids of plan G runs on its z-class planes
+bool zclass_active(const pg::Gnufft& G, int B);
 
 struct WeightArgs {
     // image-side weights applied before the forward NUFFT and after the
@@ -115,20 +122,25 @@ struct WeightArgs {
     const float* AA;
 };
 
-// forward: image(s) -> batched grids  (deapod, pad, fftshift, FFT, ifftshift)
+// forward: image(s) -> batched grids  (deapod, pad, fftshift, FFT, ifftshift;
+// with `planes`, the z-class planes of the grids: the z taps applied in the
+// z pass)
 void launch_fwd_prep_fft(const pg::Gnufft& G, const float2* img, long long img_cstride,
                          WeightArgs w, int nc, int L, float2* tbuf, float2* tbuf2, float2* grids,
-                         hipStream_t s);
-// forward: batched grids -> samples (KB interpolation, sum over segments)
+                         hipStream_t s, bool planes = false);
+// forward: batched grids (or z-class planes) -> samples (KB interpolation,
+// sum over segments)
 void launch_interp(const pg::Gnufft& G, const float2* grids, int nc, int L, const float* AA,
-                   const float* AAi, float2* out, long long out_cstride, hipStream_t s);
+                   const float* AAi, float2* out, long long out_cstride, hipStream_t s,
+                   bool planes = false);
 // adjoint: samples -> batched grids (KB spread, deterministic tiles)
 // (with resid_y set, the samples are W (resid_y - ax64), ax64 the FP64 Ax of
 // the CG, and `in` is unused; with vals_ready the per-sample values
 // G.vals have already been formed, by the PCG update kernel)
 void launch_spread(const pg::Gnufft& G, const float2* in, long long in_cstride, int nc, int L,
                    const float* AA, const float* AAi, const float2* resid_y, const double2* ax64,
-                   const float* Wd, float2* grids, bool vals_ready, hipStream_t s);
+                   const float* Wd, float2* grids, bool vals_ready, hipStream_t s,
+                   bool planes = false);
 // whether launch_spread reads per-sample values (all but the 3-D atomic path)
 bool spread_uses_values(const pg::Gnufft& G, int B);
 // whether the LDS-staged interpolation serves L segments of plan G (the
@@ -138,10 +150,11 @@ bool interp_lds_possible(const pg::Gnufft& G, int L, bool aa);
 void launch_spread_values(const pg::Gnufft& G, const float2* in, long long in_cstride, int nc,
                           int L, const float* AA, const float* AAi, const float2* resid_y,
                           const double2* ax64, const float* Wd, hipStream_t s);
-// adjoint: batched grids -> image(s) (ifftshift, IFFT, fftshift, crop,
-// deapod, x WoH, sum over segments, x conj(S))
+// adjoint: batched grids (or z-class planes, expanded along z in the z
+// pass) -> image(s) (ifftshift, IFFT, fftshift, crop, deapod, x WoH, sum
+// over segments, x conj(S))
 void launch_adj_fft_post(const pg::Gnufft& G, float2* grids, float2* tbuf, float2* tbuf2,
                          WeightArgs w, int nc, int L, float2* out, long long out_cstride,
-                         hipStream_t s);
+                         hipStream_t s, bool planes = false);
 
 }  // namespace pgk

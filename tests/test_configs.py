@@ -162,6 +162,64 @@ def test_sos3d_full_size_properties():
     assert np.all(np.diff(info["resid"]) < 0)
 
 
+def _sos3d_full_grid(orc, precs):
+    """The 3-D workload's full grid (128x128x32, 256x256x64 oversampled,
+    L=12, TV) with 2 coils and 32 partitions x 2 interleaves x 2048 samples,
+    so that the oracle's FP64 restatement finishes in seconds; 24 grids per
+    launch run on the z-class planes."""
+    p = S.make_problem(128, 2, 2048, 2, 12, Nz=32, peak_hz=200.0, niter=2, penalty="tv",
+                       beta=1e-3, delta=1e-2, seed=0)
+    Gg = P.Gnufft(p.M, 2.0, p.Nx, p.Ny, p.Nz, p.kx, p.ky, p.kz)
+    assert P.zclasses(Gg) == 32
+    Sg = P.SENSE(P.TimeSegmentation(Gg, p.fm, p.t, p.M, p.N, p.L), p.sens, p.M, p.N, p.nc)
+    refs = {}
+    for prec in precs:
+        Go = orc.Gnufft(p.M, 2.0, p.Nx, p.Ny, p.Nz, p.kx, p.ky, p.kz, prec=prec)
+        To = orc.TimeSegmentation(Go, p.fm, p.t, p.M, p.N, p.L)
+        refs[prec] = (orc.SENSE(To, p.sens.ravel(), p.M, p.N, p.nc), To, Go)
+    return p, Sg, refs
+
+
+def test_sos3d_full_grid_operator(orc):
+    print("[sos3d full grid 128x128x32 nc=2 L=12 operator]", flush=True)
+    orc.set_threads(16)
+    try:
+        p, Sg, refs = _sos3d_full_grid(orc, "fd")
+        x = p.x_true.astype(np.complex64)
+        yf = Sg * x
+        y64 = refs["d"][0] * x
+        e64, e32 = rel(yf, y64), rel(yf, refs["f"][0] * x)
+        print(f"  forward: {e64:.2e} from FP64, {e32:.2e} from FP32", flush=True)
+        assert e64 < 5e-6 and e32 < 1e-4
+        xa = Sg / y64
+        a64, a32 = rel(xa, refs["d"][0] / y64), rel(xa, refs["f"][0] / y64)
+        print(f"  adjoint: {a64:.2e} from FP64, {a32:.2e} from FP32", flush=True)
+        assert a64 < 5e-6 and a32 < 1e-4
+    finally:
+        orc.set_threads(1)
+
+
+def test_sos3d_full_grid_pcg(orc):
+    print("[sos3d full grid 128x128x32 nc=2 L=12 TV, 2 PCG iterations]", flush=True)
+    orc.set_threads(16)
+    try:
+        p, Sg, refs = _sos3d_full_grid(orc, "d")
+        So = refs["d"][0]
+        # exact field-corrected data in scanner units (x 1e-3), 40 dB noise
+        y = 1e-3 * S.add_noise(So * p.x_true.astype(np.complex64), 40.0, seed=4)
+        y = y.astype(np.complex64)
+        Rg = P.TVPenalty(p.Nx, p.Ny, p.Nz, p.beta, p.delta)
+        Ro = orc.TVPenalty(p.Nx, p.Ny, p.Nz, p.beta, p.delta, prec="d")
+        xg, info = P.reconSolve(y, Sg, Rg, niter=2)
+        x64, n64, _ = orc.reconSolve(y.astype(np.complex128), So, Ro, p.Nx, p.Ny, p.Nz, 2)
+    finally:
+        orc.set_threads(1)
+    e = rel(xg, x64)
+    print(f"  gpu vs fp64 oracle PCG {e:.2e}", flush=True)
+    assert info["iterations"] == n64 == 2
+    assert e < 1e-3
+
+
 @pytest.mark.parametrize("N,nc,L", [(96, 16, 4), (192, 8, 12)])
 def test_uneven_batch_shapes_operator(orc, N, nc, L):
     print(f"[shape N={N} nc={nc} L={L}]", flush=True)

@@ -62,13 +62,22 @@ def test_gnufft2d_matches_oracle(orc, N, W):
     check(Gg / yo, Gf / yo, Gd / yo)
 
 
+@pytest.mark.parametrize("zclass", [1, 0])
 @pytest.mark.parametrize("N,Nz", [(16, 8), (16, 32)])
-def test_gnufft3d_matches_oracle(orc, N, Nz):
-    print(f"[gnufft3d {N}x{N}x{Nz}]", flush=True)
+def test_gnufft3d_matches_oracle(orc, N, Nz, zclass):
+    # a stack of spirals: kz on the partition raster, so the plan grids 16+
+    # grids on one x-y plane per partition (z taps in the z FFT pass), or,
+    # with zclass 0, on the full 3-D grid with the tiled spread
+    print(f"[gnufft3d {N}x{N}x{Nz} zclass={zclass}]", flush=True)
     kx, ky, kz = traj(N, 4, 256, Nz=Nz)
     M = len(kx)
     x = S.ellipse_phantom(N, Nz, seed=2).astype(np.complex64)
-    Gg = P.Gnufft(M, 2.0, N, N, Nz, kx, ky, kz)
+    P.set_tuning("zclass", zclass)
+    try:
+        Gg = P.Gnufft(M, 2.0, N, N, Nz, kx, ky, kz)
+    finally:
+        P.set_tuning("zclass", 1)
+    assert P.zclasses(Gg) == (Nz if zclass else 0)
     Gf, Gd = pair(orc, lambda p: orc.Gnufft(M, 2.0, N, N, Nz, kx, ky, kz, prec=p))
     check(Gg * x, Gf * x, Gd * x)
     yo = Gd * x
@@ -77,6 +86,27 @@ def test_gnufft3d_matches_oracle(orc, N, Nz):
     sens = S.coil_maps(N, 16, Nz).astype(np.complex64)
     Sg = P.SENSE(Gg, sens, M, x.size, 16)
     Sf, Sd = orc.SENSE(Gf, sens.ravel(), M, x.size, 16), orc.SENSE(Gd, sens.ravel(), M, x.size, 16)
+    check(Sg * x, Sf * x, Sd * x)
+    ys = Sd * x
+    check(Sg / ys, Sf / ys, Sd / ys)
+
+
+def test_gnufft3d_general_trajectory_full_grid(orc):
+    # kz off the partition raster (every sample its own z window): no z
+    # classes, the full 3-D grid with the tiled spread for 16 coils
+    print("[gnufft3d general kz]", flush=True)
+    N, Nz, nc = 16, 8, 16
+    kx, ky, kz = traj(N, 4, 128, Nz=Nz)
+    rng = np.random.default_rng(7)
+    kz = np.clip(kz + rng.uniform(-0.5, 0.5, kz.shape), -Nz / 2, Nz / 2 - 1e-3).astype(np.float32)
+    M = len(kx)
+    x = S.ellipse_phantom(N, Nz, seed=2).astype(np.complex64)
+    Gg = P.Gnufft(M, 2.0, N, N, Nz, kx, ky, kz)
+    assert P.zclasses(Gg) == 0
+    Gf, Gd = pair(orc, lambda p: orc.Gnufft(M, 2.0, N, N, Nz, kx, ky, kz, prec=p))
+    sens = S.coil_maps(N, nc, Nz).astype(np.complex64)
+    Sg = P.SENSE(Gg, sens, M, x.size, nc)
+    Sf, Sd = orc.SENSE(Gf, sens.ravel(), M, x.size, nc), orc.SENSE(Gd, sens.ravel(), M, x.size, nc)
     check(Sg * x, Sf * x, Sd * x)
     ys = Sd * x
     check(Sg / ys, Sf / ys, Sd / ys)
