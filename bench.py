@@ -513,7 +513,7 @@ def autotune_rate(P, prob, yd, stream, niter, steps=10):
     finally:
         P.set_tuning("autotune", 0)
     return dict(iters_per_s=round(steps * info["iterations"] / dt, 1),
-                interp="lds tiles" if lds == 1 else "gather",
+                interp={0: "gather", 1: "lds tiles, bins of 128", 2: "lds tiles, 4 segments per stage, bins of 64"}.get(lds, "?"),
                 spread="matrix cores" if mfma == 1 else "vector units")
 
 

@@ -193,6 +193,7 @@ int pg_op_adjoint(pg_op op, const void* in_dev, void* out_dev, void* stream);
 int pg_op_forward_host(pg_op op, const void* in_host, void* out_host);
 int pg_op_adjoint_host(pg_op op, const void* in_host, void* out_host);
 
+
 /* ---- gridding alone and direct reconstruction ---------------------------
  * The oversampled grid is Gx x Gy (x Gz) complex, y fastest, then x, then z
  * (gridding.cpp:100, idx = ny + nx*Gy + nz*Gx*Gy), with no FFT shift:
@@ -297,12 +298,15 @@ int pg_recon_solve_tol(pg_op A, pg_penalty R, const void* y_dev, size_t niter, d
                        int mode, void* x_out_dev, double* resid, pg_cg_info* info, void* stream);
 /* (no counterpart in the reference, which reconstructs one slice at a time)
  * K independent reconstructions -- reconSolve on A[k], R[k] (may be null),
- * y[k], each of its own size -- as one graph launch on `stream` (required):
- * every solve's captured graph becomes a child of one parent graph with no
- * edges between them, so the device runs them concurrently, each with its
- * own CG scalars.  Stops as pg_recon_solve_tol; infos (may be null) receives
- * K results.  The K operators must not share operator objects (including
- * inner Gnufft/TimeSegmentation plans): their workspaces are used at once. */
+ * y[k], each of its own size -- ordered after the work already on `stream`
+ * (required), which waits for all of them: every solve is one captured
+ * graph with its own CG scalars, and the graphs run concurrently on up to
+ * four of the library's streams (pg_set_tuning "batch_streams"), the most
+ * expensive first, each onto the stream with the least work so far; with
+ * "batch_streams" 0 they are the children of one parent graph launch.
+ * Stops as pg_recon_solve_tol; infos (may be null) receives K results.  The
+ * K operators must not share operator objects (including inner
+ * Gnufft/TimeSegmentation plans): their workspaces are used at once. */
 int pg_recon_solve_batch(size_t K, const pg_op* A, const pg_penalty* R, const void* const* y_dev,
                          size_t niter, double tol, void* const* x_out_dev, pg_cg_info* infos,
                          void* stream);
@@ -312,10 +316,11 @@ int pg_recon_solve_batch(size_t K, const pg_op* A, const pg_penalty* R, const vo
  * grid count is a multiple of 4; 0 (default): on the vector units.  Both
  * give the same sums up to FP32 rounding order.
  * "autotune" 1: when an operator is first prepared for a batch of B grids,
- * its Gnufft plan times each variant of the 2-D batched interpolation
- * (LDS-staged tiles or gather) and spread (vector units or matrix cores) on
- * its own workspaces and keeps the faster for that B, overriding
- * "spread_mfma"; 0 (default): the fixed choice.  The variants differ in
+ * its Gnufft plan times each variant of the 2-D (or z-class plane) batched
+ * interpolation (gather, or LDS-staged tiles with sample bins of 128 or 64)
+ * and spread (vector units or matrix cores) on its own workspaces and keeps
+ * the fastest for that B, overriding "spread_mfma" and "interp_lds4";
+ * 0 (default): the fixed choice.  The variants differ in
  * FP32 summation order only, but with autotuning on, which one runs -- and
  * so the last bits of a result -- depends on the timing.
  * "zclass" 1 (default): a 3-D Gnufft plan whose samples take at most Gz
@@ -324,6 +329,11 @@ int pg_recon_solve_batch(size_t K, const pg_op* A, const pg_penalty* R, const vo
  * the separable kernel's z taps inside the z FFT pass and grid in x-y on
  * one plane per class; 0: plans made afterwards grid on the full 3-D grid.
  * The two differ in FP32 summation order only.
+ * "interp_lds4" 1 (default): the LDS-staged interpolation stages four
+ * segments at a time (256-thread blocks, items of <= 64 samples) when a
+ * batch has more than 8 segments; 2: always; 0: never (all segments at once,
+ * 512-thread blocks, items of <= 128 samples).  Same sums, FP32 order aside.
+ * "batch_streams" n (0..8, default 4): streams of pg_recon_solve_batch.
  * Settings take effect for launches (and solve graphs) made after the call;
  * "zclass" for plans created after it. */
 int pg_set_tuning(const char* key, int value);
@@ -332,8 +342,10 @@ int pg_set_tuning(const char* key, int value);
 int pg_gnufft_zclasses(pg_op h, size_t* nzc);
 /* the variants the autotuner chose for the Gnufft plan under h (a Gnufft,
  * or a TimeSegmentation or SENSE over one) at B grids: interp_lds 1 =
- * LDS-staged tiles, 0 = gather; spread_mfma 1 = matrix cores, 0 = vector
- * units; both -1 when the plan was not tuned for B */
+ * LDS-staged tiles with every segment at once over bins of <= 128 samples,
+ * 2 = four segments at a time over bins of <= 64 samples, 0 = gather;
+ * spread_mfma 1 = matrix cores, 0 = vector units; both -1 when the plan was
+ * not tuned for B */
 int pg_op_tuning(pg_op h, int B, int* interp_lds, int* spread_mfma);
 
 /* ---- measurement --------------------------------------------------------

@@ -219,7 +219,8 @@ void Gnufft::ensure_batch(int B)
     dev_free(tbuf2);
     const Geometry& g = geo;
     grids = dev_alloc<float2>((size_t)B * g.G());
-    tbuf = dev_alloc<float2>((size_t)B * g.Nz * g.Nx * g.Gy);
+    // (z-class planes: the y FFT of B*nzc image-sized planes)
+    tbuf = dev_alloc<float2>((size_t)B * std::max(g.Nz, nzc) * g.Nx * g.Gy);
     if (g.dims == 3) tbuf2 = dev_alloc<float2>((size_t)B * g.Nz * g.Gx * g.Gy);
     dev_free(vals);
     vals = dev_alloc<float2>((size_t)M * B);
@@ -245,15 +246,14 @@ void Gnufft::release()
     dev_free(partial);
     dev_free(entries);
     dev_free(iitems);
+    dev_free(iitems4);
     dev_free(irec);
     dev_free(ent_w);
     dev_free(tile_off);
     dev_free(tile_idx);
-    dev_free(zc_lo);
-    dev_free(zc_w);
+    dev_free(zmat);
+    dev_free(dapxy);
     dev_free(zcls);
-    dev_free(zent);
-    dev_free(zoff);
     nzc = 0;
     dev_free(grids);
     dev_free(tbuf);
@@ -439,12 +439,14 @@ static void gnufft_setup(Gnufft& G, size_t M, float os, size_t Nx, size_t Ny, si
     // 3-D: z classes, the distinct (z window start, FP64 z weights) of the
     // samples with a non-empty window.  A stack of 2-D trajectories (kz on a
     // Cartesian raster) has one class per partition; a plan uses them when
-    // there are at most Gz and the z FFT runs in registers.
+    // there are at most Gz (and at most 128, with classes x Nz <= 4096 for
+    // the z matrix in LDS, Nz <= 128) and the y FFT runs in registers (Gy a
+    // power of two from 16 to 1024).
     std::vector<int> pl;  // plane of every sample (2-D: none, all on plane 0)
     int npl = 1;
     G.nzc = 0;
-    if (D == 3 && pgk::g_zclass.load() && (g.Gz & (g.Gz - 1)) == 0 && g.Gz >= 16 &&
-        g.Gz <= 1024 && G.fft_ok) {
+    if (D == 3 && pgk::g_zclass.load() && G.fft_ok && (g.Gy & (g.Gy - 1)) == 0 && g.Gy >= 16 &&
+        g.Gy <= 1024) {
         std::map<std::vector<uint64_t>, int> key2c;
         std::vector<int> cls(M, -1);
         std::vector<uint64_t> key(NT + 1), last;
@@ -469,48 +471,69 @@ static void gnufft_setup(Gnufft& G, size_t M, float os, size_t Nx, size_t Ny, si
             cls[i] = lastc = it->second;
             last = key;
         }
-        if (few && !key2c.empty()) {
+        if (few && !key2c.empty() && (long long)key2c.size() * g.Nz <= 4096 &&
+            key2c.size() <= 128 && g.Nz <= 128) {
             // classes in order of (window start, weights): std::map's order
             const int nzc = (int)key2c.size();
-            std::vector<int> remap(nzc), zlo(nzc);
-            std::vector<float> zw((size_t)nzc * NT);
+            std::vector<int> remap(nzc);
             int k = 0;
-            for (auto& kv : key2c) {
-                remap[kv.second] = k;
-                zlo[k] = (int)(int32_t)(uint32_t)kv.first[0];
-                for (int j = 0; j < NT; ++j) {
-                    double v;
-                    std::memcpy(&v, &kv.first[j + 1], 8);
-                    zw[(size_t)k * NT + j] = (float)v;
-                }
-                ++k;
-            }
+            for (auto& kv : key2c) remap[kv.second] = k++;
             pl.assign(M, 0);
             for (size_t i = 0; i < M; ++i)
                 if (cls[i] >= 0) pl[i] = remap[cls[i]];
-            // for each grid z index, the classes covering it (class order)
-            std::vector<int> zoff(g.Gz + 1, 0);
-            std::vector<int2> zent;
-            for (int n = 0; n < g.Gz; ++n) {
-                for (int c = 0; c < nzc; ++c) {
-                    const int j = n - zlo[c];
-                    if (j < 0 || j >= NT) continue;
-                    const float wv = zw[(size_t)c * NT + j];
-                    if (wv == 0.0f) continue;
-                    int bits;
-                    std::memcpy(&bits, &wv, 4);
-                    zent.push_back(make_int2(c, bits));
+            // The z pipeline of the forward transform for image plane z and
+            // class k's taps, with the z part of the deapodization, as one
+            // complex matrix built in FP64:
+            //   Zm[k][z] = dz(z) sum_j wz_k[j] exp(-2 pi i p(lo_k + j) q(z) / Gz)
+            // q(z) = (z + offZ + Gz/2) mod Gz places image plane z (zero
+            // padding + fftshift), p(n) = (n + Gz/2) mod Gz is the FFT output
+            // that grid plane n holds (ifftshift), taps off the grid weigh 0
+            // (gridding.cpp:67-73), dz = 1/(os psi_z) (0 where psi_z is NaN,
+            // griddingSupport.cpp:327-353).  The adjoint applies its
+            // conjugate transpose.
+            const int Gz = g.Gz, Nzi = g.Nz;
+            std::vector<float2> zm((size_t)nzc * Nzi);
+            k = 0;
+            for (auto& kv : key2c) {
+                const int l0 = (int)(int32_t)(uint32_t)kv.first[0];
+                for (int z = 0; z < Nzi; ++z) {
+                    const long long q = ((long long)z + g.offZ + Gz / 2) % Gz;
+                    double re = 0.0, im = 0.0;
+                    for (int j = 0; j < NT; ++j) {
+                        const int n = l0 + j;
+                        double wj;
+                        std::memcpy(&wj, &kv.first[j + 1], 8);
+                        if (n < 0 || n >= Gz || wj == 0.0) continue;
+                        const long long p = ((long long)n + Gz / 2) % Gz;
+                        const double ang = -2.0 * M_PI * (double)((p * q) % Gz) / (double)Gz;
+                        re += wj * std::cos(ang);
+                        im += wj * std::sin(ang);
+                    }
+                    const double pz =
+                        kb_ft(W, beta_d, ((double)z - (double)Nzi / 2.0) / (double)Nzi);
+                    const double dz = std::isnan(pz) ? 0.0 : std::ldexp(1.0, -wexp) / ((double)os * pz);
+                    zm[(size_t)k * Nzi + z] = make_float2((float)(re * dz), (float)(im * dz));
                 }
-                zoff[n + 1] = (int)zent.size();
+                ++k;
             }
-            if (zent.empty()) zent.push_back(make_int2(0, 0));
+            // the x-y part of the deapodization per image column (y fastest)
+            std::vector<float> dxy((size_t)g.Nx * g.Ny);
+            for (int X = 0; X < g.Nx; ++X) {
+                const double ax = kb_ft(W, beta_d, ((double)X - (double)(g.Nx / 2)) / (double)g.Nx);
+                for (int Y = 0; Y < g.Ny; ++Y) {
+                    const double ay =
+                        kb_ft(W, beta_d, ((double)Y - (double)(g.Ny / 2)) / (double)g.Ny);
+                    const double kk = ax * ay;
+                    dxy[(size_t)X * g.Ny + Y] =
+                        std::isnan(kk) ? 0.0f
+                                       : (float)(std::ldexp(1.0, -2 * wexp) / ((double)os * os * kk));
+                }
+            }
             G.nzc = nzc;
             npl = nzc;
-            G.zc_lo = dev_copy(zlo.data(), zlo.size());
-            G.zc_w = dev_copy(zw.data(), zw.size());
+            G.zmat = dev_copy(zm.data(), zm.size());
+            G.dapxy = dev_copy(dxy.data(), dxy.size());
             G.zcls = dev_copy(pl.data(), pl.size());
-            G.zoff = dev_copy(zoff.data(), zoff.size());
-            G.zent = dev_copy(zent.data(), zent.size());
         }
     }
     auto plane = [&](size_t i) { return pl.empty() ? 0 : pl[i]; };
@@ -559,7 +582,7 @@ static void gnufft_setup(Gnufft& G, size_t M, float os, size_t Nx, size_t Ny, si
             std::vector<int> pos(tcnt.begin(), tcnt.end() - 1), order(M);
             for (size_t i = 0; i < M; ++i) order[pos[tof[i]]++] = (int)i;
             std::vector<float4> rec((size_t)5 * M);
-            std::vector<int4> its;
+            std::vector<int4> its, its4;
             for (long long tb = 0; tb < nbin; ++tb) {
                 const int p = (int)(tb / nt), t = (int)(tb - (long long)p * nt);
                 const int tx = t / G.ity, ty = t % G.ity;
@@ -583,12 +606,16 @@ static void gnufft_setup(Gnufft& G, size_t M, float os, size_t Nx, size_t Ny, si
                 }
                 for (int k = tcnt[tb]; k < tcnt[tb + 1]; k += kIChunk)
                     its.push_back(make_int4(t, k, std::min(k + kIChunk, tcnt[tb + 1]), p));
+                for (int k = tcnt[tb]; k < tcnt[tb + 1]; k += kIChunk4)
+                    its4.push_back(make_int4(t, k, std::min(k + kIChunk4, tcnt[tb + 1]), p));
             }
             // items stay in (plane, tile raster) order: neighbouring tiles,
             // whose staged regions overlap, run at about the same time and
             // share L2 lines
             G.n_iitems = (int)its.size();
             G.iitems = dev_copy(its.data(), its.size());
+            G.n_iitems4 = (int)its4.size();
+            G.iitems4 = dev_copy(its4.data(), its4.size());
             G.irec = dev_copy(rec.data(), rec.size());
         }  // wider windows interpolate with the gather kernel
 
@@ -1080,7 +1107,8 @@ static std::atomic<int> g_autotune{0};
 static void autotune(Gnufft& G, int nc, int L, const float* AA, const float* AAi)
 {
     const int B = nc * L;
-    if (G.geo.dims != 2 || !G.fft_ok || G.tuned_for(B)) return;
+    const bool planes = pgk::zclass_active(G, B);
+    if ((G.geo.dims != 2 && !planes) || !G.fft_ok || G.tuned_for(B)) return;
     G.ensure_batch(B);
     const hipStream_t ss = setup_stream();
     hipEvent_t e[2] = {nullptr, nullptr};
@@ -1117,19 +1145,30 @@ static void autotune(Gnufft& G, int nc, int L, const float* AA, const float* AAi
     G.tuned.push_back(Gnufft::Tuned{B, 1, 0});
     Gnufft::Tuned& t = G.tuned.back();
     if (pgk::interp_lds_possible(G, L, AA != nullptr)) {
-        auto interp = [&] { pgk::launch_interp(G, G.grids, nc, L, AA, AAi, G.vals, G.M, ss); };
-        t.lds = 1;
-        const float ms_lds = timed(interp);
-        t.lds = 0;
-        const float ms_gather = timed(interp);
-        t.lds = ms_lds <= ms_gather ? 1 : 0;
+        // gather (0), all segments staged over bins of <= 128 samples (1),
+        // four segments per stage over bins of <= 64 samples (2)
+        auto interp = [&] {
+            pgk::launch_interp(G, G.grids, nc, L, AA, AAi, G.vals, G.M, ss, planes);
+        };
+        float best = 0.f;
+        int pick = 1;
+        for (int v = 0; v <= 2; ++v) {
+            if (v == 2 && G.n_iitems4 == 0) continue;
+            t.lds = v;
+            const float ms = timed(interp);
+            if (v == 0 || ms < best) {
+                best = ms;
+                pick = v;
+            }
+        }
+        t.lds = pick;
     } else {
         t.lds = 0;
     }
     if (B % 4 == 0) {
         auto spread = [&] {
             pgk::launch_spread(G, nullptr, G.M, nc, L, AA, AAi, nullptr, nullptr, nullptr, G.grids,
-                               true, ss);
+                               true, ss, planes);
         };
         t.mfma = 0;
         const float ms_vec = timed(spread);
@@ -1800,6 +1839,67 @@ static void collect_ops(Op* A, std::vector<const Op*>& out)
     }
 }
 
+// pg_set_tuning("batch_streams", n): the solves of a batch run on n of the
+// library's streams at once (0: as one parent graph)
+static std::atomic<int> g_batch_streams{4};
+
+// the library's streams for batches on the current device (default
+// priority, non-blocking; created on first use and kept)
+static hipStream_t batch_stream(int i)
+{
+    static std::map<int, std::vector<hipStream_t>>* per_dev =
+        new std::map<int, std::vector<hipStream_t>>();  // kept for the process
+    int dev = 0;
+    PG_HIP(hipGetDevice(&dev));
+    LegacyLock lk(g_legacy_mu);
+    std::vector<hipStream_t>& v = (*per_dev)[dev];
+    while ((int)v.size() <= i) {
+        hipStream_t st = nullptr;
+        PG_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        v.push_back(st);
+    }
+    return v[i];
+}
+
+// the captured solve graphs of a batch launched on nst streams at once,
+// the most expensive first, each onto the stream with the least work so far
+// (work ~ grids x (grid points + samples) of the fused launches); the
+// streams wait for everything enqueued on s before, and s waits for them
+static void solve_batch_streams(size_t K, Op* const* A, SolveCache* const* cs, int nst,
+                                hipStream_t s, std::vector<hipEvent_t>& ev)
+{
+    const int S = (int)std::min<size_t>((size_t)nst, K);
+    std::vector<double> cost(K);
+    for (size_t k = 0; k < K; ++k) {
+        Fused fz;
+        cost[k] = fusable(A[k], fz)
+                      ? (double)fz.nc * fz.L * ((double)fz.G->geo.G() + (double)fz.G->M)
+                      : (double)A[k]->n_in + (double)A[k]->n_out;
+    }
+    std::vector<size_t> order(K);
+    for (size_t k = 0; k < K; ++k) order[k] = k;
+    std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return cost[a] > cost[b]; });
+    ev.assign(S + 1, nullptr);
+    {
+        LegacyLock lk(g_legacy_mu);
+        for (int i = 0; i <= S; ++i) PG_HIP(hipEventCreateWithFlags(&ev[i], hipEventDisableTiming));
+    }
+    std::vector<hipStream_t> st(S);
+    for (int i = 0; i < S; ++i) st[i] = batch_stream(i);
+    PG_HIP(hipEventRecord(ev[S], s));
+    for (int i = 0; i < S; ++i) PG_HIP(hipStreamWaitEvent(st[i], ev[S], 0));
+    std::vector<double> load(S, 0.0);
+    for (size_t k : order) {
+        const int i = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        PG_HIP(hipGraphLaunch(cs[k]->exec, st[i]));
+        load[i] += cost[k];
+    }
+    for (int i = 0; i < S; ++i) {
+        PG_HIP(hipEventRecord(ev[i], st[i]));
+        PG_HIP(hipStreamWaitEvent(s, ev[i], 0));
+    }
+}
+
 // K independent reconstructions (reconSolve each: W = ones, x0 = 0) as one
 // graph launch: each solve's captured graph is a child of one parent graph,
 // with no edges between them, so the device runs them concurrently
@@ -1818,6 +1918,28 @@ static void solve_batch(size_t K, Op* const* A, Penalty* const* R, const float2*
     }
     for (size_t k = 0; k < K; ++k)
         pgv::fill(cs[k]->w.x32, (long long)A[k]->n_in, make_float2(0.f, 0.f), s);
+    const int nst = g_batch_streams.load();
+    if (nst > 0) {
+        // the fork/join events are destroyed once s has drained (solve_results
+        // synchronizes it), also when a launch fails
+        std::vector<hipEvent_t> ev;
+        struct Evs {
+            std::vector<hipEvent_t>& e;
+            hipStream_t s;
+            ~Evs()
+            {
+                LegacyLock lk(g_legacy_mu);
+                hipStreamSynchronize(s);
+                for (hipEvent_t x : e)
+                    if (x) hipEventDestroy(x);
+            }
+        } guard{ev, s};
+        solve_batch_streams(K, A, cs.data(), nst, s, ev);
+        for (size_t k = 0; k < K; ++k)
+            solve_results(cs[k], (long long)A[k]->n_in, x_out[k], nullptr,
+                          infos ? infos + k : nullptr, s);
+        return;
+    }
     // the parent graph of these K solve graphs, from a small cache keyed by
     // the children's unique ids (a recaptured or destroyed solve never
     // matches a stale entry)
@@ -3087,6 +3209,10 @@ int pg_set_tuning(const char* key, int value)
         pgk::g_spread_mfma.store(value ? 1 : 0);
     else if (k == "zclass")
         pgk::g_zclass.store(value ? 1 : 0);
+    else if (k == "batch_streams")
+        g_batch_streams.store(value < 0 ? 0 : (value > 8 ? 8 : value));
+    else if (k == "interp_lds4")
+        pgk::g_interp4.store(value < 0 ? 0 : (value > 2 ? 2 : value));
     else if (k == "autotune")
         g_autotune.store(value ? 1 : 0);
     else

@@ -298,7 +298,8 @@ int pick_rpw(int n) { return n <= 256 ? 16 : 8; }
 // unconditionally at a clamped in-bounds address and masked afterwards, so
 // that all loads of a thread are in flight together.  dap[n] =
 // 1/(os^d psi(n)), 0 where psi is NaN (griddingSupport.cpp:262-274).
-// HAS: bit 0 = coil maps S, bit 1 = segment phases Wo.
+// HAS: bit 0 = coil maps S, bit 1 = segment phases Wo, bit 2 = no
+// deapodization (a plain zero-padded FFT of each image).
 // HALF: the zero-padded layout of gridos 2 (Ny = N/2 image rows placed from
 // row 3N/4 on, cyclically): entry m of a thread is an image row exactly when
 // (m - 3 R1/4) mod R1 < R1/2, whatever the thread, so the other half of the
@@ -345,7 +346,7 @@ __global__ void __launch_bounds__(256) k_fwd_col_reg(
         const bool ok = valid && y < Ny;
         const int yc = ok ? y : 0;
         float2 a = im[yc];
-        const float d = dp[yc];
+        const float d = (HAS & 4) ? 1.f : dp[yc];
         if (HAS & 1) a = pgf::cmul(Sc[yc], a);
         if (HAS & 2) a = pgf::cmul(Wl[yc], a);
         v[m] = ok ? make_float2(a.x * d, a.y * d) : make_float2(0.f, 0.f);
@@ -413,160 +414,174 @@ __global__ void __launch_bounds__(512) k_line_reg(const float2* __restrict__ src
         }
 }
 
-// z-class planes (3-D plans whose samples take few distinct z windows,
-// pg_gnufft.h).  Forward: FFT along z of lines (b, gx, gy) of U (Nz non-zero
-// inputs, HALF as in k_line_reg), ifftshifted, then the z taps of every
-// class k applied to the line:
-//   H[k*B + b][gx, gy] = sum_j zw[k][j] Z[clamp(zlo[k] + j)]
-// (taps outside a class's clamped window carry weight 0).  A block holds
-// LPB consecutive (gx, gy) lines x R2 threads; the transformed line goes
-// through LDS, then thread t applies the classes t, t + R2, ...
-template <int R1, int R2, bool HALF>
-__global__ void __launch_bounds__(512) k_zfft_planes(
-    const float2* __restrict__ src, float2* __restrict__ dst, long long inner_n, int nin, int in_off,
-    int out_off, int B, int nzc, int NT, const int* __restrict__ zlo, const float* __restrict__ zw,
-    const float2* __restrict__ tw)
-{
-    constexpr int N = R1 * R2;
-    constexpr int LPB = line_lpb<R1, R2>();
-    constexpr int KST = R2 * LPB + 16;
-    static_assert(N * LPB <= R1 * KST, "LDS reuse");
-    __shared__ float2 lds[R1 * KST];
-    const int j = threadIdx.x % LPB, t = threadIdx.x / LPB;
-    const long long b = blockIdx.y;
-    const long long inner = (long long)blockIdx.x * LPB + j;
-    const bool valid = inner < inner_n;
-    const float2* s = src + b * (long long)nin * inner_n + (valid ? inner : 0);
-
-    float2 v[R1];
-#pragma unroll
-    for (int m = 0; m < R1; ++m) {
-        if (HALF && !half_live<R1>(m)) {
-            v[m] = make_float2(0.f, 0.f);
-            continue;
-        }
-        const int i = (t + R2 * m - in_off) & (N - 1);
-        const bool ok = valid && i < nin;
-        const float2 a = s[(long long)(ok ? i : 0) * inner_n];
-        v[m] = ok ? a : make_float2(0.f, 0.f);
-    }
-    pgf::line_fft<R1, R2, -1, true>(
-        v, t, lds, [j](int k1, int q) { return k1 * KST + q * LPB + j; }, tw);
-    // Z[o] of line j at lds[o * LPB + j] (line_fft ends with a barrier)
-#pragma unroll
-    for (int r = 0; r < R1 / R2; ++r)
-#pragma unroll
-        for (int k2 = 0; k2 < R2; ++k2) {
-            const int k = t + R2 * r + R1 * k2;
-            const int o = (k - out_off) & (N - 1);
-            lds[o * LPB + j] = v[r * R2 + k2];
-        }
-    __syncthreads();
-    if (!valid) return;
-    const long long pst = (long long)B * inner_n;  // between class planes
-    float2* d = dst + b * inner_n + inner;
-    for (int k = t; k < nzc; k += R2) {
-        const int l0 = zlo[k];
-        const float* wk = zw + (long long)k * NT;
-        float ar = 0.f, ai = 0.f;
-        for (int q = 0; q < NT; ++q) {
-            const int z = min(max(l0 + q, 0), N - 1);
-            const float wq = wk[q];
-            const float2 zv = lds[z * LPB + j];
-            ar += wq * zv.x;
-            ai += wq * zv.y;
-        }
-        d[(long long)k * pst] = make_float2(ar, ai);
-    }
-}
-
-// Adjoint of the above: the z line of grid b at (gx, gy) is rebuilt from the
-// class planes, Z[n] = sum over the classes covering n of zw * P[k*B + b]
-// (zent[zoff[n] .. zoff[n+1]), class order), ifftshifted on load, IFFT along
-// z, and the Nz cropped outputs (fftshift) are stored into U (b, z, gx, gy).
+// IFFT along y of the columns of every plane b (ifftshift on load), keeping
+// the Ny rows of the crop (fftshift + crop_center_region), unweighted:
+//   out[b * out_pstride + col * Ny + y]
+// The same line layout as k_fwd_col_reg: one line per R2 threads, LPW lines
+// per wavefront, four wavefronts per block.
 template <int R1, int R2>
-__global__ void __launch_bounds__(512) k_zifft_planes(
-    const float2* __restrict__ src, float2* __restrict__ dst, long long inner_n, int nout,
-    int in_off, int out_off, int B, int nzc, const int* __restrict__ zoff,
-    const int2* __restrict__ zent, const float2* __restrict__ tw)
+__global__ void __launch_bounds__(256) k_col_ifft_crop(const float2* __restrict__ T, int ncol,
+                                                       int Ny, int offY,
+                                                       const float2* __restrict__ tw,
+                                                       float2* __restrict__ out,
+                                                       long long out_pstride)
 {
     constexpr int N = R1 * R2;
-    constexpr int LPB = line_lpb<R1, R2>();
-    constexpr int KST = R2 * LPB + 16;
-    static_assert(N * LPB <= R1 * KST, "LDS reuse");
-    __shared__ float2 lds[R1 * KST];
-    const int j = threadIdx.x % LPB, t = threadIdx.x / LPB;
+    constexpr int LPW = 64 / R2;
+    constexpr int PITCH = R1 * (R2 + 1);
+    __shared__ float2 lds[4 * LPW * PITCH];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int t = lane % R2, jl = lane / R2;
     const long long b = blockIdx.y;
-    const long long inner = (long long)blockIdx.x * LPB + j;
-    const bool valid = inner < inner_n;
-    const long long pst = (long long)B * inner_n;
-    const float2* s = src + b * inner_n + (valid ? inner : 0);
-    // class values of the line into LDS: P[k] at lds[k * LPB + j] (nzc <= N)
-    for (int k = t; k < nzc; k += R2) {
-        const float2 a = s[(long long)k * pst];
-        lds[k * LPB + j] = valid ? a : make_float2(0.f, 0.f);
-    }
-    __syncthreads();
+    const int col = (blockIdx.x * 4 + wv) * LPW + jl;
+    const bool valid = col < ncol;
+    const float2* src = T + (b * ncol + (valid ? col : 0)) * N;
     float2 v[R1];
 #pragma unroll
-    for (int m = 0; m < R1; ++m) {
-        const int n = (t + R2 * m - in_off) & (N - 1);  // grid z index at position t + R2 m
-        float ar = 0.f, ai = 0.f;
-        const int e1 = zoff[n + 1];
-        for (int e = zoff[n]; e < e1; ++e) {
-            const int2 ce = zent[e];
-            const float wq = __int_as_float(ce.y);
-            const float2 pv = lds[ce.x * LPB + j];
-            ar += wq * pv.x;
-            ai += wq * pv.y;
-        }
-        v[m] = make_float2(ar, ai);
-    }
-    __syncthreads();  // the class values are overwritten by the exchange
-    pgf::line_fft<R1, R2, 1, true>(
-        v, t, lds, [j](int k1, int q) { return k1 * KST + q * LPB + j; }, tw);
+    for (int m = 0; m < R1; ++m) v[m] = src[(t + R2 * m + N / 2) & (N - 1)];
+    float2* my = lds + (wv * LPW + jl) * PITCH;
+    pgf::line_fft<R1, R2, 1, false>(v, t, my, [](int k1, int q) { return k1 * (R2 + 1) + q; }, tw);
     if (!valid) return;
-    float2* d = dst + b * (long long)nout * inner_n + inner;
+    const int shift = (offY + N / 2) % N;  // image row y sits at (y + shift) % N
+    float2* dst = out + b * out_pstride + (long long)col * Ny;
 #pragma unroll
     for (int r = 0; r < R1 / R2; ++r)
 #pragma unroll
         for (int k2 = 0; k2 < R2; ++k2) {
-            const int k = t + R2 * r + R1 * k2;
-            const int o = (k - out_off) & (N - 1);
-            if (o < nout) d[(long long)o * inner_n] = v[r * R2 + k2];
+            const int y = (t + R2 * r + R1 * k2 - shift) & (N - 1);
+            if (y < Ny) dst[y] = v[r * R2 + k2];
         }
 }
 
-template <int N>
-void zfft_planes_n(const pg::Gnufft& G, const float2* src, float2* dst, int B, hipStream_t s)
+// z stage of a plan on z-class planes (3-D stacks of 2-D trajectories).
+// Forward: for coil c, image column col = x Ny + y (Nxy of them) and segment
+// l, with v_z = dxy(col) Wo_l S_c x(z, col) over the Nz image planes
+// (SENSE.cpp:59-63, TimeSegmentation.cpp:266-270, the x-y deapodization),
+//   H[(k B + c L + l) Nxy + col] = sum_z zm[k][z] v_z
+// zm folds the zero padding, shifts, FFT and deapodization along z with
+// class k's z taps.  Block: 64 consecutive columns x ng wavefronts; the
+// wavefront g takes classes g, g + ng, ... (at most kZsKpt), so every load
+// and store of a wavefront is 64 consecutive columns; z in chunks of 8.
+constexpr int kZsKpt = 8;
+
+__global__ void __launch_bounds__(1024) k_zstage_fwd(
+    const float2* __restrict__ img, long long img_cstride, const float2* __restrict__ S,
+    const float2* __restrict__ Wo, const float* __restrict__ dxy, int Nxy, int Nz, int L, int B,
+    int nzc, const float2* __restrict__ zm, float2* __restrict__ H)
 {
-    constexpr int R1 = pgf::Split<N>::R1, R2 = pgf::Split<N>::R2;
-    constexpr int LPB = line_lpb<R1, R2>();
-    const pg::Geometry& g = G.geo;
-    const long long inner = (long long)g.Gx * g.Gy;
-    const dim3 grid((unsigned)((inner + LPB - 1) / LPB), (unsigned)B), block(LPB * R2);
-    const int in_off = (g.offZ + g.Gz / 2) % g.Gz;
-    if (2 * g.Nz == N && in_off == 3 * N / 4)
-        hipLaunchKernelGGL((k_zfft_planes<R1, R2, true>), grid, block, 0, s, src, dst, inner, g.Nz,
-                           in_off, g.Gz / 2, B, G.nzc, g.ntaps, G.zc_lo, G.zc_w,
-                           (const float2*)G.pz.tw);
-    else
-        hipLaunchKernelGGL((k_zfft_planes<R1, R2, false>), grid, block, 0, s, src, dst, inner, g.Nz,
-                           in_off, g.Gz / 2, B, G.nzc, g.ntaps, G.zc_lo, G.zc_w,
-                           (const float2*)G.pz.tw);
+    extern __shared__ float2 szm[];  // nzc x Nz
+    for (int e = threadIdx.x; e < nzc * Nz; e += blockDim.x) szm[e] = zm[e];
+    __syncthreads();
+    const int ng = blockDim.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int g = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int c = blockIdx.y;
+    const int col = blockIdx.x * 64 + lane;
+    if (col >= Nxy) return;
+    const long long N = (long long)Nxy * Nz;
+    const float2* xc = img + c * img_cstride + col;
+    const float2* Sc = S ? S + (long long)c * N + col : nullptr;
+    const float d = dxy[col];
+    for (int l = 0; l < L; ++l) {
+        const float2* Wl = Wo ? Wo + (long long)l * N + col : nullptr;
+        float2 acc[kZsKpt];
+#pragma unroll
+        for (int i = 0; i < kZsKpt; ++i) acc[i] = make_float2(0.f, 0.f);
+        for (int z0 = 0; z0 < Nz; z0 += 8) {
+            float2 v[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const int z = min(z0 + q, Nz - 1);
+                float2 a = xc[(long long)z * Nxy];
+                if (Sc) a = pgf::cmul(Sc[(long long)z * Nxy], a);
+                if (Wl) a = pgf::cmul(Wl[(long long)z * Nxy], a);
+                v[q] = (z0 + q < Nz) ? make_float2(a.x * d, a.y * d) : make_float2(0.f, 0.f);
+            }
+#pragma unroll
+            for (int i = 0; i < kZsKpt; ++i) {
+                const int k = g + i * ng;
+                if (k >= nzc) break;
+                const float2* zr = szm + k * Nz;
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    const float2 w = zr[min(z0 + q, Nz - 1)];
+                    acc[i].x += w.x * v[q].x - w.y * v[q].y;
+                    acc[i].y += w.x * v[q].y + w.y * v[q].x;
+                }
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < kZsKpt; ++i) {
+            const int k = g + i * ng;
+            if (k >= nzc) break;
+            H[((long long)k * B + (long long)c * L + l) * Nxy + col] = acc[i];
+        }
+    }
 }
 
-template <int N>
-void zifft_planes_n(const pg::Gnufft& G, const float2* src, float2* dst, int B, hipStream_t s)
+// Adjoint z stage: for coil c and image column col, over this wavefront's 8
+// image planes z,
+//   P_c(z, col) = conj(S_c) dxy(col) sum_l WoH_l(z, col) sum_k conj(zm[k][z]) Hp[(k B + c L + l) Nxy + col]
+// (TimeSegmentation.cpp:313-323, SENSE.cpp:84-92).  Block: 64 columns x
+// ceil(Nz / 8) wavefronts.
+__global__ void __launch_bounds__(1024) k_zstage_adj(
+    const float2* __restrict__ Hp, const float2* __restrict__ S, const float2* __restrict__ WoH,
+    const float* __restrict__ dxy, int Nxy, int Nz, int L, int B, int nzc,
+    const float2* __restrict__ zm, float2* __restrict__ out, long long out_cstride)
 {
-    constexpr int R1 = pgf::Split<N>::R1, R2 = pgf::Split<N>::R2;
-    constexpr int LPB = line_lpb<R1, R2>();
-    const pg::Geometry& g = G.geo;
-    const long long inner = (long long)g.Gx * g.Gy;
-    const dim3 grid((unsigned)((inner + LPB - 1) / LPB), (unsigned)B), block(LPB * R2);
-    hipLaunchKernelGGL((k_zifft_planes<R1, R2>), grid, block, 0, s, src, dst, inner, g.Nz,
-                       g.Gz / 2, (g.offZ + g.Gz / 2) % g.Gz, B, G.nzc, G.zoff, G.zent,
-                       (const float2*)G.pz.tw);
+    extern __shared__ float2 szm[];  // nzc x Nz
+    for (int e = threadIdx.x; e < nzc * Nz; e += blockDim.x) szm[e] = zm[e];
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    const int z0 = 8 * __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int c = blockIdx.y;
+    const int col = blockIdx.x * 64 + lane;
+    if (col >= Nxy) return;
+    const long long N = (long long)Nxy * Nz;
+    float2 acc[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) acc[q] = make_float2(0.f, 0.f);
+    for (int l = 0; l < L; ++l) {
+        const float2* hl = Hp + ((long long)c * L + l) * Nxy + col;
+        float2 u[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) u[q] = make_float2(0.f, 0.f);
+        for (int k = 0; k < nzc; ++k) {
+            const float2 h = hl[(long long)k * B * Nxy];
+            const float2* zr = szm + k * Nz;
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const float2 w = zr[min(z0 + q, Nz - 1)];
+                u[q].x += w.x * h.x + w.y * h.y;  // conj(w) h
+                u[q].y += w.x * h.y - w.y * h.x;
+            }
+        }
+        if (WoH) {
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const int z = min(z0 + q, Nz - 1);
+                u[q] = pgf::cmul(u[q], WoH[(long long)l * N + (long long)z * Nxy + col]);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            acc[q].x += u[q].x;
+            acc[q].y += u[q].y;
+        }
+    }
+    const float d = dxy[col];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const int z = z0 + q;
+        if (z >= Nz) break;
+        float2 v = make_float2(acc[q].x * d, acc[q].y * d);
+        if (S) {
+            float2 sc = S[(long long)c * N + (long long)z * Nxy + col];
+            sc.y = -sc.y;
+            v = pgf::cmul(v, sc);
+        }
+        out[c * out_cstride + (long long)z * Nxy + col] = v;
+    }
 }
 
 // Adjoint tail for coil c: IFFT_y of the L segment columns, crop, deapod,
@@ -762,6 +777,41 @@ void line_fft_any(const float2* src, float2* dst, long long inner_n, long long s
                        out_off, rpw, pg::fft_args(p));
 }
 
+// the y FFT of Bp zero-padded Nx x Ny planes (image layout, y fastest),
+// unweighted: planes[b][x][y] -> T[b][x][.] (k_fwd_col_reg without maps,
+// phases or deapodization)
+template <int N>
+void fwd_col_planes_n(const pg::Gnufft& G, const float2* planes, int Bp, float2* T, hipStream_t s)
+{
+    constexpr int R1 = pgf::Split<N>::R1, R2 = pgf::Split<N>::R2;
+    const pg::Geometry& g = G.geo;
+    const int lpw = 64 / R2;
+    const int waves = (g.Nx + lpw - 1) / lpw;
+    const int wpb = (waves * (long long)Bp >= 1024) ? 4 : 1;
+    const bool half = 2 * g.Ny == N && (g.offY + N / 2) % N == 3 * N / 4;
+    const dim3 grid((waves + wpb - 1) / wpb, Bp), block(64 * wpb);
+    const long long nxy = (long long)g.Nx * g.Ny;
+    if (half)
+        hipLaunchKernelGGL((k_fwd_col_reg<R1, R2, 4, true>), grid, block, 0, s, planes, nxy,
+                           (const float2*)nullptr, (const float2*)nullptr, (const float*)nullptr,
+                           g.Nx, g.Ny, 1, g.offY, 1, (const float2*)G.py.tw, T);
+    else
+        hipLaunchKernelGGL((k_fwd_col_reg<R1, R2, 4, false>), grid, block, 0, s, planes, nxy,
+                           (const float2*)nullptr, (const float2*)nullptr, (const float*)nullptr,
+                           g.Nx, g.Ny, 1, g.offY, 1, (const float2*)G.py.tw, T);
+}
+
+template <int N>
+void col_ifft_crop_n(const pg::Gnufft& G, const float2* T, int Bp, float2* planes, hipStream_t s)
+{
+    constexpr int R1 = pgf::Split<N>::R1, R2 = pgf::Split<N>::R2;
+    const pg::Geometry& g = G.geo;
+    const int lpw = 64 / R2;
+    const dim3 grid((g.Nx + 4 * lpw - 1) / (4 * lpw), Bp), block(256);
+    hipLaunchKernelGGL((k_col_ifft_crop<R1, R2>), grid, block, 0, s, T, g.Nx, g.Ny, g.offY,
+                       (const float2*)G.py.tw, planes, (long long)g.Nx * g.Ny);
+}
+
 }  // namespace
 
 namespace pgk {
@@ -773,6 +823,26 @@ void launch_fwd_prep_fft(const pg::Gnufft& G, const float2* img, long long img_c
     const pg::Geometry& g = G.geo;
     const int B = nc * L;
     const int ncol = g.Nx * g.Nz;
+    if (planes) {
+        // z-class planes: the z stage (maps, phases, deapodization, z FFT and
+        // z taps) into B*nzc image-sized planes (tbuf2), their y FFT (tbuf)
+        // and x FFT (grids, plane (k*B + b))
+        PG_REQUIRE(zclass_active(G, B) && reg_len(g.Gy), PG_ERR_STATE,
+                   "no z-class planes for this batch");
+        const int Bp = B * G.nzc;
+        const int nxy = g.Nx * g.Ny;
+        const int ng = (G.nzc + kZsKpt - 1) / kZsKpt;
+        const size_t lds = sizeof(float2) * (size_t)G.nzc * g.Nz;
+        hipLaunchKernelGGL(k_zstage_fwd, dim3((nxy + 63) / 64, nc), dim3(64 * ng), lds, s, img,
+                           img_cstride, w.S, w.Wo, G.dapxy, nxy, g.Nz, L, B, G.nzc, G.zmat, tbuf2);
+#define PG_CALL_COLP(NN) fwd_col_planes_n<NN>(G, tbuf2, Bp, tbuf, s)
+        PG_REG_SWITCH(g.Gy, PG_CALL_COLP)
+#undef PG_CALL_COLP
+        line_fft_any<-1>(tbuf, grids, (long long)g.Gy, (long long)g.Nx * g.Gy,
+                         (long long)g.Gx * g.Gy, g.Nx, (g.offX + g.Gx / 2) % g.Gx, g.Gx, g.Gx / 2,
+                         (unsigned)Bp, G.px, s);
+        return;
+    }
     if (reg_len(g.Gy)) {
 #define PG_CALL_COL(NN) fwd_col_reg_n<NN>(G, img, img_cstride, w, B, L, tbuf, s)
         PG_REG_SWITCH(g.Gy, PG_CALL_COL)
@@ -788,14 +858,7 @@ void launch_fwd_prep_fft(const pg::Gnufft& G, const float2* img, long long img_c
     float2* xdst = (g.dims == 2) ? grids : tbuf2;
     line_fft_any<-1>(tbuf, xdst, (long long)g.Gy, (long long)g.Nx * g.Gy, (long long)g.Gx * g.Gy,
                      g.Nx, (g.offX + g.Gx / 2) % g.Gx, g.Gx, g.Gx / 2, (unsigned)B * g.Nz, G.px, s);
-    if (planes) {
-        // FFT along z and the z taps of every class: the class planes
-        PG_REQUIRE(zclass_active(G, B) && reg_len(g.Gz) && G.nzc <= g.Gz, PG_ERR_STATE,
-                   "no z-class planes for this batch");
-#define PG_CALL_ZP(NN) zfft_planes_n<NN>(G, tbuf2, grids, B, s)
-        PG_REG_SWITCH(g.Gz, PG_CALL_ZP)
-#undef PG_CALL_ZP
-    } else if (g.dims == 3) {
+    if (g.dims == 3) {
         // FFT along z: lines (b, gx, gy), Nz non-zero inputs at (z + offZ + Gz/2) % Gz
         const long long inner = (long long)g.Gx * g.Gy;
         line_fft_any<-1>(tbuf2, grids, inner, (long long)g.Nz * inner, (long long)g.Gz * inner,
@@ -811,14 +874,26 @@ void launch_adj_fft_post(const pg::Gnufft& G, float2* grids, float2* tbuf, float
     const int B = nc * L;
     const float2* xsrc = grids;
     if (planes) {
-        // z lines rebuilt from the class planes, IFFT along z, keep Nz planes
-        PG_REQUIRE(zclass_active(G, B) && reg_len(g.Gz) && G.nzc <= g.Gz, PG_ERR_STATE,
+        // z-class planes: x IFFT of every plane keeping the Nx cropped
+        // columns (tbuf), y IFFT and crop (tbuf2: B*nzc image-sized planes),
+        // then the z stage into the coils' partial images
+        PG_REQUIRE(zclass_active(G, B) && reg_len(g.Gy), PG_ERR_STATE,
                    "no z-class planes for this batch");
-#define PG_CALL_ZI(NN) zifft_planes_n<NN>(G, grids, tbuf2, B, s)
-        PG_REG_SWITCH(g.Gz, PG_CALL_ZI)
-#undef PG_CALL_ZI
-        xsrc = tbuf2;
-    } else if (g.dims == 3) {
+        const int Bp = B * G.nzc;
+        const int nxy = g.Nx * g.Ny;
+        line_fft_any<+1>(grids, tbuf, (long long)g.Gy, (long long)g.Gx * g.Gy,
+                         (long long)g.Nx * g.Gy, g.Gx, g.Gx / 2, g.Nx, (g.offX + g.Gx / 2) % g.Gx,
+                         (unsigned)Bp, G.px, s);
+#define PG_CALL_CROP(NN) col_ifft_crop_n<NN>(G, tbuf, Bp, tbuf2, s)
+        PG_REG_SWITCH(g.Gy, PG_CALL_CROP)
+#undef PG_CALL_CROP
+        const int zg = (g.Nz + 7) / 8;
+        const size_t lds = sizeof(float2) * (size_t)G.nzc * g.Nz;
+        hipLaunchKernelGGL(k_zstage_adj, dim3((nxy + 63) / 64, nc), dim3(64 * zg), lds, s, tbuf2,
+                           w.S, w.WoH, G.dapxy, nxy, g.Nz, L, B, G.nzc, G.zmat, out, out_cstride);
+        return;
+    }
+    if (g.dims == 3) {
         // IFFT along z on the spread grid (ifftshift on load), keep Nz planes
         const long long inner = (long long)g.Gx * g.Gy;
         line_fft_any<+1>(grids, tbuf2, inner, (long long)g.Gz * inner, (long long)g.Nz * inner,

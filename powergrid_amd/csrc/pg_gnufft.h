@@ -18,6 +18,7 @@ constexpr int kPatch = 4;  // 2-D spread patch edge (grid points)
 constexpr int kIT = 16;    // 2-D interp: tile edge in window starts
 constexpr int kIR = kIT + 7;  // staged grid region edge (8 taps)
 constexpr int kIChunk = 128;  // samples per interp work item
+constexpr int kIChunk4 = 64;  // the same for the four-segment staging
 
 struct Gnufft {
     Geometry geo;
@@ -65,34 +66,35 @@ struct Gnufft {
     // (origin tile*kIT - ipadl) and zero weights past the taps
     int4* iitems = nullptr;
     int n_iitems = 0;
+    int4* iitems4 = nullptr;  // the same tiles cut into items of <= kIChunk4
+    int n_iitems4 = 0;
     float4* irec = nullptr;
     int itx = 0, ity = 0, ipadl = 0;
     // 3-D trajectories whose samples take few distinct z windows (a stack of
-    // 2-D trajectories, kz on a Cartesian raster): the separable kernel is
-    // applied along z inside the z FFT pass, and the x-y gridding runs on nzc
-    // planes, one per z class, with the 2-D work lists above (built over
-    // patches and tiles of all planes: plane p, patch q at p*npp + q; interp
-    // items carry their plane in .w).  Class k has z window start zc_lo[k]
-    // and weights zc_w[k*ntaps + j] (0 outside its clamped window); the
-    // planes of B grids are stored plane-major, (k*B + b)*Gx*Gy.  For grid z
-    // index n, zent[zoff[n] .. zoff[n+1]) = {class, weight bits} lists the
-    // classes whose window covers n, in class order.  zcls[i]: sample i's
-    // class.  nzc = 0: the full 3-D grid is used.
+    // 2-D trajectories, kz on a Cartesian raster): class k's z taps, with the
+    // zero padding, shifts, FFT and deapodization along z, form one complex
+    // matrix zmat[k*Nz + z] applied to the image's z columns, and the rest is
+    // the 2-D pipeline (y FFT, x FFT, x-y gridding) on nzc planes per grid.
+    // The x-y work lists above are built over the patches and tiles of all
+    // planes (plane p, patch q at p*npp + q; interp items carry their plane in
+    // .w); the planes of B grids are stored plane-major, (k*B + b)*Gx*Gy.
+    // dapxy: the x-y part of the deapodization per image column (y fastest);
+    // zcls[i]: sample i's class.  nzc = 0: the full 3-D grid is used.
     int nzc = 0;
     int npp = 0;  // 4x4 patches per plane
-    int* zc_lo = nullptr;
-    float* zc_w = nullptr;
+    float2* zmat = nullptr;
+    float* dapxy = nullptr;
     int* zcls = nullptr;
-    int2* zent = nullptr;
-    int* zoff = nullptr;
     // 3-D spread (and B < 16): point-stationary tiles
     int ntx = 0, nty = 0, ntz = 0;
     int* tile_off = nullptr;  // ntiles + 1
     int* tile_idx = nullptr;  // concatenated sample indices
     long long tile_entries = 0;
     // kernel variants chosen by the autotuner for a batch of B grids: the
-    // LDS-staged (1) or gather (0) interpolation, the matrix-core (1) or
-    // vector-unit (0) spread; no entry: the defaults
+    // interpolation as a gather (0), LDS-staged with all segments at once over
+    // bins of <= kIChunk samples (1) or four segments at a time over bins of
+    // <= kIChunk4 samples (2); the matrix-core (1) or vector-unit (0) spread;
+    // no entry: the defaults
     struct Tuned {
         int B, lds, mfma;
     };

@@ -299,6 +299,133 @@ __global__ void __launch_bounds__(64 * kInterpWaves) k_interp_lds(
     }
 }
 
+// The same interpolation with the segments staged four at a time: one block
+// of 256 threads per (work item of <= 64 samples, coil); lane task = (sample
+// s, segment slot l4) with s = tid / 4, so a block's 256 tasks cover its
+// item's samples once per chunk of four segments.  Per chunk the 23 x 23
+// region of the four segment grids (17 KB, [x][l][y]) is loaded into
+// registers (every load of the chunk in flight together), stored to LDS
+// after the previous chunk's taps are done, and each task forms
+//   acc_l = sum_x wx[x] sum_y wy[y] g_l(ox + x, oy + y)
+// and adds AA_l acc_l to its running sum; the four slots of a sample are
+// added with two DPP steps at the end.  The small LDS footprint (26 KB)
+// lets six blocks share a CU, so one block's staging overlaps the others'
+// taps.
+constexpr int kI4Threads = 256;
+constexpr int kI4Chunk = 64;
+
+template <bool CPX>
+__global__ void __launch_bounds__(kI4Threads) k_interp_lds4(
+    const int4* __restrict__ items, const float4* __restrict__ rec, int ity, int padl,
+    const float2* __restrict__ grids, long long G, int Gx, int Gy, int nc, int L,
+    const float* __restrict__ AA, const float* __restrict__ AAi, int M, float2* __restrict__ out,
+    long long out_cstride, long long pstride)
+{
+    constexpr int R = pg::kIR;
+    constexpr int LC = 4;
+    constexpr int CH = kI4Chunk;
+    constexpr int NTH = kI4Threads;
+    constexpr int NPT = R * LC * R;            // grid points of one chunk
+    constexpr int NP = (NPT + NTH - 1) / NTH;  // staging loads per thread
+    static_assert(CH * LC == NTH, "one task per thread");
+    __shared__ float2 tile[NPT];
+    __shared__ float4 srec[CH * 5];
+    __shared__ float saa[CH * 16];
+    __shared__ float saai[CPX ? CH * 16 : 1];
+    const int tid = threadIdx.x;
+    const int c = blockIdx.x % nc;
+    const int4 it = items[blockIdx.x / nc];
+    const int n = it.z - it.y;  // 1..CH
+    const int tx = it.x / ity, ty = it.x - (it.x / ity) * ity;
+    const int rx0 = tx * pg::kIT - padl, ry0 = ty * pg::kIT - padl;
+    // item plane it.w (z-class planes; 0 for a 2-D plan)
+    const float2* gc = grids + (long long)it.w * pstride + (long long)c * L * G;
+    // records and the AA weights of every segment
+    for (int e = tid; e < n * 5; e += NTH) srec[e] = rec[(long long)it.y * 5 + e];
+    for (int e = tid; e < n * L; e += NTH) {
+        const int sl = e / L, l = e - (e / L) * L;
+        const int i = __float_as_int(rec[((long long)it.y + sl) * 5].x);
+        saa[sl * 16 + l] = AA[(long long)l * M + i];
+        if (CPX) saai[sl * 16 + l] = AAi[(long long)l * M + i];
+    }
+    const int s = tid >> 2, l4 = tid & 3;
+    const bool live = s < n;
+    int ox = 0, oy = 0, idx = 0;
+    float wx[8], wy[8];
+    float tr = 0.f, ti = 0.f;
+    for (int g = 0; g * LC < L; ++g) {
+        // stage chunk g: point p = (xr * LC + lc) * R + yr of the region
+        float2 tv[NP];
+#pragma unroll
+        for (int k = 0; k < NP; ++k) {
+            const int p = tid + k * NTH;
+            const int row = p / R, yr = p - (p / R) * R;
+            const int xr = row / LC, lc = row - (row / LC) * LC;
+            const int l = g * LC + lc, x = rx0 + xr, y = ry0 + yr;
+            const bool ok = p < NPT && l < L && x >= 0 && x < Gx && y >= 0 && y < Gy;
+            const int lcl = min(l, L - 1), xc = min(max(x, 0), Gx - 1), yc = min(max(y, 0), Gy - 1);
+            const float2 v = gc[lcl * (int)G + xc * Gy + yc];
+            tv[k] = ok ? v : make_float2(0.f, 0.f);
+        }
+        if (g > 0) __syncthreads();  // the previous chunk's taps are done
+#pragma unroll
+        for (int k = 0; k < NP; ++k) {
+            const int p = tid + k * NTH;
+            if (p < NPT) tile[p] = tv[k];
+        }
+        __syncthreads();
+        if (g == 0) {
+            const float4* rp = srec + (live ? s : 0) * 5;
+            const float4 h = rp[0];
+            idx = __float_as_int(h.x);
+            ox = __float_as_int(h.y);
+            oy = __float_as_int(h.z);
+            const float4 xa = rp[1], xb = rp[2], ya = rp[3], yb = rp[4];
+            wx[0] = xa.x; wx[1] = xa.y; wx[2] = xa.z; wx[3] = xa.w;
+            wx[4] = xb.x; wx[5] = xb.y; wx[6] = xb.z; wx[7] = xb.w;
+            wy[0] = ya.x; wy[1] = ya.y; wy[2] = ya.z; wy[3] = ya.w;
+            wy[4] = yb.x; wy[5] = yb.y; wy[6] = yb.z; wy[7] = yb.w;
+        }
+        const int l = g * LC + l4;
+        if (live && l < L) {
+            const float2* tp = tile + ((ox * LC + l4) * R + oy);
+            float ar = 0.f, ai = 0.f;
+#pragma unroll
+            for (int x = 0; x < 8; ++x) {
+                const float2* row = tp + x * LC * R;
+                float rr = 0.f, ri = 0.f;
+#pragma unroll
+                for (int y = 0; y < 8; ++y) {
+                    const float2 v = row[y];
+                    rr += wy[y] * v.x;
+                    ri += wy[y] * v.y;
+                }
+                ar += wx[x] * rr;
+                ai += wx[x] * ri;
+            }
+            const float a = saa[s * 16 + l];
+            if (CPX) {
+                const float b = saai[s * 16 + l];
+                tr += a * ar - b * ai;
+                ti += a * ai + b * ar;
+            } else {
+                tr += a * ar;
+                ti += a * ai;
+            }
+        }
+    }
+    // the four segment slots of a sample (a quad of lanes)
+#define PG_DPP_ADDQ(v, CTRL)                                                                 \
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), (CTRL), 0xF, 0xF, \
+                                                    false))
+    PG_DPP_ADDQ(tr, 0xB1);  // quad_perm [1,0,3,2]
+    PG_DPP_ADDQ(ti, 0xB1);
+    PG_DPP_ADDQ(tr, 0x4E);  // quad_perm [2,3,0,1]
+    PG_DPP_ADDQ(ti, 0x4E);
+#undef PG_DPP_ADDQ
+    if (l4 == 0 && live) out[c * out_cstride + idx] = make_float2(tr, ti);
+}
+
 // value of sample i in grid b: AA_l * (in_c[i] or W (y_c[i] - Ax_c[i])),
 // the residual formed in FP64 from the FP64 Ax of the CG
 __device__ __forceinline__ float2 spread_value(const float2* __restrict__ in, long long in_cstride,
@@ -797,6 +924,21 @@ void interp_dispatch(const pg::Gnufft& G, const float2* grids, int nc, int L, co
     const long long pstride = planes ? (long long)nc * L * gsz : 0;
     if ((!D3 || planes) && NT <= 8 && pgk::interp_lds_possible(G, L, AA != nullptr) &&
         (!tn || tn->lds)) {
+        // four segments per stage when the autotuner chose it, or else when
+        // the segment count would leave the all-at-once staging a quarter
+        // idle (L > 8 pads to 16 slots)
+        const int v4 = pgk::g_interp4.load(std::memory_order_relaxed);
+        const bool four = tn ? tn->lds == 2 : (v4 == 2 || (v4 == 1 && L > 8));
+        if (G.n_iitems4 > 0 && four) {
+            const dim3 grid((unsigned)G.n_iitems4 * nc), block(kI4Threads);
+            auto launch = [&](auto kern) {
+                hipLaunchKernelGGL(kern, grid, block, 0, s, G.iitems4, G.irec, G.ity, G.ipadl,
+                                   grids, gsz, g.Gx, g.Gy, nc, L, AA, AAi, G.M, out, out_cstride,
+                                   pstride);
+            };
+            AAi ? launch(k_interp_lds4<true>) : launch(k_interp_lds4<false>);
+            return;
+        }
         const dim3 grid((unsigned)G.n_iitems * nc), block(64 * kInterpWaves);
         auto launch = [&](auto kern) {
             hipLaunchKernelGGL(kern, grid, block, 0, s, G.iitems, G.irec, G.ity, G.ipadl, grids,
@@ -824,7 +966,8 @@ void interp_dispatch(const pg::Gnufft& G, const float2* grids, int nc, int L, co
 template <int NT, bool D3>
 void spread_dispatch(const pg::Gnufft& G, const float2* in, long long in_cstride, int nc, int L,
                      const float* AA, const float* AAi, const float2* resid_y, const double2* ax64,
-                     const float* Wd, float2* grids, bool vals_ready, hipStream_t s, bool planes)
+                     const float* Wd, float2* grids, bool vals_ready, hipStream_t s, bool planes,
+                     float2* vals, float2* partial)
 {
     const pg::Geometry& g = G.geo;
     const int B = nc * L;
@@ -838,7 +981,7 @@ void spread_dispatch(const pg::Gnufft& G, const float2* in, long long in_cstride
     }
     // per-sample values of every grid, once
     if (!vals_ready)
-        launch_values(in, in_cstride, resid_y, ax64, Wd, L, AA, AAi, G.M, B, G.vals, s);
+        launch_values(in, in_cstride, resid_y, ax64, Wd, L, AA, AAi, G.M, B, vals, s);
     if (!D3 || planes) {
         // 2-D patches (of every z-class plane); plane p of grid b at
         // (p*B + b)*Gx*Gy
@@ -850,15 +993,15 @@ void spread_dispatch(const pg::Gnufft& G, const float2* in, long long in_cstride
         if (B % 4 == 0 &&
             (tn ? tn->mfma != 0 : pgk::g_spread_mfma.load(std::memory_order_relaxed) != 0))
             hipLaunchKernelGGL(k_spread_patch_mfma, dim3(G.n_sblocks, nch), dim3(64 * kSpreadWaves),
-                               0, s, G.witems, G.bgs, G.entries, (const float4*)G.ent_w, npy, G.vals,
-                               B, grids, gsz, g.Gy, G.partial, G.n_slots, G.npp, pstride);
+                               0, s, G.witems, G.bgs, G.entries, (const float4*)G.ent_w, npy, vals,
+                               B, grids, gsz, g.Gy, partial, G.n_slots, G.npp, pstride);
         else
             hipLaunchKernelGGL(k_spread_patch, dim3(G.n_sblocks, nch), dim3(64 * kSpreadWaves), 0, s,
-                               G.witems, G.bgs, G.entries, (const float4*)G.ent_w, npy, G.vals, B,
-                               grids, gsz, g.Gy, G.partial, G.n_slots, G.npp, pstride);
+                               G.witems, G.bgs, G.entries, (const float4*)G.ent_w, npy, vals, B,
+                               grids, gsz, g.Gy, partial, G.n_slots, G.npp, pstride);
         if (G.n_comb > 0)
             hipLaunchKernelGGL(k_spread_heavy_combine, dim3(G.n_comb, nch, 16), dim3(64), 0, s, G.comb,
-                               npy, G.partial, G.n_slots, B, grids, gsz, g.Gy, G.npp, pstride);
+                               npy, partial, G.n_slots, B, grids, gsz, g.Gy, G.npp, pstride);
         return;
     }
     PG_REQUIRE(G.tile_off && G.tile_idx, PG_ERR_STATE,
@@ -867,7 +1010,7 @@ void spread_dispatch(const pg::Gnufft& G, const float2* in, long long in_cstride
     const int ntiles = G.ntx * G.nty * G.ntz;
     hipLaunchKernelGGL((k_spread_tile3<NT, BC>), dim3(ntiles, (B + BC - 1) / BC), dim3(256), 0, s,
                        G.lo, G.w, G.M, g.Gx, g.Gy, g.Gz, G.tile_off, G.tile_idx, G.ntx, G.nty,
-                       G.vals, B, grids, g.G());
+                       vals, B, grids, g.G());
 }
 
 }  // namespace
@@ -876,8 +1019,12 @@ namespace pgk {
 
 std::atomic<int> g_spread_mfma{0};
 std::atomic<int> g_zclass{1};
+std::atomic<int> g_interp4{1};
 
-bool zclass_active(const pg::Gnufft& G, int B) { return G.nzc > 0 && B >= 16; }
+bool zclass_active(const pg::Gnufft& G, int B)
+{
+    return G.nzc > 0 && B >= 16 && (long long)B * G.nzc <= 65535;
+}
 
 bool interp_lds_possible(const pg::Gnufft& G, int L, bool aa)
 {
@@ -913,35 +1060,39 @@ bool spread_uses_values(const pg::Gnufft& G, int B) { return !(G.geo.dims == 3 &
 
 void launch_spread_values(const pg::Gnufft& G, const float2* in, long long in_cstride, int nc,
                           int L, const float* AA, const float* AAi, const float2* resid_y,
-                          const double2* ax64, const float* Wd, hipStream_t s)
+                          const double2* ax64, const float* Wd, hipStream_t s, float2* vals)
 {
-    launch_values(in, in_cstride, resid_y, ax64, Wd, L, AA, AAi, G.M, nc * L, G.vals, s);
+    launch_values(in, in_cstride, resid_y, ax64, Wd, L, AA, AAi, G.M, nc * L, vals ? vals : G.vals,
+                  s);
 }
 
 void launch_spread(const pg::Gnufft& G, const float2* in, long long in_cstride, int nc, int L,
                    const float* AA, const float* AAi, const float2* resid_y, const double2* ax64,
-                   const float* Wd, float2* grids, bool vals_ready, hipStream_t s, bool planes)
+                   const float* Wd, float2* grids, bool vals_ready, hipStream_t s, bool planes,
+                   float2* vals, float2* partial)
 {
     PG_REQUIRE(!planes || zclass_active(G, nc * L), PG_ERR_STATE, "no z-class planes for this batch");
+    if (!vals) vals = G.vals;
+    if (!partial) partial = G.partial;
     const bool d3 = G.geo.dims == 3;
     switch (G.geo.ntaps) {
     case 8:
         d3 ? spread_dispatch<8, true>(G, in, in_cstride, nc, L, AA, AAi, resid_y, ax64, Wd,
-                                      grids, vals_ready, s, planes)
+                                      grids, vals_ready, s, planes, vals, partial)
            : spread_dispatch<8, false>(G, in, in_cstride, nc, L, AA, AAi, resid_y, ax64, Wd,
-                                       grids, vals_ready, s, planes);
+                                       grids, vals_ready, s, planes, vals, partial);
         break;
     case 6:
         d3 ? spread_dispatch<6, true>(G, in, in_cstride, nc, L, AA, AAi, resid_y, ax64, Wd,
-                                      grids, vals_ready, s, planes)
+                                      grids, vals_ready, s, planes, vals, partial)
            : spread_dispatch<6, false>(G, in, in_cstride, nc, L, AA, AAi, resid_y, ax64, Wd,
-                                       grids, vals_ready, s, planes);
+                                       grids, vals_ready, s, planes, vals, partial);
         break;
     case 12:
         d3 ? spread_dispatch<12, true>(G, in, in_cstride, nc, L, AA, AAi, resid_y, ax64, Wd,
-                                       grids, vals_ready, s, planes)
+                                       grids, vals_ready, s, planes, vals, partial)
            : spread_dispatch<12, false>(G, in, in_cstride, nc, L, AA, AAi, resid_y, ax64, Wd,
-                                        grids, vals_ready, s, planes);
+                                        grids, vals_ready, s, planes, vals, partial);
         break;
     default:
         throw pg::Error{PG_ERR_UNSUPPORTED, "kernel width * gridos must give 6, 8 or 12 taps"};

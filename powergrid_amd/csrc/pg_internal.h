@@ -108,6 +108,11 @@ extern std::atomic<int> g_spread_mfma;
 // z-class planes (pg_gnufft.h) when set (the default); 0 builds them for the
 // full 3-D grid
 extern std::atomic<int> g_zclass;
+// the LDS-staged interpolation stages four segments at a time in 256-thread
+// blocks over items of <= 64 samples (k_interp_lds4): 2 always, 1 (the
+// default) for more than 8 segments, 0 never (all segments at once in
+// 512-thread blocks over items of <= 128 samples)
+extern std::atomic<int> g_interp4;
 
 // whether a batched launch of B grids of plan G runs on its z-class planes
 bool zclass_active(const pg::Gnufft& G, int B);
@@ -137,10 +142,12 @@ void launch_interp(const pg::Gnufft& G, const float2* grids, int nc, int L, cons
 // (with resid_y set, the samples are W (resid_y - ax64), ax64 the FP64 Ax of
 // the CG, and `in` is unused; with vals_ready the per-sample values
 // G.vals have already been formed, by the PCG update kernel)
+// (vals, partial: the per-sample value rows and the partial-sum slots of
+// heavy patches, when not the plan's own workspaces)
 void launch_spread(const pg::Gnufft& G, const float2* in, long long in_cstride, int nc, int L,
                    const float* AA, const float* AAi, const float2* resid_y, const double2* ax64,
                    const float* Wd, float2* grids, bool vals_ready, hipStream_t s,
-                   bool planes = false);
+                   bool planes = false, float2* vals = nullptr, float2* partial = nullptr);
 // whether launch_spread reads per-sample values (all but the 3-D atomic path)
 bool spread_uses_values(const pg::Gnufft& G, int B);
 // whether the LDS-staged interpolation serves L segments of plan G (the
@@ -149,7 +156,8 @@ bool interp_lds_possible(const pg::Gnufft& G, int L, bool aa);
 // the per-sample values G.vals of a spread (see launch_spread)
 void launch_spread_values(const pg::Gnufft& G, const float2* in, long long in_cstride, int nc,
                           int L, const float* AA, const float* AAi, const float2* resid_y,
-                          const double2* ax64, const float* Wd, hipStream_t s);
+                          const double2* ax64, const float* Wd, hipStream_t s,
+                          float2* vals = nullptr);
 // adjoint: batched grids (or z-class planes, expanded along z in the z
 // pass) -> image(s) (ifftshift, IFFT, fftshift, crop, deapod, x WoH, sum
 // over segments, x conj(S))

@@ -157,6 +157,28 @@ def test_sense_ts_matches_oracle(orc, mode, L):
     check(Sg / yo, Sf / yo, Sd / yo)
 
 
+@pytest.mark.parametrize("interp4", [1, 0])
+@pytest.mark.parametrize("L", [8, 12])
+def test_interp_staging_variants(orc, L, interp4):
+    # the LDS-staged interpolation with four segments per stage (256-thread
+    # blocks, items of <= 64 samples) and with all segments at once (512
+    # threads, <= 128 samples): both against the FP64 restatement
+    print(f"[interp variants L={L} interp_lds4={interp4}]", flush=True)
+    N, nc = 64, 2
+    kx, ky, kz, t, fm, x, M = _ts_case(N, nshot=8, ns=1024, L=L)
+    sens = S.coil_maps(N, nc).astype(np.complex64)
+    Gg = P.Gnufft(M, 2.0, N, N, 1, kx, ky, kz)
+    Sg = P.SENSE(P.TimeSegmentation(Gg, fm, t, M, N * N, L), sens, M, N * N, nc)
+    Gd = orc.Gnufft(M, 2.0, N, N, 1, kx, ky, kz, prec="d")
+    Sd = orc.SENSE(orc.TimeSegmentation(Gd, fm, t, M, N * N, L), sens.ravel(), M, N * N, nc)
+    P.set_tuning("interp_lds4", interp4)
+    try:
+        yg = Sg * x
+    finally:
+        P.set_tuning("interp_lds4", 1)
+    assert rel(yg, Sd * x) < 5e-6
+
+
 def test_sense_gnufft_direct(orc):
     print("[sense-gnufft]", flush=True)
     N, nc = 64, 4
