@@ -74,8 +74,8 @@ int pg_stream_synchronize(void* stream);
  * reference hard-codes 4.0 at Gnufft.cpp:84; W*gridos must be 6, 8 or 12).
  * Requirements: gridos*Nx integer and a multiple of 4, Nx == Ny (the
  * reference's zero_pad2d/circshift2 are x-fastest and only consistent for
- * square images, griddingSupport.cpp:488,565).  Grid lengths that are not
- * 2^a 3^b leave only the spatial interpolation available (operator* and
+ * square images, griddingSupport.cpp:488,565).  Grid lengths with a prime
+ * factor above 13 leave only the spatial interpolation available (operator* and
  * operator/ then return PG_ERR_UNSUPPORTED).  ix/iy/iz are accepted for
  * signature parity and may be NULL (Gnufft does not use them). */
 int pg_gnufft_create(pg_op* out, size_t M, float gridos, size_t Nx, size_t Ny, size_t Nz,

@@ -95,19 +95,26 @@ bool fft_length_supported(int n)
     if (n < 1) return false;
     while (n % 2 == 0) n /= 2;
     while (n % 3 == 0) n /= 3;
+    for (int r : {5, 7, 11, 13})
+        while (n % r == 0) n /= r;
     return n == 1;
 }
 
 FftPlan1D make_fft_plan(int n)
 {
     PG_REQUIRE(fft_length_supported(n), PG_ERR_UNSUPPORTED,
-               "FFT length " + std::to_string(n) + " is not of the form 2^a 3^b");
+               "FFT length " + std::to_string(n) + " has a prime factor above 13");
     FftPlan1D p;
     p.n = n;
     int rem = n;
     while (rem > 1) {
         PG_REQUIRE(p.nst < kMaxStages, PG_ERR_UNSUPPORTED, "FFT length too large");
-        int r = (rem % 4 == 0) ? 4 : (rem % 2 == 0) ? 2 : 3;
+        int r = 13;
+        for (int c : {4, 2, 3, 5, 7, 11})
+            if (rem % c == 0) {
+                r = c;
+                break;
+            }
         p.radix[p.nst++] = r;
         rem /= r;
     }
@@ -1093,7 +1100,7 @@ static void require_fft(const Gnufft& G)
     PG_REQUIRE(G.fft_ok, PG_ERR_UNSUPPORTED,
                "grid lengths " + std::to_string(G.geo.Gx) + "x" + std::to_string(G.geo.Gy) + "x" +
                    std::to_string(G.geo.Gz) +
-                   " are not of the form 2^a 3^b: only the spatial interpolation is available");
+                   " have a prime factor above 13: only the spatial interpolation is available");
 }
 
 // pg_set_tuning("autotune", 1): plans pick their kernel variants by timing

@@ -13,8 +13,8 @@
 //   adjoint: [IFFT_x (keep the Nx cropped outputs)]        -> T   (3-D: IFFT_z first)
 //            [IFFT_y + crop + deapod + WoH + sum_l + conj(S)] -> image
 // Power-of-two lengths 16..1024 use the register-resident two-level FFT of
-// pg_fftreg.h (one LDS exchange per line); other 2^a 3^b lengths use a mixed
-// radix-4/2/3 Stockham FFT staged in LDS.
+// pg_fftreg.h (one LDS exchange per line); other lengths whose prime factors
+// are at most 13 use a mixed radix-4/2/3/5/7/11/13 Stockham FFT staged in LDS.
 #include "pg_internal.h"
 #include "pg_gnufft.h"
 #include "pg_fftreg.h"
@@ -27,6 +27,36 @@ __device__ __forceinline__ float2 cmul(float2 a, float2 b)
 }
 __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+
+// One radix-R butterfly of a Stockham stage for an odd prime R (5, 7, 11,
+// 13): the R inputs in[r * nb] times the stage twiddles w^(r k) (tw:
+// exp(-2 pi i m / n), m < n; kt = k n / (Ns R)), then the R-point DFT
+//   out[q Ns] = sum_r v_r exp(SIGN 2 pi i q r / R)
+// whose roots are the table entries at multiples of n / R, the exponent q r
+// reduced mod R as it is stepped.  Scalar loops over runtime R with no
+// per-thread arrays; the inputs are re-read from LDS for every output.
+template <int SIGN>
+__device__ __forceinline__ void odd_dft(const float2* in, int nb, float2* out, int Ns, int R,
+                                        const float2* __restrict__ tw, int kt, int n)
+{
+    const int step = n / R;
+    for (int q = 0; q < R; ++q) {
+        float2 acc = in[0];
+        int e = 0;
+        for (int r = 1; r < R; ++r) {
+            e += q;
+            if (e >= R) e -= R;
+            float2 w = tw[r * kt];
+            float2 ro = tw[e * step];
+            if (SIGN > 0) {
+                w.y = -w.y;
+                ro.y = -ro.y;
+            }
+            acc = cadd(acc, cmul(cmul(in[r * nb], w), ro));
+        }
+        out[q * Ns] = acc;
+    }
+}
 
 // In-LDS Stockham FFT of `nlines` lines of length f.n (line i at A + i*pitch).
 // Returns the buffer holding the result (A or B).  SIGN = -1: forward
@@ -47,6 +77,11 @@ __device__ float2* lds_fft(float2* A, float2* B, int pitch, int nlines, const pg
             const int k = j % Ns;
             const float2* in = A + line * pitch;
             float2* out = B + line * pitch;
+            const int base = (j - k) * R + k;
+            if (R >= 5) {
+                odd_dft<SIGN>(in + j, nb, out + base, Ns, R, f.tw, k * tstride, n);
+                continue;
+            }
             float2 v0 = in[j], v1 = in[j + nb], v2, v3;
             float2 w1 = f.tw[k * tstride];
             if (SIGN > 0) w1.y = -w1.y;
@@ -61,7 +96,6 @@ __device__ float2* lds_fft(float2* A, float2* B, int pitch, int nlines, const pg
                 if (SIGN > 0) w3.y = -w3.y;
                 v3 = cmul(in[j + 3 * nb], w3);
             }
-            const int base = (j - k) * R + k;
             if (R == 4) {
                 float2 t0 = cadd(v0, v2), t1 = csub(v0, v2);
                 float2 t2 = cadd(v1, v3), d = csub(v1, v3);

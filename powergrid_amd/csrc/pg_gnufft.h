@@ -38,7 +38,7 @@ struct Gnufft {
     // dap[n] = 1/(os^d psi_x psi_y psi_z), 0 where psi is NaN
     float* dap = nullptr;
     // FFT plans for the three grid lengths; fft_ok is false when a grid
-    // length is not 2^a 3^b (only the spatial interpolation is available)
+    // length has a prime factor above 13 (only the spatial interpolation is available)
     FftPlan1D px, py, pz;
     bool fft_ok = false;
     // 2-D batched spread over 4x4 patches: blocks of 8 wavefront items

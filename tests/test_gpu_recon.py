@@ -74,9 +74,10 @@ def test_gnufft_gridos3_operator(orc):
 
 
 def test_gnufft_spatial_only_grid(orc):
-    # 3 x 100 = 300 points: not 2^a 3^b, so only the spatial pair exists
-    print("[gnufft spatial-only grid 300]", flush=True)
-    N = 100
+    # 3 x 68 = 204 = 2^2 3 17 points: a prime factor above 13, so only the
+    # spatial pair exists
+    print("[gnufft spatial-only grid 204]", flush=True)
+    N = 68
     kx, ky, kz = S.spiral(N, 4, 256)
     M = len(kx)
     G = P.Gnufft(M, 3.0, N, N, 1, kx, ky, kz)

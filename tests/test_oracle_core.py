@@ -81,7 +81,8 @@ def test_tap_window_is_clamped_not_wrapped(orc):
     assert cols.min() == 0 and cols.max() <= 4  # s = 0: taps 0..4, none at G-4..G-1
 
 
-@pytest.mark.parametrize("n", [8, 12, 64, 96, 128, 192, 256, 384, 10, 14])
+@pytest.mark.parametrize("n", [8, 12, 64, 96, 128, 192, 256, 384, 10, 14, 20, 28, 160, 176,
+                               208, 224])
 def test_fft1d_matches_numpy(orc, n):
     rng = np.random.default_rng(n)
     x = (rng.standard_normal(n) + 1j * rng.standard_normal(n))
