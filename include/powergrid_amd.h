@@ -193,6 +193,18 @@ int pg_op_adjoint(pg_op op, const void* in_dev, void* out_dev, void* stream);
 int pg_op_forward_host(pg_op op, const void* in_host, void* out_host);
 int pg_op_adjoint_host(pg_op op, const void* in_host, void* out_host);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops): the forward
+ * (adjoint = 0) or adjoint (adjoint != 0) of a Gdft / GdftR2 handle, or of a
+ * SENSE handle over one in PG_MODE_BATCHED, with the partial sums in a static
+ * 64 MiB device array of the library instead of the handle's workspace.  It
+ * calls no hipMalloc or hipFree; results are bit-identical to pg_op_forward /
+ * pg_op_adjoint.  An operator needing more than 8M complex partial sums
+ * (forward: slices x coils x M; adjoint: slices x N) is rejected with
+ * PG_ERR_ARG.  Every call on a device shares that device's array: calls that
+ * overlap on one device (other streams or threads) race. */
+int pg_dft_apply_static(pg_op op, int adjoint, const void* in_dev, void* out_dev, void* stream);
+
 
 /* ---- gridding alone and direct reconstruction ---------------------------
  * The oversampled grid is Gx x Gy (x Gz) complex, y fastest, then x, then z

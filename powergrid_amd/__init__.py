@@ -150,6 +150,19 @@ class _Op:
     def adjoint(self, y, out=None, stream=None):
         return self._apply(y, out, False, stream)
 
+    def apply_static(self, x, out, adjoint=False, stream=None):
+        """pg_dft_apply_static: the forward (or adjoint) of a Gdft / GdftR2
+        operator, or of a SENSE operator over one, with its partial sums in
+        the library's static device scratch: no device allocation.  x and out
+        are DeviceArrays the caller allocated."""
+        n_in, n_out = (self.n_out, self.n_in) if adjoint else (self.n_in, self.n_out)
+        for a, n in ((x, n_in), (out, n_out)):
+            if not isinstance(a, DeviceArray) or a.size != n or a.dtype != np.complex64:
+                raise ValueError(f"expected a complex64 device array of {n} elements")
+        check(lib().pg_dft_apply_static(self._h, 1 if adjoint else 0, x.ptr, out.ptr,
+                                        _sh(stream)))
+        return out
+
     def _apply(self, x, out, fwd, stream):
         n_in, n_out = (self.n_in, self.n_out) if fwd else (self.n_out, self.n_in)
         if isinstance(x, DeviceArray):

@@ -2802,6 +2802,44 @@ int pg_op_adjoint(pg_op h, const void* in, void* out, void* stream)
     PG_CATCH
 }
 
+int pg_dft_apply_static(pg_op h, int adjoint, const void* in, void* out, void* stream)
+{
+    PG_TRY Op* op = as_op(h);
+    PG_REQUIRE(in && out && in != out, PG_ERR_ARG, "bad buffers");
+    DftOp* d = nullptr;
+    const float2* S = nullptr;
+    int nc = 1;
+    if (op->kind == K_DFT) {
+        d = static_cast<DftOp*>(op);
+    } else if (op->kind == K_SENSE && static_cast<SenseOp*>(op)->inner->kind == K_DFT) {
+        SenseOp* se = static_cast<SenseOp*>(op);
+        PG_REQUIRE(se->mode == PG_MODE_BATCHED, PG_ERR_STATE,
+                   "pg_dft_apply_static runs the all-coil kernels: the SENSE operator must be in "
+                   "PG_MODE_BATCHED");
+        d = static_cast<DftOp*>(se->inner);
+        S = se->S;
+        nc = se->nc;
+    }
+    PG_REQUIRE(d, PG_ERR_UNSUPPORTED,
+               "pg_dft_apply_static takes a Gdft or GdftR2 handle, or a SENSE handle over one");
+    const size_t need = pgd::part_elems(d->d.M, d->d.N, nc, adjoint != 0);
+    const size_t cap = pgd::static_scratch_elems();
+    PG_REQUIRE(need <= cap, PG_ERR_ARG,
+               "this operator's " + std::string(adjoint ? "adjoint" : "forward") + " needs " +
+                   std::to_string(need) + " complex partial sums; the static scratch holds " +
+                   std::to_string(cap));
+    std::unique_lock<std::recursive_mutex> lk(g_legacy_mu, std::defer_lock);
+    if (!stream) lk.lock();
+    float2* part = pgd::static_scratch();
+    const hipStream_t s = (hipStream_t)stream;
+    if (adjoint)
+        pgd::adjoint(d->d, (const float2*)in, S, nc, (float2*)out, part, s);
+    else
+        pgd::forward(d->d, (const float2*)in, S, nc, (float2*)out, part, s);
+    PG_HIP(hipGetLastError());
+    PG_CATCH
+}
+
 static int host_apply(pg_op h, const void* in, void* out, bool fwd)
 {
     PG_TRY LegacyLock lk(g_legacy_mu);

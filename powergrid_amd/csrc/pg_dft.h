@@ -29,4 +29,12 @@ void adjoint(const DftData& d, const float2* y, const float2* S, int nc, float2*
              hipStream_t s, long long ystride = -1, bool accumulate = false, int nshot = 1,
              long long s_shot = 0);
 
+// The allocation-free entry point's partial-sum buffer: a file-scope device
+// array of static_scratch_elems() float2 (64 MiB) in pg_dft.hip, one copy per
+// device for the life of the process.  static_scratch() returns its device
+// address on the current device, resolved with hipGetSymbolAddress on the
+// first call per device and kept.
+size_t static_scratch_elems();
+float2* static_scratch();
+
 }  // namespace pgd

@@ -22,7 +22,14 @@
 //   maps G staged beside the pixel coordinates.
 #include "pg_dft.h"
 
+#include <mutex>
+
 namespace {
+
+// partial sums of pg_dft_apply_static: every element a launch reads was
+// written earlier in the same launch sequence, so nothing is cleared
+constexpr size_t kScratchElems = (size_t)8 << 20;  // 8M float2 = 64 MiB
+__device__ __align__(256) float2 scratch[kScratchElems];
 
 constexpr int kT = 64;  // threads per block (one wavefront) = tile width
 
@@ -397,6 +404,28 @@ void adjoint(const DftData& d, const float2* y, const float2* S, int nc, float2*
     hipLaunchKernelGGL(k_sum_slices, dim3((unsigned)((d.N + 255) / 256)), dim3(256), 0, s, part,
                        nshot * sps, sps, (long long)d.N, (long long)d.N, (long long)d.N,
                        accumulate ? 1 : 0, out);
+}
+
+size_t static_scratch_elems() { return kScratchElems; }
+
+float2* static_scratch()
+{
+    constexpr int kMaxDev = 64;
+    static std::mutex mu;
+    static float2* addr[kMaxDev] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDev)
+        throw pg::Error{PG_ERR_HIP, "no current device for the static scratch"};
+    std::lock_guard<std::mutex> lk(mu);
+    if (!addr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            throw pg::Error{PG_ERR_HIP, std::string("hipGetSymbolAddress(scratch) failed: ") +
+                                            hipGetErrorString(e)};
+        addr[dev] = (float2*)p;
+    }
+    return addr[dev];
 }
 
 }  // namespace pgd
