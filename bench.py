@@ -428,71 +428,92 @@ def rows_rates(P, S, stream, with_cpu=True):
     return out
 
 
-def serving_rates(P, S, n=96, rate=20.0, slots=4, seed=11):
-    """A serving run: n requests with Poisson arrivals at `rate` per second
-    into powergrid_amd.serve.ReconServer, each to a relative gradient
-    tolerance of 1e-4 or 30 PCG iterations.  The mix follows the serving
-    workload at sizes that finish in seconds: 70% 2-D 128^2 (spiral 16x2048,
-    L=8, 8/16/32 coils), 25% 3-D 64x64x16 (16 partitions x 4 x 1024 spiral,
-    8 coils, L=8) and 5% the same 3-D with L=16.  The k-space of each request
-    comes from the device forward operator plus 40 dB noise.  Latency runs
-    from submission to the image on the host; the request's operator set-up
-    is inside it."""
+def serving_requests(P, S, n, seed):
+    """n requests of the serving mix: 70% 2-D (the 2-D SENSE shape: 128^2,
+    spiral 16x2048, L=8, with 8, 16 or 32 coils), 25% 3-D with L=8 and 5% 3-D
+    with L=16, the 3-D ones 64x64x16 (16 partitions x 4 interleaves x 1024)
+    or 128x128x32 (32 partitions x 8 x 2048) with 8 coils; each to a relative
+    gradient tolerance of 1e-4 or 30 iterations.  The k-space of each shape
+    comes from the device forward operator plus 40 dB noise, scaled to unit
+    peak (the solve is invariant to the data scale, and the reference's 1e25
+    bound on the line-search denominator then never ends a solve early)."""
     from powergrid_amd import serve as SV
     rng = np.random.default_rng(seed)
     kinds = rng.choice(3, size=n, p=[0.70, 0.25, 0.05])
     coils2d = rng.choice([8, 16, 32], size=n)
-    reqs = []
-    cache = {}
+    big3d = rng.random(n) < 0.5
+    reqs, cache = [], {}
     for j in range(n):
         if kinds[j] == 0:
-            key = ("2d", int(coils2d[j]))
+            key = ("2d", int(coils2d[j]), 8)
             N, Nz, nc, L, nsh, ns = 128, 1, int(coils2d[j]), 8, 16, 2048
         else:
-            key = ("3d", 8 if kinds[j] == 1 else 16)
-            N, Nz, nc, L, nsh, ns = 64, 16, 8, key[1], 4, 1024
+            L = 8 if kinds[j] == 1 else 16
+            if big3d[j]:
+                key = ("3d128x128x32", 8, L)
+                N, Nz, nc, nsh, ns = 128, 32, 8, 8, 2048
+            else:
+                key = ("3d64x64x16", 8, L)
+                N, Nz, nc, nsh, ns = 64, 16, 8, 4, 1024
         if key not in cache:
             pr = S.make_problem(N, nsh, ns, nc, L, Nz=Nz, peak_hz=120.0, seed=seed + len(cache))
             M = len(pr.kx)
             G = P.Gnufft(M, 2.0, N, N, Nz, pr.kx, pr.ky, pr.kz)
             A = P.SENSE(P.TimeSegmentation(G, pr.fm, pr.t, M, pr.N, L), pr.sens, M, pr.N, nc)
             y = S.add_noise(A * pr.x_true, 40.0, seed=seed + 100 + len(cache))
-            # unit peak data: the solve is invariant to the data scale (x
-            # scales with y), and the reference's 1e25 bound on the line
-            # search denominator then never ends a solve early
             y = (y / np.abs(y).max()).astype(np.complex64)
             cache[key] = (pr, y)
+            del A, G
         pr, y = cache[key]
         reqs.append(SV.ReconRequest(pr.kx, pr.ky, pr.kz, pr.t, pr.Nx, pr.Ny, pr.Nz, pr.nc, pr.L,
                                     pr.fm, pr.sens, y, beta=1e4, niter=30, tol=1e-4,
                                     tag=(j, key)))
-    gaps = rng.exponential(1.0 / rate, size=n)
-    t_start = time.perf_counter()
-    with SV.ReconServer(slots=slots) as srv:
-        futs = []
-        t_next = t_start
-        for r, g in zip(reqs, gaps):
-            t_next += g
-            dt = t_next - time.perf_counter()
-            if dt > 0:
-                time.sleep(dt)
-            futs.append(srv.submit(r))
-        outs = [f.result() for f in futs]
-    t_end = max(o.finished for o in outs)
-    lat = np.array([o.latency for o in outs]) * 1e3
-    its = np.array([o.iterations for o in outs])
-    per = {}
-    for o in outs:
-        per.setdefault(str(o.tag[1]), []).append(o.latency * 1e3)
-    return dict(workload=f"{n} requests, Poisson {rate:g}/s, 70% 2-D 128^2 8-32 coils, "
-                         f"25% 3-D 64x64x16 L=8, 5% L=16; to tol 1e-4 or 30 iterations",
-                slots=slots, recons_per_s=round(n / (t_end - t_start), 2),
-                latency_ms_p50=round(float(np.percentile(lat, 50)), 2),
-                latency_ms_p99=round(float(np.percentile(lat, 99)), 2),
-                mean_iterations=round(float(its.mean()), 1),
-                exit_reasons=dict(collections.Counter(o.exit_reason for o in outs)),
-                max_concurrent=srv.max_running,
-                latency_ms_p50_by_kind={k: round(float(np.median(v)), 2) for k, v in per.items()})
+    return reqs
+
+
+def serving_rates(P, S, rates=(20.0, 100.0, 200.0), n=96, slots=4, seed=11):
+    """The serving run at each arrival rate: n requests of serving_requests()
+    with Poisson arrivals into powergrid_amd.serve.ReconServer.  Latency runs
+    from submission to the image on the host; the request's operator set-up
+    is inside it.  Above the server's capacity the queue grows and latency
+    with it; recons/s is then the capacity."""
+    from powergrid_amd import serve as SV
+    reqs = serving_requests(P, S, n, seed)
+    rng = np.random.default_rng(seed + 1)
+    out = {"workload": f"{n} requests per rate: 70% 2-D 128^2 spiral 16x2048 L=8 with 8/16/32 "
+                       f"coils, 25% 3-D L=8 and 5% 3-D L=16 (64x64x16 or 128x128x32, 8 coils); "
+                       f"to tol 1e-4 or 30 PCG iterations; {slots} slots",
+           "by_rate": {}}
+    for rate in rates:
+        gaps = rng.exponential(1.0 / rate, size=n)
+        t_start = time.perf_counter()
+        with SV.ReconServer(slots=slots) as srv:
+            futs = []
+            t_next = t_start
+            for r, g in zip(reqs, gaps):
+                t_next += g
+                dt = t_next - time.perf_counter()
+                if dt > 0:
+                    time.sleep(dt)
+                futs.append(srv.submit(r))
+            outs = [f.result() for f in futs]
+        t_end = max(o.finished for o in outs)
+        lat = np.array([o.latency for o in outs]) * 1e3
+        its = np.array([o.iterations for o in outs])
+        per = {}
+        for o in outs:
+            k = o.tag[1]
+            per.setdefault(f"{k[0]} {k[1]} coils L={k[2]}", []).append(o.latency * 1e3)
+        out["by_rate"][f"{rate:g}/s"] = dict(
+            recons_per_s=round(n / (t_end - t_start), 2),
+            latency_ms_p50=round(float(np.percentile(lat, 50)), 2),
+            latency_ms_p99=round(float(np.percentile(lat, 99)), 2),
+            mean_iterations=round(float(its.mean()), 1),
+            exit_reasons=dict(collections.Counter(o.exit_reason for o in outs)),
+            max_concurrent=srv.max_running,
+            latency_ms_p50_by_kind={k: round(float(np.median(v)), 2) for k, v in per.items()})
+        log(f"[bench] serving {rate:g}/s: {out['by_rate'][f'{rate:g}/s']}")
+    return out
 
 
 def autotune_rate(P, prob, yd, stream, niter, steps=10):
@@ -633,17 +654,27 @@ def kernel_times(P, Sg, x_host, stream, nrep, M, Gsz, B, nc, L, d):
     return kinfo
 
 
-def roofline(kinfo, L, summary):
-    """the roofline object of the dominant gridding kernel"""
+def roofline(kinfo, L, summary, plane_bytes=None):
+    """the roofline object of the dominant gridding kernel.  The interpolation
+    stages all L segments of a tile (k_interp_lds) up to 8 segments and four
+    at a time (k_interp_lds4) above.  plane_bytes: on z-class planes the
+    launch moves 4*2*M + B*(8*M + 8*nzc*Gx*Gy) bytes (its x-y grids are the
+    nzc planes, the z taps having been applied in the z stage), reported
+    beside the full-grid figure of 4*d*M + B*(8*M + 8*G)."""
     dom = max(("interp", "spread"), key=lambda k: kinfo[k]["ms"])
-    lp = 8 if L <= 8 else 16
-    kname = {"interp": f"k_interp_lds<{lp}, false>",
+    kname = {"interp": "k_interp_lds4<false>" if L > 8 else "k_interp_lds<8, false>",
              "spread": "k_spread_patch"}[dom]
     tr = pmc_traffic(summary, kname)
-    return dict(bound="hbm", achieved=round(kinfo[dom]["GBs"], 1), peak=HBM_PEAK_GBS,
-                unit="GB/s", frac=round(kinfo[dom]["frac"], 4),
-                traffic=None if tr is None else round(tr), kernel=kname,
-                algorithmic_bytes=kinfo[dom]["bytes"], avg_launch_ms=round(kinfo[dom]["ms"], 5))
+    out = dict(bound="hbm", achieved=round(kinfo[dom]["GBs"], 1), peak=HBM_PEAK_GBS,
+               unit="GB/s", frac=round(kinfo[dom]["frac"], 4),
+               traffic=None if tr is None else round(tr), kernel=kname,
+               algorithmic_bytes=kinfo[dom]["bytes"], avg_launch_ms=round(kinfo[dom]["ms"], 5))
+    if plane_bytes:
+        gbs = plane_bytes / (kinfo[dom]["ms"] * 1e-3) / 1e9
+        out["plane_bytes"] = plane_bytes
+        out["achieved_plane_bytes"] = round(gbs, 1)
+        out["frac_plane_bytes"] = round(gbs / HBM_PEAK_GBS, 4)
+    return out
 
 
 def nrmse(a, b):
@@ -713,6 +744,8 @@ def sense2d_section(P, S, args, stream):
             out["nrmse_vs_fp64_oracle_beta_1e-3"] = nrmse(xl, xl64)
             xle, _ = exact_cg(P, prob, yd, R_low, niter, stream)
             out["nrmse_vs_fp64_exact_nudft_cg_beta_1e-3"] = nrmse(xl, xle)
+        out["nrmse_vs_fp64_exact_nudft_cg_by_segments"] = ts_model_error(P, prob, yd, xe, stream)
+        log(f"[bench] 2-D time-segmentation model error {out['nrmse_vs_fp64_exact_nudft_cg_by_segments']}")
     if not args.no_cpu:
         cb = cpu_baseline(prob, y)
         out["cpu_baseline"] = cb
@@ -720,6 +753,113 @@ def sense2d_section(P, S, args, stream):
         log(f"[bench] 2-D cpu baseline {cb['value']:.3f} it/s on {cb['cores']} threads")
     extras = dict(prob=prob, yd=yd, Sg=Sg)
     return out, extras
+
+
+def plain2d_section(P, S, args, stream):
+    """The plain 2-D workload: 64x64 ellipse phantom, spiral 8x512, 1 coil
+    (unit sensitivity), no field map, KB width 6 oversampled points
+    (kernelWidth 3.0, beta by the reference's formula), 10 PCG iterations,
+    QuadPenalty beta 1e-3; k-space = FP64 exact NUDFT (on the GPU) + 40 dB
+    noise.  GPU rates (whole-solve graph; plain: host-driven loop), and the
+    reconstruction against the reference's own FP32 CG image (the oracle's
+    FP32 restatement), the FP64 same-operator PCG and the FP64 exact-NUDFT
+    PCG.  At beta 1e-3 the 10 iterations amplify FP32 rounding (the FP32
+    restatement itself lands ~1e-2 from FP64); the same solve at beta 1e4 is
+    reported beside it."""
+    from oracle import oracle as O
+    prob = S.make_problem(64, 8, 512, 1, 1, field=False, niter=10, beta=1e-3, kernel_width=3.0,
+                          seed=0)
+    N, M = prob.N, prob.M
+    ones = np.ones(N, np.complex64)
+    D = P.Gdft(M, N, prob.kx, prob.ky, prob.kz, prob.ix, prob.iy, prob.iz, np.zeros(N), prob.t,
+               precision=1)
+    y0 = (D * prob.x_true.astype(np.complex64)).astype(np.complex128)
+    y = S.add_noise(y0, 40.0, seed=11).astype(np.complex64)
+    yd = P.DeviceArray.from_host(y)
+    G = P.Gnufft(M, 2.0, 64, 64, 1, prob.kx, prob.ky, prob.kz, kernel_width=3.0)
+    Sg = P.SENSE(G, ones, M, N, 1)
+    out = {"workload": "plain 2D Gnufft 64x64, spiral 8x512, 1 coil, no field map, KB width 6 "
+                       "(kernelWidth 3.0), 10 PCG iterations, QuadPenalty 1e-3"}
+    res = {}
+    for beta in (1e-3, 1e4):
+        R = P.QuadPenalty(64, 64, 1, beta)
+        value, dt, lat, x, info = timed_solves(P, yd, Sg, R, prob.niter, 50, 3, stream)
+        xg = x.to_host()
+        e = {"iters_per_s": round(value, 1),
+             "recon_latency_ms": {"p50": round(1e3 * float(np.percentile(lat, 50)), 3),
+                                  "p99": round(1e3 * float(np.percentile(lat, 99)), 3)}}
+        if not args.no_oracle:
+            O.set_threads(host_cpu()[0])
+            ref = {}
+            for prec in "fd":
+                Go = O.Gnufft(M, 2.0, 64, 64, 1, prob.kx, prob.ky, prob.kz, 3.0, prec=prec)
+                So = O.SENSE(Go, ones, M, N, 1)
+                Ro = O.QuadPenalty(64, 64, 1, beta, prec=prec)
+                yy = y if prec == "f" else y.astype(np.complex128)
+                ref[prec] = O.reconSolve(yy, So, Ro, 64, 64, 1, prob.niter)[0]
+            xe, _ = exact_cg_op(P, D, ones, prob, yd, R, prob.niter, stream)
+            e.update(nrmse_vs_fp32_reference_cg=nrmse(xg, ref["f"]),
+                     nrmse_vs_fp64_oracle=nrmse(xg, ref["d"]),
+                     nrmse_fp32_reference_vs_fp64_oracle=nrmse(ref["f"], ref["d"]),
+                     nrmse_vs_fp64_exact_nudft_cg=nrmse(xg, xe),
+                     nrmse_vs_phantom=nrmse(xg, prob.x_true))
+        res[beta] = (e, R)
+    e, R = res[1e-3]
+    out.update(e)
+    out["beta_1e4"] = res[1e4][0]
+    G2 = P.Gnufft(M, 2.0, 64, 64, 1, prob.kx, prob.ky, prob.kz, kernel_width=3.0)
+    S2 = P.SENSE(G2, ones, M, N, 1)
+    for op in (G2, S2):
+        op.set_mode(P.MODE_PLAIN)
+    P.reconSolve(yd, S2, R, niter=2, mode=P.CG_HOST, stream=stream)
+    t0 = time.perf_counter()
+    for _ in range(3):
+        _, infop = P.reconSolve(yd, S2, R, niter=prob.niter, mode=P.CG_HOST, stream=stream)
+    stream.synchronize()
+    out["plain_iters_per_s"] = round(3 * infop["iterations"] / (time.perf_counter() - t0), 1)
+    out["speedup_vs_plain"] = round(out["iters_per_s"] / out["plain_iters_per_s"], 2)
+    if not args.no_cpu:
+        threads, model = host_cpu()
+        O.set_threads(threads)
+        Go = O.Gnufft(M, 2.0, 64, 64, 1, prob.kx, prob.ky, prob.kz, 3.0)
+        So = O.SENSE(Go, ones, M, N, 1)
+        Ro = O.QuadPenalty(64, 64, 1, 1e-3)
+        O.reconSolve(y, So, Ro, 64, 64, 1, 1)
+        t0 = time.perf_counter()
+        n = 0
+        while time.perf_counter() - t0 < 3.0:
+            O.reconSolve(y, So, Ro, 64, 64, 1, prob.niter)
+            n += prob.niter
+        cpu = n / (time.perf_counter() - t0)
+        O.set_threads(1)
+        out["cpu_baseline"] = dict(value=round(cpu, 2), unit="CG iterations/s", cores=threads,
+                                   kind="port", cpu_model=model, nproc=os.cpu_count(),
+                                   sample=f"{n} PCG iterations in solves of {prob.niter} "
+                                          f"(FP32, {threads} OpenMP threads)")
+        out["speedup_vs_cpu"] = round(out["iters_per_s"] / cpu, 1)
+    return out
+
+
+def exact_cg_op(P, D, sens, prob, yd, R, niter, stream):
+    """the PCG of `exact_cg` over an existing FP64 Gdft D through SENSE"""
+    Sd = P.SENSE(D, sens, prob.M, prob.N, len(sens) // prob.N)
+    x, info = P.reconSolve(yd, Sd, R, niter=niter, mode=P.CG_GRAPH, stream=stream)
+    return x.to_host(), info
+
+
+def ts_model_error(P, prob, yd, xe, stream, Ls=(8, 16, 32)):
+    """The 2-D SENSE reconstruction (beta as the workload) with L Hanning
+    segments against xe, the FP64 exact-NUDFT PCG of the same data: the
+    time-segmentation model's own error, which falls as L grows."""
+    R = P.QuadPenalty(prob.Nx, prob.Ny, prob.Nz, prob.beta)
+    out = {}
+    for L in Ls:
+        G = P.Gnufft(prob.M, 2.0, prob.Nx, prob.Ny, prob.Nz, prob.kx, prob.ky, prob.kz)
+        T = P.TimeSegmentation(G, prob.fm, prob.t, prob.M, prob.N, L)
+        Sg = P.SENSE(T, prob.sens, prob.M, prob.N, prob.nc)
+        x, _ = P.reconSolve(yd, Sg, R, niter=prob.niter, mode=P.CG_GRAPH, stream=stream)
+        out[f"L={L}"] = nrmse(x.to_host(), xe)
+    return out
 
 
 def main():
@@ -737,6 +877,7 @@ def main():
     ap.add_argument("--sense2d-steps", type=int, default=20)
     ap.add_argument("--beta", type=float, default=None, help="2-D QuadPenalty weight override")
     ap.add_argument("--no-sense2d", action="store_true", help="skip the 2-D SENSE workload")
+    ap.add_argument("--no-plain2d", action="store_true", help="skip the plain 2-D workload")
     ap.add_argument("--no-gdft", action="store_true", help="skip the Gdft roofline")
     ap.add_argument("--no-direct", action="store_true",
                     help="skip the density-compensation / gridded-coil-image timings")
@@ -803,7 +944,9 @@ def main():
     M, B = prob.M, prob.nc * prob.L
     Gsz = 8 * prob.N  # 256 x 256 x 64
     kinfo = kernel_times(P, Sg, x_gpu, stream, 3, M, Gsz, B, prob.nc, prob.L, 3)
-    roof = roofline(kinfo, prob.L, "sos3d_pmc_traffic.md")
+    nzc = P.zclasses(G)
+    plane_b = (4 * 2 * M + B * (8 * M + 8 * nzc * 4 * prob.Nx * prob.Ny)) if nzc else None
+    roof = roofline(kinfo, prob.L, "sos3d_pmc_traffic.md", plane_b)
 
     # plain path: one NUFFT at a time, host-driven PCG (the reference's loop)
     G2, T2, Sg2, R2 = build_ops_sos3d(P, prob, P.MODE_PLAIN)
@@ -816,6 +959,10 @@ def main():
     plain = infop["iterations"] / (time.perf_counter() - t0)
     log(f"[bench] plain: {plain:.3f} it/s")
     del G2, T2, Sg2, R2
+    # the batched solve stopped after the same iteration count, for the
+    # plain-vs-batched distance
+    xb_short, _ = P.reconSolve(yd, Sg, R, niter=args.plain_iters_3d, mode=P.CG_GRAPH,
+                               stream=stream)
 
     out = {
         "metric": "SENSE-TS CG iters/s batched vs plain at NRMSE<=1e-3; p50/p99 recon latency",
@@ -845,7 +992,8 @@ def main():
         "recon_latency_ms": {"p50": round(1e3 * float(np.percentile(lat, 50)), 2),
                              "p99": round(1e3 * float(np.percentile(lat, 99)), 2)},
         "exit_reason": info["exit_reason"],
-        "nrmse_plain_vs_batched": nrmse(xp.to_host(), x_gpu),
+        "nrmse_plain_vs_batched": nrmse(xp.to_host(), xb_short.to_host()),
+        "plain_vs_batched_iterations": args.plain_iters_3d,
         "nrmse_vs_phantom": nrmse(x_gpu / 1e-3, prob.x_true),
         "kernels_us": {k: round(v["ms"] * 1e3, 1) for k, v in kinfo.items()},
         "roofline": roof,
@@ -878,6 +1026,9 @@ def main():
         if not args.no_direct:
             out["direct_recon"] = direct_recon_rates(P, prob2, stream)
             log(f"[bench] direct recon {out['direct_recon']}")
+    if not args.no_plain2d:
+        out["plain2d"] = plain2d_section(P, S, args, stream)
+        log(f"[bench] plain 2-D {out['plain2d']}")
     if not args.no_batch:
         out["uneven_batch"] = batch_rates(P, S, stream)
         log(f"[bench] uneven batch {out['uneven_batch']}")

@@ -553,6 +553,69 @@ __global__ void __launch_bounds__(1024) k_zstage_fwd(
     }
 }
 
+// The forward z stage for Nz <= NZ (8, 16 or 32): one block of four
+// wavefronts per (64 image columns, coil c), coil fastest in the grid, so
+// that the blocks of one column chunk run together and each XCD reads the
+// chunk's segment phases Wo once.  The block forms S_c x(z) for its columns
+// once, in LDS; wavefront g then takes the segments l = g, g + 4, ...:
+//   v_z = dxy Wo_l(z) S_c x(z)  (z < Nz, in registers),
+//   H[(k B + c L + l) Nxy + col] = sum_z zm[k][z] v_z  for every class k,
+// with zm read at uniform addresses (scalar loads) and one 512-byte store
+// per (k, l).  The products and the z sum run in the order of k_zstage_fwd.
+template <int NZ>
+__global__ void __launch_bounds__(256) k_zstage_fwd_reg(
+    const float2* __restrict__ img, long long img_cstride, const float2* __restrict__ S,
+    const float2* __restrict__ Wo, const float* __restrict__ dxy, int Nxy, int Nz, int L, int B,
+    int nc, int nzc, const float2* __restrict__ zm, float2* __restrict__ H)
+{
+    constexpr int NW = 4;
+    __shared__ float2 su[NZ * 64];
+    const int lane = threadIdx.x & 63;
+    const int g = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int c = blockIdx.x % nc;
+    const int col = (blockIdx.x / nc) * 64 + lane;
+    const bool ok = col < Nxy;
+    const int colc = ok ? col : Nxy - 1;
+    const long long N = (long long)Nxy * Nz;
+    for (int z = g; z < Nz; z += NW) {
+        const long long o = (long long)z * Nxy + colc;
+        float2 a = img[c * img_cstride + o];
+        if (S) a = pgf::cmul(S[(long long)c * N + o], a);
+        su[z * 64 + lane] = a;
+    }
+    __syncthreads();
+    const float d = dxy[colc];
+    for (int l = g; l < L; l += NW) {
+        float2 v[NZ];
+        if (Wo) {
+            float2 w[NZ];
+#pragma unroll
+            for (int z = 0; z < NZ; ++z)
+                w[z] = Wo[(long long)l * N + (long long)min(z, Nz - 1) * Nxy + colc];
+#pragma unroll
+            for (int z = 0; z < NZ; ++z) v[z] = pgf::cmul(w[z], su[min(z, Nz - 1) * 64 + lane]);
+        } else {
+#pragma unroll
+            for (int z = 0; z < NZ; ++z) v[z] = su[min(z, Nz - 1) * 64 + lane];
+        }
+#pragma unroll
+        for (int z = 0; z < NZ; ++z)
+            v[z] = z < Nz ? make_float2(v[z].x * d, v[z].y * d) : make_float2(0.f, 0.f);
+        float2* hp = H + ((long long)c * L + l) * Nxy + colc;
+        for (int k = 0; k < nzc; ++k) {
+            const float2* zr = zm + (long long)k * Nz;
+            float ar = 0.f, ai = 0.f;
+#pragma unroll
+            for (int z = 0; z < NZ; ++z) {
+                const float2 w = zr[min(z, Nz - 1)];
+                ar += w.x * v[z].x - w.y * v[z].y;
+                ai += w.x * v[z].y + w.y * v[z].x;
+            }
+            if (ok) hp[(long long)k * B * Nxy] = make_float2(ar, ai);
+        }
+    }
+}
+
 // Adjoint z stage: for coil c and image column col, over this wavefront's 8
 // image planes z,
 //   P_c(z, col) = conj(S_c) dxy(col) sum_l WoH_l(z, col) sum_k conj(zm[k][z]) Hp[(k B + c L + l) Nxy + col]
@@ -865,10 +928,24 @@ void launch_fwd_prep_fft(const pg::Gnufft& G, const float2* img, long long img_c
                    "no z-class planes for this batch");
         const int Bp = B * G.nzc;
         const int nxy = g.Nx * g.Ny;
-        const int ng = (G.nzc + kZsKpt - 1) / kZsKpt;
-        const size_t lds = sizeof(float2) * (size_t)G.nzc * g.Nz;
-        hipLaunchKernelGGL(k_zstage_fwd, dim3((nxy + 63) / 64, nc), dim3(64 * ng), lds, s, img,
-                           img_cstride, w.S, w.Wo, G.dapxy, nxy, g.Nz, L, B, G.nzc, G.zmat, tbuf2);
+        const unsigned nch = (unsigned)((nxy + 63) / 64);
+#define PG_ZSF(NZ)                                                                              \
+    hipLaunchKernelGGL(k_zstage_fwd_reg<NZ>, dim3(nch * nc), dim3(256), 0, s, img, img_cstride, \
+                       w.S, w.Wo, G.dapxy, nxy, g.Nz, L, B, nc, G.nzc, G.zmat, tbuf2)
+        if (g.Nz <= 8) {
+            PG_ZSF(8);
+        } else if (g.Nz <= 16) {
+            PG_ZSF(16);
+        } else if (g.Nz <= 32) {
+            PG_ZSF(32);
+        } else {
+            const int ng = (G.nzc + kZsKpt - 1) / kZsKpt;
+            const size_t lds = sizeof(float2) * (size_t)G.nzc * g.Nz;
+            hipLaunchKernelGGL(k_zstage_fwd, dim3(nch, nc), dim3(64 * ng), lds, s, img,
+                               img_cstride, w.S, w.Wo, G.dapxy, nxy, g.Nz, L, B, G.nzc, G.zmat,
+                               tbuf2);
+        }
+#undef PG_ZSF
 #define PG_CALL_COLP(NN) fwd_col_planes_n<NN>(G, tbuf2, Bp, tbuf, s)
         PG_REG_SWITCH(g.Gy, PG_CALL_COLP)
 #undef PG_CALL_COLP

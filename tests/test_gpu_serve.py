@@ -136,3 +136,43 @@ def test_recon_batch_one_launch_matches_single_solves():
     A0, y0, R0 = items[0]
     with pytest.raises(P.PgError):
         P.recon_solve_batch([(A0, y0, R0), (A0, y0, R0)], niter=2, stream=st)
+
+
+# The serving workload's shapes: 2-D 128x128 (spiral 16x2048, L=8) with 8,
+# 16 and 32 coils, 3-D 64x64x16 (16 partitions x 4 x 1024, 8 coils, L=8) and
+# 3-D 128x128x32 (32 partitions x 8 x 2048, 8 coils, L=16), each to a
+# relative gradient tolerance of 1e-4 or 30 iterations; every image is the
+# same alone and in the mix, bit for bit (the 2-D kernels and the 3-D z-class
+# planes sum in a fixed order).
+def workload_request(N, Nz, nc, L, nsh, ns, seed, tag):
+    pr = S.make_problem(N, nsh, ns, nc, L, Nz=Nz, peak_hz=120.0, seed=seed)
+    M = len(pr.kx)
+    G = P.Gnufft(M, 2.0, N, N, Nz, pr.kx, pr.ky, pr.kz)
+    A = P.SENSE(P.TimeSegmentation(G, pr.fm, pr.t, M, pr.N, L), pr.sens, M, pr.N, nc)
+    y = S.add_noise(A * pr.x_true, 40.0, seed=seed + 2)
+    y = (y / np.abs(y).max()).astype(np.complex64)
+    return SV.ReconRequest(pr.kx, pr.ky, pr.kz, pr.t, N, N, Nz, nc, L, pr.fm, pr.sens, y,
+                           beta=1e4, niter=30, tol=1e-4, tag=tag)
+
+
+def test_serve_workload_shapes_alone_and_in_the_mix():
+    print("[serve: 2-D 128^2 x {8, 32} coils, 3-D 64x64x16, 3-D 128x128x32 L=16]", flush=True)
+    reqs = [workload_request(128, 1, 8, 8, 16, 2048, 1, "2d8"),
+            workload_request(128, 1, 32, 8, 16, 2048, 2, "2d32"),
+            workload_request(64, 16, 8, 8, 4, 1024, 3, "3d64"),
+            workload_request(128, 32, 8, 16, 8, 2048, 4, "3d128"),
+            workload_request(128, 1, 16, 8, 16, 2048, 5, "2d16")]
+    st = P.Stream()
+    alone = {r.tag: SV.run_request(r, st) for r in reqs}
+    with SV.ReconServer(slots=4) as srv:
+        futs = [srv.submit(r) for r in reqs]
+        res = {f.result().tag: f.result() for f in futs}
+    assert srv.max_running >= 2
+    for r in reqs:
+        img, info = alone[r.tag]
+        out = res[r.tag]
+        print(f"  {r.tag}: {info['iterations']} iterations ({info['exit_reason']}), "
+              f"latency in the mix {1e3 * out.latency:.1f} ms", flush=True)
+        assert out.iterations == info["iterations"] and out.exit_reason == info["exit_reason"]
+        np.testing.assert_array_equal(out.image, img)
+
