@@ -538,29 +538,41 @@ def autotune_rate(P, prob, yd, stream, niter, steps=10):
                 spread="matrix cores" if mfma == 1 else "vector units")
 
 
-def batch_rates(P, S, stream, K=24, niter=20, reps=3, seed=21):
-    """Uneven reconstructions in one graph launch (pg_recon_solve_batch): K
-    slices with matrix sizes from {96, 128, 192}^2, 8/16/32 coils, L in
-    {4, 8, 12} and spirals of 16 interleaves x 512..4096 samples (8k-64k),
-    each solved for niter PCG iterations, against the same solves run one
-    after another.  k-space from the device forward operator plus 40 dB
-    noise, scaled to unit peak."""
+def batch_rates(P, S, stream, K=96, niter=20, reps=2, seed=21, nrmse_slices=True):
+    """Uneven reconstructions in one call (pg_recon_solve_batch: each slice's
+    captured solve graph, launched concurrently on the library's batch
+    streams): K slices with matrix sizes from {96, 128, 192}^2, 8/16/32
+    coils, L in {4, 8, 12}, field maps of 50-250 Hz peak and spirals of 16
+    interleaves x 512..4096 samples (8k-64k), each solved for niter PCG
+    iterations (QuadPenalty 1e4, data scaled to unit peak), against the same
+    solves run one after another, and against the plain form (one NUFFT at a
+    time, host-driven PCG, one slice after another).  k-space is each slice's
+    FP64 exact NUDFT (on the GPU) plus 40 dB noise; every slice's image is
+    compared with the FP64 exact-NUDFT PCG of its data (Gdft in FP64 through
+    SENSE as the operator)."""
     rng = np.random.default_rng(seed)
-    items, keep, grids = [], [], 0
+    items, keep, specs, grids = [], [], [], 0
     for k in range(K):
         N = int(rng.choice([96, 128, 192]))
         nc = int(rng.choice([8, 16, 32]))
         L = int(rng.choice([4, 8, 12]))
         ns = int(rng.choice([512, 1024, 2048, 4096]))
-        pr = S.make_problem(N, 16, ns, nc, L, peak_hz=float(rng.uniform(50, 250)), seed=seed + k)
+        pk = float(rng.uniform(50, 250))
+        pr = S.make_problem(N, 16, ns, nc, L, peak_hz=pk, seed=seed + k)
         M = len(pr.kx)
+        D = P.Gdft(M, pr.N, pr.kx, pr.ky, pr.kz, pr.ix, pr.iy, pr.iz, pr.fm, pr.t, precision=1)
+        Sd = P.SENSE(D, pr.sens, M, pr.N, nc)
+        xd = P.DeviceArray.from_host(pr.x_true.astype(np.complex64))
+        y = Sd.forward(xd, stream=stream).to_host(stream).astype(np.complex128)
+        y = S.add_noise(y, 40.0, seed=seed + 100 + k)
+        y = P.DeviceArray.from_host((y / np.abs(y).max()).astype(np.complex64))
         G = P.Gnufft(M, 2.0, N, N, 1, pr.kx, pr.ky, pr.kz)
         T = P.TimeSegmentation(G, pr.fm, pr.t, M, pr.N, L)
         A = P.SENSE(T, pr.sens, M, pr.N, nc)
-        y = S.add_noise(A * pr.x_true, 40.0, seed=seed + 100 + k)
-        y = P.DeviceArray.from_host((y / np.abs(y).max()).astype(np.complex64))
-        items.append((A, y, P.QuadPenalty(N, N, 1, 1e4)))
-        keep.append((G, T))
+        R = P.QuadPenalty(N, N, 1, 1e4)
+        items.append((A, y, R))
+        keep.append((G, T, D, Sd, pr))
+        specs.append(dict(N=N, nc=nc, L=L, M=M, peak_hz=round(pk, 1)))
         grids += nc * L
     P.recon_solve_batch(items, niter, stream=stream)  # capture
     for A, y, R in items:
@@ -571,18 +583,59 @@ def batch_rates(P, S, stream, K=24, niter=20, reps=3, seed=21):
         out = P.recon_solve_batch(items, niter, stream=stream)
     stream.synchronize()
     t_batch = (time.perf_counter() - t0) / reps
+    # the same call with the solves spread over 1, 2 and 8 of the library's
+    # streams (4 is the default)
+    by_streams = {4: round(1e3 * t_batch, 2)}
+    for nst in (1, 2, 8):
+        P.set_tuning("batch_streams", nst)
+        try:
+            P.recon_solve_batch(items, niter, stream=stream)
+            stream.synchronize()
+            t0 = time.perf_counter()
+            P.recon_solve_batch(items, niter, stream=stream)
+            stream.synchronize()
+            by_streams[nst] = round(1e3 * (time.perf_counter() - t0), 2)
+        finally:
+            P.set_tuning("batch_streams", 4)
     t0 = time.perf_counter()
     for _ in range(reps):
         for A, y, R in items:
             P.reconSolve(y, A, R, niter=niter, stream=stream)
     stream.synchronize()
     t_seq = (time.perf_counter() - t0) / reps
+    # the plain form, one slice after another
+    t_plain = 0.0
+    for (A, y, R), (G, T, D, Sd, pr) in zip(items, keep):
+        for op in (G, T, A):
+            op.set_mode(P.MODE_PLAIN)
+        P.reconSolve(y, A, R, niter=1, mode=P.CG_HOST, stream=stream)
+        stream.synchronize()
+        t0 = time.perf_counter()
+        P.reconSolve(y, A, R, niter=niter, mode=P.CG_HOST, stream=stream)
+        stream.synchronize()
+        t_plain += time.perf_counter() - t0
+        for op in (G, T, A):
+            op.set_mode(P.MODE_BATCHED)
     its = sum(info["iterations"] for _, info in out)
-    return dict(workload=f"{K} slices, 96-192^2, 8-32 coils, L 4-12, 8k-64k samples, "
-                         f"{niter} PCG iterations each ({grids} grids in all)",
-                ms_one_launch=round(1e3 * t_batch, 2), ms_one_by_one=round(1e3 * t_seq, 2),
-                cg_iters_per_s=round(its / t_batch, 1),
-                speedup=round(t_seq / t_batch, 3))
+    res = dict(workload=f"{K} slices, 96-192^2, 8-32 coils, L 4-12, 8k-64k samples, 50-250 Hz, "
+                        f"{niter} PCG iterations each ({grids} grids in all)",
+               ms_one_call=round(1e3 * t_batch, 2), ms_one_by_one=round(1e3 * t_seq, 2),
+               ms_plain_one_by_one=round(1e3 * t_plain, 1),
+               ms_one_call_by_streams=by_streams,
+               cg_iters_per_s=round(its / t_batch, 1),
+               speedup_vs_one_by_one=round(t_seq / t_batch, 3),
+               speedup_vs_plain=round(t_plain / t_batch, 1))
+    if nrmse_slices:
+        per = []
+        for (x, info), (A, y, R), (G, T, D, Sd, pr), sp in zip(out, items, keep, specs):
+            xe, _ = P.reconSolve(y, Sd, R, niter=niter, mode=P.CG_GRAPH, stream=stream)
+            per.append(dict(sp, nrmse_vs_fp64_exact_nudft_cg=round(nrmse(x.to_host(),
+                                                                         xe.to_host()), 5)))
+        e = np.array([p["nrmse_vs_fp64_exact_nudft_cg"] for p in per])
+        res["nrmse_vs_fp64_exact_nudft_cg"] = dict(median=round(float(np.median(e)), 5),
+                                                   max=round(float(e.max()), 5))
+        res["per_slice"] = per
+    return res
 
 
 def pmc_traffic(summary, kernel):

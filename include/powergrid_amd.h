@@ -346,6 +346,11 @@ int pg_recon_solve_batch(size_t K, const pg_op* A, const pg_penalty* R, const vo
  * batch has more than 8 segments; 2: always; 0: never (all segments at once,
  * 512-thread blocks, items of <= 128 samples).  Same sums, FP32 order aside.
  * "batch_streams" n (0..8, default 4): streams of pg_recon_solve_batch.
+ * "spread_rows" 1 (default): the 2-D (and z-class plane) spread of a
+ * time-segmented batch forms each grid's value from a compact sample row
+ * (the nc coil values and L segment weights) instead of reading a row of
+ * nc*L values; 0: value rows.  Same products, same order.
+ * "trace" 1: tracing ranges (pg_trace_ranges); 0 (default): none.
  * Settings take effect for launches (and solve graphs) made after the call;
  * "zclass" for plans created after it. */
 int pg_set_tuning(const char* key, int value);
@@ -369,6 +374,14 @@ int pg_profile_reset(void);
 /* name: kernel family, e.g. "interp", "spread", "fwd_prep_fft", "adj_fft_post",
  * "gdft_fwd"; returns the launch count and total milliseconds */
 int pg_profile_query(const char* name, int* count, double* total_ms);
+/* tracing ranges, the reference's RANGE markers (Tracer.hpp:19-34, NVTX
+ * there; roctx here, recorded by rocprofv3 --marker-trace): with
+ * pg_set_tuning("trace", 1), operator applications (named as the
+ * reference's operators, e.g. "SENSE::operator*"), their launch stages,
+ * the PCG solve and its host-driven iterations, operator set-up and penalty
+ * calls push ranges.  *available: whether libroctx64 could be opened;
+ * *pushed: ranges pushed since the process started. */
+int pg_trace_ranges(int* available, unsigned long long* pushed);
 
 #ifdef __cplusplus
 }

@@ -394,8 +394,25 @@ def set_tuning(key, value):
     units; "autotune" 1 lets each plan time its interpolation and spread
     variants when first prepared for a grid count and keep the faster;
     "zclass" 1 (default) builds 3-D plans of stacked trajectories for z-class
-    planes, 0 for the full 3-D grid (plans created afterwards)."""
+    planes, 0 for the full 3-D grid (plans created afterwards); "spread_rows"
+    1 (default) spreads time-segmented batches from compact sample rows (coil
+    values and segment weights), 0 from rows of nc*L values; "trace" 1
+    pushes roctx tracing ranges (trace())."""
     check(lib().pg_set_tuning(key.encode(), int(value)))
+
+
+def trace(on=True):
+    """Tracing ranges (roctx, rocprofv3 --marker-trace) around operator
+    applications, their launch stages, solves and set-up: the reference's
+    RANGE markers (Tracer.hpp)."""
+    set_tuning("trace", 1 if on else 0)
+
+
+def trace_ranges():
+    """(libroctx64 available, ranges pushed so far)"""
+    a, n = C.c_int(), C.c_ulonglong()
+    check(lib().pg_trace_ranges(C.byref(a), C.byref(n)))
+    return bool(a.value), int(n.value)
 
 
 def zclasses(op):
@@ -583,7 +600,8 @@ class profile:
 
 __all__ = ["DeviceArray", "Stream", "Gnufft", "Gdft", "GdftR2", "TimeSegmentation", "SENSE",
            "PcSenseTimeSeg", "PcSenseRank", "PcSENSE", "QuadPenalty", "TVPenalty", "calc3DDensityCompensation",
-           "grid_coil_images", "calcSumOfSquaresImage", "set_tuning", "op_tuning",
+           "grid_coil_images", "calcSumOfSquaresImage", "set_tuning", "op_tuning", "trace",
+           "trace_ranges", "zclasses",
            "solve_pwls_pcg", "recon_solve_batch",
            "reconSolve", "profile", "PgError", "MODE_BATCHED", "MODE_PLAIN", "CG_GRAPH",
            "CG_HOST"]

@@ -63,6 +63,7 @@ _SIGS = {
     "pg_op_forward_host": (i32, [vp, vp, vp]),
     "pg_op_adjoint_host": (i32, [vp, vp, vp]),
     "pg_dft_apply_static": (i32, [vp, C.c_int, vp, vp, vp]),
+    "pg_trace_ranges": (i32, [C.POINTER(C.c_int), C.POINTER(C.c_ulonglong)]),
     "pg_gnufft_grid_size": (i32, [vp, C.POINTER(sz), C.POINTER(sz), C.POINTER(sz)]),
     "pg_gnufft_forward_spatial": (i32, [vp, vp, vp, vp]),
     "pg_gnufft_adjoint_spatial": (i32, [vp, vp, vp, vp]),

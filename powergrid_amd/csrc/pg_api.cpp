@@ -1,6 +1,7 @@
 // pg_api.cpp -- C ABI (include/powergrid_amd.h): operator handles with the
 // reference's constructor arguments, forward/adjoint dispatch (batched or
 // one NUFFT at a time), penalties and the device-resident PWLS-PCG.
+#include <dlfcn.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -64,11 +65,60 @@ static hipStream_t setup_stream()
     return st;
 }
 
+// Tracing ranges, the reference's RANGE markers (Tracer.hpp:19-34, e.g.
+// SENSE.cpp:54,78, TimeSegmentation.cpp:256,301, gridding.cpp:37-595): with
+// pg_set_tuning("trace", 1) operator applications, their launch stages,
+// solves and set-up steps push roctx ranges, which rocprofv3 --marker-trace
+// records.  libroctx64 is opened at run time on first use; without it
+// nothing is pushed.  Ranges are host-side: they bracket the enqueueing of
+// work, which is what the reference's NVTX ranges mark as well.
+static std::atomic<int> g_trace{0};
+static std::atomic<unsigned long long> g_trace_pushed{0};
+struct RoctxApi {
+    int (*push)(const char*) = nullptr;
+    int (*pop)() = nullptr;
+};
+static const RoctxApi& roctx_api()
+{
+    static const RoctxApi api = [] {
+        RoctxApi a;
+        for (const char* name : {"libroctx64.so.4", "libroctx64.so", "/opt/rocm/lib/libroctx64.so.4"}) {
+            void* h = dlopen(name, RTLD_NOW | RTLD_LOCAL);
+            if (!h) continue;
+            a.push = reinterpret_cast<int (*)(const char*)>(dlsym(h, "roctxRangePushA"));
+            a.pop = reinterpret_cast<int (*)()>(dlsym(h, "roctxRangePop"));
+            if (a.push && a.pop) break;
+            a = RoctxApi{};
+        }
+        return a;
+    }();
+    return api;
+}
+struct TraceRange {
+    bool on = false;
+    explicit TraceRange(const char* name)
+    {
+        if (!g_trace.load(std::memory_order_relaxed)) return;
+        const RoctxApi& a = roctx_api();
+        if (!a.push) return;
+        a.push(name);
+        g_trace_pushed.fetch_add(1, std::memory_order_relaxed);
+        on = true;
+    }
+    ~TraceRange()
+    {
+        if (on) roctx_api().pop();
+    }
+    TraceRange(const TraceRange&) = delete;
+    TraceRange& operator=(const TraceRange&) = delete;
+};
+
 struct Prof {
     bool on = false;
     hipStream_t s;
     ProfRec r;
-    Prof(const char* name, hipStream_t st) : s(st)
+    TraceRange tr;
+    Prof(const char* name, hipStream_t st) : s(st), tr(name)
     {
         if (!g_prof) return;
         hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
@@ -1321,8 +1371,23 @@ static void op_prepare(Op* op)
     }
 }
 
+// the reference's class name of an operator, for its tracing ranges
+static const char* op_range_name(const Op* op, bool fwd)
+{
+    switch (op->kind) {
+    case K_NUFFT: return fwd ? "Gnufft::operator*" : "Gnufft::operator/";
+    case K_DFT: return fwd ? "Gdft::operator*" : "Gdft::operator/";
+    case K_TS: return fwd ? "TimeSegmentation::operator*" : "TimeSegmentation::operator/";
+    case K_SENSE: return fwd ? "SENSE::operator*" : "SENSE::operator/";
+    case K_PCSENSE: return fwd ? "pcSenseTimeSeg::operator*" : "pcSenseTimeSeg::operator/";
+    case K_PCSENSE_DFT: return fwd ? "pcSENSE::operator*" : "pcSENSE::operator/";
+    }
+    return fwd ? "operator*" : "operator/";
+}
+
 static void op_forward(Op* op, const float2* in, float2* out, hipStream_t s)
 {
+    TraceRange tr(op_range_name(op, true));
     Fused f;
     if (fusable(op, f)) {
         fused_forward(f, in, out, s);
@@ -1432,6 +1497,7 @@ static void op_forward(Op* op, const float2* in, float2* out, hipStream_t s)
 
 static void op_adjoint(Op* op, const float2* in, float2* out, hipStream_t s)
 {
+    TraceRange tr(op_range_name(op, false));
     Fused f;
     if (fusable(op, f)) {
         if (f.nc == 1) {
@@ -1638,6 +1704,7 @@ Op::~Op() { delete scache; }
 static void cg_iteration(Op* A, Penalty* R, const float2* y, const float* W, CgWork& w,
                          hipStream_t s, const Fused* f)
 {
+    TraceRange tr("solve_pwls_pcg iteration");
     const long long N = (long long)A->n_in, M = (long long)A->n_out;
     const pgv::PenSpec* ps = R ? &R->spec : nullptr;
     // ngrad = A / (W (y - Ax)) - R.Gradient(x)
@@ -2428,7 +2495,8 @@ int pg_gnufft_create(pg_op* out, size_t M, float gridos, size_t Nx, size_t Ny, s
     (void)ix;
     (void)iy;
     (void)iz;
-    PG_TRY PG_REQUIRE(out, PG_ERR_ARG, "null out");
+    PG_TRY TraceRange tr_("Gnufft::Gnufft");
+    PG_REQUIRE(out, PG_ERR_ARG, "null out");
     *out = nullptr;
     std::unique_ptr<NufftOp> op(new NufftOp());
     op->kind = K_NUFFT;
@@ -2448,7 +2516,8 @@ int pg_gdft_create(pg_op* out, size_t M, size_t N, const float* kx, const float*
                    const float* kz, const float* ix, const float* iy, const float* iz,
                    const float* fm, const float* t, int precision)
 {
-    PG_TRY PG_REQUIRE(out && kx && ky && ix && iy, PG_ERR_ARG, "null argument");
+    PG_TRY TraceRange tr_("Gdft::Gdft");
+    PG_REQUIRE(out && kx && ky && ix && iy, PG_ERR_ARG, "null argument");
     PG_REQUIRE(M > 0 && N > 0 && M < (1u << 31) && N < (1u << 31), PG_ERR_ARG, "bad sizes");
     PG_REQUIRE(precision == 0 || precision == 1, PG_ERR_ARG, "precision must be 0 or 1");
     *out = nullptr;
@@ -2511,7 +2580,8 @@ int pg_gdftr2_create(pg_op* out, size_t M, size_t N, const float* kx, const floa
 int pg_ts_create(pg_op* out, pg_op Gh, const float* fieldmap, const float* tvec, size_t M,
                  size_t N, size_t L, size_t interptype, size_t shots)
 {
-    PG_TRY PG_REQUIRE(out && fieldmap && tvec, PG_ERR_ARG, "null argument");
+    PG_TRY TraceRange tr_("TimeSegmentation::TimeSegmentation");
+    PG_REQUIRE(out && fieldmap && tvec, PG_ERR_ARG, "null argument");
     *out = nullptr;
     Op* G = as_op(Gh);
     *out = reinterpret_cast<pg_op>(ts_build(G, fieldmap, tvec, M, N, L, interptype, shots));
@@ -2722,7 +2792,8 @@ int pg_pcsense_gdft_create(pg_op* out, const float* kx, const float* ky, const f
 
 int pg_sense_create(pg_op* out, pg_op Gh, const float* sens, size_t M, size_t N, size_t nc)
 {
-    PG_TRY PG_REQUIRE(out && sens && nc > 0, PG_ERR_ARG, "null argument");
+    PG_TRY TraceRange tr_("SENSE::SENSE");
+    PG_REQUIRE(out && sens && nc > 0, PG_ERR_ARG, "null argument");
     *out = nullptr;
     Op* G = as_op(Gh);
     PG_REQUIRE(G->n_in == N && G->n_out == M, PG_ERR_ARG, "SENSE sizes do not match G");
@@ -3120,7 +3191,8 @@ int pg_penalty_destroy(pg_penalty h)
 
 int pg_penalty_gradient(pg_penalty h, const void* x, void* g, void* stream)
 {
-    PG_TRY Penalty* p = as_pen(h);
+    PG_TRY TraceRange tr_("Robject::Gradient");
+    Penalty* p = as_pen(h);
     pgv::pen_gradient(p->spec, (const float2*)x, (float2*)g, (hipStream_t)stream);
     PG_HIP(hipGetLastError());
     PG_CATCH
@@ -3128,7 +3200,8 @@ int pg_penalty_gradient(pg_penalty h, const void* x, void* g, void* stream)
 
 int pg_penalty_denom(pg_penalty h, const void* ddir, const void* x, double* out2, void* stream)
 {
-    PG_TRY Penalty* p = as_pen(h);
+    PG_TRY TraceRange tr_("Robject::Denom");
+    Penalty* p = as_pen(h);
     PG_REQUIRE(out2, PG_ERR_ARG, "null out2");
     hipStream_t s = (hipStream_t)stream;
     pgv::pen_denom_pdot(p->spec, (const float2*)ddir, (const float2*)x, nullptr, false, p->part,
@@ -3143,7 +3216,8 @@ int pg_penalty_denom(pg_penalty h, const void* ddir, const void* x, double* out2
 
 int pg_penalty_value(pg_penalty h, const void* x, double* out, void* stream)
 {
-    PG_TRY Penalty* p = as_pen(h);
+    PG_TRY TraceRange tr_("Robject::Penalty");
+    Penalty* p = as_pen(h);
     PG_REQUIRE(out, PG_ERR_ARG, "null out");
     hipStream_t s = (hipStream_t)stream;
     pgv::pen_value(p->spec, (const float2*)x, p->part, p->out, s);
@@ -3158,7 +3232,8 @@ int pg_solve_pwls_pcg_tol(pg_op Ah, pg_penalty Rh, const void* x0, const float* 
                           size_t niter, double tol, int mode, void* x_out, double* resid,
                           pg_cg_info* info, void* stream)
 {
-    PG_TRY Op* A = as_op(Ah);
+    PG_TRY TraceRange tr_("solve_pwls_pcg");
+    Op* A = as_op(Ah);
     Penalty* R = Rh ? as_pen(Rh) : nullptr;
     PG_REQUIRE(y && x_out, PG_ERR_ARG, "null y or x_out");
     PG_REQUIRE(mode == PG_CG_GRAPH || mode == PG_CG_HOST, PG_ERR_ARG, "bad CG mode");
@@ -3193,7 +3268,8 @@ int pg_recon_solve_batch(size_t K, const pg_op* A, const pg_penalty* R, const vo
                          size_t niter, double tol, void* const* x_out, pg_cg_info* infos,
                          void* stream)
 {
-    PG_TRY PG_REQUIRE(K > 0 && A && R && y && x_out, PG_ERR_ARG, "null argument");
+    PG_TRY TraceRange tr_("reconSolve batch");
+    PG_REQUIRE(K > 0 && A && R && y && x_out, PG_ERR_ARG, "null argument");
     PG_REQUIRE(stream, PG_ERR_ARG, "a batch runs on a stream of the caller (pg_stream_create)");
     std::vector<Op*> ops(K);
     std::vector<Penalty*> pens(K);
@@ -3254,6 +3330,10 @@ int pg_set_tuning(const char* key, int value)
         pgk::g_spread_mfma.store(value ? 1 : 0);
     else if (k == "zclass")
         pgk::g_zclass.store(value ? 1 : 0);
+    else if (k == "spread_rows")
+        pgk::g_spread_rows.store(value ? 1 : 0);
+    else if (k == "trace")
+        g_trace.store(value ? 1 : 0);
     else if (k == "batch_streams")
         g_batch_streams.store(value < 0 ? 0 : (value > 8 ? 8 : value));
     else if (k == "interp_lds4")
@@ -3264,6 +3344,14 @@ int pg_set_tuning(const char* key, int value)
         throw pg::Error{PG_ERR_ARG, "unknown tuning key " + k};
     // cached solve graphs captured the previous kernel variants
     g_ws_gen.fetch_add(1);
+    PG_CATCH
+}
+
+int pg_trace_ranges(int* available, unsigned long long* pushed)
+{
+    PG_TRY PG_REQUIRE(available && pushed, PG_ERR_ARG, "null argument");
+    *available = roctx_api().push ? 1 : 0;
+    *pushed = g_trace_pushed.load();
     PG_CATCH
 }
 

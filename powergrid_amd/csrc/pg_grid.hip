@@ -516,6 +516,56 @@ void launch_values(const float2* in, long long in_cstride, const float2* resid_y
     }
 }
 
+// Compact sample rows for the spread of a time-segmented batch: row i holds
+// the nc coil values r_c(i) (the input, or W (y - Ax) rounded to FP32, as
+// spread_value forms them) as float2, then the L segment weights AA_l(i) (and,
+// for complex weights, their imaginary parts), padded to S floats (a multiple
+// of 4).  The spread forms each grid's value AA_l r_c from the row, with the
+// products of spread_value, so both spreads sum the same values; the rows
+// are S floats per sample instead of 2 nc L.  Block: 64 samples, staged in
+// LDS and written out as contiguous rows.
+__global__ void __launch_bounds__(256) k_spread_rows(
+    const float2* __restrict__ in, long long in_cstride, const float2* __restrict__ resid_y,
+    const double2* __restrict__ ax64, const float* __restrict__ Wd, int L,
+    const float* __restrict__ AA, const float* __restrict__ AAi, int M, int nc, int S,
+    float* __restrict__ rows)
+{
+    extern __shared__ float buf[];  // 64 x S
+    const int i0 = blockIdx.x * 64;
+    const int tid = threadIdx.x;
+    const int n = min(64, M - i0);
+    for (int e = tid; e < nc * 64; e += 256) {
+        const int c = e >> 6, k = e & 63;
+        float2 v = make_float2(0.f, 0.f);
+        if (k < n) v = spread_value(in, in_cstride, resid_y, ax64, Wd, nullptr, nullptr, 1, M, c,
+                                    i0 + k);
+        buf[k * S + 2 * c] = v.x;
+        buf[k * S + 2 * c + 1] = v.y;
+    }
+    const int na = AAi ? 2 * L : L;
+    for (int e = tid; e < na * 64; e += 256) {
+        const int q = e >> 6, k = e & 63;
+        const int i = min(i0 + k, M - 1);
+        buf[k * S + 2 * nc + q] = q < L ? AA[(long long)q * M + i] : AAi[(long long)(q - L) * M + i];
+    }
+    for (int e = tid; e < (S - 2 * nc - na) * 64; e += 256) {
+        const int q = e >> 6, k = e & 63;
+        buf[k * S + 2 * nc + na + q] = 0.f;
+    }
+    __syncthreads();
+    float* dst = rows + (long long)i0 * S;
+    for (int e = tid; e < n * S; e += 256) dst[e] = buf[e];
+}
+
+void launch_rows(const float2* in, long long in_cstride, const float2* resid_y, const double2* ax64,
+                 const float* Wd, int L, const float* AA, const float* AAi, int M, int nc, int S,
+                 float2* rows, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_spread_rows, dim3((unsigned)((M + 63) / 64)), dim3(256),
+                       sizeof(float) * 64 * S, s, in, in_cstride, resid_y, ax64, Wd, L, AA, AAi, M,
+                       nc, S, reinterpret_cast<float*>(rows));
+}
+
 // ------------------------------------------------- 2-D patch spread --------
 // A block of 8 wavefronts; lane = grid (coil x segment).  Wavefront w runs
 // the work item witems[8 * block + w] = {patch (-1: none), entry begin,
@@ -534,11 +584,16 @@ void launch_values(const float2* in, long long in_cstride, const float2* resid_y
 // k_spread_heavy_combine).  No atomics, no zero-fill, deterministic.
 constexpr int kSpreadWaves = 8;
 
+// ROWS = 1 (2: complex weights): the entries' values are formed from the
+// compact rows of k_spread_rows (stride S floats, nc coils, L segments):
+// lane b = c L + l reads r_c at row + 2c and AA_l at row + 2 nc + l, and
+// forms AA_l r_c with spread_value's products (conj(AA) r for complex AA).
+template <int ROWS>
 __global__ void __launch_bounds__(64 * kSpreadWaves) k_spread_patch(
     const int4* __restrict__ witems, const int* __restrict__ bgs, const int* __restrict__ entries,
     const float4* __restrict__ ew, int npy, const float2* __restrict__ V, int B,
     float2* __restrict__ grids, long long G, int Gy, float2* __restrict__ partial, int n_slots,
-    int npp, long long pstride)
+    int npp, long long pstride, int L, int S)
 {
     __shared__ float2 red[kSpreadWaves / 2][16][64];
     __shared__ float4 sw[kSpreadWaves][64][4];
@@ -554,6 +609,11 @@ __global__ void __launch_bounds__(64 * kSpreadWaves) k_spread_patch(
 #pragma unroll
     for (int q = 0; q < 16; ++q) acc[q] = make_float2(0.f, 0.f);
     const float2* Vb = V + bb;
+    // compact rows: this lane's coil and segment offsets within a row
+    const int lc = ROWS ? bb / L : 0, ll = ROWS ? bb - lc * L : 0;
+    const int nc = ROWS ? B / L : 0;
+    const float* Rf = reinterpret_cast<const float*>(V);
+    const int roff = 2 * lc, aoff = 2 * nc + ll;
     constexpr int U = 8;  // entries per group
     float4(*mw)[4] = sw[wv];
     for (int s0 = it.y; s0 < it.z; s0 += 64) {
@@ -574,20 +634,31 @@ __global__ void __launch_bounds__(64 * kSpreadWaves) k_spread_patch(
         // is zeroed with a bit mask where it is used (no branch around the
         // loads, so that the next group's loads stay in flight while this
         // group is summed)
-        auto load_group = [&](int k0, float2 (&v)[U]) {
+        auto load_group = [&](int k0, float2 (&v)[U], float (&a)[U], float (&ai)[U]) {
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const int i = __builtin_amdgcn_readlane(ei, min(k0 + u, n - 1));
-                v[u] = Vb[(long long)i * B];
+                if (ROWS) {
+                    const float* row = Rf + (long long)i * S;
+                    v[u] = *reinterpret_cast<const float2*>(row + roff);
+                    a[u] = row[aoff];
+                    if (ROWS == 2) ai[u] = row[aoff + L];
+                } else {
+                    v[u] = Vb[(long long)i * B];
+                }
             }
         };
         // sum group k0 (rows in v) into the patch
-        auto sum_group = [&](int k0, float2 (&v)[U]) {
+        auto sum_group = [&](int k0, float2 (&v)[U], float (&a)[U], float (&ai)[U]) {
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const int m = (k0 + u < n) ? -1 : 0;
-                const float2 vu = make_float2(__int_as_float(__float_as_int(v[u].x) & m),
-                                              __int_as_float(__float_as_int(v[u].y) & m));
+                float2 val = v[u];
+                if (ROWS == 1) val = make_float2(a[u] * val.x, a[u] * val.y);
+                if (ROWS == 2)
+                    val = make_float2(a[u] * val.x + ai[u] * val.y, a[u] * val.y - ai[u] * val.x);
+                const float2 vu = make_float2(__int_as_float(__float_as_int(val.x) & m),
+                                              __int_as_float(__float_as_int(val.y) & m));
                 const int k = min(k0 + u, 63);
                 float w[16];
 #pragma unroll
@@ -610,14 +681,15 @@ __global__ void __launch_bounds__(64 * kSpreadWaves) k_spread_patch(
         // unconditionally, masked past n, so that its loads are not sunk next
         // to their use)
         float2 va[U], vb[U];
-        load_group(0, va);
+        float aa[U], ab[U], aia[U], aib[U];
+        load_group(0, va, aa, aia);
         for (int k0 = 0; k0 < n; k0 += 2 * U) {
-            load_group(k0 + U, vb);
+            load_group(k0 + U, vb, ab, aib);
             __builtin_amdgcn_sched_barrier(0);
-            sum_group(k0, va);
-            load_group(k0 + 2 * U, va);
+            sum_group(k0, va, aa, aia);
+            load_group(k0 + 2 * U, va, aa, aia);
             __builtin_amdgcn_sched_barrier(0);
-            sum_group(k0 + U, vb);
+            sum_group(k0 + U, vb, ab, aib);
         }
         pgf::wave_lds_sync();  // the weights of the next stage replace these
     }
@@ -979,9 +1051,15 @@ void spread_dispatch(const pg::Gnufft& G, const float2* in, long long in_cstride
                            AA, AAi, B, grids, g.G());
         return;
     }
-    // per-sample values of every grid, once
-    if (!vals_ready)
-        launch_values(in, in_cstride, resid_y, ax64, Wd, L, AA, AAi, G.M, B, vals, s);
+    // per-sample values of every grid (or the compact rows they are formed
+    // from), once
+    const int S = pgk::spread_row_floats(G, nc, L, AA, AAi);
+    if (!vals_ready) {
+        if (S)
+            launch_rows(in, in_cstride, resid_y, ax64, Wd, L, AA, AAi, G.M, nc, S, vals, s);
+        else
+            launch_values(in, in_cstride, resid_y, ax64, Wd, L, AA, AAi, G.M, B, vals, s);
+    }
     if (!D3 || planes) {
         // 2-D patches (of every z-class plane); plane p of grid b at
         // (p*B + b)*Gx*Gy
@@ -995,10 +1073,18 @@ void spread_dispatch(const pg::Gnufft& G, const float2* in, long long in_cstride
             hipLaunchKernelGGL(k_spread_patch_mfma, dim3(G.n_sblocks, nch), dim3(64 * kSpreadWaves),
                                0, s, G.witems, G.bgs, G.entries, (const float4*)G.ent_w, npy, vals,
                                B, grids, gsz, g.Gy, partial, G.n_slots, G.npp, pstride);
+        else if (S && AAi)
+            hipLaunchKernelGGL(k_spread_patch<2>, dim3(G.n_sblocks, nch), dim3(64 * kSpreadWaves),
+                               0, s, G.witems, G.bgs, G.entries, (const float4*)G.ent_w, npy, vals,
+                               B, grids, gsz, g.Gy, partial, G.n_slots, G.npp, pstride, L, S);
+        else if (S)
+            hipLaunchKernelGGL(k_spread_patch<1>, dim3(G.n_sblocks, nch), dim3(64 * kSpreadWaves),
+                               0, s, G.witems, G.bgs, G.entries, (const float4*)G.ent_w, npy, vals,
+                               B, grids, gsz, g.Gy, partial, G.n_slots, G.npp, pstride, L, S);
         else
-            hipLaunchKernelGGL(k_spread_patch, dim3(G.n_sblocks, nch), dim3(64 * kSpreadWaves), 0, s,
-                               G.witems, G.bgs, G.entries, (const float4*)G.ent_w, npy, vals, B,
-                               grids, gsz, g.Gy, partial, G.n_slots, G.npp, pstride);
+            hipLaunchKernelGGL(k_spread_patch<0>, dim3(G.n_sblocks, nch), dim3(64 * kSpreadWaves),
+                               0, s, G.witems, G.bgs, G.entries, (const float4*)G.ent_w, npy, vals,
+                               B, grids, gsz, g.Gy, partial, G.n_slots, G.npp, pstride, L, 0);
         if (G.n_comb > 0)
             hipLaunchKernelGGL(k_spread_heavy_combine, dim3(G.n_comb, nch, 16), dim3(64), 0, s, G.comb,
                                npy, partial, G.n_slots, B, grids, gsz, g.Gy, G.npp, pstride);
@@ -1020,6 +1106,21 @@ namespace pgk {
 std::atomic<int> g_spread_mfma{0};
 std::atomic<int> g_zclass{1};
 std::atomic<int> g_interp4{1};
+std::atomic<int> g_spread_rows{1};
+
+int spread_row_floats(const pg::Gnufft& G, int nc, int L, const float* AA, const float* AAi)
+{
+    const int B = nc * L;
+    if (!g_spread_rows.load(std::memory_order_relaxed) || !AA || L < 2) return 0;
+    if (G.geo.dims == 3 && !zclass_active(G, B)) return 0;
+    const pg::Gnufft::Tuned* tn = G.tuned_for(B);
+    if (B % 4 == 0 && (tn ? tn->mfma != 0 : g_spread_mfma.load(std::memory_order_relaxed) != 0))
+        return 0;
+    const int S = (2 * nc + (AAi ? 2 : 1) * L + 3) & ~3;
+    // the rows live in the plan's value buffer (M x B float2) and a block of
+    // 64 rows in LDS
+    return (S <= 2 * B && S <= 128) ? S : 0;
+}
 
 bool zclass_active(const pg::Gnufft& G, int B)
 {
@@ -1062,8 +1163,13 @@ void launch_spread_values(const pg::Gnufft& G, const float2* in, long long in_cs
                           int L, const float* AA, const float* AAi, const float2* resid_y,
                           const double2* ax64, const float* Wd, hipStream_t s, float2* vals)
 {
-    launch_values(in, in_cstride, resid_y, ax64, Wd, L, AA, AAi, G.M, nc * L, vals ? vals : G.vals,
-                  s);
+    const int S = spread_row_floats(G, nc, L, AA, AAi);
+    if (S)
+        launch_rows(in, in_cstride, resid_y, ax64, Wd, L, AA, AAi, G.M, nc, S, vals ? vals : G.vals,
+                    s);
+    else
+        launch_values(in, in_cstride, resid_y, ax64, Wd, L, AA, AAi, G.M, nc * L,
+                      vals ? vals : G.vals, s);
 }
 
 void launch_spread(const pg::Gnufft& G, const float2* in, long long in_cstride, int nc, int L,

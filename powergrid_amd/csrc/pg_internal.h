@@ -114,8 +114,17 @@ extern std::atomic<int> g_zclass;
 // 512-thread blocks over items of <= 128 samples)
 extern std::atomic<int> g_interp4;
 
+// the 2-D (and z-class plane) spread of a time-segmented batch forms each
+// grid's value from a compact sample row (the nc coil values and the L
+// segment weights) instead of reading a row of nc*L values, when set (the
+// default)
+extern std::atomic<int> g_spread_rows;
+
 // whether a batched launch of B grids of plan G runs on its z-class planes
 bool zclass_active(const pg::Gnufft& G, int B);
+// floats per compact sample row of a spread of nc coils x L segments with
+// weights AA (and AAi), or 0 when that spread reads value rows
+int spread_row_floats(const pg::Gnufft& G, int nc, int L, const float* AA, const float* AAi);
 
 struct WeightArgs {
     // image-side weights applied before the forward NUFFT and after the

@@ -423,3 +423,68 @@ def test_invalid_geometry_is_rejected():
         P.Gnufft(len(kx), 2.0, 32, 16, 1, kx, ky, kz)  # Nx != Ny
     with pytest.raises(P.PgError):
         P.Gnufft(len(kx), 2.0, 32, 32, 1, kx, ky, kz, kernel_width=5.0)  # 10 taps
+
+
+@pytest.mark.parametrize("case", ["2d", "3d_planes", "2d_minmax"])
+def test_spread_compact_rows_match_value_rows(orc, case):
+    # The spread of a time-segmented batch forms each grid's value from a
+    # compact sample row (coil values and segment weights) by default; with
+    # pg_set_tuning("spread_rows", 0) it reads a row of nc*L values.  The
+    # products and the summation order are the same, so the adjoint and a
+    # graph PCG agree bit for bit (complex min-max weights: to FP32 rounding)
+    print(f"[spread rows vs value rows: {case}]", flush=True)
+    if case == "3d_planes":
+        p = S.make_problem(16, 4, 256, 4, 4, Nz=8, peak_hz=150.0, seed=8)
+    else:
+        p = S.make_problem(64, 8, 1024, 8, 8, peak_hz=150.0, seed=9)
+    itype = 2 if case == "2d_minmax" else 1
+    rng = np.random.default_rng(5)
+    y = (rng.standard_normal(p.M * p.nc) + 1j * rng.standard_normal(p.M * p.nc)).astype(np.complex64)
+    out = {}
+    for rows in (1, 0):
+        P.set_tuning("spread_rows", rows)
+        try:
+            G = P.Gnufft(p.M, 2.0, p.Nx, p.Ny, p.Nz, p.kx, p.ky, p.kz)
+            T = P.TimeSegmentation(G, p.fm, p.t, p.M, p.N, p.L, interptype=itype)
+            A = P.SENSE(T, p.sens, p.M, p.N, p.nc)
+            R = P.QuadPenalty(p.Nx, p.Ny, p.Nz, 1e2)
+            xs, _ = P.reconSolve(y, A, R, niter=3)
+            out[rows] = (A / y, xs)
+        finally:
+            P.set_tuning("spread_rows", 1)
+    if case == "2d_minmax":
+        # the complex products may be contracted into FMAs differently
+        assert rel(out[1][0], out[0][0]) < 1e-5
+        assert rel(out[1][1], out[0][1]) < 1e-4
+    else:
+        np.testing.assert_array_equal(out[1][0], out[0][0])
+        np.testing.assert_array_equal(out[1][1], out[0][1])
+
+
+def test_trace_ranges_around_operators_and_solves():
+    # pg_set_tuning("trace", 1): the operator applications, their stages and
+    # the solve push roctx ranges (when libroctx64 is present); off again, none
+    print("[trace ranges]", flush=True)
+    p = S.make_problem(32, 4, 256, 2, 3, peak_hz=100.0, seed=12)
+    G = P.Gnufft(p.M, 2.0, 32, 32, 1, p.kx, p.ky, p.kz)
+    A = P.SENSE(P.TimeSegmentation(G, p.fm, p.t, p.M, p.N, p.L), p.sens, p.M, p.N, p.nc)
+    x = p.x_true.astype(np.complex64)
+    y0 = A * x
+    avail, n0 = P.trace_ranges()
+    P.trace(True)
+    try:
+        y1 = A * x
+        A / y1
+        P.reconSolve(y1, A, P.QuadPenalty(32, 32, 1, 1e2), niter=2)
+    finally:
+        P.trace(False)
+    _, n1 = P.trace_ranges()
+    A * x
+    _, n2 = P.trace_ranges()
+    print(f"  roctx available {avail}: {n1 - n0} ranges", flush=True)
+    np.testing.assert_array_equal(y0, y1)
+    if avail:
+        assert n1 - n0 >= 2 * 3 + 1  # SENSE, its stages, the solve
+    else:
+        assert n1 == n0
+    assert n2 == n1

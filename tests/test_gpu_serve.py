@@ -176,3 +176,27 @@ def test_serve_workload_shapes_alone_and_in_the_mix():
         assert out.iterations == info["iterations"] and out.exit_reason == info["exit_reason"]
         np.testing.assert_array_equal(out.image, img)
 
+
+
+def test_serve_reuses_plans_with_the_same_image():
+    # repeats of one protocol (same trajectory and grid count, their own data
+    # and field maps): from the second on they run on the first one's idle
+    # plan, and each image equals the one from a plan of its own
+    print("[serve: plan reuse]", flush=True)
+    base = workload_request(64, 1, 8, 8, 8, 1024, 7, "p0")
+    reqs = [base]
+    for j in (1, 2):
+        r = workload_request(64, 1, 8, 8, 8, 1024, 7, f"p{j}")
+        rng = np.random.default_rng(j)
+        r.data = (base.data * (1 + 0.1 * j) + 0.01 * (rng.standard_normal(base.data.shape)
+                                                       + 1j * rng.standard_normal(base.data.shape))
+                  ).astype(np.complex64)
+        r.fieldmap = (base.fieldmap * (1 + 0.05 * j)).astype(np.float32)
+        reqs.append(r)
+    st = P.Stream()
+    alone = {r.tag: SV.run_request(r, st)[0] for r in reqs}
+    with SV.ReconServer(slots=1) as srv:
+        outs = [srv.submit(r).result() for r in reqs]
+        assert srv.plan_hits == 2
+    for r, o in zip(reqs, outs):
+        np.testing.assert_array_equal(o.image, alone[r.tag])
