@@ -572,7 +572,12 @@ def batch_rates(P, S, stream, K=96, niter=20, reps=2, seed=21, nrmse_slices=True
         R = P.QuadPenalty(N, N, 1, 1e4)
         items.append((A, y, R))
         keep.append((G, T, D, Sd, pr))
-        specs.append(dict(N=N, nc=nc, L=L, M=M, peak_hz=round(pk, 1)))
+        # the segment length tau = readout / L and the largest phase a
+        # segment spans, 2 pi max|fm| tau: the Hanning time segmentation is
+        # an accurate model of the field only while it stays well below pi
+        tau = float(pr.t.max() - pr.t.min()) / L
+        specs.append(dict(N=N, nc=nc, L=L, M=M, peak_hz=round(pk, 1), tau_ms=round(1e3 * tau, 2),
+                          segment_phase_rad=round(float(np.abs(pr.fm).max()) * tau, 2)))
         grids += nc * L
     P.recon_solve_batch(items, niter, stream=stream)  # capture
     for A, y, R in items:
@@ -628,12 +633,18 @@ def batch_rates(P, S, stream, K=96, niter=20, reps=2, seed=21, nrmse_slices=True
     if nrmse_slices:
         per = []
         for (x, info), (A, y, R), (G, T, D, Sd, pr), sp in zip(out, items, keep, specs):
-            xe, _ = P.reconSolve(y, Sd, R, niter=niter, mode=P.CG_GRAPH, stream=stream)
+            xe, ie = P.reconSolve(y, Sd, R, niter=niter, mode=P.CG_GRAPH, stream=stream)
             per.append(dict(sp, nrmse_vs_fp64_exact_nudft_cg=round(nrmse(x.to_host(),
-                                                                         xe.to_host()), 5)))
+                                                                         xe.to_host()), 5),
+                            exact_cg_iterations=ie["iterations"]))
         e = np.array([p["nrmse_vs_fp64_exact_nudft_cg"] for p in per])
-        res["nrmse_vs_fp64_exact_nudft_cg"] = dict(median=round(float(np.median(e)), 5),
-                                                   max=round(float(e.max()), 5))
+        ph = np.array([p["segment_phase_rad"] for p in per])
+        res["nrmse_vs_fp64_exact_nudft_cg"] = dict(
+            median=round(float(np.median(e)), 5), max=round(float(e.max()), 5),
+            median_segment_phase_below_1_rad=round(float(np.median(e[ph < 1.0])), 5)
+            if np.any(ph < 1.0) else None,
+            median_segment_phase_above_pi=round(float(np.median(e[ph > np.pi])), 5)
+            if np.any(ph > np.pi) else None)
         res["per_slice"] = per
     return res
 
@@ -710,13 +721,14 @@ def kernel_times(P, Sg, x_host, stream, nrep, M, Gsz, B, nc, L, d):
 def roofline(kinfo, L, summary, plane_bytes=None):
     """the roofline object of the dominant gridding kernel.  The interpolation
     stages all L segments of a tile (k_interp_lds) up to 8 segments and four
-    at a time (k_interp_lds4) above.  plane_bytes: on z-class planes the
+    at a time (k_interp_lds4) above; the spread of a time-segmented batch
+    reads compact sample rows (k_spread_patch<1>).  plane_bytes: on z-class planes the
     launch moves 4*2*M + B*(8*M + 8*nzc*Gx*Gy) bytes (its x-y grids are the
     nzc planes, the z taps having been applied in the z stage), reported
     beside the full-grid figure of 4*d*M + B*(8*M + 8*G)."""
     dom = max(("interp", "spread"), key=lambda k: kinfo[k]["ms"])
     kname = {"interp": "k_interp_lds4<false>" if L > 8 else "k_interp_lds<8, false>",
-             "spread": "k_spread_patch"}[dom]
+             "spread": "k_spread_patch<1>"}[dom]
     tr = pmc_traffic(summary, kname)
     out = dict(bound="hbm", achieved=round(kinfo[dom]["GBs"], 1), peak=HBM_PEAK_GBS,
                unit="GB/s", frac=round(kinfo[dom]["frac"], 4),
