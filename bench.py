@@ -958,6 +958,9 @@ def main():
                          "pairs of the batched operator; no timing, no JSON line")
     ap.add_argument("--counters-2d", action="store_true",
                     help="the counter pass on the 2-D workload instead of the 3-D one")
+    ap.add_argument("--tune", action="append", default=[], metavar="KEY=VALUE",
+                    help="a library tuning switch set before any plan is made "
+                         "(powergrid_amd.set_tuning); repeatable")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the library has no multi-GPU path "
@@ -966,6 +969,10 @@ def main():
 
     import powergrid_amd as P
     from powergrid_amd import synth as S
+    for kv in args.tune:
+        k, v = kv.split("=", 1)
+        P.set_tuning(k, int(v))
+        log(f"[bench] tuning {k}={int(v)}")
     stream = P.Stream()
 
     if args.counters:

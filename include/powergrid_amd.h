@@ -351,6 +351,9 @@ int pg_recon_solve_batch(size_t K, const pg_op* A, const pg_penalty* R, const vo
  * (the nc coil values and L segment weights) instead of reading a row of
  * nc*L values; 0: value rows.  Same products, same order.
  * "trace" 1: tracing ranges (pg_trace_ranges); 0 (default): none.
+ * "zfuse" 1 (default): on z-class planes at gridos 2 the adjoint's y IFFT
+ * and z stage run as one kernel; 0: two kernels with the class planes in
+ * between.  Same sums, same order.
  * Settings take effect for launches (and solve graphs) made after the call;
  * "zclass" for plans created after it. */
 int pg_set_tuning(const char* key, int value);

@@ -397,7 +397,8 @@ def set_tuning(key, value):
     planes, 0 for the full 3-D grid (plans created afterwards); "spread_rows"
     1 (default) spreads time-segmented batches from compact sample rows (coil
     values and segment weights), 0 from rows of nc*L values; "trace" 1
-    pushes roctx tracing ranges (trace())."""
+    pushes roctx tracing ranges (trace()); "zfuse" 1 (default) runs the
+    adjoint y IFFT and z stage of z-class planes as one kernel."""
     check(lib().pg_set_tuning(key.encode(), int(value)))
 
 

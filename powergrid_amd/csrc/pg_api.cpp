@@ -305,6 +305,7 @@ void Gnufft::release()
     dev_free(iitems);
     dev_free(iitems4);
     dev_free(irec);
+    dev_free(irec4);
     dev_free(ent_w);
     dev_free(tile_off);
     dev_free(tile_idx);
@@ -638,33 +639,81 @@ static void gnufft_setup(Gnufft& G, size_t M, float os, size_t Nx, size_t Ny, si
             for (long long t = 0; t < nbin; ++t) tcnt[t + 1] += tcnt[t];
             std::vector<int> pos(tcnt.begin(), tcnt.end() - 1), order(M);
             for (size_t i = 0; i < M; ++i) order[pos[tof[i]]++] = (int)i;
-            std::vector<float4> rec((size_t)5 * M);
+            std::vector<float4> rec((size_t)5 * M), rec4((size_t)5 * M);
             std::vector<int4> its, its4;
+            // LDS bank balance.  A wavefront of k_interp_lds reads, per tap,
+            // one 8-byte point per (sample, segment) lane at index
+            //   ((ox + x) LP + l) R + oy + y   (R = kIR, LP = 8 segment slots)
+            // of its item's staged region, and one of k_interp_lds4 at
+            //   ((ox + x) 4 + l) R + oy + y    (four segment slots).
+            // The 8-byte bank pair of an index is its value mod 32 (64 banks;
+            // mod 16 with 32), so the samples of a wavefront -- 8 consecutive
+            // records of an item for k_interp_lds, 16 for k_interp_lds4 --
+            // read distinct bank pairs at every tap when their keys
+            // (LP R ox + oy) mod 32, resp. (4 R ox + oy) mod 32, differ.  Each
+            // item's records are sorted by key and dealt round-robin to its
+            // wavefronts' groups, which spreads the keys of a group over the
+            // 32 values.  The order of the samples changes nothing else: each
+            // is interpolated on its own.
+            auto deal = [&](int k0, int n, int group, int xmul, int kmask,
+                            const std::vector<int>& idx, const std::vector<int>& oxs,
+                            const std::vector<int>& oys, std::vector<int>& outp) {
+                std::vector<int> srt(n);
+                for (int j = 0; j < n; ++j) srt[j] = j;
+                std::stable_sort(srt.begin(), srt.end(), [&](int a, int b) {
+                    return ((xmul * oxs[k0 + a] + oys[k0 + a]) & kmask) <
+                           ((xmul * oxs[k0 + b] + oys[k0 + b]) & kmask);
+                });
+                const int ng = (n + group - 1) / group;
+                std::vector<int> fill(ng, 0);
+                int g = 0;
+                for (int j = 0; j < n; ++j) {
+                    while (fill[g] >= std::min(group, n - g * group)) g = (g + 1) % ng;
+                    outp[k0 + g * group + fill[g]++] = idx[k0 + srt[j]];
+                    g = (g + 1) % ng;
+                }
+            };
+            std::vector<int> oxs(M), oys(M), o8(M), o4(M);
+            auto put = [&](std::vector<float4>& r, int k, int i, int ox, int oy) {
+                float4 h;
+                h.x = __int_as_float_host(i);
+                h.y = __int_as_float_host(ox);
+                h.z = __int_as_float_host(oy);
+                h.w = 0.0f;
+                const float* wx = &rw[(size_t)i * 8];
+                const float* wy = &rw[((size_t)M + i) * 8];
+                r[(size_t)5 * k + 0] = h;
+                r[(size_t)5 * k + 1] = make_float4(wx[0], wx[1], wx[2], wx[3]);
+                r[(size_t)5 * k + 2] = make_float4(wx[4], wx[5], wx[6], wx[7]);
+                r[(size_t)5 * k + 3] = make_float4(wy[0], wy[1], wy[2], wy[3]);
+                r[(size_t)5 * k + 4] = make_float4(wy[4], wy[5], wy[6], wy[7]);
+            };
             for (long long tb = 0; tb < nbin; ++tb) {
                 const int p = (int)(tb / nt), t = (int)(tb - (long long)p * nt);
                 const int tx = t / G.ity, ty = t % G.ity;
                 const int rx0 = tx * kIT - padl, ry0 = ty * kIT - padl;
                 for (int k = tcnt[tb]; k < tcnt[tb + 1]; ++k) {
                     const int i = order[k];
-                    const int ox = rlo[i] - rx0, oy = rlo[M + i] - ry0;
-                    PG_REQUIRE(ox >= 0 && ox < kIT && oy >= 0 && oy < kIT, PG_ERR_ARG, "interp binning");
-                    float4 h;
-                    h.x = __int_as_float_host(i);
-                    h.y = __int_as_float_host(ox);
-                    h.z = __int_as_float_host(oy);
-                    h.w = 0.0f;
-                    const float* wx = &rw[(size_t)i * 8];
-                    const float* wy = &rw[((size_t)M + i) * 8];
-                    rec[(size_t)5 * k + 0] = h;
-                    rec[(size_t)5 * k + 1] = make_float4(wx[0], wx[1], wx[2], wx[3]);
-                    rec[(size_t)5 * k + 2] = make_float4(wx[4], wx[5], wx[6], wx[7]);
-                    rec[(size_t)5 * k + 3] = make_float4(wy[0], wy[1], wy[2], wy[3]);
-                    rec[(size_t)5 * k + 4] = make_float4(wy[4], wy[5], wy[6], wy[7]);
+                    oxs[k] = rlo[i] - rx0;
+                    oys[k] = rlo[M + i] - ry0;
+                    PG_REQUIRE(oxs[k] >= 0 && oxs[k] < kIT && oys[k] >= 0 && oys[k] < kIT,
+                               PG_ERR_ARG, "interp binning");
                 }
-                for (int k = tcnt[tb]; k < tcnt[tb + 1]; k += kIChunk)
-                    its.push_back(make_int4(t, k, std::min(k + kIChunk, tcnt[tb + 1]), p));
-                for (int k = tcnt[tb]; k < tcnt[tb + 1]; k += kIChunk4)
-                    its4.push_back(make_int4(t, k, std::min(k + kIChunk4, tcnt[tb + 1]), p));
+                for (int k = tcnt[tb]; k < tcnt[tb + 1]; k += kIChunk) {
+                    const int n = std::min(kIChunk, tcnt[tb + 1] - k);
+                    deal(k, n, 8, 8 * kIR, 31, order, oxs, oys, o8);
+                    its.push_back(make_int4(t, k, k + n, p));
+                }
+                for (int k = tcnt[tb]; k < tcnt[tb + 1]; k += kIChunk4) {
+                    const int n = std::min(kIChunk4, tcnt[tb + 1] - k);
+                    deal(k, n, 16, 4 * kIR, 31, order, oxs, oys, o4);
+                    its4.push_back(make_int4(t, k, k + n, p));
+                }
+                for (int k = tcnt[tb]; k < tcnt[tb + 1]; ++k) {
+                    const int i8 = o8[k], i4 = o4[k];
+                    put(rec, k, i8, rlo[i8] - rx0, rlo[M + i8] - ry0);
+                    put(rec4, k, i4, rlo[i4] - rx0, rlo[M + i4] - ry0);
+                }
             }
             // items stay in (plane, tile raster) order: neighbouring tiles,
             // whose staged regions overlap, run at about the same time and
@@ -674,6 +723,7 @@ static void gnufft_setup(Gnufft& G, size_t M, float os, size_t Nx, size_t Ny, si
             G.n_iitems4 = (int)its4.size();
             G.iitems4 = dev_copy(its4.data(), its4.size());
             G.irec = dev_copy(rec.data(), rec.size());
+            G.irec4 = dev_copy(rec4.data(), rec4.size());
         }  // wider windows interpolate with the gather kernel
 
         // 4x4 patch work items with per-entry patch weights; patch q of plane
@@ -3334,6 +3384,9 @@ int pg_set_tuning(const char* key, int value)
         pgk::g_spread_rows.store(value ? 1 : 0);
     else if (k == "trace")
         g_trace.store(value ? 1 : 0);
+    else if (k == "zfuse")
+        pgk::g_zfuse.store(value ? 1 : 0);
+
     else if (k == "batch_streams")
         g_batch_streams.store(value < 0 ? 0 : (value > 8 ? 8 : value));
     else if (k == "interp_lds4")

@@ -616,6 +616,147 @@ __global__ void __launch_bounds__(256) k_zstage_fwd_reg(
     }
 }
 
+constexpr int kZfFftWaves = 2;
+
+// The adjoint y IFFT of the class planes fused with the adjoint z stage
+// (gridos 2, 64 <= Ny = Gy/2 <= 256).  One block of 256 threads per (image column
+// x, coil c), c fastest in the grid (the blocks of one x share its WoH in
+// L2).  For each segment l:
+//   1. the first kZfFftWaves wavefronts transform the class columns of grid
+//      b = c L + l along y (ifftshift on load, as k_col_ifft_crop) and keep
+//      the Ny cropped rows in LDS;
+//   2. thread (group g, row y) forms u_z = sum_k conj(Zm[k][z]) h_k(y) for
+//      its ZW image planes z = g ZW, ... and adds WoH_l(z) u_z to its sums.
+// Then out = conj(S_c) dxy sum_l (...), as k_zstage_adj, whose sums it
+// forms in the same order, without the class planes in HBM.
+template <int R1, int R2, int ZW>
+__global__ void __launch_bounds__(256) k_yifft_zadj(
+    const float2* __restrict__ T, const float2* __restrict__ S, const float2* __restrict__ WoH,
+    const float* __restrict__ dxy, int Nx, int Ny, int Nz, int offY, int L, int B, int nc,
+    int nzc, const float2* __restrict__ zm, const float2* __restrict__ tw,
+    float2* __restrict__ out, long long out_cstride)
+{
+    constexpr int N = R1 * R2;  // Gy
+    constexpr int LPW = 64 / R2;
+    constexpr int PITCH = R1 * (R2 + 1);
+    extern __shared__ float2 sm[];
+    float2* sh = sm;                     // nzc x Ny cropped class columns
+    float2* ex = sm + (size_t)nzc * Ny;  // kZfFftWaves * LPW line exchange rows
+    const int tid = threadIdx.x;
+    const int c = blockIdx.x % nc, x = blockIdx.x / nc;
+    const int Nxy = Nx * Ny;
+    const long long Nimg = (long long)Nxy * Nz;
+    const int lane = tid & 63, wv = tid >> 6;
+    const int t = lane % R2, jl = lane / R2;
+    const int shift = (offY + N / 2) % N;  // image row y sits at (y + shift) % N
+    float2* my = ex + (wv * LPW + jl) * PITCH;
+    // Ny >= 64: the rows of a wavefront share their group (uniform z0)
+    const int y = tid % Ny, g = __builtin_amdgcn_readfirstlane(tid / Ny);
+    const int z0 = g * ZW;
+    const long long o = (long long)x * Ny + y;
+    float2 acc[ZW];
+#pragma unroll
+    for (int q = 0; q < ZW; ++q) acc[q] = make_float2(0.f, 0.f);
+#pragma unroll 1
+    for (int l = 0; l < L; ++l) {
+        const int b = c * L + l;
+        if (wv < kZfFftWaves) {
+#pragma unroll 1
+            for (int k0 = 0; k0 < nzc; k0 += kZfFftWaves * LPW) {
+                const int k = k0 + wv * LPW + jl;
+                const bool valid = k < nzc;
+                const float2* src = T + (((long long)(valid ? k : 0) * B + b) * Nx + x) * N;
+                // a copy of t the compiler cannot see through, so that the
+                // per-element indices and twiddles are formed in the loop
+                // instead of being held across it
+                int tt = t;
+                asm volatile("" : "+v"(tt));
+                float2 v[R1];
+#pragma unroll
+                for (int m = 0; m < R1; ++m) v[m] = src[(tt + R2 * m + N / 2) & (N - 1)];
+                pgf::line_fft<R1, R2, 1, false>(
+                    v, tt, my, [](int k1, int q) { return k1 * (R2 + 1) + q; }, tw);
+                if (!valid) continue;
+#pragma unroll
+                for (int r = 0; r < R1 / R2; ++r)
+#pragma unroll
+                    for (int k2 = 0; k2 < R2; ++k2) {
+                        const int yy = (tt + R2 * r + R1 * k2 - shift) & (N - 1);
+                        if (yy < Ny) sh[k * Ny + yy] = v[r * R2 + k2];
+                    }
+            }
+        }
+        __syncthreads();
+        float2 u[ZW];
+#pragma unroll
+        for (int q = 0; q < ZW; ++q) u[q] = make_float2(0.f, 0.f);
+#pragma unroll 1
+        for (int k = 0; k < nzc; ++k) {
+            const float2 h = sh[k * Ny + y];
+            const float2* zr = zm + (long long)k * Nz;
+#pragma unroll
+            for (int q = 0; q < ZW; ++q) {
+                const float2 w = zr[min(z0 + q, Nz - 1)];
+                u[q].x += w.x * h.x + w.y * h.y;  // conj(w) h
+                u[q].y += w.x * h.y - w.y * h.x;
+            }
+        }
+        if (WoH) {
+            const float2* wl = WoH + (long long)l * Nimg + o;
+#pragma unroll
+            for (int q = 0; q < ZW; ++q) {
+                const int z = min(z0 + q, Nz - 1);
+                u[q] = pgf::cmul(u[q], wl[(long long)z * Nxy]);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < ZW; ++q) {
+            acc[q].x += u[q].x;
+            acc[q].y += u[q].y;
+        }
+        __syncthreads();  // the next segment's columns replace these
+    }
+    const float d = dxy[o];
+#pragma unroll
+    for (int q = 0; q < ZW; ++q) {
+        const int z = z0 + q;
+        if (z >= Nz) break;
+        float2 v = make_float2(acc[q].x * d, acc[q].y * d);
+        if (S) {
+            float2 sc = S[(long long)c * Nimg + (long long)z * Nxy + o];
+            sc.y = -sc.y;
+            v = pgf::cmul(v, sc);
+        }
+        out[c * out_cstride + (long long)z * Nxy + o] = v;
+    }
+}
+
+// lines per wavefront and LDS exchange row of the register FFT of length n
+int zf_lpw(int n)
+{
+    switch (n) {
+    case 16: return 64 / pgf::Split<16>::R2;
+    case 32: return 64 / pgf::Split<32>::R2;
+    case 64: return 64 / pgf::Split<64>::R2;
+    case 128: return 64 / pgf::Split<128>::R2;
+    case 256: return 64 / pgf::Split<256>::R2;
+    case 512: return 64 / pgf::Split<512>::R2;
+    default: return 64 / pgf::Split<1024>::R2;
+    }
+}
+int zf_pitch(int n)
+{
+    switch (n) {
+    case 16: return pgf::Split<16>::R1 * (pgf::Split<16>::R2 + 1);
+    case 32: return pgf::Split<32>::R1 * (pgf::Split<32>::R2 + 1);
+    case 64: return pgf::Split<64>::R1 * (pgf::Split<64>::R2 + 1);
+    case 128: return pgf::Split<128>::R1 * (pgf::Split<128>::R2 + 1);
+    case 256: return pgf::Split<256>::R1 * (pgf::Split<256>::R2 + 1);
+    case 512: return pgf::Split<512>::R1 * (pgf::Split<512>::R2 + 1);
+    default: return pgf::Split<1024>::R1 * (pgf::Split<1024>::R2 + 1);
+    }
+}
+
 // Adjoint z stage: for coil c and image column col, over this wavefront's 8
 // image planes z,
 //   P_c(z, col) = conj(S_c) dxy(col) sum_l WoH_l(z, col) sum_k conj(zm[k][z]) Hp[(k B + c L + l) Nxy + col]
@@ -995,6 +1136,34 @@ void launch_adj_fft_post(const pg::Gnufft& G, float2* grids, float2* tbuf, float
         line_fft_any<+1>(grids, tbuf, (long long)g.Gy, (long long)g.Gx * g.Gy,
                          (long long)g.Nx * g.Gy, g.Gx, g.Gx / 2, g.Nx, (g.offX + g.Gx / 2) % g.Gx,
                          (unsigned)Bp, G.px, s);
+        const int N = g.Gy;
+        const bool half = 2 * g.Ny == N && (g.offY + N / 2) % N == 3 * N / 4;
+        const size_t fused_lds =
+            sizeof(float2) * ((size_t)G.nzc * g.Ny + (size_t)kZfFftWaves * zf_lpw(N) * zf_pitch(N));
+        const int zgroups = 256 / std::max(g.Ny, 1);
+        if (g_zfuse.load(std::memory_order_relaxed) && half && g.Ny >= 64 && g.Ny <= 256 &&
+            g.Nz <= 32 * zgroups && fused_lds <= 64 * 1024) {
+            // y IFFT, crop and the z stage in one kernel
+            const int zw = (g.Nz + zgroups - 1) / zgroups;
+            const unsigned nb = (unsigned)((long long)nc * g.Nx);
+#define PG_YZA(NN, ZW)                                                                          \
+    hipLaunchKernelGGL((k_yifft_zadj<pgf::Split<NN>::R1, pgf::Split<NN>::R2, ZW>), dim3(nb),      \
+                       dim3(256), fused_lds, s, tbuf, w.S, w.WoH, G.dapxy, g.Nx, g.Ny, g.Nz,        \
+                       g.offY, L, B, nc, G.nzc, G.zmat, (const float2*)G.py.tw, out, out_cstride)
+#define PG_YZA_N(NN)                                                                            \
+    do {                                                                                         \
+        if (zw <= 8)                                                                             \
+            PG_YZA(NN, 8);                                                                       \
+        else if (zw <= 16)                                                                       \
+            PG_YZA(NN, 16);                                                                      \
+        else                                                                                     \
+            PG_YZA(NN, 32);                                                                      \
+    } while (0)
+            PG_REG_SWITCH(g.Gy, PG_YZA_N)
+#undef PG_YZA_N
+#undef PG_YZA
+            return;
+        }
 #define PG_CALL_CROP(NN) col_ifft_crop_n<NN>(G, tbuf, Bp, tbuf2, s)
         PG_REG_SWITCH(g.Gy, PG_CALL_CROP)
 #undef PG_CALL_CROP

@@ -69,6 +69,12 @@ struct Gnufft {
     int4* iitems4 = nullptr;  // the same tiles cut into items of <= kIChunk4
     int n_iitems4 = 0;
     float4* irec = nullptr;
+    // the same records with each item's samples reordered for the
+    // four-segment kernel: within a tile the records of irec are ordered so
+    // that every wavefront's samples of k_interp_lds fall on different LDS
+    // banks, and those of irec4 for k_interp_lds4 (whose wavefronts take 16
+    // samples instead of 8, with another tile layout)
+    float4* irec4 = nullptr;
     int itx = 0, ity = 0, ipadl = 0;
     // 3-D trajectories whose samples take few distinct z windows (a stack of
     // 2-D trajectories, kz on a Cartesian raster): class k's z taps, with the

@@ -304,8 +304,9 @@ __global__ void __launch_bounds__(64 * kInterpWaves) k_interp_lds(
 // s, segment slot l4) with s = tid / 4, so a block's 256 tasks cover its
 // item's samples once per chunk of four segments.  Per chunk the 23 x 23
 // region of the four segment grids (17 KB, [x][l][y]) is loaded into
-// registers (every load of the chunk in flight together), stored to LDS
-// after the previous chunk's taps are done, and each task forms
+// registers, stored to LDS after the previous chunk's taps are done (the
+// loads of the next chunk are issued before this chunk's taps), and each
+// task forms
 //   acc_l = sum_x wx[x] sum_y wy[y] g_l(ox + x, oy + y)
 // and adds AA_l acc_l to its running sum; the four slots of a sample are
 // added with two DPP steps at the end.  The small LDS footprint (26 KB)
@@ -313,6 +314,10 @@ __global__ void __launch_bounds__(64 * kInterpWaves) k_interp_lds(
 // taps.
 constexpr int kI4Threads = 256;
 constexpr int kI4Chunk = 64;
+// the last word of a raw (stride 0) buffer resource on gfx950: 32-bit data
+// format, no swizzle; loads at offsets at or past the resource's size return
+// zero
+constexpr int kBufferWord3 = 0x00020000;
 
 template <bool CPX>
 __global__ void __launch_bounds__(kI4Threads) k_interp_lds4(
@@ -329,7 +334,9 @@ __global__ void __launch_bounds__(kI4Threads) k_interp_lds4(
     constexpr int NP = (NPT + NTH - 1) / NTH;  // staging loads per thread
     static_assert(CH * LC == NTH, "one task per thread");
     __shared__ float2 tile[NPT];
-    __shared__ float4 srec[CH * 5];
+    constexpr int NR = (CH * 5 + NTH - 1) / NTH;   // record loads per thread
+    constexpr int NA = (CH * 16 + NTH - 1) / NTH;  // weight loads per thread
+    __shared__ float4 srec[NR * NTH];  // (stored whole: no store waits behind a branch)
     __shared__ float saa[CH * 16];
     __shared__ float saai[CPX ? CH * 16 : 1];
     const int tid = threadIdx.x;
@@ -340,40 +347,87 @@ __global__ void __launch_bounds__(kI4Threads) k_interp_lds4(
     const int rx0 = tx * pg::kIT - padl, ry0 = ty * pg::kIT - padl;
     // item plane it.w (z-class planes; 0 for a 2-D plan)
     const float2* gc = grids + (long long)it.w * pstride + (long long)c * L * G;
-    // records and the AA weights of every segment
-    for (int e = tid; e < n * 5; e += NTH) srec[e] = rec[(long long)it.y * 5 + e];
-    for (int e = tid; e < n * L; e += NTH) {
-        const int sl = e / L, l = e - (e / L) * L;
-        const int i = __float_as_int(rec[((long long)it.y + sl) * 5].x);
-        saa[sl * 16 + l] = AA[(long long)l * M + i];
-        if (CPX) saai[sl * 16 + l] = AAi[(long long)l * M + i];
-    }
     const int s = tid >> 2, l4 = tid & 3;
     const bool live = s < n;
     int ox = 0, oy = 0, idx = 0;
     float wx[8], wy[8];
     float tr = 0.f, ti = 0.f;
-    for (int g = 0; g * LC < L; ++g) {
-        // stage chunk g: point p = (xr * LC + lc) * R + yr of the region
-        float2 tv[NP];
+    // Staging point p = (xr * LC + lc) * R + yr of the region sits at byte
+    // offset voff[k] from chunk g's first segment grid, the same for every
+    // chunk.  The loads go through a buffer resource over the chunk's
+    // segments (min(LC, L - g LC) grids): a point outside the x-y grid gets
+    // an offset of 2^31 and a segment slot past L one beyond the resource's
+    // size, and the hardware returns zero for both, so the loads need
+    // neither clamps nor selects and are all issued back to back.
+    int voff[NP];
 #pragma unroll
-        for (int k = 0; k < NP; ++k) {
-            const int p = tid + k * NTH;
-            const int row = p / R, yr = p - (p / R) * R;
-            const int xr = row / LC, lc = row - (row / LC) * LC;
-            const int l = g * LC + lc, x = rx0 + xr, y = ry0 + yr;
-            const bool ok = p < NPT && l < L && x >= 0 && x < Gx && y >= 0 && y < Gy;
-            const int lcl = min(l, L - 1), xc = min(max(x, 0), Gx - 1), yc = min(max(y, 0), Gy - 1);
-            const float2 v = gc[lcl * (int)G + xc * Gy + yc];
-            tv[k] = ok ? v : make_float2(0.f, 0.f);
-        }
-        if (g > 0) __syncthreads();  // the previous chunk's taps are done
+    for (int k = 0; k < NP; ++k) {
+        const int p = tid + k * NTH;
+        const int row = p / R, yr = p - (p / R) * R;
+        const int xr = row / LC, lc = row - (row / LC) * LC;
+        const int x = rx0 + xr, y = ry0 + yr;
+        const bool ok = p < NPT && x >= 0 && x < Gx && y >= 0 && y < Gy;
+        voff[k] = ok ? (lc * (int)G + x * Gy + y) * 8 : (int)0x80000000u;
+    }
+    float2 tv[NP];
+    auto load_chunk = [&](int g) {
+        const int nseg = min(LC, L - g * LC);
+        const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+            (void*)(gc + (long long)g * LC * G), (short)0, nseg * (int)G * 8, kBufferWord3);
+#pragma unroll
+        for (int k = 0; k < NP; ++k)
+            tv[k] = __builtin_bit_cast(float2,
+                                       __builtin_amdgcn_raw_buffer_load_b64(rs, voff[k], 0, 0));
+    };
+    // The item's records, the sample indices of its (sample, segment slot)
+    // pairs e = s * 16 + l, chunk 0 of the region and then the AA weights
+    // (by those indices) are all requested before the first LDS store; a
+    // weight of a slot past L or a sample past n reads zero (offset 2^31).
+    const __amdgpu_buffer_rsrc_t rr = __builtin_amdgcn_make_buffer_rsrc(
+        (void*)(rec + (long long)it.y * 5), (short)0, n * 5 * 16, kBufferWord3);
+    float4 rv[NR];
+#pragma unroll
+    for (int k = 0; k < NR; ++k)
+        rv[k] = __builtin_bit_cast(
+            float4, __builtin_amdgcn_raw_buffer_load_b128(rr, (tid + k * NTH) * 16, 0, 0));
+    int iv[NA];
+#pragma unroll
+    for (int k = 0; k < NA; ++k) {
+        const int e = tid + k * NTH;
+        iv[k] = __float_as_int(rec[((long long)it.y + min(e >> 4, n - 1)) * 5].x);
+    }
+    load_chunk(0);
+    const __amdgpu_buffer_rsrc_t ra =
+        __builtin_amdgcn_make_buffer_rsrc((void*)AA, (short)0, L * M * 4, kBufferWord3);
+    const __amdgpu_buffer_rsrc_t rai =
+        __builtin_amdgcn_make_buffer_rsrc((void*)(CPX ? AAi : AA), (short)0, L * M * 4,
+                                          kBufferWord3);
+    float av[NA], aiv[CPX ? NA : 1];
+#pragma unroll
+    for (int k = 0; k < NA; ++k) {
+        const int e = tid + k * NTH, l = e & 15;
+        const int o = (e >> 4) < n && l < L ? (l * M + iv[k]) * 4 : (int)0x80000000u;
+        av[k] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(ra, o, 0, 0));
+        if (CPX)
+            aiv[k] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rai, o, 0, 0));
+    }
+    auto store_chunk = [&]() {
 #pragma unroll
         for (int k = 0; k < NP; ++k) {
             const int p = tid + k * NTH;
             if (p < NPT) tile[p] = tv[k];
         }
-        __syncthreads();
+    };
+    store_chunk();
+#pragma unroll
+    for (int k = 0; k < NR; ++k) srec[tid + k * NTH] = rv[k];
+#pragma unroll
+    for (int k = 0; k < NA; ++k) {
+        saa[tid + k * NTH] = av[k];
+        if (CPX) saai[tid + k * NTH] = aiv[k];
+    }
+    for (int g = 0;; ++g) {
+        __syncthreads();  // chunk g is in LDS
         if (g == 0) {
             const float4* rp = srec + (live ? s : 0) * 5;
             const float4 h = rp[0];
@@ -386,6 +440,9 @@ __global__ void __launch_bounds__(kI4Threads) k_interp_lds4(
             wy[0] = ya.x; wy[1] = ya.y; wy[2] = ya.z; wy[3] = ya.w;
             wy[4] = yb.x; wy[5] = yb.y; wy[6] = yb.z; wy[7] = yb.w;
         }
+        // the next chunk's loads are in flight during this chunk's taps
+        const bool more = (g + 1) * LC < L;
+        if (more) load_chunk(g + 1);
         const int l = g * LC + l4;
         if (live && l < L) {
             const float2* tp = tile + ((ox * LC + l4) * R + oy);
@@ -413,6 +470,9 @@ __global__ void __launch_bounds__(kI4Threads) k_interp_lds4(
                 ti += a * ai;
             }
         }
+        if (!more) break;
+        __syncthreads();  // chunk g's taps are done
+        store_chunk();
     }
     // the four segment slots of a sample (a quad of lanes)
 #define PG_DPP_ADDQ(v, CTRL)                                                                 \
@@ -1001,10 +1061,12 @@ void interp_dispatch(const pg::Gnufft& G, const float2* grids, int nc, int L, co
         // idle (L > 8 pads to 16 slots)
         const int v4 = pgk::g_interp4.load(std::memory_order_relaxed);
         const bool four = tn ? tn->lds == 2 : (v4 == 2 || (v4 == 1 && L > 8));
-        if (G.n_iitems4 > 0 && four) {
+        // (its staging and weight offsets are 32-bit byte offsets)
+        if (G.n_iitems4 > 0 && four && gsz * 4 * 8 < (1LL << 31) &&
+            (long long)L * G.M * 4 < (1LL << 31)) {
             const dim3 grid((unsigned)G.n_iitems4 * nc), block(kI4Threads);
             auto launch = [&](auto kern) {
-                hipLaunchKernelGGL(kern, grid, block, 0, s, G.iitems4, G.irec, G.ity, G.ipadl,
+                hipLaunchKernelGGL(kern, grid, block, 0, s, G.iitems4, G.irec4, G.ity, G.ipadl,
                                    grids, gsz, g.Gx, g.Gy, nc, L, AA, AAi, G.M, out, out_cstride,
                                    pstride);
             };
@@ -1073,18 +1135,19 @@ void spread_dispatch(const pg::Gnufft& G, const float2* in, long long in_cstride
             hipLaunchKernelGGL(k_spread_patch_mfma, dim3(G.n_sblocks, nch), dim3(64 * kSpreadWaves),
                                0, s, G.witems, G.bgs, G.entries, (const float4*)G.ent_w, npy, vals,
                                B, grids, gsz, g.Gy, partial, G.n_slots, G.npp, pstride);
-        else if (S && AAi)
-            hipLaunchKernelGGL(k_spread_patch<2>, dim3(G.n_sblocks, nch), dim3(64 * kSpreadWaves),
-                               0, s, G.witems, G.bgs, G.entries, (const float4*)G.ent_w, npy, vals,
-                               B, grids, gsz, g.Gy, partial, G.n_slots, G.npp, pstride, L, S);
-        else if (S)
-            hipLaunchKernelGGL(k_spread_patch<1>, dim3(G.n_sblocks, nch), dim3(64 * kSpreadWaves),
-                               0, s, G.witems, G.bgs, G.entries, (const float4*)G.ent_w, npy, vals,
-                               B, grids, gsz, g.Gy, partial, G.n_slots, G.npp, pstride, L, S);
-        else
-            hipLaunchKernelGGL(k_spread_patch<0>, dim3(G.n_sblocks, nch), dim3(64 * kSpreadWaves),
-                               0, s, G.witems, G.bgs, G.entries, (const float4*)G.ent_w, npy, vals,
-                               B, grids, gsz, g.Gy, partial, G.n_slots, G.npp, pstride, L, 0);
+        else {
+            auto launch = [&](auto kern, int L_, int S_) {
+                hipLaunchKernelGGL(kern, dim3(G.n_sblocks, nch), dim3(64 * kSpreadWaves), 0, s,
+                                   G.witems, G.bgs, G.entries, (const float4*)G.ent_w, npy, vals,
+                                   B, grids, gsz, g.Gy, partial, G.n_slots, G.npp, pstride, L_, S_);
+            };
+            if (S && AAi)
+                launch(k_spread_patch<2>, L, S);
+            else if (S)
+                launch(k_spread_patch<1>, L, S);
+            else
+                launch(k_spread_patch<0>, L, 0);
+        }
         if (G.n_comb > 0)
             hipLaunchKernelGGL(k_spread_heavy_combine, dim3(G.n_comb, nch, 16), dim3(64), 0, s, G.comb,
                                npy, partial, G.n_slots, B, grids, gsz, g.Gy, G.npp, pstride);
@@ -1107,6 +1170,7 @@ std::atomic<int> g_spread_mfma{0};
 std::atomic<int> g_zclass{1};
 std::atomic<int> g_interp4{1};
 std::atomic<int> g_spread_rows{1};
+std::atomic<int> g_zfuse{1};
 
 int spread_row_floats(const pg::Gnufft& G, int nc, int L, const float* AA, const float* AAi)
 {

@@ -120,6 +120,10 @@ extern std::atomic<int> g_interp4;
 // default)
 extern std::atomic<int> g_spread_rows;
 
+// on z-class planes with gridos 2, the adjoint y IFFT and z stage run as one
+// kernel (the cropped class columns staged in LDS) when set (the default)
+extern std::atomic<int> g_zfuse;
+
 // whether a batched launch of B grids of plan G runs on its z-class planes
 bool zclass_active(const pg::Gnufft& G, int B);
 // floats per compact sample row of a spread of nc coils x L segments with

@@ -232,3 +232,27 @@ def test_uneven_batch_shapes_operator(orc, N, nc, L):
     yo = So * x
     assert rel(Sg * x, yo) < 5e-6
     assert rel(Sg / yo, So / yo) < 5e-6
+
+
+def test_sos3d_fused_z_stage_matches_two_kernel_path():
+    # the adjoint y IFFT fused with the adjoint z stage, against the two as
+    # separate kernels with the class planes in HBM between them
+    # (pg_set_tuning("zfuse", 0)): the same products and sums; the forward
+    # is the same path either way
+    print("[sos3d zfuse vs two kernels, 128x128x32 nc=2 L=12]", flush=True)
+    p = S.make_problem(128, 2, 2048, 2, 12, Nz=32, peak_hz=200.0, seed=1)
+    Gg = P.Gnufft(p.M, 2.0, p.Nx, p.Ny, p.Nz, p.kx, p.ky, p.kz)
+    Sg = P.SENSE(P.TimeSegmentation(Gg, p.fm, p.t, p.M, p.N, p.L), p.sens, p.M, p.N, p.nc)
+    x = p.x_true.astype(np.complex64)
+    rng = np.random.default_rng(3)
+    y = (rng.standard_normal(p.M * p.nc) + 1j * rng.standard_normal(p.M * p.nc)).astype(np.complex64)
+    out = {}
+    for fuse in (1, 0):
+        P.set_tuning("zfuse", fuse)
+        try:
+            out[fuse] = (Sg * x, Sg / y)
+        finally:
+            P.set_tuning("zfuse", 1)
+    ef, ea = rel(out[1][0], out[0][0]), rel(out[1][1], out[0][1])
+    print(f"  forward {ef:.1e}, adjoint {ea:.1e}", flush=True)
+    assert ef < 1e-6 and ea < 1e-6
