@@ -398,7 +398,9 @@ def set_tuning(key, value):
     1 (default) spreads time-segmented batches from compact sample rows (coil
     values and segment weights), 0 from rows of nc*L values; "trace" 1
     pushes roctx tracing ranges (trace()); "zfuse" 1 (default) runs the
-    adjoint y IFFT and z stage of z-class planes as one kernel."""
+    adjoint y IFFT and z stage of z-class planes as one kernel;
+    "spread_stage" 1 (default) stages the compact rows of each group of
+    patch entries in LDS, 0 reads them per entry."""
     check(lib().pg_set_tuning(key.encode(), int(value)))
 
 

@@ -644,17 +644,18 @@ static void gnufft_setup(Gnufft& G, size_t M, float os, size_t Nx, size_t Ny, si
             // LDS bank balance.  A wavefront of k_interp_lds reads, per tap,
             // one 8-byte point per (sample, segment) lane at index
             //   ((ox + x) LP + l) R + oy + y   (R = kIR, LP = 8 segment slots)
-            // of its item's staged region, and one of k_interp_lds4 at
-            //   ((ox + x) 4 + l) R + oy + y    (four segment slots).
-            // The 8-byte bank pair of an index is its value mod 32 (64 banks;
-            // mod 16 with 32), so the samples of a wavefront -- 8 consecutive
-            // records of an item for k_interp_lds, 16 for k_interp_lds4 --
-            // read distinct bank pairs at every tap when their keys
-            // (LP R ox + oy) mod 32, resp. (4 R ox + oy) mod 32, differ.  Each
-            // item's records are sorted by key and dealt round-robin to its
-            // wavefronts' groups, which spreads the keys of a group over the
-            // 32 values.  The order of the samples changes nothing else: each
-            // is interpolated on its own.
+            // of its item's staged region; the 8-byte bank pair of an index
+            // is its value mod 32 (64 banks; mod 16 with 32), so the 8
+            // samples of a wavefront read distinct bank pairs at every tap
+            // when their keys (LP R ox + oy) mod 32 differ.  One of
+            // k_interp_lds4 reads 32 contiguous bytes per sample (its four
+            // segment slots) at index ((ox + x) R + oy + y) 4 + l, so that
+            // 4 (8) consecutive samples cover the 32 (64) banks once when
+            // their keys (R ox + oy) mod 4 (mod 8) differ.  Each item's
+            // records are sorted by key and dealt round-robin to groups of
+            // 8 samples, which spreads the keys of a group over their values
+            // (ascending within a group).  The order of the samples changes
+            // nothing else: each is interpolated on its own.
             auto deal = [&](int k0, int n, int group, int xmul, int kmask,
                             const std::vector<int>& idx, const std::vector<int>& oxs,
                             const std::vector<int>& oys, std::vector<int>& outp) {
@@ -706,7 +707,7 @@ static void gnufft_setup(Gnufft& G, size_t M, float os, size_t Nx, size_t Ny, si
                 }
                 for (int k = tcnt[tb]; k < tcnt[tb + 1]; k += kIChunk4) {
                     const int n = std::min(kIChunk4, tcnt[tb + 1] - k);
-                    deal(k, n, 16, 4 * kIR, 31, order, oxs, oys, o4);
+                    deal(k, n, 8, kIR, 7, order, oxs, oys, o4);
                     its4.push_back(make_int4(t, k, k + n, p));
                 }
                 for (int k = tcnt[tb]; k < tcnt[tb + 1]; ++k) {
@@ -3386,6 +3387,8 @@ int pg_set_tuning(const char* key, int value)
         g_trace.store(value ? 1 : 0);
     else if (k == "zfuse")
         pgk::g_zfuse.store(value ? 1 : 0);
+    else if (k == "spread_stage")
+        pgk::g_spread_stage.store(value ? 1 : 0);
 
     else if (k == "batch_streams")
         g_batch_streams.store(value < 0 ? 0 : (value > 8 ? 8 : value));

@@ -303,10 +303,12 @@ __global__ void __launch_bounds__(64 * kInterpWaves) k_interp_lds(
 // of 256 threads per (work item of <= 64 samples, coil); lane task = (sample
 // s, segment slot l4) with s = tid / 4, so a block's 256 tasks cover its
 // item's samples once per chunk of four segments.  Per chunk the 23 x 23
-// region of the four segment grids (17 KB, [x][l][y]) is loaded into
-// registers, stored to LDS after the previous chunk's taps are done (the
-// loads of the next chunk are issued before this chunk's taps), and each
-// task forms
+// region of the four segment grids is loaded into registers in [x][l][y]
+// order (rows of 23 points along y: coalesced), stored to LDS after the
+// previous chunk's taps are done as [x][y][l] (17 KB: a point's four
+// segments side by side, so a sample's four lanes read 32 contiguous bytes
+// at every tap), the loads of the next chunk being issued before this
+// chunk's taps; each task forms
 //   acc_l = sum_x wx[x] sum_y wy[y] g_l(ox + x, oy + y)
 // and adds AA_l acc_l to its running sum; the four slots of a sample are
 // added with two DPP steps at the end.  The small LDS footprint (26 KB)
@@ -353,13 +355,13 @@ __global__ void __launch_bounds__(kI4Threads) k_interp_lds4(
     float wx[8], wy[8];
     float tr = 0.f, ti = 0.f;
     // Staging point p = (xr * LC + lc) * R + yr of the region sits at byte
-    // offset voff[k] from chunk g's first segment grid, the same for every
-    // chunk.  The loads go through a buffer resource over the chunk's
+    // offset voff[k] from chunk g's first segment grid and goes to LDS slot
+    // sidx[k] = (xr R + yr) LC + lc, the same for every chunk.  The loads go through a buffer resource over the chunk's
     // segments (min(LC, L - g LC) grids): a point outside the x-y grid gets
     // an offset of 2^31 and a segment slot past L one beyond the resource's
     // size, and the hardware returns zero for both, so the loads need
     // neither clamps nor selects and are all issued back to back.
-    int voff[NP];
+    int voff[NP], sidx[NP];
 #pragma unroll
     for (int k = 0; k < NP; ++k) {
         const int p = tid + k * NTH;
@@ -368,6 +370,7 @@ __global__ void __launch_bounds__(kI4Threads) k_interp_lds4(
         const int x = rx0 + xr, y = ry0 + yr;
         const bool ok = p < NPT && x >= 0 && x < Gx && y >= 0 && y < Gy;
         voff[k] = ok ? (lc * (int)G + x * Gy + y) * 8 : (int)0x80000000u;
+        sidx[k] = p < NPT ? (xr * R + yr) * LC + lc : -1;
     }
     float2 tv[NP];
     auto load_chunk = [&](int g) {
@@ -413,10 +416,8 @@ __global__ void __launch_bounds__(kI4Threads) k_interp_lds4(
     }
     auto store_chunk = [&]() {
 #pragma unroll
-        for (int k = 0; k < NP; ++k) {
-            const int p = tid + k * NTH;
-            if (p < NPT) tile[p] = tv[k];
-        }
+        for (int k = 0; k < NP; ++k)
+            if (sidx[k] >= 0) tile[sidx[k]] = tv[k];
     };
     store_chunk();
 #pragma unroll
@@ -445,15 +446,15 @@ __global__ void __launch_bounds__(kI4Threads) k_interp_lds4(
         if (more) load_chunk(g + 1);
         const int l = g * LC + l4;
         if (live && l < L) {
-            const float2* tp = tile + ((ox * LC + l4) * R + oy);
+            const float2* tp = tile + ((ox * R + oy) * LC + l4);
             float ar = 0.f, ai = 0.f;
 #pragma unroll
             for (int x = 0; x < 8; ++x) {
-                const float2* row = tp + x * LC * R;
+                const float2* row = tp + x * R * LC;
                 float rr = 0.f, ri = 0.f;
 #pragma unroll
                 for (int y = 0; y < 8; ++y) {
-                    const float2 v = row[y];
+                    const float2 v = row[y * LC];
                     rr += wy[y] * v.x;
                     ri += wy[y] * v.y;
                 }
@@ -626,6 +627,13 @@ void launch_rows(const float2* in, long long in_cstride, const float2* resid_y, 
                        nc, S, reinterpret_cast<float*>(rows));
 }
 
+// conj(a + i ai) v for the compact-row spreads, with the FMA contraction
+// written out so that every kernel forms the same bits
+__device__ __forceinline__ float2 row_conj_weight(float a, float ai, float2 v)
+{
+    return make_float2(__builtin_fmaf(a, v.x, ai * v.y), __builtin_fmaf(a, v.y, -(ai * v.x)));
+}
+
 // ------------------------------------------------- 2-D patch spread --------
 // A block of 8 wavefronts; lane = grid (coil x segment).  Wavefront w runs
 // the work item witems[8 * block + w] = {patch (-1: none), entry begin,
@@ -716,7 +724,7 @@ __global__ void __launch_bounds__(64 * kSpreadWaves) k_spread_patch(
                 float2 val = v[u];
                 if (ROWS == 1) val = make_float2(a[u] * val.x, a[u] * val.y);
                 if (ROWS == 2)
-                    val = make_float2(a[u] * val.x + ai[u] * val.y, a[u] * val.y - ai[u] * val.x);
+                    val = row_conj_weight(a[u], ai[u], val);
                 const float2 vu = make_float2(__int_as_float(__float_as_int(val.x) & m),
                                               __int_as_float(__float_as_int(val.y) & m));
                 const int k = min(k0 + u, 63);
@@ -783,6 +791,167 @@ __global__ void __launch_bounds__(64 * kSpreadWaves) k_spread_patch(
     for (int a = 0; a < pg::kPatch; ++a)
 #pragma unroll
         for (int q = 0; q < pg::kPatch; ++q) g[(long long)a * Gy + q] = acc[a * 4 + q];
+}
+
+// The same patch spread from compact rows with each stage's row slices
+// staged in LDS.  A stage takes up to kStEntries entries; the wavefront's
+// 64 grids b0 .. b0+63 use coils c_lo .. c_hi and all L segments, i.e. the
+// float runs [2 c_lo, 2 c_hi + 2) and [2 nc, 2 nc + F2) of each entry's row
+// (F2 = L, or 2L for complex weights, rounded up to even).  The lanes load
+// those F = F1 + F2 floats of every entry of the stage as float2 units, all
+// in flight together, and store them to LDS; the sums then read each
+// entry's coil value and segment weight from LDS (a broadcast for the lanes
+// of one coil) instead of from global memory, so a stage waits for memory
+// once instead of once per group of entries.  Same values, products and
+// summation order as k_spread_patch.  Requires F <= kStFloats for every
+// 64-grid chunk (checked by the launch).
+constexpr int kStEntries = 32;
+constexpr int kStFloats = 48;
+
+template <int ROWS>
+__global__ void __launch_bounds__(64 * kSpreadWaves, 4) k_spread_patch_st(
+    const int4* __restrict__ witems, const int* __restrict__ bgs, const int* __restrict__ entries,
+    const float4* __restrict__ ew, int npy, const float2* __restrict__ V, int B,
+    float2* __restrict__ grids, long long G, int Gy, float2* __restrict__ partial, int n_slots,
+    int npp, long long pstride, int L, int S)
+{
+    static_assert(ROWS == 1 || ROWS == 2, "compact rows only");
+    constexpr int NE = kStEntries, FM = kStFloats;
+    constexpr int NU = (NE * FM / 2 + 63) / 64;  // float2 row loads per lane per stage
+    // the stage rows of the 8 wavefronts, reused for the tree sum after the
+    // last stage (red: [kSpreadWaves / 2][16][64] float2)
+    constexpr int POOL = kSpreadWaves * NE * FM > kSpreadWaves / 2 * 16 * 64 * 2
+                             ? kSpreadWaves * NE * FM
+                             : kSpreadWaves / 2 * 16 * 64 * 2;
+    __shared__ __align__(16) float pool[POOL];
+    __shared__ float4 sw[kSpreadWaves][NE][4];
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int4 it = witems[blockIdx.x * kSpreadWaves + wv];
+    const int gs = bgs[blockIdx.x];
+    const int b0 = blockIdx.y * 64;
+    const int b = b0 + lane;
+    const bool active = b < B;
+    const int bb = active ? b : B - 1;
+    const int nc = B / L;
+    const int lc = bb / L, ll = bb - lc * L;
+    const int c_lo = b0 / L, c_hi = (min(b0 + 64, B) - 1) / L;
+    const int F1 = 2 * (c_hi - c_lo + 1);
+    const int F2 = ((ROWS == 2 ? 2 * L : L) + 1) & ~1;
+    const int U2 = (F1 + F2) / 2;        // float2 units per entry
+    const int F = 2 * U2;                // LDS floats per entry
+    const int qd = 64 / U2, qr = 64 - qd * U2;
+    float* rows = pool + wv * NE * FM;
+    float4(*mw)[4] = sw[wv];
+    const float* Rf = reinterpret_cast<const float*>(V);
+    const int voff = 2 * (lc - c_lo), aoff = F1 + ll;
+
+    float2 acc[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) acc[q] = make_float2(0.f, 0.f);
+    for (int s0 = it.y; s0 < it.z; s0 += NE) {
+        const int n = min(NE, it.z - s0);
+        const bool ok = lane < n;
+        const int e = ok ? s0 + lane : s0;
+        const int ei = entries[e];
+        // unit t = lane + 64 j of the stage: entry k = t / U2, unit u = t % U2
+        float2 rv[NU];
+        {
+            int k = lane / U2, u = lane - (lane / U2) * U2;
+            // formed anew per stage (kept out of registers across stages)
+            asm volatile("" : "+v"(k), "+v"(u));
+#pragma unroll
+            for (int j = 0; j < NU; ++j) {
+                const int src = __shfl(ei, min(k, n - 1), 64);
+                const int off = u < U2 - F2 / 2 ? 2 * c_lo + 2 * u : 2 * nc + 2 * (u - (U2 - F2 / 2));
+                rv[j] = *reinterpret_cast<const float2*>(Rf + (long long)src * S + off);
+                k += qd;
+                u += qr;
+                if (u >= U2) {
+                    u -= U2;
+                    ++k;
+                }
+            }
+        }
+        if (lane < NE) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const float4 q = ew[4 * e + r];
+                mw[lane][r] = ok ? q : make_float4(0.f, 0.f, 0.f, 0.f);
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < NU; ++j) {
+            const int t = lane + 64 * j;
+            if (t < n * U2) *reinterpret_cast<float2*>(rows + 2 * t) = rv[j];
+        }
+        pgf::wave_lds_sync();
+#pragma unroll 2
+        for (int k = 0; k < n; ++k) {
+            const float* row = rows + k * F;
+            float2 val = *reinterpret_cast<const float2*>(row + voff);
+            const float a = row[aoff];
+            if (ROWS == 1) val = make_float2(a * val.x, a * val.y);
+            if (ROWS == 2) {
+                const float ai = row[aoff + L];
+                val = row_conj_weight(a, ai, val);
+            }
+            float w[16];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const float4 q4 = mw[k][r];
+                w[4 * r] = q4.x;
+                w[4 * r + 1] = q4.y;
+                w[4 * r + 2] = q4.z;
+                w[4 * r + 3] = q4.w;
+            }
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                acc[q].x += w[q] * val.x;
+                acc[q].y += w[q] * val.y;
+            }
+        }
+        pgf::wave_lds_sync();  // the next stage replaces these rows and weights
+    }
+    float2(*red)[16][64] = reinterpret_cast<float2(*)[16][64]>(pool);
+    for (int r = 1; r < gs; r *= 2) {
+        __syncthreads();
+        if ((wv & (2 * r - 1)) == r)
+#pragma unroll
+            for (int q = 0; q < 16; ++q) red[wv >> 1][q][lane] = acc[q];
+        __syncthreads();
+        if ((wv & (2 * r - 1)) == 0)
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                const float2 o = red[(wv + r) >> 1][q][lane];
+                acc[q].x += o.x;
+                acc[q].y += o.y;
+            }
+    }
+    if (!active || it.x < 0 || (wv & (gs - 1)) != 0) return;
+    if (it.w >= 0) {
+        float2* pp = partial + ((long long)blockIdx.y * n_slots + it.w) * 16 * 64 + lane;
+#pragma unroll
+        for (int q = 0; q < 16; ++q) pp[q * 64] = acc[q];
+        return;
+    }
+    const int pln = it.x / npp, q0 = it.x - pln * npp;
+    const int px0 = (q0 / npy) * pg::kPatch, py0 = (q0 % npy) * pg::kPatch;
+    float2* g = grids + pln * pstride + (long long)b * G + (long long)px0 * Gy + py0;
+#pragma unroll
+    for (int a = 0; a < pg::kPatch; ++a)
+#pragma unroll
+        for (int q = 0; q < pg::kPatch; ++q) g[(long long)a * Gy + q] = acc[a * 4 + q];
+}
+
+// the largest staged floats per entry of a launch's 64-grid chunks
+int spread_st_floats(int B, int L, bool cpx)
+{
+    int fm = 0;
+    for (int b0 = 0; b0 < B; b0 += 64) {
+        const int f1 = 2 * ((std::min(b0 + 64, B) - 1) / L - b0 / L + 1);
+        fm = std::max(fm, f1 + (((cpx ? 2 * L : L) + 1) & ~1));
+    }
+    return fm;
 }
 
 // The same work list on the matrix cores (B a multiple of 4).  For a chunk of
@@ -1141,7 +1310,13 @@ void spread_dispatch(const pg::Gnufft& G, const float2* in, long long in_cstride
                                    G.witems, G.bgs, G.entries, (const float4*)G.ent_w, npy, vals,
                                    B, grids, gsz, g.Gy, partial, G.n_slots, G.npp, pstride, L_, S_);
             };
-            if (S && AAi)
+            const bool st = S && pgk::g_spread_stage.load(std::memory_order_relaxed) != 0 &&
+                            spread_st_floats(B, L, AAi != nullptr) <= kStFloats;
+            if (st && AAi)
+                launch(k_spread_patch_st<2>, L, S);
+            else if (st)
+                launch(k_spread_patch_st<1>, L, S);
+            else if (S && AAi)
                 launch(k_spread_patch<2>, L, S);
             else if (S)
                 launch(k_spread_patch<1>, L, S);
@@ -1171,6 +1346,7 @@ std::atomic<int> g_zclass{1};
 std::atomic<int> g_interp4{1};
 std::atomic<int> g_spread_rows{1};
 std::atomic<int> g_zfuse{1};
+std::atomic<int> g_spread_stage{1};
 
 int spread_row_floats(const pg::Gnufft& G, int nc, int L, const float* AA, const float* AAi)
 {

@@ -124,6 +124,10 @@ extern std::atomic<int> g_spread_rows;
 // kernel (the cropped class columns staged in LDS) when set (the default)
 extern std::atomic<int> g_zfuse;
 
+// the compact-row patch spread stages each stage's row slices in LDS
+// (k_spread_patch_st) when set (default), else reads them per entry
+extern std::atomic<int> g_spread_stage;
+
 // whether a batched launch of B grids of plan G runs on its z-class planes
 bool zclass_active(const pg::Gnufft& G, int B);
 // floats per compact sample row of a spread of nc coils x L segments with

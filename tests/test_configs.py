@@ -256,3 +256,28 @@ def test_sos3d_fused_z_stage_matches_two_kernel_path():
     ef, ea = rel(out[1][0], out[0][0]), rel(out[1][1], out[0][1])
     print(f"  forward {ef:.1e}, adjoint {ea:.1e}", flush=True)
     assert ef < 1e-6 and ea < 1e-6
+
+
+@pytest.mark.parametrize("N,Nz,nc,L,itype", [(128, 32, 2, 12, 1), (128, 1, 8, 8, 1),
+                                             (64, 1, 4, 6, 2)])
+def test_staged_row_spread_matches_patch_spread(N, Nz, nc, L, itype):
+    # the patch spread with each stage's row slices staged in LDS
+    # (pg_set_tuning("spread_stage", 1)) against the one that reads them per
+    # entry from global memory: the same values, products and summation
+    # order, so the same bits (real weights, complex min-max weights; 2-D and
+    # z-class planes)
+    print(f"[staged spread N={N} Nz={Nz} nc={nc} L={L} type={itype}]", flush=True)
+    p = S.make_problem(N, 2 if Nz > 1 else 8, 2048, nc, L, Nz=Nz, peak_hz=200.0, seed=2)
+    Gg = P.Gnufft(p.M, 2.0, p.Nx, p.Ny, p.Nz, p.kx, p.ky, p.kz)
+    T = P.TimeSegmentation(Gg, p.fm, p.t, p.M, p.N, p.L, interptype=itype)
+    Sg = P.SENSE(T, p.sens, p.M, p.N, p.nc)
+    rng = np.random.default_rng(5)
+    y = (rng.standard_normal(p.M * p.nc) + 1j * rng.standard_normal(p.M * p.nc)).astype(np.complex64)
+    out = {}
+    for st in (1, 0):
+        P.set_tuning("spread_stage", st)
+        try:
+            out[st] = Sg / y
+        finally:
+            P.set_tuning("spread_stage", 1)
+    np.testing.assert_array_equal(out[1], out[0])
