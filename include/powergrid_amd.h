@@ -343,7 +343,7 @@ int pg_recon_solve_batch(size_t K, const pg_op* A, const pg_penalty* R, const vo
  * The two differ in FP32 summation order only.
  * "interp_lds4" 1 (default): the LDS-staged interpolation stages four
  * segments at a time (256-thread blocks, items of <= 64 samples) when a
- * batch has more than 8 segments; 2: always; 0: never (all segments at once,
+ * batch has 4 segments or more; 2: always; 0: never (all segments at once,
  * 512-thread blocks, items of <= 128 samples).  Same sums, FP32 order aside.
  * "batch_streams" n (0..8, default 4): streams of pg_recon_solve_batch.
  * "spread_rows" 1 (default): the 2-D (and z-class plane) spread of a

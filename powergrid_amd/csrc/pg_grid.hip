@@ -303,12 +303,13 @@ __global__ void __launch_bounds__(64 * kInterpWaves) k_interp_lds(
 // of 256 threads per (work item of <= 64 samples, coil); lane task = (sample
 // s, segment slot l4) with s = tid / 4, so a block's 256 tasks cover its
 // item's samples once per chunk of four segments.  Per chunk the 23 x 23
-// region of the four segment grids is loaded into registers in [x][l][y]
-// order (rows of 23 points along y: coalesced), stored to LDS after the
-// previous chunk's taps are done as [x][y][l] (17 KB: a point's four
-// segments side by side, so a sample's four lanes read 32 contiguous bytes
-// at every tap), the loads of the next chunk being issued before this
-// chunk's taps; each task forms
+// region of the four segment grids is staged in LDS as [x][y][l] (17 KB: a
+// point's four segments side by side, so a sample's four lanes read 32
+// contiguous bytes at every tap): a thread loads its points (rows of 23
+// along y across the threads: coalesced) from each of the four grids and
+// stores each point as two 16-byte writes.  The stores follow the previous
+// chunk's taps, and the loads of the next chunk are issued before this
+// chunk's taps.  Each task forms
 //   acc_l = sum_x wx[x] sum_y wy[y] g_l(ox + x, oy + y)
 // and adds AA_l acc_l to its running sum; the four slots of a sample are
 // added with two DPP steps at the end.  The small LDS footprint (26 KB)
@@ -332,10 +333,10 @@ __global__ void __launch_bounds__(kI4Threads) k_interp_lds4(
     constexpr int LC = 4;
     constexpr int CH = kI4Chunk;
     constexpr int NTH = kI4Threads;
-    constexpr int NPT = R * LC * R;            // grid points of one chunk
-    constexpr int NP = (NPT + NTH - 1) / NTH;  // staging loads per thread
+    constexpr int NQ = R * R;                  // points of the region
+    constexpr int NP = (NQ + NTH - 1) / NTH;   // points per thread
     static_assert(CH * LC == NTH, "one task per thread");
-    __shared__ float2 tile[NPT];
+    __shared__ __align__(16) float2 tile[NQ * LC];
     constexpr int NR = (CH * 5 + NTH - 1) / NTH;   // record loads per thread
     constexpr int NA = (CH * 16 + NTH - 1) / NTH;  // weight loads per thread
     __shared__ float4 srec[NR * NTH];  // (stored whole: no store waits behind a branch)
@@ -354,33 +355,35 @@ __global__ void __launch_bounds__(kI4Threads) k_interp_lds4(
     int ox = 0, oy = 0, idx = 0;
     float wx[8], wy[8];
     float tr = 0.f, ti = 0.f;
-    // Staging point p = (xr * LC + lc) * R + yr of the region sits at byte
-    // offset voff[k] from chunk g's first segment grid and goes to LDS slot
-    // sidx[k] = (xr R + yr) LC + lc, the same for every chunk.  The loads go through a buffer resource over the chunk's
-    // segments (min(LC, L - g LC) grids): a point outside the x-y grid gets
-    // an offset of 2^31 and a segment slot past L one beyond the resource's
-    // size, and the hardware returns zero for both, so the loads need
-    // neither clamps nor selects and are all issued back to back.
-    int voff[NP], sidx[NP];
+    // Staging: point q = tid + k NTH of the region (xr = q / R, yr = q % R)
+    // sits at byte offset voff[k] in each segment grid, the same for every
+    // chunk.  The loads go through one buffer resource per segment slot of
+    // the chunk, sized zero for a slot past L: a point outside the x-y grid
+    // gets an offset of 2^31, and the hardware returns zero for it and for
+    // every load of an empty slot, so the loads need neither clamps nor
+    // selects and are all issued back to back.
+    int voff[NP];
 #pragma unroll
     for (int k = 0; k < NP; ++k) {
-        const int p = tid + k * NTH;
-        const int row = p / R, yr = p - (p / R) * R;
-        const int xr = row / LC, lc = row - (row / LC) * LC;
+        const int q = tid + k * NTH;
+        const int xr = q / R, yr = q - (q / R) * R;
         const int x = rx0 + xr, y = ry0 + yr;
-        const bool ok = p < NPT && x >= 0 && x < Gx && y >= 0 && y < Gy;
-        voff[k] = ok ? (lc * (int)G + x * Gy + y) * 8 : (int)0x80000000u;
-        sidx[k] = p < NPT ? (xr * R + yr) * LC + lc : -1;
+        const bool ok = q < NQ && x >= 0 && x < Gx && y >= 0 && y < Gy;
+        voff[k] = ok ? (x * Gy + y) * 8 : (int)0x80000000u;
     }
-    float2 tv[NP];
+    float2 tv[NP][LC];
     auto load_chunk = [&](int g) {
-        const int nseg = min(LC, L - g * LC);
-        const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
-            (void*)(gc + (long long)g * LC * G), (short)0, nseg * (int)G * 8, kBufferWord3);
 #pragma unroll
-        for (int k = 0; k < NP; ++k)
-            tv[k] = __builtin_bit_cast(float2,
-                                       __builtin_amdgcn_raw_buffer_load_b64(rs, voff[k], 0, 0));
+        for (int lc = 0; lc < LC; ++lc) {
+            const bool has = g * LC + lc < L;
+            const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+                (void*)(gc + (has ? (long long)(g * LC + lc) * G : 0LL)), (short)0,
+                has ? (int)G * 8 : 0, kBufferWord3);
+#pragma unroll
+            for (int k = 0; k < NP; ++k)
+                tv[k][lc] = __builtin_bit_cast(
+                    float2, __builtin_amdgcn_raw_buffer_load_b64(rs, voff[k], 0, 0));
+        }
     };
     // The item's records, the sample indices of its (sample, segment slot)
     // pairs e = s * 16 + l, chunk 0 of the region and then the AA weights
@@ -416,8 +419,14 @@ __global__ void __launch_bounds__(kI4Threads) k_interp_lds4(
     }
     auto store_chunk = [&]() {
 #pragma unroll
-        for (int k = 0; k < NP; ++k)
-            if (sidx[k] >= 0) tile[sidx[k]] = tv[k];
+        for (int k = 0; k < NP; ++k) {
+            const int q = tid + k * NTH;
+            if (q < NQ) {
+                float4* t4 = reinterpret_cast<float4*>(tile + q * LC);
+                t4[0] = make_float4(tv[k][0].x, tv[k][0].y, tv[k][1].x, tv[k][1].y);
+                t4[1] = make_float4(tv[k][2].x, tv[k][2].y, tv[k][3].x, tv[k][3].y);
+            }
+        }
     };
     store_chunk();
 #pragma unroll
@@ -1076,6 +1085,159 @@ __global__ void __launch_bounds__(64 * kSpreadWaves) k_spread_patch_mfma(
     }
 }
 
+// The matrix-core patch spread from compact rows: k_spread_patch_mfma's
+// products (8 v_mfma_f32_16x16x4f32 per chunk of 4 entries, the same lane
+// layouts) with the B operand formed from the row slices that
+// k_spread_patch_st stages in LDS.  Lane (kq, jq) supplies, for entry
+// k0 + kq, the re and im parts of grids b0 + 4 jq + u (u < 4): the value
+// AA_l r_c (conj(AA_l) r_c for complex weights) of coil c = b / L, segment
+// l = b % L, the products of k_spread_values, so the operands, and with the
+// same chunks of 4 entries the sums, are those of the value-row kernel.
+// B a multiple of 4; F <= kStFloats for every 64-grid chunk.
+template <int ROWS>
+__global__ void __launch_bounds__(64 * kSpreadWaves) k_spread_patch_mfma_st(
+    const int4* __restrict__ witems, const int* __restrict__ bgs, const int* __restrict__ entries,
+    const float4* __restrict__ ew, int npy, const float2* __restrict__ V, int B,
+    float2* __restrict__ grids, long long G, int Gy, float2* __restrict__ partial, int n_slots,
+    int npp, long long pstride, int L, int S)
+{
+    static_assert(ROWS == 1 || ROWS == 2, "compact rows only");
+    constexpr int NE = kStEntries, FM = kStFloats;
+    constexpr int NU = (NE * FM / 2 + 63) / 64;
+    constexpr int POOL = kSpreadWaves * NE * FM > kSpreadWaves / 2 * 32 * 64
+                             ? kSpreadWaves * NE * FM
+                             : kSpreadWaves / 2 * 32 * 64;
+    __shared__ __align__(16) float pool[POOL];
+    __shared__ float sw[kSpreadWaves][NE][16];
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int4 it = witems[blockIdx.x * kSpreadWaves + wv];
+    const int gs = bgs[blockIdx.x];
+    const int b0 = blockIdx.y * 64;
+    const int cw = min(64, B - b0);                     // grids in this chunk (multiple of 4)
+    const int kq = lane >> 4, jq = lane & 15;          // MFMA k (entry) and j (column) of this lane
+    const bool colok = 4 * jq < cw;                     // this lane's 4 grids exist
+    const int nc = B / L;
+    const int c_lo = b0 / L, c_hi = (b0 + cw - 1) / L;
+    const int F1 = 2 * (c_hi - c_lo + 1);
+    const int F2 = ((ROWS == 2 ? 2 * L : L) + 1) & ~1;
+    const int U2 = (F1 + F2) / 2;
+    const int F = 2 * U2;
+    const int qd = 64 / U2, qr = 64 - qd * U2;
+    float* rows = pool + wv * NE * FM;
+    float(*mw)[16] = sw[wv];
+    const float* Rf = reinterpret_cast<const float*>(V);
+    // this lane's four grids: coil value and segment weight offsets in a
+    // staged row
+    int vo[4], ao[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const int bq = min(b0 + 4 * jq + u, B - 1);
+        const int cq = bq / L, lq = bq - cq * L;
+        vo[u] = 2 * (cq - c_lo);
+        ao[u] = F1 + lq;
+    }
+
+    pg_f4 acc[8];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) acc[t] = pg_f4{0.f, 0.f, 0.f, 0.f};
+    for (int s0 = it.y; s0 < it.z; s0 += NE) {
+        const int n = min(NE, it.z - s0);
+        const bool ok = lane < n;
+        const int e = ok ? s0 + lane : s0;
+        const int ei = entries[e];
+        float2 rv[NU];
+        {
+            int k = lane / U2, u = lane - (lane / U2) * U2;
+            asm volatile("" : "+v"(k), "+v"(u));
+#pragma unroll
+            for (int j = 0; j < NU; ++j) {
+                const int src = __shfl(ei, min(k, n - 1), 64);
+                const int off = u < U2 - F2 / 2 ? 2 * c_lo + 2 * u : 2 * nc + 2 * (u - (U2 - F2 / 2));
+                rv[j] = *reinterpret_cast<const float2*>(Rf + (long long)src * S + off);
+                k += qd;
+                u += qr;
+                if (u >= U2) {
+                    u -= U2;
+                    ++k;
+                }
+            }
+        }
+        if (lane < NE) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const float4 q = ew[4 * e + r];
+                mw[lane][4 * r] = ok ? q.x : 0.f;
+                mw[lane][4 * r + 1] = ok ? q.y : 0.f;
+                mw[lane][4 * r + 2] = ok ? q.z : 0.f;
+                mw[lane][4 * r + 3] = ok ? q.w : 0.f;
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < NU; ++j) {
+            const int t = lane + 64 * j;
+            if (t < n * U2) *reinterpret_cast<float2*>(rows + 2 * t) = rv[j];
+        }
+        pgf::wave_lds_sync();
+        for (int k0 = 0; k0 < n; k0 += 4) {
+            const int ke = k0 + kq;
+            const int m = (ke < n && colok) ? -1 : 0;
+            const float a = mw[min(ke, NE - 1)][jq];  // A[point jq][entry kq]
+            const float* row = rows + min(ke, n - 1) * F;
+            float b[8];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                float2 val = *reinterpret_cast<const float2*>(row + vo[u]);
+                const float w = row[ao[u]];
+                if (ROWS == 1) val = make_float2(w * val.x, w * val.y);
+                if (ROWS == 2) val = row_conj_weight(w, row[ao[u] + L], val);
+                b[2 * u] = __int_as_float(__float_as_int(val.x) & m);
+                b[2 * u + 1] = __int_as_float(__float_as_int(val.y) & m);
+            }
+#pragma unroll
+            for (int t = 0; t < 8; ++t)
+                acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b[t], acc[t], 0, 0, 0);
+        }
+        pgf::wave_lds_sync();  // the next stage replaces these rows and weights
+    }
+    float(*red)[32][64] = reinterpret_cast<float(*)[32][64]>(pool);
+    for (int r = 1; r < gs; r *= 2) {
+        __syncthreads();
+        if ((wv & (2 * r - 1)) == r)
+#pragma unroll
+            for (int t = 0; t < 8; ++t)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) red[wv >> 1][4 * t + q][lane] = acc[t][q];
+        __syncthreads();
+        if ((wv & (2 * r - 1)) == 0)
+#pragma unroll
+            for (int t = 0; t < 8; ++t)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) acc[t][q] += red[(wv + r) >> 1][4 * t + q][lane];
+    }
+    if (it.x < 0 || (wv & (gs - 1)) != 0 || !colok) return;
+    const int bl = b0 + 4 * jq;
+    if (it.w >= 0) {
+        float2* pp = partial + ((long long)blockIdx.y * n_slots + it.w) * 16 * 64;
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                pp[(4 * kq + q) * 64 + 4 * jq + u] = make_float2(acc[2 * u][q], acc[2 * u + 1][q]);
+        return;
+    }
+    const int pln = it.x / npp, q0 = it.x - pln * npp;
+    const int px0 = (q0 / npy) * pg::kPatch, py0 = (q0 % npy) * pg::kPatch;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int p = 4 * kq + q;
+        const long long off = pln * pstride + (long long)(px0 + p / pg::kPatch) * Gy + py0 +
+                              p % pg::kPatch;
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            grids[(long long)(bl + u) * G + off] = make_float2(acc[2 * u][q], acc[2 * u + 1][q]);
+    }
+}
+
 // patches summed in parts: add the parts' slots in slot order; block
 // (patch, grid chunk, point)
 __global__ void __launch_bounds__(64) k_spread_heavy_combine(const int4* __restrict__ comb, int npy,
@@ -1225,11 +1387,11 @@ void interp_dispatch(const pg::Gnufft& G, const float2* grids, int nc, int L, co
     const long long pstride = planes ? (long long)nc * L * gsz : 0;
     if ((!D3 || planes) && NT <= 8 && pgk::interp_lds_possible(G, L, AA != nullptr) &&
         (!tn || tn->lds)) {
-        // four segments per stage when the autotuner chose it, or else when
-        // the segment count would leave the all-at-once staging a quarter
-        // idle (L > 8 pads to 16 slots)
+        // four segments per stage when the autotuner chose it, or else from
+        // four segments up (measured faster there: the smaller blocks and
+        // LDS footprint keep more of them per CU)
         const int v4 = pgk::g_interp4.load(std::memory_order_relaxed);
-        const bool four = tn ? tn->lds == 2 : (v4 == 2 || (v4 == 1 && L > 8));
+        const bool four = tn ? tn->lds == 2 : (v4 == 2 || (v4 == 1 && L >= 4));
         // (its staging and weight offsets are 32-bit byte offsets)
         if (G.n_iitems4 > 0 && four && gsz * 4 * 8 < (1LL << 31) &&
             (long long)L * G.M * 4 < (1LL << 31)) {
@@ -1299,8 +1461,18 @@ void spread_dispatch(const pg::Gnufft& G, const float2* in, long long in_cstride
         const int nch = (B + 63) / 64;
         const int npy = g.Gy / pg::kPatch;
         const pg::Gnufft::Tuned* tn = G.tuned_for(B);
-        if (B % 4 == 0 &&
-            (tn ? tn->mfma != 0 : pgk::g_spread_mfma.load(std::memory_order_relaxed) != 0))
+        const bool mfma =
+            B % 4 == 0 &&
+            (tn ? tn->mfma != 0 : pgk::g_spread_mfma.load(std::memory_order_relaxed) != 0);
+        if (mfma && S) {
+            // compact rows reach the matrix cores only through the staged form
+            auto launch = [&](auto kern) {
+                hipLaunchKernelGGL(kern, dim3(G.n_sblocks, nch), dim3(64 * kSpreadWaves), 0, s,
+                                   G.witems, G.bgs, G.entries, (const float4*)G.ent_w, npy, vals,
+                                   B, grids, gsz, g.Gy, partial, G.n_slots, G.npp, pstride, L, S);
+            };
+            AAi ? launch(k_spread_patch_mfma_st<2>) : launch(k_spread_patch_mfma_st<1>);
+        } else if (mfma)
             hipLaunchKernelGGL(k_spread_patch_mfma, dim3(G.n_sblocks, nch), dim3(64 * kSpreadWaves),
                                0, s, G.witems, G.bgs, G.entries, (const float4*)G.ent_w, npy, vals,
                                B, grids, gsz, g.Gy, partial, G.n_slots, G.npp, pstride);
@@ -1354,7 +1526,10 @@ int spread_row_floats(const pg::Gnufft& G, int nc, int L, const float* AA, const
     if (!g_spread_rows.load(std::memory_order_relaxed) || !AA || L < 2) return 0;
     if (G.geo.dims == 3 && !zclass_active(G, B)) return 0;
     const pg::Gnufft::Tuned* tn = G.tuned_for(B);
-    if (B % 4 == 0 && (tn ? tn->mfma != 0 : g_spread_mfma.load(std::memory_order_relaxed) != 0))
+    // the matrix-core spread reads compact rows only in its staged form
+    if (B % 4 == 0 && (tn ? tn->mfma != 0 : g_spread_mfma.load(std::memory_order_relaxed) != 0) &&
+        !(g_spread_stage.load(std::memory_order_relaxed) &&
+          spread_st_floats(B, L, AAi != nullptr) <= kStFloats))
         return 0;
     const int S = (2 * nc + (AAi ? 2 : 1) * L + 3) & ~3;
     // the rows live in the plan's value buffer (M x B float2) and a block of

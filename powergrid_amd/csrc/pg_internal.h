@@ -110,7 +110,7 @@ extern std::atomic<int> g_spread_mfma;
 extern std::atomic<int> g_zclass;
 // the LDS-staged interpolation stages four segments at a time in 256-thread
 // blocks over items of <= 64 samples (k_interp_lds4): 2 always, 1 (the
-// default) for more than 8 segments, 0 never (all segments at once in
+// default) from 4 segments up, 0 never (all segments at once in
 // 512-thread blocks over items of <= 128 samples)
 extern std::atomic<int> g_interp4;
 

@@ -281,3 +281,30 @@ def test_staged_row_spread_matches_patch_spread(N, Nz, nc, L, itype):
         finally:
             P.set_tuning("spread_stage", 1)
     np.testing.assert_array_equal(out[1], out[0])
+
+
+@pytest.mark.parametrize("N,Nz,nc,L,itype", [(128, 32, 2, 12, 1), (128, 1, 8, 8, 1),
+                                             (64, 1, 4, 6, 2)])
+def test_staged_row_mfma_spread_matches_value_row_mfma(N, Nz, nc, L, itype):
+    # the matrix-core patch spread from compact rows staged in LDS against
+    # the one that reads rows of nc*L values (spread_stage 0): the same
+    # operands and chunks of 4 entries, so the same bits
+    print(f"[staged mfma spread N={N} Nz={Nz} nc={nc} L={L} type={itype}]", flush=True)
+    p = S.make_problem(N, 2 if Nz > 1 else 8, 2048, nc, L, Nz=Nz, peak_hz=200.0, seed=2)
+    Gg = P.Gnufft(p.M, 2.0, p.Nx, p.Ny, p.Nz, p.kx, p.ky, p.kz)
+    T = P.TimeSegmentation(Gg, p.fm, p.t, p.M, p.N, p.L, interptype=itype)
+    Sg = P.SENSE(T, p.sens, p.M, p.N, p.nc)
+    rng = np.random.default_rng(6)
+    y = (rng.standard_normal(p.M * p.nc) + 1j * rng.standard_normal(p.M * p.nc)).astype(np.complex64)
+    out = {}
+    P.set_tuning("spread_mfma", 1)
+    try:
+        for st in (1, 0):
+            P.set_tuning("spread_stage", st)
+            try:
+                out[st] = Sg / y
+            finally:
+                P.set_tuning("spread_stage", 1)
+    finally:
+        P.set_tuning("spread_mfma", 0)
+    np.testing.assert_array_equal(out[1], out[0])
