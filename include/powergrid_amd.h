@@ -354,6 +354,9 @@ int pg_recon_solve_batch(size_t K, const pg_op* A, const pg_penalty* R, const vo
  * "zfuse" 1 (default): on z-class planes at gridos 2 the adjoint's y IFFT
  * and z stage run as one kernel; 0: two kernels with the class planes in
  * between.  Same sums, same order.
+ * "zstage_mfma" 1 (default): the forward z stage of z-class planes (Nz and
+ * the class count at most 32) forms its class sums on the matrix cores;
+ * 0: on the vector units.  Same products, z sums in another order.
  * "spread_stage" 1 (default): the compact-row spread loads the row slices of
  * each group of up to 32 patch entries into LDS with all loads in flight,
  * then sums from LDS (when a 64-grid chunk needs at most 48 floats of a

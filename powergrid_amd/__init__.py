@@ -400,7 +400,8 @@ def set_tuning(key, value):
     pushes roctx tracing ranges (trace()); "zfuse" 1 (default) runs the
     adjoint y IFFT and z stage of z-class planes as one kernel;
     "spread_stage" 1 (default) stages the compact rows of each group of
-    patch entries in LDS, 0 reads them per entry."""
+    patch entries in LDS, 0 reads them per entry; "zstage_mfma" 1 (default)
+    forms the forward z stage's class sums on the matrix cores."""
     check(lib().pg_set_tuning(key.encode(), int(value)))
 
 

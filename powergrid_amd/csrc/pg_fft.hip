@@ -616,6 +616,103 @@ __global__ void __launch_bounds__(256) k_zstage_fwd_reg(
     }
 }
 
+// The forward z stage on the matrix cores (Nz <= 32, nzc <= 32): the grid
+// and staging of k_zstage_fwd_reg, with the class sums as a real product
+//   [Re H; Im H] (64 x 64 columns) = [[Zr, -Zi], [Zi, Zr]] (64 x 64) [vr; vi]
+// (the -Zi block applied as -vi) in v_mfma_f32_16x16x4f32 steps (A[i][k] in
+// lane i + 16k, B[k][j] in lane j + 16k, D[i][j] in lane j + 16 (i / 4),
+// register i % 4).  Lane (p, q) =
+// (lane % 16, lane / 16) keeps the class matrix entries of rows 16 rt + p
+// and z = 4 s + q in registers for the whole kernel, and per segment and
+// 16-column tile forms v_z = dxy Wo_l(z) S_c x(z) for its column 16 ct + p
+// and those z.  Same products as k_zstage_fwd_reg; the z sums run in the
+// matrix cores' order (four terms per step).
+typedef float pgz_f4 __attribute__((ext_vector_type(4)));
+
+__global__ void __launch_bounds__(256) k_zstage_fwd_mfma(
+    const float2* __restrict__ img, long long img_cstride, const float2* __restrict__ S,
+    const float2* __restrict__ Wo, const float* __restrict__ dxy, int Nxy, int Nz, int L, int B,
+    int nc, int nzc, const float2* __restrict__ zm, float2* __restrict__ H)
+{
+    constexpr int NW = 4;
+    __shared__ float2 su[32 * 64];
+    const int lane = threadIdx.x & 63;
+    const int g = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int c = blockIdx.x % nc;
+    const int col0 = (blockIdx.x / nc) * 64;
+    const long long N = (long long)Nxy * Nz;
+    {
+        const int col = col0 + lane;
+        const int colc = col < Nxy ? col : Nxy - 1;
+        for (int z = g; z < Nz; z += NW) {
+            const long long o = (long long)z * Nxy + colc;
+            float2 a = img[c * img_cstride + o];
+            if (S) a = pgf::cmul(S[(long long)c * N + o], a);
+            su[z * 64 + lane] = a;
+        }
+    }
+    const int p = lane & 15, q = lane >> 4;
+    // class matrix entries: row k = 16 rt + p, z = 4 s + q (zero outside)
+    float zr[2][8], zi[2][8];
+#pragma unroll
+    for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+            const int k = 16 * rt + p, z = 4 * s + q;
+            const float2 w = (k < nzc && z < Nz) ? zm[(long long)k * Nz + z] : make_float2(0.f, 0.f);
+            zr[rt][s] = w.x;
+            zi[rt][s] = w.y;
+        }
+    __syncthreads();
+    for (int l = g; l < L; l += NW) {
+        float2* hp = H + ((long long)c * L + l) * Nxy;
+#pragma unroll 1
+        for (int ct = 0; ct < 4; ++ct) {
+            const int cl = 16 * ct + p;  // column in the block's chunk
+            const int col = col0 + cl;
+            const int colc = col < Nxy ? col : Nxy - 1;
+            const float d = dxy[colc];
+            float2 v[8];
+#pragma unroll
+            for (int s = 0; s < 8; ++s) {
+                const int z = 4 * s + q;
+                const int zc = z < Nz ? z : Nz - 1;
+                float2 a = su[zc * 64 + cl];
+                if (Wo) a = pgf::cmul(Wo[(long long)l * N + (long long)zc * Nxy + colc], a);
+                v[s] = z < Nz ? make_float2(a.x * d, a.y * d) : make_float2(0.f, 0.f);
+            }
+            pgz_f4 hr[2], hi[2];
+#pragma unroll
+            for (int rt = 0; rt < 2; ++rt) {
+                hr[rt] = pgz_f4{0.f, 0.f, 0.f, 0.f};
+                hi[rt] = pgz_f4{0.f, 0.f, 0.f, 0.f};
+            }
+            // Re H = Zr vr + Zi (-vi), Im H = Zi vr + Zr vi
+#pragma unroll
+            for (int s = 0; s < 16; ++s) {
+                const float b = s < 8 ? v[s].x : v[s - 8].y;
+                const float br = s < 8 ? b : -b;
+#pragma unroll
+                for (int rt = 0; rt < 2; ++rt) {
+                    const float ar = s < 8 ? zr[rt][s] : zi[rt][s - 8];
+                    const float ai = s < 8 ? zi[rt][s] : zr[rt][s - 8];
+                    hr[rt] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar, br, hr[rt], 0, 0, 0);
+                    hi[rt] = __builtin_amdgcn_mfma_f32_16x16x4f32(ai, b, hi[rt], 0, 0, 0);
+                }
+            }
+            if (col < Nxy)
+#pragma unroll
+                for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int k = 16 * rt + 4 * q + r;
+                        if (k < nzc)
+                            hp[(long long)k * B * Nxy + col] = make_float2(hr[rt][r], hi[rt][r]);
+                    }
+        }
+    }
+}
+
 constexpr int kZfFftWaves = 2;
 
 // The adjoint y IFFT of the class planes fused with the adjoint z stage
@@ -1073,7 +1170,11 @@ void launch_fwd_prep_fft(const pg::Gnufft& G, const float2* img, long long img_c
 #define PG_ZSF(NZ)                                                                              \
     hipLaunchKernelGGL(k_zstage_fwd_reg<NZ>, dim3(nch * nc), dim3(256), 0, s, img, img_cstride, \
                        w.S, w.Wo, G.dapxy, nxy, g.Nz, L, B, nc, G.nzc, G.zmat, tbuf2)
-        if (g.Nz <= 8) {
+        if (g_zmfma.load(std::memory_order_relaxed) && g.Nz <= 32 && G.nzc <= 32) {
+            hipLaunchKernelGGL(k_zstage_fwd_mfma, dim3(nch * nc), dim3(256), 0, s, img,
+                               img_cstride, w.S, w.Wo, G.dapxy, nxy, g.Nz, L, B, nc, G.nzc,
+                               G.zmat, tbuf2);
+        } else if (g.Nz <= 8) {
             PG_ZSF(8);
         } else if (g.Nz <= 16) {
             PG_ZSF(16);

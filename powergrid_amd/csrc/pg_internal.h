@@ -123,6 +123,9 @@ extern std::atomic<int> g_spread_rows;
 // on z-class planes with gridos 2, the adjoint y IFFT and z stage run as one
 // kernel (the cropped class columns staged in LDS) when set (the default)
 extern std::atomic<int> g_zfuse;
+// the forward z stage of z-class planes runs on the matrix cores (Nz and
+// the class count at most 32) when set (default), else on the vector units
+extern std::atomic<int> g_zmfma;
 
 // the compact-row patch spread stages each stage's row slices in LDS
 // (k_spread_patch_st) when set (default), else reads them per entry
