@@ -715,6 +715,45 @@ __global__ void __launch_bounds__(256) k_zstage_fwd_mfma(
 
 constexpr int kZfFftWaves = 2;
 
+// Step 1 of the fused adjoint below: the first kZfFftWaves wavefronts
+// transform the class columns of grid b along y (ifftshift on load, as
+// k_col_ifft_crop; lane = (line jl, thread t of the line's register FFT))
+// and keep the Ny cropped rows in sh (nzc x Ny).
+template <int R1, int R2>
+__device__ __forceinline__ void zf_class_columns(const float2* __restrict__ T, float2* sh,
+                                                 float2* my, int wv, int t, int jl, int nzc,
+                                                 int B, int b, int Nx, int x, int Ny, int shift,
+                                                 const float2* __restrict__ tw)
+{
+    constexpr int N = R1 * R2;
+    constexpr int LPW = 64 / R2;
+    if (wv >= kZfFftWaves) return;
+#pragma unroll 1
+    for (int k0 = 0; k0 < nzc; k0 += kZfFftWaves * LPW) {
+        const int k = k0 + wv * LPW + jl;
+        const bool valid = k < nzc;
+        const float2* src = T + (((long long)(valid ? k : 0) * B + b) * Nx + x) * N;
+        // a copy of t the compiler cannot see through, so that the
+        // per-element indices and twiddles are formed in the loop instead
+        // of being held across it
+        int tt = t;
+        asm volatile("" : "+v"(tt));
+        float2 v[R1];
+#pragma unroll
+        for (int m = 0; m < R1; ++m) v[m] = src[(tt + R2 * m + N / 2) & (N - 1)];
+        pgf::line_fft<R1, R2, 1, false>(
+            v, tt, my, [](int k1, int q) { return k1 * (R2 + 1) + q; }, tw);
+        if (!valid) continue;
+#pragma unroll
+        for (int r = 0; r < R1 / R2; ++r)
+#pragma unroll
+            for (int k2 = 0; k2 < R2; ++k2) {
+                const int yy = (tt + R2 * r + R1 * k2 - shift) & (N - 1);
+                if (yy < Ny) sh[k * Ny + yy] = v[r * R2 + k2];
+            }
+    }
+}
+
 // The adjoint y IFFT of the class planes fused with the adjoint z stage
 // (gridos 2, 64 <= Ny = Gy/2 <= 256).  One block of 256 threads per (image column
 // x, coil c), c fastest in the grid (the blocks of one x share its WoH in
@@ -756,33 +795,7 @@ __global__ void __launch_bounds__(256) k_yifft_zadj(
     for (int q = 0; q < ZW; ++q) acc[q] = make_float2(0.f, 0.f);
 #pragma unroll 1
     for (int l = 0; l < L; ++l) {
-        const int b = c * L + l;
-        if (wv < kZfFftWaves) {
-#pragma unroll 1
-            for (int k0 = 0; k0 < nzc; k0 += kZfFftWaves * LPW) {
-                const int k = k0 + wv * LPW + jl;
-                const bool valid = k < nzc;
-                const float2* src = T + (((long long)(valid ? k : 0) * B + b) * Nx + x) * N;
-                // a copy of t the compiler cannot see through, so that the
-                // per-element indices and twiddles are formed in the loop
-                // instead of being held across it
-                int tt = t;
-                asm volatile("" : "+v"(tt));
-                float2 v[R1];
-#pragma unroll
-                for (int m = 0; m < R1; ++m) v[m] = src[(tt + R2 * m + N / 2) & (N - 1)];
-                pgf::line_fft<R1, R2, 1, false>(
-                    v, tt, my, [](int k1, int q) { return k1 * (R2 + 1) + q; }, tw);
-                if (!valid) continue;
-#pragma unroll
-                for (int r = 0; r < R1 / R2; ++r)
-#pragma unroll
-                    for (int k2 = 0; k2 < R2; ++k2) {
-                        const int yy = (tt + R2 * r + R1 * k2 - shift) & (N - 1);
-                        if (yy < Ny) sh[k * Ny + yy] = v[r * R2 + k2];
-                    }
-            }
-        }
+        zf_class_columns<R1, R2>(T, sh, my, wv, t, jl, nzc, B, c * L + l, Nx, x, Ny, shift, tw);
         __syncthreads();
         float2 u[ZW];
 #pragma unroll

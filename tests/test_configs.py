@@ -310,24 +310,26 @@ def test_staged_row_mfma_spread_matches_value_row_mfma(N, Nz, nc, L, itype):
     np.testing.assert_array_equal(out[1], out[0])
 
 
-@pytest.mark.parametrize("Nz,nc,L", [(32, 2, 12), (20, 2, 8)])
-def test_sos3d_mfma_z_stage_matches_vector_z_stage(Nz, nc, L):
+@pytest.mark.parametrize("N,Nz,nc,L", [(64, 32, 2, 12), (64, 20, 2, 8), (128, 32, 2, 12)])
+def test_sos3d_mfma_z_stage_matches_vector_z_stage(N, Nz, nc, L):
     # the forward z stage on the matrix cores (pg_set_tuning("zstage_mfma",
     # 1)) against the vector-unit one: the same products, the z sums in
-    # another order, so equal to FP32 rounding; the adjoint is unaffected
-    print(f"[sos3d z stage mfma vs vector, 64x64x{Nz} nc={nc} L={L}]", flush=True)
-    p = S.make_problem(64, 2, 1024, nc, L, Nz=Nz, peak_hz=200.0, seed=4)
+    # another order, so equal to FP32 rounding; the adjoint takes the same
+    # path either way
+    print(f"[sos3d z stage mfma vs vector, {N}x{N}x{Nz} nc={nc} L={L}]", flush=True)
+    p = S.make_problem(N, 2, 1024 if N == 64 else 2048, nc, L, Nz=Nz, peak_hz=200.0, seed=4)
     Gg = P.Gnufft(p.M, 2.0, p.Nx, p.Ny, p.Nz, p.kx, p.ky, p.kz)
     Sg = P.SENSE(P.TimeSegmentation(Gg, p.fm, p.t, p.M, p.N, p.L), p.sens, p.M, p.N, p.nc)
     rng = np.random.default_rng(8)
     x = (rng.standard_normal(p.N) + 1j * rng.standard_normal(p.N)).astype(np.complex64)
+    y = (rng.standard_normal(p.M * p.nc) + 1j * rng.standard_normal(p.M * p.nc)).astype(np.complex64)
     out = {}
     for m in (1, 0):
         P.set_tuning("zstage_mfma", m)
         try:
-            out[m] = Sg * x
+            out[m] = (Sg * x, Sg / y)
         finally:
             P.set_tuning("zstage_mfma", 1)
-    e = rel(out[1], out[0])
-    print(f"  forward {e:.1e}", flush=True)
-    assert e < 1e-6
+    ef, ea = rel(out[1][0], out[0][0]), rel(out[1][1], out[0][1])
+    print(f"  forward {ef:.1e}, adjoint {ea:.1e}", flush=True)
+    assert ef < 1e-6 and ea < 1e-6
