@@ -339,7 +339,7 @@ __global__ void __launch_bounds__(kI4Threads) k_interp_lds4(
     __shared__ __align__(16) float2 tile[NQ * LC];
     constexpr int NR = (CH * 5 + NTH - 1) / NTH;   // record loads per thread
     constexpr int NA = (CH * 16 + NTH - 1) / NTH;  // weight loads per thread
-    __shared__ float4 srec[NR * NTH];  // (stored whole: no store waits behind a branch)
+    __shared__ float4 srec[CH * 5];
     __shared__ float saa[CH * 16];
     __shared__ float saai[CPX ? CH * 16 : 1];
     const int tid = threadIdx.x;
@@ -430,7 +430,8 @@ __global__ void __launch_bounds__(kI4Threads) k_interp_lds4(
     };
     store_chunk();
 #pragma unroll
-    for (int k = 0; k < NR; ++k) srec[tid + k * NTH] = rv[k];
+    for (int k = 0; k < NR; ++k)
+        if (tid + k * NTH < CH * 5) srec[tid + k * NTH] = rv[k];
 #pragma unroll
     for (int k = 0; k < NA; ++k) {
         saa[tid + k * NTH] = av[k];
@@ -822,7 +823,7 @@ __global__ void __launch_bounds__(64 * kSpreadWaves, 4) k_spread_patch_st(
     const int4* __restrict__ witems, const int* __restrict__ bgs, const int* __restrict__ entries,
     const float4* __restrict__ ew, int npy, const float2* __restrict__ V, int B,
     float2* __restrict__ grids, long long G, int Gy, float2* __restrict__ partial, int n_slots,
-    int npp, long long pstride, int L, int S)
+    int npp, long long pstride, int L, int S, int nsb, int nch)
 {
     static_assert(ROWS == 1 || ROWS == 2, "compact rows only");
     constexpr int NE = kStEntries, FM = kStFloats;
@@ -835,9 +836,16 @@ __global__ void __launch_bounds__(64 * kSpreadWaves, 4) k_spread_patch_st(
     __shared__ __align__(16) float pool[POOL];
     __shared__ float4 sw[kSpreadWaves][NE][4];
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
-    const int4 it = witems[blockIdx.x * kSpreadWaves + wv];
-    const int gs = bgs[blockIdx.x];
-    const int b0 = blockIdx.y * 64;
+    // XCD-aware order: blocks go to the 8 XCDs round-robin, so block
+    // 8 m + x runs on XCD x; the nch grid chunks of a patch block take
+    // consecutive m on one XCD, whose L2 then serves the entries' rows and
+    // weights to all of them
+    const int xcd = blockIdx.x & 7, m = blockIdx.x >> 3;
+    const int chunk = m % nch, pb = (m / nch) * 8 + xcd;
+    if (pb >= nsb) return;  // padding blocks (the whole block leaves)
+    const int4 it = witems[pb * kSpreadWaves + wv];
+    const int gs = bgs[pb];
+    const int b0 = chunk * 64;
     const int b = b0 + lane;
     const bool active = b < B;
     const int bb = active ? b : B - 1;
@@ -938,7 +946,7 @@ __global__ void __launch_bounds__(64 * kSpreadWaves, 4) k_spread_patch_st(
     }
     if (!active || it.x < 0 || (wv & (gs - 1)) != 0) return;
     if (it.w >= 0) {
-        float2* pp = partial + ((long long)blockIdx.y * n_slots + it.w) * 16 * 64 + lane;
+        float2* pp = partial + ((long long)chunk * n_slots + it.w) * 16 * 64 + lane;
 #pragma unroll
         for (int q = 0; q < 16; ++q) pp[q * 64] = acc[q];
         return;
@@ -1484,10 +1492,17 @@ void spread_dispatch(const pg::Gnufft& G, const float2* in, long long in_cstride
             };
             const bool st = S && pgk::g_spread_stage.load(std::memory_order_relaxed) != 0 &&
                             spread_st_floats(B, L, AAi != nullptr) <= kStFloats;
+            auto launch_st = [&](auto kern) {
+                const int nsb8 = (G.n_sblocks + 7) / 8 * 8;
+                hipLaunchKernelGGL(kern, dim3((unsigned)nsb8 * nch), dim3(64 * kSpreadWaves), 0, s,
+                                   G.witems, G.bgs, G.entries, (const float4*)G.ent_w, npy, vals,
+                                   B, grids, gsz, g.Gy, partial, G.n_slots, G.npp, pstride, L, S,
+                                   G.n_sblocks, nch);
+            };
             if (st && AAi)
-                launch(k_spread_patch_st<2>, L, S);
+                launch_st(k_spread_patch_st<2>);
             else if (st)
-                launch(k_spread_patch_st<1>, L, S);
+                launch_st(k_spread_patch_st<1>);
             else if (S && AAi)
                 launch(k_spread_patch<2>, L, S);
             else if (S)
