@@ -3389,6 +3389,8 @@ int pg_set_tuning(const char* key, int value)
         pgk::g_zfuse.store(value ? 1 : 0);
     else if (k == "zstage_mfma")
         pgk::g_zmfma.store(value ? 1 : 0);
+
+
     else if (k == "spread_stage")
         pgk::g_spread_stage.store(value ? 1 : 0);
 
