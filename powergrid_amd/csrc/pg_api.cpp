@@ -392,7 +392,9 @@ static void gnufft_setup(Gnufft& G, size_t M, float os, size_t Nx, size_t Ny, si
     g.Gz = (Nz == 1) ? 1 : (int)std::ceil(os * (float)Nz);
     PG_REQUIRE(std::fabs(os * (float)Nx - g.Gx) < 1e-6f && std::fabs(os * (float)Ny - g.Gy) < 1e-6f,
                PG_ERR_UNSUPPORTED, "gridos*N must be an even integer");
-    PG_REQUIRE(g.Gx % 4 == 0, PG_ERR_UNSUPPORTED, "gridos*N must be a multiple of 4");
+    // (the 4x4 spread patches tile the x-y grid)
+    PG_REQUIRE(g.Gx % 4 == 0 && g.Gy % 4 == 0, PG_ERR_UNSUPPORTED,
+               "gridos*N must be a multiple of 4");
     PG_REQUIRE(Nz == 1 || (std::fabs(os * (float)Nz - g.Gz) < 1e-6f && g.Gz % 4 == 0),
                PG_ERR_UNSUPPORTED, "gridos*Nz must be a multiple of 4");
     g.W = W;

@@ -871,6 +871,18 @@ __global__ void __launch_bounds__(64 * kSpreadWaves, 4) k_spread_patch_st(
         const bool ok = lane < n;
         const int e = ok ? s0 + lane : s0;
         const int ei = entries[e];
+        // the stage's weights: lane k < n loads entry k's 64 bytes through a
+        // buffer resource of n entries (other lanes read zeros), all four
+        // loads issued before the rows' (no branch, no wait between them)
+        float4 wq[4];
+        {
+            const __amdgpu_buffer_rsrc_t rw = __builtin_amdgcn_make_buffer_rsrc(
+                (void*)(ew + 4 * (long long)s0), (short)0, n * 64, kBufferWord3);
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                wq[r] = __builtin_bit_cast(
+                    float4, __builtin_amdgcn_raw_buffer_load_b128(rw, lane * 64 + r * 16, 0, 0));
+        }
         // unit t = lane + 64 j of the stage: entry k = t / U2, unit u = t % U2
         float2 rv[NU];
         {
@@ -892,10 +904,7 @@ __global__ void __launch_bounds__(64 * kSpreadWaves, 4) k_spread_patch_st(
         }
         if (lane < NE) {
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const float4 q = ew[4 * e + r];
-                mw[lane][r] = ok ? q : make_float4(0.f, 0.f, 0.f, 0.f);
-            }
+            for (int r = 0; r < 4; ++r) mw[lane][r] = wq[r];
         }
 #pragma unroll
         for (int j = 0; j < NU; ++j) {
@@ -954,11 +963,16 @@ __global__ void __launch_bounds__(64 * kSpreadWaves, 4) k_spread_patch_st(
     }
     const int pln = it.x / npp, q0 = it.x - pln * npp;
     const int px0 = (q0 / npy) * pg::kPatch, py0 = (q0 % npy) * pg::kPatch;
+    // each patch row is 32 contiguous bytes at a 32-byte boundary (py0, Gy
+    // and the plane and grid strides are multiples of 4 points): two
+    // 16-byte stores per row
     float2* g = grids + pln * pstride + (long long)b * G + (long long)px0 * Gy + py0;
 #pragma unroll
-    for (int a = 0; a < pg::kPatch; ++a)
-#pragma unroll
-        for (int q = 0; q < pg::kPatch; ++q) g[(long long)a * Gy + q] = acc[a * 4 + q];
+    for (int a = 0; a < pg::kPatch; ++a) {
+        float4* g4 = reinterpret_cast<float4*>(g + (long long)a * Gy);
+        g4[0] = make_float4(acc[a * 4].x, acc[a * 4].y, acc[a * 4 + 1].x, acc[a * 4 + 1].y);
+        g4[1] = make_float4(acc[a * 4 + 2].x, acc[a * 4 + 2].y, acc[a * 4 + 3].x, acc[a * 4 + 3].y);
+    }
 }
 
 // the largest staged floats per entry of a launch's 64-grid chunks
