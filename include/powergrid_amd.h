@@ -357,6 +357,11 @@ int pg_recon_solve_batch(size_t K, const pg_op* A, const pg_penalty* R, const vo
  * "zstage_mfma" 1 (default): the forward z stage of z-class planes (Nz and
  * the class count at most 32) forms its class sums on the matrix cores;
  * 0: on the vector units.  Same products, z sums in another order.
+ * "zadj_mfma" 1 (default): the fused adjoint y IFFT + z stage of z-class
+ * planes (Ny 64 or 128, Nz and the class count at most 32) runs its IFFTs on
+ * all four wavefronts and its class sums on the matrix cores; 0: two
+ * wavefronts' IFFTs and vector-unit sums.  Same products, sums in another
+ * order.
  * "spread_stage" 1 (default): the compact-row spread loads the row slices of
  * each group of up to 32 patch entries into LDS with all loads in flight,
  * then sums from LDS (when a 64-grid chunk needs at most 48 floats of a

@@ -401,7 +401,8 @@ def set_tuning(key, value):
     adjoint y IFFT and z stage of z-class planes as one kernel;
     "spread_stage" 1 (default) stages the compact rows of each group of
     patch entries in LDS, 0 reads them per entry; "zstage_mfma" 1 (default)
-    forms the forward z stage's class sums on the matrix cores."""
+    forms the forward z stage's class sums on the matrix cores; "zadj_mfma"
+    1 (default) the fused adjoint's, with its y IFFTs on four wavefronts."""
     check(lib().pg_set_tuning(key.encode(), int(value)))
 
 

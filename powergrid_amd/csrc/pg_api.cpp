@@ -3391,6 +3391,8 @@ int pg_set_tuning(const char* key, int value)
         pgk::g_zfuse.store(value ? 1 : 0);
     else if (k == "zstage_mfma")
         pgk::g_zmfma.store(value ? 1 : 0);
+    else if (k == "zadj_mfma")
+        pgk::g_zamfma.store(value ? 1 : 0);
 
 
     else if (k == "spread_stage")

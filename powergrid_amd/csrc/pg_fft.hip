@@ -719,7 +719,7 @@ constexpr int kZfFftWaves = 2;
 // transform the class columns of grid b along y (ifftshift on load, as
 // k_col_ifft_crop; lane = (line jl, thread t of the line's register FFT))
 // and keep the Ny cropped rows in sh (nzc x Ny).
-template <int R1, int R2>
+template <int R1, int R2, int FW = kZfFftWaves>
 __device__ __forceinline__ void zf_class_columns(const float2* __restrict__ T, float2* sh,
                                                  float2* my, int wv, int t, int jl, int nzc,
                                                  int B, int b, int Nx, int x, int Ny, int shift,
@@ -727,9 +727,9 @@ __device__ __forceinline__ void zf_class_columns(const float2* __restrict__ T, f
 {
     constexpr int N = R1 * R2;
     constexpr int LPW = 64 / R2;
-    if (wv >= kZfFftWaves) return;
+    if (wv >= FW) return;
 #pragma unroll 1
-    for (int k0 = 0; k0 < nzc; k0 += kZfFftWaves * LPW) {
+    for (int k0 = 0; k0 < nzc; k0 += FW * LPW) {
         const int k = k0 + wv * LPW + jl;
         const bool valid = k < nzc;
         const float2* src = T + (((long long)(valid ? k : 0) * B + b) * Nx + x) * N;
@@ -838,6 +838,116 @@ __global__ void __launch_bounds__(256) k_yifft_zadj(
             v = pgf::cmul(v, sc);
         }
         out[c * out_cstride + (long long)z * Nxy + o] = v;
+    }
+}
+
+// The fused adjoint for Ny = Gy / 2 = 64 CT (CT = 1, 2), Nz and nzc at most
+// 32, with all four wavefronts on the y IFFTs and the z stage on the matrix
+// cores: after step 1, wavefront wv takes the CT 16-row tiles
+// y = 16 (wv CT + ct) + p of the cropped class columns and forms
+//   [Re u; Im u] (z x y) = [[Zr^T, Zi^T] [hr; hi], [Zr^T, Zi^T] [hi; -hr]]
+// (u_z = sum_k conj(Zm[k][z]) h_k) in v_mfma_f32_16x16x4f32 steps: lane
+// (p, q) = (lane % 16, lane / 16) reads Zm[k][z] for z = 16 rt + p,
+// k = 4 s + q from an LDS copy of the class matrix, supplies h_k(y) for
+// k = 4 s + q, and receives u_z(y) for z = 16 rt + 4 q + r (register r),
+// which it multiplies by WoH_l(z) and adds to its sums.  The epilogue is
+// k_yifft_zadj's.  Same products; the k sums in the matrix cores' order.
+// Static LDS (class columns, four wavefronts' exchange rows, Zm): 75 KB at
+// Gy = 256, two blocks per CU.
+template <int R1, int R2, int CT>
+__global__ void __launch_bounds__(256) k_yifft_zadj_mfma(
+    const float2* __restrict__ T, const float2* __restrict__ S, const float2* __restrict__ WoH,
+    const float* __restrict__ dxy, int Nx, int Ny, int Nz, int offY, int L, int B, int nc,
+    int nzc, const float2* __restrict__ zm, const float2* __restrict__ tw,
+    float2* __restrict__ out, long long out_cstride)
+{
+    constexpr int N = R1 * R2;  // Gy
+    constexpr int NY = N / 2;
+    constexpr int LPW = 64 / R2;
+    constexpr int PITCH = R1 * (R2 + 1);
+    static_assert(NY == 64 * CT, "Ny = 64 CT");
+    __shared__ float2 sh[32 * NY];
+    __shared__ float2 ex[4 * LPW * PITCH];
+    __shared__ float2 zs[32 * 32];
+    const int tid = threadIdx.x;
+    const int c = blockIdx.x % nc, x = blockIdx.x / nc;
+    const int Nxy = Nx * Ny;
+    const long long Nimg = (long long)Nxy * Nz;
+    const int lane = tid & 63, wv = tid >> 6;
+    const int t = lane % R2, jl = lane / R2;
+    const int shift = (offY + N / 2) % N;
+    float2* my = ex + (wv * LPW + jl) * PITCH;
+    const int p = lane & 15, q = lane >> 4;
+    for (int e = tid; e < nzc * Nz; e += 256) zs[e] = zm[e];
+    float2 acc[CT][2][4];
+#pragma unroll
+    for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+        for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) acc[ct][rt][r] = make_float2(0.f, 0.f);
+#pragma unroll 1
+    for (int l = 0; l < L; ++l) {
+        zf_class_columns<R1, R2, 4>(T, sh, my, wv, t, jl, nzc, B, c * L + l, Nx, x, Ny, shift,
+                                    tw);
+        __syncthreads();
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct) {
+            const int y = 16 * (wv * CT + ct) + p;
+            pgz_f4 ur[2], ui[2];
+#pragma unroll
+            for (int rt = 0; rt < 2; ++rt) {
+                ur[rt] = pgz_f4{0.f, 0.f, 0.f, 0.f};
+                ui[rt] = pgz_f4{0.f, 0.f, 0.f, 0.f};
+            }
+#pragma unroll 2
+            for (int s = 0; s < 8; ++s) {
+                const int k = 4 * s + q;
+                const float2 h = k < nzc ? sh[k * Ny + y] : make_float2(0.f, 0.f);
+#pragma unroll
+                for (int rt = 0; rt < 2; ++rt) {
+                    const int z = 16 * rt + p;
+                    const float2 w =
+                        (k < nzc && z < Nz) ? zs[k * Nz + z] : make_float2(0.f, 0.f);
+                    ur[rt] = __builtin_amdgcn_mfma_f32_16x16x4f32(w.x, h.x, ur[rt], 0, 0, 0);
+                    ur[rt] = __builtin_amdgcn_mfma_f32_16x16x4f32(w.y, h.y, ur[rt], 0, 0, 0);
+                    ui[rt] = __builtin_amdgcn_mfma_f32_16x16x4f32(w.x, h.y, ui[rt], 0, 0, 0);
+                    ui[rt] = __builtin_amdgcn_mfma_f32_16x16x4f32(w.y, -h.x, ui[rt], 0, 0, 0);
+                }
+            }
+            const long long o = (long long)x * Ny + y;
+#pragma unroll
+            for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int z = min(16 * rt + 4 * q + r, Nz - 1);
+                    float2 u = make_float2(ur[rt][r], ui[rt][r]);
+                    if (WoH) u = pgf::cmul(u, WoH[(long long)l * Nimg + (long long)z * Nxy + o]);
+                    acc[ct][rt][r].x += u.x;
+                    acc[ct][rt][r].y += u.y;
+                }
+        }
+        __syncthreads();  // the next segment's columns replace these
+    }
+#pragma unroll
+    for (int ct = 0; ct < CT; ++ct) {
+        const int y = 16 * (wv * CT + ct) + p;
+        const long long o = (long long)x * Ny + y;
+        const float d = dxy[o];
+#pragma unroll
+        for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int z = 16 * rt + 4 * q + r;
+                if (z >= Nz) continue;
+                float2 v = make_float2(acc[ct][rt][r].x * d, acc[ct][rt][r].y * d);
+                if (S) {
+                    float2 sc = S[(long long)c * Nimg + (long long)z * Nxy + o];
+                    sc.y = -sc.y;
+                    v = pgf::cmul(v, sc);
+                }
+                out[c * out_cstride + (long long)z * Nxy + o] = v;
+            }
     }
 }
 
@@ -1255,6 +1365,21 @@ void launch_adj_fft_post(const pg::Gnufft& G, float2* grids, float2* tbuf, float
         const size_t fused_lds =
             sizeof(float2) * ((size_t)G.nzc * g.Ny + (size_t)kZfFftWaves * zf_lpw(N) * zf_pitch(N));
         const int zgroups = 256 / std::max(g.Ny, 1);
+        if (g_zfuse.load(std::memory_order_relaxed) && g_zamfma.load(std::memory_order_relaxed) &&
+            half && (g.Ny == 64 || g.Ny == 128) && g.Nz <= 32 && G.nzc <= 32) {
+            // y IFFT on four wavefronts, crop and the z stage (matrix cores)
+            const unsigned nb = (unsigned)((long long)nc * g.Nx);
+#define PG_YZM(NN, CT)                                                                          \
+    hipLaunchKernelGGL((k_yifft_zadj_mfma<pgf::Split<NN>::R1, pgf::Split<NN>::R2, CT>),          \
+                       dim3(nb), dim3(256), 0, s, tbuf, w.S, w.WoH, G.dapxy, g.Nx, g.Ny, g.Nz,      \
+                       g.offY, L, B, nc, G.nzc, G.zmat, (const float2*)G.py.tw, out, out_cstride)
+            if (g.Ny == 64)
+                PG_YZM(128, 1);
+            else
+                PG_YZM(256, 2);
+#undef PG_YZM
+            return;
+        }
         if (g_zfuse.load(std::memory_order_relaxed) && half && g.Ny >= 64 && g.Ny <= 256 &&
             g.Nz <= 32 * zgroups && fused_lds <= 64 * 1024) {
             // y IFFT, crop and the z stage in one kernel

@@ -126,6 +126,10 @@ extern std::atomic<int> g_zfuse;
 // the forward z stage of z-class planes runs on the matrix cores (Nz and
 // the class count at most 32) when set (default), else on the vector units
 extern std::atomic<int> g_zmfma;
+// the fused adjoint y IFFT + z stage of z-class planes runs its IFFTs on
+// four wavefronts and its class sums on the matrix cores (Ny 64 or 128, Nz
+// and the class count at most 32) when set (default)
+extern std::atomic<int> g_zamfma;
 
 // the compact-row patch spread stages each stage's row slices in LDS
 // (k_spread_patch_st) when set (default), else reads them per entry

@@ -312,10 +312,10 @@ def test_staged_row_mfma_spread_matches_value_row_mfma(N, Nz, nc, L, itype):
 
 @pytest.mark.parametrize("N,Nz,nc,L", [(64, 32, 2, 12), (64, 20, 2, 8), (128, 32, 2, 12)])
 def test_sos3d_mfma_z_stage_matches_vector_z_stage(N, Nz, nc, L):
-    # the forward z stage on the matrix cores (pg_set_tuning("zstage_mfma",
-    # 1)) against the vector-unit one: the same products, the z sums in
-    # another order, so equal to FP32 rounding; the adjoint takes the same
-    # path either way
+    # the z stages on the matrix cores (pg_set_tuning "zstage_mfma" 1: the
+    # forward one; "zadj_mfma" 1: the adjoint's, fused with its y IFFT)
+    # against the vector-unit ones: the same products, the class and z sums
+    # in another order, so equal to FP32 rounding
     print(f"[sos3d z stage mfma vs vector, {N}x{N}x{Nz} nc={nc} L={L}]", flush=True)
     p = S.make_problem(N, 2, 1024 if N == 64 else 2048, nc, L, Nz=Nz, peak_hz=200.0, seed=4)
     Gg = P.Gnufft(p.M, 2.0, p.Nx, p.Ny, p.Nz, p.kx, p.ky, p.kz)
@@ -326,10 +326,12 @@ def test_sos3d_mfma_z_stage_matches_vector_z_stage(N, Nz, nc, L):
     out = {}
     for m in (1, 0):
         P.set_tuning("zstage_mfma", m)
+        P.set_tuning("zadj_mfma", m)
         try:
             out[m] = (Sg * x, Sg / y)
         finally:
             P.set_tuning("zstage_mfma", 1)
+            P.set_tuning("zadj_mfma", 1)
     ef, ea = rel(out[1][0], out[0][0]), rel(out[1][1], out[0][1])
     print(f"  forward {ef:.1e}, adjoint {ea:.1e}", flush=True)
     assert ef < 1e-6 and ea < 1e-6
