@@ -664,10 +664,31 @@ __global__ void __launch_bounds__(256) k_zstage_fwd_mfma(
             zi[rt][s] = w.y;
         }
     __syncthreads();
-    for (int l = g; l < L; l += NW) {
-        float2* hp = H + ((long long)c * L + l) * Nxy;
+    // tasks (segment l = g + NW (tau / 4), column tile ct = tau % 4) of this
+    // wavefront; the next task's segment phases are requested before this
+    // task's products
+    const int ntask = ((L - g + NW - 1) / NW) * 4;
+    float2 wn[8];
+    auto load_wo = [&](int tau, float2(&w)[8]) {
+        const int l = g + NW * (tau >> 2);
+        const int col = col0 + 16 * (tau & 3) + p;
+        const int colc = col < Nxy ? col : Nxy - 1;
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+            const int zc = min(4 * s + q, Nz - 1);
+            w[s] = Wo ? Wo[(long long)l * N + (long long)zc * Nxy + colc] : make_float2(1.f, 0.f);
+        }
+    };
+    if (ntask > 0) load_wo(0, wn);
 #pragma unroll 1
-        for (int ct = 0; ct < 4; ++ct) {
+    for (int tau = 0; tau < ntask; ++tau) {
+        const int l = g + NW * (tau >> 2), ct = tau & 3;
+        float2 wc[8];
+#pragma unroll
+        for (int s = 0; s < 8; ++s) wc[s] = wn[s];
+        if (tau + 1 < ntask) load_wo(tau + 1, wn);
+        float2* hp = H + ((long long)c * L + l) * Nxy;
+        {
             const int cl = 16 * ct + p;  // column in the block's chunk
             const int col = col0 + cl;
             const int colc = col < Nxy ? col : Nxy - 1;
@@ -678,7 +699,7 @@ __global__ void __launch_bounds__(256) k_zstage_fwd_mfma(
                 const int z = 4 * s + q;
                 const int zc = z < Nz ? z : Nz - 1;
                 float2 a = su[zc * 64 + cl];
-                if (Wo) a = pgf::cmul(Wo[(long long)l * N + (long long)zc * Nxy + colc], a);
+                if (Wo) a = pgf::cmul(wc[s], a);
                 v[s] = z < Nz ? make_float2(a.x * d, a.y * d) : make_float2(0.f, 0.f);
             }
             pgz_f4 hr[2], hi[2];
