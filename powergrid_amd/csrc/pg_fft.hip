@@ -775,6 +775,49 @@ __device__ __forceinline__ void zf_class_columns(const float2* __restrict__ T, f
     }
 }
 
+// The same step split in two for a register pipeline over segments: the
+// loads of grid b's class columns (NRD rounds of 4 LPW lines, all four
+// wavefronts), then their transforms into sh.
+template <int R1, int R2, int NRD>
+__device__ __forceinline__ void zf_load_columns(const float2* __restrict__ T, int wv, int t,
+                                                int jl, int nzc, int B, int b, int Nx, int x,
+                                                float2 (&v)[NRD][R1])
+{
+    constexpr int N = R1 * R2;
+    constexpr int LPW = 64 / R2;
+#pragma unroll
+    for (int rd = 0; rd < NRD; ++rd) {
+        const int k = (rd * 4 + wv) * LPW + jl;
+        const float2* src = T + (((long long)(k < nzc ? k : 0) * B + b) * Nx + x) * N;
+#pragma unroll
+        for (int m = 0; m < R1; ++m) v[rd][m] = src[(t + R2 * m + N / 2) & (N - 1)];
+    }
+}
+
+template <int R1, int R2, int NRD>
+__device__ __forceinline__ void zf_transform_columns(float2 (&v)[NRD][R1], float2* sh, float2* my,
+                                                     int wv, int t, int jl, int nzc, int Ny,
+                                                     int shift, const float2* __restrict__ tw)
+{
+    constexpr int N = R1 * R2;
+    constexpr int LPW = 64 / R2;
+#pragma unroll
+    for (int rd = 0; rd < NRD; ++rd) {
+        const int k = (rd * 4 + wv) * LPW + jl;
+        pgf::line_fft<R1, R2, 1, false>(
+            v[rd], t, my, [](int k1, int q) { return k1 * (R2 + 1) + q; }, tw);
+        if (k < nzc) {
+#pragma unroll
+            for (int r = 0; r < R1 / R2; ++r)
+#pragma unroll
+                for (int k2 = 0; k2 < R2; ++k2) {
+                    const int yy = (t + R2 * r + R1 * k2 - shift) & (N - 1);
+                    if (yy < Ny) sh[k * Ny + yy] = v[rd][r * R2 + k2];
+                }
+        }
+    }
+}
+
 // The adjoint y IFFT of the class planes fused with the adjoint z stage
 // (gridos 2, 64 <= Ny = Gy/2 <= 256).  One block of 256 threads per (image column
 // x, coil c), c fastest in the grid (the blocks of one x share its WoH in
@@ -876,7 +919,7 @@ __global__ void __launch_bounds__(256) k_yifft_zadj(
 // Static LDS (class columns, four wavefronts' exchange rows, Zm): 75 KB at
 // Gy = 256, two blocks per CU.
 template <int R1, int R2, int CT>
-__global__ void __launch_bounds__(256) k_yifft_zadj_mfma(
+__global__ void __launch_bounds__(256, 2) k_yifft_zadj_mfma(
     const float2* __restrict__ T, const float2* __restrict__ S, const float2* __restrict__ WoH,
     const float* __restrict__ dxy, int Nx, int Ny, int Nz, int offY, int L, int B, int nc,
     int nzc, const float2* __restrict__ zm, const float2* __restrict__ tw,
@@ -907,11 +950,16 @@ __global__ void __launch_bounds__(256) k_yifft_zadj_mfma(
         for (int rt = 0; rt < 2; ++rt)
 #pragma unroll
             for (int r = 0; r < 4; ++r) acc[ct][rt][r] = make_float2(0.f, 0.f);
+    // segment l + 1's class columns are loaded while segment l's z stage
+    // runs on the matrix cores
+    constexpr int NRD = (32 + 4 * LPW - 1) / (4 * LPW);
+    float2 vc[NRD][R1];
+    zf_load_columns<R1, R2, NRD>(T, wv, t, jl, nzc, B, c * L, Nx, x, vc);
 #pragma unroll 1
     for (int l = 0; l < L; ++l) {
-        zf_class_columns<R1, R2, 4>(T, sh, my, wv, t, jl, nzc, B, c * L + l, Nx, x, Ny, shift,
-                                    tw);
+        zf_transform_columns<R1, R2, NRD>(vc, sh, my, wv, t, jl, nzc, Ny, shift, tw);
         __syncthreads();
+        if (l + 1 < L) zf_load_columns<R1, R2, NRD>(T, wv, t, jl, nzc, B, c * L + l + 1, Nx, x, vc);
 #pragma unroll
         for (int ct = 0; ct < CT; ++ct) {
             const int y = 16 * (wv * CT + ct) + p;
