@@ -844,34 +844,38 @@ static void gnufft_setup(Gnufft& G, size_t M, float os, size_t Nx, size_t Ny, si
                     run += pweight(q);
                 }
         }
-        // per group: blocks of 8 wavefront items {patch, entry begin, entry end,
-        // slot}.  A patch of n <= 512 entries is one unit taking gs = 1
-        // (n <= 64) or the power of two >= n/64 wavefronts and writes the
-        // grid (slot -1); a heavier patch is cut into ceil(n/512) units of 8
-        // wavefronts, each writing a partial slot that a second pass adds up
-        // in slot order.
+        // per group: blocks of kW = kSpreadWaves wavefront items {patch, entry
+        // begin, entry end, slot}.  A patch of n <= 64 kW entries is one unit
+        // taking gs = 1 (n <= 64) or the power of two >= n/64 wavefronts and
+        // writes the grid (slot -1); a heavier patch is cut into ceil(n/(64
+        // kW)) units of kW wavefronts, each writing a partial slot that a
+        // second pass adds up in slot order.
         struct Unit {
             int q, e0, e1, slot, gs;
         };
+        const int kW = kSpreadWaves, kUnitMax = kW * 64;
+        int nlg = 0;  // gs = 1, 2, ..., kW
+        while ((1 << nlg) < kW) ++nlg;
+        ++nlg;
         struct SBlock {
             int gs;
             int cost;
-            int4 w[8];
+            int4 w[kSpreadWaves];
         };
-        const int kW = 8, kUnitMax = kW * 64;
         std::vector<int4> comb;
         int nslots = 0;
         std::vector<std::vector<SBlock>> gblk(kXcd);
         for (int gi = 0; gi < kXcd; ++gi) {
-            std::vector<std::vector<Unit>> byg(4);  // gs = 1, 2, 4, 8
+            std::vector<std::vector<Unit>> byg(nlg);
             for (int q : grp[gi]) {
                 const int b0 = (int)cnt[q], n = (int)(cnt[q + 1] - cnt[q]);
                 if (n > kUnitMax) {
                     const int parts = (n + kUnitMax - 1) / kUnitMax;
                     comb.push_back(make_int4(q, nslots, parts, 0));
                     for (int k = 0; k < parts; ++k)
-                        byg[3].push_back(Unit{q, b0 + (int)((long long)n * k / parts),
-                                              b0 + (int)((long long)n * (k + 1) / parts), nslots + k, 8});
+                        byg[nlg - 1].push_back(Unit{q, b0 + (int)((long long)n * k / parts),
+                                                    b0 + (int)((long long)n * (k + 1) / parts),
+                                                    nslots + k, kW});
                     nslots += parts;
                     continue;
                 }
@@ -882,7 +886,7 @@ static void gnufft_setup(Gnufft& G, size_t M, float os, size_t Nx, size_t Ny, si
                 }
                 byg[lg].push_back(Unit{q, b0, b0 + n, -1, gs});
             }
-            for (int lg = 0; lg < 4; ++lg) {
+            for (int lg = 0; lg < nlg; ++lg) {
                 const int gs = 1 << lg, per = kW / gs;
                 auto& v = byg[lg];
                 std::stable_sort(v.begin(), v.end(),

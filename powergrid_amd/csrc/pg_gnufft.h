@@ -15,6 +15,10 @@
 namespace pg {
 
 constexpr int kPatch = 4;  // 2-D spread patch edge (grid points)
+// wavefronts (work items) per block of the 2-D patch spread, a power of two:
+// a patch of up to 64 kSpreadWaves entries is summed by up to that many
+// wavefronts of one block, a heavier one in parts through partial slots
+constexpr int kSpreadWaves = 8;
 constexpr int kIT = 16;    // 2-D interp: tile edge in window starts
 constexpr int kIR = kIT + 7;  // staged grid region edge (8 taps)
 constexpr int kIChunk = 128;  // samples per interp work item
@@ -41,10 +45,11 @@ struct Gnufft {
     // length has a prime factor above 13 (only the spatial interpolation is available)
     FftPlan1D px, py, pz;
     bool fft_ok = false;
-    // 2-D batched spread over 4x4 patches: blocks of 8 wavefront items
-    //   witems[8*k + w] = {patch or -1, entry begin, entry end, slot or -1}
+    // 2-D batched spread over 4x4 patches: blocks of W = kSpreadWaves
+    // wavefront items
+    //   witems[W*k + w] = {patch or -1, entry begin, entry end, slot or -1}
     // and the per-block group size bgs[k] (wavefronts sharing a patch);
-    // patches with more than 512 entries are summed in parts into partial
+    // patches with more than 64 W entries are summed in parts into partial
     // slots and combined by comb[] = {patch, first slot, parts, 0}
     int4* witems = nullptr;
     int* bgs = nullptr;

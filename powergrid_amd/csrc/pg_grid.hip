@@ -648,22 +648,22 @@ __device__ __forceinline__ float2 row_conj_weight(float a, float ai, float2 v)
 }
 
 // ------------------------------------------------- 2-D patch spread --------
-// A block of 8 wavefronts; lane = grid (coil x segment).  Wavefront w runs
-// the work item witems[8 * block + w] = {patch (-1: none), entry begin,
-// entry end, slot}: it accumulates the 4x4 patch in registers over its
-// entries.  Entry k carries its sample index and the 16 weights wx*wy of the
+// A block of W = kSpreadWaves wavefronts; lane = grid (coil x segment).
+// Wavefront w runs the work item witems[W block + w] = {patch (-1: none),
+// entry begin, entry end, slot}: it accumulates the 4x4 patch in registers
+// over its entries.  Entry k carries its sample index and the 16 weights wx*wy of the
 // patch's points (zero outside the sample's clamped window), precomputed
 // with the trajectory.  Per stage of up to 64 entries a wavefront stages the
 // weights in LDS (read back as broadcasts) and the indices one per lane
 // (readlane); the only global load per entry is the sample's contiguous row
 // of grid values, and the next 8 rows are in flight while the current 8 are
-// summed.  A patch of 65..512 entries is split over gs = 2..8 consecutive
+// summed.  A patch of 65..64 W entries is split over gs = 2..W consecutive
 // wavefronts of a block (bgs[block] = gs, the same for all groups of a
 // block), whose sums are added through LDS in a fixed tree order; the
 // group's first wavefront writes the patch, or its partial slot when the
 // patch is one of the parts of a heavier patch (added by
 // k_spread_heavy_combine).  No atomics, no zero-fill, deterministic.
-constexpr int kSpreadWaves = 8;
+using pg::kSpreadWaves;
 
 // ROWS = 1 (2: complex weights): the entries' values are formed from the
 // compact rows of k_spread_rows (stride S floats, nc coils, L segments):
@@ -831,7 +831,7 @@ __global__ void __launch_bounds__(64 * kSpreadWaves, 4) k_spread_patch_st(
     static_assert(ROWS == 1 || ROWS == 2, "compact rows only");
     constexpr int NE = kStEntries, FM = kStFloats;
     constexpr int NU = (NE * FM / 2 + 63) / 64;  // float2 row loads per lane per stage
-    // the stage rows of the 8 wavefronts, reused for the tree sum after the
+    // the stage rows of the block's wavefronts, reused for the tree sum after the
     // last stage (red: [kSpreadWaves / 2][16][64] float2)
     constexpr int POOL = kSpreadWaves * NE * FM > kSpreadWaves / 2 * 16 * 64 * 2
                              ? kSpreadWaves * NE * FM
