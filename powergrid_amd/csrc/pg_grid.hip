@@ -816,20 +816,22 @@ __global__ void __launch_bounds__(64 * kSpreadWaves) k_spread_patch(
 // entry's coil value and segment weight from LDS (a broadcast for the lanes
 // of one coil) instead of from global memory, so a stage waits for memory
 // once instead of once per group of entries.  Same values, products and
-// summation order as k_spread_patch.  Requires F <= kStFloats for every
+// summation order as k_spread_patch.  FM (32 or 48) bounds the floats staged
+// per entry; the 32-float form keeps 48 KB of LDS and 80 VGPRs per block of
+// 8 wavefronts, so six wavefronts per SIMD.  Requires F <= FM for every
 // 64-grid chunk (checked by the launch).
 constexpr int kStEntries = 32;
-constexpr int kStFloats = 48;
+constexpr int kStFloats = 48;  // the largest row slice staged per entry
 
-template <int ROWS>
-__global__ void __launch_bounds__(64 * kSpreadWaves, 4) k_spread_patch_st(
+template <int ROWS, int FM>
+__global__ void __launch_bounds__(64 * kSpreadWaves, FM <= 32 ? 6 : 4) k_spread_patch_st(
     const int4* __restrict__ witems, const int* __restrict__ bgs, const int* __restrict__ entries,
     const float4* __restrict__ ew, int npy, const float2* __restrict__ V, int B,
     float2* __restrict__ grids, long long G, int Gy, float2* __restrict__ partial, int n_slots,
     int npp, long long pstride, int L, int S)
 {
     static_assert(ROWS == 1 || ROWS == 2, "compact rows only");
-    constexpr int NE = kStEntries, FM = kStFloats;
+    constexpr int NE = kStEntries;
     constexpr int NU = (NE * FM / 2 + 63) / 64;  // float2 row loads per lane per stage
     // the stage rows of the block's wavefronts, reused for the tree sum after the
     // last stage (red: [kSpreadWaves / 2][16][64] float2)
@@ -886,6 +888,10 @@ __global__ void __launch_bounds__(64 * kSpreadWaves, 4) k_spread_patch_st(
         // unit t = lane + 64 j of the stage: entry k = t / U2, unit u = t % U2
         float2 rv[NU];
         {
+            // the rows through one buffer resource (32-bit byte offsets: the
+            // launch keeps M S 4 below 2^31)
+            const __amdgpu_buffer_rsrc_t rrow = __builtin_amdgcn_make_buffer_rsrc(
+                (void*)Rf, (short)0, 0x7fffffff, kBufferWord3);
             int k = lane / U2, u = lane - (lane / U2) * U2;
             // formed anew per stage (kept out of registers across stages)
             asm volatile("" : "+v"(k), "+v"(u));
@@ -893,7 +899,8 @@ __global__ void __launch_bounds__(64 * kSpreadWaves, 4) k_spread_patch_st(
             for (int j = 0; j < NU; ++j) {
                 const int src = __shfl(ei, min(k, n - 1), 64);
                 const int off = u < U2 - F2 / 2 ? 2 * c_lo + 2 * u : 2 * nc + 2 * (u - (U2 - F2 / 2));
-                rv[j] = *reinterpret_cast<const float2*>(Rf + (long long)src * S + off);
+                rv[j] = __builtin_bit_cast(
+                    float2, __builtin_amdgcn_raw_buffer_load_b64(rrow, (src * S + off) * 4, 0, 0));
                 k += qd;
                 u += qr;
                 if (u >= U2) {
@@ -906,13 +913,14 @@ __global__ void __launch_bounds__(64 * kSpreadWaves, 4) k_spread_patch_st(
 #pragma unroll
             for (int r = 0; r < 4; ++r) mw[lane][r] = wq[r];
         }
+        // every unit t < 64 NU <= NE FM / 2 lies in the wavefront's rows;
+        // those past n U2 are never read
+        static_assert(64 * NU <= NE * FM / 2, "stage rows");
 #pragma unroll
-        for (int j = 0; j < NU; ++j) {
-            const int t = lane + 64 * j;
-            if (t < n * U2) *reinterpret_cast<float2*>(rows + 2 * t) = rv[j];
-        }
+        for (int j = 0; j < NU; ++j)
+            *reinterpret_cast<float2*>(rows + 2 * (lane + 64 * j)) = rv[j];
         pgf::wave_lds_sync();
-#pragma unroll 2
+#pragma unroll 1
         for (int k = 0; k < n; ++k) {
             const float* row = rows + k * F;
             float2 val = *reinterpret_cast<const float2*>(row + voff);
@@ -1506,16 +1514,18 @@ void spread_dispatch(const pg::Gnufft& G, const float2* in, long long in_cstride
                                    B, grids, gsz, g.Gy, partial, G.n_slots, G.npp, pstride, L_, S_);
             };
             const bool st = S && pgk::g_spread_stage.load(std::memory_order_relaxed) != 0 &&
-                            spread_st_floats(B, L, AAi != nullptr) <= kStFloats;
+                            spread_st_floats(B, L, AAi != nullptr) <= kStFloats &&
+                            (long long)G.M * S * 4 < (1LL << 31);
             auto launch_st = [&](auto kern) {
                 hipLaunchKernelGGL(kern, dim3(G.n_sblocks, nch), dim3(64 * kSpreadWaves), 0, s,
                                    G.witems, G.bgs, G.entries, (const float4*)G.ent_w, npy, vals,
                                    B, grids, gsz, g.Gy, partial, G.n_slots, G.npp, pstride, L, S);
             };
+            const bool small = spread_st_floats(B, L, AAi != nullptr) <= 32;
             if (st && AAi)
-                launch_st(k_spread_patch_st<2>);
+                small ? launch_st(k_spread_patch_st<2, 32>) : launch_st(k_spread_patch_st<2, 48>);
             else if (st)
-                launch_st(k_spread_patch_st<1>);
+                small ? launch_st(k_spread_patch_st<1, 32>) : launch_st(k_spread_patch_st<1, 48>);
             else if (S && AAi)
                 launch(k_spread_patch<2>, L, S);
             else if (S)

@@ -259,8 +259,11 @@ def test_sos3d_fused_z_stage_matches_two_kernel_path():
 
 
 @pytest.mark.parametrize("N,Nz,nc,L,itype", [(128, 32, 2, 12, 1), (128, 1, 8, 8, 1),
-                                             (64, 1, 4, 6, 2)])
+                                             (64, 1, 4, 6, 2), (64, 1, 32, 4, 1),
+                                             (64, 1, 8, 14, 2)])
 def test_staged_row_spread_matches_patch_spread(N, Nz, nc, L, itype):
+    # (the last two stage 36 and 38 floats per entry: the 48-float form; the
+    # others 32 or fewer: the small-LDS form)
     # the patch spread with each stage's row slices staged in LDS
     # (pg_set_tuning("spread_stage", 1)) against the one that reads them per
     # entry from global memory: the same values, products and summation
