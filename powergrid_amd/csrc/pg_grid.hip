@@ -312,8 +312,9 @@ __global__ void __launch_bounds__(64 * kInterpWaves) k_interp_lds(
 // chunk's taps.  Each task forms
 //   acc_l = sum_x wx[x] sum_y wy[y] g_l(ox + x, oy + y)
 // and adds AA_l acc_l to its running sum; the four slots of a sample are
-// added with two DPP steps at the end.  The small LDS footprint (26 KB)
-// lets six blocks share a CU, so one block's staging overlaps the others'
+// added with two DPP steps at the end.  The sample records go straight to
+// registers; the small LDS footprint (21 KB: the region and the AA weights)
+// lets seven blocks share a CU, so one block's staging overlaps the others'
 // taps.
 constexpr int kI4Threads = 256;
 constexpr int kI4Chunk = 64;
@@ -337,9 +338,7 @@ __global__ void __launch_bounds__(kI4Threads) k_interp_lds4(
     constexpr int NP = (NQ + NTH - 1) / NTH;   // points per thread
     static_assert(CH * LC == NTH, "one task per thread");
     __shared__ __align__(16) float2 tile[NQ * LC];
-    constexpr int NR = (CH * 5 + NTH - 1) / NTH;   // record loads per thread
     constexpr int NA = (CH * 16 + NTH - 1) / NTH;  // weight loads per thread
-    __shared__ float4 srec[CH * 5];
     __shared__ float saa[CH * 16];
     __shared__ float saai[CPX ? CH * 16 : 1];
     const int tid = threadIdx.x;
@@ -388,17 +387,19 @@ __global__ void __launch_bounds__(kI4Threads) k_interp_lds4(
                     float2, __builtin_amdgcn_raw_buffer_load_b64(rs, voff[k], 0, 0));
         }
     };
-    // The item's records, the sample indices of its (sample, segment slot)
-    // pairs e = s * 16 + l, chunk 0 of the region and then the AA weights
-    // (by those indices) are all requested before the first LDS store; a
-    // weight of a slot past L or a sample past n reads zero (offset 2^31).
+    // The task's sample record (into registers: the four lanes of a sample
+    // read the same 80 bytes), the sample indices of the (sample, segment
+    // slot) pairs e = s * 16 + l, chunk 0 of the region and then the AA
+    // weights (by those indices) are all requested before the first LDS
+    // store; a record or weight of a sample past n or a slot past L reads
+    // zero (offset past the resource's size, or 2^31).
     const __amdgpu_buffer_rsrc_t rr = __builtin_amdgcn_make_buffer_rsrc(
         (void*)(rec + (long long)it.y * 5), (short)0, n * 5 * 16, kBufferWord3);
-    float4 rv[NR];
+    float4 rv[5];
 #pragma unroll
-    for (int k = 0; k < NR; ++k)
+    for (int k = 0; k < 5; ++k)
         rv[k] = __builtin_bit_cast(
-            float4, __builtin_amdgcn_raw_buffer_load_b128(rr, (tid + k * NTH) * 16, 0, 0));
+            float4, __builtin_amdgcn_raw_buffer_load_b128(rr, (s * 5 + k) * 16, 0, 0));
     int iv[NA];
 #pragma unroll
     for (int k = 0; k < NA; ++k) {
@@ -433,9 +434,6 @@ __global__ void __launch_bounds__(kI4Threads) k_interp_lds4(
     };
     store_chunk();
 #pragma unroll
-    for (int k = 0; k < NR; ++k)
-        if (tid + k * NTH < CH * 5) srec[tid + k * NTH] = rv[k];
-#pragma unroll
     for (int k = 0; k < NA; ++k) {
         saa[tid + k * NTH] = av[k];
         if (CPX) saai[tid + k * NTH] = aiv[k];
@@ -443,12 +441,11 @@ __global__ void __launch_bounds__(kI4Threads) k_interp_lds4(
     for (int g = 0;; ++g) {
         __syncthreads();  // chunk g is in LDS
         if (g == 0) {
-            const float4* rp = srec + (live ? s : 0) * 5;
-            const float4 h = rp[0];
+            const float4 h = rv[0];
             idx = __float_as_int(h.x);
             ox = __float_as_int(h.y);
             oy = __float_as_int(h.z);
-            const float4 xa = rp[1], xb = rp[2], ya = rp[3], yb = rp[4];
+            const float4 xa = rv[1], xb = rv[2], ya = rv[3], yb = rv[4];
             wx[0] = xa.x; wx[1] = xa.y; wx[2] = xa.z; wx[3] = xa.w;
             wx[4] = xb.x; wx[5] = xb.y; wx[6] = xb.z; wx[7] = xb.w;
             wy[0] = ya.x; wy[1] = ya.y; wy[2] = ya.z; wy[3] = ya.w;
