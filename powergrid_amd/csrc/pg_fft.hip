@@ -629,7 +629,7 @@ __global__ void __launch_bounds__(256) k_zstage_fwd_reg(
 // matrix cores' order (four terms per step).
 typedef float pgz_f4 __attribute__((ext_vector_type(4)));
 
-__global__ void __launch_bounds__(256) k_zstage_fwd_mfma(
+__global__ void __launch_bounds__(256, 4) k_zstage_fwd_mfma(
     const float2* __restrict__ img, long long img_cstride, const float2* __restrict__ S,
     const float2* __restrict__ Wo, const float* __restrict__ dxy, int Nxy, int Nz, int L, int B,
     int nc, int nzc, const float2* __restrict__ zm, float2* __restrict__ H)
