@@ -722,14 +722,18 @@ def roofline(kinfo, L, summary, plane_bytes=None):
     """the roofline object of the dominant gridding kernel.  The interpolation
     stages all L segments of a tile (k_interp_lds) below 4 segments and four
     at a time (k_interp_lds4) from 4 up; the spread of a time-segmented batch
-    stages compact sample rows in LDS (k_spread_patch_st<1>).  plane_bytes: on z-class planes the
+    stages compact sample rows in LDS (k_spread_patch_st<1, 32>, or <1, 48>
+    for row slices wider than 32 floats).  plane_bytes: on z-class planes the
     launch moves 4*2*M + B*(8*M + 8*nzc*Gx*Gy) bytes (its x-y grids are the
     nzc planes, the z taps having been applied in the z stage), reported
     beside the full-grid figure of 4*d*M + B*(8*M + 8*G)."""
     dom = max(("interp", "spread"), key=lambda k: kinfo[k]["ms"])
     kname = {"interp": "k_interp_lds4<false>" if L >= 4 else "k_interp_lds<8, false>",
-             "spread": "k_spread_patch_st<1>"}[dom]
+             "spread": "k_spread_patch_st<1, 32>"}[dom]
     tr = pmc_traffic(summary, kname)
+    if tr is None and dom == "spread":  # row slices wider than 32 floats
+        kname = "k_spread_patch_st<1, 48>"
+        tr = pmc_traffic(summary, kname)
     out = dict(bound="hbm", achieved=round(kinfo[dom]["GBs"], 1), peak=HBM_PEAK_GBS,
                unit="GB/s", frac=round(kinfo[dom]["frac"], 4),
                traffic=None if tr is None else round(tr), kernel=kname,
