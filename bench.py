@@ -683,10 +683,12 @@ def timed_solves(P, yd, Sg, R, niter, steps, warmup, stream):
     return steps * info["iterations"] / dt, dt, lat, x, info
 
 
-def kernel_times(P, Sg, x_host, stream, nrep, M, Gsz, B, nc, L, d):
+def kernel_times(P, Sg, x_host, stream, nrep, M, Gsz, B, nc, L, d, plane_g=None):
     """Per-kernel durations of the batched operator (HIP events the library
     records on its launch stream, nrep forward + adjoint pairs), each with its
-    algorithmic bytes and fraction of the HBM peak."""
+    algorithmic bytes and fraction of the HBM peak.  plane_g: on z-class
+    planes, the points per grid that the FFT passes transform (nzc*Gx*Gy
+    instead of the full grid Gsz), whose bytes and fraction are added."""
     P.profile.enable(True)
     P.profile.reset()
     xd = P.DeviceArray.from_host(np.asarray(x_host, np.complex64))
@@ -713,8 +715,13 @@ def kernel_times(P, Sg, x_host, stream, nrep, M, Gsz, B, nc, L, d):
     for k, v in kinfo.items():
         v["GBs"] = v["bytes"] / (v["ms"] * 1e-3) / 1e9 if v["ms"] > 0 else 0.0
         v["frac"] = v["GBs"] / HBM_PEAK_GBS
+        extra = ""
+        if plane_g and k in ("fwd_prep_fft", "adj_fft_post") and v["ms"] > 0:
+            v["plane_bytes"] = fft_bytes(plane_g, B)
+            v["plane_frac"] = v["plane_bytes"] / (v["ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS
+            extra = f"; {100 * v['plane_frac']:.1f}% by the class planes' bytes"
         log(f"[bench] {k:13s} {v['ms'] * 1e3:9.1f} us  {v['GBs']:8.1f} GB/s  "
-            f"({100 * v['frac']:.1f}% of HBM)")
+            f"({100 * v['frac']:.1f}% of HBM{extra})")
     return kinfo
 
 
@@ -1019,8 +1026,9 @@ def main():
         f"exit={info['exit_reason']} iters={iters_done}")
     M, B = prob.M, prob.nc * prob.L
     Gsz = 8 * prob.N  # 256 x 256 x 64
-    kinfo = kernel_times(P, Sg, x_gpu, stream, 3, M, Gsz, B, prob.nc, prob.L, 3)
     nzc = P.zclasses(G)
+    kinfo = kernel_times(P, Sg, x_gpu, stream, 3, M, Gsz, B, prob.nc, prob.L, 3,
+                         plane_g=nzc * 4 * prob.Nx * prob.Ny if nzc else None)
     plane_b = (4 * 2 * M + B * (8 * M + 8 * nzc * 4 * prob.Nx * prob.Ny)) if nzc else None
     roof = roofline(kinfo, prob.L, "sos3d_pmc_traffic.md", plane_b)
 

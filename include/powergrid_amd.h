@@ -362,6 +362,9 @@ int pg_recon_solve_batch(size_t K, const pg_op* A, const pg_penalty* R, const vo
  * all four wavefronts and its class sums on the matrix cores; 0: two
  * wavefronts' IFFTs and vector-unit sums.  Same products, sums in another
  * order.
+ * "zfwd_fuse" 1 (default): the forward z stage on the matrix cores (Ny 64 or
+ * 128 at gridos 2) and the class planes' y FFT run as one kernel, the class
+ * planes kept in LDS; 0: two kernels.  Same products, same order.
  * "spread_stage" 1 (default): the compact-row spread loads the row slices of
  * each group of up to 32 patch entries into LDS with all loads in flight,
  * then sums from LDS (when a 64-grid chunk needs at most 48 floats of a

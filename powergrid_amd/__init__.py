@@ -402,7 +402,9 @@ def set_tuning(key, value):
     "spread_stage" 1 (default) stages the compact rows of each group of
     patch entries in LDS, 0 reads them per entry; "zstage_mfma" 1 (default)
     forms the forward z stage's class sums on the matrix cores; "zadj_mfma"
-    1 (default) the fused adjoint's, with its y IFFTs on four wavefronts."""
+    1 (default) the fused adjoint's, with its y IFFTs on four wavefronts;
+    "zfwd_fuse" 1 (default) runs the matrix-core forward z stage and the
+    class planes' y FFT as one kernel (bit-identical to the two kernels)."""
     check(lib().pg_set_tuning(key.encode(), int(value)))
 
 

@@ -3397,6 +3397,8 @@ int pg_set_tuning(const char* key, int value)
         pgk::g_zmfma.store(value ? 1 : 0);
     else if (k == "zadj_mfma")
         pgk::g_zamfma.store(value ? 1 : 0);
+    else if (k == "zfwd_fuse")
+        pgk::g_zfwd.store(value ? 1 : 0);
 
 
     else if (k == "spread_stage")

@@ -734,6 +734,170 @@ __global__ void __launch_bounds__(256, 4) k_zstage_fwd_mfma(
     }
 }
 
+// The forward z stage fused with the y FFT of the class planes (gridos 2,
+// Ny = Gy / 2 = 64 CT, the zero-padded half layout of k_fwd_col_reg, Nz and
+// the class count at most 32).  One block of 256 threads per (image column
+// x, coil c), c fastest in the grid (the blocks of one x share its segment
+// phases in L2).  Wavefront wv takes the CT 16-row tiles
+// y = 16 (wv CT + ct) + p; lane (p, q) = (lane % 16, lane / 16) keeps
+// k_zstage_fwd_mfma's class matrix entries in registers, and S_c x(z) and
+// dxy of its rows and z = 4 s + q for the whole kernel.  For each segment l:
+//   1. the class sums H_k(y) of grid b = c L + l as k_zstage_fwd_mfma forms
+//      them (same products, same order), stored in LDS as nzc columns of Ny;
+//   2. all four wavefronts transform the class columns along y with
+//      k_fwd_col_reg's zero padding, shift and register FFT, and write them
+//      to T, plane k B + b, where k_fwd_col_reg writes the class planes' FFTs.
+// So the result is bit-identical to the two kernels, without the class
+// planes in HBM.  The next segment's phases are loaded during this segment's
+// products.  Static LDS (class columns, exchange rows): 66 KB at Gy = 256,
+// two blocks per CU.
+template <int R1, int R2, int CT>
+__global__ void __launch_bounds__(256, 2) k_zfwd_yfft_mfma(
+    const float2* __restrict__ img, long long img_cstride, const float2* __restrict__ S,
+    const float2* __restrict__ Wo, const float* __restrict__ dxy, int Nx, int Ny, int Nz,
+    int offY, int L, int B, int nc, int nzc, const float2* __restrict__ zm,
+    const float2* __restrict__ tw, float2* __restrict__ T)
+{
+    constexpr int N = R1 * R2;  // Gy
+    constexpr int NY = N / 2;
+    constexpr int LPW = 64 / R2;
+    constexpr int PITCH = R1 * (R2 + 1);
+    constexpr int NRD = (32 + 4 * LPW - 1) / (4 * LPW);
+    static_assert(NY == 64 * CT, "Ny = 64 CT");
+    __shared__ float2 sh[32 * NY];
+    __shared__ float2 ex[4 * LPW * PITCH];
+    const int tid = threadIdx.x;
+    const int c = blockIdx.x % nc, x = blockIdx.x / nc;
+    const int Nxy = Nx * Ny;
+    const long long Nimg = (long long)Nxy * Nz;
+    const int lane = tid & 63, wv = tid >> 6;
+    const int t = lane % R2, jl = lane / R2;
+    const int shift = (offY + N / 2) % N;  // image row y sits at (y + shift) % N
+    float2* my = ex + (wv * LPW + jl) * PITCH;
+    const int p = lane & 15, q = lane >> 4;
+    // class matrix entries: row k = 16 rt + p, z = 4 s + q (zero outside)
+    float zr[2][8], zi[2][8];
+#pragma unroll
+    for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+            const int k = 16 * rt + p, z = 4 * s + q;
+            const float2 w = (k < nzc && z < Nz) ? zm[(long long)k * Nz + z] : make_float2(0.f, 0.f);
+            zr[rt][s] = w.x;
+            zi[rt][s] = w.y;
+        }
+    // S_c x(z) and dxy of the lane's rows
+    float2 u0[CT][8];
+    float dd[CT];
+#pragma unroll
+    for (int ct = 0; ct < CT; ++ct) {
+        const long long o = (long long)x * Ny + 16 * (wv * CT + ct) + p;
+        dd[ct] = dxy[o];
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+            const int zc = min(4 * s + q, Nz - 1);
+            float2 a = img[c * img_cstride + (long long)zc * Nxy + o];
+            if (S) a = pgf::cmul(S[(long long)c * Nimg + (long long)zc * Nxy + o], a);
+            u0[ct][s] = a;
+        }
+    }
+    float2 wn[CT][8];
+    auto load_wo = [&](int l, float2(&w)[CT][8]) {
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct) {
+            const long long o = (long long)x * Ny + 16 * (wv * CT + ct) + p;
+#pragma unroll
+            for (int s = 0; s < 8; ++s) {
+                const int zc = min(4 * s + q, Nz - 1);
+                w[ct][s] = Wo ? Wo[(long long)l * Nimg + (long long)zc * Nxy + o]
+                              : make_float2(1.f, 0.f);
+            }
+        }
+    };
+    load_wo(0, wn);
+#pragma unroll 1
+    for (int l = 0; l < L; ++l) {
+        float2 wc[CT][8];
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+            for (int s = 0; s < 8; ++s) wc[ct][s] = wn[ct][s];
+        if (l + 1 < L) load_wo(l + 1, wn);
+        pgz_f4 hr[CT][2], hi[CT][2];
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct) {
+            float2 v[8];
+#pragma unroll
+            for (int s = 0; s < 8; ++s) {
+                float2 a = u0[ct][s];
+                if (Wo) a = pgf::cmul(wc[ct][s], a);
+                v[s] = 4 * s + q < Nz ? make_float2(a.x * dd[ct], a.y * dd[ct])
+                                      : make_float2(0.f, 0.f);
+            }
+#pragma unroll
+            for (int rt = 0; rt < 2; ++rt) {
+                hr[ct][rt] = pgz_f4{0.f, 0.f, 0.f, 0.f};
+                hi[ct][rt] = pgz_f4{0.f, 0.f, 0.f, 0.f};
+            }
+            // Re H = Zr vr + Zi (-vi), Im H = Zi vr + Zr vi
+#pragma unroll
+            for (int s = 0; s < 16; ++s) {
+                const float b = s < 8 ? v[s].x : v[s - 8].y;
+                const float br = s < 8 ? b : -b;
+#pragma unroll
+                for (int rt = 0; rt < 2; ++rt) {
+                    const float ar = s < 8 ? zr[rt][s] : zi[rt][s - 8];
+                    const float ai = s < 8 ? zi[rt][s] : zr[rt][s - 8];
+                    hr[ct][rt] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar, br, hr[ct][rt], 0, 0, 0);
+                    hi[ct][rt] = __builtin_amdgcn_mfma_f32_16x16x4f32(ai, b, hi[ct][rt], 0, 0, 0);
+                }
+            }
+        }
+        if (l > 0) __syncthreads();  // the previous segment's columns are read
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct) {
+            const int y = 16 * (wv * CT + ct) + p;
+#pragma unroll
+            for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int k = 16 * rt + 4 * q + r;
+                    if (k < nzc) sh[k * NY + y] = make_float2(hr[ct][rt][r], hi[ct][rt][r]);
+                }
+        }
+        __syncthreads();
+        const int b = c * L + l;
+#pragma unroll 1
+        for (int rd = 0; rd < NRD; ++rd) {
+            const int k = (rd * 4 + wv) * LPW + jl;
+            const bool valid = k < nzc;
+            const float2* col = sh + (valid ? k : 0) * NY;
+            float2 v[R1];
+#pragma unroll
+            for (int m = 0; m < R1; ++m) {
+                if (!half_live<R1>(m)) {
+                    v[m] = make_float2(0.f, 0.f);
+                    continue;
+                }
+                const int y = (t + R2 * m - shift) & (N - 1);
+                const float2 a = col[y < NY ? y : 0];
+                v[m] = valid && y < NY ? a : make_float2(0.f, 0.f);
+            }
+            pgf::line_fft<R1, R2, -1, false>(
+                v, t, my, [](int k1, int qq) { return k1 * (R2 + 1) + qq; }, tw);
+            if (!valid) continue;
+            float2* dst = T + (((long long)k * B + b) * Nx + x) * N;
+#pragma unroll
+            for (int r = 0; r < R1 / R2; ++r)
+#pragma unroll
+                for (int k2 = 0; k2 < R2; ++k2) {
+                    const int kk = t + R2 * r + R1 * k2;
+                    dst[(kk + N / 2) & (N - 1)] = v[r * R2 + k2];
+                }
+        }
+    }
+}
+
 constexpr int kZfFftWaves = 2;
 
 // Step 1 of the fused adjoint below: the first kZfFftWaves wavefronts
@@ -1359,10 +1523,31 @@ void launch_fwd_prep_fft(const pg::Gnufft& G, const float2* img, long long img_c
         const int Bp = B * G.nzc;
         const int nxy = g.Nx * g.Ny;
         const unsigned nch = (unsigned)((nxy + 63) / 64);
+        const int N = g.Gy;
+        const bool half = 2 * g.Ny == N && (g.offY + N / 2) % N == 3 * N / 4;
+        const bool mfma = g_zmfma.load(std::memory_order_relaxed) && g.Nz <= 32 && G.nzc <= 32;
+        if (mfma && g_zfwd.load(std::memory_order_relaxed) && half &&
+            (g.Ny == 64 || g.Ny == 128)) {
+            // the z stage and the y FFT in one kernel, straight into tbuf
+            const unsigned nb = (unsigned)((long long)nc * g.Nx);
+#define PG_ZFY(NN, CT)                                                                        \
+    hipLaunchKernelGGL((k_zfwd_yfft_mfma<pgf::Split<NN>::R1, pgf::Split<NN>::R2, CT>),         \
+                       dim3(nb), dim3(256), 0, s, img, img_cstride, w.S, w.Wo, G.dapxy, g.Nx,     \
+                       g.Ny, g.Nz, g.offY, L, B, nc, G.nzc, G.zmat, (const float2*)G.py.tw, tbuf)
+            if (g.Ny == 64)
+                PG_ZFY(128, 1);
+            else
+                PG_ZFY(256, 2);
+#undef PG_ZFY
+            line_fft_any<-1>(tbuf, grids, (long long)g.Gy, (long long)g.Nx * g.Gy,
+                             (long long)g.Gx * g.Gy, g.Nx, (g.offX + g.Gx / 2) % g.Gx, g.Gx,
+                             g.Gx / 2, (unsigned)Bp, G.px, s);
+            return;
+        }
 #define PG_ZSF(NZ)                                                                              \
     hipLaunchKernelGGL(k_zstage_fwd_reg<NZ>, dim3(nch * nc), dim3(256), 0, s, img, img_cstride, \
                        w.S, w.Wo, G.dapxy, nxy, g.Nz, L, B, nc, G.nzc, G.zmat, tbuf2)
-        if (g_zmfma.load(std::memory_order_relaxed) && g.Nz <= 32 && G.nzc <= 32) {
+        if (mfma) {
             hipLaunchKernelGGL(k_zstage_fwd_mfma, dim3(nch * nc), dim3(256), 0, s, img,
                                img_cstride, w.S, w.Wo, G.dapxy, nxy, g.Nz, L, B, nc, G.nzc,
                                G.zmat, tbuf2);

@@ -1555,6 +1555,7 @@ std::atomic<int> g_spread_rows{1};
 std::atomic<int> g_zfuse{1};
 std::atomic<int> g_zmfma{1};
 std::atomic<int> g_zamfma{1};
+std::atomic<int> g_zfwd{1};
 std::atomic<int> g_spread_stage{1};
 
 int spread_row_floats(const pg::Gnufft& G, int nc, int L, const float* AA, const float* AAi)

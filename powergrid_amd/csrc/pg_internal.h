@@ -130,6 +130,9 @@ extern std::atomic<int> g_zmfma;
 // four wavefronts and its class sums on the matrix cores (Ny 64 or 128, Nz
 // and the class count at most 32) when set (default)
 extern std::atomic<int> g_zamfma;
+// the forward z stage of z-class planes on the matrix cores runs fused with
+// the planes' y FFT (Ny 64 or 128 at gridos 2) when set (default)
+extern std::atomic<int> g_zfwd;
 
 // the compact-row patch spread stages each stage's row slices in LDS
 // (k_spread_patch_st) when set (default), else reads them per entry

@@ -338,3 +338,25 @@ def test_sos3d_mfma_z_stage_matches_vector_z_stage(N, Nz, nc, L):
     ef, ea = rel(out[1][0], out[0][0]), rel(out[1][1], out[0][1])
     print(f"  forward {ef:.1e}, adjoint {ea:.1e}", flush=True)
     assert ef < 1e-6 and ea < 1e-6
+
+
+@pytest.mark.parametrize("N,Nz,nc,L", [(64, 32, 2, 12), (64, 20, 2, 8), (128, 32, 3, 12)])
+def test_sos3d_fused_forward_z_stage_matches_two_kernels(N, Nz, nc, L):
+    # the forward z stage fused with the class planes' y FFT (pg_set_tuning
+    # "zfwd_fuse" 1) against the two kernels with the class planes in HBM:
+    # the same products in the same order, so bit-identical
+    print(f"[sos3d fused forward z stage, {N}x{N}x{Nz} nc={nc} L={L}]", flush=True)
+    p = S.make_problem(N, 2, 1024 if N == 64 else 2048, nc, L, Nz=Nz, peak_hz=200.0, seed=5)
+    Gg = P.Gnufft(p.M, 2.0, p.Nx, p.Ny, p.Nz, p.kx, p.ky, p.kz)
+    Sg = P.SENSE(P.TimeSegmentation(Gg, p.fm, p.t, p.M, p.N, p.L), p.sens, p.M, p.N, p.nc)
+    rng = np.random.default_rng(9)
+    x = (rng.standard_normal(p.N) + 1j * rng.standard_normal(p.N)).astype(np.complex64)
+    out = {}
+    for m in (1, 0):
+        P.set_tuning("zfwd_fuse", m)
+        try:
+            out[m] = Sg * x
+        finally:
+            P.set_tuning("zfwd_fuse", 1)
+    print(f"  forward {rel(out[1], out[0]):.1e}", flush=True)
+    np.testing.assert_array_equal(out[1], out[0])
