@@ -401,12 +401,16 @@ __global__ void __launch_bounds__(256) k_fwd_col_reg(
 
 // FFT along a strided axis (same arguments as k_line_fft).  A block holds
 // LPB consecutive inner lines x R2 threads, inner index fastest across lanes
-// so that every global access is a run of LPB consecutive elements.
+// so that every global access is a run of LPB consecutive elements.  Length
+// 256 takes 32 lines (256-byte runs, 512 threads): against 16 lines the x
+// FFTs of the 3-D workload's class planes took 1.85 ms instead of 2.13
+// (forward) and 2.02 (adjoint); 64 lines (one 1024-thread block per CU)
+// slowed the adjoint to 1.97 ms.
 template <int R1, int R2>
-constexpr int line_lpb() { return R1 * R2 >= 1024 ? 8 : 16; }
+constexpr int line_lpb() { return R1 * R2 >= 1024 ? 8 : (R1 * R2 == 256 ? 32 : 16); }
 
 template <int R1, int R2, int SIGN, bool HALF>
-__global__ void __launch_bounds__(512) k_line_reg(const float2* __restrict__ src,
+__global__ void __launch_bounds__((line_lpb<R1, R2>() * R2)) k_line_reg(const float2* __restrict__ src,
                                                   float2* __restrict__ dst, long long inner_n,
                                                   long long src_os, long long dst_os, int nin,
                                                   int in_off, int nout, int out_off,
