@@ -651,9 +651,13 @@ static void gnufft_setup(Gnufft& G, size_t M, float os, size_t Nx, size_t Ny, si
             // samples of a wavefront read distinct bank pairs at every tap
             // when their keys (LP R ox + oy) mod 32 differ.  One of
             // k_interp_lds4 reads 32 contiguous bytes per sample (its four
-            // segment slots) at index ((ox + x) R + oy + y) 4 + l, so that
-            // 4 (8) consecutive samples cover the 32 (64) banks once when
-            // their keys (R ox + oy) mod 4 (mod 8) differ.  Each item's
+            // segment slots) at index ((ox + x) P + oy + y) 4 + l, P the
+            // item's row pitch, so that 4 (8) consecutive samples cover the
+            // 32 (64) banks once when their keys (P ox + oy) mod 4 (mod 8)
+            // differ (gfx950 serves 8-byte reads a half-wavefront at a time
+            // over 64 banks: measured conflict-free when the 8 keys of each
+            // half differ).  Each item takes the pitch of kIR .. kIR + 2
+            // whose dealt keys collide least.  Each item's
             // records are sorted by key and dealt round-robin to groups of
             // 8 samples, which spreads the keys of a group over their values
             // (ascending within a group).  The order of the samples changes
@@ -676,7 +680,7 @@ static void gnufft_setup(Gnufft& G, size_t M, float os, size_t Nx, size_t Ny, si
                     g = (g + 1) % ng;
                 }
             };
-            std::vector<int> oxs(M), oys(M), o8(M), o4(M);
+            std::vector<int> oxs(M), oys(M), o8(M), o4(M), t4(M);
             auto put = [&](std::vector<float4>& r, int k, int i, int ox, int oy) {
                 float4 h;
                 h.x = __int_as_float_host(i);
@@ -709,8 +713,29 @@ static void gnufft_setup(Gnufft& G, size_t M, float os, size_t Nx, size_t Ny, si
                 }
                 for (int k = tcnt[tb]; k < tcnt[tb + 1]; k += kIChunk4) {
                     const int n = std::min(kIChunk4, tcnt[tb + 1] - k);
-                    deal(k, n, 8, kIR, 7, order, oxs, oys, o4);
-                    its4.push_back(make_int4(t, k, k + n, p));
+                    // the region's row pitch P (points) whose keys (P ox + oy)
+                    // mod 8, once dealt, give the fewest bank conflicts: the
+                    // sum over groups of 8 of the largest key multiplicity
+                    int best = -1, bestc = 0;
+                    for (int P = kIR; P < kIR + kI4Pitches; ++P) {
+                        deal(k, n, 8, P, 7, order, oxs, oys, t4);
+                        int cost = 0;
+                        for (int h = 0; h < n; h += 8) {
+                            int cnt[8] = {0, 0, 0, 0, 0, 0, 0, 0}, mx = 0;
+                            for (int j = h; j < std::min(n, h + 8); ++j) {
+                                const int i = t4[k + j];
+                                const int key = (P * (rlo[i] - rx0) + rlo[M + i] - ry0) & 7;
+                                mx = std::max(mx, ++cnt[key]);
+                            }
+                            cost += mx;
+                        }
+                        if (best < 0 || cost < bestc) {
+                            best = P;
+                            bestc = cost;
+                            std::copy(t4.begin() + k, t4.begin() + k + n, o4.begin() + k);
+                        }
+                    }
+                    its4.push_back(make_int4(t | (best - kIR) << 24, k, k + n, p));
                 }
                 for (int k = tcnt[tb]; k < tcnt[tb + 1]; ++k) {
                     const int i8 = o8[k], i4 = o4[k];

@@ -23,6 +23,9 @@ constexpr int kIT = 16;    // 2-D interp: tile edge in window starts
 constexpr int kIR = kIT + 7;  // staged grid region edge (8 taps)
 constexpr int kIChunk = 128;  // samples per interp work item
 constexpr int kIChunk4 = 64;  // the same for the four-segment staging
+// row pitches (in points) that an item of the four-segment staging may take
+// for its kIR x kIR region: kIR .. kIR + kI4Pitches - 1
+constexpr int kI4Pitches = 3;
 
 struct Gnufft {
     Geometry geo;
@@ -71,7 +74,8 @@ struct Gnufft {
     // (origin tile*kIT - ipadl) and zero weights past the taps
     int4* iitems = nullptr;
     int n_iitems = 0;
-    int4* iitems4 = nullptr;  // the same tiles cut into items of <= kIChunk4
+    // the same tiles cut into items of <= kIChunk4, x = tile | (pitch - kIR) << 24
+    int4* iitems4 = nullptr;
     int n_iitems4 = 0;
     float4* irec = nullptr;
     // the same records with each item's samples reordered for the

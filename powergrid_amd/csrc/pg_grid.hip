@@ -324,7 +324,7 @@ constexpr int kI4Chunk = 64;
 constexpr int kBufferWord3 = 0x00020000;
 
 template <bool CPX>
-__global__ void __launch_bounds__(kI4Threads) k_interp_lds4(
+__global__ void __launch_bounds__(kI4Threads, CPX ? 6 : 7) k_interp_lds4(
     const int4* __restrict__ items, const float4* __restrict__ rec, int ity, int padl,
     const float2* __restrict__ grids, long long G, int Gx, int Gy, int nc, int L,
     const float* __restrict__ AA, const float* __restrict__ AAi, int M, float2* __restrict__ out,
@@ -336,8 +336,10 @@ __global__ void __launch_bounds__(kI4Threads) k_interp_lds4(
     constexpr int NTH = kI4Threads;
     constexpr int NQ = R * R;                  // points of the region
     constexpr int NP = (NQ + NTH - 1) / NTH;   // points per thread
+    constexpr int PMAX = R + pg::kI4Pitches - 1;
     static_assert(CH * LC == NTH, "one task per thread");
-    __shared__ __align__(16) float2 tile[NQ * LC];
+    // the region's R rows at the item's pitch P (R .. PMAX points)
+    __shared__ __align__(16) float2 tile[R * PMAX * LC];
     constexpr int NA = (CH * 16 + NTH - 1) / NTH;  // weight loads per thread
     __shared__ float saa[CH * 16];
     __shared__ float saai[CPX ? CH * 16 : 1];
@@ -348,7 +350,8 @@ __global__ void __launch_bounds__(kI4Threads) k_interp_lds4(
     const int c = blockIdx.x % nc;
     const int4 it = items[blockIdx.x / nc];
     const int n = it.z - it.y;  // 1..CH
-    const int tx = it.x / ity, ty = it.x - (it.x / ity) * ity;
+    const int tcode = it.x & 0xFFFFFF, P = R + (it.x >> 24);
+    const int tx = tcode / ity, ty = tcode - (tcode / ity) * ity;
     const int rx0 = tx * pg::kIT - padl, ry0 = ty * pg::kIT - padl;
     // item plane it.w (z-class planes; 0 for a 2-D plan)
     const float2* gc = grids + (long long)it.w * pstride + (long long)c * L * G;
@@ -364,7 +367,7 @@ __global__ void __launch_bounds__(kI4Threads) k_interp_lds4(
     // gets an offset of 2^31, and the hardware returns zero for it and for
     // every load of an empty slot, so the loads need neither clamps nor
     // selects and are all issued back to back.
-    int voff[NP];
+    int voff[NP], soff[NP];
 #pragma unroll
     for (int k = 0; k < NP; ++k) {
         const int q = tid + k * NTH;
@@ -372,6 +375,7 @@ __global__ void __launch_bounds__(kI4Threads) k_interp_lds4(
         const int x = rx0 + xr, y = ry0 + yr;
         const bool ok = q < NQ && x >= 0 && x < Gx && y >= 0 && y < Gy;
         voff[k] = ok ? (x * Gy + y) * 8 : (int)0x80000000u;
+        soff[k] = (xr * P + yr) * LC;
     }
     float2 tv[NP][LC];
     auto load_chunk = [&](int g) {
@@ -426,7 +430,7 @@ __global__ void __launch_bounds__(kI4Threads) k_interp_lds4(
         for (int k = 0; k < NP; ++k) {
             const int q = tid + k * NTH;
             if (q < NQ) {
-                float4* t4 = reinterpret_cast<float4*>(tile + q * LC);
+                float4* t4 = reinterpret_cast<float4*>(tile + soff[k]);
                 t4[0] = make_float4(tv[k][0].x, tv[k][0].y, tv[k][1].x, tv[k][1].y);
                 t4[1] = make_float4(tv[k][2].x, tv[k][2].y, tv[k][3].x, tv[k][3].y);
             }
@@ -438,29 +442,26 @@ __global__ void __launch_bounds__(kI4Threads) k_interp_lds4(
         saa[tid + k * NTH] = av[k];
         if (CPX) saai[tid + k * NTH] = aiv[k];
     }
-    for (int g = 0;; ++g) {
-        __syncthreads();  // chunk g is in LDS
-        if (g == 0) {
-            const float4 h = rv[0];
-            idx = __float_as_int(h.x);
-            ox = __float_as_int(h.y);
-            oy = __float_as_int(h.z);
-            const float4 xa = rv[1], xb = rv[2], ya = rv[3], yb = rv[4];
-            wx[0] = xa.x; wx[1] = xa.y; wx[2] = xa.z; wx[3] = xa.w;
-            wx[4] = xb.x; wx[5] = xb.y; wx[6] = xb.z; wx[7] = xb.w;
-            wy[0] = ya.x; wy[1] = ya.y; wy[2] = ya.z; wy[3] = ya.w;
-            wy[4] = yb.x; wy[5] = yb.y; wy[6] = yb.z; wy[7] = yb.w;
-        }
-        // the next chunk's loads are in flight during this chunk's taps
-        const bool more = (g + 1) * LC < L;
-        if (more) load_chunk(g + 1);
+    auto take_record = [&]() {
+        const float4 h = rv[0];
+        idx = __float_as_int(h.x);
+        ox = __float_as_int(h.y);
+        oy = __float_as_int(h.z);
+        const float4 xa = rv[1], xb = rv[2], ya = rv[3], yb = rv[4];
+        wx[0] = xa.x; wx[1] = xa.y; wx[2] = xa.z; wx[3] = xa.w;
+        wx[4] = xb.x; wx[5] = xb.y; wx[6] = xb.z; wx[7] = xb.w;
+        wy[0] = ya.x; wy[1] = ya.y; wy[2] = ya.z; wy[3] = ya.w;
+        wy[4] = yb.x; wy[5] = yb.y; wy[6] = yb.z; wy[7] = yb.w;
+    };
+    // the taps of chunk g (in LDS) for this task's segment
+    auto taps = [&](int g) {
         const int l = g * LC + l4;
         if (live && l < L) {
-            const float2* tp = tile + ((ox * R + oy) * LC + l4);
+            const float2* tp = tile + ((ox * P + oy) * LC + l4);
             float ar = 0.f, ai = 0.f;
 #pragma unroll
             for (int x = 0; x < 8; ++x) {
-                const float2* row = tp + x * R * LC;
+                const float2* row = tp + x * P * LC;
                 float rr = 0.f, ri = 0.f;
 #pragma unroll
                 for (int y = 0; y < 8; ++y) {
@@ -481,6 +482,14 @@ __global__ void __launch_bounds__(kI4Threads) k_interp_lds4(
                 ti += a * ai;
             }
         }
+    };
+    for (int g = 0;; ++g) {
+        __syncthreads();  // chunk g is in LDS
+        if (g == 0) take_record();
+        // the next chunk's loads are in flight during this chunk's taps
+        const bool more = (g + 1) * LC < L;
+        if (more) load_chunk(g + 1);
+        taps(g);
         if (!more) break;
         __syncthreads();  // chunk g's taps are done
         store_chunk();
