@@ -752,10 +752,12 @@ __global__ void __launch_bounds__(256, 4) k_zstage_fwd_mfma(
 //      k_fwd_col_reg's zero padding, shift and register FFT, and write them
 //      to T, plane k B + b, where k_fwd_col_reg writes the class planes' FFTs.
 // So the result is bit-identical to the two kernels, without the class
-// planes in HBM.  The next segment's phases are loaded during this segment's
+// planes in HBM.  HS, HW: S and Wo are given (compile-time, so that their
+// loads are issued together rather than each behind a branch and a wait).
+// The next segment's phases are loaded during this segment's
 // products.  Static LDS (class columns, exchange rows): 66 KB at Gy = 256,
 // two blocks per CU.
-template <int R1, int R2, int CT>
+template <int R1, int R2, int CT, bool HS, bool HW>
 __global__ void __launch_bounds__(256, 2) k_zfwd_yfft_mfma(
     const float2* __restrict__ img, long long img_cstride, const float2* __restrict__ S,
     const float2* __restrict__ Wo, const float* __restrict__ dxy, int Nx, int Ny, int Nz,
@@ -801,7 +803,7 @@ __global__ void __launch_bounds__(256, 2) k_zfwd_yfft_mfma(
         for (int s = 0; s < 8; ++s) {
             const int zc = min(4 * s + q, Nz - 1);
             float2 a = img[c * img_cstride + (long long)zc * Nxy + o];
-            if (S) a = pgf::cmul(S[(long long)c * Nimg + (long long)zc * Nxy + o], a);
+            if (HS) a = pgf::cmul(S[(long long)c * Nimg + (long long)zc * Nxy + o], a);
             u0[ct][s] = a;
         }
     }
@@ -813,7 +815,7 @@ __global__ void __launch_bounds__(256, 2) k_zfwd_yfft_mfma(
 #pragma unroll
             for (int s = 0; s < 8; ++s) {
                 const int zc = min(4 * s + q, Nz - 1);
-                w[ct][s] = Wo ? Wo[(long long)l * Nimg + (long long)zc * Nxy + o]
+                w[ct][s] = HW ? Wo[(long long)l * Nimg + (long long)zc * Nxy + o]
                               : make_float2(1.f, 0.f);
             }
         }
@@ -834,7 +836,7 @@ __global__ void __launch_bounds__(256, 2) k_zfwd_yfft_mfma(
 #pragma unroll
             for (int s = 0; s < 8; ++s) {
                 float2 a = u0[ct][s];
-                if (Wo) a = pgf::cmul(wc[ct][s], a);
+                if (HW) a = pgf::cmul(wc[ct][s], a);
                 v[s] = 4 * s + q < Nz ? make_float2(a.x * dd[ct], a.y * dd[ct])
                                       : make_float2(0.f, 0.f);
             }
@@ -1084,9 +1086,11 @@ __global__ void __launch_bounds__(256) k_yifft_zadj(
 // k = 4 s + q, and receives u_z(y) for z = 16 rt + 4 q + r (register r),
 // which it multiplies by WoH_l(z) and adds to its sums.  The epilogue is
 // k_yifft_zadj's.  Same products; the k sums in the matrix cores' order.
+// HS: S is given (compile-time, so that the epilogue's loads of S are issued
+// together rather than each behind a branch and a wait).
 // Static LDS (class columns, four wavefronts' exchange rows, Zm): 75 KB at
 // Gy = 256, two blocks per CU.
-template <int R1, int R2, int CT>
+template <int R1, int R2, int CT, bool HS>
 __global__ void __launch_bounds__(256, 2) k_yifft_zadj_mfma(
     const float2* __restrict__ T, const float2* __restrict__ S, const float2* __restrict__ WoH,
     const float* __restrict__ dxy, int Nx, int Ny, int Nz, int offY, int L, int B, int nc,
@@ -1178,7 +1182,7 @@ __global__ void __launch_bounds__(256, 2) k_yifft_zadj_mfma(
                 const int z = 16 * rt + 4 * q + r;
                 if (z >= Nz) continue;
                 float2 v = make_float2(acc[ct][rt][r].x * d, acc[ct][rt][r].y * d);
-                if (S) {
+                if (HS) {
                     float2 sc = S[(long long)c * Nimg + (long long)z * Nxy + o];
                     sc.y = -sc.y;
                     v = pgf::cmul(v, sc);
@@ -1534,14 +1538,26 @@ void launch_fwd_prep_fft(const pg::Gnufft& G, const float2* img, long long img_c
             (g.Ny == 64 || g.Ny == 128)) {
             // the z stage and the y FFT in one kernel, straight into tbuf
             const unsigned nb = (unsigned)((long long)nc * g.Nx);
-#define PG_ZFY(NN, CT)                                                                        \
-    hipLaunchKernelGGL((k_zfwd_yfft_mfma<pgf::Split<NN>::R1, pgf::Split<NN>::R2, CT>),         \
+#define PG_ZFY(NN, CT, HS, HW)                                                                \
+    hipLaunchKernelGGL((k_zfwd_yfft_mfma<pgf::Split<NN>::R1, pgf::Split<NN>::R2, CT, HS, HW>), \
                        dim3(nb), dim3(256), 0, s, img, img_cstride, w.S, w.Wo, G.dapxy, g.Nx,     \
                        g.Ny, g.Nz, g.offY, L, B, nc, G.nzc, G.zmat, (const float2*)G.py.tw, tbuf)
+#define PG_ZFY_N(NN, CT)                                                                      \
+    do {                                                                                       \
+        if (w.S && w.Wo)                                                                       \
+            PG_ZFY(NN, CT, true, true);                                                        \
+        else if (w.S)                                                                          \
+            PG_ZFY(NN, CT, true, false);                                                       \
+        else if (w.Wo)                                                                         \
+            PG_ZFY(NN, CT, false, true);                                                       \
+        else                                                                                   \
+            PG_ZFY(NN, CT, false, false);                                                      \
+    } while (0)
             if (g.Ny == 64)
-                PG_ZFY(128, 1);
+                PG_ZFY_N(128, 1);
             else
-                PG_ZFY(256, 2);
+                PG_ZFY_N(256, 2);
+#undef PG_ZFY_N
 #undef PG_ZFY
             line_fft_any<-1>(tbuf, grids, (long long)g.Gy, (long long)g.Nx * g.Gy,
                              (long long)g.Gx * g.Gy, g.Nx, (g.offX + g.Gx / 2) % g.Gx, g.Gx,
@@ -1627,14 +1643,18 @@ void launch_adj_fft_post(const pg::Gnufft& G, float2* grids, float2* tbuf, float
             half && (g.Ny == 64 || g.Ny == 128) && g.Nz <= 32 && G.nzc <= 32) {
             // y IFFT on four wavefronts, crop and the z stage (matrix cores)
             const unsigned nb = (unsigned)((long long)nc * g.Nx);
-#define PG_YZM(NN, CT)                                                                          \
-    hipLaunchKernelGGL((k_yifft_zadj_mfma<pgf::Split<NN>::R1, pgf::Split<NN>::R2, CT>),          \
+#define PG_YZM(NN, CT, HS)                                                                      \
+    hipLaunchKernelGGL((k_yifft_zadj_mfma<pgf::Split<NN>::R1, pgf::Split<NN>::R2, CT, HS>),      \
                        dim3(nb), dim3(256), 0, s, tbuf, w.S, w.WoH, G.dapxy, g.Nx, g.Ny, g.Nz,      \
                        g.offY, L, B, nc, G.nzc, G.zmat, (const float2*)G.py.tw, out, out_cstride)
-            if (g.Ny == 64)
-                PG_YZM(128, 1);
+            if (g.Ny == 64 && w.S)
+                PG_YZM(128, 1, true);
+            else if (g.Ny == 64)
+                PG_YZM(128, 1, false);
+            else if (w.S)
+                PG_YZM(256, 2, true);
             else
-                PG_YZM(256, 2);
+                PG_YZM(256, 2, false);
 #undef PG_YZM
             return;
         }
