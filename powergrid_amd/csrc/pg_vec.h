@@ -23,7 +23,15 @@ struct CgScal {
 
 namespace pgv {
 
-constexpr int kMaxParts = 256;  // partial sums per reduction (blocks of a reducing kernel)
+// partial sums per reduction (blocks of a reducing kernel): up to
+// kMaxParts (four 256-thread blocks per CU) over vectors of kBigParts
+// elements or more, so that the bandwidth-bound CG passes over the data
+// vectors (16.7M samples at the 3-D workload: 3-D workload 64.2 -> 65.2 CG
+// it/s) keep enough loads in flight; up to kSmallParts below (at the 2-D
+// workload's sizes the extra partials cost more than they gain)
+constexpr int kMaxParts = 1024;
+constexpr int kSmallParts = 256;
+constexpr long long kBigParts = 1LL << 22;
 
 struct PenSpec {
     int kind;  // 0 quad, 1 TV

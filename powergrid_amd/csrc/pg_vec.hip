@@ -797,8 +797,9 @@ __global__ void k_cg_init(const double* part, int nparts, CgScal* sc, double tol
 inline unsigned nblk(long long n) { return (unsigned)((n + kThreads - 1) / kThreads); }
 inline unsigned nred(long long n)
 {
-    long long b = (n + kThreads - 1) / kThreads;
-    return (unsigned)(b < pgv::kMaxParts ? (b < 1 ? 1 : b) : pgv::kMaxParts);
+    const long long b = (n + kThreads - 1) / kThreads;
+    const long long cap = n >= pgv::kBigParts ? pgv::kMaxParts : pgv::kSmallParts;
+    return (unsigned)(b < cap ? (b < 1 ? 1 : b) : cap);
 }
 
 Pen to_pen(const pgv::PenSpec& s)
